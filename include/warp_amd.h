@@ -1,0 +1,222 @@
+/*
+ * warp_amd.h -- C ABI of libwarp_amd.so, the gfx950 (MI355X) library behind
+ * the sparse-solver path of Warp (bsr_set_from_triplets, bsr_mv, Jacobi,
+ * conjugate gradient).
+ *
+ * Conventions
+ *   - Every pointer is caller-owned device memory unless documented as host.
+ *   - `stream` is a hipStream_t passed as void* (NULL = legacy default stream);
+ *     all work is stream-ordered and asynchronous unless an entry point
+ *     documents a host read-back.
+ *   - int-returning entry points return 0 on success, a negative status on a
+ *     HIP or argument error; warp_amd_last_error() then describes it (per
+ *     host thread).
+ *   - Entry points taking (workspace, workspace_bytes) never allocate; the
+ *     matching *_workspace_size() query gives the byte count (host-only
+ *     call, no GPU needed).
+ *   - Reference paths below are relative to the NVIDIA Warp 1.17.0.dev4
+ *     checkout.
+ */
+#ifndef WARP_AMD_H
+#define WARP_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#else
+#include <stdbool.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Runtime                                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* Message of the last failure on this host thread
+ * (replaces wp_get_error_string, warp/native/warp.h:38). */
+const char* warp_amd_last_error(void);
+void warp_amd_clear_error(void);
+/* major * 10000 + minor * 100 + patch */
+int warp_amd_version(void);
+int warp_amd_device_count(int* count);
+
+/* Memory helpers for callers without their own allocator
+ * (replace wp_alloc_device / wp_free_device / wp_memcpy_*, warp/native/warp.h,
+ * implemented at warp/native/warp.cu:752-866, 1087).
+ * memcpy kind: 0 host->device, 1 device->host, 2 device->device. */
+int warp_amd_malloc(void** ptr, size_t bytes);
+int warp_amd_free(void* ptr);
+int warp_amd_malloc_host(void** ptr, size_t bytes);
+int warp_amd_free_host(void* ptr);
+int warp_amd_memcpy(void* dst, const void* src, size_t bytes, int kind, void* stream);
+int warp_amd_stream_synchronize(void* stream);
+
+/* Per-thread "current stream" used by the reference-named entry point below
+ * (the reference uses its current CUDA stream, warp/native/sparse.cu:1189-1192). */
+void warp_amd_set_current_stream(void* stream);
+void* warp_amd_get_current_stream(void);
+
+/* ------------------------------------------------------------------------ */
+/* Triplets -> compact BSR                                                  */
+/* ------------------------------------------------------------------------ */
+
+size_t warp_amd_bsr_from_triplets_workspace_size(int row_count, int nnz);
+
+/* Replaces wp_bsr_matrix_from_triplets_device (warp/native/warp.h:563-602,
+ * warp/native/sparse.cu:1168-1322) fused with the value accumulation kernel
+ * _bsr_accumulate_triplet_values (warp/_src/sparse.py:1073-1090, 1122-1141).
+ *
+ * nnz            number of triplets T (upper bound; tpl_nnz, when non-null,
+ *                is a device int32 holding the actual count <= T)
+ * tpl_values     T blocks of block_size scalars (scalar_size bytes each), or
+ *                NULL for a topology-only build
+ * scalar_zero_mask  non-zero: drop blocks whose scalars all satisfy
+ *                (bits & mask) == 0 (sparse.py:993-1006)
+ * masked_topology   1: keep only triplets present in the existing topology
+ *                (bsr_offsets / bsr_row_counts / bsr_columns are read first)
+ * summed_block_offsets / _indices  optional int32[T] outputs: inclusive run
+ *                end of each unique block in sorted order, and the triplet
+ *                indices in (row, col, triplet) order; NULL = kept internal
+ * bsr_offsets    int32[row_count + 1] output (input too when masked)
+ * bsr_columns    int32[>= unique count] output
+ * bsr_values     NULL, or [>= unique count] blocks receiving the sums
+ *                v[i0] + v[i1] + ... in ascending triplet order (fp32 / fp64)
+ * bsr_nnz_host   NULL, or pinned host int32 receiving offsets[row_count]
+ *                asynchronously (valid once the stream reaches this point)
+ * workspace      >= warp_amd_bsr_from_triplets_workspace_size(row_count, nnz)
+ */
+int warp_amd_bsr_from_triplets(
+    int block_size, int scalar_size, int row_count, int col_count, int nnz, const int* tpl_nnz,
+    const int* tpl_rows, const int* tpl_columns, const void* tpl_values, uint64_t scalar_zero_mask,
+    int masked_topology, int* summed_block_offsets, int* summed_block_indices, int* bsr_offsets,
+    const int* bsr_row_counts, int* bsr_columns, void* bsr_values, int* bsr_nnz_host, void* workspace,
+    size_t workspace_bytes, void* stream);
+
+/* Same name and arguments as the reference entry point
+ * (warp/native/warp.h:563-602; ctypes argtypes warp/_src/context.py:7444-7466),
+ * for callers that bind the reference library by name: topology only, runs
+ * on the stream set with warp_amd_set_current_stream, allocates its scratch
+ * stream-ordered, records bsr_nnz_event (a hipEvent_t) after the nnz copy.
+ * Returns void like the reference; failures land in warp_amd_last_error(). */
+void wp_bsr_matrix_from_triplets_device(
+    int block_size, int scalar_size, int row_count, int col_count, int nnz, const int* tpl_nnz,
+    const int* tpl_rows, const int* tpl_columns, const void* tpl_values, uint64_t scalar_zero_mask,
+    bool masked_topology, int* tpl_block_offsets, int* tpl_block_indices, int* bsr_offsets,
+    const int* bsr_row_counts, int* bsr_columns, int* bsr_nnz, void* bsr_nnz_event);
+
+/* ------------------------------------------------------------------------ */
+/* SpMV, diagonal, Jacobi                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* y = alpha * A x + beta * y for a BSR matrix with block_rows x block_cols
+ * blocks (row_counts NULL = compact).  Replaces the JIT kernels bsr_mv_kernel
+ * and bsr_mv_tiled_kernel (warp/_src/sparse.py:4627-4715, launched from
+ * bsr_mv, 4792-4913).  alpha == 0 leaves x unread, beta == 0 leaves y unread.
+ * nnz_hint: stored-block count (or upper bound) used to pick lanes per row;
+ * <= 0 for a default.  1x1 and 3x3 blocks use the vector-BSR kernels, other
+ * shapes a one-thread-per-scalar-row kernel. */
+int warp_amd_bsr_mv_f32(
+    int nrow, int ncol, int block_rows, int block_cols, const int* offsets, const int* row_counts,
+    const int* columns, const float* values, int64_t nnz_hint, const float* x, float* y, float alpha, float beta,
+    void* stream);
+int warp_amd_bsr_mv_f64(
+    int nrow, int ncol, int block_rows, int block_cols, const int* offsets, const int* row_counts,
+    const int* columns, const double* values, int64_t nnz_hint, const double* x, double* y, double alpha,
+    double beta, void* stream);
+
+/* out[r] = scale * A[r, r] (zero block where absent), r < min(nrow, ncol).
+ * Replaces _bsr_get_diag_kernel (warp/_src/sparse.py:2871-2924). */
+int warp_amd_bsr_get_diag_f32(
+    int nrow, int ncol, int block_rows, int block_cols, const int* offsets, const int* row_counts,
+    const int* columns, const float* values, float scale, float* out, void* stream);
+int warp_amd_bsr_get_diag_f64(
+    int nrow, int ncol, int block_rows, int block_cols, const int* offsets, const int* row_counts,
+    const int* columns, const double* values, double scale, double* out, void* stream);
+
+/* Jacobi data straight from the matrix.  mode 0: out[nrow * br] = 1/d of the
+ * diagonal entries of each diagonal block (1/|d| with use_abs, 1 where d == 0)
+ * -- replaces bsr_get_diag + _extract_inverse_diagonal_{blocked,scalar}
+ * (warp/_src/optim/linear.py:256-280, 2290-2320).  mode 1 (extension, no
+ * reference counterpart): out[nrow * br * br] = inverse of each diagonal
+ * block (block Jacobi; singular blocks fall back to mode 0 values). */
+int warp_amd_bsr_jacobi_f32(
+    int nrow, int ncol, int block_rows, int block_cols, const int* offsets, const int* row_counts,
+    const int* columns, const float* values, int use_abs, int mode, float* out, void* stream);
+int warp_amd_bsr_jacobi_f64(
+    int nrow, int ncol, int block_rows, int block_cols, const int* offsets, const int* row_counts,
+    const int* columns, const double* values, int use_abs, int mode, double* out, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Conjugate gradient                                                       */
+/* ------------------------------------------------------------------------ */
+
+typedef int (*warp_amd_cg_callback)(int iteration, double residual_norm, double abs_tolerance, void* user);
+
+size_t warp_amd_cg_workspace_size(int64_t n, int scalar_size);
+/* byte offset inside the CG workspace of: 0 the iteration count (int32),
+ * 1 r.r (double), 2 atol^2 (double) -- the device results of check_every == 0 */
+size_t warp_amd_cg_state_offset(int which);
+
+/* Preconditioned CG on a compact BSR matrix with 1x1 or 3x3 blocks.
+ * Replaces CG._run / cg (warp/_src/optim/linear.py:893-1121) with its kernels
+ * _cg_kernel_1/2 (2332-2373), TiledDot (309-612), _initialize_tolerance
+ * (2051-2083) and the host loop _run_solver_loop (2182-2243).
+ *
+ * prec_kind 0 none, 1 diagonal (prec = n scalars), 2 block (prec = nrow
+ * block_size^2 scalars).  tol / atol: NaN = None (defaults of
+ * linear.py:2010-2032).  maxiter <= 0 = n.  check_every > 0: residual checks
+ * and callback every check_every iterations, results in *iterations,
+ * *residual_norm, *abs_tolerance (host).  check_every == 0: no host checks;
+ * the convergence flag stays on the device and results are read from the
+ * workspace (warp_amd_cg_state_offset).  restart > 0: recompute r = b - A x
+ * every `restart` iterations.  x: initial guess in, solution out. */
+int warp_amd_cg_solve_f32(
+    int nrow, int block_size, const int* offsets, const int* columns, const float* values, int64_t nnz_hint,
+    int prec_kind, const float* prec, const float* b, float* x, double tol, double atol, int maxiter,
+    int check_every, int restart, warp_amd_cg_callback callback, void* callback_user, void* workspace,
+    size_t workspace_bytes, int* iterations, double* residual_norm, double* abs_tolerance, void* stream);
+int warp_amd_cg_solve_f64(
+    int nrow, int block_size, const int* offsets, const int* columns, const double* values, int64_t nnz_hint,
+    int prec_kind, const double* prec, const double* b, double* x, double tol, double atol, int maxiter,
+    int check_every, int restart, warp_amd_cg_callback callback, void* callback_user, void* workspace,
+    size_t workspace_bytes, int* iterations, double* residual_norm, double* abs_tolerance, void* stream);
+
+/* Generic CG building blocks for caller-supplied operators.
+ * *out = sum a[i] b[i] accumulated in FP64, deterministic (replaces TiledDot
+ * block_dot_kernel / block_sum_kernel, linear.py:580-612); the workspace
+ * (warp_amd_dot_workspace_size) must be zero-filled before its first use. */
+size_t warp_amd_dot_workspace_size(int64_t n);
+int warp_amd_dot_f32(int64_t n, const float* a, const float* b, double* out, void* workspace, size_t workspace_bytes,
+                     void* stream);
+int warp_amd_dot_f64(int64_t n, const double* a, const double* b, double* out, void* workspace,
+                     size_t workspace_bytes, void* stream);
+/* *atol_sq = max(rtol * sqrt(*bb), atol)^2 with the tolerance defaults
+ * (linear.py:2010-2060); device scalars. */
+int warp_amd_cg_tolerance(const double* bb, double tol, double atol, int is_f64, double* atol_sq, void* stream);
+/* alpha = rr > atol_sq ? rz / pAp : 0; x += alpha p; r -= alpha Ap (_cg_kernel_1, linear.py:2332-2353) */
+int warp_amd_cg_step_xr_f32(int64_t n, float* x, float* r, const float* p, const float* Ap, const double* rr,
+                            const double* atol_sq, const double* rz, const double* pAp, void* stream);
+int warp_amd_cg_step_xr_f64(int64_t n, double* x, double* r, const double* p, const double* Ap, const double* rr,
+                            const double* atol_sq, const double* rz, const double* pAp, void* stream);
+/* beta = rr > atol_sq ? rz_new / rz_old : 0; p = z + beta p (_cg_kernel_2, linear.py:2355-2373) */
+int warp_amd_cg_step_p_f32(int64_t n, const float* z, float* p, const double* rr, const double* atol_sq,
+                           const double* rz_new, const double* rz_old, void* stream);
+int warp_amd_cg_step_p_f64(int64_t n, const double* z, double* p, const double* rr, const double* atol_sq,
+                           const double* rz_new, const double* rz_old, void* stream);
+/* z = alpha (d o x) + beta y (_diag_mv_kernel, linear.py:2271-2287) */
+int warp_amd_diag_mv_f32(int64_t n, const float* d, const float* x, const float* y, float* z, float alpha, float beta,
+                         void* stream);
+int warp_amd_diag_mv_f64(int64_t n, const double* d, const double* x, const double* y, double* z, double alpha,
+                         double beta, void* stream);
+/* z = alpha A x + beta y, A dense row-major m x k (_dense_mv_kernel, linear.py:2246-2268) */
+int warp_amd_dense_mv_f32(int m, int k, const float* A, const float* x, const float* y, float* z, float alpha,
+                          float beta, void* stream);
+int warp_amd_dense_mv_f64(int m, int k, const double* A, const double* x, const double* y, double* z, double alpha,
+                          double beta, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WARP_AMD_H */

@@ -1,0 +1,240 @@
+"""CPU oracle for the sparse-solver path -- TEST INFRASTRUCTURE ONLY.
+
+Wraps ``warp_oracle.c`` (a plain-C restatement of the reference algorithms,
+each function citing the reference lines it follows) and, when it has been
+built, ``_ref/libwarp_ref.so``: the reference's own CPU native library
+compiled from its unmodified sources (``make ref``), used only to check the
+restatement's triplet topology.
+
+Only tests, the package smoke check and the bench's ``cpu_baseline`` leg use
+this module, always as the checker / baseline and never as the product path.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libwarp_ref.so")
+
+_lib = None
+_ref = None
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+
+
+def build(ref: bool = False) -> None:
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    if ref:
+        subprocess.run(["make", "-s", "-j8", "-C", HERE, "ref"], check=True)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        L.orc_bsr_from_triplets.restype = _I
+        L.orc_bsr_from_triplets.argtypes = [_I, _I, _I, _I, _I, _P, _P, _P, _P, ctypes.c_uint64, _P, _P, _P, _P, _P]
+        for s, S in (("f32", ctypes.c_float), ("f64", ctypes.c_double)):
+            fn = getattr(L, f"orc_bsr_mv_{s}")
+            fn.restype = None
+            fn.argtypes = [_I, _I, _I, _P, _P, _P, _P, _P, _P, S, S]
+            fn = getattr(L, f"orc_bsr_get_diag_{s}")
+            fn.restype = None
+            fn.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _P]
+            fn = getattr(L, f"orc_inverse_diag_{s}")
+            fn.restype = None
+            fn.argtypes = [_I, _I, _I, _P, _I, _P]
+        for s in ("f32", "f64", "f32d"):
+            fn = getattr(L, f"orc_cg_{s}")
+            fn.restype = None
+            fn.argtypes = [_I, _I, _P, _P, _P, _P, _P, _D, _D, _I, _I, _P, _I, _I, _P, _P, _I]
+        L.orc_residual_f32_f64.restype = None
+        L.orc_residual_f32_f64.argtypes = [_I, _I, _P, _P, _P, _P, _P, _P]
+        _lib = L
+    return _lib
+
+
+def ref_lib() -> ctypes.CDLL | None:
+    """The reference's CPU native library, or None when it was not built."""
+    global _ref
+    if _ref is None and os.path.exists(REF_LIB):
+        L = ctypes.CDLL(REF_LIB)
+        L.wp_bsr_matrix_from_triplets_host.restype = None
+        L.wp_bsr_matrix_from_triplets_host.argtypes = [
+            _I, _I, _I, _I, _I, _P, _P, _P, _P, ctypes.c_uint64, ctypes.c_bool, _P, _P, _P, _P, _P, _P, _P,
+        ]
+        _ref = L
+    return _ref
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+_ZERO_MASK = {4: 0x7FFFFFFF, 8: 0x7FFFFFFFFFFFFFFF, 2: 0x7FFF}
+
+
+def bsr_from_triplets(nrow, ncol, rows, cols, values=None, prune=True, count=None):
+    """Compact BSR from triplets (stable order; see warp_oracle.c).
+
+    ``values``: (T,) or (T, br, bc) float32/float64, or None (topology only).
+    Returns dict(offsets, columns, values, summed_offsets, summed_indices, nnz).
+    """
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    T = rows.shape[0]
+    if values is not None:
+        values = np.ascontiguousarray(values)
+        bshape = values.shape[1:] if values.ndim == 3 else ()
+        block_size = int(np.prod(bshape)) if bshape else 1
+        sb = values.dtype.itemsize
+        mask = _ZERO_MASK[sb] if prune else 0
+    else:
+        bshape, block_size, sb, mask = (), 1, 0, 0
+    so = np.zeros(max(T, 1), np.int32)
+    si = np.zeros(max(T, 1), np.int32)
+    off = np.zeros(nrow + 1, np.int32)
+    colsout = np.zeros(max(T, 1), np.int32)
+    out_vals = np.zeros((max(T, 1), *bshape), values.dtype) if values is not None else None
+    cnt = None if count is None else np.array([count], np.int32)
+    nnz = lib().orc_bsr_from_triplets(
+        block_size, sb, nrow, ncol, T, _p(cnt), _p(rows), _p(cols), _p(values), mask, _p(so), _p(si), _p(off),
+        _p(colsout), _p(out_vals),
+    )
+    valid = int(so[nnz - 1]) if nnz > 0 else 0
+    return dict(
+        offsets=off,
+        columns=colsout[:nnz],
+        values=None if out_vals is None else out_vals[:nnz],
+        summed_offsets=so[:nnz],
+        summed_indices=si[:valid],
+        nnz=nnz,
+    )
+
+
+def ref_bsr_from_triplets(nrow, ncol, rows, cols, values=None, prune=True, count=None):
+    """Topology from the reference's own host function (sparse.cpp:267-413)."""
+    L = ref_lib()
+    if L is None:
+        return None
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    T = rows.shape[0]
+    if values is not None:
+        values = np.ascontiguousarray(values)
+        block_size = int(np.prod(values.shape[1:])) if values.ndim == 3 else 1
+        sb = values.dtype.itemsize
+        mask = _ZERO_MASK[sb] if prune else 0
+    else:
+        block_size, sb, mask = 1, 0, 0
+    so = np.zeros(max(T, 1), np.int32)
+    si = np.zeros(max(T, 1), np.int32)
+    off = np.zeros(nrow + 1, np.int32)
+    colsout = np.zeros(max(T, 1), np.int32)
+    nnz = np.zeros(1, np.int32)
+    cnt = None if count is None else np.array([count], np.int32)
+    L.wp_bsr_matrix_from_triplets_host(
+        block_size, sb, nrow, ncol, T, _p(cnt), _p(rows), _p(cols), _p(values), mask, False, _p(so), _p(si), _p(off),
+        None, _p(colsout), _p(nnz), None,
+    )
+    n = int(nnz[0])
+    return dict(offsets=off, columns=colsout[:n], summed_offsets=so[:n], summed_indices=si, nnz=n)
+
+
+def bsr_mv(offsets, columns, values, x, y=None, alpha=1.0, beta=0.0, row_counts=None):
+    """y = alpha A x + beta y, reference row-kernel order (sparse.py:4627-4665)."""
+    values = np.ascontiguousarray(values)
+    dt = values.dtype
+    br, bc = (values.shape[1], values.shape[2]) if values.ndim == 3 else (1, 1)
+    nrow = offsets.shape[0] - 1
+    x = np.ascontiguousarray(x, dtype=dt).reshape(-1)
+    yo = np.zeros(nrow * br, dt) if y is None else np.ascontiguousarray(y, dtype=dt).reshape(-1).copy()
+    fn = lib().orc_bsr_mv_f64 if dt == np.float64 else lib().orc_bsr_mv_f32
+    offsets = np.ascontiguousarray(offsets, np.int32)
+    columns = np.ascontiguousarray(columns, np.int32)
+    rc = None if row_counts is None else np.ascontiguousarray(row_counts, np.int32)
+    fn(nrow, br, bc, _p(offsets), _p(rc), _p(columns), _p(values), _p(x), _p(yo), alpha, beta)
+    return yo
+
+
+def bsr_get_diag(offsets, columns, values, ncol=None, row_counts=None):
+    values = np.ascontiguousarray(values)
+    dt = values.dtype
+    br, bc = (values.shape[1], values.shape[2]) if values.ndim == 3 else (1, 1)
+    nrow = offsets.shape[0] - 1
+    ncol = nrow if ncol is None else ncol
+    dim = min(nrow, ncol)
+    out = np.zeros((dim, br, bc), dt)
+    fn = lib().orc_bsr_get_diag_f64 if dt == np.float64 else lib().orc_bsr_get_diag_f32
+    offsets = np.ascontiguousarray(offsets, np.int32)
+    columns = np.ascontiguousarray(columns, np.int32)
+    rc = None if row_counts is None else np.ascontiguousarray(row_counts, np.int32)
+    fn(nrow, ncol, br, bc, _p(offsets), _p(rc), _p(columns), _p(values), _p(out))
+    return out if values.ndim == 3 else out.reshape(dim)
+
+
+def inverse_diag(diag_blocks, use_abs=False):
+    d = np.ascontiguousarray(diag_blocks)
+    dt = d.dtype
+    br, bc = (d.shape[1], d.shape[2]) if d.ndim == 3 else (1, 1)
+    dim = d.shape[0]
+    out = np.zeros(dim * br, dt)
+    fn = lib().orc_inverse_diag_f64 if dt == np.float64 else lib().orc_inverse_diag_f32
+    fn(dim, br, bc, _p(d), 1 if use_abs else 0, _p(out))
+    return out
+
+
+PREC_NONE, PREC_DIAG, PREC_BLOCK = 0, 1, 2
+
+
+def cg(offsets, columns, values, b, x0, tol=None, atol=None, maxiter=0, prec_kind=PREC_NONE, prec=None,
+       check_every=10, restart=0, dots="fp64", history_cap=4096):
+    """Reference-semantics CG (see orc_cg in warp_oracle.c).
+
+    dots: "fp64" (FP64 dots and alpha/beta, fp32 vectors) or "native" (all in
+    the system scalar type, the reference's precision).
+    Returns (x, iterations, residual_norm, atol, history).
+    """
+    values = np.ascontiguousarray(values)
+    dt = values.dtype
+    br = values.shape[1] if values.ndim == 3 else 1
+    nrow = offsets.shape[0] - 1
+    b = np.ascontiguousarray(b, dtype=dt).reshape(-1)
+    x = np.ascontiguousarray(x0, dtype=dt).reshape(-1).copy()
+    if dt == np.float64:
+        fn = lib().orc_cg_f64
+    else:
+        fn = lib().orc_cg_f32d if dots == "fp64" else lib().orc_cg_f32
+    pr = None if prec is None else np.ascontiguousarray(prec, dtype=dt).reshape(-1)
+    out = np.zeros(4)
+    hist = np.zeros(history_cap)
+    nan = float("nan")
+    fn(
+        nrow, br, _p(np.ascontiguousarray(offsets, np.int32)), _p(np.ascontiguousarray(columns, np.int32)),
+        _p(values), _p(b), _p(x), nan if tol is None else float(tol), nan if atol is None else float(atol),
+        int(maxiter), int(prec_kind), _p(pr), int(check_every), int(restart), _p(out), _p(hist), history_cap,
+    )
+    return x, int(out[0]), float(out[1]), float(out[2]), hist[: int(out[3])]
+
+
+def residual_f32_f64(offsets, columns, values, x, b):
+    values = np.ascontiguousarray(values, dtype=np.float32)
+    br = values.shape[1] if values.ndim == 3 else 1
+    nrow = offsets.shape[0] - 1
+    r = np.zeros(nrow * br)
+    lib().orc_residual_f32_f64(
+        nrow, br, _p(np.ascontiguousarray(offsets, np.int32)), _p(np.ascontiguousarray(columns, np.int32)),
+        _p(values), _p(np.ascontiguousarray(x, np.float64).reshape(-1)),
+        _p(np.ascontiguousarray(b, np.float64).reshape(-1)), _p(r),
+    )
+    return r

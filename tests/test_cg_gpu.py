@@ -1,0 +1,192 @@
+"""GPU conjugate gradient: reference solver scenarios and parity with the oracle.
+
+Scenarios of the reference suite (test_linear_solvers.py:15-126, 129-253)
+check err <= atol and the true residual (<= 2 atol in fp64, <= 32 atol in
+fp32).  On FEM systems the native solver is compared with the oracle CG
+that uses the same FP64-dot policy: same iteration count (checks happen at
+the same multiples of check_every) and solutions within 1e-5 relative.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import warp_amd as wa
+from warp_amd import problems, sparse
+from warp_amd.optim import linear
+
+pytestmark = pytest.mark.gpu
+
+
+def _spd(n, seed, dtype, device):
+    rng = np.random.default_rng(seed)
+    C = rng.uniform(-100, 100, (n, n))
+    f = rng.uniform(-100, 100, n)
+    return torch.tensor(C @ C.T, dtype=dtype, device=device), torch.tensor(C @ f, dtype=dtype, device=device)
+
+
+def _check_linear_solve(A, b, **kwargs):
+    x = torch.zeros_like(b)
+    niter, err, atol = linear.cg(A, b, x, use_cuda_graph=True, **kwargs)
+    assert err <= atol
+    # device-side convergence (check_every = 0) returns device tensors
+    x.zero_()
+    niter0, err0, atol0 = linear.cg(A, b, x, check_every=0, **kwargs)
+    assert math.sqrt(float(err0.item())) <= math.sqrt(float(atol0.item()))
+    # warm start
+    niter_warm, err, atol = linear.cg(A, b, x, use_cuda_graph=False, **kwargs)
+    assert err <= atol
+    An = A.cpu().numpy().astype(np.float64) if isinstance(A, torch.Tensor) else None
+    if An is not None:
+        res = np.linalg.norm(An @ x.cpu().numpy().astype(np.float64) - b.cpu().numpy())
+        assert res <= (2.0 if b.dtype == torch.float64 else 32.0) * atol
+
+
+def test_cg_reference_dense_scenarios(device):
+    """test_cg (test_linear_solvers.py:112-126), generic operator route."""
+    A, b = _spd(64, 123, torch.float64, device)
+    M = linear.preconditioner(A, "diag")
+    _check_linear_solve(A, b, maxiter=1000)
+    _check_linear_solve(A, b, M=M, maxiter=1000)
+    A, b = _spd(16, 321, torch.float32, device)
+    M = linear.preconditioner(A, "diag")
+    _check_linear_solve(A, b, maxiter=1000)
+    _check_linear_solve(A, b, M=M, maxiter=1000)
+    rng = np.random.default_rng(321)
+    A = torch.eye(5, dtype=torch.float32, device=device)
+    b = torch.tensor(rng.uniform(-1, 1, 5), dtype=torch.float32, device=device)
+    _check_linear_solve(A, b, maxiter=30)
+
+
+def test_cg_restart_callbacks(device):
+    """test_conjugate_solver_restart (test_linear_solvers.py:129-172) for cg."""
+    rng = np.random.default_rng(2027)
+    C = rng.standard_normal((12, 12))
+    A_np = C.T @ C + 4.0 * np.eye(12)
+    b_np = rng.standard_normal(12)
+    A = torch.tensor(A_np, device=device)
+    b = torch.tensor(b_np, device=device)
+    M = linear.preconditioner(A, "diag")
+    for prec in (None, M):
+        its = []
+        x = torch.zeros_like(b)
+        niter, err, atol = linear.cg(A, b, x, tol=1e-10, maxiter=120, M=prec, callback=lambda i, r, t: its.append(i),
+                                     check_every=1, restart=3)
+        assert err <= atol and niter % 3 == 0
+        assert its[0] == 0 and its[-1] == niter and all(i % 3 == 0 for i in its)
+        assert np.linalg.norm(A_np @ x.cpu().numpy() - b_np) <= atol
+
+
+def _bsr_from(device, nn, rows, cols, vals, prune=True):
+    block = wa.mat33 if vals.ndim == 3 else wa.float32
+    A = sparse.bsr_zeros(nn, nn, block, device=device)
+    sparse.bsr_set_from_triplets(A, rows.to(device), cols.to(device), vals.to(device), prune_numerical_zeros=prune)
+    A.nnz_sync()
+    return A
+
+
+@pytest.mark.parametrize("n", [32, 128])
+def test_cg_poisson_vs_oracle(device, oracle_mod, n):
+    """2D Q1 Poisson, Jacobi CG, tol 1e-6, maxiter 2000 (the CPU-runnable workload at n=128)."""
+    rows, cols, vals, b, nd = problems.poisson2d_q1(n, 0)
+    A = _bsr_from(device, nd, torch.from_numpy(rows), torch.from_numpy(cols), torch.from_numpy(vals))
+    M = linear.preconditioner(A, "diag")
+    bt = torch.from_numpy(b).to(device)
+    x = torch.zeros_like(bt)
+    its = []
+    niter, err, atol = linear.cg(A, bt, x, tol=1e-6, maxiter=2000, M=M, callback=lambda i, r, t: its.append((i, r)))
+    o = oracle_mod.bsr_from_triplets(nd, nd, rows, cols, vals)
+    inv = oracle_mod.inverse_diag(oracle_mod.bsr_get_diag(o["offsets"], o["columns"], o["values"]))
+    xo, it_o, err_o, atol_o, hist = oracle_mod.cg(o["offsets"], o["columns"], o["values"], b, np.zeros(nd), tol=1e-6,
+                                                 maxiter=2000, prec_kind=oracle_mod.PREC_DIAG, prec=inv)
+    print(f"[cg poisson n={n}] gpu iters {niter} err {err:.3e} | oracle iters {it_o} err {err_o:.3e} atol {atol:.3e}")
+    assert err <= atol
+    assert atol == pytest.approx(atol_o, rel=1e-12)
+    assert abs(niter - it_o) <= 10
+    xg = x.cpu().numpy()
+    if niter == it_o:
+        assert np.linalg.norm(xg - xo) <= 1e-5 * np.linalg.norm(xo)
+    # FP64 true residual of the FP32 solution
+    import scipy.sparse as sp
+
+    A64 = sp.csr_matrix((o["values"].astype(np.float64), o["columns"], o["offsets"]), shape=(nd, nd))
+    assert np.linalg.norm(b - A64 @ xg.astype(np.float64)) <= 32 * atol
+    assert its[0][0] == 0 and its[-1][0] == niter
+
+
+@pytest.mark.parametrize("ptype", ["diag", "block_diag", "id"])
+def test_cg_elasticity_vs_oracle(device, oracle_mod, ptype):
+    n = 8
+    rows, cols, vals, b, nn = problems.elasticity3d_q1(n, device="cpu")
+    A = _bsr_from(device, nn, rows, cols, vals, prune=False)
+    M = linear.preconditioner(A, ptype)
+    bt = b.to(device)
+    x = torch.zeros_like(bt)
+    niter, err, atol = linear.cg(A, bt, x, tol=1e-6, atol=0.0, maxiter=5000, M=M)
+    o = oracle_mod.bsr_from_triplets(nn, nn, rows.numpy(), cols.numpy(), vals.numpy(), prune=False)
+    if ptype == "id":
+        pk, pd = oracle_mod.PREC_NONE, None
+    elif ptype == "diag":
+        pk, pd = oracle_mod.PREC_DIAG, M._prec_data.cpu().numpy()
+    else:
+        pk, pd = oracle_mod.PREC_BLOCK, M._prec_data.cpu().numpy()
+    xo, it_o, err_o, atol_o, _ = oracle_mod.cg(o["offsets"], o["columns"], o["values"], b.numpy(), np.zeros(3 * nn),
+                                              tol=1e-6, atol=0.0, maxiter=5000, prec_kind=pk, prec=pd)
+    print(f"[cg elasticity {ptype}] gpu iters {niter} err {err:.3e} | oracle iters {it_o} err {err_o:.3e}")
+    assert err <= atol
+    assert abs(niter - it_o) <= max(10, it_o // 10)
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+
+    A64 = sp.bsr_matrix((o["values"].astype(np.float64), o["columns"], o["offsets"]), shape=(3 * nn, 3 * nn)).tocsr()
+    xs = spl.spsolve(A64.tocsc(), b.numpy().astype(np.float64).reshape(-1))
+    xg = x.cpu().numpy().reshape(-1).astype(np.float64)
+    rel64 = np.linalg.norm(xg - xs) / np.linalg.norm(xs)
+    print(f"[cg elasticity {ptype}] rel. error vs FP64 direct solve = {rel64:.3e}")
+    assert np.linalg.norm(b.numpy().reshape(-1) - A64 @ xg) <= 32 * atol
+    if niter == it_o:
+        assert np.linalg.norm(xg - xo) <= 1e-4 * np.linalg.norm(xo)
+
+
+def test_cg_native_check_every_zero_and_restart(device, oracle_mod):
+    rows, cols, vals, b, nd = problems.poisson2d_q1(24, 0)
+    A = _bsr_from(device, nd, torch.from_numpy(rows), torch.from_numpy(cols), torch.from_numpy(vals))
+    M = linear.preconditioner(A, "diag")
+    bt = torch.from_numpy(b).to(device)
+    x = torch.zeros_like(bt)
+    it, rr, at = linear.cg(A, bt, x, tol=1e-6, atol=0.0, maxiter=500, M=M, check_every=0)
+    assert isinstance(it, torch.Tensor)
+    assert float(rr.item()) <= float(at.item())
+    x1 = torch.zeros_like(bt)
+    n1, e1, a1 = linear.cg(A, bt, x1, tol=1e-6, atol=0.0, maxiter=500, M=M, check_every=1)
+    # device-side convergence stops at the first converged iteration, like check_every=1
+    assert int(it.item()) == n1
+    np.testing.assert_array_equal(x.cpu().numpy(), x1.cpu().numpy())
+    # restart: true residual at the cycle boundary
+    x2 = torch.zeros_like(bt)
+    n2, e2, a2 = linear.cg(A, bt, x2, tol=1e-6, atol=0.0, maxiter=512, M=M, check_every=32, restart=32)
+    assert n2 % 32 == 0 and e2 <= a2
+    o = oracle_mod.bsr_from_triplets(nd, nd, rows, cols, vals)
+    import scipy.sparse as sp
+
+    A64 = sp.csr_matrix((o["values"].astype(np.float64), o["columns"], o["offsets"]), shape=(nd, nd))
+    assert np.linalg.norm(b - A64 @ x2.cpu().numpy()) <= 2 * a2
+
+
+def test_cg_functor_reuse(device):
+    rows, cols, vals, b, nd = problems.poisson2d_q1(16, 0)
+    A = _bsr_from(device, nd, torch.from_numpy(rows), torch.from_numpy(cols), torch.from_numpy(vals))
+    bt = torch.from_numpy(b).to(device)
+    x = torch.zeros_like(bt)
+    solver = linear.cg(A, bt, x, tol=1e-6, run=False)
+    n1, e1, a1 = solver()
+    x2 = torch.zeros_like(bt)
+    n2, e2, a2 = solver(b=bt, x=x2)
+    assert n1 == n2 and e1 == e2
+    np.testing.assert_array_equal(x.cpu().numpy(), x2.cpu().numpy())
+    with pytest.raises(ValueError, match="Incompatible b.shape"):
+        solver(b=torch.zeros(nd + 1, device=device))

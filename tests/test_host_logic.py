@@ -1,0 +1,115 @@
+"""Host-side logic of the drop-in API (CPU only): validation messages of the
+reference, no CPU fallback, workload generators."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import warp_amd as wa
+from warp_amd import problems, sparse
+from warp_amd.optim import linear
+
+
+def test_types():
+    assert wa.mat33._shape_ == (3, 3)
+    assert wa.types.type_scalar_type(wa.mat33d) is wa.float64
+    assert wa.types.type_size(wa.matrix((3, 2), wa.float32)) == 6
+    assert wa.types.type_to_scalar(float) is wa.float32
+
+
+def test_triplets_validation_messages():
+    A = sparse.bsr_zeros(4, 9, wa.matrix((3, 2), wa.float32), device="cpu")
+    rows = torch.zeros(5, dtype=torch.float32)
+    with pytest.raises(TypeError, match=r"Rows and columns arrays must be of type int32$"):
+        sparse.bsr_set_from_triplets(A, rows, rows, torch.zeros(5, 3, 2))
+    with pytest.raises(ValueError, match="same length"):
+        sparse.bsr_set_from_triplets(A, torch.zeros(5, dtype=torch.int32), torch.zeros(4, dtype=torch.int32))
+    with pytest.raises(ValueError, match="block shape"):
+        sparse.bsr_set_from_triplets(
+            A, torch.zeros(5, dtype=torch.int32), torch.zeros(5, dtype=torch.int32), torch.zeros(5, 3, 3)
+        )
+    # valid input on a CPU tensor: no CPU fallback
+    with pytest.raises(RuntimeError, match="ROCm GPU devices only"):
+        sparse.bsr_set_from_triplets(
+            A, torch.zeros(5, dtype=torch.int32), torch.zeros(5, dtype=torch.int32), torch.ones(5, 3, 2)
+        )
+    # zero-length input resets the matrix without touching the device
+    sparse.bsr_set_from_triplets(A, torch.zeros(0, dtype=torch.int32), torch.zeros(0, dtype=torch.int32))
+    assert A.nnz == 0
+
+
+def test_bsr_mv_validation_messages():
+    A = sparse.bsr_zeros(2, 3, wa.mat33, device="cpu")
+    x = torch.zeros(9)
+    y = torch.zeros(6)
+    A.ncol += 1
+    with pytest.raises(ValueError, match="Incompatible 'x'"):
+        sparse.bsr_mv(A, x, y)
+    A.ncol -= 1
+    A.nrow -= 1
+    with pytest.raises(ValueError, match="Incompatible 'y'"):
+        sparse.bsr_mv(A, x, y)
+    A.nrow += 1
+    with pytest.raises(RuntimeError, match="ROCm GPU devices only"):
+        sparse.bsr_mv(A, x, y)
+
+
+def test_preconditioner_and_solver_arguments():
+    assert linear.preconditioner(None, "id") is None
+    with pytest.raises(ValueError, match="Unsupported preconditioner type"):
+        linear.preconditioner(torch.eye(3), "ilu")
+    with pytest.raises(TypeError, match="restart must be an integer"):
+        linear.cg(torch.eye(3), torch.zeros(3), torch.zeros(3), restart=True)
+    with pytest.raises(ValueError, match="restart must be positive"):
+        linear.cg(torch.eye(3), torch.zeros(3), torch.zeros(3), restart=0)
+    with pytest.raises(ValueError, match="max_batch_length requires batch_offsets"):
+        linear.LinearOperator((3, 3), wa.float32, torch.device("cpu"), None, max_batch_length=3)
+
+
+def test_poisson2d_generator_counts():
+    rows, cols, vals, b, n = problems.poisson2d_q1(128, 0)
+    assert n == 127 * 127
+    assert rows.shape[0] == 128 * 128 * 16
+    ok = (rows >= 0) & (cols >= 0)
+    keys = np.unique(rows[ok].astype(np.int64) * n + cols[ok])
+    assert keys.size == (3 * 127 - 2) ** 2 == 143641
+
+
+def test_hex_stiffness_properties():
+    K = problems.hex8_elasticity_stiffness(1.0 / 64)
+    assert np.allclose(K, K.T)
+    ev = np.linalg.eigvalsh(K)
+    assert np.sum(np.abs(ev) < 1e-10 * ev.max()) == 6  # 3 translations + 3 rotations
+    assert ev.min() > -1e-12 * ev.max()
+    # rigid motions are in the null space
+    corners = np.array([[a, b, c] for c in (0, 1) for b in (0, 1) for a in (0, 1)], dtype=float)
+    for t in np.eye(3):
+        assert np.abs(K @ np.tile(t, 8)).max() < 1e-10 * np.abs(K).max()
+    rot = np.cross(np.array([0.0, 0.0, 1.0]), corners).reshape(-1)
+    assert np.abs(K @ rot).max() < 1e-10 * np.abs(K).max()
+
+
+def test_elasticity_generator_small():
+    rows, cols, vals, b, nn = problems.elasticity3d_q1(4, device="cpu")
+    assert nn == 125
+    assert rows.shape[0] == 64 * 4**3 + 25
+    keys = torch.unique(rows.to(torch.int64) * nn + cols.to(torch.int64))
+    assert keys.numel() == (3 * 4 + 1) ** 3
+    assert torch.all(b[torch.arange(0, nn, 5)] == 0)
+
+
+def test_fv7_operator_properties():
+    k = torch.exp(problems.gaussian_random_field(8, 1.5, 2.0, seed=1))
+    assert abs(float(torch.log(k).std()) - 1.5) < 1e-3
+    off, col, val = problems.fv7_poisson(k)
+    n = 8
+    assert off[-1].item() == 7 * n**3 - 6 * n**2
+    import scipy.sparse as sp
+
+    A = sp.csr_matrix((val.numpy().astype(np.float64), col.numpy(), off.numpy()), shape=(n**3, n**3))
+    assert abs(A - A.T).max() < 1e-6 * abs(A).max()
+    # strictly diagonally dominant on boundary rows, weakly inside: SPD
+    assert np.linalg.eigvalsh(A.toarray()).min() > 0
+    assert np.all(np.asarray(A.sum(axis=1)).ravel() >= -1e-5)

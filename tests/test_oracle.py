@@ -1,0 +1,263 @@
+"""Pin the CPU oracle (CPU only, no GPU).
+
+The oracle is checked against (a) the golden fixtures written by the
+reference's own compiled host assembly, (b) that library live when it is
+built, (c) closed forms (numpy unique / bincount, dense products) and
+(d) textbook special cases of CG, chosen so that a dropped term, a wrong
+sign or a transposed operand fails at least one of them.
+"""
+
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+import oracle
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "triplets_ref.npz")
+
+
+def _golden_cases():
+    data = np.load(GOLDEN, allow_pickle=False)
+    names = sorted({k.split("/")[0] for k in data.files})
+    return data, names
+
+
+def _closed_form_topology(nrow, ncol, rows, cols, vals, count=None):
+    """Structure by numpy: filter, unique keys, row histogram (no sort order assumptions)."""
+    T = rows.shape[0] if count is None else min(count, rows.shape[0])
+    r, c = rows[:T].astype(np.int64), cols[:T].astype(np.int64)
+    ok = (r >= 0) & (r < nrow) & (c >= 0) & (c < ncol)
+    if vals is not None:
+        v = vals[:T].reshape(T, -1)
+        bits = v.view(np.uint32 if v.dtype == np.float32 else np.uint64)
+        mask = 0x7FFFFFFF if v.dtype == np.float32 else 0x7FFFFFFFFFFFFFFF
+        ok &= ((bits & np.array(mask, dtype=bits.dtype)) != 0).any(axis=1)
+    keys = np.unique((r[ok] << 32) | c[ok])
+    offsets = np.concatenate([[0], np.cumsum(np.bincount(keys >> 32, minlength=nrow))]).astype(np.int32)
+    return offsets, (keys & 0x7FFFFFFF).astype(np.int32)
+
+
+def test_oracle_matches_reference_golden():
+    data, names = _golden_cases()
+    assert len(names) >= 5
+    for name in names:
+        nrow, ncol, T, count = (int(v) for v in data[f"{name}/shape"])
+        count = None if count < 0 else count
+        rows, cols, vals = data[f"{name}/rows"], data[f"{name}/cols"], data[f"{name}/vals"]
+        o = oracle.bsr_from_triplets(nrow, ncol, rows, cols, vals, prune=True, count=count)
+        np.testing.assert_array_equal(o["offsets"], data[f"{name}/offsets"], err_msg=name)
+        np.testing.assert_array_equal(o["columns"], data[f"{name}/columns"], err_msg=name)
+        np.testing.assert_array_equal(o["summed_offsets"], data[f"{name}/summed_offsets"], err_msg=name)
+        # stable order: each run lists its triplets in ascending index order
+        np.testing.assert_array_equal(o["summed_indices"], data[f"{name}/run_indices_sorted"], err_msg=name)
+
+
+def test_golden_against_closed_form():
+    data, names = _golden_cases()
+    for name in names:
+        nrow, ncol, T, count = (int(v) for v in data[f"{name}/shape"])
+        count = None if count < 0 else count
+        off, cols = _closed_form_topology(nrow, ncol, data[f"{name}/rows"], data[f"{name}/cols"], data[f"{name}/vals"], count)
+        np.testing.assert_array_equal(off, data[f"{name}/offsets"], err_msg=name)
+        np.testing.assert_array_equal(cols, data[f"{name}/columns"], err_msg=name)
+
+
+@pytest.mark.skipif(oracle.ref_lib() is None, reason="reference CPU library not built (make -C oracle ref)")
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_oracle_matches_reference_live(seed):
+    rng = np.random.default_rng(seed)
+    nrow, ncol, T = 300, 200, 20000
+    rows = rng.integers(-2, nrow + 2, T).astype(np.int32)
+    cols = rng.integers(-2, ncol + 2, T).astype(np.int32)
+    vals = rng.standard_normal((T, 3, 3)).astype(np.float32)
+    vals[rng.random(T) < 0.05] = 0.0
+    o = oracle.bsr_from_triplets(nrow, ncol, rows, cols, vals)
+    r = oracle.ref_bsr_from_triplets(nrow, ncol, rows, cols, vals)
+    assert o["nnz"] == r["nnz"]
+    np.testing.assert_array_equal(o["offsets"], r["offsets"])
+    np.testing.assert_array_equal(o["columns"], r["columns"])
+    np.testing.assert_array_equal(o["summed_offsets"], r["summed_offsets"])
+
+
+def test_value_sums_sequential_order():
+    """Values are v[i0] + v[i1] + ... in ascending triplet order, in fp32."""
+    rng = np.random.default_rng(5)
+    T = 400
+    rows = rng.integers(0, 4, T).astype(np.int32)
+    cols = rng.integers(0, 3, T).astype(np.int32)
+    vals = (rng.standard_normal(T) * 10.0 ** rng.integers(-4, 4, T)).astype(np.float32)
+    o = oracle.bsr_from_triplets(4, 3, rows, cols, vals)
+    for u in range(o["nnz"]):
+        r = np.searchsorted(o["offsets"], u, side="right") - 1
+        c = o["columns"][u]
+        idx = np.nonzero((rows == r) & (cols == c))[0]
+        acc = np.float32(vals[idx[0]])
+        for i in idx[1:]:
+            acc = np.float32(acc + vals[i])
+        assert o["values"][u].tobytes() == acc.tobytes()
+        # and close to the exact sum
+        assert abs(float(o["values"][u]) - float(np.sum(vals[idx].astype(np.float64)))) <= 1e-5 * np.sum(np.abs(vals[idx]))
+
+
+def test_known_answers_from_reference_tests():
+    # test_bsr_stale_compact_tail (test_sparse.py:225-252)
+    o = oracle.bsr_from_triplets(2, 2, np.array([0, 0, 1, 1]), np.array([0, 0, 1, 1]), np.array([1.0, 2.0, 3.0, 4.0], np.float32))
+    np.testing.assert_array_equal(o["offsets"], [0, 1, 2])
+    np.testing.assert_array_equal(o["columns"], [0, 1])
+    np.testing.assert_array_equal(o["values"], [3.0, 7.0])
+    # topology only (test_sparse.py:164-172)
+    o = oracle.bsr_from_triplets(2, 3, np.array([1, 0, 1]), np.array([2, 1, 2]))
+    assert o["nnz"] == 2
+    np.testing.assert_array_equal(o["offsets"], [0, 1, 2])
+    np.testing.assert_array_equal(o["columns"], [1, 2])
+    # prune_numerical_zeros (test_sparse.py:199-222)
+    r, c, v = np.array([1, 0, 2, 3]), np.array([0, 1, 2, 3]), np.zeros(4, np.float32)
+    assert oracle.bsr_from_triplets(12, 12, r, c, v, prune=False)["nnz"] == 4
+    assert oracle.bsr_from_triplets(12, 12, r, c, v, prune=True)["nnz"] == 0
+    # duplicates summing to zero stay as explicit zeros (sparse.py:1540-1542)
+    o = oracle.bsr_from_triplets(1, 1, np.array([0, 0]), np.array([0, 0]), np.array([1.5, -1.5], np.float32))
+    assert o["nnz"] == 1 and o["values"][0] == 0.0
+
+
+def _dense(o, nrow, ncol, bshape):
+    br, bc = bshape
+    A = np.zeros((nrow * br, ncol * bc))
+    for r in range(nrow):
+        for k in range(o["offsets"][r], o["offsets"][r + 1]):
+            c = o["columns"][k]
+            A[r * br : (r + 1) * br, c * bc : (c + 1) * bc] += np.asarray(o["values"][k], np.float64).reshape(br, bc)
+    return A
+
+
+@pytest.mark.parametrize("bshape,dtype", [((1, 1), np.float32), ((1, 3), np.float32), ((3, 3), np.float64), ((3, 2), np.float32)])
+def test_bsr_mv_matches_dense(bshape, dtype):
+    """make_test_bsr_mv scenarios (test_sparse.py:830-903): alpha/beta pairs, dense reference."""
+    rng = np.random.default_rng(123)
+    nrow, ncol, nnz = 2, 3, 6
+    rows = rng.integers(0, nrow, nnz).astype(np.int32)
+    cols = rng.integers(0, ncol, nnz).astype(np.int32)
+    vals = rng.random((nnz, *bshape)).astype(dtype)
+    o = oracle.bsr_from_triplets(nrow, ncol, rows, cols, vals if bshape != (1, 1) else vals.reshape(nnz))
+    A = _dense(o, nrow, ncol, bshape)
+    x = rng.random(ncol * bshape[1]).astype(dtype)
+    y = rng.random(nrow * bshape[0]).astype(dtype)
+    vv = o["values"] if bshape != (1, 1) else o["values"].reshape(-1, 1, 1)
+    for alpha, beta in [(-1.0, 2.0), (0.0, -1.0), (1.0, 0.0)]:
+        ref = alpha * A @ x + beta * y
+        res = oracle.bsr_mv(o["offsets"], o["columns"], vv, x, y, alpha, beta)
+        np.testing.assert_allclose(res, ref, rtol=1e-5, atol=1e-5)
+    # alpha == 0 leaves x unread, beta == 0 leaves y unread
+    nan_x = np.full_like(x, np.nan)
+    np.testing.assert_array_equal(oracle.bsr_mv(o["offsets"], o["columns"], vv, nan_x, y, 0.0, 2.0), 2.0 * y)
+    nan_y = np.full_like(y, np.nan)
+    assert np.all(np.isfinite(oracle.bsr_mv(o["offsets"], o["columns"], vv, x, nan_y, 1.0, 0.0)))
+
+
+def test_diag_and_jacobi_closed_form():
+    # test_bsr_get_set_diag (test_sparse.py:486-510)
+    rng = np.random.default_rng(123)
+    rows = np.array([0, 1, 2, 3, 2, 1], np.int32)
+    cols = np.array([1, 1, 1, 3, 2, 2], np.int32)
+    vals = rng.random((6, 3, 3)).astype(np.float32)
+    o = oracle.bsr_from_triplets(4, 4, rows, cols, vals)
+    d = oracle.bsr_get_diag(o["offsets"], o["columns"], o["values"])
+    np.testing.assert_array_equal(d[0], np.zeros((3, 3)))
+    np.testing.assert_array_equal(d[1], vals[1])
+    np.testing.assert_array_equal(d[2], vals[4])
+    np.testing.assert_array_equal(d[3], vals[3])
+    inv = oracle.inverse_diag(d)
+    expect = np.where(np.diagonal(d, axis1=1, axis2=2) == 0, 1.0, 1.0 / np.diagonal(d, axis1=1, axis2=2)).reshape(-1)
+    np.testing.assert_allclose(inv, expect, rtol=1e-7)
+    dd = np.array([-2.0, 0.0, 4.0], np.float64)
+    np.testing.assert_array_equal(oracle.inverse_diag(dd, use_abs=True), [0.5, 1.0, 0.25])
+    np.testing.assert_array_equal(oracle.inverse_diag(dd), [-0.5, 1.0, 0.25])
+
+
+def _csr(A):
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    return A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data
+
+
+def test_cg_identity_and_diagonal():
+    n = 5
+    rng = np.random.default_rng(321)
+    b = rng.uniform(-1, 1, n)
+    off, col, val = _csr(np.eye(n))
+    x, it, err, atol, hist = oracle.cg(off, col, val.astype(np.float32), b, np.zeros(n), maxiter=30)
+    # converges after one iteration, reported at the first check (check_every=10)
+    assert it == 10 and err <= atol
+    np.testing.assert_array_equal(x, b.astype(np.float32))
+    x, it, err, atol, _ = oracle.cg(off, col, val.astype(np.float32), b, np.zeros(n), maxiter=30, check_every=1)
+    assert it == 1
+    # diagonal SPD with Jacobi: one iteration
+    d = rng.uniform(1, 10, n)
+    off, col, val = _csr(np.diag(d))
+    x, it, err, atol, _ = oracle.cg(off, col, val, b, np.zeros(n), tol=1e-12, prec_kind=oracle.PREC_DIAG, prec=1 / d, check_every=1)
+    assert it == 1
+    np.testing.assert_allclose(x, b / d, rtol=1e-14)
+
+
+def test_cg_finite_termination_and_reference_bounds():
+    # SPD C C^T (test_linear_solvers.py:64-80, n=16 fp64): n distinct eigenvalues -> <= n iterations
+    rng = np.random.default_rng(123)
+    n = 16
+    C = rng.uniform(-100, 100, (n, n))
+    f = rng.uniform(-100, 100, n)
+    A = C @ C.T
+    b = C @ f
+    off, col, val = _csr(A)
+    x, it, err, atol, hist = oracle.cg(off, col, val, b, np.zeros(n), tol=1e-10, maxiter=1000, check_every=1)
+    assert err <= atol
+    xs = np.linalg.solve(A, b)
+    assert np.linalg.norm(A @ x - b) <= 2.0 * atol
+    np.testing.assert_allclose(x, xs, rtol=1e-4)
+    # callback history: first entry at iteration 0 is ||b||, last is the final residual
+    assert hist[0] == pytest.approx(np.linalg.norm(b), rel=1e-12)
+    assert hist[-1] == pytest.approx(err)
+
+
+def test_cg_matches_scipy_on_poisson():
+    from warp_amd import problems
+
+    rows, cols, vals, b, n = problems.poisson2d_q1(32, 0)
+    o = oracle.bsr_from_triplets(n, n, rows, cols, vals)
+    A = sp.csr_matrix((o["values"].astype(np.float64), o["columns"], o["offsets"]), shape=(n, n))
+    inv = oracle.inverse_diag(oracle.bsr_get_diag(o["offsets"], o["columns"], o["values"]))
+    for dots in ("fp64", "native"):
+        x, it, err, atol, _ = oracle.cg(o["offsets"], o["columns"], o["values"], b, np.zeros(n), tol=1e-6, atol=0.0,
+                                       maxiter=2000, prec_kind=oracle.PREC_DIAG, prec=inv, dots=dots)
+        assert err <= atol and it % 10 == 0
+        assert np.linalg.norm(b - A @ x) <= 32 * atol
+        xs = spl.spsolve(A.tocsc(), b.astype(np.float64))
+        assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-4
+
+
+def test_cg_restart_semantics():
+    rng = np.random.default_rng(2027)
+    C = rng.standard_normal((12, 12))
+    A = C.T @ C + 4.0 * np.eye(12)
+    b = rng.standard_normal(12)
+    off, col, val = _csr(A)
+    x, it, err, atol, hist = oracle.cg(off, col, val, b, np.zeros(12), tol=1e-10, maxiter=120, check_every=1, restart=3)
+    assert err <= atol and it % 3 == 0
+    # with restart the reported residual is the true one
+    assert np.linalg.norm(A @ x - b) <= atol
+
+
+def test_residual_fp64():
+    rng = np.random.default_rng(9)
+    A = sp.random(50, 50, density=0.2, random_state=1, format="csr") + sp.eye(50)
+    A = A.tocsr()
+    A.sort_indices()
+    x = rng.standard_normal(50)
+    b = rng.standard_normal(50)
+    v32 = A.data.astype(np.float32)
+    r = oracle.residual_f32_f64(A.indptr.astype(np.int32), A.indices.astype(np.int32), v32, x, b)
+    A64 = sp.csr_matrix((v32.astype(np.float64), A.indices, A.indptr), shape=A.shape)
+    np.testing.assert_allclose(r, b - A64 @ x, rtol=1e-13, atol=1e-13)

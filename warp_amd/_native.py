@@ -1,0 +1,134 @@
+"""ctypes binding of the gfx950 HIP library (``include/warp_amd.h``).
+
+The library is loaded after ``torch`` so that both share the single HIP
+runtime already mapped by torch.  There is no fallback: if the library is
+missing, every GPU entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch  # noqa: F401  (loads the HIP runtime the library binds to)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("WARP_AMD_LIB", os.path.join(_HERE, "_lib", "libwarp_amd.so"))
+
+_lib = None
+_lock = threading.Lock()
+
+c_int = ctypes.c_int
+c_i64 = ctypes.c_int64
+c_u64 = ctypes.c_uint64
+c_size = ctypes.c_size_t
+c_f32 = ctypes.c_float
+c_f64 = ctypes.c_double
+c_ptr = ctypes.c_void_p
+
+CG_CALLBACK = ctypes.CFUNCTYPE(c_int, c_int, c_f64, c_f64, c_ptr)
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    "warp_amd_last_error": (ctypes.c_char_p, []),
+    "warp_amd_clear_error": (None, []),
+    "warp_amd_version": (c_int, []),
+    "warp_amd_device_count": (c_int, [ctypes.POINTER(c_int)]),
+    "warp_amd_malloc": (c_int, [ctypes.POINTER(c_ptr), c_size]),
+    "warp_amd_free": (c_int, [c_ptr]),
+    "warp_amd_malloc_host": (c_int, [ctypes.POINTER(c_ptr), c_size]),
+    "warp_amd_free_host": (c_int, [c_ptr]),
+    "warp_amd_memcpy": (c_int, [c_ptr, c_ptr, c_size, c_int, c_ptr]),
+    "warp_amd_stream_synchronize": (c_int, [c_ptr]),
+    "warp_amd_set_current_stream": (None, [c_ptr]),
+    "warp_amd_get_current_stream": (c_ptr, []),
+    "warp_amd_bsr_from_triplets_workspace_size": (c_size, [c_int, c_int]),
+    "warp_amd_bsr_from_triplets": (
+        c_int,
+        [c_int, c_int, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_u64, c_int]
+        + [c_ptr] * 7
+        + [c_ptr, c_size, c_ptr],
+    ),
+    "wp_bsr_matrix_from_triplets_device": (
+        None,
+        [c_int, c_int, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_u64, ctypes.c_bool] + [c_ptr] * 7,
+    ),
+    "warp_amd_cg_workspace_size": (c_size, [c_i64, c_int]),
+    "warp_amd_cg_state_offset": (c_size, [c_int]),
+    "warp_amd_dot_workspace_size": (c_size, [c_i64]),
+    "warp_amd_cg_tolerance": (c_int, [c_ptr, c_f64, c_f64, c_int, c_ptr, c_ptr]),
+}
+
+for _sfx, _s in (("f32", c_f32), ("f64", c_f64)):
+    _SIGNATURES.update(
+        {
+            f"warp_amd_bsr_mv_{_sfx}": (c_int, [c_int, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_i64, c_ptr, c_ptr, _s, _s, c_ptr]),
+            f"warp_amd_bsr_get_diag_{_sfx}": (c_int, [c_int, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, _s, c_ptr, c_ptr]),
+            f"warp_amd_bsr_jacobi_{_sfx}": (c_int, [c_int, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_int, c_ptr, c_ptr]),
+            f"warp_amd_cg_solve_{_sfx}": (
+                c_int,
+                [c_int, c_int, c_ptr, c_ptr, c_ptr, c_i64, c_int, c_ptr, c_ptr, c_ptr, c_f64, c_f64, c_int, c_int, c_int,
+                 CG_CALLBACK, c_ptr, c_ptr, c_size, ctypes.POINTER(c_int), ctypes.POINTER(c_f64), ctypes.POINTER(c_f64), c_ptr],
+            ),
+            f"warp_amd_dot_{_sfx}": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
+            f"warp_amd_cg_step_xr_{_sfx}": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]),
+            f"warp_amd_cg_step_p_{_sfx}": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]),
+            f"warp_amd_diag_mv_{_sfx}": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr, _s, _s, c_ptr]),
+            f"warp_amd_dense_mv_{_sfx}": (c_int, [c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, _s, _s, c_ptr]),
+        }
+    )
+
+
+class NativeError(RuntimeError):
+    """Raised when a native entry point returns a non-zero status."""
+
+
+def lib() -> ctypes.CDLL:
+    """Load (once) and return the HIP library; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise NativeError(
+                    f"warp_amd HIP library not found at {LIB_PATH}; build it with "
+                    "`python -m warp_amd.csrc.build` (there is no CPU fallback)"
+                )
+            handle = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in _SIGNATURES.items():
+                fn = getattr(handle, name, None)
+                if fn is None:
+                    continue
+                fn.restype = res
+                fn.argtypes = args
+            _lib = handle
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return sorted(_SIGNATURES)
+
+
+def check(status: int, what: str = "") -> None:
+    if status != 0:
+        msg = lib().warp_amd_last_error()
+        msg = msg.decode() if msg else "unknown error"
+        raise NativeError(f"{what or 'warp_amd'} failed (status {status}): {msg}")
+
+
+def call(name: str, *args):
+    """Call an int-returning entry point and raise on failure."""
+    check(getattr(lib(), name)(*args), name)
+
+
+def ptr(t) -> int | None:
+    """Device pointer of a tensor (None for None)."""
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def stream_of(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream

@@ -1,0 +1,190 @@
+// Shared definitions for the gfx950 sparse-solver library.
+//
+// Conventions of the C ABI (include/warp_amd.h): every entry point returns an
+// int status (0 = success, < 0 = HIP or argument error) and records a message
+// readable through warp_amd_last_error().  Work is enqueued on the stream
+// passed as `void* stream` (null = the legacy default stream) and is never
+// synchronised unless the entry point documents a host read-back.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#define WA_API extern "C" __attribute__((visibility("default")))
+
+namespace wa {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------
+
+enum Status : int {
+    kOk = 0,
+    kErrHip = -1,
+    kErrArg = -2,
+    kErrUnsupported = -3,
+    kErrWorkspace = -4,
+};
+
+void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+void clear_error();
+const char* last_error_cstr();
+
+#define WA_HIP(call)                                                                                                  \
+    do {                                                                                                              \
+        hipError_t wa_err_ = (call);                                                                                  \
+        if (wa_err_ != hipSuccess) {                                                                                  \
+            ::wa::set_error("%s:%d: %s failed: %s", __FILE__, __LINE__, #call, hipGetErrorString(wa_err_));           \
+            return ::wa::kErrHip;                                                                                     \
+        }                                                                                                             \
+    } while (0)
+
+#define WA_LAUNCH_CHECK()                                                                                             \
+    do {                                                                                                              \
+        hipError_t wa_err_ = hipGetLastError();                                                                       \
+        if (wa_err_ != hipSuccess) {                                                                                  \
+            ::wa::set_error("%s:%d: kernel launch failed: %s", __FILE__, __LINE__, hipGetErrorString(wa_err_));       \
+            return ::wa::kErrHip;                                                                                     \
+        }                                                                                                             \
+    } while (0)
+
+#define WA_REQUIRE(cond, ...)                                                                                         \
+    do {                                                                                                              \
+        if (!(cond)) {                                                                                                \
+            ::wa::set_error(__VA_ARGS__);                                                                             \
+            return ::wa::kErrArg;                                                                                     \
+        }                                                                                                             \
+    } while (0)
+
+#define WA_TRY(expr)                                                                                                  \
+    do {                                                                                                              \
+        int wa_st_ = (expr);                                                                                          \
+        if (wa_st_ != 0)                                                                                              \
+            return wa_st_;                                                                                            \
+    } while (0)
+
+inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Bump allocator over a caller-provided workspace (256-byte aligned slices).
+struct Carve {
+    char* base;
+    size_t used = 0;
+    size_t cap;
+    Carve(void* p, size_t c) : base(static_cast<char*>(p)), cap(c) {}
+    template <typename T> T* take(size_t count)
+    {
+        size_t off = align_up(used, 256);
+        used = off + count * sizeof(T);
+        return base ? reinterpret_cast<T*>(base + off) : nullptr;
+    }
+    bool fits() const { return used <= cap; }
+};
+
+// Number of compute units of the current device (cached per device id).
+int device_cu_count();
+
+// Exclusive scan of n int32 values: out[i] = sum(in[0..i)), out[n] = total.
+// `in` and `out` may alias.  Workspace from scan_workspace_bytes(n).
+size_t scan_workspace_bytes(int64_t n);
+int exclusive_scan_i32(const int* in, int* out, int64_t n, void* ws, hipStream_t stream);
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+
+template <typename T> __device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Sum over aligned groups of G lanes (G a power of two <= 64).
+template <int G, typename T> __device__ __forceinline__ T group_sum(T v)
+{
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Block-wide sum of one value per thread; result valid in every thread.
+template <int BLOCK, typename T> __device__ __forceinline__ T block_sum(T v, T* lds /* >= BLOCK/64 entries */)
+{
+    v = wave_sum(v);
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (lane == 0)
+        lds[wave] = v;
+    __syncthreads();
+    T total = T(0);
+#pragma unroll
+    for (int w = 0; w < BLOCK / kWave; ++w)
+        total += lds[w];
+    __syncthreads();
+    return total;
+}
+
+// Device-coherent loads/stores for values exchanged between workgroups of
+// one kernel (bypass the per-CU vector cache).
+template <typename T> __device__ __forceinline__ T load_agent(const T* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T> __device__ __forceinline__ void store_agent(T* p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Deterministic grid reduction of NV doubles per workgroup.
+//
+// Each workgroup writes its NV block sums to partials[v * gridDim.x + block],
+// then takes a ticket; the workgroup that takes the last ticket sums the
+// partials of every workgroup in a fixed order (so the result does not depend
+// on scheduling), returns true with the totals in `out` (valid in all its
+// threads), and re-arms the ticket for the next launch.
+template <int BLOCK, int NV>
+__device__ bool grid_reduce_last(
+    const double (&mine)[NV], double* partials, unsigned int* ticket, double (&out)[NV], double* lds /* >= BLOCK */
+)
+{
+    __shared__ bool is_last;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        double s = block_sum<BLOCK>(mine[v], lds);
+        if (threadIdx.x == 0)
+            store_agent(&partials[(size_t)v * gridDim.x + blockIdx.x], s);
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        unsigned int t = atomicAdd(ticket, 1u);
+        is_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!is_last)
+        return false;
+    __threadfence();
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        double acc = 0.0;
+        for (unsigned int b = threadIdx.x; b < gridDim.x; b += BLOCK)
+            acc += load_agent(&partials[(size_t)v * gridDim.x + b]);
+        out[v] = block_sum<BLOCK>(acc, lds);
+    }
+    if (threadIdx.x == 0)
+        store_agent(ticket, 0u);
+    return true;
+}
+
+}  // namespace wa

@@ -1,0 +1,744 @@
+// Conjugate gradient on gfx950.
+//
+// Replaces the reference CG iteration (warp/_src/optim/linear.py:893-1121)
+// and its JIT kernels: TiledDot (linear.py:309-612), _initialize_tolerance
+// (2051-2083), _diag_mv_kernel (2271-2287), _cg_kernel_1/2 (2332-2373) and
+// the host loop _run_solver_loop (2182-2243).
+//
+// Precision policy: vectors stay in the system scalar type S (fp32 or fp64);
+// every dot product is accumulated in FP64 with a deterministic two-level
+// reduction, and alpha/beta are formed in FP64.
+//
+// One iteration = three fused kernels on a device-resident state:
+//   spmv_pap : Ap = A p, pAp = p.Ap            -> alpha = rz / pAp  (0 once converged)
+//   update   : x += alpha p, r -= alpha Ap, z = M r, rr = r.r, rz = r.z -> beta, done
+//   direction: p = z + beta p
+// The convergence flag `done` (r.r <= atol^2) makes later kernels no-ops, the
+// device-side equivalent of the reference's frozen alpha/beta.
+#include "bsr_core.h"
+
+#include <cmath>
+#include <vector>
+
+namespace wa {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxGrid = 2048;
+
+struct CgState {
+    double bb;
+    double rr;
+    double rz;
+    double pAp;
+    double atol_sq;
+    double alpha;
+    double beta;
+    double rtol;
+    double atol;
+    int iter;
+    int done;
+    unsigned int ticket[4];
+};
+
+enum PrecKind { kPrecNone = 0, kPrecDiag = 1, kPrecBlock = 2 };
+
+struct CgWs {
+    CgState* state;
+    double* partials;  // 3 * kMaxGrid
+    void* r;
+    void* z;
+    void* p;
+    void* Ap;
+};
+
+CgWs carve_cg(void* base, size_t bytes, int64_t n, size_t scalar_bytes, size_t* needed)
+{
+    Carve c(base, bytes);
+    CgWs w;
+    w.state = c.take<CgState>(1);
+    w.partials = c.take<double>(3 * kMaxGrid);
+    w.r = c.take<char>((size_t)n * scalar_bytes);
+    w.z = c.take<char>((size_t)n * scalar_bytes);
+    w.p = c.take<char>((size_t)n * scalar_bytes);
+    w.Ap = c.take<char>((size_t)n * scalar_bytes);
+    *needed = c.used;
+    return w;
+}
+
+__device__ __forceinline__ int xcd_contiguous_block(int bid, int nblocks)
+{
+    constexpr int kXcd = 8;
+    const int q = nblocks / kXcd;
+    const int r = nblocks % kXcd;
+    const int xcd = bid % kXcd;
+    return xcd * q + (xcd < r ? xcd : r) + bid / kXcd;
+}
+
+// Rows [lo, hi) of the contiguous chunk owned by this workgroup.
+__device__ __forceinline__ void chunk_range(int nitems, int align, int& lo, int& hi)
+{
+    const int block = xcd_contiguous_block(blockIdx.x, gridDim.x);
+    int per = (nitems + gridDim.x - 1) / gridDim.x;
+    per = (per + align - 1) / align * align;
+    lo = block * per;
+    hi = min(nitems, lo + per);
+}
+
+// z = M r for one block row.
+template <typename S, int BR, int PREC>
+__device__ __forceinline__ void apply_prec(const S* __restrict__ prec, int row, const S (&r)[BR], S (&z)[BR])
+{
+    if (PREC == kPrecDiag) {
+#pragma unroll
+        for (int s = 0; s < BR; ++s)
+            z[s] = prec[(size_t)row * BR + s] * r[s];
+    } else if (PREC == kPrecBlock) {
+        const S* m = prec + (size_t)row * BR * BR;
+#pragma unroll
+        for (int s = 0; s < BR; ++s) {
+            S v = S(0);
+#pragma unroll
+            for (int c = 0; c < BR; ++c)
+                v += m[s * BR + c] * r[c];
+            z[s] = v;
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < BR; ++s)
+            z[s] = r[s];
+    }
+}
+
+// r = b - A x ; z = M r ; p = z ; (b.b, r.r, r.z).  `first` also forms the
+// absolute tolerance (linear.py:2051-2083) and resets the iteration count.
+template <typename S, int BR, int G, int PREC>
+__global__ __launch_bounds__(kBlock) void cg_init_kernel(
+    int nrow, const int* __restrict__ offsets, const int* __restrict__ columns, const S* __restrict__ values,
+    const S* __restrict__ prec, const S* __restrict__ b, const S* __restrict__ x, S* __restrict__ r, S* __restrict__ z,
+    S* __restrict__ p, CgState* state, double* partials, int first
+)
+{
+    __shared__ double lds[kBlock];
+    constexpr int kGroups = kBlock / G;
+    int lo, hi;
+    chunk_range(nrow, kGroups, lo, hi);
+    const int lane = threadIdx.x % G;
+    double acc3[3] = {0.0, 0.0, 0.0};
+    for (int row0 = lo; row0 < hi; row0 += kGroups) {
+        const int row = row0 + threadIdx.x / G;
+        if (row >= hi)
+            continue;
+        S ax[BR];
+        group_row_product<S, BR, BR, G>(row, lane, offsets, nullptr, columns, values, x, ax);
+        if (lane == 0) {
+            S rv[BR], zv[BR];
+#pragma unroll
+            for (int s = 0; s < BR; ++s) {
+                const S bv = b[(size_t)row * BR + s];
+                rv[s] = bv - ax[s];
+                acc3[0] += (double)bv * (double)bv;
+            }
+            apply_prec<S, BR, PREC>(prec, row, rv, zv);
+#pragma unroll
+            for (int s = 0; s < BR; ++s) {
+                const size_t i = (size_t)row * BR + s;
+                r[i] = rv[s];
+                if (PREC != kPrecNone)
+                    z[i] = zv[s];
+                p[i] = zv[s];
+                acc3[1] += (double)rv[s] * (double)rv[s];
+                acc3[2] += (double)rv[s] * (double)zv[s];
+            }
+        }
+    }
+    double tot[3];
+    if (grid_reduce_last<kBlock, 3>(acc3, partials, &state->ticket[0], tot, lds) && threadIdx.x == 0) {
+        if (first) {
+            state->bb = tot[0];
+            const double a = fmax(state->rtol * sqrt(tot[0]), state->atol);
+            state->atol_sq = a * a;
+            state->iter = 0;
+        }
+        state->rr = tot[1];
+        state->rz = PREC != kPrecNone ? tot[2] : tot[1];
+        state->done = !(tot[1] > state->atol_sq);
+        state->alpha = 0.0;
+        state->beta = 0.0;
+    }
+}
+
+template <typename S, int BR, int G>
+__global__ __launch_bounds__(kBlock) void cg_spmv_kernel(
+    int nrow, const int* __restrict__ offsets, const int* __restrict__ columns, const S* __restrict__ values,
+    const S* __restrict__ p, S* __restrict__ Ap, CgState* state, double* partials
+)
+{
+    __shared__ double lds[kBlock];
+    if (state->done)
+        return;
+    constexpr int kGroups = kBlock / G;
+    int lo, hi;
+    chunk_range(nrow, kGroups, lo, hi);
+    const int lane = threadIdx.x % G;
+    double acc[1] = {0.0};
+    for (int row0 = lo; row0 < hi; row0 += kGroups) {
+        const int row = row0 + threadIdx.x / G;
+        if (row >= hi)
+            continue;
+        S ap[BR];
+        group_row_product<S, BR, BR, G>(row, lane, offsets, nullptr, columns, values, p, ap);
+        if (lane < BR) {
+            const size_t i = (size_t)row * BR + lane;
+            const S v = pick(ap, lane);
+            Ap[i] = v;
+            acc[0] += (double)p[i] * (double)v;
+        }
+    }
+    double tot[1];
+    if (grid_reduce_last<kBlock, 1>(acc, partials, &state->ticket[1], tot, lds) && threadIdx.x == 0) {
+        state->pAp = tot[0];
+        state->alpha = state->rr > state->atol_sq ? state->rz / tot[0] : 0.0;
+    }
+}
+
+template <typename S, int BR, int PREC>
+__global__ __launch_bounds__(kBlock) void cg_update_kernel(
+    int nrow, const S* __restrict__ prec, S* __restrict__ x, S* __restrict__ r, S* __restrict__ z,
+    const S* __restrict__ p, const S* __restrict__ Ap, CgState* state, double* partials
+)
+{
+    __shared__ double lds[kBlock];
+    if (state->done)
+        return;
+    const S alpha = (S)state->alpha;
+    int lo, hi;
+    chunk_range(nrow, kBlock, lo, hi);
+    double acc[2] = {0.0, 0.0};
+    for (int row = lo + threadIdx.x; row < hi; row += kBlock) {
+        S rv[BR], zv[BR];
+#pragma unroll
+        for (int s = 0; s < BR; ++s) {
+            const size_t i = (size_t)row * BR + s;
+            x[i] = x[i] + alpha * p[i];
+            rv[s] = r[i] - alpha * Ap[i];
+            r[i] = rv[s];
+        }
+        apply_prec<S, BR, PREC>(prec, row, rv, zv);
+#pragma unroll
+        for (int s = 0; s < BR; ++s) {
+            if (PREC != kPrecNone)
+                z[(size_t)row * BR + s] = zv[s];
+            acc[0] += (double)rv[s] * (double)rv[s];
+            acc[1] += (double)rv[s] * (double)zv[s];
+        }
+    }
+    double tot[2];
+    if (grid_reduce_last<kBlock, 2>(acc, partials, &state->ticket[2], tot, lds) && threadIdx.x == 0) {
+        const double rz_old = state->rz;
+        const double rz_new = PREC != kPrecNone ? tot[1] : tot[0];
+        state->rr = tot[0];
+        state->rz = rz_new;
+        state->beta = tot[0] > state->atol_sq ? rz_new / rz_old : 0.0;
+        state->done = !(tot[0] > state->atol_sq);
+        state->iter += 1;
+    }
+}
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void cg_direction_kernel(
+    int64_t n, const S* __restrict__ z, S* __restrict__ p, const CgState* state
+)
+{
+    if (state->done)
+        return;
+    const S beta = (S)state->beta;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        p[i] = z[i] + beta * p[i];
+}
+
+__global__ void cg_state_init_kernel(CgState* state, double rtol, double atol)
+{
+    state->rtol = rtol;
+    state->atol = atol;
+    state->atol_sq = 0.0;
+    state->done = 0;
+    state->iter = 0;
+    for (int k = 0; k < 4; ++k)
+        state->ticket[k] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Host driver
+// ---------------------------------------------------------------------------
+
+// Tolerance defaults (linear.py:2010-2032); NaN stands for None.
+void get_tolerances(bool is_f64, double tol, double atol, double* rtol_out, double* atol_out)
+{
+    const double eps = is_f64 ? 1.0e-16 : 1.0e-8;
+    const double default_tol = std::pow(eps, 0.75);
+    const double min_tol = std::pow(eps, 2.25);
+    if (std::isnan(tol) && std::isnan(atol)) {
+        tol = atol = default_tol;
+    } else if (std::isnan(tol)) {
+        tol = atol;
+    } else if (std::isnan(atol)) {
+        atol = tol;
+    }
+    *rtol_out = tol;
+    *atol_out = std::max(atol, min_tol);
+}
+
+// Pinned staging for state read-backs; one per host thread, kept for the
+// life of the process (freeing it at thread exit could race runtime teardown).
+struct PinnedState {
+    CgState* host = nullptr;
+};
+
+int read_state(const CgState* dev, CgState* out, hipStream_t stream)
+{
+    static thread_local PinnedState pinned;
+    if (!pinned.host)
+        WA_HIP(hipHostMalloc((void**)&pinned.host, sizeof(CgState), hipHostMallocDefault));
+    WA_HIP(hipMemcpyAsync(pinned.host, dev, sizeof(CgState), hipMemcpyDeviceToHost, stream));
+    WA_HIP(hipStreamSynchronize(stream));
+    *out = *pinned.host;
+    return kOk;
+}
+
+int group_for_square(int64_t nnz_hint, int nrow, int br)
+{
+    const double mean = nrow > 0 && nnz_hint > 0 ? (double)nnz_hint / nrow : 8.0;
+    int g = 1;
+    while (g < mean && g < 64)
+        g <<= 1;
+    const int lo = br == 1 ? 1 : 4;
+    return g < lo ? lo : g;
+}
+
+typedef int (*CgCallback)(int iteration, double err, double atol, void* user);
+
+struct CgProblem {
+    int nrow;
+    int br;
+    const int* offsets;
+    const int* columns;
+    const void* values;
+    int64_t nnz_hint;
+    int prec_kind;
+    const void* prec;
+    const void* b;
+    void* x;
+};
+
+template <typename S> struct CgLauncher {
+    CgProblem pb;
+    CgWs ws;
+    hipStream_t stream;
+    int grid_rows;   // workgroups for row kernels
+    int grid_vec;    // workgroups for the vector kernel
+
+    template <int BR, int G, int PREC> int init_t(int first)
+    {
+        cg_init_kernel<S, BR, G, PREC><<<grid_rows, kBlock, 0, stream>>>(
+            pb.nrow, pb.offsets, pb.columns, (const S*)pb.values, (const S*)pb.prec, (const S*)pb.b,
+            (const S*)pb.x, (S*)ws.r, (S*)ws.z, (S*)ws.p, ws.state, ws.partials, first
+        );
+        WA_LAUNCH_CHECK();
+        return kOk;
+    }
+    template <int BR, int G> int spmv_t()
+    {
+        cg_spmv_kernel<S, BR, G><<<grid_rows, kBlock, 0, stream>>>(
+            pb.nrow, pb.offsets, pb.columns, (const S*)pb.values, (const S*)ws.p, (S*)ws.Ap, ws.state, ws.partials
+        );
+        WA_LAUNCH_CHECK();
+        return kOk;
+    }
+    template <int BR, int PREC> int update_t()
+    {
+        cg_update_kernel<S, BR, PREC><<<grid_vec, kBlock, 0, stream>>>(
+            pb.nrow, (const S*)pb.prec, (S*)pb.x, (S*)ws.r, (S*)ws.z, (const S*)ws.p, (const S*)ws.Ap, ws.state,
+            ws.partials
+        );
+        WA_LAUNCH_CHECK();
+        return kOk;
+    }
+    int direction()
+    {
+        const S* z = pb.prec_kind != kPrecNone ? (const S*)ws.z : (const S*)ws.r;
+        cg_direction_kernel<S><<<grid_vec, kBlock, 0, stream>>>((int64_t)pb.nrow * pb.br, z, (S*)ws.p, ws.state);
+        WA_LAUNCH_CHECK();
+        return kOk;
+    }
+
+    int g() const { return group_for_square(pb.nnz_hint, pb.nrow, pb.br); }
+
+#define WA_CG_DISPATCH_G(BR, CALL)                                                                                    \
+    switch (g()) {                                                                                                    \
+    case 1: if (BR == 1) return CALL(BR, 1); [[fallthrough]];                                                         \
+    case 2: if (BR == 1) return CALL(BR, 2); [[fallthrough]];                                                         \
+    case 4: return CALL(BR, 4);                                                                                       \
+    case 8: return CALL(BR, 8);                                                                                       \
+    case 16: return CALL(BR, 16);                                                                                     \
+    case 32: return CALL(BR, 32);                                                                                     \
+    default: return CALL(BR, 64);                                                                                     \
+    }
+
+    int init(int first)
+    {
+#define WA_INIT_CALL(BR, G) init_prec<BR, G>(first)
+        if (pb.br == 1) {
+            WA_CG_DISPATCH_G(1, WA_INIT_CALL)
+        }
+        WA_CG_DISPATCH_G(3, WA_INIT_CALL)
+#undef WA_INIT_CALL
+    }
+    template <int BR, int G> int init_prec(int first)
+    {
+        switch (pb.prec_kind) {
+        case kPrecDiag: return init_t<BR, G, kPrecDiag>(first);
+        case kPrecBlock: return init_t<BR, G, kPrecBlock>(first);
+        default: return init_t<BR, G, kPrecNone>(first);
+        }
+    }
+    int spmv()
+    {
+#define WA_SPMV_CALL(BR, G) spmv_t<BR, G>()
+        if (pb.br == 1) {
+            WA_CG_DISPATCH_G(1, WA_SPMV_CALL)
+        }
+        WA_CG_DISPATCH_G(3, WA_SPMV_CALL)
+#undef WA_SPMV_CALL
+    }
+    int update()
+    {
+        if (pb.br == 1) {
+            switch (pb.prec_kind) {
+            case kPrecDiag: return update_t<1, kPrecDiag>();
+            case kPrecBlock: return update_t<1, kPrecBlock>();
+            default: return update_t<1, kPrecNone>();
+            }
+        }
+        switch (pb.prec_kind) {
+        case kPrecDiag: return update_t<3, kPrecDiag>();
+        case kPrecBlock: return update_t<3, kPrecBlock>();
+        default: return update_t<3, kPrecNone>();
+        }
+    }
+#undef WA_CG_DISPATCH_G
+
+    int iteration(bool update_direction)
+    {
+        WA_TRY(spmv());
+        WA_TRY(update());
+        if (update_direction)
+            WA_TRY(direction());
+        return kOk;
+    }
+};
+
+template <typename S>
+int cg_solve_impl(
+    const CgProblem& pb, double tol, double atol, int maxiter, int check_every, int restart, CgCallback cb,
+    void* cb_user, void* workspace, size_t workspace_bytes, int* iters_out, double* rnorm_out, double* atol_out,
+    hipStream_t stream
+)
+{
+    WA_REQUIRE(pb.br == 1 || pb.br == 3, "cg_solve: block size must be 1 or 3, got %d", pb.br);
+    WA_REQUIRE(pb.prec_kind >= 0 && pb.prec_kind <= 2, "cg_solve: invalid preconditioner kind %d", pb.prec_kind);
+    WA_REQUIRE(pb.prec_kind == kPrecNone || pb.prec, "cg_solve: null preconditioner data");
+    const int64_t n = (int64_t)pb.nrow * pb.br;
+    size_t needed = 0;
+    CgWs ws = carve_cg(workspace, workspace_bytes, n, sizeof(S), &needed);
+    WA_REQUIRE(workspace && needed <= workspace_bytes, "cg_solve: workspace too small (%zu < %zu bytes)",
+               workspace_bytes, needed);
+    if (maxiter <= 0)
+        maxiter = (int)n;
+
+    CgLauncher<S> L{pb, ws, stream, 1, 1};
+    L.grid_rows = (int)std::max<int64_t>(1, std::min<int64_t>(kMaxGrid, ceil_div(pb.nrow, kBlock / L.g())));
+    L.grid_rows = std::min(L.grid_rows, device_cu_count() * 8);
+    L.grid_vec = (int)std::max<int64_t>(1, std::min<int64_t>(kMaxGrid, ceil_div(pb.nrow, kBlock)));
+    L.grid_vec = std::min(L.grid_vec, device_cu_count() * 4);
+
+    int cycle = 1;
+    if (restart > 0) {
+        cycle = std::max(1, std::min(restart, maxiter));
+        if (check_every > 0)
+            check_every = std::max(check_every, cycle);
+    }
+
+    double rtol_v, atol_v;
+    get_tolerances(sizeof(S) == 8, tol, atol, &rtol_v, &atol_v);
+    cg_state_init_kernel<<<1, 1, 0, stream>>>(ws.state, rtol_v, atol_v);
+    WA_LAUNCH_CHECK();
+    WA_TRY(L.init(1));
+
+    auto run_cycle = [&]() -> int {
+        for (int k = 0; k < cycle; ++k)
+            WA_TRY(L.iteration(!(restart > 0 && k == cycle - 1)));
+        if (restart > 0)
+            WA_TRY(L.init(0));
+        return kOk;
+    };
+
+    if (check_every <= 0) {
+        // Device-side convergence only: enqueue every cycle, no host read-back.
+        for (int it = 0; it < maxiter; it += cycle)
+            WA_TRY(run_cycle());
+        if (iters_out || rnorm_out || atol_out) {
+            CgState h;
+            WA_TRY(read_state(ws.state, &h, stream));
+            if (iters_out)
+                *iters_out = h.iter;
+            if (rnorm_out)
+                *rnorm_out = std::sqrt(h.rr);
+            if (atol_out)
+                *atol_out = std::sqrt(h.atol_sq);
+        }
+        return kOk;
+    }
+
+    CgState h;
+    WA_TRY(read_state(ws.state, &h, stream));
+    const double atol_out_v = std::sqrt(h.atol_sq);
+    int cur_iter = 0;
+    if (cb)
+        cb(cur_iter, std::sqrt(h.rr), atol_out_v, cb_user);
+    if (!(h.rr <= h.atol_sq)) {
+        for (;;) {
+            WA_TRY(run_cycle());
+            cur_iter += cycle;
+            if (cur_iter >= maxiter)
+                break;
+            if ((cur_iter % check_every) < cycle) {
+                WA_TRY(read_state(ws.state, &h, stream));
+                if (h.rr <= h.atol_sq)
+                    break;
+                if (cb)
+                    cb(cur_iter, std::sqrt(h.rr), atol_out_v, cb_user);
+            }
+        }
+        WA_TRY(read_state(ws.state, &h, stream));
+        if (cb)
+            cb(cur_iter, std::sqrt(h.rr), atol_out_v, cb_user);
+    }
+    if (iters_out)
+        *iters_out = cur_iter;
+    if (rnorm_out)
+        *rnorm_out = std::sqrt(h.rr);
+    if (atol_out)
+        *atol_out = atol_out_v;
+    return kOk;
+}
+
+// ---------------------------------------------------------------------------
+// Generic primitives (operators supplied by the caller)
+// ---------------------------------------------------------------------------
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void dot_kernel(
+    int64_t n, const S* __restrict__ a, const S* __restrict__ b, double* out, double* partials, unsigned int* ticket
+)
+{
+    __shared__ double lds[kBlock];
+    double acc[1] = {0.0};
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        acc[0] += (double)a[i] * (double)b[i];
+    double tot[1];
+    if (grid_reduce_last<kBlock, 1>(acc, partials, ticket, tot, lds) && threadIdx.x == 0)
+        *out = tot[0];
+}
+
+// x += alpha p ; r -= alpha Ap with alpha = rr > atol_sq ? rz / pAp : 0
+template <typename S>
+__global__ __launch_bounds__(kBlock) void step_xr_kernel(
+    int64_t n, S* __restrict__ x, S* __restrict__ r, const S* __restrict__ p, const S* __restrict__ Ap,
+    const double* rr, const double* atol_sq, const double* rz, const double* pAp
+)
+{
+    const double a = *rr > *atol_sq ? *rz / *pAp : 0.0;
+    const S alpha = (S)a;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        x[i] = x[i] + alpha * p[i];
+        r[i] = r[i] - alpha * Ap[i];
+    }
+}
+
+// p = z + beta p with beta = rr > atol_sq ? rz_new / rz_old : 0
+template <typename S>
+__global__ __launch_bounds__(kBlock) void step_p_kernel(
+    int64_t n, const S* __restrict__ z, S* __restrict__ p, const double* rr, const double* atol_sq,
+    const double* rz_new, const double* rz_old
+)
+{
+    const double bt = *rr > *atol_sq ? *rz_new / *rz_old : 0.0;
+    const S beta = (S)bt;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        p[i] = z[i] + beta * p[i];
+}
+
+// atol_sq = max(rtol * sqrt(bb), atol)^2 (linear.py:2051-2060)
+__global__ void tolerance_kernel(const double* bb, double rtol, double atol, double* atol_sq)
+{
+    const double a = fmax(rtol * sqrt(*bb), atol);
+    *atol_sq = a * a;
+}
+
+// z = alpha * (d o x) + beta * y (linear.py:2271-2287)
+template <typename S>
+__global__ __launch_bounds__(kBlock) void diag_mv_kernel(
+    int64_t n, const S* __restrict__ d, const S* __restrict__ x, const S* __restrict__ y, S* __restrict__ z, S alpha,
+    S beta
+)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        S s = S(0);
+        if (alpha != S(0))
+            s += alpha * (d[i] * x[i]);
+        if (beta != S(0))
+            s += beta * y[i];
+        z[i] = s;
+    }
+}
+
+// z = alpha * A x + beta * y for a dense row-major m x k matrix, one wave per
+// row (linear.py:2246-2268).
+template <typename S>
+__global__ __launch_bounds__(kBlock) void dense_mv_kernel(
+    int m, int k, const S* __restrict__ A, const S* __restrict__ x, const S* __restrict__ y, S* __restrict__ z,
+    S alpha, S beta
+)
+{
+    const int row = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (row >= m)
+        return;
+    S s = S(0);
+    if (alpha != S(0))
+        for (int c = lane; c < k; c += kWave)
+            s += A[(size_t)row * k + c] * x[c];
+    s = wave_sum(s) * alpha;
+    if (lane == 0) {
+        if (beta != S(0))
+            s += beta * y[row];
+        z[row] = s;
+    }
+}
+
+int vec_grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(kMaxGrid, ceil_div(n, kBlock))); }
+
+}  // namespace
+}  // namespace wa
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+WA_API size_t warp_amd_cg_workspace_size(int64_t n, int scalar_size)
+{
+    size_t needed = 0;
+    wa::carve_cg(nullptr, 0, n, (size_t)scalar_size, &needed);
+    return needed + 256;
+}
+
+// Byte offsets of the device-resident scalars inside the CG workspace, so
+// callers can read them without a host copy of the layout:
+// 0 iterations (int32), 1 r.r (double), 2 atol^2 (double).
+WA_API size_t warp_amd_cg_state_offset(int which)
+{
+    switch (which) {
+    case 0: return offsetof(wa::CgState, iter);
+    case 1: return offsetof(wa::CgState, rr);
+    case 2: return offsetof(wa::CgState, atol_sq);
+    default: return 0;
+    }
+}
+
+#define WA_DEFINE_CG(SUFFIX, S)                                                                                       \
+    WA_API int warp_amd_cg_solve_##SUFFIX(                                                                            \
+        int nrow, int block_size, const int* offsets, const int* columns, const S* values, int64_t nnz_hint,          \
+        int prec_kind, const S* prec, const S* b, S* x, double tol, double atol, int maxiter, int check_every,        \
+        int restart, wa::CgCallback callback, void* callback_user, void* workspace, size_t workspace_bytes,           \
+        int* iterations, double* residual_norm, double* abs_tolerance, void* stream                                   \
+    )                                                                                                                 \
+    {                                                                                                                 \
+        wa::CgProblem pb{nrow, block_size, offsets, columns, values, nnz_hint, prec_kind, prec, b, x};                \
+        return wa::cg_solve_impl<S>(                                                                                  \
+            pb, tol, atol, maxiter, check_every, restart, callback, callback_user, workspace, workspace_bytes,       \
+            iterations, residual_norm, abs_tolerance, wa::as_stream(stream)                                           \
+        );                                                                                                            \
+    }                                                                                                                 \
+    WA_API int warp_amd_dot_##SUFFIX(                                                                                 \
+        int64_t n, const S* a, const S* b, double* out, void* workspace, size_t workspace_bytes, void* stream         \
+    )                                                                                                                 \
+    {                                                                                                                 \
+        const int grid = wa::vec_grid(n);                                                                             \
+        WA_REQUIRE(workspace && workspace_bytes >= 256 + sizeof(double) * grid,                                     \
+                   "dot: workspace too small (need %zu bytes)", 256 + sizeof(double) * grid);                         \
+        unsigned int* ticket = (unsigned int*)workspace;                                                              \
+        double* partials = (double*)((char*)workspace + 256);                                                         \
+        wa::dot_kernel<S><<<grid, wa::kBlock, 0, wa::as_stream(stream)>>>(n, a, b, out, partials, ticket);           \
+        WA_LAUNCH_CHECK();                                                                                            \
+        return wa::kOk;                                                                                               \
+    }                                                                                                                 \
+    WA_API int warp_amd_cg_step_xr_##SUFFIX(                                                                          \
+        int64_t n, S* x, S* r, const S* p, const S* Ap, const double* rr, const double* atol_sq, const double* rz,    \
+        const double* pAp, void* stream                                                                               \
+    )                                                                                                                 \
+    {                                                                                                                 \
+        wa::step_xr_kernel<S><<<wa::vec_grid(n), wa::kBlock, 0, wa::as_stream(stream)>>>(                             \
+            n, x, r, p, Ap, rr, atol_sq, rz, pAp                                                                      \
+        );                                                                                                            \
+        WA_LAUNCH_CHECK();                                                                                            \
+        return wa::kOk;                                                                                               \
+    }                                                                                                                 \
+    WA_API int warp_amd_cg_step_p_##SUFFIX(                                                                           \
+        int64_t n, const S* z, S* p, const double* rr, const double* atol_sq, const double* rz_new,                   \
+        const double* rz_old, void* stream                                                                            \
+    )                                                                                                                 \
+    {                                                                                                                 \
+        wa::step_p_kernel<S><<<wa::vec_grid(n), wa::kBlock, 0, wa::as_stream(stream)>>>(                              \
+            n, z, p, rr, atol_sq, rz_new, rz_old                                                                      \
+        );                                                                                                            \
+        WA_LAUNCH_CHECK();                                                                                            \
+        return wa::kOk;                                                                                               \
+    }                                                                                                                 \
+    WA_API int warp_amd_diag_mv_##SUFFIX(                                                                             \
+        int64_t n, const S* d, const S* x, const S* y, S* z, S alpha, S beta, void* stream                            \
+    )                                                                                                                 \
+    {                                                                                                                 \
+        if (n <= 0)                                                                                                   \
+            return wa::kOk;                                                                                           \
+        wa::diag_mv_kernel<S><<<wa::vec_grid(n), wa::kBlock, 0, wa::as_stream(stream)>>>(n, d, x, y, z, alpha, beta); \
+        WA_LAUNCH_CHECK();                                                                                            \
+        return wa::kOk;                                                                                               \
+    }                                                                                                                 \
+    WA_API int warp_amd_dense_mv_##SUFFIX(                                                                            \
+        int m, int k, const S* A, const S* x, const S* y, S* z, S alpha, S beta, void* stream                         \
+    )                                                                                                                 \
+    {                                                                                                                 \
+        if (m <= 0)                                                                                                   \
+            return wa::kOk;                                                                                           \
+        const int rows_per_block = wa::kBlock / wa::kWave;                                                            \
+        wa::dense_mv_kernel<S><<<(unsigned)wa::ceil_div(m, rows_per_block), wa::kBlock, 0, wa::as_stream(stream)>>>(  \
+            m, k, A, x, y, z, alpha, beta                                                                             \
+        );                                                                                                            \
+        WA_LAUNCH_CHECK();                                                                                            \
+        return wa::kOk;                                                                                               \
+    }
+
+WA_DEFINE_CG(f32, float)
+WA_DEFINE_CG(f64, double)
+
+WA_API size_t warp_amd_dot_workspace_size(int64_t n) { return 256 + sizeof(double) * wa::vec_grid(n); }
+
+WA_API int warp_amd_cg_tolerance(const double* bb, double tol, double atol, int is_f64, double* atol_sq, void* stream)
+{
+    double rtol_v, atol_v;
+    wa::get_tolerances(is_f64 != 0, tol, atol, &rtol_v, &atol_v);
+    wa::tolerance_kernel<<<1, 1, 0, wa::as_stream(stream)>>>(bb, rtol_v, atol_v, atol_sq);
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}

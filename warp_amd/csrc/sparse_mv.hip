@@ -1,0 +1,309 @@
+// bsr_mv, bsr_get_diag and Jacobi extraction for gfx950.
+//
+// Replaces the reference's JIT kernels bsr_mv_kernel / bsr_mv_tiled_kernel
+// (warp/_src/sparse.py:4627-4715), _bsr_get_diag_kernel (sparse.py:2871-2884)
+// and _extract_inverse_diagonal_* (warp/_src/optim/linear.py:2290-2320).
+#include "bsr_core.h"
+
+namespace wa {
+namespace {
+
+constexpr int kBlock = 256;
+
+// Map the hardware workgroup id so that each of the 8 XCDs (which receive
+// workgroups round-robin) processes one contiguous range of block rows: rows
+// that share x entries then share one L2.
+__device__ __forceinline__ int xcd_contiguous_block(int bid, int nblocks)
+{
+    constexpr int kXcd = 8;
+    const int q = nblocks / kXcd;
+    const int r = nblocks % kXcd;
+    const int xcd = bid % kXcd;
+    const int local = bid / kXcd;
+    return xcd * q + (xcd < r ? xcd : r) + local;
+}
+
+template <typename S, int BR, int BC, int G>
+__global__ __launch_bounds__(kBlock) void bsr_mv_group_kernel(
+    int nrow, const int* __restrict__ offsets, const int* __restrict__ row_counts, const int* __restrict__ columns,
+    const S* __restrict__ values, const S* __restrict__ x, S* __restrict__ y, S alpha, S beta
+)
+{
+    constexpr int kRowsPerBlock = kBlock / G;
+    const int block = xcd_contiguous_block(blockIdx.x, gridDim.x);
+    const int row = block * kRowsPerBlock + threadIdx.x / G;
+    const int lane = threadIdx.x % G;
+    if (row >= nrow)
+        return;
+    S acc[BR];
+    if (alpha != S(0)) {
+        group_row_product<S, BR, BC, G>(row, lane, offsets, row_counts, columns, values, x, acc);
+    } else {
+#pragma unroll
+        for (int s = 0; s < BR; ++s)
+            acc[s] = S(0);
+    }
+    if (lane < BR) {
+        const size_t yi = (size_t)row * BR + lane;
+        S v = pick(acc, lane) * alpha;
+        if (beta != S(0))
+            v += beta * y[yi];
+        y[yi] = v;
+    }
+}
+
+// Any block shape: one thread per scalar row, blocks in storage order (the
+// reference row kernel's order).
+template <typename S>
+__global__ __launch_bounds__(kBlock) void bsr_mv_generic_kernel(
+    int nrow, int br, int bc, const int* __restrict__ offsets, const int* __restrict__ row_counts,
+    const int* __restrict__ columns, const S* __restrict__ values, const S* __restrict__ x, S* __restrict__ y,
+    S alpha, S beta
+)
+{
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= (int64_t)nrow * br)
+        return;
+    const int row = (int)(t / br);
+    const int s = (int)(t % br);
+    S v = S(0);
+    if (alpha != S(0)) {
+        const int end = row_end(offsets, row_counts, row);
+        for (int b = offsets[row]; b < end; ++b) {
+            const S* blk = values + (size_t)b * br * bc + (size_t)s * bc;
+            const S* xs = x + (size_t)columns[b] * bc;
+            for (int c = 0; c < bc; ++c)
+                v += blk[c] * xs[c];
+        }
+        v *= alpha;
+    }
+    if (beta != S(0))
+        v += beta * y[t];
+    y[t] = v;
+}
+
+template <typename S, int BR, int BC, int G>
+int launch_group(
+    int nrow, const int* offsets, const int* row_counts, const int* columns, const S* values, const S* x, S* y,
+    S alpha, S beta, hipStream_t stream
+)
+{
+    constexpr int kRowsPerBlock = kBlock / G;
+    const int64_t blocks = ceil_div(nrow, kRowsPerBlock);
+    bsr_mv_group_kernel<S, BR, BC, G>
+        <<<(unsigned)blocks, kBlock, 0, stream>>>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta);
+    WA_LAUNCH_CHECK();
+    return kOk;
+}
+
+// Lanes per block row: the power of two nearest above the mean row length,
+// within [lo, 64].
+int group_for(int64_t nnz_hint, int nrow, int lo)
+{
+    double mean = nrow > 0 && nnz_hint > 0 ? (double)nnz_hint / nrow : 8.0;
+    int g = 1;
+    while (g < mean && g < 64)
+        g <<= 1;
+    return g < lo ? lo : g;
+}
+
+template <typename S>
+int bsr_mv_impl(
+    int nrow, int ncol, int br, int bc, const int* offsets, const int* row_counts, const int* columns,
+    const S* values, int64_t nnz_hint, const S* x, S* y, S alpha, S beta, void* stream_ptr
+)
+{
+    WA_REQUIRE(nrow >= 0 && ncol >= 0 && br > 0 && bc > 0, "bsr_mv: invalid shape (%d, %d, %d, %d)", nrow, ncol, br, bc);
+    if (nrow == 0)
+        return kOk;
+    WA_REQUIRE(offsets && y, "bsr_mv: null offsets or y");
+    WA_REQUIRE(alpha == S(0) || (columns && values && x), "bsr_mv: null matrix or x with nonzero alpha");
+    hipStream_t stream = as_stream(stream_ptr);
+    if (br == 1 && bc == 1) {
+        switch (group_for(nnz_hint, nrow, 1)) {
+        case 1: return launch_group<S, 1, 1, 1>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+        case 2: return launch_group<S, 1, 1, 2>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+        case 4: return launch_group<S, 1, 1, 4>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+        case 8: return launch_group<S, 1, 1, 8>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+        case 16: return launch_group<S, 1, 1, 16>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+        case 32: return launch_group<S, 1, 1, 32>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+        default: return launch_group<S, 1, 1, 64>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+        }
+    }
+    if (br == 3 && bc == 3) {
+        switch (group_for(nnz_hint, nrow, 4)) {
+        case 4: return launch_group<S, 3, 3, 4>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+        case 8: return launch_group<S, 3, 3, 8>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+        case 16: return launch_group<S, 3, 3, 16>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+        case 32: return launch_group<S, 3, 3, 32>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+        default: return launch_group<S, 3, 3, 64>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+        }
+    }
+    const int64_t threads = (int64_t)nrow * br;
+    bsr_mv_generic_kernel<S><<<(unsigned)ceil_div(threads, kBlock), kBlock, 0, stream>>>(
+        nrow, br, bc, offsets, row_counts, columns, values, x, y, alpha, beta
+    );
+    WA_LAUNCH_CHECK();
+    return kOk;
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal blocks and Jacobi
+// ---------------------------------------------------------------------------
+
+// Index of the block (row, row) among the sorted active blocks of `row`, or -1
+// (reference bsr_block_index, warp/_src/sparse.py:1881-1905).
+__device__ __forceinline__ int diag_block_index(int row, const int* offsets, const int* row_counts, const int* columns)
+{
+    int lo = offsets[row];
+    const int end = row_end(offsets, row_counts, row);
+    int hi = end;
+    while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (columns[mid] < row)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return (lo < end && columns[lo] == row) ? lo : -1;
+}
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void get_diag_kernel(
+    int dim, int bsz, const int* offsets, const int* row_counts, const int* columns, const S* values, S scale, S* out
+)
+{
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= (int64_t)dim * bsz)
+        return;
+    const int row = (int)(t / bsz);
+    const int k = (int)(t % bsz);
+    const int b = diag_block_index(row, offsets, row_counts, columns);
+    out[t] = b >= 0 ? scale * values[(size_t)b * bsz + k] : S(0);
+}
+
+template <typename S> __device__ __forceinline__ S inverse_coefficient(S d, int use_abs)
+{
+    // 1/d (or 1/|d|), and 1 where d == 0 (linear.py:2283-2287)
+    const S den = use_abs ? (d < S(0) ? -d : d) : d;
+    return d == S(0) ? S(1) : S(1) / den;
+}
+
+// mode 0: inverse of the diagonal entries of each diagonal block (point
+// Jacobi, out[nrow*br]).  mode 1: inverse of each br x br diagonal block
+// (block Jacobi, out[nrow*br*br]); br == 3 uses the closed-form adjugate in
+// FP64, and a singular or absent block falls back to the point inverse.
+template <typename S>
+__global__ __launch_bounds__(kBlock) void jacobi_kernel(
+    int nrow, int dim, int br, const int* offsets, const int* row_counts, const int* columns, const S* values,
+    int use_abs, int mode, S* out
+)
+{
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    if (row >= nrow)
+        return;
+    const int bsz = br * br;
+    const int b = row < dim ? diag_block_index(row, offsets, row_counts, columns) : -1;
+    const S* blk = b >= 0 ? values + (size_t)b * bsz : nullptr;
+    if (mode == 0) {
+        for (int s = 0; s < br; ++s)
+            out[(size_t)row * br + s] = inverse_coefficient(blk ? blk[s * br + s] : S(0), use_abs);
+        return;
+    }
+    S* o = out + (size_t)row * bsz;
+    if (br == 3 && blk) {
+        const double a00 = blk[0], a01 = blk[1], a02 = blk[2];
+        const double a10 = blk[3], a11 = blk[4], a12 = blk[5];
+        const double a20 = blk[6], a21 = blk[7], a22 = blk[8];
+        const double c00 = a11 * a22 - a12 * a21, c01 = a02 * a21 - a01 * a22, c02 = a01 * a12 - a02 * a11;
+        const double c10 = a12 * a20 - a10 * a22, c11 = a00 * a22 - a02 * a20, c12 = a02 * a10 - a00 * a12;
+        const double c20 = a10 * a21 - a11 * a20, c21 = a01 * a20 - a00 * a21, c22 = a00 * a11 - a01 * a10;
+        const double det = a00 * c00 + a01 * c10 + a02 * c20;
+        if (det != 0.0 && isfinite(det)) {
+            const double inv = 1.0 / det;
+            o[0] = S(c00 * inv); o[1] = S(c01 * inv); o[2] = S(c02 * inv);
+            o[3] = S(c10 * inv); o[4] = S(c11 * inv); o[5] = S(c12 * inv);
+            o[6] = S(c20 * inv); o[7] = S(c21 * inv); o[8] = S(c22 * inv);
+            return;
+        }
+    }
+    if (br == 1 && blk) {
+        o[0] = inverse_coefficient(blk[0], use_abs);
+        return;
+    }
+    for (int s = 0; s < br; ++s)
+        for (int c = 0; c < br; ++c)
+            o[s * br + c] = s == c ? inverse_coefficient(blk ? blk[s * br + s] : S(0), use_abs) : S(0);
+}
+
+template <typename S>
+int get_diag_impl(
+    int nrow, int ncol, int br, int bc, const int* offsets, const int* row_counts, const int* columns,
+    const S* values, S scale, S* out, void* stream
+)
+{
+    WA_REQUIRE(nrow >= 0 && ncol >= 0 && br > 0 && bc > 0, "bsr_get_diag: invalid shape");
+    const int dim = nrow < ncol ? nrow : ncol;
+    if (dim == 0)
+        return kOk;
+    const int64_t threads = (int64_t)dim * br * bc;
+    get_diag_kernel<S><<<(unsigned)ceil_div(threads, kBlock), kBlock, 0, as_stream(stream)>>>(
+        dim, br * bc, offsets, row_counts, columns, values, scale, out
+    );
+    WA_LAUNCH_CHECK();
+    return kOk;
+}
+
+template <typename S>
+int jacobi_impl(
+    int nrow, int ncol, int br, int bc, const int* offsets, const int* row_counts, const int* columns,
+    const S* values, int use_abs, int mode, S* out, void* stream
+)
+{
+    WA_REQUIRE(br == bc && br > 0, "bsr_jacobi: blocks must be square, got %dx%d", br, bc);
+    WA_REQUIRE(mode == 0 || mode == 1, "bsr_jacobi: mode must be 0 (point) or 1 (block)");
+    if (nrow == 0)
+        return kOk;
+    const int dim = nrow < ncol ? nrow : ncol;
+    jacobi_kernel<S><<<(unsigned)ceil_div(nrow, kBlock), kBlock, 0, as_stream(stream)>>>(
+        nrow, dim, br, offsets, row_counts, columns, values, use_abs, mode, out
+    );
+    WA_LAUNCH_CHECK();
+    return kOk;
+}
+
+}  // namespace
+}  // namespace wa
+
+#define WA_DEFINE_SPARSE_MV(SUFFIX, S)                                                                                \
+    WA_API int warp_amd_bsr_mv_##SUFFIX(                                                                              \
+        int nrow, int ncol, int block_rows, int block_cols, const int* offsets, const int* row_counts,                \
+        const int* columns, const S* values, int64_t nnz_hint, const S* x, S* y, S alpha, S beta, void* stream        \
+    )                                                                                                                 \
+    {                                                                                                                 \
+        return wa::bsr_mv_impl<S>(                                                                                    \
+            nrow, ncol, block_rows, block_cols, offsets, row_counts, columns, values, nnz_hint, x, y, alpha, beta,    \
+            stream                                                                                                    \
+        );                                                                                                            \
+    }                                                                                                                 \
+    WA_API int warp_amd_bsr_get_diag_##SUFFIX(                                                                        \
+        int nrow, int ncol, int block_rows, int block_cols, const int* offsets, const int* row_counts,                \
+        const int* columns, const S* values, S scale, S* out, void* stream                                            \
+    )                                                                                                                 \
+    {                                                                                                                 \
+        return wa::get_diag_impl<S>(                                                                                  \
+            nrow, ncol, block_rows, block_cols, offsets, row_counts, columns, values, scale, out, stream              \
+        );                                                                                                            \
+    }                                                                                                                 \
+    WA_API int warp_amd_bsr_jacobi_##SUFFIX(                                                                          \
+        int nrow, int ncol, int block_rows, int block_cols, const int* offsets, const int* row_counts,                \
+        const int* columns, const S* values, int use_abs, int mode, S* out, void* stream                              \
+    )                                                                                                                 \
+    {                                                                                                                 \
+        return wa::jacobi_impl<S>(                                                                                    \
+            nrow, ncol, block_rows, block_cols, offsets, row_counts, columns, values, use_abs, mode, out, stream      \
+        );                                                                                                            \
+    }
+
+WA_DEFINE_SPARSE_MV(f32, float)
+WA_DEFINE_SPARSE_MV(f64, double)

@@ -1,0 +1,547 @@
+// COO triplets -> compact BSR on gfx950.
+//
+// Replaces wp_bsr_matrix_from_triplets_device (warp/native/sparse.cu:1168-1322,
+// declared at warp/native/warp.h:563-602) and the value accumulation kernel
+// _bsr_accumulate_triplet_values (warp/_src/sparse.py:1073-1090, 1122-1141).
+//
+// Algorithm (an MSD radix sort whose first digit is the whole row index):
+//   1. classify: drop out-of-range / masked-out / all-zero triplets, count
+//      triplets per row (atomic histogram);
+//   2. exclusive scan of the histogram -> bucket start of every row;
+//   3. scatter key = (col << 32) | triplet_index into the row's bucket;
+//   4. sort every bucket by key (LDS bitonic sort, one wave per row of <= 256
+//      triplets, one workgroup per larger row, with global rank merges beyond
+//      4096): this yields exactly the order of a stable sort by (row, col),
+//      whatever order the scatter produced;
+//   5. per-row unique count -> exclusive scan -> bsr offsets;
+//   6. per row: columns and inclusive run ends (summed_block_offsets);
+//   7. per unique block: values = v[i0] + v[i1] + ... sequentially in
+//      ascending triplet order (bit-identical to the reference CUDA order).
+#include "common.h"
+
+#include <climits>
+
+namespace wa {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kSmallCap = 256;    // rows sorted by one wave
+constexpr int kLargeCap = 4096;   // rows sorted in one workgroup's LDS
+constexpr uint64_t kPadKey = ~uint64_t(0);
+
+struct TripletWs {
+    int* trow;
+    int* hist;
+    int* row_start;
+    uint64_t* keys;
+    uint64_t* alt;
+    int* large_rows;
+    int* counters;
+    void* scan_ws;
+    int* summed_offsets;
+    int* summed_indices;
+};
+
+TripletWs carve_ws(void* base, size_t bytes, int nrow, int T, bool own_summed, size_t* needed)
+{
+    Carve c(base, bytes);
+    TripletWs w;
+    w.trow = c.take<int>((size_t)T);
+    w.hist = c.take<int>((size_t)nrow + 1);
+    w.row_start = c.take<int>((size_t)nrow + 1);
+    w.keys = c.take<uint64_t>((size_t)T);
+    w.alt = c.take<uint64_t>((size_t)T);
+    w.large_rows = c.take<int>((size_t)nrow + 1);
+    w.counters = c.take<int>(64);
+    w.scan_ws = c.take<char>(scan_workspace_bytes((int64_t)nrow + 1));
+    w.summed_offsets = own_summed ? c.take<int>((size_t)T) : nullptr;
+    w.summed_indices = own_summed ? c.take<int>((size_t)T) : nullptr;
+    *needed = c.used;
+    return w;
+}
+
+__global__ void fill_i32_kernel(int* p, int64_t n, int v)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+// Scalar pattern test of one block: true when every scalar has no bit in `mask`
+// (sparse.cu:245-268; sparse.cpp:22-29).
+template <typename U> __device__ __forceinline__ bool block_is_zero(const U* v, int block_size, U mask)
+{
+    for (int k = 0; k < block_size; ++k)
+        if ((v[k] & mask) != U(0))
+            return false;
+    return true;
+}
+
+// Existing-topology test for masked assembly (sparse.cu:277-310).
+__device__ __forceinline__ bool in_mask(int row, int col, const int* offsets, const int* row_counts, const int* columns)
+{
+    int lo = offsets[row];
+    int hi = row_counts ? lo + row_counts[row] : offsets[row + 1];
+    const int end = hi;
+    while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (columns[mid] < col)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo < end && columns[lo] == col;
+}
+
+template <typename U>
+__global__ __launch_bounds__(kBlock) void classify_kernel(
+    int T, const int* tpl_count, int nrow, int ncol, const int* rows, const int* cols, const U* values,
+    int block_size, U zero_mask, const int* mask_offsets, const int* mask_row_counts, const int* mask_columns,
+    int* trow, int* hist
+)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= T)
+        return;
+    const int limit = tpl_count ? *tpl_count : T;
+    const int r = rows[i];
+    const int c = cols[i];
+    bool valid = i < limit && r >= 0 && r < nrow && c >= 0 && c < ncol;
+    if (valid && mask_offsets)
+        valid = in_mask(r, c, mask_offsets, mask_row_counts, mask_columns);
+    if (valid && values)
+        valid = !block_is_zero(values + (size_t)i * block_size, block_size, zero_mask);
+    trow[i] = valid ? r : -1;
+    if (valid)
+        atomicAdd(&hist[r], 1);
+}
+
+__global__ __launch_bounds__(kBlock) void scatter_kernel(
+    int T, const int* trow, const int* cols, const int* row_start, int* hist, uint64_t* keys
+)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= T)
+        return;
+    const int r = trow[i];
+    if (r < 0)
+        return;
+    const int pos = row_start[r] + atomicSub(&hist[r], 1) - 1;
+    keys[pos] = ((uint64_t)(uint32_t)cols[i] << 32) | (uint32_t)i;
+}
+
+// In-LDS bitonic sort of `n` (power of two) keys by `nthreads` threads.
+__device__ __forceinline__ void lds_bitonic(uint64_t* s, int n, int tid, int nthreads)
+{
+    for (int k = 2; k <= n; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = tid; t < n / 2; t += nthreads) {
+                const int lo = 2 * t - (t & (j - 1));
+                const int hi = lo + j;
+                const bool up = (lo & k) == 0;
+                const uint64_t a = s[lo];
+                const uint64_t b = s[hi];
+                if ((a > b) == up) {
+                    s[lo] = b;
+                    s[hi] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__device__ __forceinline__ int pow2_at_least(int n, int lo)
+{
+    int p = lo;
+    while (p < n)
+        p <<= 1;
+    return p;
+}
+
+// Writes the sorted bucket back, the sorted triplet indices, and the unique
+// column count of the row.
+__device__ void emit_sorted_row(
+    const uint64_t* s, int size, int base, int row, int tid, int nthreads, uint64_t* keys, int* summed_indices,
+    int* urow, int* lds_count
+)
+{
+    int heads = 0;
+    for (int k = tid; k < size; k += nthreads) {
+        const uint64_t key = s[k];
+        keys[base + k] = key;
+        summed_indices[base + k] = (int)(uint32_t)key;
+        heads += (k == 0 || (s[k - 1] >> 32) != (key >> 32)) ? 1 : 0;
+    }
+    atomicAdd(lds_count, heads);
+    __syncthreads();
+    if (tid == 0)
+        urow[row] = *lds_count;
+}
+
+__global__ __launch_bounds__(kWave) void sort_small_rows_kernel(
+    int nrow, const int* row_start, uint64_t* keys, int* summed_indices, int* urow, int* large_rows, int* counters
+)
+{
+    __shared__ uint64_t s[kSmallCap];
+    __shared__ int count;
+    const int row = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int base = row_start[row];
+    const int size = row_start[row + 1] - base;
+    if (size > kSmallCap) {
+        if (tid == 0) {
+            const int slot = atomicAdd(&counters[0], 1);
+            large_rows[slot] = row;
+        }
+        return;
+    }
+    if (size == 0) {
+        if (tid == 0)
+            urow[row] = 0;
+        return;
+    }
+    const int n = pow2_at_least(size, 2);
+    for (int k = tid; k < n; k += kWave)
+        s[k] = k < size ? keys[base + k] : kPadKey;
+    if (tid == 0)
+        count = 0;
+    __syncthreads();
+    lds_bitonic(s, n, tid, kWave);
+    emit_sorted_row(s, size, base, row, tid, kWave, keys, summed_indices, urow, &count);
+}
+
+// Rows above kSmallCap: one workgroup each (grid-stride over the row list).
+__global__ __launch_bounds__(kBlock) void sort_large_rows_kernel(
+    const int* row_start, uint64_t* keys, uint64_t* alt, int* summed_indices, int* urow, const int* large_rows,
+    const int* counters
+)
+{
+    __shared__ uint64_t s[kLargeCap];
+    __shared__ int count;
+    const int nlarge = counters[0];
+    const int tid = threadIdx.x;
+    for (int li = blockIdx.x; li < nlarge; li += gridDim.x) {
+        const int row = large_rows[li];
+        const int base = row_start[row];
+        const int size = row_start[row + 1] - base;
+        if (tid == 0)
+            count = 0;
+        if (size <= kLargeCap) {
+            const int n = pow2_at_least(size, 2);
+            for (int k = tid; k < n; k += kBlock)
+                s[k] = k < size ? keys[base + k] : kPadKey;
+            __syncthreads();
+            lds_bitonic(s, n, tid, kBlock);
+            emit_sorted_row(s, size, base, row, tid, kBlock, keys, summed_indices, urow, &count);
+            __syncthreads();
+            continue;
+        }
+        // sort chunks of kLargeCap in LDS
+        for (int c0 = 0; c0 < size; c0 += kLargeCap) {
+            const int csize = size - c0 < kLargeCap ? size - c0 : kLargeCap;
+            const int n = pow2_at_least(csize, 2);
+            for (int k = tid; k < n; k += kBlock)
+                s[k] = k < csize ? keys[base + c0 + k] : kPadKey;
+            __syncthreads();
+            lds_bitonic(s, n, tid, kBlock);
+            for (int k = tid; k < csize; k += kBlock)
+                store_agent(&keys[base + c0 + k], s[k]);
+            __threadfence();
+            __syncthreads();
+        }
+        // merge sorted runs by rank (keys are unique: the triplet index is in the low bits)
+        uint64_t* src = keys + base;
+        uint64_t* dst = alt + base;
+        for (int w = kLargeCap; w < size; w <<= 1) {
+            for (int p = tid; p < size; p += kBlock) {
+                const int run0 = (p / (2 * w)) * (2 * w);
+                const int a0 = run0, a1 = min(run0 + w, size);
+                const int b0 = a1, b1 = min(run0 + 2 * w, size);
+                const uint64_t key = load_agent(&src[p]);
+                const bool in_a = p < a1;
+                int lo = in_a ? b0 : a0;
+                int hi = in_a ? b1 : a1;
+                const int other0 = lo;
+                while (lo < hi) {
+                    const int mid = lo + (hi - lo) / 2;
+                    if (load_agent(&src[mid]) < key)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                const int rank_other = lo - other0;
+                const int own = in_a ? p - a0 : p - b0;
+                store_agent(&dst[run0 + own + rank_other], key);
+            }
+            __threadfence();
+            __syncthreads();
+            uint64_t* t = src;
+            src = dst;
+            dst = t;
+        }
+        int heads = 0;
+        for (int k = tid; k < size; k += kBlock) {
+            const uint64_t key = load_agent(&src[k]);
+            if (src != keys + base)
+                keys[base + k] = key;
+            summed_indices[base + k] = (int)(uint32_t)key;
+            heads += (k == 0 || (load_agent(&src[k - 1]) >> 32) != (key >> 32)) ? 1 : 0;
+        }
+        atomicAdd(&count, heads);
+        __syncthreads();
+        if (tid == 0)
+            urow[row] = count;
+        __threadfence();
+        __syncthreads();
+    }
+}
+
+// One wave per row: columns and inclusive run ends of the row's unique blocks.
+__global__ __launch_bounds__(kWave) void finalize_rows_kernel(
+    const int* row_start, const uint64_t* keys, const int* bsr_offsets, int* bsr_columns, int* summed_offsets
+)
+{
+    const int row = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int rs = row_start[row];
+    const int re = row_start[row + 1];
+    const int out0 = bsr_offsets[row];
+    int runs = 0;
+    for (int p0 = rs; p0 < re; p0 += kWave) {
+        const int p = p0 + lane;
+        const bool active = p < re;
+        int col = -1, prev = -1, next = -1;
+        if (active) {
+            col = (int)(keys[p] >> 32);
+            prev = p > rs ? (int)(keys[p - 1] >> 32) : -1;
+            next = p + 1 < re ? (int)(keys[p + 1] >> 32) : -1;
+        }
+        const bool head = active && col != prev;
+        const bool tail = active && col != next;
+        const uint64_t mask = __ballot(head);
+        const uint64_t below = lane == 63 ? ~uint64_t(0) : ((uint64_t(1) << (lane + 1)) - 1);
+        const int run = runs + __popcll(mask & below) - 1;
+        if (head)
+            bsr_columns[out0 + run] = col;
+        if (tail)
+            summed_offsets[out0 + run] = p + 1;
+        runs += __popcll(mask);
+    }
+}
+
+// Sequential run sums per (unique block, scalar); one wave per row.
+template <typename S>
+__global__ __launch_bounds__(kWave) void accumulate_values_kernel(
+    const int* bsr_offsets, const int* summed_offsets, const int* summed_indices, const S* tpl_values, int block_size,
+    S* bsr_values
+)
+{
+    const int row = blockIdx.x;
+    const int u0 = bsr_offsets[row];
+    const int u1 = bsr_offsets[row + 1];
+    const int64_t tasks = (int64_t)(u1 - u0) * block_size;
+    for (int64_t t = threadIdx.x; t < tasks; t += kWave) {
+        const int u = u0 + (int)(t / block_size);
+        const int s = (int)(t % block_size);
+        const int beg = u == 0 ? 0 : summed_offsets[u - 1];
+        const int end = summed_offsets[u];
+        S acc = tpl_values[(size_t)summed_indices[beg] * block_size + s];
+        for (int k = beg + 1; k < end; ++k)
+            acc += tpl_values[(size_t)summed_indices[k] * block_size + s];
+        bsr_values[(size_t)u * block_size + s] = acc;
+    }
+}
+
+int launch_fill(int* p, int64_t n, int v, hipStream_t stream)
+{
+    if (n <= 0)
+        return kOk;
+    fill_i32_kernel<<<(unsigned)ceil_div(n, kBlock), kBlock, 0, stream>>>(p, n, v);
+    WA_LAUNCH_CHECK();
+    return kOk;
+}
+
+}  // namespace
+
+size_t triplets_workspace_bytes(int nrow, int T)
+{
+    size_t needed = 0;
+    carve_ws(nullptr, 0, nrow, T, true, &needed);
+    return needed + 256;
+}
+
+int bsr_from_triplets(
+    int block_size, int scalar_size, int nrow, int ncol, int T, const int* tpl_count, const int* rows, const int* cols,
+    const void* tpl_values, uint64_t zero_mask, int masked, int* summed_offsets, int* summed_indices, int* bsr_offsets,
+    const int* bsr_row_counts, int* bsr_columns, void* bsr_values, int* bsr_nnz_host, void* workspace,
+    size_t workspace_bytes, hipStream_t stream
+)
+{
+    WA_REQUIRE(nrow >= 0 && ncol >= 0 && T >= 0 && block_size > 0, "bsr_from_triplets: invalid sizes");
+    WA_REQUIRE(bsr_offsets != nullptr, "bsr_from_triplets: null offsets");
+    WA_REQUIRE(T == 0 || (rows && cols && bsr_columns), "bsr_from_triplets: null triplet or column arrays");
+    const bool own_summed = !(summed_offsets && summed_indices);
+    size_t needed = 0;
+    TripletWs w = carve_ws(workspace, workspace_bytes, nrow, T, own_summed, &needed);
+    WA_REQUIRE(workspace && needed <= workspace_bytes,
+               "bsr_from_triplets: workspace too small (%zu < %zu bytes)", workspace_bytes, needed);
+    if (own_summed) {
+        summed_offsets = w.summed_offsets;
+        summed_indices = w.summed_indices;
+    }
+    const bool prune = tpl_values && zero_mask != 0;
+    const bool sum_values = tpl_values && bsr_values;
+    if (sum_values)
+        WA_REQUIRE(scalar_size == 4 || scalar_size == 8, "bsr_from_triplets: value accumulation needs fp32 or fp64");
+    if (prune)
+        WA_REQUIRE(scalar_size == 1 || scalar_size == 2 || scalar_size == 4 || scalar_size == 8,
+                   "bsr_from_triplets: unsupported scalar size %d", scalar_size);
+    const int* mo = masked ? bsr_offsets : nullptr;
+    const int* mrc = masked ? bsr_row_counts : nullptr;
+    const int* mc = masked ? bsr_columns : nullptr;
+
+    WA_TRY(launch_fill(w.hist, (int64_t)nrow + 1, 0, stream));
+    WA_TRY(launch_fill(w.counters, 64, 0, stream));
+    if (T > 0) {
+        const unsigned grid = (unsigned)ceil_div(T, kBlock);
+        switch (prune ? scalar_size : 0) {
+        case 1:
+            classify_kernel<uint8_t><<<grid, kBlock, 0, stream>>>(
+                T, tpl_count, nrow, ncol, rows, cols, (const uint8_t*)tpl_values, block_size, (uint8_t)zero_mask, mo,
+                mrc, mc, w.trow, w.hist
+            );
+            break;
+        case 2:
+            classify_kernel<uint16_t><<<grid, kBlock, 0, stream>>>(
+                T, tpl_count, nrow, ncol, rows, cols, (const uint16_t*)tpl_values, block_size, (uint16_t)zero_mask, mo,
+                mrc, mc, w.trow, w.hist
+            );
+            break;
+        case 4:
+            classify_kernel<uint32_t><<<grid, kBlock, 0, stream>>>(
+                T, tpl_count, nrow, ncol, rows, cols, (const uint32_t*)tpl_values, block_size, (uint32_t)zero_mask, mo,
+                mrc, mc, w.trow, w.hist
+            );
+            break;
+        case 8:
+            classify_kernel<uint64_t><<<grid, kBlock, 0, stream>>>(
+                T, tpl_count, nrow, ncol, rows, cols, (const uint64_t*)tpl_values, block_size, (uint64_t)zero_mask, mo,
+                mrc, mc, w.trow, w.hist
+            );
+            break;
+        default:
+            classify_kernel<uint32_t><<<grid, kBlock, 0, stream>>>(
+                T, tpl_count, nrow, ncol, rows, cols, nullptr, block_size, 0u, mo, mrc, mc, w.trow, w.hist
+            );
+            break;
+        }
+        WA_LAUNCH_CHECK();
+    }
+    WA_TRY(exclusive_scan_i32(w.hist, w.row_start, nrow, w.scan_ws, stream));
+    if (T > 0) {
+        scatter_kernel<<<(unsigned)ceil_div(T, kBlock), kBlock, 0, stream>>>(
+            T, w.trow, cols, w.row_start, w.hist, w.keys
+        );
+        WA_LAUNCH_CHECK();
+    }
+    if (nrow > 0) {
+        // unique counts land in bsr_offsets[0..nrow), scanned in place below
+        sort_small_rows_kernel<<<(unsigned)nrow, kWave, 0, stream>>>(
+            nrow, w.row_start, w.keys, summed_indices, bsr_offsets, w.large_rows, w.counters
+        );
+        WA_LAUNCH_CHECK();
+        const int large_grid = device_cu_count() * 2;
+        sort_large_rows_kernel<<<(unsigned)large_grid, kBlock, 0, stream>>>(
+            w.row_start, w.keys, w.alt, summed_indices, bsr_offsets, w.large_rows, w.counters
+        );
+        WA_LAUNCH_CHECK();
+    }
+    WA_TRY(exclusive_scan_i32(bsr_offsets, bsr_offsets, nrow, w.scan_ws, stream));
+    if (bsr_nnz_host)
+        WA_HIP(hipMemcpyAsync(bsr_nnz_host, bsr_offsets + nrow, sizeof(int), hipMemcpyDeviceToHost, stream));
+    if (nrow > 0) {
+        finalize_rows_kernel<<<(unsigned)nrow, kWave, 0, stream>>>(
+            w.row_start, w.keys, bsr_offsets, bsr_columns, summed_offsets
+        );
+        WA_LAUNCH_CHECK();
+        if (sum_values) {
+            if (scalar_size == 4)
+                accumulate_values_kernel<float><<<(unsigned)nrow, kWave, 0, stream>>>(
+                    bsr_offsets, summed_offsets, summed_indices, (const float*)tpl_values, block_size,
+                    (float*)bsr_values
+                );
+            else
+                accumulate_values_kernel<double><<<(unsigned)nrow, kWave, 0, stream>>>(
+                    bsr_offsets, summed_offsets, summed_indices, (const double*)tpl_values, block_size,
+                    (double*)bsr_values
+                );
+            WA_LAUNCH_CHECK();
+        }
+    }
+    return kOk;
+}
+
+}  // namespace wa
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+WA_API size_t warp_amd_bsr_from_triplets_workspace_size(int row_count, int nnz)
+{
+    return wa::triplets_workspace_bytes(row_count, nnz);
+}
+
+WA_API int warp_amd_bsr_from_triplets(
+    int block_size, int scalar_size, int row_count, int col_count, int nnz, const int* tpl_nnz, const int* tpl_rows,
+    const int* tpl_columns, const void* tpl_values, uint64_t scalar_zero_mask, int masked_topology,
+    int* summed_block_offsets, int* summed_block_indices, int* bsr_offsets, const int* bsr_row_counts,
+    int* bsr_columns, void* bsr_values, int* bsr_nnz_host, void* workspace, size_t workspace_bytes, void* stream
+)
+{
+    return wa::bsr_from_triplets(
+        block_size, scalar_size, row_count, col_count, nnz, tpl_nnz, tpl_rows, tpl_columns, tpl_values,
+        scalar_zero_mask, masked_topology, summed_block_offsets, summed_block_indices, bsr_offsets, bsr_row_counts,
+        bsr_columns, bsr_values, bsr_nnz_host, workspace, workspace_bytes, wa::as_stream(stream)
+    );
+}
+
+namespace {
+thread_local void* g_current_stream = nullptr;
+}
+
+WA_API void warp_amd_set_current_stream(void* stream) { g_current_stream = stream; }
+
+WA_API void* warp_amd_get_current_stream(void) { return g_current_stream; }
+
+// Same name and arguments as the reference native entry point, so a caller of
+// the reference library can bind this one unchanged.  Uses the stream set by
+// warp_amd_set_current_stream (the reference uses its current CUDA stream),
+// allocates its scratch with the stream-ordered allocator, and records
+// `bsr_nnz_event` (a hipEvent_t) after the nnz copy when given.  Errors are
+// reported through warp_amd_last_error() (the reference function returns void).
+WA_API void wp_bsr_matrix_from_triplets_device(
+    int block_size, int scalar_size, int row_count, int col_count, int nnz, const int* tpl_nnz, const int* tpl_rows,
+    const int* tpl_columns, const void* tpl_values, uint64_t scalar_zero_mask, bool masked_topology,
+    int* tpl_block_offsets, int* tpl_block_indices, int* bsr_offsets, const int* bsr_row_counts, int* bsr_columns,
+    int* bsr_nnz, void* bsr_nnz_event
+)
+{
+    hipStream_t stream = wa::as_stream(g_current_stream);
+    const size_t bytes = wa::triplets_workspace_bytes(row_count, nnz);
+    void* ws = nullptr;
+    hipError_t err = hipMallocAsync(&ws, bytes, stream);
+    if (err != hipSuccess) {
+        wa::set_error("wp_bsr_matrix_from_triplets_device: hipMallocAsync failed: %s", hipGetErrorString(err));
+        return;
+    }
+    wa::bsr_from_triplets(
+        block_size, scalar_size, row_count, col_count, nnz, tpl_nnz, tpl_rows, tpl_columns, tpl_values,
+        scalar_zero_mask, masked_topology ? 1 : 0, tpl_block_offsets, tpl_block_indices, bsr_offsets, bsr_row_counts,
+        bsr_columns, nullptr, bsr_nnz, ws, bytes, stream
+    );
+    if (bsr_nnz && bsr_nnz_event)
+        (void)hipEventRecord(static_cast<hipEvent_t>(bsr_nnz_event), stream);
+    (void)hipFreeAsync(ws, stream);
+}

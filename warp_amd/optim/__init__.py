@@ -1,0 +1,3 @@
+"""Iterative solvers (``warp.optim`` drop-in for the CG path)."""
+
+from . import linear
