@@ -125,6 +125,12 @@ int warp_amd_bsr_mv_f64(
     const int* columns, const double* values, int64_t nnz_hint, const double* x, double* y, double alpha,
     double beta, void* stream);
 
+/* Lanes per block row G the vector-BSR kernels (bsr_mv and the fused CG
+ * kernels) use for this shape and nnz_hint; 1 for the generic kernel.  Each
+ * lane sums blocks g, g+G, ... with fused multiply-adds, then the group
+ * combines by an xor butterfly: callers can reproduce the bits. */
+int warp_amd_bsr_mv_lanes(int nrow, int block_rows, int block_cols, int64_t nnz_hint);
+
 /* out[r] = scale * A[r, r] (zero block where absent), r < min(nrow, ncol).
  * Replaces _bsr_get_diag_kernel (warp/_src/sparse.py:2871-2924). */
 int warp_amd_bsr_get_diag_f32(

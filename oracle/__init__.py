@@ -48,6 +48,9 @@ def lib() -> ctypes.CDLL:
             fn = getattr(L, f"orc_bsr_mv_{s}")
             fn.restype = None
             fn.argtypes = [_I, _I, _I, _P, _P, _P, _P, _P, _P, S, S]
+            fn = getattr(L, f"orc_bsr_mv_grouped_{s}")
+            fn.restype = None
+            fn.argtypes = [_I, _I, _I, _P, _P, _P, _P, _P, _P, S, S, _I]
             fn = getattr(L, f"orc_bsr_get_diag_{s}")
             fn.restype = None
             fn.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _P]
@@ -57,7 +60,7 @@ def lib() -> ctypes.CDLL:
         for s in ("f32", "f64", "f32d"):
             fn = getattr(L, f"orc_cg_{s}")
             fn.restype = None
-            fn.argtypes = [_I, _I, _P, _P, _P, _P, _P, _D, _D, _I, _I, _P, _I, _I, _P, _P, _I]
+            fn.argtypes = [_I, _I, _P, _P, _P, _P, _P, _D, _D, _I, _I, _P, _I, _I, _I, _P, _P, _I]
         L.orc_residual_f32_f64.restype = None
         L.orc_residual_f32_f64.argtypes = [_I, _I, _P, _P, _P, _P, _P, _P]
         _lib = L
@@ -151,19 +154,20 @@ def ref_bsr_from_triplets(nrow, ncol, rows, cols, values=None, prune=True, count
     return dict(offsets=off, columns=colsout[:n], summed_offsets=so[:n], summed_indices=si, nnz=n)
 
 
-def bsr_mv(offsets, columns, values, x, y=None, alpha=1.0, beta=0.0, row_counts=None):
-    """y = alpha A x + beta y, reference row-kernel order (sparse.py:4627-4665)."""
+def bsr_mv(offsets, columns, values, x, y=None, alpha=1.0, beta=0.0, row_counts=None, group=0):
+    """y = alpha A x + beta y.  group 0: reference row-kernel order
+    (sparse.py:4627-4665); group G > 0: the GPU lane-group order."""
     values = np.ascontiguousarray(values)
     dt = values.dtype
     br, bc = (values.shape[1], values.shape[2]) if values.ndim == 3 else (1, 1)
     nrow = offsets.shape[0] - 1
     x = np.ascontiguousarray(x, dtype=dt).reshape(-1)
     yo = np.zeros(nrow * br, dt) if y is None else np.ascontiguousarray(y, dtype=dt).reshape(-1).copy()
-    fn = lib().orc_bsr_mv_f64 if dt == np.float64 else lib().orc_bsr_mv_f32
+    fn = lib().orc_bsr_mv_grouped_f64 if dt == np.float64 else lib().orc_bsr_mv_grouped_f32
     offsets = np.ascontiguousarray(offsets, np.int32)
     columns = np.ascontiguousarray(columns, np.int32)
     rc = None if row_counts is None else np.ascontiguousarray(row_counts, np.int32)
-    fn(nrow, br, bc, _p(offsets), _p(rc), _p(columns), _p(values), _p(x), _p(yo), alpha, beta)
+    fn(nrow, br, bc, _p(offsets), _p(rc), _p(columns), _p(values), _p(x), _p(yo), alpha, beta, int(group))
     return yo
 
 
@@ -198,11 +202,12 @@ PREC_NONE, PREC_DIAG, PREC_BLOCK = 0, 1, 2
 
 
 def cg(offsets, columns, values, b, x0, tol=None, atol=None, maxiter=0, prec_kind=PREC_NONE, prec=None,
-       check_every=10, restart=0, dots="fp64", history_cap=4096):
+       check_every=10, restart=0, dots="fp64", history_cap=4096, group=0):
     """Reference-semantics CG (see orc_cg in warp_oracle.c).
 
     dots: "fp64" (FP64 dots and alpha/beta, fp32 vectors) or "native" (all in
-    the system scalar type, the reference's precision).
+    the system scalar type, the reference's precision).  group: SpMV order
+    (0 reference row order, G > 0 the GPU lane-group order).
     Returns (x, iterations, residual_norm, atol, history).
     """
     values = np.ascontiguousarray(values)
@@ -222,7 +227,8 @@ def cg(offsets, columns, values, b, x0, tol=None, atol=None, maxiter=0, prec_kin
     fn(
         nrow, br, _p(np.ascontiguousarray(offsets, np.int32)), _p(np.ascontiguousarray(columns, np.int32)),
         _p(values), _p(b), _p(x), nan if tol is None else float(tol), nan if atol is None else float(atol),
-        int(maxiter), int(prec_kind), _p(pr), int(check_every), int(restart), _p(out), _p(hist), history_cap,
+        int(maxiter), int(prec_kind), _p(pr), int(check_every), int(restart), int(group), _p(out), _p(hist),
+        history_cap,
     )
     return x, int(out[0]), float(out[1]), float(out[2]), hist[: int(out[3])]
 

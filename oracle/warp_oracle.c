@@ -209,6 +209,56 @@ ORC_API int orc_bsr_from_triplets(
 ORC_DEFINE_BSR_MV(f32, float)
 ORC_DEFINE_BSR_MV(f64, double)
 
+/* Same product in the lane-group order of the GPU vector-BSR kernels
+ * (warp_amd/csrc/bsr_core.h): for G lanes per block row, lane g sums blocks
+ * g, g+G, ... (block columns in order, fused multiply-adds), then lanes are
+ * combined by an xor butterfly (off = G/2 .. 1); the result is scaled by alpha
+ * and beta*y is added with one fused multiply-add.  G == 0 falls back to the
+ * reference row order above.  Reproduces the GPU bits exactly; it does not
+ * restate the reference (whose FP32 order is unpinnable). */
+#define ORC_DEFINE_BSR_MV_GROUPED(SUFFIX, S, FMA)                                                                     \
+    ORC_API void orc_bsr_mv_grouped_##SUFFIX(                                                                         \
+        int nrow, int br, int bc, const int* offsets, const int* row_counts, const int* columns, const S* values,     \
+        const S* x, S* y, S alpha, S beta, int G                                                                      \
+    )                                                                                                                 \
+    {                                                                                                                 \
+        if (G <= 0) {                                                                                                 \
+            orc_bsr_mv_##SUFFIX(nrow, br, bc, offsets, row_counts, columns, values, x, y, alpha, beta);              \
+            return;                                                                                                   \
+        }                                                                                                             \
+        S a[64], t[64];                                                                                               \
+        for (int row = 0; row < nrow; ++row) {                                                                        \
+            int beg = offsets[row];                                                                                   \
+            int end = row_counts ? beg + row_counts[row] : offsets[row + 1];                                          \
+            for (int s = 0; s < br; ++s) {                                                                            \
+                for (int g = 0; g < G; ++g) {                                                                         \
+                    a[g] = (S)0;                                                                                      \
+                    if (alpha == (S)0)                                                                                \
+                        continue;                                                                                     \
+                    for (int b = beg + g; b < end; b += G) {                                                          \
+                        const S* blk = values + (size_t)b * br * bc + (size_t)s * bc;                                 \
+                        const S* xs = x + (size_t)columns[b] * bc;                                                    \
+                        for (int c = 0; c < bc; ++c)                                                                  \
+                            a[g] = FMA(blk[c], xs[c], a[g]);                                                          \
+                    }                                                                                                 \
+                }                                                                                                     \
+                for (int off = G / 2; off > 0; off >>= 1) {                                                           \
+                    for (int g = 0; g < G; ++g)                                                                       \
+                        t[g] = a[g] + a[g ^ off];                                                                     \
+                    for (int g = 0; g < G; ++g)                                                                       \
+                        a[g] = t[g];                                                                                  \
+                }                                                                                                     \
+                S v = a[0] * alpha;                                                                                   \
+                if (beta != (S)0)                                                                                     \
+                    v = FMA(beta, y[(size_t)row * br + s], v);                                                        \
+                y[(size_t)row * br + s] = v;                                                                          \
+            }                                                                                                         \
+        }                                                                                                             \
+    }
+
+ORC_DEFINE_BSR_MV_GROUPED(f32, float, fmaf)
+ORC_DEFINE_BSR_MV_GROUPED(f64, double, fma)
+
 /* ------------------------------------------------------------------------- */
 /* Diagonal and Jacobi                                                        */
 /* ------------------------------------------------------------------------- */
@@ -303,7 +353,9 @@ static void orc_get_tolerances(int is_f64, double tol, double atol, double* rtol
  * (the tile structure of linear.py:309-612; order inside a tile is
  * sequential here).
  *
- * Operator: BSR (br == bc) as in orc_bsr_mv.  Preconditioner: none,
+ * Operator: BSR (br == bc); `group` 0 sums rows in the reference order
+ * (orc_bsr_mv), G > 0 in the GPU lane-group order (orc_bsr_mv_grouped);
+ * vector updates are single fused multiply-adds.  Preconditioner: none,
  * diagonal (`prec` holds n scalars), or block (`prec` holds nrow br x br
  * blocks, row-major, applied as z = P r per block row).
  *
@@ -311,7 +363,7 @@ static void orc_get_tolerances(int is_f64, double tol, double atol, double* rtol
  * `history` (nullable, capacity `history_cap`) receives ||r|| at every
  * check, as the reference callback would see it; out[3] = entries written.
  */
-#define ORC_DEFINE_CG(SUFFIX, S, D, IS_F64)                                                                           \
+#define ORC_DEFINE_CG(SUFFIX, S, D, IS_F64, FMA)                                                                           \
     static D orc_dot_##SUFFIX(const S* a, const S* b, int n)                                                          \
     {                                                                                                                 \
         D total = (D)0;                                                                                               \
@@ -337,7 +389,7 @@ static void orc_get_tolerances(int is_f64, double tol, double atol, double* rtol
                 for (int s = 0; s < br; ++s) {                                                                        \
                     S v = (S)0;                                                                                       \
                     for (int c = 0; c < br; ++c)                                                                      \
-                        v += prec[(size_t)b * br * br + (size_t)s * br + c] * r[(size_t)b * br + c];                  \
+                        v = FMA(prec[(size_t)b * br * br + (size_t)s * br + c], r[(size_t)b * br + c], v);            \
                     z[(size_t)b * br + s] = v;                                                                        \
                 }                                                                                                     \
         } else {                                                                                                      \
@@ -346,7 +398,7 @@ static void orc_get_tolerances(int is_f64, double tol, double atol, double* rtol
     }                                                                                                                 \
     ORC_API void orc_cg_##SUFFIX(                                                                                     \
         int nrow, int br, const int* offsets, const int* columns, const S* values, const S* b, S* x, double tol,      \
-        double atol, int maxiter, int prec_kind, const S* prec, int check_every, int restart, double* out,            \
+        double atol, int maxiter, int prec_kind, const S* prec, int check_every, int restart, int group, double* out, \
         double* history, int history_cap                                                                              \
     )                                                                                                                 \
     {                                                                                                                 \
@@ -375,7 +427,7 @@ static void orc_get_tolerances(int is_f64, double tol, double atol, double* rtol
         D atol_sq = a_tol * a_tol;                                                                                    \
         /* r = b - A x  (LinearOperator.matvec(x, b, r, -1, 1), linear.py:159-164) */                                  \
         memcpy(r, b, sizeof(S) * (size_t)n);                                                                          \
-        orc_bsr_mv_##SUFFIX##_op(nrow, br, offsets, columns, values, x, r, (S)-1, (S)1);                              \
+        orc_bsr_mv_##SUFFIX##_op(nrow, br, offsets, columns, values, x, r, (S)-1, (S)1, group);                              \
         D rr, rz;                                                                                                     \
         orc_prec_apply_##SUFFIX(prec_kind, nrow, br, prec, r, z);                                                     \
         rr = orc_dot_##SUFFIX(r, r, n);                                                                               \
@@ -389,12 +441,12 @@ static void orc_get_tolerances(int is_f64, double tol, double atol, double* rtol
                 for (int k = 0; k < cycle; ++k) {                                                                     \
                     int update_direction = !(restart > 0 && k == cycle - 1);                                          \
                     D rz_old = rz;                                                                                    \
-                    orc_bsr_mv_##SUFFIX##_op(nrow, br, offsets, columns, values, p, Ap, (S)1, (S)0);                  \
+                    orc_bsr_mv_##SUFFIX##_op(nrow, br, offsets, columns, values, p, Ap, (S)1, (S)0, group);                  \
                     D pAp = orc_dot_##SUFFIX(p, Ap, n);                                                               \
                     D alpha = rr > atol_sq ? rz_old / pAp : (D)0;                                                     \
                     for (int i = 0; i < n; ++i) {                                                                     \
-                        x[i] = x[i] + (S)alpha * p[i];                                                                \
-                        r[i] = r[i] - (S)alpha * Ap[i];                                                               \
+                        x[i] = FMA((S)alpha, p[i], x[i]);                                                             \
+                        r[i] = FMA(-(S)alpha, Ap[i], r[i]);                                                           \
                     }                                                                                                 \
                     if (update_direction) {                                                                           \
                         orc_prec_apply_##SUFFIX(prec_kind, nrow, br, prec, r, z);                                     \
@@ -403,12 +455,12 @@ static void orc_get_tolerances(int is_f64, double tol, double atol, double* rtol
                         D beta = rr > atol_sq ? rz / rz_old : (D)0;                                                   \
                         const S* zz = prec_kind ? z : r;                                                              \
                         for (int i = 0; i < n; ++i)                                                                   \
-                            p[i] = zz[i] + (S)beta * p[i];                                                            \
+                            p[i] = FMA((S)beta, p[i], zz[i]);                                                         \
                     }                                                                                                 \
                 }                                                                                                     \
                 if (restart > 0) {                                                                                    \
                     memcpy(r, b, sizeof(S) * (size_t)n);                                                              \
-                    orc_bsr_mv_##SUFFIX##_op(nrow, br, offsets, columns, values, x, r, (S)-1, (S)1);                  \
+                    orc_bsr_mv_##SUFFIX##_op(nrow, br, offsets, columns, values, x, r, (S)-1, (S)1, group);                  \
                     orc_prec_apply_##SUFFIX(prec_kind, nrow, br, prec, r, z);                                         \
                     rr = orc_dot_##SUFFIX(r, r, n);                                                                   \
                     rz = prec_kind ? orc_dot_##SUFFIX(r, z, n) : rr;                                                  \
@@ -441,26 +493,26 @@ static void orc_get_tolerances(int is_f64, double tol, double atol, double* rtol
         free(Ap);                                                                                                     \
     }
 
-/* Square-block operator adaptor for the CG loop. */
+/* Square-block operator adaptor for the CG loop (group 0 = reference order). */
 static void orc_bsr_mv_f32_op(
     int nrow, int br, const int* offsets, const int* columns, const float* values, const float* x, float* y,
-    float alpha, float beta
+    float alpha, float beta, int group
 )
 {
-    orc_bsr_mv_f32(nrow, br, br, offsets, NULL, columns, values, x, y, alpha, beta);
+    orc_bsr_mv_grouped_f32(nrow, br, br, offsets, NULL, columns, values, x, y, alpha, beta, group);
 }
 static void orc_bsr_mv_f64_op(
     int nrow, int br, const int* offsets, const int* columns, const double* values, const double* x, double* y,
-    double alpha, double beta
+    double alpha, double beta, int group
 )
 {
-    orc_bsr_mv_f64(nrow, br, br, offsets, NULL, columns, values, x, y, alpha, beta);
+    orc_bsr_mv_grouped_f64(nrow, br, br, offsets, NULL, columns, values, x, y, alpha, beta, group);
 }
 #define orc_bsr_mv_f32d_op orc_bsr_mv_f32_op
 
-ORC_DEFINE_CG(f32, float, float, 0)
-ORC_DEFINE_CG(f64, double, double, 1)
-ORC_DEFINE_CG(f32d, float, double, 0)
+ORC_DEFINE_CG(f32, float, float, 0, fmaf)
+ORC_DEFINE_CG(f64, double, double, 1, fma)
+ORC_DEFINE_CG(f32d, float, double, 0, fmaf)
 
 /* ------------------------------------------------------------------------- */
 /* FP64 residual for mixed precision                                          */

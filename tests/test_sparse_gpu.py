@@ -183,6 +183,13 @@ def test_triplets_elasticity_pattern(device, oracle_mod):
     assert A.nnz == (3 * 8 + 1) ** 3
 
 
+def _lanes(A):
+    from warp_amd import _native
+
+    br, bc = A.block_shape
+    return _native.lib().warp_amd_bsr_mv_lanes(A.nrow, br, bc, int(A.nnz))
+
+
 def _dense_of(A):
     nnz = A.nnz_sync()
     br, bc = A.block_shape
@@ -255,6 +262,10 @@ def test_bsr_mv_poisson_vs_oracle(device, oracle_mod, n, dtype):
         got = yg.cpu().numpy()
         scale = np.abs(alpha) * oracle_mod.bsr_mv(o["offsets"], o["columns"], np.abs(o["values"]).reshape(-1, 1, 1), np.abs(x)) + np.abs(beta * y0)
         assert np.all(np.abs(got - ref) <= 1e-5 * scale + 1e-30)
+        # bit-exact against the oracle run in the kernel's lane-group order
+        G = _lanes(A)
+        ref_g = oracle_mod.bsr_mv(o["offsets"], o["columns"], o["values"].reshape(-1, 1, 1), x, y0, alpha, beta, group=G)
+        assert got.tobytes() == ref_g.tobytes(), f"G={G}"
 
 
 @pytest.mark.parametrize("n", [6, 16])
@@ -268,6 +279,8 @@ def test_bsr_mv_elasticity_vs_oracle(device, oracle_mod, n):
     scale = oracle_mod.bsr_mv(o["offsets"], o["columns"], np.abs(o["values"]), np.abs(x))
     got = sparse.bsr_mv(A, _t(x, device)).cpu().numpy().reshape(-1)
     assert np.all(np.abs(got - ref) <= 1e-5 * scale)
+    ref_g = oracle_mod.bsr_mv(o["offsets"], o["columns"], o["values"], x, group=_lanes(A))
+    assert got.tobytes() == ref_g.tobytes()
     # FP64 error of the FP32 product, stated
     ref64 = oracle_mod.bsr_mv(o["offsets"], o["columns"], o["values"].astype(np.float64), x.astype(np.float64))
     rel = np.linalg.norm(got - ref64) / np.linalg.norm(ref64)

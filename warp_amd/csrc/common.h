@@ -93,6 +93,9 @@ struct Carve {
 // Number of compute units of the current device (cached per device id).
 int device_cu_count();
 
+// Lanes per block row used by the vector-BSR kernels (sparse_mv.hip).
+int bsr_mv_lanes(int nrow, int br, int bc, int64_t nnz_hint);
+
 // Exclusive scan of n int32 values: out[i] = sum(in[0..i)), out[n] = total.
 // `in` and `out` may alias.  Workspace from scan_workspace_bytes(n).
 size_t scan_workspace_bytes(int64_t n);
@@ -147,44 +150,29 @@ template <typename T> __device__ __forceinline__ void store_agent(T* p, T v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Deterministic grid reduction of NV doubles per workgroup.
-//
-// Each workgroup writes its NV block sums to partials[v * gridDim.x + block],
-// then takes a ticket; the workgroup that takes the last ticket sums the
-// partials of every workgroup in a fixed order (so the result does not depend
-// on scheduling), returns true with the totals in `out` (valid in all its
-// threads), and re-arms the ticket for the next launch.
+// Block partial sums for a following finalize kernel: thread 0 of workgroup b
+// writes the NV workgroup sums to partials[v * gridDim.x + b] with plain
+// stores (the kernel boundary makes them visible; no per-workgroup fences,
+// which on a multi-XCD GPU flush the L2 and serialise the grid).
 template <int BLOCK, int NV>
-__device__ bool grid_reduce_last(
-    const double (&mine)[NV], double* partials, unsigned int* ticket, double (&out)[NV], double* lds /* >= BLOCK */
-)
+__device__ __forceinline__ void write_block_partials(const double (&mine)[NV], double* partials, double* lds)
 {
-    __shared__ bool is_last;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-        double s = block_sum<BLOCK>(mine[v], lds);
+        const double s = block_sum<BLOCK>(mine[v], lds);
         if (threadIdx.x == 0)
-            store_agent(&partials[(size_t)v * gridDim.x + blockIdx.x], s);
+            partials[(size_t)v * gridDim.x + blockIdx.x] = s;
     }
-    if (threadIdx.x == 0) {
-        __threadfence();
-        unsigned int t = atomicAdd(ticket, 1u);
-        is_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!is_last)
-        return false;
-    __threadfence();
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        double acc = 0.0;
-        for (unsigned int b = threadIdx.x; b < gridDim.x; b += BLOCK)
-            acc += load_agent(&partials[(size_t)v * gridDim.x + b]);
-        out[v] = block_sum<BLOCK>(acc, lds);
-    }
-    if (threadIdx.x == 0)
-        store_agent(ticket, 0u);
-    return true;
+}
+
+// Fixed-order sum of `count` partials by one workgroup (result in all threads).
+template <int BLOCK>
+__device__ __forceinline__ double sum_partials(const double* partials, int count, double* lds)
+{
+    double a = 0.0;
+    for (int k = threadIdx.x; k < count; k += BLOCK)
+        a += partials[k];
+    return block_sum<BLOCK>(a, lds);
 }
 
 }  // namespace wa

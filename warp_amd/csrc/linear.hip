@@ -9,7 +9,9 @@
 // every dot product is accumulated in FP64 with a deterministic two-level
 // reduction, and alpha/beta are formed in FP64.
 //
-// One iteration = three fused kernels on a device-resident state:
+// One iteration = three fused kernels on a device-resident state, the first
+// two followed by a one-workgroup finalize kernel that sums their per-
+// workgroup partials in a fixed order:
 //   spmv_pap : Ap = A p, pAp = p.Ap            -> alpha = rz / pAp  (0 once converged)
 //   update   : x += alpha p, r -= alpha Ap, z = M r, rr = r.r, rz = r.z -> beta, done
 //   direction: p = z + beta p
@@ -38,26 +40,30 @@ struct CgState {
     double atol;
     int iter;
     int done;
-    unsigned int ticket[4];
 };
 
 enum PrecKind { kPrecNone = 0, kPrecDiag = 1, kPrecBlock = 2 };
 
 struct CgWs {
     CgState* state;
-    double* partials;  // 3 * kMaxGrid
+    double* partials;  // 3 * max_blocks
+    int max_blocks;
     void* r;
     void* z;
     void* p;
     void* Ap;
 };
 
-CgWs carve_cg(void* base, size_t bytes, int64_t n, size_t scalar_bytes, size_t* needed)
+// Largest grid of any CG kernel: one workgroup per 256/G block rows, G <= 64.
+int cg_max_blocks(int64_t nrow) { return (int)std::max<int64_t>(1, ceil_div(nrow, 4)); }
+
+CgWs carve_cg(void* base, size_t bytes, int64_t nrow, int64_t n, size_t scalar_bytes, size_t* needed)
 {
     Carve c(base, bytes);
     CgWs w;
+    w.max_blocks = cg_max_blocks(nrow);
     w.state = c.take<CgState>(1);
-    w.partials = c.take<double>(3 * kMaxGrid);
+    w.partials = c.take<double>((size_t)3 * w.max_blocks);
     w.r = c.take<char>((size_t)n * scalar_bytes);
     w.z = c.take<char>((size_t)n * scalar_bytes);
     w.p = c.take<char>((size_t)n * scalar_bytes);
@@ -66,24 +72,7 @@ CgWs carve_cg(void* base, size_t bytes, int64_t n, size_t scalar_bytes, size_t* 
     return w;
 }
 
-__device__ __forceinline__ int xcd_contiguous_block(int bid, int nblocks)
-{
-    constexpr int kXcd = 8;
-    const int q = nblocks / kXcd;
-    const int r = nblocks % kXcd;
-    const int xcd = bid % kXcd;
-    return xcd * q + (xcd < r ? xcd : r) + bid / kXcd;
-}
-
-// Rows [lo, hi) of the contiguous chunk owned by this workgroup.
-__device__ __forceinline__ void chunk_range(int nitems, int align, int& lo, int& hi)
-{
-    const int block = xcd_contiguous_block(blockIdx.x, gridDim.x);
-    int per = (nitems + gridDim.x - 1) / gridDim.x;
-    per = (per + align - 1) / align * align;
-    lo = block * per;
-    hi = min(nitems, lo + per);
-}
+constexpr int kFinalizeBlock = 1024;
 
 // z = M r for one block row.
 template <typename S, int BR, int PREC>
@@ -100,7 +89,7 @@ __device__ __forceinline__ void apply_prec(const S* __restrict__ prec, int row, 
             S v = S(0);
 #pragma unroll
             for (int c = 0; c < BR; ++c)
-                v += m[s * BR + c] * r[c];
+                v = fma(m[s * BR + c], r[c], v);
             z[s] = v;
         }
     } else {
@@ -112,31 +101,26 @@ __device__ __forceinline__ void apply_prec(const S* __restrict__ prec, int row, 
 
 // r = b - A x ; z = M r ; p = z ; (b.b, r.r, r.z).  `first` also forms the
 // absolute tolerance (linear.py:2051-2083) and resets the iteration count.
+// One block row per group of G lanes; the grid covers the matrix.
 template <typename S, int BR, int G, int PREC>
 __global__ __launch_bounds__(kBlock) void cg_init_kernel(
     int nrow, const int* __restrict__ offsets, const int* __restrict__ columns, const S* __restrict__ values,
     const S* __restrict__ prec, const S* __restrict__ b, const S* __restrict__ x, S* __restrict__ r, S* __restrict__ z,
-    S* __restrict__ p, CgState* state, double* partials, int first
+    S* __restrict__ p, CgWs ws, int first
 )
 {
-    __shared__ double lds[kBlock];
-    constexpr int kGroups = kBlock / G;
-    int lo, hi;
-    chunk_range(nrow, kGroups, lo, hi);
+    __shared__ double lds[kBlock / kWave];
+    const int row = blockIdx.x * (kBlock / G) + threadIdx.x / G;
     const int lane = threadIdx.x % G;
     double acc3[3] = {0.0, 0.0, 0.0};
-    for (int row0 = lo; row0 < hi; row0 += kGroups) {
-        const int row = row0 + threadIdx.x / G;
-        if (row >= hi)
-            continue;
-        S ax[BR];
-        group_row_product<S, BR, BR, G>(row, lane, offsets, nullptr, columns, values, x, ax);
+    if (row < nrow) {
+        const RowAcc<S, BR> ax = group_row_product<S, BR, BR, G>(row, lane, offsets, nullptr, columns, values, x);
         if (lane == 0) {
             S rv[BR], zv[BR];
 #pragma unroll
             for (int s = 0; s < BR; ++s) {
                 const S bv = b[(size_t)row * BR + s];
-                rv[s] = bv - ax[s];
+                rv[s] = bv - ax.v[s];
                 acc3[0] += (double)bv * (double)bv;
             }
             apply_prec<S, BR, PREC>(prec, row, rv, zv);
@@ -152,76 +136,53 @@ __global__ __launch_bounds__(kBlock) void cg_init_kernel(
             }
         }
     }
-    double tot[3];
-    if (grid_reduce_last<kBlock, 3>(acc3, partials, &state->ticket[0], tot, lds) && threadIdx.x == 0) {
-        if (first) {
-            state->bb = tot[0];
-            const double a = fmax(state->rtol * sqrt(tot[0]), state->atol);
-            state->atol_sq = a * a;
-            state->iter = 0;
-        }
-        state->rr = tot[1];
-        state->rz = PREC != kPrecNone ? tot[2] : tot[1];
-        state->done = !(tot[1] > state->atol_sq);
-        state->alpha = 0.0;
-        state->beta = 0.0;
-    }
+    write_block_partials<kBlock, 3>(acc3, ws.partials, lds);
 }
 
 template <typename S, int BR, int G>
 __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(
     int nrow, const int* __restrict__ offsets, const int* __restrict__ columns, const S* __restrict__ values,
-    const S* __restrict__ p, S* __restrict__ Ap, CgState* state, double* partials
+    const S* __restrict__ p, S* __restrict__ Ap, CgWs ws
 )
 {
-    __shared__ double lds[kBlock];
-    if (state->done)
+    __shared__ double lds[kBlock / kWave];
+    if (ws.state->done)
         return;
-    constexpr int kGroups = kBlock / G;
-    int lo, hi;
-    chunk_range(nrow, kGroups, lo, hi);
+    const int row = blockIdx.x * (kBlock / G) + threadIdx.x / G;
     const int lane = threadIdx.x % G;
     double acc[1] = {0.0};
-    for (int row0 = lo; row0 < hi; row0 += kGroups) {
-        const int row = row0 + threadIdx.x / G;
-        if (row >= hi)
-            continue;
-        S ap[BR];
-        group_row_product<S, BR, BR, G>(row, lane, offsets, nullptr, columns, values, p, ap);
+    if (row < nrow) {
+        const RowAcc<S, BR> ap = group_row_product<S, BR, BR, G>(row, lane, offsets, nullptr, columns, values, p);
         if (lane < BR) {
             const size_t i = (size_t)row * BR + lane;
             const S v = pick(ap, lane);
             Ap[i] = v;
-            acc[0] += (double)p[i] * (double)v;
+            acc[0] = (double)p[i] * (double)v;
         }
     }
-    double tot[1];
-    if (grid_reduce_last<kBlock, 1>(acc, partials, &state->ticket[1], tot, lds) && threadIdx.x == 0) {
-        state->pAp = tot[0];
-        state->alpha = state->rr > state->atol_sq ? state->rz / tot[0] : 0.0;
-    }
+    write_block_partials<kBlock, 1>(acc, ws.partials, lds);
 }
 
+// One block row per thread.
 template <typename S, int BR, int PREC>
 __global__ __launch_bounds__(kBlock) void cg_update_kernel(
     int nrow, const S* __restrict__ prec, S* __restrict__ x, S* __restrict__ r, S* __restrict__ z,
-    const S* __restrict__ p, const S* __restrict__ Ap, CgState* state, double* partials
+    const S* __restrict__ p, const S* __restrict__ Ap, CgWs ws
 )
 {
-    __shared__ double lds[kBlock];
-    if (state->done)
+    __shared__ double lds[kBlock / kWave];
+    if (ws.state->done)
         return;
-    const S alpha = (S)state->alpha;
-    int lo, hi;
-    chunk_range(nrow, kBlock, lo, hi);
+    const S alpha = (S)ws.state->alpha;
+    const int row = blockIdx.x * kBlock + threadIdx.x;
     double acc[2] = {0.0, 0.0};
-    for (int row = lo + threadIdx.x; row < hi; row += kBlock) {
+    if (row < nrow) {
         S rv[BR], zv[BR];
 #pragma unroll
         for (int s = 0; s < BR; ++s) {
             const size_t i = (size_t)row * BR + s;
-            x[i] = x[i] + alpha * p[i];
-            rv[s] = r[i] - alpha * Ap[i];
+            x[i] = fma(alpha, p[i], x[i]);
+            rv[s] = fma(-alpha, Ap[i], r[i]);
             r[i] = rv[s];
         }
         apply_prec<S, BR, PREC>(prec, row, rv, zv);
@@ -233,10 +194,45 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(
             acc[1] += (double)rv[s] * (double)zv[s];
         }
     }
-    double tot[2];
-    if (grid_reduce_last<kBlock, 2>(acc, partials, &state->ticket[2], tot, lds) && threadIdx.x == 0) {
+    write_block_partials<kBlock, 2>(acc, ws.partials, lds);
+}
+
+// Finalize kernels: one workgroup sums the partials of the preceding kernel
+// (fixed order) and updates the device state.
+enum FinalizeStep { kFinInit = 0, kFinSpmv = 1, kFinUpdate = 2 };
+
+template <int STEP>
+__global__ __launch_bounds__(kFinalizeBlock) void cg_finalize_kernel(CgWs ws, int nparts, int has_prec, int first)
+{
+    __shared__ double lds[kFinalizeBlock / kWave];
+    CgState* state = ws.state;
+    if (STEP != kFinInit && state->done)
+        return;
+    constexpr int NV = STEP == kFinInit ? 3 : (STEP == kFinSpmv ? 1 : 2);
+    double tot[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+        tot[v] = sum_partials<kFinalizeBlock>(ws.partials + (size_t)v * nparts, nparts, lds);
+    if (threadIdx.x != 0)
+        return;
+    if (STEP == kFinInit) {
+        if (first) {
+            state->bb = tot[0];
+            const double a = fmax(state->rtol * sqrt(tot[0]), state->atol);
+            state->atol_sq = a * a;
+            state->iter = 0;
+        }
+        state->rr = tot[1];
+        state->rz = has_prec ? tot[NV - 1] : tot[1];
+        state->done = !(tot[1] > state->atol_sq);
+        state->alpha = 0.0;
+        state->beta = 0.0;
+    } else if (STEP == kFinSpmv) {
+        state->pAp = tot[0];
+        state->alpha = state->rr > state->atol_sq ? state->rz / tot[0] : 0.0;
+    } else {
         const double rz_old = state->rz;
-        const double rz_new = PREC != kPrecNone ? tot[1] : tot[0];
+        const double rz_new = has_prec ? tot[NV - 1] : tot[0];
         state->rr = tot[0];
         state->rz = rz_new;
         state->beta = tot[0] > state->atol_sq ? rz_new / rz_old : 0.0;
@@ -253,19 +249,19 @@ __global__ __launch_bounds__(kBlock) void cg_direction_kernel(
     if (state->done)
         return;
     const S beta = (S)state->beta;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-        p[i] = z[i] + beta * p[i];
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n)
+        p[i] = fma(beta, p[i], z[i]);
 }
 
-__global__ void cg_state_init_kernel(CgState* state, double rtol, double atol)
+__global__ void cg_state_init_kernel(CgWs ws, double rtol, double atol)
 {
+    CgState* state = ws.state;
     state->rtol = rtol;
     state->atol = atol;
     state->atol_sq = 0.0;
     state->done = 0;
     state->iter = 0;
-    for (int k = 0; k < 4; ++k)
-        state->ticket[k] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -306,15 +302,7 @@ int read_state(const CgState* dev, CgState* out, hipStream_t stream)
     return kOk;
 }
 
-int group_for_square(int64_t nnz_hint, int nrow, int br)
-{
-    const double mean = nrow > 0 && nnz_hint > 0 ? (double)nnz_hint / nrow : 8.0;
-    int g = 1;
-    while (g < mean && g < 64)
-        g <<= 1;
-    const int lo = br == 1 ? 1 : 4;
-    return g < lo ? lo : g;
-}
+int group_for_square(int64_t nnz_hint, int nrow, int br) { return bsr_mv_lanes(nrow, br, br, nnz_hint); }
 
 typedef int (*CgCallback)(int iteration, double err, double atol, void* user);
 
@@ -335,39 +323,50 @@ template <typename S> struct CgLauncher {
     CgProblem pb;
     CgWs ws;
     hipStream_t stream;
-    int grid_rows;   // workgroups for row kernels
-    int grid_vec;    // workgroups for the vector kernel
 
-    template <int BR, int G, int PREC> int init_t(int first)
+    template <int G> unsigned rows_grid() const { return (unsigned)ceil_div(pb.nrow, kBlock / G); }
+
+    template <int STEP> int finalize(unsigned nparts, int first)
     {
-        cg_init_kernel<S, BR, G, PREC><<<grid_rows, kBlock, 0, stream>>>(
-            pb.nrow, pb.offsets, pb.columns, (const S*)pb.values, (const S*)pb.prec, (const S*)pb.b,
-            (const S*)pb.x, (S*)ws.r, (S*)ws.z, (S*)ws.p, ws.state, ws.partials, first
+        cg_finalize_kernel<STEP><<<1, kFinalizeBlock, 0, stream>>>(
+            ws, (int)nparts, pb.prec_kind != kPrecNone ? 1 : 0, first
         );
         WA_LAUNCH_CHECK();
         return kOk;
+    }
+    template <int BR, int G, int PREC> int init_t(int first)
+    {
+        const unsigned grid = rows_grid<G>();
+        cg_init_kernel<S, BR, G, PREC><<<grid, kBlock, 0, stream>>>(
+            pb.nrow, pb.offsets, pb.columns, (const S*)pb.values, (const S*)pb.prec, (const S*)pb.b,
+            (const S*)pb.x, (S*)ws.r, (S*)ws.z, (S*)ws.p, ws, first
+        );
+        WA_LAUNCH_CHECK();
+        return finalize<kFinInit>(grid, first);
     }
     template <int BR, int G> int spmv_t()
     {
-        cg_spmv_kernel<S, BR, G><<<grid_rows, kBlock, 0, stream>>>(
-            pb.nrow, pb.offsets, pb.columns, (const S*)pb.values, (const S*)ws.p, (S*)ws.Ap, ws.state, ws.partials
+        const unsigned grid = rows_grid<G>();
+        cg_spmv_kernel<S, BR, G><<<grid, kBlock, 0, stream>>>(
+            pb.nrow, pb.offsets, pb.columns, (const S*)pb.values, (const S*)ws.p, (S*)ws.Ap, ws
         );
         WA_LAUNCH_CHECK();
-        return kOk;
+        return finalize<kFinSpmv>(grid, 0);
     }
     template <int BR, int PREC> int update_t()
     {
-        cg_update_kernel<S, BR, PREC><<<grid_vec, kBlock, 0, stream>>>(
-            pb.nrow, (const S*)pb.prec, (S*)pb.x, (S*)ws.r, (S*)ws.z, (const S*)ws.p, (const S*)ws.Ap, ws.state,
-            ws.partials
+        const unsigned grid = (unsigned)ceil_div(pb.nrow, kBlock);
+        cg_update_kernel<S, BR, PREC><<<grid, kBlock, 0, stream>>>(
+            pb.nrow, (const S*)pb.prec, (S*)pb.x, (S*)ws.r, (S*)ws.z, (const S*)ws.p, (const S*)ws.Ap, ws
         );
         WA_LAUNCH_CHECK();
-        return kOk;
+        return finalize<kFinUpdate>(grid, 0);
     }
     int direction()
     {
         const S* z = pb.prec_kind != kPrecNone ? (const S*)ws.z : (const S*)ws.r;
-        cg_direction_kernel<S><<<grid_vec, kBlock, 0, stream>>>((int64_t)pb.nrow * pb.br, z, (S*)ws.p, ws.state);
+        const int64_t n = (int64_t)pb.nrow * pb.br;
+        cg_direction_kernel<S><<<(unsigned)ceil_div(n, kBlock), kBlock, 0, stream>>>(n, z, (S*)ws.p, ws.state);
         WA_LAUNCH_CHECK();
         return kOk;
     }
@@ -449,18 +448,15 @@ int cg_solve_impl(
     WA_REQUIRE(pb.prec_kind >= 0 && pb.prec_kind <= 2, "cg_solve: invalid preconditioner kind %d", pb.prec_kind);
     WA_REQUIRE(pb.prec_kind == kPrecNone || pb.prec, "cg_solve: null preconditioner data");
     const int64_t n = (int64_t)pb.nrow * pb.br;
+    WA_REQUIRE(pb.nrow > 0, "cg_solve: empty system");
     size_t needed = 0;
-    CgWs ws = carve_cg(workspace, workspace_bytes, n, sizeof(S), &needed);
+    CgWs ws = carve_cg(workspace, workspace_bytes, pb.nrow, n, sizeof(S), &needed);
     WA_REQUIRE(workspace && needed <= workspace_bytes, "cg_solve: workspace too small (%zu < %zu bytes)",
                workspace_bytes, needed);
     if (maxiter <= 0)
         maxiter = (int)n;
 
-    CgLauncher<S> L{pb, ws, stream, 1, 1};
-    L.grid_rows = (int)std::max<int64_t>(1, std::min<int64_t>(kMaxGrid, ceil_div(pb.nrow, kBlock / L.g())));
-    L.grid_rows = std::min(L.grid_rows, device_cu_count() * 8);
-    L.grid_vec = (int)std::max<int64_t>(1, std::min<int64_t>(kMaxGrid, ceil_div(pb.nrow, kBlock)));
-    L.grid_vec = std::min(L.grid_vec, device_cu_count() * 4);
+    CgLauncher<S> L{pb, ws, stream};
 
     int cycle = 1;
     if (restart > 0) {
@@ -471,7 +467,7 @@ int cg_solve_impl(
 
     double rtol_v, atol_v;
     get_tolerances(sizeof(S) == 8, tol, atol, &rtol_v, &atol_v);
-    cg_state_init_kernel<<<1, 1, 0, stream>>>(ws.state, rtol_v, atol_v);
+    cg_state_init_kernel<<<1, 1, 0, stream>>>(ws, rtol_v, atol_v);
     WA_LAUNCH_CHECK();
     WA_TRY(L.init(1));
 
@@ -538,17 +534,23 @@ int cg_solve_impl(
 // ---------------------------------------------------------------------------
 
 template <typename S>
-__global__ __launch_bounds__(kBlock) void dot_kernel(
-    int64_t n, const S* __restrict__ a, const S* __restrict__ b, double* out, double* partials, unsigned int* ticket
+__global__ __launch_bounds__(kBlock) void dot_partials_kernel(
+    int64_t n, const S* __restrict__ a, const S* __restrict__ b, double* partials
 )
 {
-    __shared__ double lds[kBlock];
+    __shared__ double lds[kBlock / kWave];
     double acc[1] = {0.0};
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
         acc[0] += (double)a[i] * (double)b[i];
-    double tot[1];
-    if (grid_reduce_last<kBlock, 1>(acc, partials, ticket, tot, lds) && threadIdx.x == 0)
-        *out = tot[0];
+    write_block_partials<kBlock, 1>(acc, partials, lds);
+}
+
+__global__ __launch_bounds__(kFinalizeBlock) void dot_finalize_kernel(const double* partials, int nparts, double* out)
+{
+    __shared__ double lds[kFinalizeBlock / kWave];
+    const double t = sum_partials<kFinalizeBlock>(partials, nparts, lds);
+    if (threadIdx.x == 0)
+        *out = t;
 }
 
 // x += alpha p ; r -= alpha Ap with alpha = rr > atol_sq ? rz / pAp : 0
@@ -561,8 +563,8 @@ __global__ __launch_bounds__(kBlock) void step_xr_kernel(
     const double a = *rr > *atol_sq ? *rz / *pAp : 0.0;
     const S alpha = (S)a;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        x[i] = x[i] + alpha * p[i];
-        r[i] = r[i] - alpha * Ap[i];
+        x[i] = fma(alpha, p[i], x[i]);
+        r[i] = fma(-alpha, Ap[i], r[i]);
     }
 }
 
@@ -576,7 +578,7 @@ __global__ __launch_bounds__(kBlock) void step_p_kernel(
     const double bt = *rr > *atol_sq ? *rz_new / *rz_old : 0.0;
     const S beta = (S)bt;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-        p[i] = z[i] + beta * p[i];
+        p[i] = fma(beta, p[i], z[i]);
 }
 
 // atol_sq = max(rtol * sqrt(bb), atol)^2 (linear.py:2051-2060)
@@ -636,10 +638,12 @@ int vec_grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(kMa
 // C ABI
 // ---------------------------------------------------------------------------
 
+// n scalar unknowns; the reduction scratch is sized for n block rows, an
+// upper bound for every block size.
 WA_API size_t warp_amd_cg_workspace_size(int64_t n, int scalar_size)
 {
     size_t needed = 0;
-    wa::carve_cg(nullptr, 0, n, (size_t)scalar_size, &needed);
+    wa::carve_cg(nullptr, 0, n, n, (size_t)scalar_size, &needed);
     return needed + 256;
 }
 
@@ -675,11 +679,12 @@ WA_API size_t warp_amd_cg_state_offset(int which)
     )                                                                                                                 \
     {                                                                                                                 \
         const int grid = wa::vec_grid(n);                                                                             \
-        WA_REQUIRE(workspace && workspace_bytes >= 256 + sizeof(double) * grid,                                     \
-                   "dot: workspace too small (need %zu bytes)", 256 + sizeof(double) * grid);                         \
-        unsigned int* ticket = (unsigned int*)workspace;                                                              \
-        double* partials = (double*)((char*)workspace + 256);                                                         \
-        wa::dot_kernel<S><<<grid, wa::kBlock, 0, wa::as_stream(stream)>>>(n, a, b, out, partials, ticket);           \
+        WA_REQUIRE(workspace && workspace_bytes >= sizeof(double) * grid,                                            \
+                   "dot: workspace too small (need %zu bytes)", sizeof(double) * grid);                               \
+        double* partials = (double*)workspace;                                                                        \
+        wa::dot_partials_kernel<S><<<grid, wa::kBlock, 0, wa::as_stream(stream)>>>(n, a, b, partials);               \
+        WA_LAUNCH_CHECK();                                                                                            \
+        wa::dot_finalize_kernel<<<1, wa::kFinalizeBlock, 0, wa::as_stream(stream)>>>(partials, grid, out);           \
         WA_LAUNCH_CHECK();                                                                                            \
         return wa::kOk;                                                                                               \
     }                                                                                                                 \
@@ -732,7 +737,7 @@ WA_API size_t warp_amd_cg_state_offset(int which)
 WA_DEFINE_CG(f32, float)
 WA_DEFINE_CG(f64, double)
 
-WA_API size_t warp_amd_dot_workspace_size(int64_t n) { return 256 + sizeof(double) * wa::vec_grid(n); }
+WA_API size_t warp_amd_dot_workspace_size(int64_t n) { return sizeof(double) * wa::vec_grid(n); }
 
 WA_API int warp_amd_cg_tolerance(const double* bb, double tol, double atol, int is_f64, double* atol_sq, void* stream)
 {

@@ -10,19 +10,6 @@ namespace {
 
 constexpr int kBlock = 256;
 
-// Map the hardware workgroup id so that each of the 8 XCDs (which receive
-// workgroups round-robin) processes one contiguous range of block rows: rows
-// that share x entries then share one L2.
-__device__ __forceinline__ int xcd_contiguous_block(int bid, int nblocks)
-{
-    constexpr int kXcd = 8;
-    const int q = nblocks / kXcd;
-    const int r = nblocks % kXcd;
-    const int xcd = bid % kXcd;
-    const int local = bid / kXcd;
-    return xcd * q + (xcd < r ? xcd : r) + local;
-}
-
 template <typename S, int BR, int BC, int G>
 __global__ __launch_bounds__(kBlock) void bsr_mv_group_kernel(
     int nrow, const int* __restrict__ offsets, const int* __restrict__ row_counts, const int* __restrict__ columns,
@@ -30,24 +17,23 @@ __global__ __launch_bounds__(kBlock) void bsr_mv_group_kernel(
 )
 {
     constexpr int kRowsPerBlock = kBlock / G;
-    const int block = xcd_contiguous_block(blockIdx.x, gridDim.x);
-    const int row = block * kRowsPerBlock + threadIdx.x / G;
+    const int row = blockIdx.x * kRowsPerBlock + threadIdx.x / G;
     const int lane = threadIdx.x % G;
     if (row >= nrow)
         return;
-    S acc[BR];
+    RowAcc<S, BR> acc;
     if (alpha != S(0)) {
-        group_row_product<S, BR, BC, G>(row, lane, offsets, row_counts, columns, values, x, acc);
+        acc = group_row_product<S, BR, BC, G>(row, lane, offsets, row_counts, columns, values, x);
     } else {
 #pragma unroll
         for (int s = 0; s < BR; ++s)
-            acc[s] = S(0);
+            acc.v[s] = S(0);
     }
     if (lane < BR) {
         const size_t yi = (size_t)row * BR + lane;
         S v = pick(acc, lane) * alpha;
         if (beta != S(0))
-            v += beta * y[yi];
+            v = fma(beta, y[yi], v);
         y[yi] = v;
     }
 }
@@ -73,12 +59,12 @@ __global__ __launch_bounds__(kBlock) void bsr_mv_generic_kernel(
             const S* blk = values + (size_t)b * br * bc + (size_t)s * bc;
             const S* xs = x + (size_t)columns[b] * bc;
             for (int c = 0; c < bc; ++c)
-                v += blk[c] * xs[c];
+                v = fma(blk[c], xs[c], v);
         }
         v *= alpha;
     }
     if (beta != S(0))
-        v += beta * y[t];
+        v = fma(beta, y[t], v);
     y[t] = v;
 }
 
@@ -273,7 +259,25 @@ int jacobi_impl(
 }
 
 }  // namespace
+
+int bsr_mv_lanes(int nrow, int br, int bc, int64_t nnz_hint)
+{
+    if (br == 1 && bc == 1)
+        return group_for(nnz_hint, nrow, 1);
+    if (br == 3 && bc == 3)
+        return group_for(nnz_hint, nrow, 4);
+    return 1;
+}
+
 }  // namespace wa
+
+// Lanes per block row the bsr_mv / CG kernels use for this shape (1 for the
+// one-thread-per-scalar-row kernel), so callers can reproduce the summation
+// order: per lane, blocks g, g+G, ... in order, then an xor butterfly.
+WA_API int warp_amd_bsr_mv_lanes(int nrow, int block_rows, int block_cols, int64_t nnz_hint)
+{
+    return wa::bsr_mv_lanes(nrow, block_rows, block_cols, nnz_hint);
+}
 
 #define WA_DEFINE_SPARSE_MV(SUFFIX, S)                                                                                \
     WA_API int warp_amd_bsr_mv_##SUFFIX(                                                                              \
