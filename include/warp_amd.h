@@ -125,6 +125,19 @@ int warp_amd_bsr_mv_f64(
     const int* columns, const double* values, int64_t nnz_hint, const double* x, double* y, double alpha,
     double beta, void* stream);
 
+/* Same with explicit kernel shape: `lanes` per block row (power of two in
+ * [1, 64]; the reference's bsr_mv `tile_size` plays this role) and
+ * `rows_per_group` consecutive rows per lane group (1, 2 or 4); <= 0 picks the
+ * default.  Results depend on `lanes` only. */
+int warp_amd_bsr_mv_ex_f32(
+    int nrow, int ncol, int block_rows, int block_cols, const int* offsets, const int* row_counts,
+    const int* columns, const float* values, int64_t nnz_hint, const float* x, float* y, float alpha, float beta,
+    int lanes, int rows_per_group, void* stream);
+int warp_amd_bsr_mv_ex_f64(
+    int nrow, int ncol, int block_rows, int block_cols, const int* offsets, const int* row_counts,
+    const int* columns, const double* values, int64_t nnz_hint, const double* x, double* y, double alpha,
+    double beta, int lanes, int rows_per_group, void* stream);
+
 /* Lanes per block row G the vector-BSR kernels (bsr_mv and the fused CG
  * kernels) use for this shape and nnz_hint; 1 for the generic kernel.  Each
  * lane sums blocks g, g+G, ... with fused multiply-adds, then the group

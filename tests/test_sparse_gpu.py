@@ -316,3 +316,62 @@ def test_bsr_get_diag_and_jacobi(device, oracle_mod):
         np.testing.assert_allclose(inv[r] @ d[r].astype(np.float64), np.eye(3), atol=2e-4)
     # absent diagonal block: block Jacobi falls back to the point inverse (identity here)
     np.testing.assert_array_equal(inv[0], np.eye(3))
+
+
+@pytest.mark.parametrize("bshape", [(1, 1), (3, 3)])
+def test_bsr_mv_every_kernel_shape(device, oracle_mod, bshape):
+    """Every lanes x rows-per-group kernel writes every output and matches the
+    oracle's lane-order sum bit for bit (y starts as NaN, so a skipped store fails)."""
+    from warp_amd import _native
+
+    lib = _native.lib()
+    rng = np.random.default_rng(31)
+    nrow, ncol = 203, 190
+    lengths = rng.integers(0, 40, nrow)
+    lengths[::17] = 0
+    rows = np.repeat(np.arange(nrow), lengths).astype(np.int32)
+    cols = rng.integers(0, ncol, rows.size).astype(np.int32)
+    br, bc = bshape
+    vals = rng.standard_normal((rows.size, br, bc)).astype(np.float32)
+    A = _assemble(device, nrow, ncol, rows, cols, vals if br > 1 else vals.reshape(-1))
+    nnz = A.nnz_sync()
+    o = oracle_mod.bsr_from_triplets(nrow, ncol, rows, cols, vals)
+    x = rng.standard_normal(ncol * bc).astype(np.float32)
+    xt = _t(x, device)
+    rpgs = [1, 2, 4, 8] if br == 1 else [1, 2, 4]
+    for lanes in [1, 2, 4, 8, 16, 32, 64]:
+        ref = oracle_mod.bsr_mv(o["offsets"], o["columns"], o["values"], x, group=lanes)
+        for rpg in rpgs:
+            y = torch.full((nrow * br,), float("nan"), device=device)
+            st = lib.warp_amd_bsr_mv_ex_f32(
+                nrow, ncol, br, bc, A.offsets.data_ptr(), None, A.columns.data_ptr(), A.values.data_ptr(), nnz,
+                xt.data_ptr(), y.data_ptr(), 1.0, 0.0, lanes, rpg, torch.cuda.current_stream(device).cuda_stream,
+            )
+            assert st == 0, lib.warp_amd_last_error()
+            got = y.cpu().numpy()
+            assert got.tobytes() == ref.tobytes(), f"lanes={lanes} rows_per_group={rpg}"
+
+
+def test_bsr_mv_graph_capture(device):
+    """bsr_mv captured in a HIP graph (as test_capturability, test_sparse.py:1005-1041):
+    every replay, with eager work in between, equals the eager result."""
+    rows, cols, vals, b, nn = problems.elasticity3d_q1(6, device="cpu")
+    A = sparse.bsr_zeros(nn, nn, wa.mat33, device=device)
+    sparse.bsr_set_from_triplets(A, rows.to(device), cols.to(device), vals.to(device), prune_numerical_zeros=False)
+    A.nnz_sync()
+    x = torch.randn(3 * nn, device=device)
+    y = torch.zeros(3 * nn, device=device)
+    eager = sparse.bsr_mv(A, x).reshape(-1).clone()
+    s = torch.cuda.Stream(device)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            sparse.bsr_mv(A, x, y, alpha=1.0, beta=0.0)
+    for k in range(4):
+        y.fill_(float(k))
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(y, eager), f"replay {k}"
+        eager2 = sparse.bsr_mv(A, x).reshape(-1)
+        assert torch.equal(eager2, eager)

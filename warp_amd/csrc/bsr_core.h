@@ -62,24 +62,111 @@ __device__ __forceinline__ RowAcc<S, BR> group_row_product(
     return acc;
 }
 
-// acc.v[i] for a lane-dependent i.  Written as explicit selects on named
-// values: a loop of selects gets turned back into a dynamic index, which moves
-// the accumulators to LDS.
+// Several consecutive block rows per lane group: lane g takes block g, g+G, ...
+// of each of the RPG rows row0 .. row0+RPG-1, issuing the loads of all rows
+// before the FMAs (more bytes in flight per wave).  Each row is summed in the
+// same order as group_row_product, so results are identical to it.  On return
+// acc.v[j*BR + s] holds (A x)[(row0+j)*BR + s] in every lane of the group;
+// rows at or beyond nrow contribute zeros.
+template <typename S, int BR, int BC, int G, int RPG>
+__device__ __forceinline__ RowAcc<S, BR * RPG> group_rows_product(
+    int row0, int nrow, int lane, const int* __restrict__ offsets, const int* __restrict__ row_counts,
+    const int* __restrict__ columns, const S* __restrict__ values, const S* __restrict__ x
+)
+{
+    RowAcc<S, BR * RPG> acc;
+#pragma unroll
+    for (int s = 0; s < BR * RPG; ++s)
+        acc.v[s] = S(0);
+    int beg[RPG], cnt[RPG];
+    int nmax = 0;
+#pragma unroll
+    for (int j = 0; j < RPG; ++j) {
+        const int r = row0 + j;
+        beg[j] = 0;
+        cnt[j] = 0;
+        if (r < nrow) {
+            beg[j] = offsets[r];
+            cnt[j] = row_end(offsets, row_counts, r) - beg[j];
+        }
+        nmax = cnt[j] > nmax ? cnt[j] : nmax;
+    }
+    for (int k = lane; k < nmax; k += G) {
+        bool ok[RPG];
+        int col[RPG];
+        S m[RPG][BR * BC];
+#pragma unroll
+        for (int j = 0; j < RPG; ++j) {
+            ok[j] = k < cnt[j];
+            // inactive slots read block 0 (always valid when any row has blocks)
+            const int b = ok[j] ? beg[j] + k : 0;
+            col[j] = columns[b];
+            const S* blk = values + (size_t)b * (BR * BC);
+#pragma unroll
+            for (int q = 0; q < BR * BC; ++q)
+                m[j][q] = blk[q];
+        }
+#pragma unroll
+        for (int j = 0; j < RPG; ++j) {
+            S xv[BC];
+#pragma unroll
+            for (int c = 0; c < BC; ++c)
+                xv[c] = x[(size_t)col[j] * BC + c];
+#pragma unroll
+            for (int s = 0; s < BR; ++s) {
+                S a = acc.v[j * BR + s];
+#pragma unroll
+                for (int c = 0; c < BC; ++c)
+                    a = fma(m[j][s * BC + c], xv[c], a);
+                acc.v[j * BR + s] = ok[j] ? a : acc.v[j * BR + s];
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < BR * RPG; ++s)
+        acc.v[s] = group_sum<G>(acc.v[s]);
+    return acc;
+}
+
+// acc.v[i] for a lane-dependent i.  Written as an explicit select chain on
+// named values: a loop of selects gets turned back into a dynamic index,
+// which moves the accumulators to LDS.
 template <typename S, int N> __device__ __forceinline__ S pick(const RowAcc<S, N>& acc, int i)
 {
-    static_assert(N >= 1 && N <= 4, "pick supports up to 4 rows per block");
-    if constexpr (N == 1) {
-        return acc.v[0];
-    } else if constexpr (N == 2) {
-        const S a = acc.v[0], b = acc.v[1];
-        return i == 1 ? b : a;
-    } else if constexpr (N == 3) {
-        const S a = acc.v[0], b = acc.v[1], c = acc.v[2];
-        return i == 2 ? c : (i == 1 ? b : a);
-    } else {
-        const S a = acc.v[0], b = acc.v[1], c = acc.v[2], d = acc.v[3];
-        return i == 3 ? d : (i == 2 ? c : (i == 1 ? b : a));
+    static_assert(N >= 1 && N <= 16, "pick supports up to 16 values");
+    S v = acc.v[0];
+#define WA_PICK_STEP(K)                                                                                               \
+    if constexpr (N > K) {                                                                                            \
+        const S t = acc.v[K];                                                                                         \
+        v = i == K ? t : v;                                                                                           \
     }
+    WA_PICK_STEP(1) WA_PICK_STEP(2) WA_PICK_STEP(3) WA_PICK_STEP(4) WA_PICK_STEP(5) WA_PICK_STEP(6) WA_PICK_STEP(7)
+    WA_PICK_STEP(8) WA_PICK_STEP(9) WA_PICK_STEP(10) WA_PICK_STEP(11) WA_PICK_STEP(12) WA_PICK_STEP(13)
+    WA_PICK_STEP(14) WA_PICK_STEP(15)
+#undef WA_PICK_STEP
+    return v;
+}
+
+// Rows per lane group of the vector-BSR kernels (measured on MI355X: two
+// 3x3 rows, four scalar rows).
+constexpr int rows_per_group_for(int br, int bc) { return (br == 1 && bc == 1) ? 4 : 2; }
+
+// Lanes per block row and rows per lane group for a matrix with `nnz_hint`
+// blocks over `nrow` rows: about one block per lane for block rows, two
+// entries per lane for scalar rows (lanes = the power of two covering that).
+struct BsrShape {
+    int lanes;
+    int rows_per_group;
+};
+
+inline BsrShape default_bsr_shape(int nrow, int br, int bc, int64_t nnz_hint)
+{
+    const double mean = nrow > 0 && nnz_hint > 0 ? (double)nnz_hint / nrow : 8.0;
+    const double per_lane = (br == 1 && bc == 1) ? 2.0 : 1.0;
+    int g = 1;
+    while (g * per_lane < mean && g < 64)
+        g <<= 1;
+    return BsrShape{g, rows_per_group_for(br, bc)};
 }
 
 }  // namespace wa

@@ -99,28 +99,30 @@ __device__ __forceinline__ void apply_prec(const S* __restrict__ prec, int row, 
     }
 }
 
-// r = b - A x ; z = M r ; p = z ; (b.b, r.r, r.z).  `first` also forms the
-// absolute tolerance (linear.py:2051-2083) and resets the iteration count.
-// One block row per group of G lanes; the grid covers the matrix.
-template <typename S, int BR, int G, int PREC>
+
+// r = b - A x ; z = M r ; p = z ; partials of (b.b, r.r, r.z).  The grid
+// covers the matrix: kCgRpg consecutive block rows per group of G lanes.
+template <typename S, int BR, int G, int PREC, int kCgRpg = rows_per_group_for(BR, BR)>
 __global__ __launch_bounds__(kBlock) void cg_init_kernel(
     int nrow, const int* __restrict__ offsets, const int* __restrict__ columns, const S* __restrict__ values,
     const S* __restrict__ prec, const S* __restrict__ b, const S* __restrict__ x, S* __restrict__ r, S* __restrict__ z,
-    S* __restrict__ p, CgWs ws, int first
+    S* __restrict__ p, CgWs ws
 )
 {
     __shared__ double lds[kBlock / kWave];
-    const int row = blockIdx.x * (kBlock / G) + threadIdx.x / G;
+    const int row0 = (blockIdx.x * (kBlock / G) + threadIdx.x / G) * kCgRpg;
     const int lane = threadIdx.x % G;
     double acc3[3] = {0.0, 0.0, 0.0};
-    if (row < nrow) {
-        const RowAcc<S, BR> ax = group_row_product<S, BR, BR, G>(row, lane, offsets, nullptr, columns, values, x);
-        if (lane == 0) {
+    if (row0 < nrow) {
+        const RowAcc<S, BR * kCgRpg> ax =
+            group_rows_product<S, BR, BR, G, kCgRpg>(row0, nrow, lane, offsets, nullptr, columns, values, x);
+        for (int j = lane; j < kCgRpg && row0 + j < nrow; j += G) {
+            const int row = row0 + j;
             S rv[BR], zv[BR];
 #pragma unroll
             for (int s = 0; s < BR; ++s) {
                 const S bv = b[(size_t)row * BR + s];
-                rv[s] = bv - ax.v[s];
+                rv[s] = bv - pick(ax, j * BR + s);
                 acc3[0] += (double)bv * (double)bv;
             }
             apply_prec<S, BR, PREC>(prec, row, rv, zv);
@@ -139,7 +141,7 @@ __global__ __launch_bounds__(kBlock) void cg_init_kernel(
     write_block_partials<kBlock, 3>(acc3, ws.partials, lds);
 }
 
-template <typename S, int BR, int G>
+template <typename S, int BR, int G, int kCgRpg = rows_per_group_for(BR, BR)>
 __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(
     int nrow, const int* __restrict__ offsets, const int* __restrict__ columns, const S* __restrict__ values,
     const S* __restrict__ p, S* __restrict__ Ap, CgWs ws
@@ -148,16 +150,17 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(
     __shared__ double lds[kBlock / kWave];
     if (ws.state->done)
         return;
-    const int row = blockIdx.x * (kBlock / G) + threadIdx.x / G;
+    const int row0 = (blockIdx.x * (kBlock / G) + threadIdx.x / G) * kCgRpg;
     const int lane = threadIdx.x % G;
     double acc[1] = {0.0};
-    if (row < nrow) {
-        const RowAcc<S, BR> ap = group_row_product<S, BR, BR, G>(row, lane, offsets, nullptr, columns, values, p);
-        if (lane < BR) {
-            const size_t i = (size_t)row * BR + lane;
-            const S v = pick(ap, lane);
+    if (row0 < nrow) {
+        const RowAcc<S, BR * kCgRpg> ap =
+            group_rows_product<S, BR, BR, G, kCgRpg>(row0, nrow, lane, offsets, nullptr, columns, values, p);
+        for (int e = lane; e < BR * kCgRpg && row0 + e / BR < nrow; e += G) {
+            const size_t i = (size_t)row0 * BR + e;
+            const S v = pick(ap, e);
             Ap[i] = v;
-            acc[0] = (double)p[i] * (double)v;
+            acc[0] += (double)p[i] * (double)v;
         }
     }
     write_block_partials<kBlock, 1>(acc, ws.partials, lds);
@@ -324,7 +327,10 @@ template <typename S> struct CgLauncher {
     CgWs ws;
     hipStream_t stream;
 
-    template <int G> unsigned rows_grid() const { return (unsigned)ceil_div(pb.nrow, kBlock / G); }
+    template <int BR, int G> unsigned rows_grid() const
+    {
+        return (unsigned)ceil_div(pb.nrow, (kBlock / G) * rows_per_group_for(BR, BR));
+    }
 
     template <int STEP> int finalize(unsigned nparts, int first)
     {
@@ -336,17 +342,17 @@ template <typename S> struct CgLauncher {
     }
     template <int BR, int G, int PREC> int init_t(int first)
     {
-        const unsigned grid = rows_grid<G>();
+        const unsigned grid = rows_grid<BR, G>();
         cg_init_kernel<S, BR, G, PREC><<<grid, kBlock, 0, stream>>>(
             pb.nrow, pb.offsets, pb.columns, (const S*)pb.values, (const S*)pb.prec, (const S*)pb.b,
-            (const S*)pb.x, (S*)ws.r, (S*)ws.z, (S*)ws.p, ws, first
+            (const S*)pb.x, (S*)ws.r, (S*)ws.z, (S*)ws.p, ws
         );
         WA_LAUNCH_CHECK();
         return finalize<kFinInit>(grid, first);
     }
     template <int BR, int G> int spmv_t()
     {
-        const unsigned grid = rows_grid<G>();
+        const unsigned grid = rows_grid<BR, G>();
         cg_spmv_kernel<S, BR, G><<<grid, kBlock, 0, stream>>>(
             pb.nrow, pb.offsets, pb.columns, (const S*)pb.values, (const S*)ws.p, (S*)ws.Ap, ws
         );
@@ -375,8 +381,8 @@ template <typename S> struct CgLauncher {
 
 #define WA_CG_DISPATCH_G(BR, CALL)                                                                                    \
     switch (g()) {                                                                                                    \
-    case 1: if (BR == 1) return CALL(BR, 1); [[fallthrough]];                                                         \
-    case 2: if (BR == 1) return CALL(BR, 2); [[fallthrough]];                                                         \
+    case 1: return CALL(BR, 1);                                                                                       \
+    case 2: return CALL(BR, 2);                                                                                       \
     case 4: return CALL(BR, 4);                                                                                       \
     case 8: return CALL(BR, 8);                                                                                       \
     case 16: return CALL(BR, 16);                                                                                     \

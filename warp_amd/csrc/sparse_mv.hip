@@ -10,28 +10,28 @@ namespace {
 
 constexpr int kBlock = 256;
 
-template <typename S, int BR, int BC, int G>
-__global__ __launch_bounds__(kBlock) void bsr_mv_group_kernel(
+// RPG consecutive block rows per group of G lanes (see group_rows_product).
+template <typename S, int BR, int BC, int G, int RPG>
+__global__ __launch_bounds__(kBlock) void bsr_mv_rows_kernel(
     int nrow, const int* __restrict__ offsets, const int* __restrict__ row_counts, const int* __restrict__ columns,
     const S* __restrict__ values, const S* __restrict__ x, S* __restrict__ y, S alpha, S beta
 )
 {
-    constexpr int kRowsPerBlock = kBlock / G;
-    const int row = blockIdx.x * kRowsPerBlock + threadIdx.x / G;
+    const int row0 = (blockIdx.x * (kBlock / G) + threadIdx.x / G) * RPG;
     const int lane = threadIdx.x % G;
-    if (row >= nrow)
+    if (row0 >= nrow)
         return;
-    RowAcc<S, BR> acc;
+    RowAcc<S, BR * RPG> acc;
     if (alpha != S(0)) {
-        acc = group_row_product<S, BR, BC, G>(row, lane, offsets, row_counts, columns, values, x);
+        acc = group_rows_product<S, BR, BC, G, RPG>(row0, nrow, lane, offsets, row_counts, columns, values, x);
     } else {
 #pragma unroll
-        for (int s = 0; s < BR; ++s)
+        for (int s = 0; s < BR * RPG; ++s)
             acc.v[s] = S(0);
     }
-    if (lane < BR) {
-        const size_t yi = (size_t)row * BR + lane;
-        S v = pick(acc, lane) * alpha;
+    for (int e = lane; e < BR * RPG && row0 + e / BR < nrow; e += G) {
+        const size_t yi = (size_t)row0 * BR + e;
+        S v = pick(acc, e) * alpha;
         if (beta != S(0))
             v = fma(beta, y[yi], v);
         y[yi] = v;
@@ -68,35 +68,66 @@ __global__ __launch_bounds__(kBlock) void bsr_mv_generic_kernel(
     y[t] = v;
 }
 
-template <typename S, int BR, int BC, int G>
-int launch_group(
+template <typename S, int BR, int BC, int G, int RPG>
+int launch_rows(
     int nrow, const int* offsets, const int* row_counts, const int* columns, const S* values, const S* x, S* y,
     S alpha, S beta, hipStream_t stream
 )
 {
-    constexpr int kRowsPerBlock = kBlock / G;
+    constexpr int kRowsPerBlock = (kBlock / G) * RPG;
     const int64_t blocks = ceil_div(nrow, kRowsPerBlock);
-    bsr_mv_group_kernel<S, BR, BC, G>
+    bsr_mv_rows_kernel<S, BR, BC, G, RPG>
         <<<(unsigned)blocks, kBlock, 0, stream>>>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta);
     WA_LAUNCH_CHECK();
     return kOk;
 }
 
-// Lanes per block row: the power of two nearest above the mean row length,
-// within [lo, 64].
-int group_for(int64_t nnz_hint, int nrow, int lo)
+template <typename S, int BR, int BC, int G>
+int launch_rpg(
+    int rpg, int nrow, const int* offsets, const int* row_counts, const int* columns, const S* values, const S* x,
+    S* y, S alpha, S beta, hipStream_t stream
+)
 {
-    double mean = nrow > 0 && nnz_hint > 0 ? (double)nnz_hint / nrow : 8.0;
-    int g = 1;
-    while (g < mean && g < 64)
-        g <<= 1;
-    return g < lo ? lo : g;
+    switch (rpg) {
+    case 1: return launch_rows<S, BR, BC, G, 1>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+    case 2: return launch_rows<S, BR, BC, G, 2>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+    case 8:
+        if constexpr (BR * BC == 1)
+            return launch_rows<S, BR, BC, G, 8>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+        [[fallthrough]];
+    default: return launch_rows<S, BR, BC, G, 4>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+    }
+}
+
+template <typename S, int BR, int BC>
+int launch_lanes(
+    int g, int rpg, int nrow, const int* offsets, const int* row_counts, const int* columns, const S* values,
+    const S* x, S* y, S alpha, S beta, hipStream_t stream
+)
+{
+#define WA_LANES_CASE(G)                                                                                              \
+    case G:                                                                                                           \
+        return launch_rpg<S, BR, BC, G>(rpg, nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+    switch (g) {
+        WA_LANES_CASE(1)
+        WA_LANES_CASE(2)
+        WA_LANES_CASE(4)
+        WA_LANES_CASE(8)
+        WA_LANES_CASE(16)
+        WA_LANES_CASE(32)
+        WA_LANES_CASE(64)
+    default:
+        set_error("bsr_mv: lanes per row must be a power of two in [1, 64], got %d", g);
+        return kErrArg;
+    }
+#undef WA_LANES_CASE
 }
 
 template <typename S>
 int bsr_mv_impl(
     int nrow, int ncol, int br, int bc, const int* offsets, const int* row_counts, const int* columns,
-    const S* values, int64_t nnz_hint, const S* x, S* y, S alpha, S beta, void* stream_ptr
+    const S* values, int64_t nnz_hint, const S* x, S* y, S alpha, S beta, int lanes, int rows_per_group,
+    void* stream_ptr
 )
 {
     WA_REQUIRE(nrow >= 0 && ncol >= 0 && br > 0 && bc > 0, "bsr_mv: invalid shape (%d, %d, %d, %d)", nrow, ncol, br, bc);
@@ -104,27 +135,17 @@ int bsr_mv_impl(
         return kOk;
     WA_REQUIRE(offsets && y, "bsr_mv: null offsets or y");
     WA_REQUIRE(alpha == S(0) || (columns && values && x), "bsr_mv: null matrix or x with nonzero alpha");
+    WA_REQUIRE(rows_per_group <= 0 || rows_per_group == 1 || rows_per_group == 2 || rows_per_group == 4 ||
+                   (rows_per_group == 8 && br * bc == 1),
+               "bsr_mv: rows per group must be 1, 2 or 4 (or 8 for scalar blocks), got %d", rows_per_group);
     hipStream_t stream = as_stream(stream_ptr);
-    if (br == 1 && bc == 1) {
-        switch (group_for(nnz_hint, nrow, 1)) {
-        case 1: return launch_group<S, 1, 1, 1>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
-        case 2: return launch_group<S, 1, 1, 2>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
-        case 4: return launch_group<S, 1, 1, 4>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
-        case 8: return launch_group<S, 1, 1, 8>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
-        case 16: return launch_group<S, 1, 1, 16>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
-        case 32: return launch_group<S, 1, 1, 32>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
-        default: return launch_group<S, 1, 1, 64>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
-        }
-    }
-    if (br == 3 && bc == 3) {
-        switch (group_for(nnz_hint, nrow, 4)) {
-        case 4: return launch_group<S, 3, 3, 4>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
-        case 8: return launch_group<S, 3, 3, 8>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
-        case 16: return launch_group<S, 3, 3, 16>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
-        case 32: return launch_group<S, 3, 3, 32>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
-        default: return launch_group<S, 3, 3, 64>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
-        }
-    }
+    const BsrShape shape = default_bsr_shape(nrow, br, bc, nnz_hint);
+    const int g = lanes > 0 ? lanes : shape.lanes;
+    const int rpg = rows_per_group > 0 ? rows_per_group : shape.rows_per_group;
+    if (br == 1 && bc == 1)
+        return launch_lanes<S, 1, 1>(g, rpg, nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+    if (br == 3 && bc == 3)
+        return launch_lanes<S, 3, 3>(g, rpg, nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
     const int64_t threads = (int64_t)nrow * br;
     bsr_mv_generic_kernel<S><<<(unsigned)ceil_div(threads, kBlock), kBlock, 0, stream>>>(
         nrow, br, bc, offsets, row_counts, columns, values, x, y, alpha, beta
@@ -262,10 +283,8 @@ int jacobi_impl(
 
 int bsr_mv_lanes(int nrow, int br, int bc, int64_t nnz_hint)
 {
-    if (br == 1 && bc == 1)
-        return group_for(nnz_hint, nrow, 1);
-    if (br == 3 && bc == 3)
-        return group_for(nnz_hint, nrow, 4);
+    if ((br == 1 && bc == 1) || (br == 3 && bc == 3))
+        return default_bsr_shape(nrow, br, bc, nnz_hint).lanes;
     return 1;
 }
 
@@ -287,7 +306,18 @@ WA_API int warp_amd_bsr_mv_lanes(int nrow, int block_rows, int block_cols, int64
     {                                                                                                                 \
         return wa::bsr_mv_impl<S>(                                                                                    \
             nrow, ncol, block_rows, block_cols, offsets, row_counts, columns, values, nnz_hint, x, y, alpha, beta,    \
-            stream                                                                                                    \
+            0, 0, stream                                                                                              \
+        );                                                                                                            \
+    }                                                                                                                 \
+    WA_API int warp_amd_bsr_mv_ex_##SUFFIX(                                                                           \
+        int nrow, int ncol, int block_rows, int block_cols, const int* offsets, const int* row_counts,                \
+        const int* columns, const S* values, int64_t nnz_hint, const S* x, S* y, S alpha, S beta, int lanes,          \
+        int rows_per_group, void* stream                                                                              \
+    )                                                                                                                 \
+    {                                                                                                                 \
+        return wa::bsr_mv_impl<S>(                                                                                    \
+            nrow, ncol, block_rows, block_cols, offsets, row_counts, columns, values, nnz_hint, x, y, alpha, beta,    \
+            lanes, rows_per_group, stream                                                                             \
         );                                                                                                            \
     }                                                                                                                 \
     WA_API int warp_amd_bsr_get_diag_##SUFFIX(                                                                        \

@@ -458,8 +458,9 @@ def bsr_mv(
 
     ``alpha == 0`` leaves ``x`` unread, ``beta == 0`` leaves ``y`` unread.
     ``x`` and ``y`` may alias (``work_buffer`` then holds the old ``y``).
-    ``tile_size`` is accepted for signature compatibility; the lane count per
-    block row is chosen from the mean row length.
+    ``tile_size``: lanes per block row (a power of two <= 64) when positive;
+    negative selects one lane per row (sums in storage order, like the
+    reference row kernel); 0 chooses from the mean row length.
     """
     A, A_scale = _extract_matrix_and_scale(A)
     alpha = float(alpha) * A_scale
@@ -504,9 +505,17 @@ def bsr_mv(
         raise ValueError("Incompatible 'y' vector for bsr_mv") from err
 
     _require_gpu(A.values.device, "bsr_mv")
-    fn = "warp_amd_bsr_mv_f64" if torch_dtype == torch.float64 else "warp_amd_bsr_mv_f32"
     if torch_dtype not in (torch.float32, torch.float64):
         raise TypeError(f"bsr_mv supports float32 and float64 matrices, got {torch_dtype}")
+    fn = "warp_amd_bsr_mv_ex_f64" if torch_dtype == torch.float64 else "warp_amd_bsr_mv_ex_f32"
+    # tile_size > 0: lanes per block row; < 0: one lane per row (row-sequential
+    # sums, the reference row kernel's order); 0: chosen from the mean row length
+    if tile_size > 0:
+        if tile_size & (tile_size - 1) or tile_size > 64:
+            raise ValueError(f"tile_size must be a power of two <= 64, got {tile_size}")
+        lanes = int(tile_size)
+    else:
+        lanes = 1 if tile_size < 0 else 0
     _native.call(
         fn,
         nrow,
@@ -522,6 +531,8 @@ def bsr_mv(
         _native.ptr(y_view),
         alpha,
         beta,
+        lanes,
+        0,
         _native.stream_of(A.values.device),
     )
     return y
