@@ -234,6 +234,22 @@ int warp_amd_dense_mv_f32(int m, int k, const float* A, const float* x, const fl
 int warp_amd_dense_mv_f64(int m, int k, const double* A, const double* x, const double* y, double* z, double alpha,
                           double beta, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Mixed precision: FP64 iterative refinement around the FP32 CG            */
+/* (no reference counterpart: the reference solves in one precision)        */
+/* ------------------------------------------------------------------------ */
+
+size_t warp_amd_bsr_residual_workspace_size(int nrow);
+/* r64 = b - A x, FP64 accumulation of the FP32 matrix values (1x1 or 3x3
+ * blocks, compact), FP64 x and b; r32 = (float) r64; *rr (device) = r64.r64
+ * summed in a fixed order. */
+int warp_amd_bsr_residual_f32_f64(
+    int nrow, int block_size, const int* offsets, const int* columns, const float* values, int64_t nnz_hint,
+    const double* x, const double* b, double* r64, float* r32, double* rr, void* workspace, size_t workspace_bytes,
+    void* stream);
+/* x64 += alpha * (double) d32 */
+int warp_amd_axpy_f32_to_f64(int64_t n, double alpha, const float* d, double* x, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,10 +68,13 @@ __device__ __forceinline__ RowAcc<S, BR> group_row_product(
 // same order as group_row_product, so results are identical to it.  On return
 // acc.v[j*BR + s] holds (A x)[(row0+j)*BR + s] in every lane of the group;
 // rows at or beyond nrow contribute zeros.
-template <typename S, int BR, int BC, int G, int RPG>
+//
+// V is the stored value type and S the vector / accumulation type (V = float,
+// S = double gives the FP64 residual of an FP32 matrix).
+template <typename S, int BR, int BC, int G, int RPG, typename V = S>
 __device__ __forceinline__ RowAcc<S, BR * RPG> group_rows_product(
     int row0, int nrow, int lane, const int* __restrict__ offsets, const int* __restrict__ row_counts,
-    const int* __restrict__ columns, const S* __restrict__ values, const S* __restrict__ x
+    const int* __restrict__ columns, const V* __restrict__ values, const S* __restrict__ x
 )
 {
     RowAcc<S, BR * RPG> acc;
@@ -94,14 +97,14 @@ __device__ __forceinline__ RowAcc<S, BR * RPG> group_rows_product(
     for (int k = lane; k < nmax; k += G) {
         bool ok[RPG];
         int col[RPG];
-        S m[RPG][BR * BC];
+        V m[RPG][BR * BC];
 #pragma unroll
         for (int j = 0; j < RPG; ++j) {
             ok[j] = k < cnt[j];
             // inactive slots read block 0 (always valid when any row has blocks)
             const int b = ok[j] ? beg[j] + k : 0;
             col[j] = columns[b];
-            const S* blk = values + (size_t)b * (BR * BC);
+            const V* blk = values + (size_t)b * (BR * BC);
 #pragma unroll
             for (int q = 0; q < BR * BC; ++q)
                 m[j][q] = blk[q];
@@ -117,7 +120,7 @@ __device__ __forceinline__ RowAcc<S, BR * RPG> group_rows_product(
                 S a = acc.v[j * BR + s];
 #pragma unroll
                 for (int c = 0; c < BC; ++c)
-                    a = fma(m[j][s * BC + c], xv[c], a);
+                    a = fma((S)m[j][s * BC + c], xv[c], a);
                 acc.v[j * BR + s] = ok[j] ? a : acc.v[j * BR + s];
             }
         }

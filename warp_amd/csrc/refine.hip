@@ -1,0 +1,131 @@
+// Mixed-precision pieces for FP64 iterative refinement around the FP32 CG.
+//
+// No reference counterpart: the reference solves in one precision
+// (warp/_src/optim/linear.py).  Classical iterative refinement (e.g. Carson &
+// Higham, SIAM J. Sci. Comput. 40(2), 2018): r = b - A x in FP64 with the FP32
+// matrix promoted in-kernel, solve A d = r in FP32, x += d in FP64.
+#include "bsr_core.h"
+
+namespace wa {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kFinalizeBlock = 1024;
+
+// r64 = b - A x (FP64 accumulate, FP32 values), r32 = (float) r64, partials of r.r.
+template <int BR, int G, int RPG>
+__global__ __launch_bounds__(kBlock) void residual_kernel(
+    int nrow, const int* __restrict__ offsets, const int* __restrict__ columns, const float* __restrict__ values,
+    const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ r64, float* __restrict__ r32,
+    double* __restrict__ partials
+)
+{
+    __shared__ double lds[kBlock / kWave];
+    const int row0 = (blockIdx.x * (kBlock / G) + threadIdx.x / G) * RPG;
+    const int lane = threadIdx.x % G;
+    double acc[1] = {0.0};
+    if (row0 < nrow) {
+        const RowAcc<double, BR * RPG> ax =
+            group_rows_product<double, BR, BR, G, RPG, float>(row0, nrow, lane, offsets, nullptr, columns, values, x);
+        for (int e = lane; e < BR * RPG && row0 + e / BR < nrow; e += G) {
+            const size_t i = (size_t)row0 * BR + e;
+            const double rv = b[i] - pick(ax, e);
+            r64[i] = rv;
+            r32[i] = (float)rv;
+            acc[0] += rv * rv;
+        }
+    }
+    write_block_partials<kBlock, 1>(acc, partials, lds);
+}
+
+__global__ __launch_bounds__(kFinalizeBlock) void finalize_sum_kernel(const double* partials, int nparts, double* out)
+{
+    __shared__ double lds[kFinalizeBlock / kWave];
+    const double t = sum_partials<kFinalizeBlock>(partials, nparts, lds);
+    if (threadIdx.x == 0)
+        *out = t;
+}
+
+__global__ __launch_bounds__(kBlock) void axpy_f32_to_f64_kernel(int64_t n, double alpha, const float* __restrict__ d,
+                                                                 double* __restrict__ x)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n)
+        x[i] = fma(alpha, (double)d[i], x[i]);
+}
+
+template <int BR, int G>
+int launch_residual(
+    int nrow, const int* offsets, const int* columns, const float* values, const double* x, const double* b,
+    double* r64, float* r32, double* partials, double* rr, hipStream_t stream
+)
+{
+    constexpr int RPG = rows_per_group_for(BR, BR);
+    const unsigned grid = (unsigned)ceil_div(nrow, (kBlock / G) * RPG);
+    residual_kernel<BR, G, RPG><<<grid, kBlock, 0, stream>>>(nrow, offsets, columns, values, x, b, r64, r32, partials);
+    WA_LAUNCH_CHECK();
+    finalize_sum_kernel<<<1, kFinalizeBlock, 0, stream>>>(partials, (int)grid, rr);
+    WA_LAUNCH_CHECK();
+    return kOk;
+}
+
+template <int BR>
+int residual_dispatch(
+    int g, int nrow, const int* offsets, const int* columns, const float* values, const double* x, const double* b,
+    double* r64, float* r32, double* partials, double* rr, hipStream_t stream
+)
+{
+    switch (g) {
+    case 1: return launch_residual<BR, 1>(nrow, offsets, columns, values, x, b, r64, r32, partials, rr, stream);
+    case 2: return launch_residual<BR, 2>(nrow, offsets, columns, values, x, b, r64, r32, partials, rr, stream);
+    case 4: return launch_residual<BR, 4>(nrow, offsets, columns, values, x, b, r64, r32, partials, rr, stream);
+    case 8: return launch_residual<BR, 8>(nrow, offsets, columns, values, x, b, r64, r32, partials, rr, stream);
+    case 16: return launch_residual<BR, 16>(nrow, offsets, columns, values, x, b, r64, r32, partials, rr, stream);
+    case 32: return launch_residual<BR, 32>(nrow, offsets, columns, values, x, b, r64, r32, partials, rr, stream);
+    default: return launch_residual<BR, 64>(nrow, offsets, columns, values, x, b, r64, r32, partials, rr, stream);
+    }
+}
+
+}  // namespace
+}  // namespace wa
+
+// Partials of the residual kernel: one double per workgroup, at most one
+// workgroup per 4 block rows.
+WA_API size_t warp_amd_bsr_residual_workspace_size(int nrow)
+{
+    return sizeof(double) * (size_t)(wa::ceil_div(nrow > 0 ? nrow : 1, 4) + 1);
+}
+
+// r64 = b - A x and r32 = (float) r64 for a compact BSR matrix with FP32
+// values (1x1 or 3x3 blocks) and FP64 vectors; *rr (device) = r64 . r64,
+// summed in a fixed order.
+WA_API int warp_amd_bsr_residual_f32_f64(
+    int nrow, int block_size, const int* offsets, const int* columns, const float* values, int64_t nnz_hint,
+    const double* x, const double* b, double* r64, float* r32, double* rr, void* workspace, size_t workspace_bytes,
+    void* stream
+)
+{
+    WA_REQUIRE(block_size == 1 || block_size == 3, "bsr_residual: block size must be 1 or 3, got %d", block_size);
+    WA_REQUIRE(nrow > 0, "bsr_residual: empty matrix");
+    WA_REQUIRE(workspace && workspace_bytes >= warp_amd_bsr_residual_workspace_size(nrow),
+               "bsr_residual: workspace too small");
+    const int g = wa::default_bsr_shape(nrow, block_size, block_size, nnz_hint).lanes;
+    double* partials = static_cast<double*>(workspace);
+    if (block_size == 1)
+        return wa::residual_dispatch<1>(g, nrow, offsets, columns, values, x, b, r64, r32, partials, rr,
+                                        wa::as_stream(stream));
+    return wa::residual_dispatch<3>(g, nrow, offsets, columns, values, x, b, r64, r32, partials, rr,
+                                    wa::as_stream(stream));
+}
+
+// x64 += alpha * (double) d32
+WA_API int warp_amd_axpy_f32_to_f64(int64_t n, double alpha, const float* d, double* x, void* stream)
+{
+    if (n <= 0)
+        return wa::kOk;
+    wa::axpy_f32_to_f64_kernel<<<(unsigned)wa::ceil_div(n, wa::kBlock), wa::kBlock, 0, wa::as_stream(stream)>>>(
+        n, alpha, d, x
+    );
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}

@@ -1,3 +1,4 @@
 """Iterative solvers (``warp.optim`` drop-in for the CG path)."""
 
 from . import linear
+from . import refine

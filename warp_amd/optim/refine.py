@@ -1,0 +1,111 @@
+"""FP64 iterative refinement around the FP32 conjugate gradient.
+
+No reference counterpart (the reference CG runs in the system's single
+precision, warp/_src/optim/linear.py:893-1121).  Classical mixed-precision
+iterative refinement (Carson & Higham, SIAM J. Sci. Comput. 40(2), 2018):
+
+    repeat:  r = b - A x          FP64 vectors, FP32 matrix promoted in-kernel
+             stop if ||r|| <= tol ||b||
+             solve A d = r         FP32 Jacobi / block-Jacobi CG, FP64 dots,
+                                   to relative tolerance ``inner_tol``
+             x = x + d             FP64
+
+The bulk of the bytes (SpMV, preconditioner, vector updates) moves in FP32;
+FP64 is used for the true residual, the update of x and the CG scalars.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import torch
+
+from .. import _native, sparse
+from . import linear
+
+
+@dataclass
+class RefinementResult:
+    outer_iterations: int
+    inner_iterations: int
+    relative_residual: float
+    history: list = field(default_factory=list)  # (outer, inner so far, ||r||/||b||)
+
+
+def cg_refined(
+    A: sparse.BsrMatrix,
+    b: torch.Tensor,
+    x: torch.Tensor,
+    tol: float = 1.0e-10,
+    inner_tol: float = 1.0e-4,
+    M: str | None = "diag",
+    max_outer: int = 50,
+    inner_maxiter: int = 0,
+    check_every: int = 10,
+    callback=None,
+) -> RefinementResult:
+    """Solve A x = b to an FP64 relative residual ``tol`` (x: FP64 initial guess in, solution out).
+
+    ``A``: compact BsrMatrix with FP32 values and 1x1 or 3x3 blocks.  ``b``,
+    ``x``: FP64 tensors with ``A.shape[0]`` entries.  ``M``: "diag",
+    "block_diag" or None for the inner FP32 solve.
+    """
+    if A.scalar_type.torch_dtype != torch.float32:
+        raise TypeError("cg_refined expects a float32 matrix")
+    br, bc = A.block_shape
+    if br != bc or br not in (1, 3):
+        raise ValueError("cg_refined supports 1x1 and 3x3 blocks")
+    if A.row_counts is not None:
+        raise ValueError("cg_refined needs a compact matrix")
+    if b.dtype != torch.float64 or x.dtype != torch.float64:
+        raise TypeError("b and x must be float64")
+    n = A.nrow * br
+    if b.numel() != n or x.numel() != n:
+        raise ValueError(f"b and x must have {n} entries")
+    sparse._require_gpu(A.device, "cg_refined")
+    dev = A.device
+    lib = _native.lib()
+    stream = _native.stream_of(dev)
+    bf = b.reshape(-1)
+    xf = x.view(-1)
+    r64 = torch.empty(n, dtype=torch.float64, device=dev)
+    r32 = torch.empty(n, dtype=torch.float32, device=dev)
+    d32 = torch.zeros(n, dtype=torch.float32, device=dev)
+    rr = torch.zeros(1, dtype=torch.float64, device=dev)
+    ws_bytes = lib.warp_amd_bsr_residual_workspace_size(A.nrow)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    dot_bytes = lib.warp_amd_dot_workspace_size(n)
+    dot_ws = torch.zeros(dot_bytes, dtype=torch.uint8, device=dev)
+    _native.call("warp_amd_dot_f64", n, _native.ptr(bf), _native.ptr(bf), _native.ptr(rr), _native.ptr(dot_ws),
+                 dot_bytes, stream)
+    bnorm = math.sqrt(float(rr.item()))
+    Mop = linear.preconditioner(A, M) if M else None
+    inner = linear.CG(A, r32, d32, tol=inner_tol, atol=0.0, maxiter=inner_maxiter or n, M=Mop, check_every=check_every)
+
+    def residual() -> float:
+        _native.call(
+            "warp_amd_bsr_residual_f32_f64", A.nrow, br, _native.ptr(A.offsets), _native.ptr(A.columns),
+            _native.ptr(A.values), int(A.nnz), _native.ptr(xf), _native.ptr(bf), _native.ptr(r64), _native.ptr(r32),
+            _native.ptr(rr), _native.ptr(ws), ws_bytes, stream,
+        )
+        return math.sqrt(float(rr.item()))
+
+    total_inner = 0
+    history = []
+    rel = residual() / bnorm if bnorm > 0 else 0.0
+    history.append((0, 0, rel))
+    outer = 0
+    while rel > tol and outer < max_outer:
+        d32.zero_()
+        it, _, _ = inner(b=r32, x=d32)
+        total_inner += it
+        _native.call("warp_amd_axpy_f32_to_f64", n, 1.0, _native.ptr(d32), _native.ptr(xf), stream)
+        outer += 1
+        rel = residual() / bnorm
+        history.append((outer, total_inner, rel))
+        if callback is not None:
+            callback(outer, total_inner, rel)
+        if it == 0:
+            break  # the FP32 solve can no longer reduce the residual
+    return RefinementResult(outer, total_inner, rel, history)
