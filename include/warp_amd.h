@@ -250,6 +250,43 @@ int warp_amd_bsr_residual_f32_f64(
 /* x64 += alpha * (double) d32 */
 int warp_amd_axpy_f32_to_f64(int64_t n, double alpha, const float* d, double* x, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Learned preconditioner: 3D U-Net building blocks (no reference           */
+/* counterpart).  Fields on an n^3 grid are channels-last FP32 arrays       */
+/* [voxel][channel], voxel = x + n (y + n z); convolutions 3x3x3, stride 1, */
+/* zero padding, weights W[tap][ci][co], tap = (dz+1)*9 + (dy+1)*3 + dx+1.  */
+/* ------------------------------------------------------------------------ */
+
+/* y (+)= conv3(x) + b, ReLU when relu != 0 */
+int warp_amd_conv3_fwd_f32(int n, int cin, int cout, const float* x, const float* W, const float* b, float* y,
+                           int relu, int accumulate, void* stream);
+/* dx (+)= conv3^T(dy) */
+int warp_amd_conv3_bwd_data_f32(int n, int cin, int cout, const float* dy, const float* W, float* dx,
+                                int accumulate, void* stream);
+size_t warp_amd_conv3_bwd_weight_workspace_size(int n, int cin, int cout);
+/* dW = sum_v x[v + d] dy[v], db = sum_v dy[v]; chunked, fixed-order sums */
+int warp_amd_conv3_bwd_weight_f32(int n, int cin, int cout, const float* x, const float* dy, float* dW, float* db,
+                                  void* workspace, size_t workspace_bytes, void* stream);
+/* out = dy * (y > 0) */
+int warp_amd_relu_mask_f32(int64_t count, const float* y, const float* dy, float* out, void* stream);
+/* 2x2x2 average pooling (n even) and its adjoint */
+int warp_amd_pool2_f32(int n, int ch, const float* x, float* y, void* stream);
+int warp_amd_pool2_bwd_f32(int n, int ch, const float* dy, float* dx, int accumulate, void* stream);
+/* nearest 2x upsampling of an n^3 field plus an optional fine skip field, and its adjoint */
+int warp_amd_up2_add_f32(int n, int ch, const float* x, const float* skip, float* y, void* stream);
+int warp_amd_up2_bwd_f32(int n, int ch, const float* dy, float* dx, void* stream);
+/* dout = (out - target) / count; *loss = 0.5 mean((out - target)^2) (device) */
+size_t warp_amd_mse_workspace_size(int64_t count);
+int warp_amd_mse_loss_f32(int64_t count, const float* out, const float* target, float* dout, double* loss,
+                          void* workspace, size_t workspace_bytes, void* stream);
+/* Adam step; replaces adam_step_kernel_float (warp/_src/optim/adam.py:30-47) */
+int warp_amd_adam_step_f32(int64_t count, const float* g, float* m, float* v, float* param, float lr, float beta1,
+                           float beta2, float t, float eps, void* stream);
+/* network input (r * inv_scale, standardized log k) and output z = scale * out; scales on the device */
+int warp_amd_unet_pack_input(int64_t nvox, const double* r, const float* logk_std, const double* inv_scale,
+                             float* inp, void* stream);
+int warp_amd_unet_unpack_output(int64_t nvox, const float* out, const double* scale, double* z, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

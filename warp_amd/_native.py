@@ -58,6 +58,20 @@ _SIGNATURES = {
     "warp_amd_bsr_residual_workspace_size": (c_size, [c_int]),
     "warp_amd_bsr_residual_f32_f64": (c_int, [c_int, c_int, c_ptr, c_ptr, c_ptr, c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
     "warp_amd_axpy_f32_to_f64": (c_int, [c_i64, c_f64, c_ptr, c_ptr, c_ptr]),
+    "warp_amd_conv3_fwd_f32": (c_int, [c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_int, c_ptr]),
+    "warp_amd_conv3_bwd_data_f32": (c_int, [c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr]),
+    "warp_amd_conv3_bwd_weight_workspace_size": (c_size, [c_int, c_int, c_int]),
+    "warp_amd_conv3_bwd_weight_f32": (c_int, [c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
+    "warp_amd_relu_mask_f32": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr]),
+    "warp_amd_pool2_f32": (c_int, [c_int, c_int, c_ptr, c_ptr, c_ptr]),
+    "warp_amd_pool2_bwd_f32": (c_int, [c_int, c_int, c_ptr, c_ptr, c_int, c_ptr]),
+    "warp_amd_up2_add_f32": (c_int, [c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr]),
+    "warp_amd_up2_bwd_f32": (c_int, [c_int, c_int, c_ptr, c_ptr, c_ptr]),
+    "warp_amd_mse_workspace_size": (c_size, [c_i64]),
+    "warp_amd_mse_loss_f32": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
+    "warp_amd_adam_step_f32": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_f32, c_f32, c_f32, c_f32, c_f32, c_ptr]),
+    "warp_amd_unet_pack_input": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]),
+    "warp_amd_unet_unpack_output": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr]),
     "warp_amd_cg_workspace_size": (c_size, [c_i64, c_int]),
     "warp_amd_cg_state_offset": (c_size, [c_int]),
     "warp_amd_dot_workspace_size": (c_size, [c_i64]),
