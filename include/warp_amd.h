@@ -249,6 +249,23 @@ int warp_amd_bsr_residual_f32_f64(
     void* stream);
 /* x64 += alpha * (double) d32 */
 int warp_amd_axpy_f32_to_f64(int64_t n, double alpha, const float* d, double* x, void* stream);
+/* y64 = A x64 (FP32 values promoted, FP64 accumulation); *xy (device) = x.y.
+ * Workspace: warp_amd_bsr_residual_workspace_size(nrow). */
+int warp_amd_bsr_mv_f32_f64(
+    int nrow, int block_size, const int* offsets, const int* columns, const float* values, int64_t nnz_hint,
+    const double* x, double* y, double* xy, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Flexible CG pieces (Notay, SIAM J. Sci. Comput. 22(4), 2000; truncation 1),
+ * FP64 vectors, every scalar on the device so an iteration can be captured in
+ * a HIP graph.  scale = sqrt(*ss / n), inv_scale = 1 / scale. */
+int warp_amd_rms_scales(const double* ss, int64_t n, double* scale, double* inv_scale, void* stream);
+/* p = z - (*zq / *pq_prev) p_prev */
+int warp_amd_fcg_direction(int64_t n, const double* z, const double* p_prev, const double* zq, const double* pq_prev,
+                           double* p, void* stream);
+/* alpha = *pr / *pq; x += alpha p; r -= alpha q; *rr = r.r (device, fixed-order sum) */
+size_t warp_amd_fcg_update_workspace_size(int64_t n);
+int warp_amd_fcg_update(int64_t n, const double* pr, const double* pq, const double* p, const double* q, double* x,
+                        double* r, double* rr, void* workspace, size_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Learned preconditioner: 3D U-Net building blocks (no reference           */

@@ -58,6 +58,11 @@ _SIGNATURES = {
     "warp_amd_bsr_residual_workspace_size": (c_size, [c_int]),
     "warp_amd_bsr_residual_f32_f64": (c_int, [c_int, c_int, c_ptr, c_ptr, c_ptr, c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
     "warp_amd_axpy_f32_to_f64": (c_int, [c_i64, c_f64, c_ptr, c_ptr, c_ptr]),
+    "warp_amd_bsr_mv_f32_f64": (c_int, [c_int, c_int, c_ptr, c_ptr, c_ptr, c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
+    "warp_amd_rms_scales": (c_int, [c_ptr, c_i64, c_ptr, c_ptr, c_ptr]),
+    "warp_amd_fcg_direction": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]),
+    "warp_amd_fcg_update_workspace_size": (c_size, [c_i64]),
+    "warp_amd_fcg_update": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
     "warp_amd_conv3_fwd_f32": (c_int, [c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_int, c_ptr]),
     "warp_amd_conv3_bwd_data_f32": (c_int, [c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr]),
     "warp_amd_conv3_bwd_weight_workspace_size": (c_size, [c_int, c_int, c_int]),

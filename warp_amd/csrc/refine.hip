@@ -12,7 +12,8 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kFinalizeBlock = 1024;
 
-// r64 = b - A x (FP64 accumulate, FP32 values), r32 = (float) r64, partials of r.r.
+// With b: r64 = b - A x (FP64 accumulate, FP32 values), r32 = (float) r64
+// (when non-null), partials of r.r.  Without b: r64 = A x, partials of x.(A x).
 template <int BR, int G, int RPG>
 __global__ __launch_bounds__(kBlock) void residual_kernel(
     int nrow, const int* __restrict__ offsets, const int* __restrict__ columns, const float* __restrict__ values,
@@ -29,11 +30,58 @@ __global__ __launch_bounds__(kBlock) void residual_kernel(
             group_rows_product<double, BR, BR, G, RPG, float>(row0, nrow, lane, offsets, nullptr, columns, values, x);
         for (int e = lane; e < BR * RPG && row0 + e / BR < nrow; e += G) {
             const size_t i = (size_t)row0 * BR + e;
-            const double rv = b[i] - pick(ax, e);
-            r64[i] = rv;
-            r32[i] = (float)rv;
-            acc[0] += rv * rv;
+            if (b) {
+                const double rv = b[i] - pick(ax, e);
+                r64[i] = rv;
+                if (r32)
+                    r32[i] = (float)rv;
+                acc[0] += rv * rv;
+            } else {
+                const double yv = pick(ax, e);
+                r64[i] = yv;
+                acc[0] += x[i] * yv;
+            }
         }
+    }
+    write_block_partials<kBlock, 1>(acc, partials, lds);
+}
+
+// scale = sqrt(ss / n), inv_scale = 1 / scale (0 when scale is 0)
+__global__ void rms_scales_kernel(const double* ss, int64_t n, double* scale, double* inv_scale)
+{
+    const double s = sqrt(*ss / (double)n);
+    *scale = s;
+    *inv_scale = s > 0.0 ? 1.0 / s : 0.0;
+}
+
+// Flexible CG direction (Notay 2000, truncation 1): p = z - (zq / pq_prev) p_prev
+__global__ __launch_bounds__(kBlock) void fcg_direction_kernel(
+    int64_t n, const double* __restrict__ z, const double* __restrict__ p_prev, const double* zq,
+    const double* pq_prev, double* __restrict__ p
+)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    const double beta = *pq_prev != 0.0 ? *zq / *pq_prev : 0.0;
+    p[i] = fma(-beta, p_prev[i], z[i]);
+}
+
+// alpha = pr / pq; x += alpha p; r -= alpha q; partials of r.r
+__global__ __launch_bounds__(kBlock) void fcg_update_kernel(
+    int64_t n, const double* pr, const double* pq, const double* __restrict__ p, const double* __restrict__ q,
+    double* __restrict__ x, double* __restrict__ r, double* __restrict__ partials
+)
+{
+    __shared__ double lds[kBlock / kWave];
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const double alpha = *pq != 0.0 ? *pr / *pq : 0.0;
+    double acc[1] = {0.0};
+    if (i < n) {
+        x[i] = fma(alpha, p[i], x[i]);
+        const double rv = fma(-alpha, q[i], r[i]);
+        r[i] = rv;
+        acc[0] = rv * rv;
     }
     write_block_partials<kBlock, 1>(acc, partials, lds);
 }
@@ -116,6 +164,59 @@ WA_API int warp_amd_bsr_residual_f32_f64(
                                         wa::as_stream(stream));
     return wa::residual_dispatch<3>(g, nrow, offsets, columns, values, x, b, r64, r32, partials, rr,
                                     wa::as_stream(stream));
+}
+
+// y64 = A x64 (FP32 values promoted, FP64 accumulation) and *xy (device) = x.y
+WA_API int warp_amd_bsr_mv_f32_f64(
+    int nrow, int block_size, const int* offsets, const int* columns, const float* values, int64_t nnz_hint,
+    const double* x, double* y, double* xy, void* workspace, size_t workspace_bytes, void* stream
+)
+{
+    WA_REQUIRE(block_size == 1 || block_size == 3, "bsr_mv_f32_f64: block size must be 1 or 3, got %d", block_size);
+    WA_REQUIRE(nrow > 0, "bsr_mv_f32_f64: empty matrix");
+    WA_REQUIRE(workspace && workspace_bytes >= warp_amd_bsr_residual_workspace_size(nrow),
+               "bsr_mv_f32_f64: workspace too small");
+    const int g = wa::default_bsr_shape(nrow, block_size, block_size, nnz_hint).lanes;
+    double* partials = static_cast<double*>(workspace);
+    if (block_size == 1)
+        return wa::residual_dispatch<1>(g, nrow, offsets, columns, values, x, nullptr, y, nullptr, partials, xy,
+                                        wa::as_stream(stream));
+    return wa::residual_dispatch<3>(g, nrow, offsets, columns, values, x, nullptr, y, nullptr, partials, xy,
+                                    wa::as_stream(stream));
+}
+
+WA_API int warp_amd_rms_scales(const double* ss, int64_t n, double* scale, double* inv_scale, void* stream)
+{
+    wa::rms_scales_kernel<<<1, 1, 0, wa::as_stream(stream)>>>(ss, n, scale, inv_scale);
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}
+
+WA_API int warp_amd_fcg_direction(int64_t n, const double* z, const double* p_prev, const double* zq,
+                                  const double* pq_prev, double* p, void* stream)
+{
+    wa::fcg_direction_kernel<<<(unsigned)wa::ceil_div(n, wa::kBlock), wa::kBlock, 0, wa::as_stream(stream)>>>(
+        n, z, p_prev, zq, pq_prev, p
+    );
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}
+
+// Workspace: one double per 256 entries.
+WA_API size_t warp_amd_fcg_update_workspace_size(int64_t n) { return sizeof(double) * (size_t)(wa::ceil_div(n, 256) + 1); }
+
+WA_API int warp_amd_fcg_update(int64_t n, const double* pr, const double* pq, const double* p, const double* q,
+                               double* x, double* r, double* rr, void* workspace, size_t workspace_bytes, void* stream)
+{
+    WA_REQUIRE(workspace && workspace_bytes >= warp_amd_fcg_update_workspace_size(n), "fcg_update: workspace too small");
+    hipStream_t s = wa::as_stream(stream);
+    const unsigned grid = (unsigned)wa::ceil_div(n, wa::kBlock);
+    double* partials = static_cast<double*>(workspace);
+    wa::fcg_update_kernel<<<grid, wa::kBlock, 0, s>>>(n, pr, pq, p, q, x, r, partials);
+    WA_LAUNCH_CHECK();
+    wa::finalize_sum_kernel<<<1, wa::kFinalizeBlock, 0, s>>>(partials, (int)grid, rr);
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
 }
 
 // x64 += alpha * (double) d32

@@ -249,3 +249,243 @@ class UNet:
             if "adam.t" in f.files:
                 net.t = int(f["adam.t"][0])
         return net
+
+
+# ---------------------------------------------------------------------------
+# Operator family, training pairs, training
+# ---------------------------------------------------------------------------
+
+
+class FV7System:
+    """-div(k grad u) = f on an n^3 grid (7-point FV, k = exp(GRF)), FP32 matrix.
+
+    ``b`` is FP64; ``logk_std`` is the standardised log-coefficient fed to
+    the network.
+    """
+
+    def __init__(self, n: int, sigma: float, corr_len: float, seed: int, device):
+        from .. import problems, sparse
+        from .. import types as wt
+
+        dev = torch.device(device)
+        self.n = n
+        off, col, val, b, k = problems.fv7_problem(n, sigma, corr_len, seed, seed + 1_000_003, device=dev)
+        A = sparse.bsr_zeros(n**3, n**3, wt.float32, device=dev)
+        A.offsets, A.columns, A.values, A.nnz = off, col, val, col.numel()
+        self.A = A
+        self.b = b.to(torch.float64)
+        lk = torch.log(k).reshape(-1)
+        self.logk_std = ((lk - lk.mean()) / lk.std().clamp_min(1e-12)).to(torch.float32).contiguous()
+        self.device = dev
+
+    def residual(self, x: torch.Tensor, r: torch.Tensor, rr: torch.Tensor, ws: torch.Tensor) -> None:
+        A = self.A
+        _native.call(
+            "warp_amd_bsr_residual_f32_f64", A.nrow, 1, _native.ptr(A.offsets), _native.ptr(A.columns),
+            _native.ptr(A.values), int(A.nnz), _native.ptr(x), _native.ptr(self.b), _native.ptr(r), None,
+            _native.ptr(rr), _native.ptr(ws), ws.numel(), _native.stream_of(self.device),
+        )
+
+
+def generate_pairs(system: FV7System, count: int = 10, tol: float = 1e-10, maxiter: int = 20000):
+    """(input, target) pairs from one FP64 Jacobi-CG solve of ``system``.
+
+    Snapshots x_k are taken at ``count`` iterations spread geometrically over
+    the solve; with x* the converged solution, e_k = x* - x_k and
+    r_k = b - A x_k (FP64).  Input = (r_k / rms(r_k), standardised log k),
+    target = e_k / rms(e_k).  Returns a list of (inp [n^3*2] f32, target [n^3] f32).
+    """
+    from .. import sparse
+    from .. import types as wt
+    from . import linear
+
+    A32 = system.A
+    dev = system.device
+    N = A32.nrow
+    A64 = sparse.bsr_zeros(N, N, wt.float64, device=dev)
+    A64.offsets, A64.columns, A64.values, A64.nnz = A32.offsets, A32.columns, A32.values.to(torch.float64), A32.nnz
+    M = linear.preconditioner(A64, "diag")
+    x = torch.zeros(N, dtype=torch.float64, device=dev)
+    # first pass: iteration count of the full solve
+    it_total, _, _ = linear.cg(A64, system.b, x, tol=tol, atol=0.0, maxiter=maxiter, M=M, check_every=50)
+    targets_it = sorted({max(1, int(round(it_total ** (j / count)))) for j in range(1, count + 1)})
+    targets_it = [t for t in targets_it if t < it_total][:count]
+    x_star = x.clone()
+    snaps = {}
+
+    def cb(it, err, atol):
+        if it in targets_it and it not in snaps:
+            snaps[it] = xw.clone()
+
+    xw = torch.zeros(N, dtype=torch.float64, device=dev)
+    linear.cg(A64, system.b, xw, tol=tol, atol=0.0, maxiter=max(targets_it) + 1 if targets_it else 1, M=M,
+              check_every=1, callback=cb)
+    r = torch.empty(N, dtype=torch.float64, device=dev)
+    rr = torch.zeros(1, dtype=torch.float64, device=dev)
+    ws = torch.empty(_native.lib().warp_amd_bsr_residual_workspace_size(N), dtype=torch.uint8, device=dev)
+    pairs = []
+    for it in sorted(snaps):
+        xk = snaps[it]
+        system.residual(xk, r, rr, ws)
+        e = x_star - xk
+        rs = torch.sqrt(rr / N).item()
+        es = torch.sqrt(torch.mean(e * e)).item()
+        if rs <= 0 or es <= 0:
+            continue
+        inp = torch.stack([(r / rs).to(torch.float32), system.logk_std], dim=1).reshape(-1).contiguous()
+        pairs.append((inp, (e / es).to(torch.float32).contiguous()))
+    return pairs
+
+
+def train(net: UNet, pairs: list, epochs: int = 3, lr: float = 1e-3, lr_final: float = 1e-4, seed: int = 0,
+          log=None) -> list[float]:
+    """Adam on the per-pair MSE of the predicted (normalised) error; returns mean loss per epoch."""
+    rng = np.random.default_rng(seed)
+    history = []
+    steps = epochs * len(pairs)
+    step = 0
+    for ep in range(epochs):
+        order = rng.permutation(len(pairs))
+        losses = []
+        for i in order:
+            # cosine decay from lr to lr_final
+            cur = lr_final + 0.5 * (lr - lr_final) * (1 + math.cos(math.pi * step / max(steps - 1, 1)))
+            inp, tgt = pairs[i]
+            losses.append(net.train_step(inp, tgt, lr=cur))
+            step += 1
+        mean = float(torch.stack(losses).mean().item())
+        history.append(mean)
+        if log:
+            log(f"[train] epoch {ep}: mean loss {mean:.5f}")
+    return history
+
+
+# ---------------------------------------------------------------------------
+# Flexible CG with the learned preconditioner
+# ---------------------------------------------------------------------------
+
+
+class LearnedFCG:
+    """Flexible CG (Notay 2000, truncation 1) in FP64 with z = s U(r / s) (+ w D^-1 r).
+
+    Every scalar stays on the device; two iterations (ping-pong buffers) are
+    captured in one HIP graph and replayed, with a host check of ||r|| after
+    each replay.  On stagnation the solve falls back to Jacobi CG with FP64
+    refinement from the current iterate.
+    """
+
+    def __init__(self, system: FV7System, net: UNet, jacobi_weight: float = 0.0):
+        self.sys = system
+        self.net = net
+        self.w = float(jacobi_weight)
+        dev = system.device
+        N = system.A.nrow
+        self.N = N
+        f64 = dict(dtype=torch.float64, device=dev)
+        self.r = torch.zeros(N, **f64)
+        self.z = torch.zeros(N, **f64)
+        self.p = [torch.zeros(N, **f64), torch.zeros(N, **f64)]
+        self.q = [torch.zeros(N, **f64), torch.zeros(N, **f64)]
+        self.sc = torch.zeros(16, **f64)  # 0 rr, 1 scale, 2 inv, 3 zq, 4/5 pq[0/1], 6 pr
+        self.inp = torch.empty(N * 2, dtype=torch.float32, device=dev)
+        lib = _native.lib()
+        self.ws_mv = torch.empty(lib.warp_amd_bsr_residual_workspace_size(N), dtype=torch.uint8, device=dev)
+        self.ws_up = torch.empty(lib.warp_amd_fcg_update_workspace_size(N), dtype=torch.uint8, device=dev)
+        self.dot_bytes = lib.warp_amd_dot_workspace_size(N)
+        self.ws_dot = torch.zeros(self.dot_bytes, dtype=torch.uint8, device=dev)
+        if self.w != 0.0:
+            from . import linear
+
+            self.dinv = linear.preconditioner(system.A, "diag")._prec_data.to(torch.float64).reshape(-1).contiguous()
+        self._graph = None
+
+    def _s(self):
+        return _native.stream_of(self.sys.device)
+
+    def _sp(self, i):
+        return self.sc.data_ptr() + 8 * i
+
+    def _dot(self, a, b, slot):
+        _native.call("warp_amd_dot_f64", self.N, _native.ptr(a), _native.ptr(b), self._sp(slot), _native.ptr(self.ws_dot),
+                     self.dot_bytes, self._s())
+
+    def precondition(self) -> None:
+        """z = s U(r / s) (+ w D^-1 r), s = rms(r) from the current rr."""
+        s = self._s()
+        _native.call("warp_amd_rms_scales", self._sp(0), self.N, self._sp(1), self._sp(2), s)
+        _native.call("warp_amd_unet_pack_input", self.N, _native.ptr(self.r), _native.ptr(self.sys.logk_std),
+                     self._sp(2), _native.ptr(self.inp), s)
+        out = self.net.forward(self.inp)
+        _native.call("warp_amd_unet_unpack_output", self.N, _native.ptr(out), self._sp(1), _native.ptr(self.z), s)
+        if self.w != 0.0:
+            _native.call("warp_amd_diag_mv_f64", self.N, _native.ptr(self.dinv), _native.ptr(self.r), _native.ptr(self.z),
+                         _native.ptr(self.z), self.w, 1.0, s)
+
+    def iteration(self, cur: int) -> None:
+        prev = 1 - cur
+        A = self.sys.A
+        s = self._s()
+        self.precondition()
+        self._dot(self.z, self.q[prev], 3)
+        _native.call("warp_amd_fcg_direction", self.N, _native.ptr(self.z), _native.ptr(self.p[prev]), self._sp(3),
+                     self._sp(4 + prev), _native.ptr(self.p[cur]), s)
+        _native.call("warp_amd_bsr_mv_f32_f64", A.nrow, 1, _native.ptr(A.offsets), _native.ptr(A.columns),
+                     _native.ptr(A.values), int(A.nnz), _native.ptr(self.p[cur]), _native.ptr(self.q[cur]),
+                     self._sp(4 + cur), _native.ptr(self.ws_mv), self.ws_mv.numel(), s)
+        self._dot(self.p[cur], self.r, 6)
+        _native.call("warp_amd_fcg_update", self.N, self._sp(6), self._sp(4 + cur), _native.ptr(self.p[cur]),
+                     _native.ptr(self.q[cur]), _native.ptr(self.x), _native.ptr(self.r), self._sp(0),
+                     _native.ptr(self.ws_up), self.ws_up.numel(), s)
+
+    def solve(self, x: torch.Tensor, tol: float = 1e-8, maxiter: int = 500, use_graph: bool = True,
+              stagnation_window: int = 20, stagnation_factor: float = 0.5, fallback: bool = True):
+        """Solve A x = b from the initial guess x (FP64, updated in place).
+
+        Returns dict(iterations, relative_residual (recursive), history,
+        fallback_iterations)."""
+        self.x = x
+        self.sys.residual(x, self.r, self.sc[0:1], self.ws_mv)
+        bnorm = float(torch.linalg.vector_norm(self.sys.b).item())
+        for t in self.p + self.q:
+            t.zero_()
+        self.sc[4] = 1.0
+        self.sc[5] = 1.0
+        hist = [math.sqrt(float(self.sc[0].item())) / bnorm]
+        it = 0
+        if use_graph and self._graph is None:
+            # warm up both halves eagerly (allocates every buffer), then capture
+            saved = (x.clone(), self.r.clone(), self.sc.clone())
+            self.iteration(0)
+            self.iteration(1)
+            torch.cuda.synchronize()
+            x.copy_(saved[0]); self.r.copy_(saved[1]); self.sc.copy_(saved[2])
+            for t in self.p + self.q:
+                t.zero_()
+            g = torch.cuda.CUDAGraph()
+            cs = torch.cuda.Stream(self.sys.device)
+            cs.wait_stream(torch.cuda.current_stream(self.sys.device))
+            with torch.cuda.stream(cs):
+                with torch.cuda.graph(g, stream=cs):
+                    self.iteration(0)
+                    self.iteration(1)
+            torch.cuda.current_stream(self.sys.device).wait_stream(cs)
+            self._graph = g
+        while hist[-1] > tol and it < maxiter:
+            if use_graph:
+                self._graph.replay()
+            else:
+                self.iteration(0)
+                self.iteration(1)
+            it += 2
+            hist.append(math.sqrt(max(float(self.sc[0].item()), 0.0)) / bnorm)
+            w = stagnation_window // 2
+            if len(hist) > w and hist[-1] > stagnation_factor * hist[-1 - w]:
+                break
+        fb_iters = 0
+        if hist[-1] > tol and fallback:
+            from . import refine
+
+            res = refine.cg_refined(self.sys.A, self.sys.b, x, tol=tol, M="diag")
+            fb_iters = res.inner_iterations
+            hist.append(res.relative_residual)
+        return dict(iterations=it, relative_residual=hist[-1], history=hist, fallback_iterations=fb_iters)
