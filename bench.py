@@ -27,6 +27,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+COPY_BYTES = 1 << 30
+COPY_REPS = 5
 METRIC = "time to 1e-8 rel. residual: learned-FCG vs Jacobi-CG; bsr_mv HBM GB/s"
 
 
@@ -71,6 +73,20 @@ def time_cpu_spmv(offsets, columns, values, x, min_seconds: float = 10.0, max_re
     return el / reps, reps
 
 
+def copy_calibration(dev) -> int:
+    """A plain device-to-device copy of COPY_BYTES, repeated COPY_REPS times:
+    the counters of its kernel give the scale of FETCH_SIZE / WRITE_SIZE."""
+    import torch
+
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(COPY_REPS):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": COPY_BYTES, "copy_bytes_written": COPY_BYTES, "reps": COPY_REPS}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +95,10 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=64, help="elements per side of the elasticity mesh")
     ap.add_argument("--no-extras", action="store_true", help="skip CG / assembly / CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--eager", action="store_true",
+                    help="launch bsr_mv one call at a time instead of replaying a HIP graph (for counter runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of known size (calibrates FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -97,6 +117,8 @@ def main() -> int:
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.copy_calibration:
+        return copy_calibration(dev)
     n = args.n
     log(f"[bench] generating 3D Q1 elasticity {n}^3 on {torch.cuda.get_device_name(dev)}")
     rows, cols, vals, b, nn = problems.elasticity3d_q1(n, device=dev)
@@ -121,6 +143,21 @@ def main() -> int:
     for _ in range(max(args.warmup, 1)):
         sparse.bsr_mv(A, x, y)
     torch.cuda.synchronize()
+    if args.eager:
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record()
+        for _ in range(args.steps):
+            sparse.bsr_mv(A, x, y)
+        e1.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        nbytes = spmv_bytes(nn, nnzb, 3, 3)
+        ms_kernel = e0.elapsed_time(e1) / args.steps
+        log(f"[bench] eager bsr_mv 3x3: {ms_kernel * 1e3:.1f} us/call ({nbytes / (ms_kernel * 1e-3) / 1e9:.0f} GB/s), "
+            f"{args.steps} calls")
+        return 0
     cap_stream = torch.cuda.Stream(dev)
     graph = torch.cuda.CUDAGraph()
     with torch.cuda.stream(cap_stream):
