@@ -95,6 +95,8 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=64, help="elements per side of the elasticity mesh")
     ap.add_argument("--no-extras", action="store_true", help="skip CG / assembly / CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--fcg-seeds", type=int, default=10,
+                    help="held-out 128^3 systems for the learned-FCG vs Jacobi-CG comparison (0: skip)")
     ap.add_argument("--eager", action="store_true",
                     help="launch bsr_mv one call at a time instead of replaying a HIP graph (for counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -184,7 +186,7 @@ def main() -> int:
     # the graph's result equals an eager call
     y_ref = sparse.bsr_mv(A, x)
     torch.cuda.synchronize()
-    if not torch.equal(y_ref, y):
+    if not torch.equal(y_ref.reshape(-1), y.reshape(-1)):
         raise RuntimeError("graph-replayed bsr_mv differs from the eager call")
     del graph
     nbytes = spmv_bytes(nn, nnzb, 3, 3)
@@ -266,6 +268,33 @@ def main() -> int:
         result["assembly"] = {"triplets": T3, "blocks": u3, "ms": round(t_a * 1e3, 3),
                               "triplets_per_s": round(T3 / t_a), "alg_GBps": round(asm_bytes / t_a / 1e9, 1)}
         del r3, c3, v3, A3
+
+        # learned flexible CG vs Jacobi-CG on held-out 128^3 variable-coefficient Poisson systems
+        ckpt = os.path.join(ROOT, "warp_amd", "data", "fv7_unet_128.npz")
+        if args.fcg_seeds > 0 and os.path.exists(ckpt):
+            from warp_amd.optim import learned
+
+            net, meta = learned.load_checkpoint(ckpt, 128, dev)
+            recs = learned.evaluate(net, meta, range(5000, 5000 + args.fcg_seeds), dev, tol=1e-8, log=log)
+            med = lambda key: float(np.median([r[key] for r in recs]))
+            speed = [min(r["jacobi64_ms"], r["ir_ms"]) / r["fcg_ms"] for r in recs]
+            result["learned_fcg"] = {
+                "workload": "-div(k grad u)=f, 7-point FV 128^3, log-k GRF sigma 1.5, corr 8, held-out seeds 5000+",
+                "systems": len(recs),
+                "tol": 1e-8,
+                "median_speedup_vs_best_jacobi": round(float(np.median(speed)), 3),
+                "median_fcg_ms": round(med("fcg_ms"), 2),
+                "median_fcg_iterations": med("fcg_it"),
+                "fallbacks": sum(1 for r in recs if r["fcg_fallback_it"] > 0),
+                "median_jacobi64_ms": round(med("jacobi64_ms"), 2),
+                "median_jacobi64_iterations": med("jacobi64_it"),
+                "median_ir_ms": round(med("ir_ms"), 2),
+                "max_true_rel_residual": max(max(r["fcg_true"], r["jacobi64_true"], r["ir_true"]) for r in recs),
+                "median_graph_setup_ms": round(med("fcg_setup_ms"), 2),
+                "checkpoint": os.path.relpath(ckpt, ROOT),
+            }
+        else:
+            result["learned_fcg"] = {"skipped": "no checkpoint" if args.fcg_seeds > 0 else "--fcg-seeds 0"}
 
         # CPU baseline: the oracle's serial C bsr_mv on the same matrix
         offs = A.offsets.cpu().numpy()

@@ -304,6 +304,40 @@ int warp_amd_unet_pack_input(int64_t nvox, const double* r, const float* logk_st
                              float* inp, void* stream);
 int warp_amd_unet_unpack_output(int64_t nvox, const float* out, const double* scale, double* z, void* stream);
 
+/* Smoothing around the learned correction (no reference counterpart): one
+ * damped Jacobi sweep z_out = z + omega * dinv * (r - A z) for a 1x1 compact
+ * BSR matrix with FP32 values and FP64 vectors (z_out must not alias z), and
+ * z += mult * (*scale) * out for the network output. */
+int warp_amd_jacobi_sweep_f32_f64(int nrow, const int* offsets, const int* columns, const float* values,
+                                  int64_t nnz_hint, const double* z, const double* r, const double* dinv, double omega,
+                                  double* z_out, void* stream);
+int warp_amd_unet_unpack_add(int64_t n, const float* out, const double* scale, double mult, double* z, void* stream);
+/* Training loss of the learned preconditioner (no reference counterpart): the
+ * one-step energy reduction L = 1 - (r.z)^2 / ((r.e)(z.Az)) of a direction z
+ * for A e = r.  sc (device) = {r.z, z.Az, r.e}; writes g = dL/dz and *loss. */
+int warp_amd_energy_loss_grad(int64_t n, const double* r, const double* az, const double* sc, double* g, double* loss,
+                              void* stream);
+/* out32 = mult * (*scale) * x64 */
+int warp_amd_scale_to_f32(int64_t n, const double* x, const double* scale, double mult, float* out, void* stream);
+
+/* BF16 MFMA 3x3x3 convolution (no reference counterpart).  Activations are
+ * channels-last BF16 [m^3][c]; weights are FP32 masters W[27][cin][cout]
+ * swizzled once into BF16 MFMA operand tiles (transpose_flip = 1 builds the
+ * backward-data operator of a (cout -> cin) layer).  in_mode 1 adds the 2x
+ * nearest upsampling of `coarse` to x; flags: 1 ReLU, 2 also write the 2x2x2
+ * average pool, 4 FP32 output yf, 8 accumulate into yf.  Channel pairs:
+ * 2-16, 16-32, 32-64, 64-64, 64-32, 32-16, 16-1, 16-16, 1-16, 16-2. */
+size_t warp_amd_conv3_mfma_weight_size(int cin, int cout);
+int warp_amd_conv3_mfma_swizzle(int cin, int cout, int transpose_flip, const float* W, void* wsw, void* stream);
+int warp_amd_conv3_mfma(int m, int cin, int cout, int in_mode, const void* x, const void* coarse, const void* wsw,
+                        const float* bias, int flags, void* y, float* yf, void* pooled, void* stream);
+/* FP32 <-> BF16 (round to nearest even) */
+int warp_amd_f32_to_bf16(int64_t count, const float* in, void* out, void* stream);
+int warp_amd_bf16_to_f32(int64_t count, const void* in, float* out, void* stream);
+/* BF16 network input [nvox][2] = (r * inv_scale, standardized log k) */
+int warp_amd_unet_pack_input_bf16(int64_t nvox, const double* r, const float* logk_std, const double* inv_scale,
+                                  void* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

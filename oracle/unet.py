@@ -87,6 +87,13 @@ def up2_bwd(dy):
     return dy.reshape(n, 2, n, 2, n, 2, dy.shape[3]).sum(axis=(1, 3, 5))
 
 
+def bf16(a):
+    """Round to BF16 (via float32, round to nearest even); returns float64."""
+    u = np.ascontiguousarray(np.asarray(a, np.float64).astype(np.float32)).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
 def init_params(seed=0):
     """He-normal weights, zero biases (float64)."""
     rng = np.random.default_rng(seed)
@@ -115,6 +122,22 @@ def forward(p, inp):
     c["u0"] = up2(c["b1"]) + c["a0"]
     c["out"] = conv3_fwd(c["u0"], p["d0.W"], p["d0.b"], False)
     return c["out"], c
+
+
+def forward_bf16(p, inp):
+    """The inference network as the BF16 kernels evaluate it: input, weights
+    and every stored activation rounded to BF16 (pool of the stored values,
+    upsample + skip rounded once), sums in float64, FP32-free final output."""
+    W = {k: (bf16(v) if k.endswith(".W") else v) for k, v in p.items()}
+    conv = lambda x, name, relu: conv3_fwd(x, W[name + ".W"], W[name + ".b"], relu)
+    x = bf16(inp)
+    a0 = bf16(conv(x, "e0", True))
+    a1 = bf16(conv(bf16(pool2(a0)), "e1", True))
+    a2 = bf16(conv(bf16(pool2(a1)), "e2", True))
+    a3 = bf16(conv(bf16(pool2(a2)), "e3", True))
+    b2 = bf16(conv(bf16(up2(a3) + a2), "d2", True))
+    b1 = bf16(conv(bf16(up2(b2) + a1), "d1", True))
+    return conv(bf16(up2(b1) + a0), "d0", False)
 
 
 def backward(p, c, dout):
@@ -150,3 +173,40 @@ def adam_step(param, grad, m, v, lr, beta1, beta2, t, eps):
     mhat = m / (1.0 - beta1 ** (t + 1.0))
     vhat = v / (1.0 - beta2 ** (t + 1.0))
     param[...] = param - lr * mhat / (np.sqrt(vhat) + eps)
+
+
+# ---------------------------------------------------------------------------
+# Smoothing composite around the network and its one-step energy loss
+# ---------------------------------------------------------------------------
+
+
+def energy_loss(r, e, z, Az):
+    """L = 1 - (r.z)^2 / ((r.e)(z.Az)) and dL/dz (A symmetric, A e = r)."""
+    a, b, c = float(r @ z), float(z @ Az), float(r @ e)
+    L = 1.0 - a * a / (c * b)
+    g = (-2.0 * a / (c * b)) * r + (2.0 * a * a / (c * b * b)) * Az
+    return L, g
+
+
+def composite_forward(p, A, dinv, logk, r, nu, omega, gain, n):
+    """z = nu Jacobi sweeps from 0; r1 = r - A z; z += gain s U(r1 / s, logk); nu sweeps."""
+    z = np.zeros_like(r)
+    for _ in range(nu):
+        z = z + omega * dinv * (r - A @ z)
+    r1 = r - A @ z
+    s = float(np.sqrt(np.mean(r1 * r1)))
+    inp = np.stack([r1 / s, logk], axis=-1).reshape(n, n, n, 2)
+    out, cache = forward(p, inp)
+    z = z + gain * s * out.reshape(-1)
+    for _ in range(nu):
+        z = z + omega * dinv * (r - A @ z)
+    return z, cache, s
+
+
+def composite_grads(p, A, dinv, logk, r, e, nu, omega, gain, n):
+    """Loss and parameter gradients of the composite (adjoint of a sweep: g - A (omega D^-1 g))."""
+    z, cache, s = composite_forward(p, A, dinv, logk, r, nu, omega, gain, n)
+    L, g = energy_loss(r, e, z, A @ z)
+    for _ in range(nu):
+        g = g - A @ (omega * dinv * g)
+    return L, backward(p, cache, (gain * s * g).reshape(n, n, n, 1))

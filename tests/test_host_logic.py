@@ -113,3 +113,20 @@ def test_fv7_operator_properties():
     # strictly diagonally dominant on boundary rows, weakly inside: SPD
     assert np.linalg.eigvalsh(A.toarray()).min() > 0
     assert np.all(np.asarray(A.sum(axis=1)).ravel() >= -1e-5)
+
+
+def test_learned_checkpoint_roundtrip(tmp_path):
+    from warp_amd.optim import learned
+
+    net = learned.UNet(32, "cpu", seed=3)
+    net.m["e0.W"].fill_(0.25)
+    net.t = 7
+    meta = dict(n=32, sigma=1.5, corr=8.0, smooth_steps=1, omega=2.0 / 3.0)
+    path = str(tmp_path / "ckpt.npz")
+    learned.save_checkpoint(path, net, meta)
+    back, meta2 = learned.load_checkpoint(path, 32, "cpu")
+    assert set(back.params) == set(net.params)
+    for k in net.params:
+        assert torch.equal(back.params[k], net.params[k])
+    assert torch.equal(back.m["e0.W"], net.m["e0.W"]) and back.t == 7
+    assert meta2["smooth_steps"] == 1 and meta2["sigma"] == 1.5

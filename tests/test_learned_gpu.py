@@ -1,0 +1,106 @@
+"""Learned flexible CG: energy-loss kernel and trainer gradients against the
+numpy oracle (FP64), stagnation fallback, and the gain of a trained network
+over the same solver without it."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import scipy.sparse as sps
+import torch
+
+from oracle import unet as ounet
+from warp_amd import _native
+from warp_amd.optim import learned
+
+pytestmark = pytest.mark.gpu
+
+
+def _csr(system):
+    A = system.A
+    N = A.nrow
+    return sps.csr_matrix((A.values.double().cpu().numpy(), A.columns.cpu().numpy(), A.offsets.cpu().numpy()),
+                          shape=(N, N))
+
+
+def test_energy_loss_kernel_vs_oracle(device):
+    rng = np.random.default_rng(0)
+    n = 5000
+    r, az, e, z = (rng.standard_normal(n) for _ in range(4))
+    sc = np.array([r @ z, z @ az, r @ e])
+    L_ref, g_ref = ounet.energy_loss(r, e, z, az)
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=device)
+    g = torch.empty(n, dtype=torch.float64, device=device)
+    loss = torch.zeros(1, dtype=torch.float64, device=device)
+    _native.call("warp_amd_energy_loss_grad", n, _native.ptr(t(r)), _native.ptr(t(az)), _native.ptr(t(sc)),
+                 _native.ptr(g), _native.ptr(loss), _native.stream_of(device))
+    torch.cuda.synchronize()
+    assert float(loss.item()) == pytest.approx(L_ref, rel=1e-13)
+    np.testing.assert_allclose(g.cpu().numpy(), g_ref, rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("nu", [0, 1])
+def test_trainer_gradients_vs_oracle(device, nu):
+    n = 32
+    system = learned.FV7System(n, 1.5, 2.0, seed=11, device=device)
+    A = _csr(system)
+    rng = np.random.default_rng(3)
+    e = rng.standard_normal(n**3)
+    r = A @ e
+    p = {k: v.astype(np.float64) for k, v in learned.init_params(2).items()}
+    for k in p:
+        if k.endswith(".b"):
+            p[k] = rng.standard_normal(p[k].shape) * 0.05
+    net = learned.UNet(n, device, params={k: v.astype(np.float32) for k, v in p.items()})
+    trainer = learned.EnergyTrainer(net, smooth_steps=nu)
+    loss = trainer.step(system, torch.tensor(r, device=device), torch.tensor(e, device=device), lr=0.0)
+    torch.cuda.synchronize()
+    dinv = system.dinv.cpu().numpy()
+    lk = system.logk_std.double().cpu().numpy()
+    # the oracle sees the FP32 parameters and FP32-rounded logk the kernels see
+    p32 = {k: v.astype(np.float32).astype(np.float64) for k, v in p.items()}
+    L_ref, g_ref = ounet.composite_grads(p32, A, dinv, lk, r, e, nu, trainer.omega, 1.0 / system.dmean, n)
+    assert float(loss.item()) == pytest.approx(L_ref, rel=1e-4)
+    for k in p:
+        got = net.grads[k].double().cpu().numpy().reshape(g_ref[k].shape)
+        rel = np.linalg.norm(got - g_ref[k]) / max(np.linalg.norm(g_ref[k]), 1e-30)
+        assert rel < 2e-3, (k, rel)
+
+
+def test_zero_network_stagnates_then_falls_back(device):
+    """A network with zero weights gives z = the smoothing composite only;
+    with nu = 0 it gives z = 0: the stagnation detector hands over to the
+    Jacobi-CG fallback, which still reaches the FP64-checked tolerance."""
+    n = 32
+    system = learned.FV7System(n, 1.5, 2.0, seed=21, device=device)
+    p = {k: np.zeros_like(v) for k, v in learned.init_params(0).items()}
+    net = learned.UNet(n, device, params=p)
+    fcg = learned.LearnedFCG(system, net, smooth_steps=0)
+    x = torch.zeros(n**3, dtype=torch.float64, device=device)
+    out = fcg.solve(x, tol=1e-8, maxiter=200, stagnation_window=10)
+    assert out["iterations"] <= 12
+    assert out["fallback_iterations"] > 0
+    assert out["true_relative_residual"] < 2e-8
+
+
+def test_trained_network_beats_smoothing_alone(device):
+    """A short training run (energy loss on plain-CG pairs, then on the FCG's
+    own iterates) cuts the FCG iteration count well below the same composite
+    with the network output removed."""
+    n = 32
+    systems = [learned.FV7System(n, 1.5, 2.0, seed=100 + i, device=device) for i in range(6)]
+    net = learned.UNet(n, device, seed=0)
+    result = learned.train_learned_preconditioner(net, systems, smooth_steps=1, steps=600, dagger_rounds=1,
+                                                  dagger_steps=300, seed=0)
+    assert result["pcg_pairs"] > 0 and result["fcg_pairs"] > 0
+    test = learned.FV7System(n, 1.5, 2.0, seed=999, device=device)
+    x = torch.zeros(n**3, dtype=torch.float64, device=device)
+    trained = learned.LearnedFCG(test, net, smooth_steps=1).solve(x, tol=1e-8, maxiter=400, fallback=False,
+                                                                   stagnation_window=1000)
+    zero = learned.UNet(n, device, params={k: np.zeros_like(v) for k, v in learned.init_params(0).items()})
+    x.zero_()
+    smooth = learned.LearnedFCG(test, zero, smooth_steps=1).solve(x, tol=1e-8, maxiter=400, fallback=False,
+                                                                  stagnation_window=1000)
+    assert trained["true_relative_residual"] < 2e-8
+    assert smooth["true_relative_residual"] < 2e-8
+    assert trained["iterations"] * 2 <= smooth["iterations"], (trained["iterations"], smooth["iterations"])

@@ -1,4 +1,5 @@
-// Mixed-precision pieces for FP64 iterative refinement around the FP32 CG.
+// Mixed-precision pieces for FP64 iterative refinement around the FP32 CG,
+// and the vector kernels of the flexible CG with the learned preconditioner.
 //
 // No reference counterpart: the reference solves in one precision
 // (warp/_src/optim/linear.py).  Classical iterative refinement (e.g. Carson &
@@ -44,6 +45,64 @@ __global__ __launch_bounds__(kBlock) void residual_kernel(
         }
     }
     write_block_partials<kBlock, 1>(acc, partials, lds);
+}
+
+// One damped Jacobi sweep out of place: z_out = z + omega D^-1 (r - A z)
+// (FP32 matrix promoted, FP64 vectors).
+template <int BR, int G, int RPG>
+__global__ __launch_bounds__(kBlock) void jacobi_sweep_kernel(
+    int nrow, const int* __restrict__ offsets, const int* __restrict__ columns, const float* __restrict__ values,
+    const double* __restrict__ z, const double* __restrict__ r, const double* __restrict__ dinv, double omega,
+    double* __restrict__ z_out
+)
+{
+    const int row0 = (blockIdx.x * (kBlock / G) + threadIdx.x / G) * RPG;
+    const int lane = threadIdx.x % G;
+    if (row0 >= nrow)
+        return;
+    const RowAcc<double, BR * RPG> az =
+        group_rows_product<double, BR, BR, G, RPG, float>(row0, nrow, lane, offsets, nullptr, columns, values, z);
+    for (int e = lane; e < BR * RPG && row0 + e / BR < nrow; e += G) {
+        const size_t i = (size_t)row0 * BR + e;
+        z_out[i] = fma(omega * dinv[i], r[i] - pick(az, e), z[i]);
+    }
+}
+
+// z += mult * (*scale) * out
+__global__ __launch_bounds__(kBlock) void unpack_add_kernel(int64_t n, const float* __restrict__ out, const double* scale,
+                                                           double mult, double* __restrict__ z)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n)
+        z[i] = fma(mult * (*scale), (double)out[i], z[i]);
+}
+
+// One-step energy reduction of a search direction z for the residual r of
+// A e = r:  L = 1 - (r.z)^2 / ((r.e)(z.A z)), the fraction of the A-norm error
+// left after the exact line search along z.  With sc = {r.z, z.Az, r.e}:
+// g = dL/dz = -2 a / (c b) r + 2 a^2 / (c b^2) A z, and loss = L.
+__global__ __launch_bounds__(kBlock) void energy_grad_kernel(int64_t n, const double* __restrict__ r,
+                                                            const double* __restrict__ az, const double* sc,
+                                                            double* __restrict__ g, double* loss)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const double a = sc[0], b = sc[1], c = sc[2];
+    const double cb = c * b;
+    const double ca = cb != 0.0 ? -2.0 * a / cb : 0.0;
+    const double cz = (cb != 0.0 && b != 0.0) ? 2.0 * a * a / (cb * b) : 0.0;
+    if (i == 0)
+        *loss = cb != 0.0 ? 1.0 - a * a / cb : 1.0;
+    if (i < n)
+        g[i] = fma(ca, r[i], cz * az[i]);
+}
+
+// out32 = (float)(mult * (*scale) * x)
+__global__ __launch_bounds__(kBlock) void scale_to_f32_kernel(int64_t n, const double* __restrict__ x, const double* scale,
+                                                             double mult, float* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n)
+        out[i] = (float)(mult * (*scale) * x[i]);
 }
 
 // scale = sqrt(ss / n), inv_scale = 1 / scale (0 when scale is 0)
@@ -132,6 +191,17 @@ int residual_dispatch(
     case 32: return launch_residual<BR, 32>(nrow, offsets, columns, values, x, b, r64, r32, partials, rr, stream);
     default: return launch_residual<BR, 64>(nrow, offsets, columns, values, x, b, r64, r32, partials, rr, stream);
     }
+}
+
+template <int G>
+int launch_sweep(int nrow, const int* offsets, const int* columns, const float* values, const double* z,
+                 const double* r, const double* dinv, double omega, double* z_out, hipStream_t stream)
+{
+    constexpr int RPG = rows_per_group_for(1, 1);
+    const unsigned grid = (unsigned)ceil_div(nrow, (kBlock / G) * RPG);
+    jacobi_sweep_kernel<1, G, RPG><<<grid, kBlock, 0, stream>>>(nrow, offsets, columns, values, z, r, dinv, omega, z_out);
+    WA_LAUNCH_CHECK();
+    return kOk;
 }
 
 }  // namespace
@@ -226,6 +296,66 @@ WA_API int warp_amd_axpy_f32_to_f64(int64_t n, double alpha, const float* d, dou
         return wa::kOk;
     wa::axpy_f32_to_f64_kernel<<<(unsigned)wa::ceil_div(n, wa::kBlock), wa::kBlock, 0, wa::as_stream(stream)>>>(
         n, alpha, d, x
+    );
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}
+
+// One damped Jacobi sweep for a scalar (1x1) compact BSR matrix with FP32
+// values: z_out = z + omega * dinv * (r - A z); z_out must not alias z.
+WA_API int warp_amd_jacobi_sweep_f32_f64(int nrow, const int* offsets, const int* columns, const float* values,
+                                         int64_t nnz_hint, const double* z, const double* r, const double* dinv,
+                                         double omega, double* z_out, void* stream)
+{
+    WA_REQUIRE(nrow > 0, "jacobi_sweep: empty matrix");
+    WA_REQUIRE(z_out != z, "jacobi_sweep: z_out must not alias z");
+    const int g = wa::default_bsr_shape(nrow, 1, 1, nnz_hint).lanes;
+    hipStream_t s = wa::as_stream(stream);
+    switch (g) {
+    case 1: return wa::launch_sweep<1>(nrow, offsets, columns, values, z, r, dinv, omega, z_out, s);
+    case 2: return wa::launch_sweep<2>(nrow, offsets, columns, values, z, r, dinv, omega, z_out, s);
+    case 4: return wa::launch_sweep<4>(nrow, offsets, columns, values, z, r, dinv, omega, z_out, s);
+    case 8: return wa::launch_sweep<8>(nrow, offsets, columns, values, z, r, dinv, omega, z_out, s);
+    case 16: return wa::launch_sweep<16>(nrow, offsets, columns, values, z, r, dinv, omega, z_out, s);
+    case 32: return wa::launch_sweep<32>(nrow, offsets, columns, values, z, r, dinv, omega, z_out, s);
+    default: return wa::launch_sweep<64>(nrow, offsets, columns, values, z, r, dinv, omega, z_out, s);
+    }
+}
+
+// z += mult * (*scale) * out (network output back to the FP64 correction)
+WA_API int warp_amd_unet_unpack_add(int64_t n, const float* out, const double* scale, double mult, double* z,
+                                    void* stream)
+{
+    if (n <= 0)
+        return wa::kOk;
+    wa::unpack_add_kernel<<<(unsigned)wa::ceil_div(n, wa::kBlock), wa::kBlock, 0, wa::as_stream(stream)>>>(
+        n, out, scale, mult, z
+    );
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}
+
+// Loss and gradient of the one-step energy reduction (see energy_grad_kernel):
+// sc (device) = {r.z, z.Az, r.e}; g = dL/dz; *loss (device) = L.
+WA_API int warp_amd_energy_loss_grad(int64_t n, const double* r, const double* az, const double* sc, double* g,
+                                     double* loss, void* stream)
+{
+    WA_REQUIRE(n > 0, "energy_loss_grad: empty vectors");
+    wa::energy_grad_kernel<<<(unsigned)wa::ceil_div(n, wa::kBlock), wa::kBlock, 0, wa::as_stream(stream)>>>(
+        n, r, az, sc, g, loss
+    );
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}
+
+// out32 = mult * (*scale) * x64
+WA_API int warp_amd_scale_to_f32(int64_t n, const double* x, const double* scale, double mult, float* out,
+                                 void* stream)
+{
+    if (n <= 0)
+        return wa::kOk;
+    wa::scale_to_f32_kernel<<<(unsigned)wa::ceil_div(n, wa::kBlock), wa::kBlock, 0, wa::as_stream(stream)>>>(
+        n, x, scale, mult, out
     );
     WA_LAUNCH_CHECK();
     return wa::kOk;

@@ -65,6 +65,7 @@ class UNet:
         self.t = 0
         self._buf: dict[str, torch.Tensor] = {}
         self._lib = _native.lib()
+        self._wsw = None  # swizzled BF16 weights, rebuilt after every update
 
     # -- buffers -----------------------------------------------------------
     def side(self, level: int) -> int:
@@ -121,6 +122,65 @@ class UNet:
         self._conv("d1", u1, b1)
         self._up_add(1, 16, b1, a0, u0)
         self._conv("d0", u0, out)
+        return out
+
+    # -- BF16 MFMA inference ---------------------------------------------------
+    def sync_bf16(self) -> None:
+        """Refresh the swizzled BF16 weights from the FP32 masters (after training)."""
+        s = self._s()
+        self._wsw = {}
+        for name, ci, co, _, _ in LAYERS:
+            nbytes = self._lib.warp_amd_conv3_mfma_weight_size(ci, co)
+            t = torch.empty(nbytes, dtype=torch.uint8, device=self.dev)
+            _native.call("warp_amd_conv3_mfma_swizzle", ci, co, 0, _native.ptr(self.params[name + ".W"]),
+                         _native.ptr(t), s)
+            self._wsw[name] = t
+
+    def _hbuf(self, name: str, level: int, ch: int) -> torch.Tensor:
+        """BF16 activation buffer (stored as int16 bit patterns)."""
+        count = self.side(level) ** 3 * ch
+        key = "h:" + name
+        t = self._buf.get(key)
+        if t is None or t.numel() != count:
+            t = torch.empty(count, dtype=torch.int16, device=self.dev)
+            self._buf[key] = t
+        return t
+
+    def _mconv(self, name, x, y=None, pooled=None, coarse=None, yf=None):
+        ci, co, relu, lvl = LAYER_INDEX[name]
+        flags = (1 if relu else 0) | (2 if pooled is not None else 0) | (4 if yf is not None else 0)
+        _native.call(
+            "warp_amd_conv3_mfma", self.side(lvl), ci, co, 1 if coarse is not None else 0, _native.ptr(x),
+            _native.ptr(coarse) if coarse is not None else None, _native.ptr(self._wsw[name]),
+            _native.ptr(self.params[name + ".b"]), flags, _native.ptr(y) if y is not None else None,
+            _native.ptr(yf) if yf is not None else None, _native.ptr(pooled) if pooled is not None else None,
+            self._s(),
+        )
+
+    def forward_bf16(self, inp_bf16: torch.Tensor) -> torch.Tensor:
+        """inp_bf16: BF16 [n^3][2] (int16 bit patterns); returns the FP32 n^3 output.
+
+        Seven MFMA launches: pooling is fused into the encoder convolutions and
+        the upsample + skip add into the decoder convolutions' operand loads.
+        The grid side must be a multiple of 32 (coarsest level side >= 4)."""
+        if self.n % 32:
+            raise ValueError("the BF16 U-Net needs a grid side divisible by 32")
+        if getattr(self, "_wsw", None) is None:
+            self.sync_bf16()
+        H = self._hbuf
+        a0, p0 = H("a0", 0, 16), H("p0", 1, 16)
+        a1, p1 = H("a1", 1, 32), H("p1", 2, 32)
+        a2, p2 = H("a2", 2, 64), H("p2", 3, 64)
+        a3 = H("a3", 3, 64)
+        b2, b1 = H("b2", 2, 32), H("b1", 1, 16)
+        out = self.buf("out16", 0, 1)
+        self._mconv("e0", inp_bf16, y=a0, pooled=p0)
+        self._mconv("e1", p0, y=a1, pooled=p1)
+        self._mconv("e2", p1, y=a2, pooled=p2)
+        self._mconv("e3", p2, y=a3)
+        self._mconv("d2", a2, y=b2, coarse=a3)
+        self._mconv("d1", a1, y=b1, coarse=b2)
+        self._mconv("d0", a0, yf=out, coarse=b1)
         return out
 
     def _layer_grads(self, name, x, y, dy, masked):
@@ -214,6 +274,7 @@ class UNet:
         return loss, dout
 
     def adam_step(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8) -> None:
+        self._wsw = None
         for k, p in self.params.items():
             _native.call("warp_amd_adam_step_f32", p.numel(), _native.ptr(self.grads[k]), _native.ptr(self.m[k]),
                          _native.ptr(self.v[k]), _native.ptr(p), lr, beta1, beta2, float(self.t), eps, self._s())
@@ -240,7 +301,7 @@ class UNet:
     @classmethod
     def load(cls, path: str, n: int, device) -> "UNet":
         with np.load(path, allow_pickle=False) as f:
-            params = {k: f[k] for k in f.files if not k.startswith("adam.")}
+            params = {k: f[k] for k in f.files if k.split(".")[0] in LAYER_INDEX}
             net = cls(n, device, params=params)
             for k in net.params:
                 if "adam.m." + k in f.files:
@@ -277,6 +338,12 @@ class FV7System:
         lk = torch.log(k).reshape(-1)
         self.logk_std = ((lk - lk.mean()) / lk.std().clamp_min(1e-12)).to(torch.float32).contiguous()
         self.device = dev
+        from . import linear
+
+        # FP64 inverse diagonal (Jacobi) and the mean diagonal, which sets the
+        # scale of the network's correction
+        self.dinv = linear.preconditioner(A, "diag")._prec_data.to(torch.float64).reshape(-1).contiguous()
+        self.dmean = float(torch.mean(1.0 / self.dinv).item())
 
     def residual(self, x: torch.Tensor, r: torch.Tensor, rr: torch.Tensor, ws: torch.Tensor) -> None:
         A = self.A
@@ -360,43 +427,290 @@ def train(net: UNet, pairs: list, epochs: int = 3, lr: float = 1e-3, lr_final: f
     return history
 
 
+def generate_energy_pairs(system: FV7System, count: int = 10, tol: float = 1e-10, maxiter: int = 20000,
+                          return_solution: bool = False):
+    """(r, e) pairs with A e = r from one FP64 Jacobi-CG solve of ``system``.
+
+    Snapshots x_k at k = 0 and at ``count - 1`` iterations spread
+    geometrically over the solve; e_k = x* - x_k (x* the converged solution),
+    r_k = b - A x_k, both divided by rms(r_k) (the energy loss is invariant
+    to that common scale).  Returns a list of (r, e) FP32 n^3 tensors."""
+    from .. import sparse
+    from .. import types as wt
+    from . import linear
+
+    A32 = system.A
+    dev = system.device
+    N = A32.nrow
+    A64 = sparse.bsr_zeros(N, N, wt.float64, device=dev)
+    A64.offsets, A64.columns, A64.values, A64.nnz = A32.offsets, A32.columns, A32.values.to(torch.float64), A32.nnz
+    M = linear.preconditioner(A64, "diag")
+    x_star = torch.zeros(N, dtype=torch.float64, device=dev)
+    it_total, _, _ = linear.cg(A64, system.b, x_star, tol=tol, atol=0.0, maxiter=maxiter, M=M, check_every=50)
+    want = sorted({max(1, int(round(it_total ** (j / (count - 1))))) for j in range(1, count)})
+    want = [t for t in want if t < it_total]
+    snaps = {0: torch.zeros(N, dtype=torch.float64, device=dev)}
+    xw = torch.zeros(N, dtype=torch.float64, device=dev)
+
+    def cb(it, err, atol):
+        if it in want and it not in snaps:
+            snaps[it] = xw.clone()
+
+    if want:
+        linear.cg(A64, system.b, xw, tol=tol, atol=0.0, maxiter=max(want) + 1, M=M, check_every=1, callback=cb)
+    r = torch.empty(N, dtype=torch.float64, device=dev)
+    rr = torch.zeros(1, dtype=torch.float64, device=dev)
+    ws = torch.empty(_native.lib().warp_amd_bsr_residual_workspace_size(N), dtype=torch.uint8, device=dev)
+    pairs = []
+    for it in sorted(snaps):
+        system.residual(snaps[it], r, rr, ws)
+        rs = math.sqrt(float(rr.item()) / N)
+        if rs <= 0.0:
+            continue
+        pairs.append(((r / rs).to(torch.float32), ((x_star - snaps[it]) / rs).to(torch.float32)))
+    return (pairs, x_star) if return_solution else pairs
+
+
+class EnergyTrainer:
+    """Trains the U-Net inside the smoothing composite used by :class:`LearnedFCG`.
+
+    Loss: the one-step energy reduction 1 - (r.z)^2 / ((r.e)(z.Az)) of the
+    preconditioned direction z = M(r) -- the fraction of the A-norm error
+    that an exact line search along z leaves.  Every step runs on library
+    kernels: the composite forward (Jacobi sweeps, FP32 network), the loss
+    gradient, its adjoint back through the post-smoothing sweeps
+    (g <- g - omega A D^-1 g), the network backward and Adam.
+    """
+
+    def __init__(self, net: UNet, smooth_steps: int = 1, omega: float = 2.0 / 3.0):
+        self.net = net
+        self.nu = int(smooth_steps)
+        self.omega = float(omega)
+        dev = net.dev
+        N = net.n**3
+        self.N = N
+        f64 = dict(dtype=torch.float64, device=dev)
+        self.r = torch.zeros(N, **f64)
+        self.e = torch.zeros(N, **f64)
+        self.r1 = torch.zeros(N, **f64)
+        self.z = torch.zeros(N, **f64)
+        self.zt = torch.zeros(N, **f64)
+        self.y = torch.zeros(N, **f64)
+        self.g = torch.zeros(N, **f64)
+        self.gt = torch.zeros(N, **f64)
+        self.h = torch.zeros(N, **f64)
+        self.sc = torch.zeros(16, **f64)  # 1 scale, 2 inv, 7 r1.r1, 10 r.z, 11 z.Az, 12 r.e, 13 loss, 14 g.g
+        self.inp = torch.empty(N * 2, dtype=torch.float32, device=dev)
+        self.dout = torch.empty(N, dtype=torch.float32, device=dev)
+        lib = _native.lib()
+        self.ws_mv = torch.empty(lib.warp_amd_bsr_residual_workspace_size(N), dtype=torch.uint8, device=dev)
+        self.dot_bytes = lib.warp_amd_dot_workspace_size(N)
+        self.ws_dot = torch.zeros(self.dot_bytes, dtype=torch.uint8, device=dev)
+
+    def _sp(self, i):
+        return self.sc.data_ptr() + 8 * i
+
+    def _s(self):
+        return _native.stream_of(self.net.dev)
+
+    def _residual(self, system, b, x, out, slot):
+        A = system.A
+        _native.call("warp_amd_bsr_residual_f32_f64", A.nrow, 1, _native.ptr(A.offsets), _native.ptr(A.columns),
+                     _native.ptr(A.values), int(A.nnz), _native.ptr(x), _native.ptr(b), _native.ptr(out), None,
+                     self._sp(slot), _native.ptr(self.ws_mv), self.ws_mv.numel(), self._s())
+
+    def _dot(self, a, b, slot):
+        _native.call("warp_amd_dot_f64", self.N, _native.ptr(a), _native.ptr(b), self._sp(slot),
+                     _native.ptr(self.ws_dot), self.dot_bytes, self._s())
+
+    def forward(self, system: FV7System, r: torch.Tensor) -> torch.Tensor:
+        """z = M(r) with the FP32 network (activations kept for backward); returns z."""
+        s = self._s()
+        A = system.A
+        gain = 1.0 / system.dmean
+        if self.nu == 0:
+            self.z.zero_()
+            self.r1.copy_(r)
+            self._dot(r, r, 7)
+        else:
+            _native.call("warp_amd_diag_mv_f64", self.N, _native.ptr(system.dinv), _native.ptr(r), _native.ptr(self.z),
+                         _native.ptr(self.z), self.omega, 0.0, s)
+            for _ in range(self.nu - 1):
+                self._sweep(system, r, self.z, self.zt)
+                self.z, self.zt = self.zt, self.z
+            self._residual(system, r, self.z, self.r1, 7)
+        _native.call("warp_amd_rms_scales", self._sp(7), self.N, self._sp(1), self._sp(2), s)
+        _native.call("warp_amd_unet_pack_input", self.N, _native.ptr(self.r1), _native.ptr(system.logk_std),
+                     self._sp(2), _native.ptr(self.inp), s)
+        out = self.net.forward(self.inp)
+        _native.call("warp_amd_unet_unpack_add", self.N, _native.ptr(out), self._sp(1), gain, _native.ptr(self.z), s)
+        for _ in range(self.nu):
+            self._sweep(system, r, self.z, self.zt)
+            self.z, self.zt = self.zt, self.z
+        return self.z
+
+    def _sweep(self, system, r, z_in, z_out):
+        A = system.A
+        _native.call("warp_amd_jacobi_sweep_f32_f64", A.nrow, _native.ptr(A.offsets), _native.ptr(A.columns),
+                     _native.ptr(A.values), int(A.nnz), _native.ptr(z_in), _native.ptr(r), _native.ptr(system.dinv),
+                     self.omega, _native.ptr(z_out), self._s())
+
+    def step(self, system: FV7System, r32: torch.Tensor, e32: torch.Tensor, lr: float) -> torch.Tensor:
+        """One Adam step on the pair (r, e) of ``system``; returns the loss (device FP64 view)."""
+        s = self._s()
+        A = system.A
+        self.r.copy_(r32.reshape(-1))
+        self.e.copy_(e32.reshape(-1))
+        z = self.forward(system, self.r)
+        _native.call("warp_amd_bsr_mv_f32_f64", A.nrow, 1, _native.ptr(A.offsets), _native.ptr(A.columns),
+                     _native.ptr(A.values), int(A.nnz), _native.ptr(z), _native.ptr(self.y), self._sp(11),
+                     _native.ptr(self.ws_mv), self.ws_mv.numel(), s)
+        self._dot(self.r, z, 10)
+        self._dot(self.r, self.e, 12)
+        _native.call("warp_amd_energy_loss_grad", self.N, _native.ptr(self.r), _native.ptr(self.y), self._sp(10),
+                     _native.ptr(self.g), self._sp(13), s)
+        # adjoint of each post-smoothing sweep z <- z + omega D^-1 (r - A z): g <- g - A (omega D^-1 g)
+        for _ in range(self.nu):
+            _native.call("warp_amd_diag_mv_f64", self.N, _native.ptr(system.dinv), _native.ptr(self.g),
+                         _native.ptr(self.h), _native.ptr(self.h), self.omega, 0.0, s)
+            self._residual(system, self.g, self.h, self.gt, 14)
+            self.g, self.gt = self.gt, self.g
+        _native.call("warp_amd_scale_to_f32", self.N, _native.ptr(self.g), self._sp(1), 1.0 / system.dmean,
+                     _native.ptr(self.dout), s)
+        self.net.backward(self.dout)
+        self.net.adam_step(lr)
+        return self.sc[13:14]
+
+    def loss(self, system: FV7System, r32: torch.Tensor, e32: torch.Tensor) -> float:
+        """Loss of one pair without a training step."""
+        A = system.A
+        self.r.copy_(r32.reshape(-1))
+        self.e.copy_(e32.reshape(-1))
+        z = self.forward(system, self.r)
+        _native.call("warp_amd_bsr_mv_f32_f64", A.nrow, 1, _native.ptr(A.offsets), _native.ptr(A.columns),
+                     _native.ptr(A.values), int(A.nnz), _native.ptr(z), _native.ptr(self.y), self._sp(11),
+                     _native.ptr(self.ws_mv), self.ws_mv.numel(), self._s())
+        self._dot(self.r, z, 10)
+        self._dot(self.r, self.e, 12)
+        _native.call("warp_amd_energy_loss_grad", self.N, _native.ptr(self.r), _native.ptr(self.y), self._sp(10),
+                     _native.ptr(self.g), self._sp(13), self._s())
+        return float(self.sc[13].item())
+
+    def train(self, systems: list, pairs: list, steps: int, lr: float = 1e-3, lr_final: float = 1e-5, seed: int = 0,
+              log=None, log_every: int = 250) -> list[float]:
+        """Adam with cosine decay over ``steps`` steps on random pairs.
+
+        pairs: list of (system index, r32, e32).  Returns the running loss at
+        each log point."""
+        rng = np.random.default_rng(seed)
+        hist = []
+        acc = []
+        for step in range(steps):
+            i = int(rng.integers(len(pairs)))
+            si, r32, e32 = pairs[i]
+            cur = lr_final + 0.5 * (lr - lr_final) * (1 + math.cos(math.pi * step / max(steps - 1, 1)))
+            acc.append(self.step(systems[si], r32, e32, cur))
+            if (step + 1) % log_every == 0 or step == steps - 1:
+                mean = float(torch.cat(acc).mean().item())
+                acc = []
+                hist.append(mean)
+                if log:
+                    log(f"[train] step {step + 1}/{steps}: mean loss {mean:.4f}")
+        return hist
+
+
+def train_learned_preconditioner(net: UNet, systems: list, smooth_steps: int = 1, omega: float = 2.0 / 3.0,
+                                 pairs_per_solve: int = 10, steps: int = 2000, lr: float = 1e-3,
+                                 dagger_rounds: int = 2, dagger_steps: int = 1000, dagger_lr: float = 5e-4,
+                                 dagger_iters: int = 40, dagger_every: int = 8, seed: int = 0, log=None) -> dict:
+    """Train ``net`` as the preconditioner of :class:`LearnedFCG` on ``systems``.
+
+    1. (r, e) pairs from one FP64 Jacobi-CG solve per system
+       (:func:`generate_energy_pairs`), ``steps`` Adam steps on the energy loss.
+    2. ``dagger_rounds`` times: run the learned FCG on every training system,
+       add (r_k, x* - x_k) of its own iterates (every ``dagger_every``-th of
+       the first ``dagger_iters``) to the pairs, train ``dagger_steps`` more.
+       Residuals met inside FCG differ from plain-CG residuals; without this
+       step the trained network stalls FCG after a few iterations.
+    Returns dict(pcg_pairs, fcg_pairs, loss_history, seconds)."""
+    t0 = time.time()
+    trainer = EnergyTrainer(net, smooth_steps, omega)
+    pairs = []
+    x_stars = []
+    for si, sysm in enumerate(systems):
+        ps, x_star = generate_energy_pairs(sysm, count=pairs_per_solve, return_solution=True)
+        pairs += [(si, r, e) for r, e in ps]
+        x_stars.append(x_star)
+    n_pcg = len(pairs)
+    if log:
+        log(f"[train] {n_pcg} plain-CG pairs from {len(systems)} systems in {time.time() - t0:.1f}s")
+    hist = trainer.train(systems, pairs, steps, lr=lr, seed=seed, log=log)
+    n_fcg = 0
+    for rd in range(dagger_rounds):
+        new = []
+        for si, sysm in enumerate(systems):
+            fcg = LearnedFCG(sysm, net, smooth_steps=smooth_steps, omega=omega)
+            new += [(si, r, e) for r, e in fcg.collect_pairs(x_stars[si], dagger_iters, dagger_every)]
+        pairs += new
+        n_fcg += len(new)
+        if log:
+            log(f"[train] round {rd}: +{len(new)} FCG pairs ({len(pairs)} in total)")
+        hist += trainer.train(systems, pairs, dagger_steps, lr=dagger_lr, seed=seed + 1 + rd, log=log)
+    return dict(pcg_pairs=n_pcg, fcg_pairs=n_fcg, loss_history=hist, seconds=time.time() - t0)
+
+
 # ---------------------------------------------------------------------------
 # Flexible CG with the learned preconditioner
 # ---------------------------------------------------------------------------
 
 
 class LearnedFCG:
-    """Flexible CG (Notay 2000, truncation 1) in FP64 with z = s U(r / s) (+ w D^-1 r).
+    """Flexible CG (Notay 2000, truncation 1) in FP64 with a learned preconditioner.
 
-    Every scalar stays on the device; two iterations (ping-pong buffers) are
-    captured in one HIP graph and replayed, with a host check of ||r|| after
-    each replay.  On stagnation the solve falls back to Jacobi CG with FP64
-    refinement from the current iterate.
+    Preconditioner (all vectors FP64, matrix FP32):
+      z  = nu damped Jacobi sweeps on A z = r from z = 0
+      r1 = r - A z,  s = rms(r1)
+      z += gain * s * U(r1 / s, log k)        (U: the BF16 MFMA U-Net; gain = 1 / mean diag)
+      z  = nu more sweeps from that z
+    With nu = 0 it is z = gain * s * U(r / s).  Every scalar stays on the
+    device; two iterations (ping-pong buffers) are captured in one HIP graph
+    and replayed, with a host check of ||r|| after each replay.  On
+    stagnation the solve falls back to block-Jacobi CG with FP64 refinement
+    from the current iterate.
     """
 
-    def __init__(self, system: FV7System, net: UNet, jacobi_weight: float = 0.0):
+    def __init__(self, system: FV7System, net: UNet, smooth_steps: int = 1, omega: float = 2.0 / 3.0,
+                 gain: float | None = None, precision: str = "bf16"):
+        if precision not in ("bf16", "fp32"):
+            raise ValueError("precision must be 'bf16' or 'fp32'")
         self.sys = system
         self.net = net
-        self.w = float(jacobi_weight)
+        self.nu = int(smooth_steps)
+        self.omega = float(omega)
+        self.gain = 1.0 / system.dmean if gain is None else float(gain)
+        self.precision = precision
         dev = system.device
         N = system.A.nrow
         self.N = N
         f64 = dict(dtype=torch.float64, device=dev)
         self.r = torch.zeros(N, **f64)
+        self.r1 = torch.zeros(N, **f64)
         self.z = torch.zeros(N, **f64)
+        self.zt = torch.zeros(N, **f64)
         self.p = [torch.zeros(N, **f64), torch.zeros(N, **f64)]
         self.q = [torch.zeros(N, **f64), torch.zeros(N, **f64)]
-        self.sc = torch.zeros(16, **f64)  # 0 rr, 1 scale, 2 inv, 3 zq, 4/5 pq[0/1], 6 pr
-        self.inp = torch.empty(N * 2, dtype=torch.float32, device=dev)
+        # 0 rr, 1 scale, 2 inv, 3 zq, 4/5 pq[0/1], 6 pr, 7 r1.r1
+        self.sc = torch.zeros(16, **f64)
+        if precision == "bf16":
+            self.inp = torch.empty(N * 2, dtype=torch.int16, device=dev)
+        else:
+            self.inp = torch.empty(N * 2, dtype=torch.float32, device=dev)
         lib = _native.lib()
         self.ws_mv = torch.empty(lib.warp_amd_bsr_residual_workspace_size(N), dtype=torch.uint8, device=dev)
         self.ws_up = torch.empty(lib.warp_amd_fcg_update_workspace_size(N), dtype=torch.uint8, device=dev)
         self.dot_bytes = lib.warp_amd_dot_workspace_size(N)
         self.ws_dot = torch.zeros(self.dot_bytes, dtype=torch.uint8, device=dev)
-        if self.w != 0.0:
-            from . import linear
-
-            self.dinv = linear.preconditioner(system.A, "diag")._prec_data.to(torch.float64).reshape(-1).contiguous()
+        self.dinv = system.dinv
         self._graph = None
 
     def _s(self):
@@ -409,17 +723,46 @@ class LearnedFCG:
         _native.call("warp_amd_dot_f64", self.N, _native.ptr(a), _native.ptr(b), self._sp(slot), _native.ptr(self.ws_dot),
                      self.dot_bytes, self._s())
 
-    def precondition(self) -> None:
-        """z = s U(r / s) (+ w D^-1 r), s = rms(r) from the current rr."""
+    def _sweep(self, z_in, z_out):
+        A = self.sys.A
+        _native.call("warp_amd_jacobi_sweep_f32_f64", A.nrow, _native.ptr(A.offsets), _native.ptr(A.columns),
+                     _native.ptr(A.values), int(A.nnz), _native.ptr(z_in), _native.ptr(self.r), _native.ptr(self.dinv),
+                     self.omega, _native.ptr(z_out), self._s())
+
+    def _net(self, src, ss_slot) -> torch.Tensor:
+        """Network output for the residual `src` whose squared norm is in slot ss_slot."""
         s = self._s()
-        _native.call("warp_amd_rms_scales", self._sp(0), self.N, self._sp(1), self._sp(2), s)
-        _native.call("warp_amd_unet_pack_input", self.N, _native.ptr(self.r), _native.ptr(self.sys.logk_std),
+        _native.call("warp_amd_rms_scales", self._sp(ss_slot), self.N, self._sp(1), self._sp(2), s)
+        if self.precision == "bf16":
+            _native.call("warp_amd_unet_pack_input_bf16", self.N, _native.ptr(src), _native.ptr(self.sys.logk_std),
+                         self._sp(2), _native.ptr(self.inp), s)
+            return self.net.forward_bf16(self.inp)
+        _native.call("warp_amd_unet_pack_input", self.N, _native.ptr(src), _native.ptr(self.sys.logk_std),
                      self._sp(2), _native.ptr(self.inp), s)
-        out = self.net.forward(self.inp)
-        _native.call("warp_amd_unet_unpack_output", self.N, _native.ptr(out), self._sp(1), _native.ptr(self.z), s)
-        if self.w != 0.0:
-            _native.call("warp_amd_diag_mv_f64", self.N, _native.ptr(self.dinv), _native.ptr(self.r), _native.ptr(self.z),
-                         _native.ptr(self.z), self.w, 1.0, s)
+        return self.net.forward(self.inp)
+
+    def precondition(self) -> None:
+        """z = M(r) (see the class docstring); uses self.r and slot 0 (r.r)."""
+        s = self._s()
+        A = self.sys.A
+        if self.nu == 0:
+            self.z.zero_()
+            out = self._net(self.r, 0)
+        else:
+            # first sweep from zero: z = omega D^-1 r
+            _native.call("warp_amd_diag_mv_f64", self.N, _native.ptr(self.dinv), _native.ptr(self.r),
+                         _native.ptr(self.z), _native.ptr(self.z), self.omega, 0.0, s)
+            for _ in range(self.nu - 1):
+                self._sweep(self.z, self.zt)
+                self.z, self.zt = self.zt, self.z
+            _native.call("warp_amd_bsr_residual_f32_f64", A.nrow, 1, _native.ptr(A.offsets), _native.ptr(A.columns),
+                         _native.ptr(A.values), int(A.nnz), _native.ptr(self.z), _native.ptr(self.r),
+                         _native.ptr(self.r1), None, self._sp(7), _native.ptr(self.ws_mv), self.ws_mv.numel(), s)
+            out = self._net(self.r1, 7)
+        _native.call("warp_amd_unet_unpack_add", self.N, _native.ptr(out), self._sp(1), self.gain, _native.ptr(self.z), s)
+        for _ in range(self.nu):
+            self._sweep(self.z, self.zt)
+            self.z, self.zt = self.zt, self.z
 
     def iteration(self, cur: int) -> None:
         prev = 1 - cur
@@ -437,28 +780,53 @@ class LearnedFCG:
                      _native.ptr(self.q[cur]), _native.ptr(self.x), _native.ptr(self.r), self._sp(0),
                      _native.ptr(self.ws_up), self.ws_up.numel(), s)
 
+    def _start(self, x0: torch.Tensor | None = None) -> float:
+        """Initial state (x = x0 or 0, r = b - A x, fresh directions); returns ||b||."""
+        if getattr(self, "x", None) is None:
+            self.x = torch.empty_like(self.r)  # a captured graph keeps this address
+        if x0 is None:
+            self.x.zero_()
+        else:
+            self.x.copy_(x0)
+        self.sys.residual(self.x, self.r, self.sc[0:1], self.ws_mv)
+        for t in self.p + self.q:
+            t.zero_()
+        self.sc[4] = 1.0
+        self.sc[5] = 1.0
+        return float(torch.linalg.vector_norm(self.sys.b).item())
+
+    def collect_pairs(self, x_star: torch.Tensor, iterations: int = 40, every: int = 8) -> list:
+        """(r_k, x* - x_k) of this solver's own iterates from x = 0, every
+        ``every``-th of the first ``iterations``, both divided by rms(r_k);
+        FP32 n^3 tensors (eager, no graph)."""
+        self._start()
+        out = []
+        N = self.N
+        for k in range(1, iterations + 1):
+            self.iteration((k - 1) % 2)
+            if k % every == 0:
+                rs = math.sqrt(max(float(self.sc[0].item()), 0.0) / N)
+                if rs <= 0.0:
+                    break
+                out.append(((self.r / rs).to(torch.float32), ((x_star - self.x) / rs).to(torch.float32)))
+        return out
+
     def solve(self, x: torch.Tensor, tol: float = 1e-8, maxiter: int = 500, use_graph: bool = True,
               stagnation_window: int = 20, stagnation_factor: float = 0.5, fallback: bool = True):
         """Solve A x = b from the initial guess x (FP64, updated in place).
 
         Returns dict(iterations, relative_residual (recursive), history,
-        fallback_iterations)."""
-        self.x = x
-        self.sys.residual(x, self.r, self.sc[0:1], self.ws_mv)
-        bnorm = float(torch.linalg.vector_norm(self.sys.b).item())
-        for t in self.p + self.q:
-            t.zero_()
-        self.sc[4] = 1.0
-        self.sc[5] = 1.0
+        fallback_iterations, true_relative_residual (FP64 b - A x))."""
+        bnorm = self._start(x)
         hist = [math.sqrt(float(self.sc[0].item())) / bnorm]
         it = 0
         if use_graph and self._graph is None:
             # warm up both halves eagerly (allocates every buffer), then capture
-            saved = (x.clone(), self.r.clone(), self.sc.clone())
+            saved = (self.x.clone(), self.r.clone(), self.sc.clone())
             self.iteration(0)
             self.iteration(1)
             torch.cuda.synchronize()
-            x.copy_(saved[0]); self.r.copy_(saved[1]); self.sc.copy_(saved[2])
+            self.x.copy_(saved[0]); self.r.copy_(saved[1]); self.sc.copy_(saved[2])
             for t in self.p + self.q:
                 t.zero_()
             g = torch.cuda.CUDAGraph()
@@ -481,6 +849,7 @@ class LearnedFCG:
             w = stagnation_window // 2
             if len(hist) > w and hist[-1] > stagnation_factor * hist[-1 - w]:
                 break
+        x.copy_(self.x)
         fb_iters = 0
         if hist[-1] > tol and fallback:
             from . import refine
@@ -488,4 +857,154 @@ class LearnedFCG:
             res = refine.cg_refined(self.sys.A, self.sys.b, x, tol=tol, M="diag")
             fb_iters = res.inner_iterations
             hist.append(res.relative_residual)
-        return dict(iterations=it, relative_residual=hist[-1], history=hist, fallback_iterations=fb_iters)
+        self.sys.residual(x, self.r1, self.sc[8:9], self.ws_mv)
+        true_rel = math.sqrt(max(float(self.sc[8].item()), 0.0)) / bnorm
+        return dict(iterations=it, relative_residual=hist[-1], history=hist, fallback_iterations=fb_iters,
+                    true_relative_residual=true_rel)
+
+
+# ---------------------------------------------------------------------------
+# Checkpoints and command line
+# ---------------------------------------------------------------------------
+
+
+def save_checkpoint(path: str, net: UNet, meta: dict) -> None:
+    """Weights, Adam state (m, v, t) and the training recipe in one .npz."""
+    d = net.state_dict()
+    for k, v in meta.items():
+        d["meta." + k] = np.asarray(v)
+    np.savez(path, **d)
+
+
+def load_checkpoint(path: str, n: int, device) -> tuple[UNet, dict]:
+    net = UNet.load(path, n, device)
+    with np.load(path, allow_pickle=False) as f:
+        meta = {k[5:]: f[k].item() if f[k].ndim == 0 else f[k] for k in f.files if k.startswith("meta.")}
+    return net, meta
+
+
+def evaluate(net: UNet, meta: dict, seeds, device, tol: float = 1e-8, log=None) -> list[dict]:
+    """Learned FCG vs FP64 Jacobi-CG and FP32 Jacobi-CG + FP64 refinement on
+    held-out systems; every solution checked by the FP64 true residual."""
+    from .. import sparse
+    from .. import types as wt
+    from . import linear, refine
+
+    n = net.n
+    res = []
+    for seed in seeds:
+        sysm = FV7System(n, float(meta["sigma"]), float(meta["corr"]), int(seed), device)
+        N = n**3
+        A64 = sparse.bsr_zeros(N, N, wt.float64, device=device)
+        A64.offsets, A64.columns, A64.values, A64.nnz = sysm.A.offsets, sysm.A.columns, sysm.A.values.double(), sysm.A.nnz
+        M64 = linear.preconditioner(A64, "diag")
+        x = torch.zeros(N, dtype=torch.float64, device=device)
+        linear.cg(A64, sysm.b, x, tol=tol, atol=0.0, maxiter=10, M=M64)  # kernels warm
+        x.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        it64, _, _ = linear.cg(A64, sysm.b, x, tol=tol, atol=0.0, maxiter=100000, M=M64)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t0
+        r = torch.empty(N, dtype=torch.float64, device=device)
+        rr = torch.zeros(1, dtype=torch.float64, device=device)
+        ws = torch.empty(_native.lib().warp_amd_bsr_residual_workspace_size(N), dtype=torch.uint8, device=device)
+        bn = float(torch.linalg.vector_norm(sysm.b).item())
+        sysm.residual(x, r, rr, ws)
+        true64 = math.sqrt(float(rr.item())) / bn
+        x.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ir = refine.cg_refined(sysm.A, sysm.b, x, tol=tol, inner_tol=1e-4, M="diag")
+        torch.cuda.synchronize()
+        tir = time.perf_counter() - t0
+        sysm.residual(x, r, rr, ws)
+        true_ir = math.sqrt(float(rr.item())) / bn
+        fcg = LearnedFCG(sysm, net, smooth_steps=int(meta["smooth_steps"]), omega=float(meta["omega"]))
+        x.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fcg.solve(x, tol=tol, maxiter=2)  # graph capture (setup)
+        torch.cuda.synchronize()
+        t_setup = time.perf_counter() - t0
+        x.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fcg.solve(x, tol=tol, maxiter=2000)
+        torch.cuda.synchronize()
+        tf = time.perf_counter() - t0
+        rec = dict(seed=int(seed), jacobi64_it=it64, jacobi64_ms=t64 * 1e3, jacobi64_true=true64,
+                   ir_inner_it=ir.inner_iterations, ir_ms=tir * 1e3, ir_true=true_ir,
+                   fcg_it=out["iterations"], fcg_fallback_it=out["fallback_iterations"], fcg_ms=tf * 1e3,
+                   fcg_setup_ms=t_setup * 1e3, fcg_true=out["true_relative_residual"])
+        res.append(rec)
+        if log:
+            log(f"[eval] seed {seed}: Jacobi-CG fp64 {it64} it {t64 * 1e3:.1f} ms ({true64:.1e}) | IR {ir.inner_iterations} it "
+                f"{tir * 1e3:.1f} ms ({true_ir:.1e}) | FCG {out['iterations']}+{out['fallback_iterations']} it "
+                f"{tf * 1e3:.1f} ms ({out['true_relative_residual']:.1e}) | speedup {min(t64, tir) / tf:.2f}x")
+    return res
+
+
+def main(argv=None) -> int:
+    import argparse
+
+    ap = argparse.ArgumentParser(prog="python -m warp_amd.optim.learned")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train", help="train the U-Net preconditioner on the GPU and write a checkpoint")
+    t.add_argument("--n", type=int, default=128)
+    t.add_argument("--sigma", type=float, default=1.5)
+    t.add_argument("--corr", type=float, default=8.0)
+    t.add_argument("--train-seeds", type=int, default=200)
+    t.add_argument("--seed0", type=int, default=1000)
+    t.add_argument("--pairs-per-solve", type=int, default=10)
+    t.add_argument("--steps", type=int, default=4000)
+    t.add_argument("--dagger-rounds", type=int, default=2)
+    t.add_argument("--dagger-steps", type=int, default=2000)
+    t.add_argument("--smooth-steps", type=int, default=1)
+    t.add_argument("--init-seed", type=int, default=0)
+    t.add_argument("--eval-seeds", type=int, default=5)
+    t.add_argument("--out", required=True)
+    e = sub.add_parser("evaluate", help="learned FCG vs Jacobi-CG on held-out seeds")
+    e.add_argument("--ckpt", required=True)
+    e.add_argument("--n", type=int, default=128)
+    e.add_argument("--seeds", type=int, default=50)
+    e.add_argument("--seed0", type=int, default=5000)
+    args = ap.parse_args(argv)
+    dev = torch.device("cuda", 0)
+
+    def log(msg):
+        print(msg, flush=True)
+
+    if args.cmd == "train":
+        t0 = time.time()
+        systems = []
+        for i in range(args.train_seeds):
+            systems.append(FV7System(args.n, args.sigma, args.corr, args.seed0 + i, dev))
+            if (i + 1) % 20 == 0:
+                log(f"[train] {i + 1}/{args.train_seeds} systems ({time.time() - t0:.0f}s)")
+        net = UNet(args.n, dev, seed=args.init_seed)
+        info = train_learned_preconditioner(net, systems, smooth_steps=args.smooth_steps,
+                                            pairs_per_solve=args.pairs_per_solve, steps=args.steps,
+                                            dagger_rounds=args.dagger_rounds, dagger_steps=args.dagger_steps,
+                                            seed=args.init_seed, log=log)
+        meta = dict(n=args.n, sigma=args.sigma, corr=args.corr, train_seed0=args.seed0, train_seeds=args.train_seeds,
+                    smooth_steps=args.smooth_steps, omega=2.0 / 3.0, steps=args.steps, dagger_rounds=args.dagger_rounds,
+                    dagger_steps=args.dagger_steps, init_seed=args.init_seed, pcg_pairs=info["pcg_pairs"],
+                    fcg_pairs=info["fcg_pairs"], train_seconds=info["seconds"])
+        save_checkpoint(args.out, net, meta)
+        log(f"[train] wrote {args.out} after {time.time() - t0:.0f}s")
+        del systems
+        if args.eval_seeds:
+            evaluate(net, meta, range(5000, 5000 + args.eval_seeds), dev, log=log)
+        return 0
+    net, meta = load_checkpoint(args.ckpt, args.n, dev)
+    res = evaluate(net, meta, range(args.seed0, args.seed0 + args.seeds), dev, log=log)
+    sp = sorted(min(r["jacobi64_ms"], r["ir_ms"]) / r["fcg_ms"] for r in res)
+    log(f"[eval] median speedup {sp[len(sp) // 2]:.2f}x over {len(res)} held-out systems")
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+
+    sys.exit(main())
