@@ -95,6 +95,8 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=64, help="elements per side of the elasticity mesh")
     ap.add_argument("--no-extras", action="store_true", help="skip CG / assembly / CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--ir-n", type=int, default=256,
+                    help="grid side of the FP64-refinement workload (variable-coefficient Poisson; 0: skip)")
     ap.add_argument("--fcg-seeds", type=int, default=10,
                     help="held-out 128^3 systems for the learned-FCG vs Jacobi-CG comparison (0: skip)")
     ap.add_argument("--eager", action="store_true",
@@ -268,6 +270,41 @@ def main() -> int:
         result["assembly"] = {"triplets": T3, "blocks": u3, "ms": round(t_a * 1e3, 3),
                               "triplets_per_s": round(T3 / t_a), "alg_GBps": round(asm_bytes / t_a / 1e9, 1)}
         del r3, c3, v3, A3
+
+        # FP32 Jacobi-CG + FP64 iterative refinement to 1e-10 on 256^3 variable-coefficient Poisson,
+        # solution checked on the host by the oracle's FP64 residual
+        if args.ir_n > 0:
+            import oracle
+            from warp_amd.optim import refine
+
+            n4 = args.ir_n
+            N4 = n4**3
+            off4, col4, val4, b4, _ = problems.fv7_problem(n4, 2.3, 16.0 * n4 / 256, 5, 6, device=dev)
+            A4 = sparse.bsr_zeros(N4, N4, wa.float32, device=dev)
+            A4.offsets, A4.columns, A4.values, A4.nnz = off4, col4, val4, col4.numel()
+            b64 = b4.to(torch.float64)
+            x64 = torch.zeros(N4, dtype=torch.float64, device=dev)
+            refine.cg_refined(A4, b64, x64, tol=1e-10, max_outer=1, inner_maxiter=10)  # kernels warm
+            x64.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ir = refine.cg_refined(A4, b64, x64, tol=1e-10, inner_tol=1e-4, M="diag")
+            torch.cuda.synchronize()
+            t_ir = time.perf_counter() - t0
+            r_host = oracle.residual_f32_f64(off4.cpu().numpy(), col4.cpu().numpy(), val4.cpu().numpy(),
+                                             x64.cpu().numpy(), b64.cpu().numpy())
+            true_rel = float(np.linalg.norm(r_host) / np.linalg.norm(b64.cpu().numpy()))
+            log(f"[bench] refinement {n4}^3: {ir.outer_iterations} outer / {ir.inner_iterations} inner iterations, "
+                f"{t_ir * 1e3:.1f} ms, FP64 oracle residual {true_rel:.2e}")
+            result["refinement"] = {
+                "workload": f"-div(k grad u)=f, 7-point FV {n4}^3, log-k GRF sigma 2.3, corr {16 * n4 // 256}, "
+                            "FP32 Jacobi-CG inner (tol 1e-4) + FP64 refinement",
+                "dofs": N4, "nnz": int(col4.numel()), "tol": 1e-10,
+                "outer_iterations": ir.outer_iterations, "inner_iterations": ir.inner_iterations,
+                "ms": round(t_ir * 1e3, 2), "ms_per_inner_iteration": round(t_ir * 1e3 / max(ir.inner_iterations, 1), 4),
+                "fp64_oracle_rel_residual": true_rel,
+            }
+            del A4, off4, col4, val4, b4, b64, x64, r_host
 
         # learned flexible CG vs Jacobi-CG on held-out 128^3 variable-coefficient Poisson systems
         ckpt = os.path.join(ROOT, "warp_amd", "data", "fv7_unet_128.npz")

@@ -29,10 +29,11 @@ def test_energy_loss_kernel_vs_oracle(device):
     r, az, e, z = (rng.standard_normal(n) for _ in range(4))
     sc = np.array([r @ z, z @ az, r @ e])
     L_ref, g_ref = ounet.energy_loss(r, e, z, az)
-    t = lambda a: torch.tensor(a, dtype=torch.float64, device=device)
+    # keep every device tensor referenced until the kernel has run
+    rt, azt, sct = (torch.tensor(a, dtype=torch.float64, device=device) for a in (r, az, sc))
     g = torch.empty(n, dtype=torch.float64, device=device)
     loss = torch.zeros(1, dtype=torch.float64, device=device)
-    _native.call("warp_amd_energy_loss_grad", n, _native.ptr(t(r)), _native.ptr(t(az)), _native.ptr(t(sc)),
+    _native.call("warp_amd_energy_loss_grad", n, _native.ptr(rt), _native.ptr(azt), _native.ptr(sct),
                  _native.ptr(g), _native.ptr(loss), _native.stream_of(device))
     torch.cuda.synchronize()
     assert float(loss.item()) == pytest.approx(L_ref, rel=1e-13)
@@ -83,24 +84,40 @@ def test_zero_network_stagnates_then_falls_back(device):
     assert out["true_relative_residual"] < 2e-8
 
 
-def test_trained_network_beats_smoothing_alone(device):
-    """A short training run (energy loss on plain-CG pairs, then on the FCG's
-    own iterates) cuts the FCG iteration count well below the same composite
-    with the network output removed."""
+def test_training_lowers_the_energy_loss(device):
+    """A short run of the training pipeline (plain-CG pairs, then the FCG's own
+    iterates) lowers the energy loss of held-out pairs."""
     n = 32
-    systems = [learned.FV7System(n, 1.5, 2.0, seed=100 + i, device=device) for i in range(6)]
+    systems = [learned.FV7System(n, 1.5, 2.0, seed=100 + i, device=device) for i in range(4)]
+    held = learned.FV7System(n, 1.5, 2.0, seed=999, device=device)
+    pairs = learned.generate_energy_pairs(held, count=10)
+    assert len(pairs) == 10
     net = learned.UNet(n, device, seed=0)
-    result = learned.train_learned_preconditioner(net, systems, smooth_steps=1, steps=600, dagger_rounds=1,
-                                                  dagger_steps=300, seed=0)
-    assert result["pcg_pairs"] > 0 and result["fcg_pairs"] > 0
-    test = learned.FV7System(n, 1.5, 2.0, seed=999, device=device)
-    x = torch.zeros(n**3, dtype=torch.float64, device=device)
-    trained = learned.LearnedFCG(test, net, smooth_steps=1).solve(x, tol=1e-8, maxiter=400, fallback=False,
-                                                                   stagnation_window=1000)
-    zero = learned.UNet(n, device, params={k: np.zeros_like(v) for k, v in learned.init_params(0).items()})
+    trainer = learned.EnergyTrainer(net, smooth_steps=1)
+    before = np.mean([trainer.loss(held, r, e) for r, e in pairs])
+    result = learned.train_learned_preconditioner(net, systems, smooth_steps=1, steps=300, dagger_rounds=1,
+                                                  dagger_steps=100, seed=0)
+    assert result["pcg_pairs"] == 40 and result["fcg_pairs"] > 0
+    after = np.mean([trainer.loss(held, r, e) for r, e in pairs])
+    assert after < 0.8 * before, (before, after)
+
+
+def test_shipped_network_gain_over_smoothing_alone(device):
+    """The shipped 128^3 network against the same flexible CG with the network
+    output removed (Jacobi smoothing only), on a held-out system: both reach an
+    FP64-checked 1e-8, the network cuts the iteration count at least 4x."""
+    import os
+
+    ckpt = os.path.join(os.path.dirname(learned.__file__), "..", "data", "fv7_unet_128.npz")
+    net, meta = learned.load_checkpoint(ckpt, 128, device)
+    test = learned.FV7System(128, float(meta["sigma"]), float(meta["corr"]), seed=5003, device=device)
+    x = torch.zeros(128**3, dtype=torch.float64, device=device)
+    trained = learned.LearnedFCG(test, net, smooth_steps=int(meta["smooth_steps"])).solve(
+        x, tol=1e-8, maxiter=2000, fallback=False, stagnation_window=1000)
+    zero = learned.UNet(128, device, params={k: np.zeros_like(v) for k, v in learned.init_params(0).items()})
     x.zero_()
-    smooth = learned.LearnedFCG(test, zero, smooth_steps=1).solve(x, tol=1e-8, maxiter=400, fallback=False,
-                                                                  stagnation_window=1000)
-    assert trained["true_relative_residual"] < 2e-8
-    assert smooth["true_relative_residual"] < 2e-8
-    assert trained["iterations"] * 2 <= smooth["iterations"], (trained["iterations"], smooth["iterations"])
+    smooth = learned.LearnedFCG(test, zero, smooth_steps=int(meta["smooth_steps"])).solve(
+        x, tol=1e-8, maxiter=4000, fallback=False, stagnation_window=10000)
+    assert trained["true_relative_residual"] <= 1e-8
+    assert smooth["true_relative_residual"] <= 1e-8
+    assert trained["iterations"] * 4 <= smooth["iterations"], (trained["iterations"], smooth["iterations"])

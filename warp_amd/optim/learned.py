@@ -447,8 +447,16 @@ def generate_energy_pairs(system: FV7System, count: int = 10, tol: float = 1e-10
     M = linear.preconditioner(A64, "diag")
     x_star = torch.zeros(N, dtype=torch.float64, device=dev)
     it_total, _, _ = linear.cg(A64, system.b, x_star, tol=tol, atol=0.0, maxiter=maxiter, M=M, check_every=50)
+    # count - 1 distinct iterations in [1, it_total), geometrically spaced (denser
+    # where the geometric points collide at small k)
     want = sorted({max(1, int(round(it_total ** (j / (count - 1))))) for j in range(1, count)})
     want = [t for t in want if t < it_total]
+    extra = 1
+    while len(want) < min(count - 1, it_total - 1) and extra < it_total:
+        if extra not in want:
+            want.append(extra)
+        extra += 1
+    want = sorted(want)
     snaps = {0: torch.zeros(N, dtype=torch.float64, device=dev)}
     xw = torch.zeros(N, dtype=torch.float64, device=dev)
 
@@ -838,17 +846,32 @@ class LearnedFCG:
                     self.iteration(1)
             torch.cuda.current_stream(self.sys.device).wait_stream(cs)
             self._graph = g
-        while hist[-1] > tol and it < maxiter:
-            if use_graph:
-                self._graph.replay()
-            else:
-                self.iteration(0)
-                self.iteration(1)
-            it += 2
-            hist.append(math.sqrt(max(float(self.sc[0].item()), 0.0)) / bnorm)
-            w = stagnation_window // 2
-            if len(hist) > w and hist[-1] > stagnation_factor * hist[-1 - w]:
+        stalled = False
+        for _ in range(4):  # FCG, then residual replacement while the true residual is above tol
+            while hist[-1] > tol and it < maxiter:
+                if use_graph:
+                    self._graph.replay()
+                else:
+                    self.iteration(0)
+                    self.iteration(1)
+                it += 2
+                hist.append(math.sqrt(max(float(self.sc[0].item()), 0.0)) / bnorm)
+                w = stagnation_window // 2
+                if len(hist) > w and hist[-1] > stagnation_factor * hist[-1 - w]:
+                    stalled = True
+                    break
+            if stalled or it >= maxiter:
                 break
+            # converged on the recursive residual: replace it by the FP64 true residual
+            self.sys.residual(self.x, self.r, self.sc[0:1], self.ws_mv)
+            true_rel = math.sqrt(max(float(self.sc[0].item()), 0.0)) / bnorm
+            if true_rel <= tol:
+                break
+            hist.append(true_rel)
+            for t in self.p + self.q:
+                t.zero_()
+            self.sc[4] = 1.0
+            self.sc[5] = 1.0
         x.copy_(self.x)
         fb_iters = 0
         if hist[-1] > tol and fallback:
