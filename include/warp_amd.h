@@ -299,6 +299,8 @@ int warp_amd_mse_loss_f32(int64_t count, const float* out, const float* target, 
 /* Adam step; replaces adam_step_kernel_float (warp/_src/optim/adam.py:30-47) */
 int warp_amd_adam_step_f32(int64_t count, const float* g, float* m, float* v, float* param, float lr, float beta1,
                            float beta2, float t, float eps, void* stream);
+/* y = alpha * x + beta * y (FP32; gradient accumulation over a batch) */
+int warp_amd_axpby_f32(int64_t count, float alpha, const float* x, float beta, float* y, void* stream);
 /* network input (r * inv_scale, standardized log k) and output z = scale * out; scales on the device */
 int warp_amd_unet_pack_input(int64_t nvox, const double* r, const float* logk_std, const double* inv_scale,
                              float* inp, void* stream);
@@ -312,6 +314,25 @@ int warp_amd_jacobi_sweep_f32_f64(int nrow, const int* offsets, const int* colum
                                   int64_t nnz_hint, const double* z, const double* r, const double* dinv, double omega,
                                   double* z_out, void* stream);
 int warp_amd_unet_unpack_add(int64_t n, const float* out, const double* scale, double mult, double* z, void* stream);
+/* Fused flexible-CG steps for a scalar (1x1) compact BSR matrix with FP32
+ * values and FP64 vectors, one damped Jacobi sweep before and after the
+ * network correction (no reference counterpart).  Scalars are device
+ * pointers; workspace >= warp_amd_fcg_fused_workspace_size(nrow).
+ *   presmooth:  r1 = r - A (omega D^-1 r); *rr1 = r1.r1, *scale = rms(r1), *inv_scale = 1 / rms(r1)
+ *   postsmooth: z2 = omega D^-1 r + gain * (*scale) * out; z = z2 + omega D^-1 (r - A z2); *zq = z.q_prev
+ *   direction_spmv: p = z - (*zq / *pq_prev) p_prev; q = A p; *pq = p.q; *pr = p.r */
+size_t warp_amd_fcg_fused_workspace_size(int nrow);
+int warp_amd_fcg_presmooth(int nrow, const int* offsets, const int* columns, const float* values, int64_t nnz_hint,
+                           const double* r, const double* dinv, double omega, double* r1, double* rr1, double* scale,
+                           double* inv_scale, void* workspace, size_t workspace_bytes, void* stream);
+int warp_amd_fcg_postsmooth(int nrow, const int* offsets, const int* columns, const float* values, int64_t nnz_hint,
+                            const double* r, const double* dinv, double omega, const float* out, const double* scale,
+                            double gain, const double* q_prev, double* z, double* zq, void* workspace,
+                            size_t workspace_bytes, void* stream);
+int warp_amd_fcg_direction_spmv(int nrow, const int* offsets, const int* columns, const float* values,
+                                int64_t nnz_hint, const double* z, const double* p_prev, const double* zq,
+                                const double* pq_prev, const double* r, double* p, double* q, double* pq, double* pr,
+                                void* workspace, size_t workspace_bytes, void* stream);
 /* Training loss of the learned preconditioner (no reference counterpart): the
  * one-step energy reduction L = 1 - (r.z)^2 / ((r.e)(z.Az)) of a direction z
  * for A e = r.  sc (device) = {r.z, z.Az, r.e}; writes g = dL/dz and *loss. */

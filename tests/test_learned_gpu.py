@@ -121,3 +121,19 @@ def test_shipped_network_gain_over_smoothing_alone(device):
     assert trained["true_relative_residual"] <= 1e-8
     assert smooth["true_relative_residual"] <= 1e-8
     assert trained["iterations"] * 4 <= smooth["iterations"], (trained["iterations"], smooth["iterations"])
+
+
+def test_fused_iteration_matches_unfused(device):
+    """The fused flexible-CG iteration (3 fused SpMV kernels) follows the
+    unfused one (diag_mv, residual, sweep, dots, direction, SpMV) to rounding."""
+    n = 32
+    system = learned.FV7System(n, 1.5, 2.0, seed=31, device=device)
+    net = learned.UNet(n, device, seed=4)
+    hist = {}
+    for fused in (False, True):
+        fcg = learned.LearnedFCG(system, net, smooth_steps=1, fused=fused)
+        x = torch.zeros(n**3, dtype=torch.float64, device=device)
+        out = fcg.solve(x, tol=1e-30, maxiter=8, use_graph=False, fallback=False, stagnation_window=1000)
+        hist[fused] = (np.array(out["history"]), x.cpu().numpy())
+    np.testing.assert_allclose(hist[True][0], hist[False][0], rtol=1e-9)
+    np.testing.assert_allclose(hist[True][1], hist[False][1], rtol=1e-9, atol=1e-12 * np.abs(hist[False][1]).max())

@@ -75,12 +75,27 @@ _SIGNATURES = {
     "warp_amd_mse_workspace_size": (c_size, [c_i64]),
     "warp_amd_mse_loss_f32": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
     "warp_amd_adam_step_f32": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_f32, c_f32, c_f32, c_f32, c_f32, c_ptr]),
+    "warp_amd_axpby_f32": (c_int, [c_i64, c_f32, c_ptr, c_f32, c_ptr, c_ptr]),
     "warp_amd_unet_pack_input": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]),
     "warp_amd_unet_unpack_output": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr]),
     "warp_amd_jacobi_sweep_f32_f64": (
         c_int, [c_int, c_ptr, c_ptr, c_ptr, c_i64, c_ptr, c_ptr, c_ptr, c_f64, c_ptr, c_ptr]
     ),
     "warp_amd_unet_unpack_add": (c_int, [c_i64, c_ptr, c_ptr, c_f64, c_ptr, c_ptr]),
+    "warp_amd_fcg_fused_workspace_size": (c_size, [c_int]),
+    "warp_amd_fcg_presmooth": (
+        c_int, [c_int, c_ptr, c_ptr, c_ptr, c_i64, c_ptr, c_ptr, c_f64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]
+    ),
+    "warp_amd_fcg_postsmooth": (
+        c_int,
+        [c_int, c_ptr, c_ptr, c_ptr, c_i64, c_ptr, c_ptr, c_f64, c_ptr, c_ptr, c_f64, c_ptr, c_ptr, c_ptr, c_ptr,
+         c_size, c_ptr],
+    ),
+    "warp_amd_fcg_direction_spmv": (
+        c_int,
+        [c_int, c_ptr, c_ptr, c_ptr, c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr,
+         c_size, c_ptr],
+    ),
     "warp_amd_energy_loss_grad": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]),
     "warp_amd_scale_to_f32": (c_int, [c_i64, c_ptr, c_ptr, c_f64, c_ptr, c_ptr]),
     "warp_amd_conv3_mfma_weight_size": (c_size, [c_int, c_int]),

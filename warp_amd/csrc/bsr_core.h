@@ -71,10 +71,13 @@ __device__ __forceinline__ RowAcc<S, BR> group_row_product(
 //
 // V is the stored value type and S the vector / accumulation type (V = float,
 // S = double gives the FP64 residual of an FP32 matrix).
-template <typename S, int BR, int BC, int G, int RPG, typename V = S>
-__device__ __forceinline__ RowAcc<S, BR * RPG> group_rows_product(
+//
+// XF: x(i) returns entry i of the vector being multiplied (a plain load, or
+// a value computed from several arrays by fused kernels).
+template <typename S, int BR, int BC, int G, int RPG, typename V, typename XF>
+__device__ __forceinline__ RowAcc<S, BR * RPG> group_rows_product_f(
     int row0, int nrow, int lane, const int* __restrict__ offsets, const int* __restrict__ row_counts,
-    const int* __restrict__ columns, const V* __restrict__ values, const S* __restrict__ x
+    const int* __restrict__ columns, const V* __restrict__ values, XF x
 )
 {
     RowAcc<S, BR * RPG> acc;
@@ -114,7 +117,7 @@ __device__ __forceinline__ RowAcc<S, BR * RPG> group_rows_product(
             S xv[BC];
 #pragma unroll
             for (int c = 0; c < BC; ++c)
-                xv[c] = x[(size_t)col[j] * BC + c];
+                xv[c] = x((size_t)col[j] * BC + c);
 #pragma unroll
             for (int s = 0; s < BR; ++s) {
                 S a = acc.v[j * BR + s];
@@ -129,6 +132,21 @@ __device__ __forceinline__ RowAcc<S, BR * RPG> group_rows_product(
     for (int s = 0; s < BR * RPG; ++s)
         acc.v[s] = group_sum<G>(acc.v[s]);
     return acc;
+}
+
+template <typename S> struct LoadX {
+    const S* __restrict__ p;
+    __device__ __forceinline__ S operator()(size_t i) const { return p[i]; }
+};
+
+template <typename S, int BR, int BC, int G, int RPG, typename V = S>
+__device__ __forceinline__ RowAcc<S, BR * RPG> group_rows_product(
+    int row0, int nrow, int lane, const int* __restrict__ offsets, const int* __restrict__ row_counts,
+    const int* __restrict__ columns, const V* __restrict__ values, const S* __restrict__ x
+)
+{
+    return group_rows_product_f<S, BR, BC, G, RPG, V>(row0, nrow, lane, offsets, row_counts, columns, values,
+                                                      LoadX<S>{x});
 }
 
 // acc.v[i] for a lane-dependent i.  Written as an explicit select chain on

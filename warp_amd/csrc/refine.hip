@@ -68,6 +68,150 @@ __global__ __launch_bounds__(kBlock) void jacobi_sweep_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fused flexible-CG iteration for a scalar (1x1) FP32 matrix, FP64 vectors,
+// one Jacobi sweep before and after the network correction.  Vectors the
+// SpMV gathers are formed on the fly from their parts (XF functors), so the
+// sweep results are never written and re-read.
+// ---------------------------------------------------------------------------
+
+struct PreSmoothX {  // omega D^-1 r
+    const double* __restrict__ dinv;
+    const double* __restrict__ r;
+    double omega;
+    __device__ __forceinline__ double operator()(size_t i) const { return omega * dinv[i] * r[i]; }
+};
+
+struct CorrectedX {  // omega D^-1 r + c * out
+    const double* __restrict__ dinv;
+    const double* __restrict__ r;
+    const float* __restrict__ out;
+    double omega, c;
+    __device__ __forceinline__ double operator()(size_t i) const
+    {
+        return fma(c, (double)out[i], omega * dinv[i] * r[i]);
+    }
+};
+
+struct DirectionX {  // z - beta p_prev
+    const double* __restrict__ z;
+    const double* __restrict__ p_prev;
+    double beta;
+    __device__ __forceinline__ double operator()(size_t i) const { return fma(-beta, p_prev[i], z[i]); }
+};
+
+// r1 = r - A (omega D^-1 r); partials of r1.r1
+template <int G, int RPG>
+__global__ __launch_bounds__(kBlock) void fcg_pre_kernel(int nrow, const int* __restrict__ offsets,
+                                                        const int* __restrict__ columns,
+                                                        const float* __restrict__ values, const double* __restrict__ r,
+                                                        const double* __restrict__ dinv, double omega,
+                                                        double* __restrict__ r1, double* __restrict__ partials)
+{
+    __shared__ double lds[kBlock / kWave];
+    const int row0 = (blockIdx.x * (kBlock / G) + threadIdx.x / G) * RPG;
+    const int lane = threadIdx.x % G;
+    double acc[1] = {0.0};
+    if (row0 < nrow) {
+        const PreSmoothX xf{dinv, r, omega};
+        const RowAcc<double, RPG> az =
+            group_rows_product_f<double, 1, 1, G, RPG, float>(row0, nrow, lane, offsets, nullptr, columns, values, xf);
+        for (int e = lane; e < RPG && row0 + e < nrow; e += G) {
+            const size_t i = (size_t)row0 + e;
+            const double v = r[i] - pick(az, e);
+            r1[i] = v;
+            acc[0] += v * v;
+        }
+    }
+    write_block_partials<kBlock, 1>(acc, partials, lds);
+}
+
+// z2 = omega D^-1 r + gain * scale * out;  z = z2 + omega D^-1 (r - A z2);
+// partials of z.q_prev
+template <int G, int RPG>
+__global__ __launch_bounds__(kBlock) void fcg_post_kernel(int nrow, const int* __restrict__ offsets,
+                                                         const int* __restrict__ columns,
+                                                         const float* __restrict__ values, const double* __restrict__ r,
+                                                         const double* __restrict__ dinv, double omega,
+                                                         const float* __restrict__ out, const double* scale,
+                                                         double gain, const double* __restrict__ q_prev,
+                                                         double* __restrict__ z, double* __restrict__ partials)
+{
+    __shared__ double lds[kBlock / kWave];
+    const int row0 = (blockIdx.x * (kBlock / G) + threadIdx.x / G) * RPG;
+    const int lane = threadIdx.x % G;
+    double acc[1] = {0.0};
+    if (row0 < nrow) {
+        const CorrectedX xf{dinv, r, out, omega, gain * (*scale)};
+        const RowAcc<double, RPG> az =
+            group_rows_product_f<double, 1, 1, G, RPG, float>(row0, nrow, lane, offsets, nullptr, columns, values, xf);
+        for (int e = lane; e < RPG && row0 + e < nrow; e += G) {
+            const size_t i = (size_t)row0 + e;
+            const double z2 = xf(i);
+            const double v = fma(omega * dinv[i], r[i] - pick(az, e), z2);
+            z[i] = v;
+            acc[0] += v * q_prev[i];
+        }
+    }
+    write_block_partials<kBlock, 1>(acc, partials, lds);
+}
+
+// beta = zq / pq_prev;  p = z - beta p_prev;  q = A p;  partials of p.q, p.r
+template <int G, int RPG>
+__global__ __launch_bounds__(kBlock) void fcg_dir_spmv_kernel(int nrow, const int* __restrict__ offsets,
+                                                             const int* __restrict__ columns,
+                                                             const float* __restrict__ values,
+                                                             const double* __restrict__ z,
+                                                             const double* __restrict__ p_prev, const double* zq,
+                                                             const double* pq_prev, const double* __restrict__ r,
+                                                             double* __restrict__ p, double* __restrict__ q,
+                                                             double* __restrict__ partials)
+{
+    __shared__ double lds[kBlock / kWave];
+    const int row0 = (blockIdx.x * (kBlock / G) + threadIdx.x / G) * RPG;
+    const int lane = threadIdx.x % G;
+    double acc[2] = {0.0, 0.0};
+    if (row0 < nrow) {
+        const double beta = *pq_prev != 0.0 ? *zq / *pq_prev : 0.0;
+        const DirectionX xf{z, p_prev, beta};
+        const RowAcc<double, RPG> ap =
+            group_rows_product_f<double, 1, 1, G, RPG, float>(row0, nrow, lane, offsets, nullptr, columns, values, xf);
+        for (int e = lane; e < RPG && row0 + e < nrow; e += G) {
+            const size_t i = (size_t)row0 + e;
+            const double pv = xf(i);
+            const double qv = pick(ap, e);
+            p[i] = pv;
+            q[i] = qv;
+            acc[0] += pv * qv;
+            acc[1] += pv * r[i];
+        }
+    }
+    write_block_partials<kBlock, 2>(acc, partials, lds);
+}
+
+// out0 = sum of partials[0, nparts); with out1: out1 = sum of partials[nparts, 2 nparts);
+// with scale/inv_scale: scale = sqrt(out0 / n), inv_scale = 1 / scale (0 if scale is 0)
+__global__ __launch_bounds__(kFinalizeBlock) void finalize_fcg_kernel(const double* partials, int nparts,
+                                                                     double* out0, double* out1, int64_t n,
+                                                                     double* scale, double* inv_scale)
+{
+    __shared__ double lds[kFinalizeBlock / kWave];
+    const double t0 = sum_partials<kFinalizeBlock>(partials, nparts, lds);
+    double t1 = 0.0;
+    if (out1)
+        t1 = sum_partials<kFinalizeBlock>(partials + nparts, nparts, lds);
+    if (threadIdx.x == 0) {
+        *out0 = t0;
+        if (out1)
+            *out1 = t1;
+        if (scale) {
+            const double sv = sqrt(t0 / (double)n);
+            *scale = sv;
+            *inv_scale = sv > 0.0 ? 1.0 / sv : 0.0;
+        }
+    }
+}
+
 // z += mult * (*scale) * out
 __global__ __launch_bounds__(kBlock) void unpack_add_kernel(int64_t n, const float* __restrict__ out, const double* scale,
                                                            double mult, double* __restrict__ z)
@@ -191,6 +335,33 @@ int residual_dispatch(
     case 32: return launch_residual<BR, 32>(nrow, offsets, columns, values, x, b, r64, r32, partials, rr, stream);
     default: return launch_residual<BR, 64>(nrow, offsets, columns, values, x, b, r64, r32, partials, rr, stream);
     }
+}
+
+template <int G>
+int launch_fcg_fused(int which, int nrow, const int* offsets, const int* columns, const float* values,
+                     const double* a0, const double* a1, const float* out, const double* s0, const double* s1,
+                     double omega, double gain, double* o0, double* o1, double* f0, double* f1, double* scale,
+                     double* inv_scale, double* partials, hipStream_t stream)
+{
+    constexpr int RPG = rows_per_group_for(1, 1);
+    const unsigned grid = (unsigned)ceil_div(nrow, (kBlock / G) * RPG);
+    if (which == 0) {  // a0 = r, a1 = dinv -> o0 = r1
+        fcg_pre_kernel<G, RPG><<<grid, kBlock, 0, stream>>>(nrow, offsets, columns, values, a0, a1, omega, o0, partials);
+        WA_LAUNCH_CHECK();
+        finalize_fcg_kernel<<<1, kFinalizeBlock, 0, stream>>>(partials, (int)grid, f0, nullptr, nrow, scale, inv_scale);
+    } else if (which == 1) {  // a0 = r, a1 = dinv, out, s0 = scale, o1 = q_prev (read) -> o0 = z
+        fcg_post_kernel<G, RPG><<<grid, kBlock, 0, stream>>>(nrow, offsets, columns, values, a0, a1, omega, out, s0,
+                                                              gain, o1, o0, partials);
+        WA_LAUNCH_CHECK();
+        finalize_fcg_kernel<<<1, kFinalizeBlock, 0, stream>>>(partials, (int)grid, f0, nullptr, nrow, nullptr, nullptr);
+    } else {  // a0 = z, a1 = p_prev, s0 = zq, s1 = pq_prev, scale = r (read) -> o0 = p, o1 = q
+        fcg_dir_spmv_kernel<G, RPG><<<grid, kBlock, 0, stream>>>(nrow, offsets, columns, values, a0, a1, s0, s1,
+                                                                  scale, o0, o1, partials);
+        WA_LAUNCH_CHECK();
+        finalize_fcg_kernel<<<1, kFinalizeBlock, 0, stream>>>(partials, (int)grid, f0, f1, nrow, nullptr, nullptr);
+    }
+    WA_LAUNCH_CHECK();
+    return kOk;
 }
 
 template <int G>
@@ -360,3 +531,63 @@ WA_API int warp_amd_scale_to_f32(int64_t n, const double* x, const double* scale
     WA_LAUNCH_CHECK();
     return wa::kOk;
 }
+
+// ---------------------------------------------------------------------------
+// Fused flexible-CG steps (scalar FP32 matrix, FP64 vectors).  Workspace:
+// warp_amd_fcg_fused_workspace_size(nrow) bytes of partial sums.
+// ---------------------------------------------------------------------------
+
+WA_API size_t warp_amd_fcg_fused_workspace_size(int nrow)
+{
+    return sizeof(double) * 2 * (size_t)(wa::ceil_div(nrow > 0 ? nrow : 1, 4) + 1);
+}
+
+#define WA_FCG_DISPATCH(WHICH, ...)                                                                            \
+    do {                                                                                                       \
+        WA_REQUIRE(nrow > 0, "fcg: empty matrix");                                                             \
+        WA_REQUIRE(workspace && workspace_bytes >= warp_amd_fcg_fused_workspace_size(nrow),                    \
+                   "fcg: workspace too small");                                                                \
+        const int g = wa::default_bsr_shape(nrow, 1, 1, nnz_hint).lanes;                                       \
+        double* partials = static_cast<double*>(workspace);                                                    \
+        hipStream_t s = wa::as_stream(stream);                                                                 \
+        switch (g) {                                                                                           \
+        case 1: return wa::launch_fcg_fused<1>(WHICH, nrow, offsets, columns, values, __VA_ARGS__, partials, s); \
+        case 2: return wa::launch_fcg_fused<2>(WHICH, nrow, offsets, columns, values, __VA_ARGS__, partials, s); \
+        case 4: return wa::launch_fcg_fused<4>(WHICH, nrow, offsets, columns, values, __VA_ARGS__, partials, s); \
+        case 8: return wa::launch_fcg_fused<8>(WHICH, nrow, offsets, columns, values, __VA_ARGS__, partials, s); \
+        case 16: return wa::launch_fcg_fused<16>(WHICH, nrow, offsets, columns, values, __VA_ARGS__, partials, s); \
+        case 32: return wa::launch_fcg_fused<32>(WHICH, nrow, offsets, columns, values, __VA_ARGS__, partials, s); \
+        default: return wa::launch_fcg_fused<64>(WHICH, nrow, offsets, columns, values, __VA_ARGS__, partials, s); \
+        }                                                                                                      \
+    } while (0)
+
+// r1 = r - A (omega D^-1 r) (one Jacobi sweep from zero, then the residual);
+// *rr1 = r1.r1, *scale = rms(r1), *inv_scale = 1 / rms(r1)  (device)
+WA_API int warp_amd_fcg_presmooth(int nrow, const int* offsets, const int* columns, const float* values,
+                                  int64_t nnz_hint, const double* r, const double* dinv, double omega, double* r1,
+                                  double* rr1, double* scale, double* inv_scale, void* workspace,
+                                  size_t workspace_bytes, void* stream)
+{
+    WA_FCG_DISPATCH(0, r, dinv, nullptr, nullptr, nullptr, omega, 0.0, r1, nullptr, rr1, nullptr, scale, inv_scale);
+}
+
+// z2 = omega D^-1 r + gain * (*scale) * out;  z = z2 + omega D^-1 (r - A z2);
+// *zq = z.q_prev  (device)
+WA_API int warp_amd_fcg_postsmooth(int nrow, const int* offsets, const int* columns, const float* values,
+                                   int64_t nnz_hint, const double* r, const double* dinv, double omega,
+                                   const float* out, const double* scale, double gain, const double* q_prev,
+                                   double* z, double* zq, void* workspace, size_t workspace_bytes, void* stream)
+{
+    WA_FCG_DISPATCH(1, r, dinv, out, scale, nullptr, omega, gain, z, const_cast<double*>(q_prev), zq, nullptr,
+                    nullptr, nullptr);
+}
+
+// p = z - (zq / pq_prev) p_prev;  q = A p;  *pq = p.q, *pr = p.r  (device)
+WA_API int warp_amd_fcg_direction_spmv(int nrow, const int* offsets, const int* columns, const float* values,
+                                       int64_t nnz_hint, const double* z, const double* p_prev, const double* zq,
+                                       const double* pq_prev, const double* r, double* p, double* q, double* pq,
+                                       double* pr, void* workspace, size_t workspace_bytes, void* stream)
+{
+    WA_FCG_DISPATCH(2, z, p_prev, nullptr, zq, pq_prev, 0.0, 0.0, p, q, pq, pr, const_cast<double*>(r), nullptr);
+}
+#undef WA_FCG_DISPATCH

@@ -277,6 +277,15 @@ __global__ __launch_bounds__(kBlock) void adam_kernel(
     p[i] = p[i] - lr * mhat / (sqrtf(vhat) + eps);
 }
 
+// y = alpha * x + beta * y (gradient accumulation over a batch)
+__global__ __launch_bounds__(kBlock) void axpby_f32_kernel(int64_t count, float alpha, const float* __restrict__ x,
+                                                          float beta, float* __restrict__ y)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < count)
+        y[i] = beta == 0.0f ? alpha * x[i] : fmaf(alpha, x[i], beta * y[i]);
+}
+
 // inp[v] = (r[v] * inv_scale, logk_std[v]); inv_scale read from device memory.
 __global__ __launch_bounds__(kBlock) void pack_input_kernel(
     int64_t nvox, const double* __restrict__ r, const float* __restrict__ logk_std, const double* inv_scale,
@@ -486,6 +495,15 @@ WA_API int warp_amd_adam_step_f32(int64_t count, const float* g, float* m, float
 {
     wa::adam_kernel<<<wa::grid1(count), wa::kBlock, 0, wa::as_stream(stream)>>>(count, g, m, v, param, lr, beta1,
                                                                                   beta2, t, eps);
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}
+
+WA_API int warp_amd_axpby_f32(int64_t count, float alpha, const float* x, float beta, float* y, void* stream)
+{
+    if (count <= 0)
+        return wa::kOk;
+    wa::axpby_f32_kernel<<<wa::grid1(count), wa::kBlock, 0, wa::as_stream(stream)>>>(count, alpha, x, beta, y);
     WA_LAUNCH_CHECK();
     return wa::kOk;
 }

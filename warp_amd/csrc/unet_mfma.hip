@@ -30,6 +30,7 @@ namespace wa {
 namespace {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kBlock = 256;  // 4 wavefronts, one voxel cube each
@@ -116,6 +117,44 @@ constexpr int kHalo = kHX * kHY * kHZ;
 // padding for CI >= 16 to spread the 8-byte reads over the banks.
 __host__ __device__ constexpr int lds_stride_words(int ci) { return ci <= 2 ? 1 : ci / 2 + 2; }
 
+// Stage the 10 x 10 x 6 input halo of the tile at (X0, Y0, Z0) in LDS
+// (zeros outside the grid; with kInUpAdd the input is x + upsample(coarse),
+// rounded to BF16 once per voxel).
+template <int CI, int IN>
+__device__ __forceinline__ void stage_tile(unsigned* tile, int m, int X0, int Y0, int Z0,
+                                           const unsigned short* __restrict__ x,
+                                           const unsigned short* __restrict__ coarse)
+{
+    constexpr int SW = lds_stride_words(CI);
+    const int mh = m / 2;
+    if constexpr (CI <= 2) {
+        for (int hv = threadIdx.x; hv < kHalo; hv += kBlock) {
+            const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
+            const int gx = X0 - 1 + hx, gy = Y0 - 1 + hy, gz = Z0 - 1 + hz;
+            unsigned v = 0;
+            if (gx >= 0 && gx < m && gy >= 0 && gy < m && gz >= 0 && gz < m) {
+                const size_t g = (size_t)(gz * m + gy) * m + gx;
+                v = CI == 2 ? reinterpret_cast<const unsigned*>(x)[g] : (unsigned)x[g];
+            }
+            tile[hv] = v;
+        }
+    } else {
+        constexpr int QPV = CI / 4;  // 8-byte channel quads per voxel
+        for (int c = threadIdx.x; c < kHalo * QPV; c += kBlock) {
+            const int hv = c / QPV, qd = c % QPV;
+            const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
+            const int gx = X0 - 1 + hx, gy = Y0 - 1 + hy, gz = Z0 - 1 + hz;
+            bf16x4 v = zero4();
+            if (gx >= 0 && gx < m && gy >= 0 && gy < m && gz >= 0 && gz < m) {
+                v = load4(x + ((size_t)(gz * m + gy) * m + gx) * CI + 4 * qd);
+                if constexpr (IN == kInUpAdd)
+                    v = add4(v, load4(coarse + ((size_t)((gz >> 1) * mh + (gy >> 1)) * mh + (gx >> 1)) * CI + 4 * qd));
+            }
+            *reinterpret_cast<bf16x4*>(&tile[hv * SW + 2 * qd]) = v;
+        }
+    }
+}
+
 // x: input [m^3][CI] BF16; with kInUpAdd the input is x[v] + coarse[v / 2]
 // (coarse: [(m/2)^3][CI]), rounded to BF16.  Output: y BF16 [m^3][CO] or, with
 // kOutF32, yf FP32 [m^3][CO] (kOutAccumulate adds to yf); pooled BF16
@@ -143,33 +182,7 @@ __global__ __launch_bounds__(kBlock) void conv3_mfma_kernel(
     const int Z0 = blockIdx.y * kTZ;
     const int mh = m / 2;
 
-    // ---- stage the halo tile --------------------------------------------
-    if constexpr (CI <= 2) {
-        for (int hv = threadIdx.x; hv < kHalo; hv += kBlock) {
-            const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
-            const int gx = X0 - 1 + hx, gy = Y0 - 1 + hy, gz = Z0 - 1 + hz;
-            unsigned v = 0;
-            if (gx >= 0 && gx < m && gy >= 0 && gy < m && gz >= 0 && gz < m) {
-                const size_t g = (size_t)(gz * m + gy) * m + gx;
-                v = CI == 2 ? reinterpret_cast<const unsigned*>(x)[g] : (unsigned)x[g];
-            }
-            tile[hv] = v;
-        }
-    } else {
-        constexpr int QPV = CI / 4;  // 8-byte channel quads per voxel
-        for (int c = threadIdx.x; c < kHalo * QPV; c += kBlock) {
-            const int hv = c / QPV, qd = c % QPV;
-            const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
-            const int gx = X0 - 1 + hx, gy = Y0 - 1 + hy, gz = Z0 - 1 + hz;
-            bf16x4 v = zero4();
-            if (gx >= 0 && gx < m && gy >= 0 && gy < m && gz >= 0 && gz < m) {
-                v = load4(x + ((size_t)(gz * m + gy) * m + gx) * CI + 4 * qd);
-                if constexpr (IN == kInUpAdd)
-                    v = add4(v, load4(coarse + ((size_t)((gz >> 1) * mh + (gy >> 1)) * mh + (gx >> 1)) * CI + 4 * qd));
-            }
-            *reinterpret_cast<bf16x4*>(&tile[hv * SW + 2 * qd]) = v;
-        }
-    }
+    stage_tile<CI, IN>(tile, m, X0, Y0, Z0, x, coarse);
     __syncthreads();
 
     const int lane = threadIdx.x % 64;
@@ -306,6 +319,75 @@ __global__ __launch_bounds__(kBlock) void conv3_mfma_kernel(
     }
 }
 
+// One output channel (the network's last layer): MFMA would waste 15 of its
+// 16 rows, so each thread computes one voxel with packed BF16 dot products,
+// v_dot2_f32_bf16, against weights wpk[tap][ci / 2] (BF16 pairs), reading
+// the LDS-staged halo.  Output FP32; flags: ReLU, accumulate.
+template <int CI, int IN>
+__global__ __launch_bounds__(kBlock) void conv3_co1_kernel(int m, const unsigned short* __restrict__ x,
+                                                          const unsigned short* __restrict__ coarse,
+                                                          const unsigned* __restrict__ wpk,
+                                                          const float* __restrict__ bias, int flags,
+                                                          float* __restrict__ yf)
+{
+    constexpr int SW = lds_stride_words(CI);
+    static_assert(CI >= 16 && CI % 16 == 0, "one-output kernel expects a multiple of 16 input channels");
+    __shared__ unsigned tile[kHalo * SW];
+    const int tiles_x = (m + kTX - 1) / kTX;
+    const int X0 = (blockIdx.x % tiles_x) * kTX;
+    const int Y0 = (blockIdx.x / tiles_x) * kTY;
+    const int Z0 = blockIdx.y * kTZ;
+    stage_tile<CI, IN>(tile, m, X0, Y0, Z0, x, coarse);
+    __syncthreads();
+
+    const int lx = threadIdx.x & 7, ly = (threadIdx.x >> 3) & 7, lz = threadIdx.x >> 6;  // 8 x 8 x 4
+    float acc = bias ? bias[0] : 0.0f;
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+        const int hv = ((lz + t / 9) * kHY + (ly + (t / 3) % 3)) * kHX + (lx + t % 3);
+        const unsigned* src = &tile[hv * SW];
+#pragma unroll
+        for (int j = 0; j < CI / 2; ++j)
+            acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, src[j]),
+                                                  __builtin_bit_cast(bf16x2, wpk[t * (CI / 2) + j]), acc, false);
+    }
+    const int vx = X0 + lx, vy = Y0 + ly, vz = Z0 + lz;
+    if (vx < m && vy < m && vz < m) {
+        if (flags & kOutRelu)
+            acc = fmaxf(acc, 0.0f);
+        float* d = yf + (size_t)(vz * m + vy) * m + vx;
+        *d = (flags & kOutAccumulate) ? *d + acc : acc;
+    }
+}
+
+// wpk[tap][c / 2] = (W(tap, c, 0), W(tap, c + 1, 0)) as a BF16 pair (low, high)
+__global__ void pack_co1_kernel(int ci, int transpose_flip, const float* __restrict__ W, unsigned* __restrict__ out)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= 27 * (ci / 2))
+        return;
+    const int tap = idx / (ci / 2), c = 2 * (idx % (ci / 2));
+    // (ci -> 1): W[tap][c][0];  transposed (layer 1 -> ci): W[26 - tap][0][c]
+    const float w0 = transpose_flip ? W[(size_t)(26 - tap) * ci + c] : W[(size_t)tap * ci + c];
+    const float w1 = transpose_flip ? W[(size_t)(26 - tap) * ci + c + 1] : W[(size_t)tap * ci + c + 1];
+    out[idx] = (unsigned)f2bf(w0) | ((unsigned)f2bf(w1) << 16);
+}
+
+template <int CI>
+int launch_co1(int m, int in_mode, const unsigned short* x, const unsigned short* coarse, const void* wpk,
+               const float* bias, int flags, float* yf, hipStream_t s)
+{
+    const int txy = (m + kTX - 1) / kTX;
+    const dim3 grid((unsigned)(txy * txy), (unsigned)(m / kTZ));
+    const unsigned* w = static_cast<const unsigned*>(wpk);
+    if (in_mode == kInUpAdd)
+        conv3_co1_kernel<CI, kInUpAdd><<<grid, kBlock, 0, s>>>(m, x, coarse, w, bias, flags, yf);
+    else
+        conv3_co1_kernel<CI, kInPlain><<<grid, kBlock, 0, s>>>(m, x, coarse, w, bias, flags, yf);
+    WA_LAUNCH_CHECK();
+    return kOk;
+}
+
 template <int CI, int CO>
 int launch_conv(int m, int in_mode, const unsigned short* x, const unsigned short* coarse, const void* wsw,
                 const float* bias, int flags, unsigned short* y, float* yf, unsigned short* pooled, hipStream_t s)
@@ -376,9 +458,12 @@ bool conv_pair_supported(int ci, int co)
 }
 }  // namespace
 
-// Bytes of the swizzled BF16 weights of a (ci -> co) layer.
+// Bytes of the swizzled BF16 weights of a (ci -> co) layer (for one output
+// channel: BF16 pairs [27][ci / 2] for the packed-dot kernel).
 WA_API size_t warp_amd_conv3_mfma_weight_size(int cin, int cout)
 {
+    if (cout == 1 && cin >= 16)
+        return sizeof(unsigned) * 27 * (size_t)(cin / 2);
     return sizeof(short) * 4 * 64 * (size_t)wa::tap_loops(cin) * wa::k_blocks(cin) * wa::c_blocks(cout);
 }
 
@@ -388,6 +473,14 @@ WA_API size_t warp_amd_conv3_mfma_weight_size(int cin, int cout)
 WA_API int warp_amd_conv3_mfma_swizzle(int cin, int cout, int transpose_flip, const float* W, void* wsw, void* stream)
 {
     WA_REQUIRE(conv_pair_supported(cin, cout), "conv3_mfma: unsupported channel pair %d -> %d", cin, cout);
+    if (cout == 1 && cin >= 16) {
+        const int pairs = 27 * (cin / 2);
+        wa::pack_co1_kernel<<<(unsigned)wa::ceil_div(pairs, 256), 256, 0, wa::as_stream(stream)>>>(
+            cin, transpose_flip, W, static_cast<unsigned*>(wsw)
+        );
+        WA_LAUNCH_CHECK();
+        return wa::kOk;
+    }
     const int total = (int)(warp_amd_conv3_mfma_weight_size(cin, cout) / 8);
     wa::swizzle_kernel<<<(unsigned)wa::ceil_div(total, 256), 256, 0, wa::as_stream(stream)>>>(
         cin, cout, transpose_flip, W, static_cast<wa::bf16x4*>(wsw), total
@@ -425,7 +518,10 @@ WA_API int warp_amd_conv3_mfma(int m, int cin, int cout, int in_mode, const void
     WA_CONV_CASE(64, 64)
     WA_CONV_CASE(64, 32)
     WA_CONV_CASE(32, 16)
-    WA_CONV_CASE(16, 1)
+    if (cin == 16 && cout == 1) {
+        WA_REQUIRE(flags & wa::kOutF32, "conv3_mfma: a one-channel output is FP32");
+        return wa::launch_co1<16>(m, in_mode, xi, ci, wsw, bias, flags, yf, s);
+    }
     WA_CONV_CASE(16, 16)
     WA_CONV_CASE(1, 16)
     WA_CONV_CASE(16, 2)

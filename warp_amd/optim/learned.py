@@ -565,6 +565,31 @@ class EnergyTrainer:
 
     def step(self, system: FV7System, r32: torch.Tensor, e32: torch.Tensor, lr: float) -> torch.Tensor:
         """One Adam step on the pair (r, e) of ``system``; returns the loss (device FP64 view)."""
+        loss = self.gradient(system, r32, e32)
+        self.net.adam_step(lr)
+        return loss
+
+    def batch_step(self, batch: list, lr: float) -> torch.Tensor:
+        """One Adam step on the mean gradient of [(system, r, e), ...]; returns the mean loss (device)."""
+        if len(batch) == 1:
+            return self.step(*batch[0], lr)
+        s = self._s()
+        if getattr(self, "_gacc", None) is None:
+            self._gacc = {k: torch.zeros_like(v) for k, v in self.net.grads.items()}
+        losses = []
+        w = 1.0 / len(batch)
+        for j, (system, r32, e32) in enumerate(batch):
+            losses.append(self.gradient(system, r32, e32).clone())
+            for k, g in self.net.grads.items():
+                _native.call("warp_amd_axpby_f32", g.numel(), w, _native.ptr(g), 0.0 if j == 0 else 1.0,
+                             _native.ptr(self._gacc[k]), s)
+        for k, g in self.net.grads.items():
+            g.copy_(self._gacc[k])
+        self.net.adam_step(lr)
+        return torch.cat(losses).mean().reshape(1)
+
+    def gradient(self, system: FV7System, r32: torch.Tensor, e32: torch.Tensor) -> torch.Tensor:
+        """Loss of the pair (device FP64 view) and its parameter gradients in net.grads."""
         s = self._s()
         A = system.A
         self.r.copy_(r32.reshape(-1))
@@ -586,7 +611,6 @@ class EnergyTrainer:
         _native.call("warp_amd_scale_to_f32", self.N, _native.ptr(self.g), self._sp(1), 1.0 / system.dmean,
                      _native.ptr(self.dout), s)
         self.net.backward(self.dout)
-        self.net.adam_step(lr)
         return self.sc[13:14]
 
     def loss(self, system: FV7System, r32: torch.Tensor, e32: torch.Tensor) -> float:
@@ -605,19 +629,18 @@ class EnergyTrainer:
         return float(self.sc[13].item())
 
     def train(self, systems: list, pairs: list, steps: int, lr: float = 1e-3, lr_final: float = 1e-5, seed: int = 0,
-              log=None, log_every: int = 250) -> list[float]:
-        """Adam with cosine decay over ``steps`` steps on random pairs.
+              log=None, log_every: int = 250, batch: int = 1) -> list[float]:
+        """Adam with cosine decay over ``steps`` steps, each on ``batch`` random pairs.
 
-        pairs: list of (system index, r32, e32).  Returns the running loss at
+        pairs: list of (system index, r32, e32).  Returns the mean loss at
         each log point."""
         rng = np.random.default_rng(seed)
         hist = []
         acc = []
         for step in range(steps):
-            i = int(rng.integers(len(pairs)))
-            si, r32, e32 = pairs[i]
             cur = lr_final + 0.5 * (lr - lr_final) * (1 + math.cos(math.pi * step / max(steps - 1, 1)))
-            acc.append(self.step(systems[si], r32, e32, cur))
+            chosen = [pairs[int(i)] for i in rng.integers(len(pairs), size=batch)]
+            acc.append(self.batch_step([(systems[si], r32, e32) for si, r32, e32 in chosen], cur))
             if (step + 1) % log_every == 0 or step == steps - 1:
                 mean = float(torch.cat(acc).mean().item())
                 acc = []
@@ -630,7 +653,8 @@ class EnergyTrainer:
 def train_learned_preconditioner(net: UNet, systems: list, smooth_steps: int = 1, omega: float = 2.0 / 3.0,
                                  pairs_per_solve: int = 10, steps: int = 2000, lr: float = 1e-3,
                                  dagger_rounds: int = 2, dagger_steps: int = 1000, dagger_lr: float = 5e-4,
-                                 dagger_iters: int = 40, dagger_every: int = 8, seed: int = 0, log=None) -> dict:
+                                 dagger_iters: int = 40, dagger_every: int = 8, seed: int = 0, log=None,
+                                 batch: int = 1) -> dict:
     """Train ``net`` as the preconditioner of :class:`LearnedFCG` on ``systems``.
 
     1. (r, e) pairs from one FP64 Jacobi-CG solve per system
@@ -652,7 +676,7 @@ def train_learned_preconditioner(net: UNet, systems: list, smooth_steps: int = 1
     n_pcg = len(pairs)
     if log:
         log(f"[train] {n_pcg} plain-CG pairs from {len(systems)} systems in {time.time() - t0:.1f}s")
-    hist = trainer.train(systems, pairs, steps, lr=lr, seed=seed, log=log)
+    hist = trainer.train(systems, pairs, steps, lr=lr, seed=seed, log=log, batch=batch)
     n_fcg = 0
     for rd in range(dagger_rounds):
         new = []
@@ -663,7 +687,7 @@ def train_learned_preconditioner(net: UNet, systems: list, smooth_steps: int = 1
         n_fcg += len(new)
         if log:
             log(f"[train] round {rd}: +{len(new)} FCG pairs ({len(pairs)} in total)")
-        hist += trainer.train(systems, pairs, dagger_steps, lr=dagger_lr, seed=seed + 1 + rd, log=log)
+        hist += trainer.train(systems, pairs, dagger_steps, lr=dagger_lr, seed=seed + 1 + rd, log=log, batch=batch)
     return dict(pcg_pairs=n_pcg, fcg_pairs=n_fcg, loss_history=hist, seconds=time.time() - t0)
 
 
@@ -688,7 +712,7 @@ class LearnedFCG:
     """
 
     def __init__(self, system: FV7System, net: UNet, smooth_steps: int = 1, omega: float = 2.0 / 3.0,
-                 gain: float | None = None, precision: str = "bf16"):
+                 gain: float | None = None, precision: str = "bf16", fused: bool | None = None):
         if precision not in ("bf16", "fp32"):
             raise ValueError("precision must be 'bf16' or 'fp32'")
         self.sys = system
@@ -719,6 +743,11 @@ class LearnedFCG:
         self.dot_bytes = lib.warp_amd_dot_workspace_size(N)
         self.ws_dot = torch.zeros(self.dot_bytes, dtype=torch.uint8, device=dev)
         self.dinv = system.dinv
+        # one sweep each side, BF16 network: the fused kernels (16 launches per iteration)
+        self.fused = (self.nu == 1 and precision == "bf16") if fused is None else bool(fused)
+        if self.fused and (self.nu != 1 or precision != "bf16"):
+            raise ValueError("the fused iteration needs smooth_steps=1 and precision='bf16'")
+        self.ws_fused = torch.empty(lib.warp_amd_fcg_fused_workspace_size(N), dtype=torch.uint8, device=dev)
         self._graph = None
 
     def _s(self):
@@ -772,7 +801,29 @@ class LearnedFCG:
             self._sweep(self.z, self.zt)
             self.z, self.zt = self.zt, self.z
 
+    def _iteration_fused(self, cur: int) -> None:
+        prev = 1 - cur
+        A = self.sys.A
+        s = self._s()
+        P = _native.ptr
+        mat = (A.nrow, P(A.offsets), P(A.columns), P(A.values), int(A.nnz))
+        ws = (P(self.ws_fused), self.ws_fused.numel(), s)
+        _native.call("warp_amd_fcg_presmooth", *mat, P(self.r), P(self.dinv), self.omega, P(self.r1), self._sp(7),
+                     self._sp(1), self._sp(2), *ws)
+        _native.call("warp_amd_unet_pack_input_bf16", self.N, P(self.r1), P(self.sys.logk_std), self._sp(2),
+                     P(self.inp), s)
+        out = self.net.forward_bf16(self.inp)
+        _native.call("warp_amd_fcg_postsmooth", *mat, P(self.r), P(self.dinv), self.omega, P(out), self._sp(1),
+                     self.gain, P(self.q[prev]), P(self.z), self._sp(3), *ws)
+        _native.call("warp_amd_fcg_direction_spmv", *mat, P(self.z), P(self.p[prev]), self._sp(3),
+                     self._sp(4 + prev), P(self.r), P(self.p[cur]), P(self.q[cur]), self._sp(4 + cur), self._sp(6), *ws)
+        _native.call("warp_amd_fcg_update", self.N, self._sp(6), self._sp(4 + cur), P(self.p[cur]), P(self.q[cur]),
+                     P(self.x), P(self.r), self._sp(0), P(self.ws_up), self.ws_up.numel(), s)
+
     def iteration(self, cur: int) -> None:
+        if self.fused:
+            self._iteration_fused(cur)
+            return
         prev = 1 - cur
         A = self.sys.A
         s = self._s()
@@ -985,6 +1036,11 @@ def main(argv=None) -> int:
     t.add_argument("--dagger-steps", type=int, default=2000)
     t.add_argument("--smooth-steps", type=int, default=1)
     t.add_argument("--init-seed", type=int, default=0)
+    t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--dagger-lr", type=float, default=5e-4)
+    t.add_argument("--batch", type=int, default=1)
+    t.add_argument("--dagger-iters", type=int, default=40)
+    t.add_argument("--dagger-every", type=int, default=8)
     t.add_argument("--eval-seeds", type=int, default=5)
     t.add_argument("--out", required=True)
     e = sub.add_parser("evaluate", help="learned FCG vs Jacobi-CG on held-out seeds")
@@ -1009,10 +1065,14 @@ def main(argv=None) -> int:
         info = train_learned_preconditioner(net, systems, smooth_steps=args.smooth_steps,
                                             pairs_per_solve=args.pairs_per_solve, steps=args.steps,
                                             dagger_rounds=args.dagger_rounds, dagger_steps=args.dagger_steps,
-                                            seed=args.init_seed, log=log)
+                                            seed=args.init_seed, log=log, lr=args.lr, dagger_lr=args.dagger_lr,
+                                            batch=args.batch, dagger_iters=args.dagger_iters,
+                                            dagger_every=args.dagger_every)
         meta = dict(n=args.n, sigma=args.sigma, corr=args.corr, train_seed0=args.seed0, train_seeds=args.train_seeds,
                     smooth_steps=args.smooth_steps, omega=2.0 / 3.0, steps=args.steps, dagger_rounds=args.dagger_rounds,
-                    dagger_steps=args.dagger_steps, init_seed=args.init_seed, pcg_pairs=info["pcg_pairs"],
+                    dagger_steps=args.dagger_steps, init_seed=args.init_seed, lr=args.lr, dagger_lr=args.dagger_lr,
+                    batch=args.batch, dagger_iters=args.dagger_iters, dagger_every=args.dagger_every,
+                    pcg_pairs=info["pcg_pairs"],
                     fcg_pairs=info["fcg_pairs"], train_seconds=info["seconds"])
         save_checkpoint(args.out, net, meta)
         log(f"[train] wrote {args.out} after {time.time() - t0:.0f}s")
