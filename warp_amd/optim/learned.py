@@ -5,14 +5,21 @@ preconditioners only, warp/_src/optim/linear.py:236-280).  Components:
 
 * :class:`UNet` -- 4-level 3D U-Net on an n^3 cell grid (channels 16 / 32 /
   64 / 64, 3x3x3 convolutions, ReLU, 2x average pooling, nearest upsampling
-  with additive skips, linear 1-channel output).  Forward, backward and Adam
-  run in the library's hand-written kernels; weights are FP32 masters.
-* :func:`generate_pairs` -- training pairs (normalised residual, error) taken
-  from FP64 CG solves of the operator family.
-* :func:`train` -- Adam on the mean-squared error of the predicted error.
+  with additive skips, linear 1-channel output).  FP32 masters with FP32
+  forward / backward / Adam kernels for training; BF16 MFMA kernels for
+  inference.
+* :class:`FV7System` -- the operator family -div(k grad u) = f, 7-point
+  finite volumes, k = exp(Gaussian random field), FP32 matrix.
+* :class:`LearnedFCG` -- flexible CG (FP64 vectors) whose preconditioner is
+  Jacobi smoothing around the network correction; graph-replayed, with a
+  stagnation fallback and an FP64 true-residual check.
+* :class:`EnergyTrainer`, :func:`train_learned_preconditioner` -- training
+  on the one-step energy reduction 1 - (r.z)^2 / ((r.e)(z.Az)) of the
+  preconditioned direction, first on (r, e) pairs from plain FP64 Jacobi-CG
+  solves, then on pairs from the learned FCG's own iterates.
 
-The network input is (r / rms(r), standardised log k) and its output is
-scaled back by rms(r), so the preconditioner is scale-equivariant.
+The network sees (r1 / rms(r1), standardised log k) and its output is scaled
+back by rms(r1), so the preconditioner is scale-equivariant.
 """
 
 from __future__ import annotations
@@ -352,79 +359,6 @@ class FV7System:
             _native.ptr(A.values), int(A.nnz), _native.ptr(x), _native.ptr(self.b), _native.ptr(r), None,
             _native.ptr(rr), _native.ptr(ws), ws.numel(), _native.stream_of(self.device),
         )
-
-
-def generate_pairs(system: FV7System, count: int = 10, tol: float = 1e-10, maxiter: int = 20000):
-    """(input, target) pairs from one FP64 Jacobi-CG solve of ``system``.
-
-    Snapshots x_k are taken at ``count`` iterations spread geometrically over
-    the solve; with x* the converged solution, e_k = x* - x_k and
-    r_k = b - A x_k (FP64).  Input = (r_k / rms(r_k), standardised log k),
-    target = e_k / rms(e_k).  Returns a list of (inp [n^3*2] f32, target [n^3] f32).
-    """
-    from .. import sparse
-    from .. import types as wt
-    from . import linear
-
-    A32 = system.A
-    dev = system.device
-    N = A32.nrow
-    A64 = sparse.bsr_zeros(N, N, wt.float64, device=dev)
-    A64.offsets, A64.columns, A64.values, A64.nnz = A32.offsets, A32.columns, A32.values.to(torch.float64), A32.nnz
-    M = linear.preconditioner(A64, "diag")
-    x = torch.zeros(N, dtype=torch.float64, device=dev)
-    # first pass: iteration count of the full solve
-    it_total, _, _ = linear.cg(A64, system.b, x, tol=tol, atol=0.0, maxiter=maxiter, M=M, check_every=50)
-    targets_it = sorted({max(1, int(round(it_total ** (j / count)))) for j in range(1, count + 1)})
-    targets_it = [t for t in targets_it if t < it_total][:count]
-    x_star = x.clone()
-    snaps = {}
-
-    def cb(it, err, atol):
-        if it in targets_it and it not in snaps:
-            snaps[it] = xw.clone()
-
-    xw = torch.zeros(N, dtype=torch.float64, device=dev)
-    linear.cg(A64, system.b, xw, tol=tol, atol=0.0, maxiter=max(targets_it) + 1 if targets_it else 1, M=M,
-              check_every=1, callback=cb)
-    r = torch.empty(N, dtype=torch.float64, device=dev)
-    rr = torch.zeros(1, dtype=torch.float64, device=dev)
-    ws = torch.empty(_native.lib().warp_amd_bsr_residual_workspace_size(N), dtype=torch.uint8, device=dev)
-    pairs = []
-    for it in sorted(snaps):
-        xk = snaps[it]
-        system.residual(xk, r, rr, ws)
-        e = x_star - xk
-        rs = torch.sqrt(rr / N).item()
-        es = torch.sqrt(torch.mean(e * e)).item()
-        if rs <= 0 or es <= 0:
-            continue
-        inp = torch.stack([(r / rs).to(torch.float32), system.logk_std], dim=1).reshape(-1).contiguous()
-        pairs.append((inp, (e / es).to(torch.float32).contiguous()))
-    return pairs
-
-
-def train(net: UNet, pairs: list, epochs: int = 3, lr: float = 1e-3, lr_final: float = 1e-4, seed: int = 0,
-          log=None) -> list[float]:
-    """Adam on the per-pair MSE of the predicted (normalised) error; returns mean loss per epoch."""
-    rng = np.random.default_rng(seed)
-    history = []
-    steps = epochs * len(pairs)
-    step = 0
-    for ep in range(epochs):
-        order = rng.permutation(len(pairs))
-        losses = []
-        for i in order:
-            # cosine decay from lr to lr_final
-            cur = lr_final + 0.5 * (lr - lr_final) * (1 + math.cos(math.pi * step / max(steps - 1, 1)))
-            inp, tgt = pairs[i]
-            losses.append(net.train_step(inp, tgt, lr=cur))
-            step += 1
-        mean = float(torch.stack(losses).mean().item())
-        history.append(mean)
-        if log:
-            log(f"[train] epoch {ep}: mean loss {mean:.5f}")
-    return history
 
 
 def generate_energy_pairs(system: FV7System, count: int = 10, tol: float = 1e-10, maxiter: int = 20000,
