@@ -173,15 +173,23 @@ __global__ __launch_bounds__(kBlock) void conv3_mfma_kernel(
     constexpr int TL = tap_loops(CI);
     constexpr int SW = lds_stride_words(CI);
     static_assert(CB % CBG == 0, "channel blocks per workgroup must divide the channel blocks");
-    const int cb0 = blockIdx.z * CBG;
     __shared__ unsigned tile[kHalo * SW];
 
     const int tiles_x = (m + kTX - 1) / kTX;
-    const int X0 = (blockIdx.x % tiles_x) * kTX;
-    const int Y0 = (blockIdx.x / tiles_x) * kTY;
-    const int Z0 = blockIdx.y * kTZ;
+    const int tiles_xy = tiles_x * tiles_x;
+    const int nsplit = CB / CBG;
+    const int ntiles = tiles_xy * (m / kTZ) * nsplit;
     const int mh = m / 2;
+    // persistent loop over tiles; the channel split varies fastest so that
+    // concurrent workgroups share one input tile in L2
+    for (int tile_id = blockIdx.x; tile_id < ntiles; tile_id += gridDim.x) {
+    const int cb0 = (tile_id % nsplit) * CBG;
+    const int t2 = tile_id / nsplit;
+    const int X0 = (t2 % tiles_x) * kTX;
+    const int Y0 = ((t2 / tiles_x) % tiles_x) * kTY;
+    const int Z0 = (t2 / tiles_xy) * kTZ;
 
+    __syncthreads();  // the previous tile's LDS reads are done
     stage_tile<CI, IN>(tile, m, X0, Y0, Z0, x, coarse);
     __syncthreads();
 
@@ -255,7 +263,7 @@ __global__ __launch_bounds__(kBlock) void conv3_mfma_kernel(
         }
     }
     if (!valid)
-        return;  // whole wavefront (only when m < 8)
+        continue;  // whole wavefront (only when m < 8); still joins the next tile's barriers
 
     // epilogue: lane holds output channels 16 cb + 4 q + i of voxel (vx, vy, vz0 + g)
     const bool relu = flags & kOutRelu;
@@ -317,6 +325,23 @@ __global__ __launch_bounds__(kBlock) void conv3_mfma_kernel(
             }
         }
     }
+    }  // tiles
+}
+
+// Persistent launch: as many workgroups as can be resident at once (from the
+// occupancy of this kernel, queried once), each looping over tiles.
+template <auto Kernel, typename... Args>
+void launch_tiles(int ntiles, hipStream_t s, Args... args)
+{
+    static const int per_cu = [] {  // one cache per kernel (Kernel is a template argument)
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, Kernel, kBlock, 0) != hipSuccess || nb < 1)
+            nb = 1;
+        return nb;
+    }();
+    const int slots = per_cu * device_cu_count();
+    const unsigned grid = (unsigned)(ntiles < slots ? ntiles : slots);
+    Kernel<<<grid, kBlock, 0, s>>>(args...);
 }
 
 // One output channel (the network's last layer): MFMA would waste 15 of its
@@ -334,9 +359,12 @@ __global__ __launch_bounds__(kBlock) void conv3_co1_kernel(int m, const unsigned
     static_assert(CI >= 16 && CI % 16 == 0, "one-output kernel expects a multiple of 16 input channels");
     __shared__ unsigned tile[kHalo * SW];
     const int tiles_x = (m + kTX - 1) / kTX;
-    const int X0 = (blockIdx.x % tiles_x) * kTX;
-    const int Y0 = (blockIdx.x / tiles_x) * kTY;
-    const int Z0 = blockIdx.y * kTZ;
+    const int ntiles = tiles_x * tiles_x * (m / kTZ);
+    for (int tile_id = blockIdx.x; tile_id < ntiles; tile_id += gridDim.x) {
+    const int X0 = (tile_id % tiles_x) * kTX;
+    const int Y0 = ((tile_id / tiles_x) % tiles_x) * kTY;
+    const int Z0 = (tile_id / (tiles_x * tiles_x)) * kTZ;
+    __syncthreads();  // the previous tile's LDS reads are done
     stage_tile<CI, IN>(tile, m, X0, Y0, Z0, x, coarse);
     __syncthreads();
 
@@ -358,6 +386,7 @@ __global__ __launch_bounds__(kBlock) void conv3_co1_kernel(int m, const unsigned
         float* d = yf + (size_t)(vz * m + vy) * m + vx;
         *d = (flags & kOutAccumulate) ? *d + acc : acc;
     }
+    }  // tiles
 }
 
 // wpk[tap][c / 2] = (W(tap, c, 0), W(tap, c + 1, 0)) as a BF16 pair (low, high)
@@ -378,12 +407,12 @@ int launch_co1(int m, int in_mode, const unsigned short* x, const unsigned short
                const float* bias, int flags, float* yf, hipStream_t s)
 {
     const int txy = (m + kTX - 1) / kTX;
-    const dim3 grid((unsigned)(txy * txy), (unsigned)(m / kTZ));
+    const int ntiles = txy * txy * (m / kTZ);
     const unsigned* w = static_cast<const unsigned*>(wpk);
     if (in_mode == kInUpAdd)
-        conv3_co1_kernel<CI, kInUpAdd><<<grid, kBlock, 0, s>>>(m, x, coarse, w, bias, flags, yf);
+        launch_tiles<conv3_co1_kernel<CI, kInUpAdd>>(ntiles, s, m, x, coarse, w, bias, flags, yf);
     else
-        conv3_co1_kernel<CI, kInPlain><<<grid, kBlock, 0, s>>>(m, x, coarse, w, bias, flags, yf);
+        launch_tiles<conv3_co1_kernel<CI, kInPlain>>(ntiles, s, m, x, coarse, w, bias, flags, yf);
     WA_LAUNCH_CHECK();
     return kOk;
 }
@@ -397,20 +426,23 @@ int launch_conv(int m, int in_mode, const unsigned short* x, const unsigned shor
     const int tiles = txy * txy * (m / kTZ);
     // one channel block per workgroup while that is needed to fill the chip
     const bool split = CB > 1 && tiles < 1024;
-    const dim3 grid((unsigned)(txy * txy), (unsigned)(m / kTZ), split ? (unsigned)CB : 1u);
+    const int ntiles = tiles * (split ? CB : 1);
     const bf16x4* w = static_cast<const bf16x4*>(wsw);
     if (in_mode == kInUpAdd) {
         if constexpr (CI > 2) {
             if (split)
-                conv3_mfma_kernel<CI, CO, kInUpAdd, 1><<<grid, kBlock, 0, s>>>(m, x, coarse, w, bias, flags, y, yf, pooled);
+                launch_tiles<conv3_mfma_kernel<CI, CO, kInUpAdd, 1>>(ntiles, s, m, x, coarse, w, bias, flags, y, yf,
+                             pooled);
             else
-                conv3_mfma_kernel<CI, CO, kInUpAdd, CB><<<grid, kBlock, 0, s>>>(m, x, coarse, w, bias, flags, y, yf, pooled);
+                launch_tiles<conv3_mfma_kernel<CI, CO, kInUpAdd, CB>>(ntiles, s, m, x, coarse, w, bias, flags, y, yf,
+                             pooled);
         }
     } else {
         if (split)
-            conv3_mfma_kernel<CI, CO, kInPlain, 1><<<grid, kBlock, 0, s>>>(m, x, coarse, w, bias, flags, y, yf, pooled);
+            launch_tiles<conv3_mfma_kernel<CI, CO, kInPlain, 1>>(ntiles, s, m, x, coarse, w, bias, flags, y, yf, pooled);
         else
-            conv3_mfma_kernel<CI, CO, kInPlain, CB><<<grid, kBlock, 0, s>>>(m, x, coarse, w, bias, flags, y, yf, pooled);
+            launch_tiles<conv3_mfma_kernel<CI, CO, kInPlain, CB>>(ntiles, s, m, x, coarse, w, bias, flags, y, yf,
+                         pooled);
     }
     WA_LAUNCH_CHECK();
     return kOk;
