@@ -976,6 +976,7 @@ def main(argv=None) -> int:
     t.add_argument("--dagger-iters", type=int, default=40)
     t.add_argument("--dagger-every", type=int, default=8)
     t.add_argument("--eval-seeds", type=int, default=5)
+    t.add_argument("--eval-seed0", type=int, default=4000, help="validation seeds (held out from training and test)")
     t.add_argument("--out", required=True)
     e = sub.add_parser("evaluate", help="learned FCG vs Jacobi-CG on held-out seeds")
     e.add_argument("--ckpt", required=True)
@@ -1012,7 +1013,7 @@ def main(argv=None) -> int:
         log(f"[train] wrote {args.out} after {time.time() - t0:.0f}s")
         del systems
         if args.eval_seeds:
-            evaluate(net, meta, range(5000, 5000 + args.eval_seeds), dev, log=log)
+            evaluate(net, meta, range(args.eval_seed0, args.eval_seed0 + args.eval_seeds), dev, log=log)
         return 0
     net, meta = load_checkpoint(args.ckpt, args.n, dev)
     res = evaluate(net, meta, range(args.seed0, args.seed0 + args.seeds), dev, log=log)
