@@ -80,10 +80,19 @@ def copy_calibration(dev) -> int:
 
     src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device=dev)
     dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
     for _ in range(COPY_REPS):
         dst.copy_(src)
+    e1.record()
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_read": COPY_BYTES, "copy_bytes_written": COPY_BYTES, "reps": COPY_REPS}), flush=True)
+    ms = e0.elapsed_time(e1) / COPY_REPS
+    print(json.dumps({"copy_bytes_read": COPY_BYTES, "copy_bytes_written": COPY_BYTES, "reps": COPY_REPS + 1,
+                      "ms_per_copy": round(ms, 4), "GBps_read_plus_write": round(2 * COPY_BYTES / ms / 1e6, 1)}),
+          flush=True)
     return 0
 
 

@@ -31,6 +31,7 @@ namespace {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kBlock = 256;  // 4 wavefronts, one voxel cube each
@@ -93,6 +94,23 @@ __device__ __forceinline__ bf16x4 zero4()
 
 __device__ __forceinline__ bf16x4 load4(const unsigned short* p) { return *reinterpret_cast<const bf16x4*>(p); }
 
+__device__ __forceinline__ bf16x8 zero8()
+{
+    bf16x8 z = {0, 0, 0, 0, 0, 0, 0, 0};
+    return z;
+}
+
+__device__ __forceinline__ bf16x8 load8(const unsigned short* p) { return *reinterpret_cast<const bf16x8*>(p); }
+
+__device__ __forceinline__ bf16x8 add8(bf16x8 a, bf16x8 b)
+{
+    bf16x8 r;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        r[j] = (short)f2bf(bf2f((unsigned short)a[j]) + bf2f((unsigned short)b[j]));
+    return r;
+}
+
 __device__ __forceinline__ bf16x4 add4(bf16x4 a, bf16x4 b)
 {
     bf16x4 r;
@@ -113,9 +131,10 @@ constexpr int kTX = 8, kTY = 8, kTZ = 4;
 constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2;
 constexpr int kHalo = kHX * kHY * kHZ;
 
-// LDS words (4 bytes) per halo voxel: CI channels of BF16, plus 2 words of
-// padding for CI >= 16 to spread the 8-byte reads over the banks.
-__host__ __device__ constexpr int lds_stride_words(int ci) { return ci <= 2 ? 1 : ci / 2 + 2; }
+// LDS words (4 bytes) per halo voxel: CI channels of BF16, plus 4 words of
+// padding for CI >= 16 to spread the 8-byte reads over the banks (a
+// multiple of 4 words keeps the 16-byte staging stores aligned).
+__host__ __device__ constexpr int lds_stride_words(int ci) { return ci <= 2 ? 1 : ci / 2 + 4; }
 
 // Stage the 10 x 10 x 6 input halo of the tile at (X0, Y0, Z0) in LDS
 // (zeros outside the grid; with kInUpAdd the input is x + upsample(coarse),
@@ -127,30 +146,61 @@ __device__ __forceinline__ void stage_tile(unsigned* tile, int m, int X0, int Y0
 {
     constexpr int SW = lds_stride_words(CI);
     const int mh = m / 2;
+    // All loads of a batch are issued before any is used: the halo is ~10
+    // loads per thread, and issuing them one round trip at a time made the
+    // staging, not the MFMAs, the cost of the layer.
     if constexpr (CI <= 2) {
-        for (int hv = threadIdx.x; hv < kHalo; hv += kBlock) {
+        constexpr int ITERS = (kHalo + kBlock - 1) / kBlock;
+        unsigned v[ITERS];
+#pragma unroll
+        for (int u = 0; u < ITERS; ++u) {
+            const int hv = u * kBlock + threadIdx.x;
             const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
             const int gx = X0 - 1 + hx, gy = Y0 - 1 + hy, gz = Z0 - 1 + hz;
-            unsigned v = 0;
-            if (gx >= 0 && gx < m && gy >= 0 && gy < m && gz >= 0 && gz < m) {
+            v[u] = 0;
+            if (hv < kHalo && gx >= 0 && gx < m && gy >= 0 && gy < m && gz >= 0 && gz < m) {
                 const size_t g = (size_t)(gz * m + gy) * m + gx;
-                v = CI == 2 ? reinterpret_cast<const unsigned*>(x)[g] : (unsigned)x[g];
+                v[u] = CI == 2 ? reinterpret_cast<const unsigned*>(x)[g] : (unsigned)x[g];
             }
-            tile[hv] = v;
+        }
+#pragma unroll
+        for (int u = 0; u < ITERS; ++u) {
+            const int hv = u * kBlock + threadIdx.x;
+            if (hv < kHalo)
+                tile[hv] = v[u];
         }
     } else {
-        constexpr int QPV = CI / 4;  // 8-byte channel quads per voxel
-        for (int c = threadIdx.x; c < kHalo * QPV; c += kBlock) {
-            const int hv = c / QPV, qd = c % QPV;
-            const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
-            const int gx = X0 - 1 + hx, gy = Y0 - 1 + hy, gz = Z0 - 1 + hz;
-            bf16x4 v = zero4();
-            if (gx >= 0 && gx < m && gy >= 0 && gy < m && gz >= 0 && gz < m) {
-                v = load4(x + ((size_t)(gz * m + gy) * m + gx) * CI + 4 * qd);
-                if constexpr (IN == kInUpAdd)
-                    v = add4(v, load4(coarse + ((size_t)((gz >> 1) * mh + (gy >> 1)) * mh + (gx >> 1)) * CI + 4 * qd));
+        constexpr int QPV = CI / 8;  // 16-byte channel octets per voxel
+        constexpr int TOTAL = kHalo * QPV;
+        constexpr int ITERS = (TOTAL + kBlock - 1) / kBlock;
+        constexpr int BATCH = ITERS < 4 ? ITERS : 4;
+        for (int base = 0; base < ITERS; base += BATCH) {
+            bf16x8 v[BATCH];
+            bf16x8 cv[BATCH];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                const int c = (base + u) * kBlock + threadIdx.x;
+                const int hv = c / QPV, od = c % QPV;
+                const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
+                const int gx = X0 - 1 + hx, gy = Y0 - 1 + hy, gz = Z0 - 1 + hz;
+                v[u] = zero8();
+                cv[u] = zero8();
+                if (base + u < ITERS && c < TOTAL && gx >= 0 && gx < m && gy >= 0 && gy < m && gz >= 0 && gz < m) {
+                    v[u] = load8(x + ((size_t)(gz * m + gy) * m + gx) * CI + 8 * od);
+                    if constexpr (IN == kInUpAdd)
+                        cv[u] = load8(coarse + ((size_t)((gz >> 1) * mh + (gy >> 1)) * mh + (gx >> 1)) * CI + 8 * od);
+                }
             }
-            *reinterpret_cast<bf16x4*>(&tile[hv * SW + 2 * qd]) = v;
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                const int c = (base + u) * kBlock + threadIdx.x;
+                if (base + u < ITERS && c < TOTAL) {
+                    bf16x8 w = v[u];
+                    if constexpr (IN == kInUpAdd)
+                        w = add8(w, cv[u]);
+                    *reinterpret_cast<bf16x8*>(&tile[(c / QPV) * SW + 4 * (c % QPV)]) = w;
+                }
+            }
         }
     }
 }
