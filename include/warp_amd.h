@@ -125,6 +125,19 @@ int warp_amd_bsr_mv_f64(
     const int* columns, const double* values, int64_t nnz_hint, const double* x, double* y, double alpha,
     double beta, void* stream);
 
+/* y = alpha * A^T x + beta * y (replaces bsr_mv with transpose=True and its
+ * bsr_mv_transpose_kernel, warp/_src/sparse.py:4718-4747, 4880-4895).  Shapes
+ * are A's: x has nrow * block_rows entries, y has ncol * block_cols.  Like
+ * the reference, an atomic scatter: the last bits of y vary between runs. */
+int warp_amd_bsr_mv_transpose_f32(
+    int nrow, int ncol, int block_rows, int block_cols, const int* offsets, const int* row_counts,
+    const int* columns, const float* values, int64_t nnz_hint, const float* x, float* y, float alpha, float beta,
+    void* stream);
+int warp_amd_bsr_mv_transpose_f64(
+    int nrow, int ncol, int block_rows, int block_cols, const int* offsets, const int* row_counts,
+    const int* columns, const double* values, int64_t nnz_hint, const double* x, double* y, double alpha,
+    double beta, void* stream);
+
 /* Same with explicit kernel shape: `lanes` per block row (power of two in
  * [1, 64]; the reference's bsr_mv `tile_size` plays this role) and
  * `rows_per_group` consecutive rows per lane group (1, 2 or 4); <= 0 picks the

@@ -207,7 +207,7 @@ def _dense_of(A):
     "bshape,dtype", [((1, 1), torch.float32), ((1, 3), torch.float32), ((3, 3), torch.float64), ((3, 3), torch.float32), ((3, 2), torch.float32)]
 )
 def test_bsr_mv_reference_scenario(device, bshape, dtype):
-    """make_test_bsr_mv (test_sparse.py:830-903) without the transposed product."""
+    """make_test_bsr_mv (test_sparse.py:830-903), including the transposed product y @ (alpha A)."""
     rng = np.random.default_rng(123)
     nrow, ncol, nnz = 2, 3, 6
     rows = _t(rng.integers(0, nrow, nnz).astype(np.int32), device)
@@ -227,6 +227,11 @@ def test_bsr_mv_reference_scenario(device, bshape, dtype):
         else:
             sparse.bsr_mv(A, x, y, alpha, beta, work_buffer=work)
         np.testing.assert_allclose(y.cpu().numpy().reshape(-1), ref, atol=1e-4)
+    # transposed product (test_sparse.py:878-881): x = y @ (A * alpha) = alpha A^T y
+    alpha = -1.0
+    ref = alpha * y.cpu().numpy().astype(np.float64).reshape(-1) @ D
+    xt = y.reshape(-1) @ (A * alpha)
+    np.testing.assert_allclose(xt.cpu().numpy().reshape(-1), ref, rtol=1e-4, atol=1e-4)
     # aliasing x == y on a square product A A^T
     D2 = D @ D.T
     r2, c2 = np.nonzero(np.ones((nrow, nrow)))
@@ -375,3 +380,30 @@ def test_bsr_mv_graph_capture(device):
         assert torch.equal(y, eager), f"replay {k}"
         eager2 = sparse.bsr_mv(A, x).reshape(-1)
         assert torch.equal(eager2, eager)
+
+
+@pytest.mark.parametrize("bshape,dtype", [((1, 1), torch.float32), ((3, 3), torch.float32), ((3, 3), torch.float64),
+                                          ((2, 3), torch.float64)])
+def test_bsr_mv_transpose_vs_dense(device, bshape, dtype):
+    """alpha A^T x + beta y against the dense product; the atomic scatter
+    (as in the reference) changes only the last bits between runs."""
+    rng = np.random.default_rng(17)
+    nrow, ncol, T = 300, 200, 4000
+    rows = rng.integers(0, nrow, T).astype(np.int32)
+    cols = rng.integers(0, ncol, T).astype(np.int32)
+    vals = rng.standard_normal((T, *bshape))
+    A = sparse.bsr_zeros(nrow, ncol, wa.matrix(bshape, wa.types.type_to_scalar(dtype)), device=device)
+    sparse.bsr_set_from_triplets(A, _t(rows, device), _t(cols, device), _t(vals, device, dtype))
+    D = _dense_of(A)
+    x = rng.standard_normal(nrow * bshape[0])
+    y0 = rng.standard_normal(ncol * bshape[1])
+    tol = dict(rtol=1e-4, atol=1e-4) if dtype == torch.float32 else dict(rtol=1e-10, atol=1e-10)
+    for alpha, beta in [(1.0, 0.0), (-0.5, 2.0), (0.0, 3.0), (2.0, 1.0)]:
+        y = _t(y0, device, dtype)
+        sparse.bsr_mv(A, _t(x, device, dtype), y, alpha, beta, transpose=True)
+        ref = alpha * D.T @ x + beta * y0
+        np.testing.assert_allclose(y.cpu().numpy().astype(np.float64).reshape(-1), ref, **tol)
+    # y = None allocates the result with A's column blocks
+    out = sparse.bsr_mv(A, _t(x, device, dtype), transpose=True)
+    assert tuple(out.shape) == ((ncol,) if bshape[1] == 1 else (ncol, bshape[1]))
+    np.testing.assert_allclose(out.cpu().numpy().astype(np.float64).reshape(-1), D.T @ x, **tol)

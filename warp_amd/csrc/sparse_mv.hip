@@ -279,6 +279,112 @@ int jacobi_impl(
     return kOk;
 }
 
+// ---------------------------------------------------------------------------
+// Transposed product y = alpha A^T x + beta y (reference: bsr_mv with
+// transpose=True, warp/_src/sparse.py:4718-4747, 4880-4895).  Like the
+// reference, a scatter with atomic adds: the summation order of y's entries,
+// and so their last bits, vary from run to run.
+// ---------------------------------------------------------------------------
+
+// y = beta * y (beta == 0: y = 0 without reading it)
+template <typename S>
+__global__ __launch_bounds__(kBlock) void scale_vector_kernel(int64_t n, S beta, S* __restrict__ y)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n)
+        y[i] = beta == S(0) ? S(0) : beta * y[i];
+}
+
+// G lanes per block row; each lane scatters alpha * A_b^T x_row into y.
+template <typename S, int BR, int BC, int G>
+__global__ __launch_bounds__(kBlock) void bsr_mv_transpose_kernel(
+    int nrow, const int* __restrict__ offsets, const int* __restrict__ row_counts, const int* __restrict__ columns,
+    const S* __restrict__ values, const S* __restrict__ x, S* __restrict__ y, S alpha
+)
+{
+    const int row = blockIdx.x * (kBlock / G) + threadIdx.x / G;
+    const int lane = threadIdx.x % G;
+    if (row >= nrow)
+        return;
+    const int beg = offsets[row];
+    const int end = row_end(offsets, row_counts, row);
+    S xr[BR];
+#pragma unroll
+    for (int r = 0; r < BR; ++r)
+        xr[r] = alpha * x[(size_t)row * BR + r];
+    for (int k = beg + lane; k < end; k += G) {
+        const int col = columns[k];
+        const S* blk = values + (size_t)k * (BR * BC);
+#pragma unroll
+        for (int c = 0; c < BC; ++c) {
+            S sum = S(0);
+#pragma unroll
+            for (int r = 0; r < BR; ++r)
+                sum = fma(blk[r * BC + c], xr[r], sum);
+            atomicAdd(&y[(size_t)col * BC + c], sum);
+        }
+    }
+}
+
+// Any block shape: one thread per (block row, stored block).
+template <typename S>
+__global__ __launch_bounds__(kBlock) void bsr_mv_transpose_generic_kernel(
+    int nrow, int br, int bc, const int* __restrict__ offsets, const int* __restrict__ row_counts,
+    const int* __restrict__ columns, const S* __restrict__ values, const S* __restrict__ x, S* __restrict__ y, S alpha
+)
+{
+    const int row = blockIdx.x;
+    if (row >= nrow)
+        return;
+    const int beg = offsets[row];
+    const int end = row_end(offsets, row_counts, row);
+    for (int k = beg + (int)threadIdx.x; k < end; k += kBlock) {
+        const int col = columns[k];
+        const S* blk = values + (size_t)k * br * bc;
+        for (int c = 0; c < bc; ++c) {
+            S sum = S(0);
+            for (int r = 0; r < br; ++r)
+                sum = fma(blk[r * bc + c], alpha * x[(size_t)row * br + r], sum);
+            atomicAdd(&y[(size_t)col * bc + c], sum);
+        }
+    }
+}
+
+template <typename S>
+int bsr_mv_transpose_impl(int nrow, int ncol, int br, int bc, const int* offsets, const int* row_counts,
+                          const int* columns, const S* values, int64_t nnz_hint, const S* x, S* y, S alpha, S beta,
+                          hipStream_t stream)
+{
+    WA_REQUIRE(nrow >= 0 && ncol >= 0 && br > 0 && bc > 0, "bsr_mv: bad shape");
+    const int64_t ny = (int64_t)ncol * bc;
+    if (ny > 0 && beta != S(1)) {
+        scale_vector_kernel<S><<<(unsigned)ceil_div(ny, kBlock), kBlock, 0, stream>>>(ny, beta, y);
+        WA_LAUNCH_CHECK();
+    }
+    if (alpha == S(0) || nrow == 0)
+        return kOk;
+    if (br == bc && (br == 1 || br == 3)) {
+        const int g = bsr_mv_lanes(nrow, br, bc, nnz_hint);
+#define WA_TRANSPOSE_CASE(B, G)                                                                                      \
+    if (br == B && g == G) {                                                                                           \
+        bsr_mv_transpose_kernel<S, B, B, G><<<(unsigned)ceil_div(nrow, kBlock / G), kBlock, 0, stream>>>(            \
+            nrow, offsets, row_counts, columns, values, x, y, alpha                                                  \
+        );                                                                                                           \
+        WA_LAUNCH_CHECK();                                                                                           \
+        return kOk;                                                                                                  \
+    }
+        WA_TRANSPOSE_CASE(1, 1) WA_TRANSPOSE_CASE(1, 2) WA_TRANSPOSE_CASE(1, 4) WA_TRANSPOSE_CASE(1, 8)
+        WA_TRANSPOSE_CASE(1, 16) WA_TRANSPOSE_CASE(1, 32) WA_TRANSPOSE_CASE(1, 64)
+        WA_TRANSPOSE_CASE(3, 1) WA_TRANSPOSE_CASE(3, 2) WA_TRANSPOSE_CASE(3, 4) WA_TRANSPOSE_CASE(3, 8)
+        WA_TRANSPOSE_CASE(3, 16) WA_TRANSPOSE_CASE(3, 32) WA_TRANSPOSE_CASE(3, 64)
+#undef WA_TRANSPOSE_CASE
+    }
+    bsr_mv_transpose_generic_kernel<S><<<(unsigned)nrow, kBlock, 0, stream>>>(nrow, br, bc, offsets, row_counts,
+                                                                             columns, values, x, y, alpha);
+    WA_LAUNCH_CHECK();
+    return kOk;
+}
+
 }  // namespace
 
 int bsr_mv_lanes(int nrow, int br, int bc, int64_t nnz_hint)
@@ -341,3 +447,19 @@ WA_API int warp_amd_bsr_mv_lanes(int nrow, int block_rows, int block_cols, int64
 
 WA_DEFINE_SPARSE_MV(f32, float)
 WA_DEFINE_SPARSE_MV(f64, double)
+
+// y = alpha * A^T x + beta * y for an nrow x ncol BSR matrix of
+// block_rows x block_cols blocks: x has nrow * block_rows entries, y has
+// ncol * block_cols.  Atomic scatter, as the reference: non-deterministic
+// last bits.
+#define WA_DEFINE_MV_TRANSPOSE(SUFFIX, S)                                                                            \
+    WA_API int warp_amd_bsr_mv_transpose_##SUFFIX(                                                                    \
+        int nrow, int ncol, int block_rows, int block_cols, const int* offsets, const int* row_counts,                \
+        const int* columns, const S* values, int64_t nnz_hint, const S* x, S* y, S alpha, S beta, void* stream        \
+    )                                                                                                                 \
+    {                                                                                                                 \
+        return wa::bsr_mv_transpose_impl<S>(nrow, ncol, block_rows, block_cols, offsets, row_counts, columns,       \
+                                            values, nnz_hint, x, y, alpha, beta, wa::as_stream(stream));             \
+    }
+WA_DEFINE_MV_TRANSPOSE(f32, float)
+WA_DEFINE_MV_TRANSPOSE(f64, double)

@@ -156,6 +156,12 @@ class BsrMatrix:
             return bsr_mv(self, x)
         return NotImplemented
 
+    def __rmatmul__(self, x):
+        """x @ A = A^T x (reference BsrMatrix.__rmatmul__)."""
+        if isinstance(x, torch.Tensor):
+            return bsr_mv(self, x, transpose=True)
+        return NotImplemented
+
     def __mul__(self, s):
         return _BsrScalingExpression(self, s)
 
@@ -179,6 +185,11 @@ class _BsrScalingExpression:
     def __matmul__(self, x):
         if isinstance(x, torch.Tensor):
             return bsr_mv(self, x)
+        return NotImplemented
+
+    def __rmatmul__(self, x):
+        if isinstance(x, torch.Tensor):
+            return bsr_mv(self, x, transpose=True)
         return NotImplemented
 
     def __mul__(self, s):
@@ -465,11 +476,13 @@ def bsr_mv(
     A, A_scale = _extract_matrix_and_scale(A)
     alpha = float(alpha) * A_scale
 
+    # shapes of the operator applied: A, or A^T when transposed
     if transpose:
-        raise NotImplementedError("transposed bsr_mv is not provided by warp_amd")
-
-    br, bc = A.block_shape
-    nrow, ncol = A.nrow, A.ncol
+        bc, br = A.block_shape
+        nrow, ncol = A.ncol, A.nrow
+    else:
+        br, bc = A.block_shape
+        nrow, ncol = A.nrow, A.ncol
     torch_dtype = A.scalar_type.torch_dtype
 
     if y is None:
@@ -507,7 +520,16 @@ def bsr_mv(
     _require_gpu(A.values.device, "bsr_mv")
     if torch_dtype not in (torch.float32, torch.float64):
         raise TypeError(f"bsr_mv supports float32 and float64 matrices, got {torch_dtype}")
-    fn = "warp_amd_bsr_mv_ex_f64" if torch_dtype == torch.float64 else "warp_amd_bsr_mv_ex_f32"
+    sfx = "f64" if torch_dtype == torch.float64 else "f32"
+    if transpose:
+        # atomic scatter, as the reference: the last bits of y vary between runs
+        _native.call(
+            "warp_amd_bsr_mv_transpose_" + sfx, A.nrow, A.ncol, A.block_shape[0], A.block_shape[1],
+            _native.ptr(A.offsets), _native.ptr(A.row_counts), _native.ptr(A.columns), _native.ptr(A.values),
+            int(A.nnz), _native.ptr(x_view), _native.ptr(y_view), alpha, beta, _native.stream_of(A.values.device),
+        )
+        return y
+    fn = "warp_amd_bsr_mv_ex_" + sfx
     # tile_size > 0: lanes per block row; < 0: one lane per row (row-sequential
     # sums, the reference row kernel's order); 0: chosen from the mean row length
     if tile_size > 0:
