@@ -275,10 +275,12 @@ int warp_amd_rms_scales(const double* ss, int64_t n, double* scale, double* inv_
 /* p = z - (*zq / *pq_prev) p_prev */
 int warp_amd_fcg_direction(int64_t n, const double* z, const double* p_prev, const double* zq, const double* pq_prev,
                            double* p, void* stream);
-/* alpha = *pr / *pq; x += alpha p; r -= alpha q; *rr = r.r (device, fixed-order sum) */
+/* alpha = *pr / *pq; x += alpha p; r -= alpha q; *rr = r.r (device, fixed-order
+ * sum); with w non-null also w = (float)(omega * dinv * r) */
 size_t warp_amd_fcg_update_workspace_size(int64_t n);
 int warp_amd_fcg_update(int64_t n, const double* pr, const double* pq, const double* p, const double* q, double* x,
-                        double* r, double* rr, void* workspace, size_t workspace_bytes, void* stream);
+                        double* r, double* rr, const double* dinv, double omega, float* w, void* workspace,
+                        size_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Learned preconditioner: 3D U-Net building blocks (no reference           */
@@ -329,21 +331,25 @@ int warp_amd_jacobi_sweep_f32_f64(int nrow, const int* offsets, const int* colum
 int warp_amd_unet_unpack_add(int64_t n, const float* out, const double* scale, double mult, double* z, void* stream);
 /* Fused flexible-CG steps for a scalar (1x1) compact BSR matrix with FP32
  * values and FP64 vectors, one damped Jacobi sweep before and after the
- * network correction (no reference counterpart).  Scalars are device
- * pointers; workspace >= warp_amd_fcg_fused_workspace_size(nrow).
- *   presmooth:  r1 = r - A (omega D^-1 r); *rr1 = r1.r1, *scale = rms(r1), *inv_scale = 1 / rms(r1)
- *   postsmooth: z2 = omega D^-1 r + gain * (*scale) * out; z = z2 + omega D^-1 (r - A z2); *zq = z.q_prev
+ * network correction (no reference counterpart).  w = (float)(omega D^-1 r)
+ * is the first sweep, kept current by warp_amd_fcg_update; the
+ * preconditioned z is FP32.  Scalars are device pointers; workspace >=
+ * warp_amd_fcg_fused_workspace_size(nrow).
+ *   weight:     w = (float)(omega D^-1 r)
+ *   presmooth:  r1 = r - A w; *rr1 = r1.r1, *scale = rms(r1), *inv_scale = 1 / rms(r1)
+ *   postsmooth: z2 = w + gain * (*scale) * out; z = (float)(z2 + omega D^-1 (r - A z2)); *zq = z.q_prev
  *   direction_spmv: p = z - (*zq / *pq_prev) p_prev; q = A p; *pq = p.q; *pr = p.r */
 size_t warp_amd_fcg_fused_workspace_size(int nrow);
+int warp_amd_fcg_weight(int64_t n, const double* dinv, const double* r, double omega, float* w, void* stream);
 int warp_amd_fcg_presmooth(int nrow, const int* offsets, const int* columns, const float* values, int64_t nnz_hint,
-                           const double* r, const double* dinv, double omega, double* r1, double* rr1, double* scale,
-                           double* inv_scale, void* workspace, size_t workspace_bytes, void* stream);
+                           const double* r, const float* w, double* r1, double* rr1, double* scale, double* inv_scale,
+                           void* workspace, size_t workspace_bytes, void* stream);
 int warp_amd_fcg_postsmooth(int nrow, const int* offsets, const int* columns, const float* values, int64_t nnz_hint,
-                            const double* r, const double* dinv, double omega, const float* out, const double* scale,
-                            double gain, const double* q_prev, double* z, double* zq, void* workspace,
-                            size_t workspace_bytes, void* stream);
+                            const double* r, const double* dinv, double omega, const float* w, const float* out,
+                            const double* scale, double gain, const double* q_prev, float* z, double* zq,
+                            void* workspace, size_t workspace_bytes, void* stream);
 int warp_amd_fcg_direction_spmv(int nrow, const int* offsets, const int* columns, const float* values,
-                                int64_t nnz_hint, const double* z, const double* p_prev, const double* zq,
+                                int64_t nnz_hint, const float* z, const double* p_prev, const double* zq,
                                 const double* pq_prev, const double* r, double* p, double* q, double* pq, double* pr,
                                 void* workspace, size_t workspace_bytes, void* stream);
 /* Training loss of the learned preconditioner (no reference counterpart): the

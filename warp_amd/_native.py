@@ -62,7 +62,9 @@ _SIGNATURES = {
     "warp_amd_rms_scales": (c_int, [c_ptr, c_i64, c_ptr, c_ptr, c_ptr]),
     "warp_amd_fcg_direction": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]),
     "warp_amd_fcg_update_workspace_size": (c_size, [c_i64]),
-    "warp_amd_fcg_update": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
+    "warp_amd_fcg_update": (
+        c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_f64, c_ptr, c_ptr, c_size, c_ptr]
+    ),
     "warp_amd_conv3_fwd_f32": (c_int, [c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_int, c_ptr]),
     "warp_amd_conv3_bwd_data_f32": (c_int, [c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr]),
     "warp_amd_conv3_bwd_weight_workspace_size": (c_size, [c_int, c_int, c_int]),
@@ -83,13 +85,14 @@ _SIGNATURES = {
     ),
     "warp_amd_unet_unpack_add": (c_int, [c_i64, c_ptr, c_ptr, c_f64, c_ptr, c_ptr]),
     "warp_amd_fcg_fused_workspace_size": (c_size, [c_int]),
+    "warp_amd_fcg_weight": (c_int, [c_i64, c_ptr, c_ptr, c_f64, c_ptr, c_ptr]),
     "warp_amd_fcg_presmooth": (
-        c_int, [c_int, c_ptr, c_ptr, c_ptr, c_i64, c_ptr, c_ptr, c_f64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]
+        c_int, [c_int, c_ptr, c_ptr, c_ptr, c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]
     ),
     "warp_amd_fcg_postsmooth": (
         c_int,
-        [c_int, c_ptr, c_ptr, c_ptr, c_i64, c_ptr, c_ptr, c_f64, c_ptr, c_ptr, c_f64, c_ptr, c_ptr, c_ptr, c_ptr,
-         c_size, c_ptr],
+        [c_int, c_ptr, c_ptr, c_ptr, c_i64, c_ptr, c_ptr, c_f64, c_ptr, c_ptr, c_ptr, c_f64, c_ptr, c_ptr, c_ptr,
+         c_ptr, c_size, c_ptr],
     ),
     "warp_amd_fcg_direction_spmv": (
         c_int,

@@ -75,50 +75,44 @@ __global__ __launch_bounds__(kBlock) void jacobi_sweep_kernel(
 // sweep results are never written and re-read.
 // ---------------------------------------------------------------------------
 
-struct PreSmoothX {  // omega D^-1 r
-    const double* __restrict__ dinv;
-    const double* __restrict__ r;
-    double omega;
-    __device__ __forceinline__ double operator()(size_t i) const { return omega * dinv[i] * r[i]; }
+struct WeightX {  // w = omega D^-1 r, kept in FP32 by the update kernel
+    const float* __restrict__ w;
+    __device__ __forceinline__ double operator()(size_t i) const { return (double)w[i]; }
 };
 
-struct CorrectedX {  // omega D^-1 r + c * out
-    const double* __restrict__ dinv;
-    const double* __restrict__ r;
+struct CorrectedX {  // z2 = w + c * out
+    const float* __restrict__ w;
     const float* __restrict__ out;
-    double omega, c;
-    __device__ __forceinline__ double operator()(size_t i) const
-    {
-        return fma(c, (double)out[i], omega * dinv[i] * r[i]);
-    }
+    double c;
+    __device__ __forceinline__ double operator()(size_t i) const { return fma(c, (double)out[i], (double)w[i]); }
 };
 
 struct DirectionX {  // z - beta p_prev
-    const double* __restrict__ z;
+    const float* __restrict__ z;
     const double* __restrict__ p_prev;
     double beta;
-    __device__ __forceinline__ double operator()(size_t i) const { return fma(-beta, p_prev[i], z[i]); }
+    __device__ __forceinline__ double operator()(size_t i) const { return fma(-beta, p_prev[i], (double)z[i]); }
 };
 
-// r1 = r - A (omega D^-1 r); partials of r1.r1
+// r1 = r - A w; partials of r1.r1
 template <int G, int RPG>
 __global__ __launch_bounds__(kBlock) void fcg_pre_kernel(int nrow, const int* __restrict__ offsets,
                                                         const int* __restrict__ columns,
                                                         const float* __restrict__ values, const double* __restrict__ r,
-                                                        const double* __restrict__ dinv, double omega,
-                                                        double* __restrict__ r1, double* __restrict__ partials)
+                                                        const float* __restrict__ w, double* __restrict__ r1,
+                                                        double* __restrict__ partials)
 {
     __shared__ double lds[kBlock / kWave];
     const int row0 = (blockIdx.x * (kBlock / G) + threadIdx.x / G) * RPG;
     const int lane = threadIdx.x % G;
     double acc[1] = {0.0};
     if (row0 < nrow) {
-        const PreSmoothX xf{dinv, r, omega};
-        const RowAcc<double, RPG> az =
-            group_rows_product_f<double, 1, 1, G, RPG, float>(row0, nrow, lane, offsets, nullptr, columns, values, xf);
+        const RowAcc<double, RPG> aw = group_rows_product_f<double, 1, 1, G, RPG, float>(
+            row0, nrow, lane, offsets, nullptr, columns, values, WeightX{w}
+        );
         for (int e = lane; e < RPG && row0 + e < nrow; e += G) {
             const size_t i = (size_t)row0 + e;
-            const double v = r[i] - pick(az, e);
+            const double v = r[i] - pick(aw, e);
             r1[i] = v;
             acc[0] += v * v;
         }
@@ -126,31 +120,31 @@ __global__ __launch_bounds__(kBlock) void fcg_pre_kernel(int nrow, const int* __
     write_block_partials<kBlock, 1>(acc, partials, lds);
 }
 
-// z2 = omega D^-1 r + gain * scale * out;  z = z2 + omega D^-1 (r - A z2);
+// z2 = w + gain * scale * out;  z = z2 + omega D^-1 (r - A z2) (stored FP32);
 // partials of z.q_prev
 template <int G, int RPG>
 __global__ __launch_bounds__(kBlock) void fcg_post_kernel(int nrow, const int* __restrict__ offsets,
                                                          const int* __restrict__ columns,
                                                          const float* __restrict__ values, const double* __restrict__ r,
                                                          const double* __restrict__ dinv, double omega,
-                                                         const float* __restrict__ out, const double* scale,
-                                                         double gain, const double* __restrict__ q_prev,
-                                                         double* __restrict__ z, double* __restrict__ partials)
+                                                         const float* __restrict__ w, const float* __restrict__ out,
+                                                         const double* scale, double gain,
+                                                         const double* __restrict__ q_prev, float* __restrict__ z,
+                                                         double* __restrict__ partials)
 {
     __shared__ double lds[kBlock / kWave];
     const int row0 = (blockIdx.x * (kBlock / G) + threadIdx.x / G) * RPG;
     const int lane = threadIdx.x % G;
     double acc[1] = {0.0};
     if (row0 < nrow) {
-        const CorrectedX xf{dinv, r, out, omega, gain * (*scale)};
+        const CorrectedX xf{w, out, gain * (*scale)};
         const RowAcc<double, RPG> az =
             group_rows_product_f<double, 1, 1, G, RPG, float>(row0, nrow, lane, offsets, nullptr, columns, values, xf);
         for (int e = lane; e < RPG && row0 + e < nrow; e += G) {
             const size_t i = (size_t)row0 + e;
-            const double z2 = xf(i);
-            const double v = fma(omega * dinv[i], r[i] - pick(az, e), z2);
+            const float v = (float)fma(omega * dinv[i], r[i] - pick(az, e), xf(i));
             z[i] = v;
-            acc[0] += v * q_prev[i];
+            acc[0] += (double)v * q_prev[i];
         }
     }
     write_block_partials<kBlock, 1>(acc, partials, lds);
@@ -161,7 +155,7 @@ template <int G, int RPG>
 __global__ __launch_bounds__(kBlock) void fcg_dir_spmv_kernel(int nrow, const int* __restrict__ offsets,
                                                              const int* __restrict__ columns,
                                                              const float* __restrict__ values,
-                                                             const double* __restrict__ z,
+                                                             const float* __restrict__ z,
                                                              const double* __restrict__ p_prev, const double* zq,
                                                              const double* pq_prev, const double* __restrict__ r,
                                                              double* __restrict__ p, double* __restrict__ q,
@@ -187,6 +181,16 @@ __global__ __launch_bounds__(kBlock) void fcg_dir_spmv_kernel(int nrow, const in
         }
     }
     write_block_partials<kBlock, 2>(acc, partials, lds);
+}
+
+// w = (float)(omega * dinv * r)
+__global__ __launch_bounds__(kBlock) void fcg_weight_kernel(int64_t n, const double* __restrict__ dinv,
+                                                           const double* __restrict__ r, double omega,
+                                                           float* __restrict__ w)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n)
+        w[i] = (float)(omega * dinv[i] * r[i]);
 }
 
 // out0 = sum of partials[0, nparts); with out1: out1 = sum of partials[nparts, 2 nparts);
@@ -271,9 +275,11 @@ __global__ __launch_bounds__(kBlock) void fcg_direction_kernel(
 }
 
 // alpha = pr / pq; x += alpha p; r -= alpha q; partials of r.r
+// (with w: also w = (float)(omega * dinv * r), the next pre-smoothing sweep)
 __global__ __launch_bounds__(kBlock) void fcg_update_kernel(
     int64_t n, const double* pr, const double* pq, const double* __restrict__ p, const double* __restrict__ q,
-    double* __restrict__ x, double* __restrict__ r, double* __restrict__ partials
+    double* __restrict__ x, double* __restrict__ r, const double* __restrict__ dinv, double omega,
+    float* __restrict__ w, double* __restrict__ partials
 )
 {
     __shared__ double lds[kBlock / kWave];
@@ -284,6 +290,8 @@ __global__ __launch_bounds__(kBlock) void fcg_update_kernel(
         x[i] = fma(alpha, p[i], x[i]);
         const double rv = fma(-alpha, q[i], r[i]);
         r[i] = rv;
+        if (w)
+            w[i] = (float)(omega * dinv[i] * rv);
         acc[0] = rv * rv;
     }
     write_block_partials<kBlock, 1>(acc, partials, lds);
@@ -338,31 +346,45 @@ int residual_dispatch(
 }
 
 template <int G>
-int launch_fcg_fused(int which, int nrow, const int* offsets, const int* columns, const float* values,
-                     const double* a0, const double* a1, const float* out, const double* s0, const double* s1,
-                     double omega, double gain, double* o0, double* o1, double* f0, double* f1, double* scale,
-                     double* inv_scale, double* partials, hipStream_t stream)
-{
-    constexpr int RPG = rows_per_group_for(1, 1);
-    const unsigned grid = (unsigned)ceil_div(nrow, (kBlock / G) * RPG);
-    if (which == 0) {  // a0 = r, a1 = dinv -> o0 = r1
-        fcg_pre_kernel<G, RPG><<<grid, kBlock, 0, stream>>>(nrow, offsets, columns, values, a0, a1, omega, o0, partials);
+struct FcgLaunch {
+    static constexpr int RPG = rows_per_group_for(1, 1);
+    static unsigned grid(int nrow) { return (unsigned)ceil_div(nrow, (kBlock / G) * RPG); }
+    static int pre(int nrow, const int* offsets, const int* columns, const float* values, const double* r,
+                   const float* w, double* r1, double* rr1, double* scale, double* inv_scale, double* partials,
+                   hipStream_t stream)
+    {
+        const unsigned g = grid(nrow);
+        fcg_pre_kernel<G, RPG><<<g, kBlock, 0, stream>>>(nrow, offsets, columns, values, r, w, r1, partials);
         WA_LAUNCH_CHECK();
-        finalize_fcg_kernel<<<1, kFinalizeBlock, 0, stream>>>(partials, (int)grid, f0, nullptr, nrow, scale, inv_scale);
-    } else if (which == 1) {  // a0 = r, a1 = dinv, out, s0 = scale, o1 = q_prev (read) -> o0 = z
-        fcg_post_kernel<G, RPG><<<grid, kBlock, 0, stream>>>(nrow, offsets, columns, values, a0, a1, omega, out, s0,
-                                                              gain, o1, o0, partials);
+        finalize_fcg_kernel<<<1, kFinalizeBlock, 0, stream>>>(partials, (int)g, rr1, nullptr, nrow, scale, inv_scale);
         WA_LAUNCH_CHECK();
-        finalize_fcg_kernel<<<1, kFinalizeBlock, 0, stream>>>(partials, (int)grid, f0, nullptr, nrow, nullptr, nullptr);
-    } else {  // a0 = z, a1 = p_prev, s0 = zq, s1 = pq_prev, scale = r (read) -> o0 = p, o1 = q
-        fcg_dir_spmv_kernel<G, RPG><<<grid, kBlock, 0, stream>>>(nrow, offsets, columns, values, a0, a1, s0, s1,
-                                                                  scale, o0, o1, partials);
-        WA_LAUNCH_CHECK();
-        finalize_fcg_kernel<<<1, kFinalizeBlock, 0, stream>>>(partials, (int)grid, f0, f1, nrow, nullptr, nullptr);
+        return kOk;
     }
-    WA_LAUNCH_CHECK();
-    return kOk;
-}
+    static int post(int nrow, const int* offsets, const int* columns, const float* values, const double* r,
+                    const double* dinv, double omega, const float* w, const float* out, const double* scale,
+                    double gain, const double* q_prev, float* z, double* zq, double* partials, hipStream_t stream)
+    {
+        const unsigned g = grid(nrow);
+        fcg_post_kernel<G, RPG><<<g, kBlock, 0, stream>>>(nrow, offsets, columns, values, r, dinv, omega, w, out,
+                                                           scale, gain, q_prev, z, partials);
+        WA_LAUNCH_CHECK();
+        finalize_fcg_kernel<<<1, kFinalizeBlock, 0, stream>>>(partials, (int)g, zq, nullptr, nrow, nullptr, nullptr);
+        WA_LAUNCH_CHECK();
+        return kOk;
+    }
+    static int dir(int nrow, const int* offsets, const int* columns, const float* values, const float* z,
+                   const double* p_prev, const double* zq, const double* pq_prev, const double* r, double* p, double* q,
+                   double* pq, double* pr, double* partials, hipStream_t stream)
+    {
+        const unsigned g = grid(nrow);
+        fcg_dir_spmv_kernel<G, RPG><<<g, kBlock, 0, stream>>>(nrow, offsets, columns, values, z, p_prev, zq, pq_prev,
+                                                               r, p, q, partials);
+        WA_LAUNCH_CHECK();
+        finalize_fcg_kernel<<<1, kFinalizeBlock, 0, stream>>>(partials, (int)g, pq, pr, nrow, nullptr, nullptr);
+        WA_LAUNCH_CHECK();
+        return kOk;
+    }
+};
 
 template <int G>
 int launch_sweep(int nrow, const int* offsets, const int* columns, const float* values, const double* z,
@@ -447,13 +469,15 @@ WA_API int warp_amd_fcg_direction(int64_t n, const double* z, const double* p_pr
 WA_API size_t warp_amd_fcg_update_workspace_size(int64_t n) { return sizeof(double) * (size_t)(wa::ceil_div(n, 256) + 1); }
 
 WA_API int warp_amd_fcg_update(int64_t n, const double* pr, const double* pq, const double* p, const double* q,
-                               double* x, double* r, double* rr, void* workspace, size_t workspace_bytes, void* stream)
+                               double* x, double* r, double* rr, const double* dinv, double omega, float* w,
+                               void* workspace, size_t workspace_bytes, void* stream)
 {
+    WA_REQUIRE(!w || dinv, "fcg_update: the weighted residual needs dinv");
     WA_REQUIRE(workspace && workspace_bytes >= warp_amd_fcg_update_workspace_size(n), "fcg_update: workspace too small");
     hipStream_t s = wa::as_stream(stream);
     const unsigned grid = (unsigned)wa::ceil_div(n, wa::kBlock);
     double* partials = static_cast<double*>(workspace);
-    wa::fcg_update_kernel<<<grid, wa::kBlock, 0, s>>>(n, pr, pq, p, q, x, r, partials);
+    wa::fcg_update_kernel<<<grid, wa::kBlock, 0, s>>>(n, pr, pq, p, q, x, r, dinv, omega, w, partials);
     WA_LAUNCH_CHECK();
     wa::finalize_sum_kernel<<<1, wa::kFinalizeBlock, 0, s>>>(partials, (int)grid, rr);
     WA_LAUNCH_CHECK();
@@ -542,7 +566,7 @@ WA_API size_t warp_amd_fcg_fused_workspace_size(int nrow)
     return sizeof(double) * 2 * (size_t)(wa::ceil_div(nrow > 0 ? nrow : 1, 4) + 1);
 }
 
-#define WA_FCG_DISPATCH(WHICH, ...)                                                                            \
+#define WA_FCG_DISPATCH(FN, ...)                                                                               \
     do {                                                                                                       \
         WA_REQUIRE(nrow > 0, "fcg: empty matrix");                                                             \
         WA_REQUIRE(workspace && workspace_bytes >= warp_amd_fcg_fused_workspace_size(nrow),                    \
@@ -551,43 +575,52 @@ WA_API size_t warp_amd_fcg_fused_workspace_size(int nrow)
         double* partials = static_cast<double*>(workspace);                                                    \
         hipStream_t s = wa::as_stream(stream);                                                                 \
         switch (g) {                                                                                           \
-        case 1: return wa::launch_fcg_fused<1>(WHICH, nrow, offsets, columns, values, __VA_ARGS__, partials, s); \
-        case 2: return wa::launch_fcg_fused<2>(WHICH, nrow, offsets, columns, values, __VA_ARGS__, partials, s); \
-        case 4: return wa::launch_fcg_fused<4>(WHICH, nrow, offsets, columns, values, __VA_ARGS__, partials, s); \
-        case 8: return wa::launch_fcg_fused<8>(WHICH, nrow, offsets, columns, values, __VA_ARGS__, partials, s); \
-        case 16: return wa::launch_fcg_fused<16>(WHICH, nrow, offsets, columns, values, __VA_ARGS__, partials, s); \
-        case 32: return wa::launch_fcg_fused<32>(WHICH, nrow, offsets, columns, values, __VA_ARGS__, partials, s); \
-        default: return wa::launch_fcg_fused<64>(WHICH, nrow, offsets, columns, values, __VA_ARGS__, partials, s); \
+        case 1: return wa::FcgLaunch<1>::FN(nrow, offsets, columns, values, __VA_ARGS__, partials, s);          \
+        case 2: return wa::FcgLaunch<2>::FN(nrow, offsets, columns, values, __VA_ARGS__, partials, s);          \
+        case 4: return wa::FcgLaunch<4>::FN(nrow, offsets, columns, values, __VA_ARGS__, partials, s);          \
+        case 8: return wa::FcgLaunch<8>::FN(nrow, offsets, columns, values, __VA_ARGS__, partials, s);          \
+        case 16: return wa::FcgLaunch<16>::FN(nrow, offsets, columns, values, __VA_ARGS__, partials, s);        \
+        case 32: return wa::FcgLaunch<32>::FN(nrow, offsets, columns, values, __VA_ARGS__, partials, s);        \
+        default: return wa::FcgLaunch<64>::FN(nrow, offsets, columns, values, __VA_ARGS__, partials, s);        \
         }                                                                                                      \
     } while (0)
 
-// r1 = r - A (omega D^-1 r) (one Jacobi sweep from zero, then the residual);
-// *rr1 = r1.r1, *scale = rms(r1), *inv_scale = 1 / rms(r1)  (device)
-WA_API int warp_amd_fcg_presmooth(int nrow, const int* offsets, const int* columns, const float* values,
-                                  int64_t nnz_hint, const double* r, const double* dinv, double omega, double* r1,
-                                  double* rr1, double* scale, double* inv_scale, void* workspace,
-                                  size_t workspace_bytes, void* stream)
+// w = (float)(omega D^-1 r) (the first pre-smoothing sweep; fcg_update keeps it current)
+WA_API int warp_amd_fcg_weight(int64_t n, const double* dinv, const double* r, double omega, float* w, void* stream)
 {
-    WA_FCG_DISPATCH(0, r, dinv, nullptr, nullptr, nullptr, omega, 0.0, r1, nullptr, rr1, nullptr, scale, inv_scale);
+    if (n <= 0)
+        return wa::kOk;
+    wa::fcg_weight_kernel<<<(unsigned)wa::ceil_div(n, wa::kBlock), wa::kBlock, 0, wa::as_stream(stream)>>>(
+        n, dinv, r, omega, w
+    );
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
 }
 
-// z2 = omega D^-1 r + gain * (*scale) * out;  z = z2 + omega D^-1 (r - A z2);
-// *zq = z.q_prev  (device)
-WA_API int warp_amd_fcg_postsmooth(int nrow, const int* offsets, const int* columns, const float* values,
-                                   int64_t nnz_hint, const double* r, const double* dinv, double omega,
-                                   const float* out, const double* scale, double gain, const double* q_prev,
-                                   double* z, double* zq, void* workspace, size_t workspace_bytes, void* stream)
+// r1 = r - A w; *rr1 = r1.r1, *scale = rms(r1), *inv_scale = 1 / rms(r1)  (device)
+WA_API int warp_amd_fcg_presmooth(int nrow, const int* offsets, const int* columns, const float* values,
+                                  int64_t nnz_hint, const double* r, const float* w, double* r1, double* rr1,
+                                  double* scale, double* inv_scale, void* workspace, size_t workspace_bytes,
+                                  void* stream)
 {
-    WA_FCG_DISPATCH(1, r, dinv, out, scale, nullptr, omega, gain, z, const_cast<double*>(q_prev), zq, nullptr,
-                    nullptr, nullptr);
+    WA_FCG_DISPATCH(pre, r, w, r1, rr1, scale, inv_scale);
+}
+
+// z2 = w + gain * (*scale) * out;  z = (float)(z2 + omega D^-1 (r - A z2));  *zq = z.q_prev  (device)
+WA_API int warp_amd_fcg_postsmooth(int nrow, const int* offsets, const int* columns, const float* values,
+                                   int64_t nnz_hint, const double* r, const double* dinv, double omega, const float* w,
+                                   const float* out, const double* scale, double gain, const double* q_prev, float* z,
+                                   double* zq, void* workspace, size_t workspace_bytes, void* stream)
+{
+    WA_FCG_DISPATCH(post, r, dinv, omega, w, out, scale, gain, q_prev, z, zq);
 }
 
 // p = z - (zq / pq_prev) p_prev;  q = A p;  *pq = p.q, *pr = p.r  (device)
 WA_API int warp_amd_fcg_direction_spmv(int nrow, const int* offsets, const int* columns, const float* values,
-                                       int64_t nnz_hint, const double* z, const double* p_prev, const double* zq,
+                                       int64_t nnz_hint, const float* z, const double* p_prev, const double* zq,
                                        const double* pq_prev, const double* r, double* p, double* q, double* pq,
                                        double* pr, void* workspace, size_t workspace_bytes, void* stream)
 {
-    WA_FCG_DISPATCH(2, z, p_prev, nullptr, zq, pq_prev, 0.0, 0.0, p, q, pq, pr, const_cast<double*>(r), nullptr);
+    WA_FCG_DISPATCH(dir, z, p_prev, zq, pq_prev, r, p, q, pq, pr);
 }
 #undef WA_FCG_DISPATCH

@@ -682,6 +682,8 @@ class LearnedFCG:
         if self.fused and (self.nu != 1 or precision != "bf16"):
             raise ValueError("the fused iteration needs smooth_steps=1 and precision='bf16'")
         self.ws_fused = torch.empty(lib.warp_amd_fcg_fused_workspace_size(N), dtype=torch.uint8, device=dev)
+        self.w32 = torch.zeros(N, dtype=torch.float32, device=dev)  # omega D^-1 r (fused path)
+        self.z32 = torch.zeros(N, dtype=torch.float32, device=dev)  # preconditioned residual (fused path)
         self._graph = None
 
     def _s(self):
@@ -742,17 +744,18 @@ class LearnedFCG:
         P = _native.ptr
         mat = (A.nrow, P(A.offsets), P(A.columns), P(A.values), int(A.nnz))
         ws = (P(self.ws_fused), self.ws_fused.numel(), s)
-        _native.call("warp_amd_fcg_presmooth", *mat, P(self.r), P(self.dinv), self.omega, P(self.r1), self._sp(7),
-                     self._sp(1), self._sp(2), *ws)
+        _native.call("warp_amd_fcg_presmooth", *mat, P(self.r), P(self.w32), P(self.r1), self._sp(7), self._sp(1),
+                     self._sp(2), *ws)
         _native.call("warp_amd_unet_pack_input_bf16", self.N, P(self.r1), P(self.sys.logk_std), self._sp(2),
                      P(self.inp), s)
         out = self.net.forward_bf16(self.inp)
-        _native.call("warp_amd_fcg_postsmooth", *mat, P(self.r), P(self.dinv), self.omega, P(out), self._sp(1),
-                     self.gain, P(self.q[prev]), P(self.z), self._sp(3), *ws)
-        _native.call("warp_amd_fcg_direction_spmv", *mat, P(self.z), P(self.p[prev]), self._sp(3),
+        _native.call("warp_amd_fcg_postsmooth", *mat, P(self.r), P(self.dinv), self.omega, P(self.w32), P(out),
+                     self._sp(1), self.gain, P(self.q[prev]), P(self.z32), self._sp(3), *ws)
+        _native.call("warp_amd_fcg_direction_spmv", *mat, P(self.z32), P(self.p[prev]), self._sp(3),
                      self._sp(4 + prev), P(self.r), P(self.p[cur]), P(self.q[cur]), self._sp(4 + cur), self._sp(6), *ws)
         _native.call("warp_amd_fcg_update", self.N, self._sp(6), self._sp(4 + cur), P(self.p[cur]), P(self.q[cur]),
-                     P(self.x), P(self.r), self._sp(0), P(self.ws_up), self.ws_up.numel(), s)
+                     P(self.x), P(self.r), self._sp(0), P(self.dinv), self.omega, P(self.w32), P(self.ws_up),
+                     self.ws_up.numel(), s)
 
     def iteration(self, cur: int) -> None:
         if self.fused:
@@ -770,7 +773,7 @@ class LearnedFCG:
                      self._sp(4 + cur), _native.ptr(self.ws_mv), self.ws_mv.numel(), s)
         self._dot(self.p[cur], self.r, 6)
         _native.call("warp_amd_fcg_update", self.N, self._sp(6), self._sp(4 + cur), _native.ptr(self.p[cur]),
-                     _native.ptr(self.q[cur]), _native.ptr(self.x), _native.ptr(self.r), self._sp(0),
+                     _native.ptr(self.q[cur]), _native.ptr(self.x), _native.ptr(self.r), self._sp(0), None, 0.0, None,
                      _native.ptr(self.ws_up), self.ws_up.numel(), s)
 
     def _start(self, x0: torch.Tensor | None = None) -> float:
@@ -782,11 +785,18 @@ class LearnedFCG:
         else:
             self.x.copy_(x0)
         self.sys.residual(self.x, self.r, self.sc[0:1], self.ws_mv)
+        self._weight()
         for t in self.p + self.q:
             t.zero_()
         self.sc[4] = 1.0
         self.sc[5] = 1.0
         return float(torch.linalg.vector_norm(self.sys.b).item())
+
+    def _weight(self) -> None:
+        """w = omega D^-1 r for the fused path (kept current by its update kernel)."""
+        if self.fused:
+            _native.call("warp_amd_fcg_weight", self.N, _native.ptr(self.dinv), _native.ptr(self.r), self.omega,
+                         _native.ptr(self.w32), self._s())
 
     def collect_pairs(self, x_star: torch.Tensor, iterations: int = 40, every: int = 8) -> list:
         """(r_k, x* - x_k) of this solver's own iterates from x = 0, every
@@ -849,6 +859,7 @@ class LearnedFCG:
                 break
             # converged on the recursive residual: replace it by the FP64 true residual
             self.sys.residual(self.x, self.r, self.sc[0:1], self.ws_mv)
+            self._weight()
             true_rel = math.sqrt(max(float(self.sc[0].item()), 0.0)) / bnorm
             if true_rel <= tol:
                 break
