@@ -105,6 +105,22 @@ void wp_bsr_matrix_from_triplets_device(
     bool masked_topology, int* tpl_block_offsets, int* tpl_block_indices, int* bsr_offsets,
     const int* bsr_row_counts, int* bsr_columns, int* bsr_nnz, void* bsr_nnz_event);
 
+/* Sort and coalesce the active blocks of each row in place; replaces
+ * wp_bsr_compress_inplace_device (warp/native/warp.h; host twin
+ * warp/native/sparse.cpp:521-607, per-row semantics sparse_util.h:82-146) and
+ * bsr_compress (warp/_src/sparse.py:1571-1672).  Active blocks: positions
+ * [offsets[r], offsets[r] + row_counts[r]) (row_counts NULL: up to
+ * offsets[r+1]) below T.  Negative columns are dropped, and with
+ * prune_numerical_zeros blocks whose scalars all satisfy (bits & zero_mask)
+ * == 0; duplicates are summed left to right in storage order (values NULL:
+ * topology only; fp32 / fp64).  make_compact = 1: rows packed, offsets
+ * rewritten, positions >= nnz get column -1 and zero values, *nnz_host = nnz
+ * (pinned, asynchronous); 0: offsets kept, row_counts rewritten. */
+size_t warp_amd_bsr_compress_workspace_size(int nrow, int T, int block_bytes);
+int warp_amd_bsr_compress(int block_size, int scalar_size, int nrow, int ncol, int T, int prune_numerical_zeros,
+                          uint64_t zero_mask, int make_compact, int* offsets, int* row_counts, int* columns,
+                          void* values, int* nnz_host, void* workspace, size_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* SpMV, diagonal, Jacobi                                                   */
 /* ------------------------------------------------------------------------ */

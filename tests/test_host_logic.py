@@ -130,3 +130,30 @@ def test_learned_checkpoint_roundtrip(tmp_path):
         assert torch.equal(back.params[k], net.params[k])
     assert torch.equal(back.m["e0.W"], net.m["e0.W"]) and back.t == 7
     assert meta2["smooth_steps"] == 1 and meta2["sigma"] == 1.5
+
+
+def test_padded_bsr_zeros_and_uncompress_rows():
+    from warp_amd import sparse
+
+    A = sparse.bsr_zeros(3, 4, wa.float32, device="cpu", row_capacity=2)
+    np.testing.assert_array_equal(A.offsets.numpy(), [0, 2, 4, 6])
+    np.testing.assert_array_equal(A.row_counts.numpy(), [0, 0, 0])
+    assert A.nnz == 6 and (A.columns.numpy() == -1).all()
+    B = sparse.bsr_zeros(3, 4, wa.float32, device="cpu", row_capacity=torch.tensor([1, 0, 3], dtype=torch.int32))
+    np.testing.assert_array_equal(B.offsets.numpy(), [0, 1, 1, 4])
+    B.row_counts = torch.tensor([1, 0, 2], dtype=torch.int32)
+    np.testing.assert_array_equal(B.uncompress_rows().numpy(), [0, 2, 2, -1])
+    with pytest.raises(ValueError, match="row_capacity requires topology='padded'"):
+        sparse.bsr_zeros(3, 4, wa.float32, device="cpu", topology="compact", row_capacity=2)
+    with pytest.raises(ValueError, match="row_capacity must be nonnegative"):
+        sparse.bsr_zeros(3, 4, wa.float32, device="cpu", row_capacity=-1)
+    with pytest.raises(TypeError, match="row_capacity array must have dtype int32"):
+        sparse.bsr_zeros(3, 4, wa.float32, device="cpu", row_capacity=torch.tensor([1, 2, 3]))
+    with pytest.raises(ValueError, match="nnz_capacity must be nonnegative"):
+        sparse.bsr_zeros(3, 4, wa.float32, device="cpu", nnz_capacity=-2)
+    with pytest.raises(ValueError, match="Unsupported topology policy"):
+        sparse.bsr_compress(A, topology="banded")
+    with pytest.raises(ValueError, match="requires a concrete BsrMatrix"):
+        sparse.bsr_compress(A * 2.0)
+    with pytest.raises(RuntimeError, match="ROCm GPU devices only"):
+        sparse.bsr_compress(A)

@@ -407,3 +407,132 @@ def test_bsr_mv_transpose_vs_dense(device, bshape, dtype):
     out = sparse.bsr_mv(A, _t(x, device, dtype), transpose=True)
     assert tuple(out.shape) == ((ncol,) if bshape[1] == 1 else (ncol, bshape[1]))
     np.testing.assert_allclose(out.cpu().numpy().astype(np.float64).reshape(-1), D.T @ x, **tol)
+
+
+def _active_dense(A):
+    """Dense matrix of the active blocks (row_counts honoured)."""
+    br, bc = A.block_shape
+    off = A.offsets.cpu().numpy()
+    rc = None if A.row_counts is None else A.row_counts.cpu().numpy()
+    col = A.columns.cpu().numpy()
+    val = A.values.cpu().numpy().astype(np.float64).reshape(-1, br, bc)
+    D = np.zeros((A.nrow * br, A.ncol * bc))
+    cols = []
+    for r in range(A.nrow):
+        end = off[r] + rc[r] if rc is not None else off[r + 1]
+        cols.append([int(c) for c in col[off[r]:end]])
+        for k in range(off[r], end):
+            D[r * br:(r + 1) * br, col[k] * bc:(col[k] + 1) * bc] += val[k]
+    return D, cols
+
+
+def test_bsr_compress_reference_scenarios(device):
+    """test_sparse.py:343-379 (padded and compact in-place compression) and 1040-1075 (trailing capacity)."""
+    cand = sparse.bsr_zeros(2, 3, wa.float32, device=device, row_capacity=torch.tensor([4, 3], dtype=torch.int32))
+    cand.row_counts = _t(np.array([4, 3], np.int32), device)
+    cand.columns = _t(np.array([2, 0, 2, 1, 1, 2, 1], np.int32), device)
+    cand.values = _t(np.array([5.0, 2.0, -1.0, 3.0, 4.0, 8.0, -4.0], np.float32), device)
+    sparse.bsr_compress(cand, inplace=True, topology="padded")
+    np.testing.assert_array_equal(cand.row_counts.cpu().numpy(), [3, 2])
+    D, cols = _active_dense(cand)
+    assert cols == [[0, 1, 2], [1, 2]]
+    np.testing.assert_array_equal(D, [[2.0, 3.0, 4.0], [0.0, 0.0, 8.0]])
+
+    long_cols = (np.arange(160, dtype=np.int32) * 37 + 11) % 97
+    long_vals = (np.arange(160, dtype=np.float32) % 11.0) - 5.0
+    expect = np.zeros(97, np.float32)
+    for c, v in zip(long_cols, long_vals):
+        expect[c] += v
+    expect_cols = np.unique(long_cols[long_vals != 0.0]).astype(np.int32)
+    long_row = sparse.bsr_zeros(2, 97, wa.float32, device=device,
+                                row_capacity=torch.tensor([long_cols.size, 0], dtype=torch.int32))
+    long_row.row_counts = _t(np.array([long_cols.size, 0], np.int32), device)
+    long_row.columns = _t(long_cols, device)
+    long_row.values = _t(long_vals, device)
+    sparse.bsr_compress(long_row, inplace=True, topology="compact")
+    assert long_row.row_counts is None
+    nnz = long_row.nnz_sync()
+    assert nnz == expect_cols.size
+    np.testing.assert_array_equal(long_row.offsets.cpu().numpy(), [0, nnz, nnz])
+    np.testing.assert_array_equal(long_row.columns.cpu().numpy()[:nnz], expect_cols)
+    np.testing.assert_allclose(long_row.values.cpu().numpy()[:nnz], expect[expect_cols])
+
+    for inplace in (False, True):
+        m = sparse.bsr_zeros(3, 3, wa.float32, device=device)
+        m.nnz = 6
+        m.offsets = _t(np.array([0, 1, 2, 3], np.int32), device)
+        m.columns = _t(np.array([0, 1, 2, 0, 0, 0], np.int32), device)
+        m.values = _t(np.array([3.0, 7.0, 11.0, 42.0, 42.0, 42.0], np.float32), device)
+        sparse.bsr_compress(m, prune_numerical_zeros=False, inplace=inplace, topology="compact")
+        assert m.row_counts is None and m.nnz_sync() == 3
+        np.testing.assert_array_equal(m.offsets.cpu().numpy(), [0, 1, 2, 3])
+        np.testing.assert_array_equal(m.columns.cpu().numpy()[:3], [0, 1, 2])
+        np.testing.assert_allclose(m.values.cpu().numpy()[:3], [3.0, 7.0, 11.0])
+
+
+def test_bsr_compress_prunes_and_keeps_capacity(device):
+    """test_sparse.py:441-475 without the tape: a zero block is pruned, duplicates summed."""
+    for topology in ("compact", "padded"):
+        m = sparse.bsr_zeros(1, 3, wa.float32, device=device)
+        m.nnz = 4
+        m.offsets = _t(np.array([0, 4], np.int32), device)
+        m.row_counts = _t(np.array([4], np.int32), device)
+        m.columns = _t(np.array([1, 0, 1, 2], np.int32), device)
+        m.values = _t(np.array([1.0, 2.0, 3.0, 0.0], np.float32), device)
+        sparse.bsr_compress(m, topology=topology)
+        D, cols = _active_dense(m)
+        np.testing.assert_array_equal(D, [[2.0, 4.0, 0.0]])
+        assert cols == [[0, 1]]
+        if topology == "compact":
+            assert m.row_counts is None and m.nnz_sync() == 2
+            np.testing.assert_array_equal(m.offsets.cpu().numpy(), [0, 2])
+        else:
+            np.testing.assert_array_equal(m.offsets.cpu().numpy(), [0, 4])
+            np.testing.assert_array_equal(m.row_counts.cpu().numpy(), [2])
+
+
+@pytest.mark.parametrize("topology", ["compact", "padded"])
+def test_bsr_compress_row_candidates_vs_oracle(device, oracle_mod, topology):
+    """fem 'row_compress' construction: each row's candidate blocks written into a
+    padded matrix in element order, then compressed.  Structure and values are
+    bit-exact against the oracle assembly of the same (row, column, value)
+    candidates in storage order (a stable sort and left-to-right sums)."""
+    rng = np.random.default_rng(5)
+    nrow, ncol = 500, 400
+    cap = rng.integers(0, 60, nrow).astype(np.int32)
+    counts = np.minimum(cap, rng.integers(0, 60, nrow)).astype(np.int32)
+    A = sparse.bsr_zeros(nrow, ncol, wa.mat33, device=device, row_capacity=torch.tensor(cap))
+    total = int(cap.sum())
+    off = np.concatenate([[0], np.cumsum(cap)]).astype(np.int32)
+    cols = np.full(total, -1, np.int32)
+    vals = np.zeros((total, 3, 3), np.float32)
+    rows_t, cols_t, vals_t = [], [], []
+    for r in range(nrow):
+        c = rng.integers(0, 25, counts[r]).astype(np.int32)   # many duplicates per row
+        v = rng.standard_normal((counts[r], 3, 3)).astype(np.float32)
+        v[rng.random(counts[r]) < 0.1] = 0.0                   # some zero blocks to prune
+        cols[off[r]:off[r] + counts[r]] = c
+        vals[off[r]:off[r] + counts[r]] = v
+        rows_t.append(np.full(counts[r], r, np.int32)); cols_t.append(c); vals_t.append(v)
+    A.row_counts = _t(counts, device)
+    A.columns = _t(cols, device)
+    A.values = _t(vals, device)
+    sparse.bsr_compress(A, topology=topology)
+    ref = oracle_mod.bsr_from_triplets(nrow, ncol, np.concatenate(rows_t), np.concatenate(cols_t),
+                                       np.concatenate(vals_t))
+    ref_cnt = np.diff(ref["offsets"])
+    if topology == "compact":
+        nnz = A.nnz_sync()
+        np.testing.assert_array_equal(A.offsets.cpu().numpy(), ref["offsets"])
+        np.testing.assert_array_equal(A.columns.cpu().numpy()[:nnz], ref["columns"])
+        np.testing.assert_array_equal(A.values.cpu().numpy()[:nnz], ref["values"])
+        assert (A.columns.cpu().numpy()[nnz:] == -1).all()
+    else:
+        np.testing.assert_array_equal(A.offsets.cpu().numpy(), off)
+        np.testing.assert_array_equal(A.row_counts.cpu().numpy(), ref_cnt)
+        gc = A.columns.cpu().numpy()
+        gv = A.values.cpu().numpy()
+        for r in range(nrow):
+            k0, k1 = ref["offsets"][r], ref["offsets"][r + 1]
+            np.testing.assert_array_equal(gc[off[r]:off[r] + ref_cnt[r]], ref["columns"][k0:k1])
+            np.testing.assert_array_equal(gv[off[r]:off[r] + ref_cnt[r]], ref["values"][k0:k1])

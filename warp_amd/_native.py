@@ -55,6 +55,11 @@ _SIGNATURES = {
         [c_int, c_int, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_u64, ctypes.c_bool] + [c_ptr] * 7,
     ),
     "warp_amd_bsr_mv_lanes": (c_int, [c_int, c_int, c_int, c_i64]),
+    "warp_amd_bsr_compress_workspace_size": (c_size, [c_int, c_int, c_int]),
+    "warp_amd_bsr_compress": (
+        c_int,
+        [c_int, c_int, c_int, c_int, c_int, c_int, c_u64, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr],
+    ),
     "warp_amd_bsr_residual_workspace_size": (c_size, [c_int]),
     "warp_amd_bsr_residual_f32_f64": (c_int, [c_int, c_int, c_ptr, c_ptr, c_ptr, c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
     "warp_amd_axpy_f32_to_f64": (c_int, [c_i64, c_f64, c_ptr, c_ptr, c_ptr]),

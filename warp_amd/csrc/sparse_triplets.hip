@@ -482,6 +482,95 @@ int bsr_from_triplets(
     return kOk;
 }
 
+
+// ---------------------------------------------------------------------------
+// bsr_compress: sort and coalesce the active blocks of each row (reference
+// bsr_compress, warp/_src/sparse.py:1571-1672; per-row semantics of
+// bsr_compress_inplace_row, warp/native/sparse_util.h:82-146).  The active
+// blocks become triplets (row, column, storage position) for the assembly
+// above: a stable sort by (row, column, position) and sequential sums in
+// storage order are exactly the reference's in-row insertion sort and
+// left-to-right accumulation.
+// ---------------------------------------------------------------------------
+
+// rows[k] = r for every k in row r's active range [offsets[r], end), k < T
+__global__ __launch_bounds__(kWave) void active_rows_kernel(int nrow, const int* __restrict__ offsets,
+                                                           const int* __restrict__ row_counts, int T,
+                                                           int* __restrict__ rows)
+{
+    const int r = blockIdx.x;
+    if (r >= nrow)
+        return;
+    const int beg = offsets[r];
+    int end = row_counts ? beg + row_counts[r] : offsets[r + 1];
+    end = end < T ? end : T;
+    for (int k = beg + (int)threadIdx.x; k < end; k += kWave)
+        rows[k] = r;
+}
+
+// Padded result: row r's coalesced blocks go to the start of its range.
+__global__ __launch_bounds__(kWave) void padded_scatter_kernel(int nrow, const int* __restrict__ offsets,
+                                                              int* __restrict__ row_counts,
+                                                              const int* __restrict__ c_offsets,
+                                                              const int* __restrict__ c_columns,
+                                                              const unsigned char* __restrict__ c_values,
+                                                              int block_bytes, int* __restrict__ columns,
+                                                              unsigned char* __restrict__ values)
+{
+    const int r = blockIdx.x;
+    if (r >= nrow)
+        return;
+    const int src = c_offsets[r];
+    const int cnt = c_offsets[r + 1] - src;
+    const int dst = offsets[r];
+    if (threadIdx.x == 0)
+        row_counts[r] = cnt;
+    for (int j = threadIdx.x; j < cnt; j += kWave)
+        columns[dst + j] = c_columns[src + j];
+    if (values)
+        for (int b = threadIdx.x; b < cnt * block_bytes; b += kWave)
+            values[(size_t)dst * block_bytes + b] = c_values[(size_t)src * block_bytes + b];
+}
+
+// Compact result: positions at or beyond nnz = offsets[nrow] get column -1 and zero values.
+__global__ __launch_bounds__(kBlock) void clear_tail_kernel(int nrow, const int* __restrict__ offsets, int T,
+                                                           int block_bytes, int* __restrict__ columns,
+                                                           unsigned char* __restrict__ values)
+{
+    const int nnz = offsets[nrow];
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= T || i < nnz)
+        return;
+    columns[i] = -1;
+    if (values)
+        for (int b = 0; b < block_bytes; ++b)
+            values[i * block_bytes + b] = 0;
+}
+
+struct CompressWs {
+    int* rows;
+    int* c_offsets;
+    int* c_columns;
+    unsigned char* c_values;
+    void* tws;
+    size_t tws_bytes;
+};
+
+CompressWs carve_compress(void* base, size_t bytes, int nrow, int T, int block_bytes, size_t* needed)
+{
+    Carve c(base, bytes);
+    CompressWs w;
+    w.rows = c.take<int>((size_t)T);
+    w.c_offsets = c.take<int>((size_t)nrow + 1);
+    w.c_columns = c.take<int>((size_t)T);
+    w.c_values = block_bytes > 0 ? c.take<unsigned char>((size_t)T * block_bytes) : nullptr;
+    const size_t tneed = triplets_workspace_bytes(nrow, T);
+    w.tws = c.take<char>(tneed);
+    w.tws_bytes = tneed;
+    *needed = c.used;
+    return w;
+}
+
 }  // namespace wa
 
 // ---------------------------------------------------------------------------
@@ -544,4 +633,77 @@ WA_API void wp_bsr_matrix_from_triplets_device(
     if (bsr_nnz && bsr_nnz_event)
         (void)hipEventRecord(static_cast<hipEvent_t>(bsr_nnz_event), stream);
     (void)hipFreeAsync(ws, stream);
+}
+
+// ---------------------------------------------------------------------------
+// bsr_compress
+// ---------------------------------------------------------------------------
+
+// Workspace of warp_amd_bsr_compress for nrow block rows, T storage
+// positions and blocks of block_bytes bytes (0 for a topology-only call).
+WA_API size_t warp_amd_bsr_compress_workspace_size(int nrow, int T, int block_bytes)
+{
+    size_t needed = 0;
+    wa::carve_compress(nullptr, 0, nrow, T, block_bytes, &needed);
+    return needed + 256;
+}
+
+// Sort and coalesce the active blocks of each row in place (replaces
+// wp_bsr_compress_inplace_device / _host, warp/native/sparse.cpp:521-607,
+// and bsr_compress, warp/_src/sparse.py:1571-1672).  Active blocks are the
+// positions [offsets[r], offsets[r] + row_counts[r]) (row_counts NULL:
+// [offsets[r], offsets[r+1])) below T; negative columns are dropped, and with
+// prune_numerical_zeros so are blocks whose scalars all satisfy
+// (bits & zero_mask) == 0.  Duplicates are summed left to right in storage
+// order (values NULL: topology only).  make_compact = 1 packs the rows
+// (offsets rewritten, positions beyond the new nnz get column -1 and zero
+// values, *nnz_host = nnz when given); 0 keeps offsets and writes
+// row_counts.  Only fp32 / fp64 values can be summed.
+WA_API int warp_amd_bsr_compress(int block_size, int scalar_size, int nrow, int ncol, int T, int prune_numerical_zeros,
+                                 uint64_t zero_mask, int make_compact, int* offsets, int* row_counts, int* columns,
+                                 void* values, int* nnz_host, void* workspace, size_t workspace_bytes, void* stream)
+{
+    WA_REQUIRE(nrow >= 0 && ncol >= 0 && T >= 0 && block_size > 0, "bsr_compress: invalid sizes");
+    WA_REQUIRE(offsets != nullptr && (T == 0 || columns != nullptr), "bsr_compress: null topology arrays");
+    WA_REQUIRE(make_compact || row_counts != nullptr, "bsr_compress: padded output needs row_counts");
+    WA_REQUIRE(!values || scalar_size == 4 || scalar_size == 8, "bsr_compress: values must be fp32 or fp64");
+    hipStream_t s = wa::as_stream(stream);
+    const int block_bytes = values ? block_size * scalar_size : 0;
+    size_t needed = 0;
+    wa::CompressWs w = wa::carve_compress(workspace, workspace_bytes, nrow, T, block_bytes, &needed);
+    WA_REQUIRE(workspace && needed <= workspace_bytes, "bsr_compress: workspace too small (%zu < %zu bytes)",
+               workspace_bytes, needed);
+    if (nrow == 0) {
+        if (make_compact && nnz_host)
+            *nnz_host = 0;
+        return wa::kOk;
+    }
+    WA_TRY(wa::launch_fill(w.rows, T, -1, s));
+    if (T > 0) {
+        wa::active_rows_kernel<<<(unsigned)nrow, wa::kWave, 0, s>>>(nrow, offsets, row_counts, T, w.rows);
+        WA_LAUNCH_CHECK();
+    }
+    WA_TRY(wa::bsr_from_triplets(block_size, scalar_size, nrow, ncol, T, nullptr, w.rows, columns, values,
+                                 prune_numerical_zeros && values ? zero_mask : 0, 0, nullptr, nullptr, w.c_offsets,
+                                 nullptr, w.c_columns, values ? (void*)w.c_values : nullptr,
+                                 make_compact ? nnz_host : nullptr, w.tws, w.tws_bytes, s));
+    if (make_compact) {
+        WA_HIP(hipMemcpyAsync(offsets, w.c_offsets, sizeof(int) * ((size_t)nrow + 1), hipMemcpyDeviceToDevice, s));
+        if (T > 0) {
+            WA_HIP(hipMemcpyAsync(columns, w.c_columns, sizeof(int) * (size_t)T, hipMemcpyDeviceToDevice, s));
+            if (values)
+                WA_HIP(hipMemcpyAsync(values, w.c_values, (size_t)T * block_bytes, hipMemcpyDeviceToDevice, s));
+            wa::clear_tail_kernel<<<(unsigned)wa::ceil_div(T, wa::kBlock), wa::kBlock, 0, s>>>(
+                nrow, offsets, T, block_bytes, columns, static_cast<unsigned char*>(values)
+            );
+            WA_LAUNCH_CHECK();
+        }
+    } else {
+        wa::padded_scatter_kernel<<<(unsigned)nrow, wa::kWave, 0, s>>>(
+            nrow, offsets, row_counts, w.c_offsets, w.c_columns, w.c_values, block_bytes, columns,
+            static_cast<unsigned char*>(values)
+        );
+        WA_LAUNCH_CHECK();
+    }
+    return wa::kOk;
 }

@@ -139,15 +139,22 @@ class BsrMatrix:
         _bsr_ensure_fits(self, capacity=nnz_capacity)
 
     def uncompress_rows(self, out: torch.Tensor | None = None) -> torch.Tensor:
-        """Row index of every stored block slot, ``-1`` beyond the active blocks (sparse.py:164-193)."""
-        counts = (self.offsets[1 : self.nrow + 1] - self.offsets[: self.nrow]).to(torch.int64)
+        """Row index of every stored block slot, ``-1`` outside the active blocks (sparse.py:164-193)."""
+        starts = self.offsets[: self.nrow].to(torch.int64)
+        if self.row_counts is None:
+            counts = self.offsets[1 : self.nrow + 1].to(torch.int64) - starts
+        else:
+            counts = self.row_counts[: self.nrow].to(torch.int64)
         rows = torch.repeat_interleave(torch.arange(self.nrow, device=self.device, dtype=torch.int32), counts)
+        # position of each active block: its row start plus its rank within the row
+        first = torch.repeat_interleave(torch.cumsum(counts, 0) - counts, counts)
+        pos = torch.repeat_interleave(starts, counts) + torch.arange(rows.numel(), device=self.device) - first
         if out is None:
             out = torch.full((self.nnz,), -1, dtype=torch.int32, device=self.device)
         else:
             out[: self.nnz].fill_(-1)
-        k = min(rows.numel(), self.nnz)
-        out[:k] = rows[:k]
+        keep = pos < self.nnz
+        out[pos[keep]] = rows[keep]
         return out
 
     # -- operators -------------------------------------------------------
@@ -255,13 +262,34 @@ def bsr_zeros(
     row_capacity=None,
     nnz_capacity: int | None = None,
 ) -> BsrMatrix:
-    """Construct an empty compact BSR or CSR matrix (sparse.py:639-690)."""
+    """Construct an empty BSR or CSR matrix (sparse.py:639-690).
+
+    ``topology="padded"`` (implied by ``row_capacity``) reserves
+    ``row_capacity`` slots per row (an int, or an int32 tensor of per-row
+    capacities) with ``row_counts`` zero; such matrices are filled in place
+    and coalesced by :func:`bsr_compress`."""
     if topology is None:
         topology = "padded" if row_capacity is not None else "compact"
     elif topology not in ("compact", "padded"):
         raise ValueError(f"Unsupported topology policy: {topology}")
-    if topology == "padded":
-        raise NotImplementedError("padded row-capacity topologies are not provided by warp_amd")
+    if nnz_capacity is not None:
+        nnz_capacity = int(nnz_capacity)
+        if nnz_capacity < 0:
+            raise ValueError("nnz_capacity must be nonnegative")
+    if row_capacity is not None:
+        if topology != "padded":
+            raise ValueError("row_capacity requires topology='padded'")
+        if isinstance(row_capacity, torch.Tensor):
+            if row_capacity.ndim != 1 or row_capacity.shape[0] != int(rows_of_blocks):
+                raise ValueError(
+                    f"row_capacity array must have shape ({int(rows_of_blocks)},), got {tuple(row_capacity.shape)}"
+                )
+            if row_capacity.dtype != torch.int32:
+                raise TypeError("row_capacity array must have dtype int32")
+        else:
+            row_capacity = int(row_capacity)
+            if row_capacity < 0:
+                raise ValueError("row_capacity must be nonnegative")
 
     if not type_is_matrix(block_type):
         if type_is_vector_like(block_type):
@@ -280,6 +308,23 @@ def bsr_zeros(
         bsr.values = torch.empty((0, *block_type._shape_), dtype=torch_dtype, device=dev)
     else:
         bsr.values = torch.empty(0, dtype=torch_dtype, device=dev)
+    if topology == "padded":
+        # row r owns [offsets[r], offsets[r+1]), with row_counts[r] active blocks (none yet)
+        if isinstance(row_capacity, torch.Tensor):
+            bsr.offsets[1:] = torch.cumsum(row_capacity.to(dev), 0).to(torch.int32)
+            nnz = int(bsr.offsets[bsr.nrow]) if nnz_capacity is None else nnz_capacity
+        elif row_capacity is not None:
+            bsr.offsets = torch.arange(bsr.nrow + 1, dtype=torch.int32, device=dev) * row_capacity
+            nnz = bsr.nrow * row_capacity
+        else:
+            nnz = 0
+        bsr.row_counts = torch.zeros(bsr.nrow, dtype=torch.int32, device=dev)
+        capacity = max(nnz, nnz_capacity or 0)
+        _bsr_ensure_fits(bsr, nnz=capacity)
+        bsr.columns[:capacity].fill_(-1)
+        bsr.values[:capacity].zero_()
+        bsr.nnz = nnz
+        return bsr
     _bsr_ensure_fits(bsr, nnz=int(nnz_capacity or 0))
     bsr.nnz = 0
     return bsr
@@ -440,6 +485,71 @@ def bsr_from_triplets(
     A = bsr_zeros(rows_of_blocks, cols_of_blocks, block_type, device=values.device)
     bsr_set_from_triplets(A, rows, columns, values, prune_numerical_zeros=prune_numerical_zeros)
     return A
+
+
+def bsr_compress(src: BsrMatrix, prune_numerical_zeros: bool = True, inplace: bool = False, topology: str | None = None):
+    """Sort and coalesce the active blocks of each row of ``src``; returns ``src`` (sparse.py:1571-1672).
+
+    Active blocks are ``offsets[r] : offsets[r] + row_counts[r]`` (or up to
+    ``offsets[r+1]`` for compact matrices); negative columns are ignored and,
+    with ``prune_numerical_zeros``, so are all-zero blocks.  Duplicates are
+    summed left to right in storage order, as the reference's in-place row
+    compression does.  ``topology="compact"`` (default) packs the rows and
+    sets ``row_counts`` to None; ``"padded"`` keeps each row's capacity and
+    updates ``row_counts``.  ``inplace`` is accepted for signature
+    compatibility: both modes run the same native, non-differentiable path
+    (as in the reference, callers should not rely on buffer identity).
+    """
+    if not isinstance(src, BsrMatrix):
+        raise ValueError("bsr_compress() requires a concrete BsrMatrix")
+    if topology is None:
+        topology = "compact"
+    elif topology not in ("compact", "padded"):
+        raise ValueError(f"Unsupported topology policy: {topology}")
+    _require_gpu(src.device, "bsr_compress")
+    compact = topology == "compact"
+    if not compact and src.row_counts is None:
+        src.row_counts = (src.offsets[1 : src.nrow + 1] - src.offsets[: src.nrow]).to(torch.int32)
+    if src.nrow == 0:
+        if compact:
+            src.row_counts = None
+            src.nnz = 0
+        return src
+    scalar_type = src.scalar_type
+    if scalar_type.torch_dtype not in (torch.float32, torch.float64):
+        raise TypeError("bsr_compress supports float32 and float64 matrices")
+    # storage positions that can hold active blocks
+    T = min(int(src.offsets[src.nrow]), src.columns.numel(), src.values.shape[0])
+    lib = _native.lib()
+    block_bytes = src.block_size * scalar_type.size
+    ws_bytes = lib.warp_amd_bsr_compress_workspace_size(src.nrow, T, block_bytes)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=src.device)
+    zero_mask = _zero_value_masks.get(scalar_type, 0) if prune_numerical_zeros else 0
+    nnz_buf, nnz_event = src._setup_nnz_transfer()
+    _native.call(
+        "warp_amd_bsr_compress",
+        src.block_size,
+        scalar_type.size,
+        src.nrow,
+        src.ncol,
+        T,
+        1 if prune_numerical_zeros else 0,
+        zero_mask,
+        1 if compact else 0,
+        _native.ptr(src.offsets),
+        _native.ptr(src.row_counts),
+        _native.ptr(src.columns),
+        _native.ptr(src.values),
+        nnz_buf.data_ptr() if compact else None,
+        _native.ptr(ws),
+        ws_bytes,
+        _native.stream_of(src.device),
+    )
+    if compact:
+        nnz_event.record(torch.cuda.current_stream(src.device))
+        src._nnz_pending = True
+        src.row_counts = None
+    return src
 
 
 def _vec_view(array: torch.Tensor, torch_dtype: torch.dtype, expected_scalar_count: int) -> torch.Tensor:
