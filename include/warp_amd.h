@@ -121,6 +121,108 @@ int warp_amd_bsr_compress(int block_size, int scalar_size, int nrow, int ncol, i
                           uint64_t zero_mask, int make_compact, int* offsets, int* row_counts, int* columns,
                           void* values, int* nnz_host, void* workspace, size_t workspace_bytes, void* stream);
 
+/* Triplets into the row capacity of a padded matrix (bsr_set_from_triplets
+ * with topology="padded", warp/_src/sparse.py:1335-1420, whose scatter is
+ * _bsr_set_from_triplets_padded_scatter_compact_topology, 1173-1208).
+ * offsets[0..nrow] is read as capacity and left unchanged; row r gets its
+ * sorted unique columns at offsets[r].. and their number in row_counts[r]; a
+ * row that does not fit is left empty and raises *status (sticky) to 1
+ * (BSR_STATUS_ROW_CAPACITY_EXCEEDED).  With values the first `capacity` blocks
+ * are cleared and the sums written, duplicates added in ascending triplet
+ * order; invalid and (zero_mask != 0) all-zero triplets are dropped. */
+size_t warp_amd_bsr_topology_workspace_size(int nrow, int T, int block_bytes);
+int warp_amd_bsr_set_from_triplets_padded(int block_size, int scalar_size, int nrow, int ncol, int T,
+                                          const int* tpl_count, const int* rows, const int* cols,
+                                          const void* tpl_values, uint64_t zero_mask, const int* offsets,
+                                          int* row_counts, int* columns, void* values, int64_t capacity,
+                                          int* status, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Triplets into an existing topology, left unchanged (bsr_set_from_triplets
+ * with topology="masked", warp/_src/sparse.py:1422-1441 and the kernel
+ * _bsr_set_from_triplets_masked_values, 1145-1170): the first `capacity`
+ * blocks of values are cleared, then each active block (r, c) receives the sum
+ * of the valid triplets at (r, c) in ascending triplet order (the reference
+ * adds with atomics, in no fixed order).  No zero pruning. */
+int warp_amd_bsr_set_from_triplets_masked(int block_size, int scalar_size, int nrow, int ncol, int T,
+                                          const int* tpl_count, const int* rows, const int* cols,
+                                          const void* tpl_values, const int* offsets, const int* row_counts,
+                                          const int* columns, void* values, int64_t capacity, void* workspace,
+                                          size_t workspace_bytes, void* stream);
+
+/* Topology of the transpose of an nrow x ncol block matrix; replaces
+ * wp_bsr_transpose_device / _host (warp/native/sparse.cu:1400-1492,
+ * sparse.cpp:415-519; called from bsr_set_transpose,
+ * warp/_src/sparse.py:2636-2833).  Active source blocks (positions < T) are
+ * ordered by (column, row, position), none merged.  t_row_counts NULL:
+ * compact, t_offsets[0..ncol] and t_columns / block_map[0..nnz) written.
+ * Otherwise padded: t_offsets read as row capacity; t_row_counts, t_columns
+ * and block_map written over the capacity (block_map -1 on unused slots); an
+ * overflowing row is left empty and raises *status.  block_map[b] = source
+ * position of transposed block b. */
+size_t warp_amd_bsr_transpose_workspace_size(int ncol, int T);
+int warp_amd_bsr_transpose(int nrow, int ncol, int T, const int* offsets, const int* row_counts, const int* columns,
+                           int* t_offsets, int* t_row_counts, int* t_columns, int* block_map, int* status,
+                           void* workspace, size_t workspace_bytes, void* stream);
+
+/* Same name and arguments as the reference's wp_bsr_transpose_device
+ * (warp/native/warp.h; bound in warp/_src/context.py), so its binding can
+ * point here.  Stream from warp_amd_set_current_stream; scratch allocated
+ * stream-ordered; errors through warp_amd_last_error(). */
+void wp_bsr_transpose_device(int row_count, int col_count, int nnz, const int* bsr_offsets, const int* bsr_row_counts,
+                             const int* bsr_columns, int* transposed_bsr_offsets, int* transposed_bsr_row_counts,
+                             int* transposed_bsr_columns, int* src_block_indices, int* status);
+
+/* dst[b] = scale * transpose(src[block_map[b]]) for b < min(count, *total)
+ * with block_map[b] >= 0 (*total: device int, the transposed offsets[nrow]);
+ * source blocks br x bc; fp32 (4) or fp64 (8).  Replaces the kernel
+ * _bsr_transpose_values, warp/_src/sparse.py:2604-2633. */
+int warp_amd_bsr_transpose_values(int scalar_size, int br, int bc, int64_t count, const int* total,
+                                  const int* block_map, const void* src, double scale, void* dst, void* stream);
+
+/* Masked transpose values (bsr_set_transpose with topology="masked"): each
+ * active destination block (r, c) takes scale * transpose of source block
+ * (c, r) when present (binary search in source row c).  Destination values
+ * are not cleared here. */
+int warp_amd_bsr_transpose_masked_values(int scalar_size, int br, int bc, int t_nrow, const int* t_offsets,
+                                         const int* t_row_counts, const int* t_columns, int nrow, const int* offsets,
+                                         const int* row_counts, const int* columns, const void* src, double scale,
+                                         void* dst, void* stream);
+
+/* Block coordinates of an assignment that reshapes blocks (replaces the
+ * kernel _bsr_assign_list_blocks of bsr_assign, warp/_src/sparse.py:2258-2452):
+ * each source block (positions < T) splits into src_subrows x src_subcols
+ * pieces; destination blocks group dest_subrows x dest_subcols pieces.
+ * Entry (p * src_subcols + subcol) * src_subrows + subrow receives the
+ * destination (row, col) of that piece, (-1, -1) for inactive positions. */
+int warp_amd_bsr_assign_list_blocks(int T, int src_subrows, int src_subcols, int dest_subrows, int dest_subcols,
+                                    int nrow, const int* offsets, const int* row_counts, const int* columns,
+                                    int* dest_rows, int* dest_cols, void* stream);
+
+/* Copy every active source block's pieces into the existing destination
+ * topology (replaces _bsr_assign_copy_blocks): value = (dest type)(scale *
+ * source value); pieces without a destination block are skipped; destination
+ * values are not cleared.  Scalar sizes 4 or 8. */
+int warp_amd_bsr_assign_copy_blocks(int src_scalar_size, int dest_scalar_size, int T, int src_subrows,
+                                    int src_subcols, int dest_subrows, int dest_subcols, int nrow, const int* offsets,
+                                    const int* row_counts, const int* columns, int src_br, int src_bc,
+                                    const void* src_values, double scale, const int* dest_offsets,
+                                    const int* dest_row_counts, const int* dest_columns, int dest_br, int dest_bc,
+                                    void* dest_values, void* stream);
+
+/* Padded assignment with the same block shape (bsr_assign with
+ * topology="padded", warp/_src/sparse.py:2331-2368, kernels
+ * _bsr_assign_padded_row_ranges / _bsr_assign_padded_copy_blocks): each
+ * source row's active blocks, in storage order, go to the start of the
+ * destination row's capacity; a row that does not fit is left empty and
+ * raises *status to 1.  src_values NULL: structure only. */
+int warp_amd_bsr_assign_padded(int src_scalar_size, int dest_scalar_size, int nrow, int block_size,
+                               const int* offsets, const int* row_counts, const int* columns, const void* src_values,
+                               double scale, const int* dest_offsets, int* dest_row_counts, int* dest_columns,
+                               void* dest_values, int* status, void* stream);
+
+/* x[0..n) *= alpha, fp32 (4) or fp64 (8) (bsr_scale's value kernel). */
+int warp_amd_scale_values(int scalar_size, int64_t n, void* x, double alpha, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* SpMV, diagonal, Jacobi                                                   */
 /* ------------------------------------------------------------------------ */

@@ -1,10 +1,11 @@
 """CPU oracle for the sparse-solver path -- TEST INFRASTRUCTURE ONLY.
 
 Wraps ``warp_oracle.c`` (a plain-C restatement of the reference algorithms,
-each function citing the reference lines it follows) and, when it has been
-built, ``_ref/libwarp_ref.so``: the reference's own CPU native library
-compiled from its unmodified sources (``make ref``), used only to check the
-restatement's triplet topology.
+each function citing the reference lines it follows), holds a few numpy
+restatements of index-only algorithms (the transposed topology), and, when it
+has been built, loads ``_ref/libwarp_ref.so``: the reference's own CPU native
+library compiled from its unmodified sources (``make ref``), used only to
+check the restatements' topologies.
 
 Only tests, the package smoke check and the bench's ``cpu_baseline`` leg use
 this module, always as the checker / baseline and never as the product path.
@@ -76,6 +77,8 @@ def ref_lib() -> ctypes.CDLL | None:
         L.wp_bsr_matrix_from_triplets_host.argtypes = [
             _I, _I, _I, _I, _I, _P, _P, _P, _P, ctypes.c_uint64, ctypes.c_bool, _P, _P, _P, _P, _P, _P, _P,
         ]
+        L.wp_bsr_transpose_host.restype = None
+        L.wp_bsr_transpose_host.argtypes = [_I, _I, _I] + [_P] * 8
         _ref = L
     return _ref
 
@@ -152,6 +155,92 @@ def ref_bsr_from_triplets(nrow, ncol, rows, cols, values=None, prune=True, count
     )
     n = int(nnz[0])
     return dict(offsets=off, columns=colsout[:n], summed_offsets=so[:n], summed_indices=si, nnz=n)
+
+
+def _active_positions(nrow, offsets, row_counts, T):
+    """(positions, rows) of the active blocks below T; a row's active range
+    never passes its capacity end (sparse.cpp:445-447)."""
+    offsets = np.asarray(offsets, np.int64)
+    T = int(min(T, offsets[nrow])) if nrow > 0 else 0
+    if T <= 0:
+        return np.zeros(0, np.int64), np.zeros(0, np.int64)
+    ends = offsets[1 : nrow + 1] if row_counts is None else np.minimum(
+        offsets[:nrow] + np.asarray(row_counts, np.int64)[:nrow], offsets[1 : nrow + 1])
+    p = np.arange(T, dtype=np.int64)
+    r = np.searchsorted(offsets[: nrow + 1], p, side="right") - 1
+    keep = p < ends[r]
+    return p[keep], r[keep]
+
+
+def bsr_transpose(nrow, ncol, offsets, columns, row_counts=None, T=None, capacity=None):
+    """Transposed topology (reference wp_bsr_transpose_host, sparse.cpp:415-519;
+    device twin sparse.cu:1400-1492).  The active blocks are ordered by
+    (column, row) -- the host's comparator -- with ties (repeated blocks) by
+    position, the order of the device's stable radix sort.
+
+    capacity None: compact result dict(offsets, columns, block_map).
+    capacity = transposed offsets read as row capacity: padded result
+    dict(row_counts, columns, block_map, status) over the capacity, with
+    columns -1 where unwritten and block_map -1 on unused slots; a row that
+    does not fit is left empty and sets status 1.
+    """
+    columns = np.asarray(columns, np.int64)
+    if T is None:
+        T = int(np.asarray(offsets)[nrow])
+    p, r = _active_positions(nrow, offsets, row_counts, T)
+    c = columns[p]
+    order = np.lexsort((p, r, c))
+    sp, t_cols, t_rows = p[order], r[order], c[order]
+    counts = np.bincount(t_rows, minlength=ncol).astype(np.int64)
+    t_off = np.concatenate([[0], np.cumsum(counts)])
+    if capacity is None:
+        return dict(offsets=t_off.astype(np.int32), columns=t_cols.astype(np.int32), block_map=sp.astype(np.int32))
+    capacity = np.asarray(capacity, np.int64)
+    slots = int(capacity[ncol])
+    out_cols = np.full(slots, -1, np.int32)
+    out_map = np.full(slots, -1, np.int32)
+    out_cnt = np.zeros(ncol, np.int32)
+    status = 0
+    for row in range(ncol):
+        beg, cap = capacity[row], capacity[row + 1] - capacity[row]
+        n = counts[row]
+        if n > cap:
+            status = 1
+            continue
+        out_cnt[row] = n
+        out_cols[beg : beg + n] = t_cols[t_off[row] : t_off[row] + n]
+        out_map[beg : beg + n] = sp[t_off[row] : t_off[row] + n]
+    return dict(row_counts=out_cnt, columns=out_cols, block_map=out_map, status=status)
+
+
+def ref_bsr_transpose(nrow, ncol, offsets, columns, row_counts=None, T=None, capacity=None):
+    """The same, from the reference's own host function (sparse.cpp:415-519),
+    or None when it was not built.  Same result layout as bsr_transpose."""
+    L = ref_lib()
+    if L is None:
+        return None
+    offsets = np.ascontiguousarray(offsets, np.int32)
+    columns = np.ascontiguousarray(columns, np.int32)
+    rc = None if row_counts is None else np.ascontiguousarray(row_counts, np.int32)
+    if T is None:
+        T = int(offsets[nrow])
+    if capacity is None:
+        t_off = np.zeros(ncol + 1, np.int32)
+        t_cols = np.zeros(max(T, 1), np.int32)
+        bmap = np.zeros(max(2 * T, 1), np.int32)
+        L.wp_bsr_transpose_host(nrow, ncol, T, _p(offsets), _p(rc), _p(columns), _p(t_off), None, _p(t_cols),
+                                _p(bmap), None)
+        n = int(t_off[ncol])
+        return dict(offsets=t_off, columns=t_cols[:n], block_map=bmap[:n])
+    cap = np.ascontiguousarray(capacity, np.int32)
+    slots = int(cap[ncol])
+    t_cnt = np.zeros(ncol, np.int32)
+    t_cols = np.full(max(slots, 1), -1, np.int32)
+    bmap = np.zeros(max(slots, 2 * T, 1), np.int32)
+    status = np.zeros(1, np.int32)
+    L.wp_bsr_transpose_host(nrow, ncol, T, _p(offsets), _p(rc), _p(columns), _p(cap), _p(t_cnt), _p(t_cols),
+                            _p(bmap), _p(status))
+    return dict(row_counts=t_cnt, columns=t_cols[:slots], block_map=bmap[:slots], status=int(status[0]))
 
 
 def bsr_mv(offsets, columns, values, x, y=None, alpha=1.0, beta=0.0, row_counts=None, group=0):

@@ -261,3 +261,68 @@ def test_residual_fp64():
     r = oracle.residual_f32_f64(A.indptr.astype(np.int32), A.indices.astype(np.int32), v32, x, b)
     A64 = sp.csr_matrix((v32.astype(np.float64), A.indices, A.indptr), shape=A.shape)
     np.testing.assert_allclose(r, b - A64 @ x, rtol=1e-13, atol=1e-13)
+
+
+def _random_rows(rng, nrow, ncol, max_per_row, padded):
+    """A random topology with sorted unique columns per row; padded rows get spare capacity."""
+    counts = rng.integers(0, max_per_row + 1, nrow).astype(np.int32)
+    counts = np.minimum(counts, ncol)
+    cap = counts + (rng.integers(0, 3, nrow) if padded else 0)
+    off = np.concatenate([[0], np.cumsum(cap)]).astype(np.int32)
+    cols = np.full(max(int(off[-1]), 1), -1, np.int32)
+    for r in range(nrow):
+        cols[off[r] : off[r] + counts[r]] = np.sort(rng.choice(ncol, size=counts[r], replace=False))
+    return off, cols, counts
+
+
+@pytest.mark.skipif(oracle.ref_lib() is None, reason="reference CPU library not built (make -C oracle ref)")
+@pytest.mark.parametrize("seed", range(4))
+def test_transpose_oracle_matches_reference_host(seed):
+    """oracle.bsr_transpose against the reference's own wp_bsr_transpose_host
+    (compact and padded sources, compact and padded results with overflowing rows)."""
+    rng = np.random.default_rng(100 + seed)
+    for _ in range(25):
+        nrow, ncol = int(rng.integers(0, 40)), int(rng.integers(1, 40))
+        padded = bool(rng.integers(0, 2))
+        off, cols, counts = _random_rows(rng, nrow, ncol, 8, padded)
+        rc = counts if padded else None
+        T = int(off[-1])
+        for capacity in (None, np.concatenate([[0], np.cumsum(rng.integers(0, 6, ncol))]).astype(np.int32)):
+            a = oracle.bsr_transpose(nrow, ncol, off, cols, row_counts=rc, T=T, capacity=capacity)
+            b = oracle.ref_bsr_transpose(nrow, ncol, off, cols, row_counts=rc, T=T, capacity=capacity)
+            for key in a:
+                np.testing.assert_array_equal(np.asarray(a[key]), np.asarray(b[key]), err_msg=key)
+
+
+def test_transpose_oracle_dense_and_known_answers():
+    """The transposed topology reproduces the dense transpose, keeps the ascending
+    row order the reference tests expect, and the reference's known answers."""
+    rng = np.random.default_rng(7)
+    nrow, ncol = 30, 20
+    off, cols, _ = _random_rows(rng, nrow, ncol, 6, False)
+    vals = rng.standard_normal(int(off[-1]))
+    t = oracle.bsr_transpose(nrow, ncol, off, cols)
+    D = np.zeros((nrow, ncol))
+    for r in range(nrow):
+        D[r, cols[off[r] : off[r + 1]]] = vals[off[r] : off[r + 1]]
+    Dt = np.zeros((ncol, nrow))
+    for r in range(ncol):
+        seg = slice(t["offsets"][r], t["offsets"][r + 1])
+        assert np.all(np.diff(t["columns"][seg]) > 0)       # ascending source rows
+        Dt[r, t["columns"][seg]] = vals[t["block_map"][seg]]
+    np.testing.assert_array_equal(Dt, D.T)
+    # test_sparse.py:255-320: gapped 2x3 CSR -> transposed rows [[0], [1], [0, 1]]
+    g = oracle.bsr_transpose(2, 3, np.array([0, 3, 6]), np.array([0, 2, -1, 1, 2, -1]), row_counts=np.array([2, 2]),
+                             capacity=np.array([0, 2, 4, 6]))
+    assert g["status"] == 0
+    np.testing.assert_array_equal(g["row_counts"], [1, 1, 2])
+    np.testing.assert_array_equal(g["columns"][[0, 2, 4, 5]], [0, 1, 0, 1])
+    np.testing.assert_array_equal(g["block_map"], [0, -1, 3, -1, 1, 4])
+    # test_sparse.py:631-672: triplets (0,1) (1,4) (2,1) -> transposed rows 1: [0, 2], 4: [1]
+    c = oracle.bsr_transpose(4, 5, np.array([0, 1, 2, 3, 3]), np.array([1, 4, 1]))
+    np.testing.assert_array_equal(c["offsets"], [0, 0, 2, 2, 2, 3])
+    np.testing.assert_array_equal(c["columns"], [0, 2, 1])
+    # a row over capacity is left empty and raises the status
+    o = oracle.bsr_transpose(2, 2, np.array([0, 2, 4]), np.array([0, 1, 0, 1]), capacity=np.array([0, 1, 3]))
+    assert o["status"] == 1
+    np.testing.assert_array_equal(o["row_counts"], [0, 2])

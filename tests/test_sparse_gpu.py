@@ -428,7 +428,7 @@ def _active_dense(A):
 
 def test_bsr_compress_reference_scenarios(device):
     """test_sparse.py:343-379 (padded and compact in-place compression) and 1040-1075 (trailing capacity)."""
-    cand = sparse.bsr_zeros(2, 3, wa.float32, device=device, row_capacity=torch.tensor([4, 3], dtype=torch.int32))
+    cand = sparse.bsr_zeros(2, 3, wa.float32, device=device, row_capacity=_t(np.array([4, 3], np.int32), device))
     cand.row_counts = _t(np.array([4, 3], np.int32), device)
     cand.columns = _t(np.array([2, 0, 2, 1, 1, 2, 1], np.int32), device)
     cand.values = _t(np.array([5.0, 2.0, -1.0, 3.0, 4.0, 8.0, -4.0], np.float32), device)
@@ -445,7 +445,7 @@ def test_bsr_compress_reference_scenarios(device):
         expect[c] += v
     expect_cols = np.unique(long_cols[long_vals != 0.0]).astype(np.int32)
     long_row = sparse.bsr_zeros(2, 97, wa.float32, device=device,
-                                row_capacity=torch.tensor([long_cols.size, 0], dtype=torch.int32))
+                                row_capacity=_t(np.array([long_cols.size, 0], np.int32), device))
     long_row.row_counts = _t(np.array([long_cols.size, 0], np.int32), device)
     long_row.columns = _t(long_cols, device)
     long_row.values = _t(long_vals, device)
@@ -501,7 +501,7 @@ def test_bsr_compress_row_candidates_vs_oracle(device, oracle_mod, topology):
     nrow, ncol = 500, 400
     cap = rng.integers(0, 60, nrow).astype(np.int32)
     counts = np.minimum(cap, rng.integers(0, 60, nrow)).astype(np.int32)
-    A = sparse.bsr_zeros(nrow, ncol, wa.mat33, device=device, row_capacity=torch.tensor(cap))
+    A = sparse.bsr_zeros(nrow, ncol, wa.mat33, device=device, row_capacity=_t(cap, device))
     total = int(cap.sum())
     off = np.concatenate([[0], np.cumsum(cap)]).astype(np.int32)
     cols = np.full(total, -1, np.int32)
@@ -536,3 +536,298 @@ def test_bsr_compress_row_candidates_vs_oracle(device, oracle_mod, topology):
             k0, k1 = ref["offsets"][r], ref["offsets"][r + 1]
             np.testing.assert_array_equal(gc[off[r]:off[r] + ref_cnt[r]], ref["columns"][k0:k1])
             np.testing.assert_array_equal(gv[off[r]:off[r] + ref_cnt[r]], ref["values"][k0:k1])
+
+
+# ---------------------------------------------------------------------------
+# Padded / masked topologies, status codes, transposes, assignment
+# ---------------------------------------------------------------------------
+
+def _f(a, device):
+    return _t(np.asarray(a, np.float32), device)
+
+
+def _i(a, device):
+    return _t(np.asarray(a, np.int32), device)
+
+
+def _gapped_csr(device):
+    """test_sparse.py:96-106: a 2x3 CSR with row capacity 3 filled by padded triplets."""
+    A = sparse.bsr_zeros(2, 3, wa.float32, device=device, row_capacity=3)
+    sparse.bsr_set_from_triplets(A, _i([0, 0, 1, 1], device), _i([0, 2, 1, 2], device), _f([1.0, 2.0, 3.0, 4.0], device),
+                                 topology="padded")
+    return A
+
+
+def test_padded_gapped_layout_reference_scenario(device):
+    """test_sparse.py:255-350: padded triplets, copies, assignment, transpose and
+    their row-capacity status."""
+    A = _gapped_csr(device)
+    expected = np.array([[1.0, 0.0, 2.0], [0.0, 3.0, 4.0]])
+    np.testing.assert_array_equal(A.offsets.cpu().numpy(), [0, 3, 6])
+    np.testing.assert_array_equal(A.row_counts.cpu().numpy(), [2, 2])
+    D, cols = _active_dense(A)
+    assert cols == [[0, 2], [1, 2]]
+    np.testing.assert_array_equal(D, expected)
+    np.testing.assert_array_equal(A.uncompress_rows().cpu().numpy(), [0, 0, -1, 1, 1, -1])
+    np.testing.assert_allclose(sparse.bsr_mv(A, _f([5.0, 7.0, 11.0], device)).cpu().numpy(), [27.0, 65.0])
+    np.testing.assert_allclose(sparse.bsr_get_diag(A).cpu().numpy(), [1.0, 3.0])
+    assert A.status_sync() == sparse.BSR_STATUS_SUCCESS and A.status_message() == "success"
+
+    compact = sparse.bsr_copy(A)
+    assert compact.row_counts is None
+    np.testing.assert_array_equal(_active_dense(compact)[0], expected)
+    padded_copy = sparse.bsr_copy(2.0 * A, topology="padded")
+    np.testing.assert_array_equal(padded_copy.offsets.cpu().numpy(), [0, 3, 6])
+    np.testing.assert_array_equal(padded_copy.row_counts.cpu().numpy(), [2, 2])
+    np.testing.assert_array_equal(_active_dense(padded_copy)[0], 2.0 * expected)
+
+    dest = sparse.bsr_zeros(2, 3, wa.float32, device=device, row_capacity=3)
+    sparse.bsr_assign(dest, A, topology="padded")
+    D, cols = _active_dense(dest)
+    assert cols == [[0, 2], [1, 2]]
+    np.testing.assert_array_equal(D, expected)
+    small = sparse.bsr_zeros(2, 3, wa.float32, device=device, row_capacity=1)
+    sparse.bsr_assign(small, A, topology="padded")
+    assert small.status_sync() == sparse.BSR_STATUS_ROW_CAPACITY_EXCEEDED
+    assert small.status_message() == "row capacity exceeded"
+    small.clear_status()
+    assert small.status_sync() == sparse.BSR_STATUS_SUCCESS
+
+    tdest = sparse.bsr_zeros(3, 2, wa.float32, device=device, row_capacity=2)
+    sparse.bsr_set_transpose(tdest, A, topology="padded")
+    D, cols = _active_dense(tdest)
+    assert cols == [[0], [1], [0, 1]]
+    np.testing.assert_array_equal(D, expected.T)
+    tsmall = sparse.bsr_zeros(3, 2, wa.float32, device=device, row_capacity=1)
+    sparse.bsr_set_transpose(tsmall, A, topology="padded")
+    assert tsmall.status_sync() == sparse.BSR_STATUS_ROW_CAPACITY_EXCEEDED
+    np.testing.assert_array_equal(tsmall.row_counts.cpu().numpy(), [1, 1, 0])
+
+    trip = sparse.bsr_zeros(2, 3, wa.float32, device=device, row_capacity=_i([2, 3], device))
+    sparse.bsr_set_from_triplets(trip, _i([1, 0, 1, 0, 1], device), _i([2, 1, 2, 0, 1], device),
+                                 _f([4.0, 1.0, 5.0, 2.0, 3.0], device), topology="padded")
+    D, cols = _active_dense(trip)
+    assert cols == [[0, 1], [1, 2]]
+    np.testing.assert_array_equal(D, [[2.0, 1.0, 0.0], [0.0, 3.0, 9.0]])
+    over = sparse.bsr_zeros(2, 3, wa.float32, device=device, row_capacity=1)
+    sparse.bsr_set_from_triplets(over, _i([0, 0, 1], device), _i([0, 2, 1], device), _f([1.0, 2.0, 3.0], device),
+                                 topology="padded")
+    assert over.status_sync() == sparse.BSR_STATUS_ROW_CAPACITY_EXCEEDED
+    np.testing.assert_array_equal(over.row_counts.cpu().numpy(), [0, 1])
+
+
+def test_padded_alloc_reference_scenario(device):
+    """test_sparse.py:1314-1390: per-row capacities, padded reset, nnz_capacity."""
+    reserved = sparse.bsr_zeros(3, 4, wa.float32, device=device, row_capacity=_i([1, 0, 3], device))
+    sparse.bsr_set_from_triplets(reserved, _i([0, 2, 2], device), _i([0, 1, 3], device), _f([1.0, 2.0, 3.0], device),
+                                 topology="padded")
+    assert reserved.nnz == 4
+    np.testing.assert_array_equal(reserved.offsets.cpu().numpy(), [0, 1, 1, 4])
+    np.testing.assert_array_equal(reserved.row_counts.cpu().numpy(), [1, 0, 2])
+    np.testing.assert_array_equal(_active_dense(reserved)[0], [[1, 0, 0, 0], [0, 0, 0, 0], [0, 2, 0, 3]])
+    sparse.bsr_set_zero(reserved, topology="padded")
+    assert reserved.nnz == 4
+    np.testing.assert_array_equal(reserved.offsets.cpu().numpy(), [0, 1, 1, 4])
+    np.testing.assert_array_equal(reserved.row_counts.cpu().numpy(), [0, 0, 0])
+    sparse.bsr_set_zero(reserved, topology="padded", row_capacity=_i([0, 2, 1], device))
+    assert reserved.nnz_sync() == 3
+    np.testing.assert_array_equal(reserved.offsets.cpu().numpy(), [0, 0, 2, 3])
+
+    over = sparse.bsr_zeros(3, 4, wa.float32, device=device, row_capacity=_i([1, 0, 1], device), nnz_capacity=8)
+    sparse.bsr_set_from_triplets(over, _i([0, 2], device), _i([1, 3], device), _f([2.0, 5.0], device),
+                                 topology="padded")
+    assert over.nnz == 8
+    np.testing.assert_array_equal(over.offsets.cpu().numpy(), [0, 1, 1, 2])
+    np.testing.assert_array_equal(over.row_counts.cpu().numpy(), [1, 0, 1])
+    np.testing.assert_array_equal(_active_dense(over)[0], [[0, 2, 0, 0], [0, 0, 0, 0], [0, 0, 0, 5]])
+
+    # test_sparse.py:165-180: topology only into a hand-made padded matrix
+    t = sparse.bsr_zeros(2, 3, wa.float32, device=device)
+    t.nnz = 5
+    t.offsets = _i([0, 2, 5], device)
+    t.row_counts = _i([0, 0], device)
+    t.columns = _i([-9] * 5, device)
+    t.values = torch.empty(5, dtype=torch.float32, device=device)
+    sparse.bsr_set_from_triplets(t, _i([1, 0, 1], device), _i([2, 1, 2], device), values=None, topology="padded")
+    np.testing.assert_array_equal(t.offsets.cpu().numpy(), [0, 2, 5])
+    np.testing.assert_array_equal(t.row_counts.cpu().numpy(), [1, 1])
+    assert _active_dense(t)[1] == [[1], [2]]
+
+
+@pytest.mark.parametrize("bshape,dtype", [((1, 1), np.float32), ((3, 3), np.float64), ((2, 3), np.float32)])
+def test_transpose_reference_scenario(device, bshape, dtype):
+    """test_sparse.py:631-676: scaled transpose through the expression API, block-shape
+    check, and the masked transpose into a smaller topology."""
+    rng = np.random.default_rng(123)
+    rows = np.array([0, 1, 2, 3, 2, 1], np.int32)
+    cols = np.array([1, 4, 1, 3, 0, 2], np.int32)
+    vals = rng.random(size=(6, *bshape)).astype(dtype)
+    btype = wa.types.type_to_scalar(_t(vals[:1], "cpu").dtype)
+    A = sparse.bsr_zeros(4, 5, btype if bshape == (1, 1) else wa.matrix(bshape, btype), device=device)
+    v = vals.reshape(6) if bshape == (1, 1) else vals
+    sparse.bsr_set_from_triplets(A, _i(rows, device), _i(cols, device), _t(v, device))
+    ref = 2.0 * _active_dense(A)[0].T
+    At = (2.0 * A).transpose().eval()
+    np.testing.assert_allclose(_active_dense(At)[0], ref, rtol=1e-6)
+    if bshape[0] != bshape[1]:
+        with pytest.raises(ValueError, match="Destination block shape must be"):
+            sparse.bsr_set_transpose(dest=A, src=A)
+    sparse.bsr_set_from_triplets(A, _i(rows[:3], device), _i(cols[:3], device), _t(v[:3], device))
+    nt = At.nnz_sync()
+    At = sparse.bsr_from_triplets(At.nrow, At.ncol, At.uncompress_rows()[:3], At.columns[:3], At.values[:3])
+    np.testing.assert_array_equal(At.uncompress_rows().cpu().numpy()[:3], [0, 1, 1])
+    np.testing.assert_array_equal(At.columns.cpu().numpy()[:3], [2, 0, 2])
+    assert nt == 6
+    sparse.bsr_set_transpose(At, A, topology="masked")
+    pruned = sparse.bsr_from_triplets(At.nrow, At.ncol, At.uncompress_rows(), At.columns[: At.nnz], At.values[: At.nnz])
+    assert pruned.nnz_sync() == 2
+
+
+@pytest.mark.parametrize("padded_src,padded_dest", [(False, False), (True, False), (False, True), (True, True)])
+def test_transpose_vs_oracle(device, oracle_mod, padded_src, padded_dest):
+    """Transposed topology bit-exact against the oracle (itself pinned on the
+    reference's host transpose); values are exact copies times 2."""
+    rng = np.random.default_rng(11 + 2 * padded_src + padded_dest)
+    nrow, ncol = 700, 600
+    counts = rng.integers(0, 40, nrow).astype(np.int32)
+    cap = counts + (rng.integers(0, 4, nrow) if padded_src else 0)
+    off = np.concatenate([[0], np.cumsum(cap)]).astype(np.int32)
+    T = int(off[-1])
+    cols = np.full(T, -1, np.int32)
+    for r in range(nrow):
+        cols[off[r]:off[r] + counts[r]] = np.sort(rng.choice(ncol, size=counts[r], replace=False))
+    vals = rng.standard_normal((T, 3, 2)).astype(np.float32)
+    A = sparse.bsr_zeros(nrow, ncol, wa.matrix((3, 2), wa.float32), device=device)
+    A.nnz = T
+    A.offsets, A.columns, A.values = _i(off, device), _i(cols, device), _t(vals, device)
+    A.row_counts = _i(counts, device) if padded_src else None
+    if padded_dest:
+        tcap = rng.integers(0, 45, ncol).astype(np.int32)
+        dest = sparse.bsr_zeros(ncol, nrow, wa.matrix((2, 3), wa.float32), device=device, row_capacity=_t(tcap, device))
+        sparse.bsr_set_transpose(dest, 2.0 * A, topology="padded")
+        toff = np.concatenate([[0], np.cumsum(tcap)]).astype(np.int32)
+        ref = oracle_mod.bsr_transpose(nrow, ncol, off, cols, row_counts=counts if padded_src else None, capacity=toff)
+        assert dest.status_sync() == ref["status"]
+        np.testing.assert_array_equal(dest.row_counts.cpu().numpy(), ref["row_counts"])
+        gc, gv = dest.columns.cpu().numpy(), dest.values.cpu().numpy()
+        for r in range(ncol):
+            n = ref["row_counts"][r]
+            seg = slice(toff[r], toff[r] + n)
+            np.testing.assert_array_equal(gc[seg], ref["columns"][seg])
+            np.testing.assert_array_equal(gv[seg], 2.0 * vals[ref["block_map"][seg]].transpose(0, 2, 1))
+    else:
+        dest = sparse.bsr_transposed(2.0 * A)
+        ref = oracle_mod.bsr_transpose(nrow, ncol, off, cols, row_counts=counts if padded_src else None)
+        n = dest.nnz_sync()
+        assert dest.row_counts is None and n == ref["columns"].size
+        np.testing.assert_array_equal(dest.offsets.cpu().numpy(), ref["offsets"])
+        np.testing.assert_array_equal(dest.columns.cpu().numpy()[:n], ref["columns"])
+        np.testing.assert_array_equal(dest.values.cpu().numpy()[:n],
+                                      2.0 * vals[ref["block_map"]].transpose(0, 2, 1))
+
+
+def test_padded_triplets_vs_oracle(device, oracle_mod):
+    """Padded assembly: each row is the oracle's compact row (bit-exact sums) when it
+    fits its capacity, empty otherwise, with the sticky status raised."""
+    rng = np.random.default_rng(21)
+    nrow, ncol, T = 400, 300, 30000
+    rows = rng.integers(0, nrow, T).astype(np.int32)
+    cols = rng.integers(0, 60, T).astype(np.int32)
+    vals = rng.standard_normal((T, 3, 3)).astype(np.float32)
+    vals[rng.random(T) < 0.05] = 0.0
+    ref = oracle_mod.bsr_from_triplets(nrow, ncol, rows, cols, vals)
+    ref_cnt = np.diff(ref["offsets"])
+    cap = (ref_cnt + rng.integers(-3, 5, nrow)).clip(0).astype(np.int32)
+    A = sparse.bsr_zeros(nrow, ncol, wa.mat33, device=device, row_capacity=_t(cap, device))
+    sparse.bsr_set_from_triplets(A, _t(rows, device), _t(cols, device), _t(vals, device), topology="padded")
+    off = np.concatenate([[0], np.cumsum(cap)])
+    fits = ref_cnt <= cap
+    assert A.status_sync() == (0 if fits.all() else 1)
+    np.testing.assert_array_equal(A.row_counts.cpu().numpy(), np.where(fits, ref_cnt, 0))
+    gc, gv = A.columns.cpu().numpy(), A.values.cpu().numpy()
+    for r in np.flatnonzero(fits):
+        k0, k1 = ref["offsets"][r], ref["offsets"][r + 1]
+        np.testing.assert_array_equal(gc[off[r]:off[r] + ref_cnt[r]], ref["columns"][k0:k1])
+        assert gv[off[r]:off[r] + ref_cnt[r]].tobytes() == ref["values"][k0:k1].tobytes()
+    over = np.flatnonzero(~fits & (cap > 0))
+    assert not gv[off[over]].any()                      # overflowing rows keep cleared values
+
+
+def test_masked_triplets_vs_oracle(device, oracle_mod):
+    """Masked assembly: the topology is kept; each of its blocks holds the
+    sequential sum of its triplets (bit-exact), zero when it has none."""
+    rng = np.random.default_rng(22)
+    nrow, ncol, T = 300, 200, 20000
+    mask = oracle_mod.bsr_from_triplets(nrow, ncol, rng.integers(0, nrow, 3000).astype(np.int32),
+                                        rng.integers(0, ncol, 3000).astype(np.int32))
+    nm = mask["nnz"]
+    A = sparse.bsr_zeros(nrow, ncol, wa.float64, device=device)
+    A.nnz = nm
+    A.offsets, A.columns = _t(mask["offsets"], device), _t(mask["columns"], device)
+    A.values = torch.full((nm,), 7.0, dtype=torch.float64, device=device)
+    rows = rng.integers(-2, nrow + 2, T).astype(np.int32)
+    cols = rng.integers(0, ncol, T).astype(np.int32)
+    vals = rng.standard_normal(T)
+    count = T - 1000
+    sparse.bsr_set_from_triplets(A, _t(rows, device), _t(cols, device), _t(vals, device),
+                                 count=torch.tensor([count], dtype=torch.int32, device=device), topology="masked")
+    # expected: triplets inside the mask, summed in ascending order (no pruning)
+    sel = np.zeros(count, bool)
+    mrow = np.repeat(np.arange(nrow), np.diff(mask["offsets"]))
+    keys = set(zip(mrow.tolist(), mask["columns"].tolist()))
+    for k in range(count):
+        sel[k] = (int(rows[k]), int(cols[k])) in keys
+    ref = oracle_mod.bsr_from_triplets(nrow, ncol, rows[:count][sel], cols[:count][sel], vals[:count][sel],
+                                       prune=False)
+    got = A.values.cpu().numpy()
+    expect = np.zeros(nm)
+    lookup = {(r, c): k for k, (r, c) in enumerate(zip(np.repeat(np.arange(nrow), np.diff(ref["offsets"])).tolist(),
+                                                       ref["columns"].tolist()))}
+    for k, key in enumerate(zip(mrow.tolist(), mask["columns"].tolist())):
+        if key in lookup:
+            expect[k] = ref["values"][lookup[key]]
+    assert got.tobytes() == expect.tobytes()
+    np.testing.assert_array_equal(A.offsets.cpu().numpy(), mask["offsets"])
+    with pytest.warns(DeprecationWarning):
+        sparse.bsr_set_from_triplets(A, _t(rows, device), _t(cols, device), _t(vals, device), masked=True)
+
+
+@pytest.mark.parametrize("src_shape,dest_shape,dest_dtype", [
+    ((3, 2), (1, 1), np.float64), ((3, 2), (3, 2), np.float32), ((1, 1), (3, 2), np.float32),
+    ((3, 2), (6, 4), np.float64), ((2, 2), (1, 2), np.float32)])
+def test_bsr_assign_reblock_vs_dense(device, src_shape, dest_shape, dest_dtype):
+    """bsr_assign / bsr_copy with another block shape or scalar type reproduce the
+    scaled dense matrix exactly (test_sparse.py:554-602 split/merge)."""
+    rng = np.random.default_rng(31)
+    nrow, ncol, T = 24, 36, 300
+    rows = rng.integers(0, nrow, T).astype(np.int32)
+    cols = rng.integers(0, ncol, T).astype(np.int32)
+    vals = rng.standard_normal((T, *src_shape)).astype(np.float32)
+    stype = wa.float32 if src_shape == (1, 1) else wa.matrix(src_shape, wa.float32)
+    A = sparse.bsr_zeros(nrow, ncol, stype, device=device)
+    sparse.bsr_set_from_triplets(A, _t(rows, device), _t(cols, device),
+                                 _t(vals.reshape(T) if src_shape == (1, 1) else vals, device))
+    D = _active_dense(A)[0]
+    st = wa.float64 if dest_dtype == np.float64 else wa.float32
+    B = sparse.bsr_copy(-0.5 * A, scalar_type=st, block_shape=dest_shape)
+    assert B.block_shape == tuple(dest_shape)
+    B.nnz_sync()
+    Db, _ = _active_dense(B)
+    np.testing.assert_array_equal(Db, -0.5 * D)
+    M = sparse.bsr_zeros(B.nrow, B.ncol, B.dtype, device=device)
+    sparse.bsr_assign(M, B, structure_only=True)
+    assert M.nnz_sync() == B.nnz_sync()
+    np.testing.assert_array_equal(M.columns.cpu().numpy()[: M.nnz], B.columns.cpu().numpy()[: B.nnz])
+    # masked: keep only the diagonal blocks of the destination
+    diag = sparse.bsr_from_triplets(B.nrow, B.ncol, _t(np.arange(min(B.nrow, B.ncol), dtype=np.int32), device),
+                                    _t(np.arange(min(B.nrow, B.ncol), dtype=np.int32), device),
+                                    torch.ones((min(B.nrow, B.ncol), *dest_shape) if dest_shape != (1, 1)
+                                               else (min(B.nrow, B.ncol),), dtype=B.values.dtype, device=device),
+                                    prune_numerical_zeros=False)
+    sparse.bsr_assign(diag, A, topology="masked")
+    Dd, _ = _active_dense(diag)
+    br, bc = dest_shape
+    keep = np.zeros_like(D, dtype=bool)
+    for k in range(min(B.nrow, B.ncol)):
+        keep[k * br:(k + 1) * br, k * bc:(k + 1) * bc] = True
+    np.testing.assert_array_equal(Dd, np.where(keep, D, 0.0))

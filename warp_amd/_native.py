@@ -60,6 +60,34 @@ _SIGNATURES = {
         c_int,
         [c_int, c_int, c_int, c_int, c_int, c_int, c_u64, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr],
     ),
+    "warp_amd_bsr_topology_workspace_size": (c_size, [c_int, c_int, c_int]),
+    "warp_amd_bsr_set_from_triplets_padded": (
+        c_int,
+        [c_int, c_int, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_u64, c_ptr, c_ptr, c_ptr, c_ptr, c_i64]
+        + [c_ptr, c_ptr, c_size, c_ptr],
+    ),
+    "warp_amd_bsr_set_from_triplets_masked": (
+        c_int,
+        [c_int, c_int, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_i64]
+        + [c_ptr, c_size, c_ptr],
+    ),
+    "warp_amd_bsr_transpose_workspace_size": (c_size, [c_int, c_int]),
+    "warp_amd_bsr_transpose": (
+        c_int, [c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]
+    ),
+    "wp_bsr_transpose_device": (None, [c_int, c_int, c_int] + [c_ptr] * 8),
+    "warp_amd_bsr_transpose_values": (c_int, [c_int, c_int, c_int, c_i64, c_ptr, c_ptr, c_ptr, c_f64, c_ptr, c_ptr]),
+    "warp_amd_bsr_transpose_masked_values": (
+        c_int, [c_int, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_f64, c_ptr, c_ptr]
+    ),
+    "warp_amd_bsr_assign_list_blocks": (c_int, [c_int] * 6 + [c_ptr] * 5 + [c_ptr]),
+    "warp_amd_bsr_assign_copy_blocks": (
+        c_int, [c_int] * 8 + [c_ptr, c_ptr, c_ptr, c_int, c_int, c_ptr, c_f64, c_ptr, c_ptr, c_ptr, c_int, c_int, c_ptr, c_ptr]
+    ),
+    "warp_amd_bsr_assign_padded": (
+        c_int, [c_int, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_f64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]
+    ),
+    "warp_amd_scale_values": (c_int, [c_int, c_i64, c_ptr, c_f64, c_ptr]),
     "warp_amd_bsr_residual_workspace_size": (c_size, [c_int]),
     "warp_amd_bsr_residual_f32_f64": (c_int, [c_int, c_int, c_ptr, c_ptr, c_ptr, c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
     "warp_amd_axpy_f32_to_f64": (c_int, [c_i64, c_f64, c_ptr, c_ptr, c_ptr]),

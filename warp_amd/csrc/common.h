@@ -32,6 +32,11 @@ enum Status : int {
     kErrWorkspace = -4,
 };
 
+// Sticky device status codes of padded-topology operations (reference
+// BSR_STATUS_*, warp/_src/sparse.py:80-84).
+constexpr int kStatusSuccess = 0;
+constexpr int kStatusRowCapacityExceeded = 1;
+
 void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
 void clear_error();
 const char* last_error_cstr();

@@ -143,6 +143,8 @@ __global__ __launch_bounds__(kScanBlock) void scan_tile_kernel(
     }
 }
 
+__global__ void scan_empty_kernel(int* out) { *out = 0; }
+
 }  // namespace
 
 size_t scan_workspace_bytes(int64_t n)
@@ -154,7 +156,9 @@ size_t scan_workspace_bytes(int64_t n)
 int exclusive_scan_i32(const int* in, int* out, int64_t n, void* ws, hipStream_t stream)
 {
     if (n <= 0) {
-        WA_HIP(hipMemsetAsync(out, 0, sizeof(int), stream));
+        // a kernel rather than a memset node, so the call can be graph-captured
+        scan_empty_kernel<<<1, 1, 0, stream>>>(out);
+        WA_LAUNCH_CHECK();
         return kOk;
     }
     const int64_t tiles = ceil_div(n, kScanTile);

@@ -371,36 +371,20 @@ size_t triplets_workspace_bytes(int nrow, int T)
     return needed + 256;
 }
 
-int bsr_from_triplets(
+// Steps 1-4: bucket the valid triplets by row and sort every bucket by
+// (col, triplet index).  Leaves the sorted keys at w.keys[w.row_start[r] ...],
+// the sorted triplet indices in summed_indices and the unique column count of
+// every row in urow[0..nrow).
+int bucket_sort_rows(
     int block_size, int scalar_size, int nrow, int ncol, int T, const int* tpl_count, const int* rows, const int* cols,
-    const void* tpl_values, uint64_t zero_mask, int masked, int* summed_offsets, int* summed_indices, int* bsr_offsets,
-    const int* bsr_row_counts, int* bsr_columns, void* bsr_values, int* bsr_nnz_host, void* workspace,
-    size_t workspace_bytes, hipStream_t stream
+    const void* tpl_values, uint64_t zero_mask, const int* mo, const int* mrc, const int* mc, const TripletWs& w,
+    int* summed_indices, int* urow, hipStream_t stream
 )
 {
-    WA_REQUIRE(nrow >= 0 && ncol >= 0 && T >= 0 && block_size > 0, "bsr_from_triplets: invalid sizes");
-    WA_REQUIRE(bsr_offsets != nullptr, "bsr_from_triplets: null offsets");
-    WA_REQUIRE(T == 0 || (rows && cols && bsr_columns), "bsr_from_triplets: null triplet or column arrays");
-    const bool own_summed = !(summed_offsets && summed_indices);
-    size_t needed = 0;
-    TripletWs w = carve_ws(workspace, workspace_bytes, nrow, T, own_summed, &needed);
-    WA_REQUIRE(workspace && needed <= workspace_bytes,
-               "bsr_from_triplets: workspace too small (%zu < %zu bytes)", workspace_bytes, needed);
-    if (own_summed) {
-        summed_offsets = w.summed_offsets;
-        summed_indices = w.summed_indices;
-    }
     const bool prune = tpl_values && zero_mask != 0;
-    const bool sum_values = tpl_values && bsr_values;
-    if (sum_values)
-        WA_REQUIRE(scalar_size == 4 || scalar_size == 8, "bsr_from_triplets: value accumulation needs fp32 or fp64");
     if (prune)
         WA_REQUIRE(scalar_size == 1 || scalar_size == 2 || scalar_size == 4 || scalar_size == 8,
                    "bsr_from_triplets: unsupported scalar size %d", scalar_size);
-    const int* mo = masked ? bsr_offsets : nullptr;
-    const int* mrc = masked ? bsr_row_counts : nullptr;
-    const int* mc = masked ? bsr_columns : nullptr;
-
     WA_TRY(launch_fill(w.hist, (int64_t)nrow + 1, 0, stream));
     WA_TRY(launch_fill(w.counters, 64, 0, stream));
     if (T > 0) {
@@ -446,17 +430,49 @@ int bsr_from_triplets(
         WA_LAUNCH_CHECK();
     }
     if (nrow > 0) {
-        // unique counts land in bsr_offsets[0..nrow), scanned in place below
         sort_small_rows_kernel<<<(unsigned)nrow, kWave, 0, stream>>>(
-            nrow, w.row_start, w.keys, summed_indices, bsr_offsets, w.large_rows, w.counters
+            nrow, w.row_start, w.keys, summed_indices, urow, w.large_rows, w.counters
         );
         WA_LAUNCH_CHECK();
         const int large_grid = device_cu_count() * 2;
         sort_large_rows_kernel<<<(unsigned)large_grid, kBlock, 0, stream>>>(
-            w.row_start, w.keys, w.alt, summed_indices, bsr_offsets, w.large_rows, w.counters
+            w.row_start, w.keys, w.alt, summed_indices, urow, w.large_rows, w.counters
         );
         WA_LAUNCH_CHECK();
     }
+    return kOk;
+}
+
+// Triplets -> compact BSR.  `mo`, `mrc`, `mc` (offsets, row counts, columns of
+// an existing topology, or null) restrict the result to blocks of that
+// topology; they may alias the outputs (the reference's masked mode), as they
+// are read before the outputs are written.
+int bsr_from_triplets(
+    int block_size, int scalar_size, int nrow, int ncol, int T, const int* tpl_count, const int* rows, const int* cols,
+    const void* tpl_values, uint64_t zero_mask, const int* mo, const int* mrc, const int* mc, int* summed_offsets,
+    int* summed_indices, int* bsr_offsets, int* bsr_columns, void* bsr_values, int* bsr_nnz_host, void* workspace,
+    size_t workspace_bytes, hipStream_t stream
+)
+{
+    WA_REQUIRE(nrow >= 0 && ncol >= 0 && T >= 0 && block_size > 0, "bsr_from_triplets: invalid sizes");
+    WA_REQUIRE(bsr_offsets != nullptr, "bsr_from_triplets: null offsets");
+    WA_REQUIRE(T == 0 || (rows && cols && bsr_columns), "bsr_from_triplets: null triplet or column arrays");
+    const bool own_summed = !(summed_offsets && summed_indices);
+    size_t needed = 0;
+    TripletWs w = carve_ws(workspace, workspace_bytes, nrow, T, own_summed, &needed);
+    WA_REQUIRE(workspace && needed <= workspace_bytes,
+               "bsr_from_triplets: workspace too small (%zu < %zu bytes)", workspace_bytes, needed);
+    if (own_summed) {
+        summed_offsets = w.summed_offsets;
+        summed_indices = w.summed_indices;
+    }
+    const bool sum_values = tpl_values && bsr_values;
+    if (sum_values)
+        WA_REQUIRE(scalar_size == 4 || scalar_size == 8, "bsr_from_triplets: value accumulation needs fp32 or fp64");
+
+    // unique counts land in bsr_offsets[0..nrow), scanned in place below
+    WA_TRY(bucket_sort_rows(block_size, scalar_size, nrow, ncol, T, tpl_count, rows, cols, tpl_values, zero_mask, mo,
+                            mrc, mc, w, summed_indices, bsr_offsets, stream));
     WA_TRY(exclusive_scan_i32(bsr_offsets, bsr_offsets, nrow, w.scan_ws, stream));
     if (bsr_nnz_host)
         WA_HIP(hipMemcpyAsync(bsr_nnz_host, bsr_offsets + nrow, sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -493,7 +509,9 @@ int bsr_from_triplets(
 // left-to-right accumulation.
 // ---------------------------------------------------------------------------
 
-// rows[k] = r for every k in row r's active range [offsets[r], end), k < T
+// rows[k] = r for every k in row r's active range [offsets[r], end), k < T;
+// the range never passes the row's capacity end offsets[r+1] (as the
+// reference's bsr_active_row_end clipping, warp/native/sparse.cpp:445-447)
 __global__ __launch_bounds__(kWave) void active_rows_kernel(int nrow, const int* __restrict__ offsets,
                                                            const int* __restrict__ row_counts, int T,
                                                            int* __restrict__ rows)
@@ -502,20 +520,26 @@ __global__ __launch_bounds__(kWave) void active_rows_kernel(int nrow, const int*
     if (r >= nrow)
         return;
     const int beg = offsets[r];
-    int end = row_counts ? beg + row_counts[r] : offsets[r + 1];
+    const int cap_end = offsets[r + 1];
+    int end = row_counts ? beg + row_counts[r] : cap_end;
+    end = end < cap_end ? end : cap_end;
     end = end < T ? end : T;
     for (int k = beg + (int)threadIdx.x; k < end; k += kWave)
         rows[k] = r;
 }
 
-// Padded result: row r's coalesced blocks go to the start of its range.
+// Padded result: row r's compact blocks go to the start of its capacity
+// range [offsets[r], offsets[r+1]) and row_counts[r] is set.  A row whose
+// blocks do not fit gets row_counts[r] = 0 and raises *status to
+// ROW_CAPACITY_EXCEEDED (reference _bsr_set_from_triplets_padded_scatter_
+// compact_topology, warp/_src/sparse.py:1173-1208).  Values move as 32-bit words.
 __global__ __launch_bounds__(kWave) void padded_scatter_kernel(int nrow, const int* __restrict__ offsets,
                                                               int* __restrict__ row_counts,
                                                               const int* __restrict__ c_offsets,
                                                               const int* __restrict__ c_columns,
-                                                              const unsigned char* __restrict__ c_values,
-                                                              int block_bytes, int* __restrict__ columns,
-                                                              unsigned char* __restrict__ values)
+                                                              const uint32_t* __restrict__ c_values, int block_words,
+                                                              int* __restrict__ columns, uint32_t* __restrict__ values,
+                                                              int* __restrict__ status)
 {
     const int r = blockIdx.x;
     if (r >= nrow)
@@ -523,19 +547,65 @@ __global__ __launch_bounds__(kWave) void padded_scatter_kernel(int nrow, const i
     const int src = c_offsets[r];
     const int cnt = c_offsets[r + 1] - src;
     const int dst = offsets[r];
+    if (cnt > offsets[r + 1] - dst) {
+        if (threadIdx.x == 0) {
+            row_counts[r] = 0;
+            if (status)
+                atomicMax(status, kStatusRowCapacityExceeded);
+        }
+        return;
+    }
     if (threadIdx.x == 0)
         row_counts[r] = cnt;
     for (int j = threadIdx.x; j < cnt; j += kWave)
         columns[dst + j] = c_columns[src + j];
     if (values)
-        for (int b = threadIdx.x; b < cnt * block_bytes; b += kWave)
-            values[(size_t)dst * block_bytes + b] = c_values[(size_t)src * block_bytes + b];
+        for (int b = threadIdx.x; b < cnt * block_words; b += kWave)
+            values[(size_t)dst * block_words + b] = c_values[(size_t)src * block_words + b];
+}
+
+// Masked result: every active block (r, c) of the existing topology takes the
+// compact block (r, c) when there is one; a repeated column keeps only its
+// first slot, like the reference's bsr_block_index lookup
+// (_bsr_set_from_triplets_masked_values, warp/_src/sparse.py:1145-1170).
+// Values were cleared beforehand.
+__global__ __launch_bounds__(kWave) void masked_scatter_kernel(int nrow, const int* __restrict__ offsets,
+                                                              const int* __restrict__ row_counts,
+                                                              const int* __restrict__ columns,
+                                                              const int* __restrict__ c_offsets,
+                                                              const int* __restrict__ c_columns,
+                                                              const uint32_t* __restrict__ c_values, int block_words,
+                                                              uint32_t* __restrict__ values)
+{
+    const int r = blockIdx.x;
+    if (r >= nrow)
+        return;
+    const int beg = offsets[r];
+    const int end = row_counts ? beg + row_counts[r] : offsets[r + 1];
+    const int c0 = c_offsets[r];
+    const int c1 = c_offsets[r + 1];
+    for (int k = beg + (int)threadIdx.x; k < end; k += kWave) {
+        const int col = columns[k];
+        if (k > beg && columns[k - 1] == col)
+            continue;
+        int lo = c0, hi = c1;
+        while (lo < hi) {
+            const int mid = lo + (hi - lo) / 2;
+            if (c_columns[mid] < col)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        if (lo < c1 && c_columns[lo] == col)
+            for (int b = 0; b < block_words; ++b)
+                values[(size_t)k * block_words + b] = c_values[(size_t)lo * block_words + b];
+    }
 }
 
 // Compact result: positions at or beyond nnz = offsets[nrow] get column -1 and zero values.
 __global__ __launch_bounds__(kBlock) void clear_tail_kernel(int nrow, const int* __restrict__ offsets, int T,
-                                                           int block_bytes, int* __restrict__ columns,
-                                                           unsigned char* __restrict__ values)
+                                                           int block_words, int* __restrict__ columns,
+                                                           uint32_t* __restrict__ values)
 {
     const int nnz = offsets[nrow];
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -543,32 +613,216 @@ __global__ __launch_bounds__(kBlock) void clear_tail_kernel(int nrow, const int*
         return;
     columns[i] = -1;
     if (values)
-        for (int b = 0; b < block_bytes; ++b)
-            values[i * block_bytes + b] = 0;
+        for (int b = 0; b < block_words; ++b)
+            values[i * block_words + b] = 0u;
 }
 
-struct CompressWs {
+// ---------------------------------------------------------------------------
+// Transpose (reference wp_bsr_transpose_device, warp/native/sparse.cu:1400-1492,
+// and bsr_set_transpose, warp/_src/sparse.py:2636-2833).  Every active block
+// (r, c) at storage position p becomes the triplet (row c, column r, index p);
+// the row bucketing and per-row sort of the assembly order each transposed row
+// by (r, p), the same order as the reference's sort of (c, r) keys with the
+// source block index as tie-break.  Nothing is merged: a repeated block stays
+// repeated.
+// ---------------------------------------------------------------------------
+
+// Compact transposed topology: columns and source block of each sorted key.
+__global__ __launch_bounds__(kBlock) void transpose_emit_kernel(int T, const int* __restrict__ total,
+                                                               const uint64_t* __restrict__ keys,
+                                                               int* __restrict__ t_columns, int* __restrict__ block_map)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= T || i >= *total)
+        return;
+    const uint64_t key = keys[i];
+    t_columns[i] = (int)(key >> 32);
+    block_map[i] = (int)(uint32_t)key;
+}
+
+// Padded transposed topology: row r's sorted keys go to the start of its
+// capacity range, block_map is -1 on the rest of the range; an overflowing row
+// is left empty and raises *status (reference bsr_transpose_scatter_padded,
+// warp/native/sparse.cu:383-428).
+__global__ __launch_bounds__(kWave) void transpose_padded_kernel(int nrow, const int* __restrict__ row_start,
+                                                                const uint64_t* __restrict__ keys,
+                                                                const int* __restrict__ t_offsets,
+                                                                int* __restrict__ t_row_counts,
+                                                                int* __restrict__ t_columns,
+                                                                int* __restrict__ block_map, int* __restrict__ status)
+{
+    const int r = blockIdx.x;
+    if (r >= nrow)
+        return;
+    const int src = row_start[r];
+    const int cnt = row_start[r + 1] - src;
+    const int beg = t_offsets[r];
+    const int cap = t_offsets[r + 1] - beg;
+    const bool fits = cnt <= cap;
+    if (threadIdx.x == 0) {
+        t_row_counts[r] = fits ? cnt : 0;
+        if (!fits && status)
+            atomicMax(status, kStatusRowCapacityExceeded);
+    }
+    const int used = fits ? cnt : 0;
+    for (int j = threadIdx.x; j < cap; j += kWave) {
+        if (j < used) {
+            const uint64_t key = keys[src + j];
+            t_columns[beg + j] = (int)(key >> 32);
+            block_map[beg + j] = (int)(uint32_t)key;
+        } else {
+            block_map[beg + j] = -1;
+        }
+    }
+}
+
+// dst[b] = scale * transpose(src[block_map[b]]) for b < *total with a source
+// block (reference _bsr_transpose_values, warp/_src/sparse.py:2604-2633); one
+// thread per destination scalar.  Source blocks are br x bc.
+template <typename S>
+__global__ __launch_bounds__(kBlock) void transpose_values_kernel(int64_t count, const int* __restrict__ total, int br,
+                                                                 int bc, const int* __restrict__ block_map,
+                                                                 const S* __restrict__ src, S scale,
+                                                                 S* __restrict__ dst)
+{
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int bs = br * bc;
+    if (t >= count * bs)
+        return;
+    const int64_t b = t / bs;
+    if (b >= *total)
+        return;
+    const int sb = block_map[b];
+    if (sb < 0)
+        return;
+    const int e = (int)(t - b * bs);
+    const int i = e / br;   // destination block row = source column
+    const int j = e - i * br;
+    dst[t] = src[(size_t)sb * bs + (size_t)j * bc + i] * scale;
+}
+
+// Masked transpose: each active destination block (r, c) takes
+// scale * transpose of source block (c, r) when the source has it (a binary
+// search in source row c); values were cleared beforehand.  One wave per
+// destination row.
+template <typename S>
+__global__ __launch_bounds__(kWave) void transpose_masked_values_kernel(
+    int t_nrow, const int* __restrict__ t_offsets, const int* __restrict__ t_row_counts,
+    const int* __restrict__ t_columns, int nrow, const int* __restrict__ offsets, const int* __restrict__ row_counts,
+    const int* __restrict__ columns, int br, int bc, const S* __restrict__ src, S scale, S* __restrict__ dst)
+{
+    const int r = blockIdx.x;
+    if (r >= t_nrow)
+        return;
+    const int beg = t_offsets[r];
+    const int end = t_row_counts ? beg + t_row_counts[r] : t_offsets[r + 1];
+    const int bs = br * bc;
+    for (int k = beg + (int)threadIdx.x; k < end; k += kWave) {
+        const int c = t_columns[k];
+        if (c < 0 || c >= nrow)
+            continue;
+        const int s0 = offsets[c];
+        const int s1 = row_counts ? s0 + row_counts[c] : offsets[c + 1];
+        int lo = s0, hi = s1;
+        while (lo < hi) {
+            const int mid = lo + (hi - lo) / 2;
+            if (columns[mid] < r)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        if (lo >= s1 || columns[lo] != r)
+            continue;
+        for (int e = 0; e < bs; ++e) {
+            const int i = e / br;
+            const int j = e - i * br;
+            dst[(size_t)k * bs + e] = src[(size_t)lo * bs + (size_t)j * bc + i] * scale;
+        }
+    }
+}
+
+// Workspace of the compress / padded / masked / transpose drivers: source row
+// of every storage position (compress, transpose), a compact staging result,
+// and the assembly's own workspace.
+struct StageWs {
     int* rows;
     int* c_offsets;
     int* c_columns;
-    unsigned char* c_values;
+    uint32_t* c_values;
+    int* urow;
     void* tws;
     size_t tws_bytes;
 };
 
-CompressWs carve_compress(void* base, size_t bytes, int nrow, int T, int block_bytes, size_t* needed)
+StageWs carve_stage(void* base, size_t bytes, int nrow, int bucket_rows, int T, int block_bytes, bool need_rows,
+                    size_t* needed)
 {
     Carve c(base, bytes);
-    CompressWs w;
-    w.rows = c.take<int>((size_t)T);
+    StageWs w;
+    w.rows = need_rows ? c.take<int>((size_t)T) : nullptr;
     w.c_offsets = c.take<int>((size_t)nrow + 1);
     w.c_columns = c.take<int>((size_t)T);
-    w.c_values = block_bytes > 0 ? c.take<unsigned char>((size_t)T * block_bytes) : nullptr;
-    const size_t tneed = triplets_workspace_bytes(nrow, T);
+    w.c_values = block_bytes > 0 ? c.take<uint32_t>((size_t)T * (block_bytes / 4)) : nullptr;
+    w.urow = c.take<int>((size_t)bucket_rows + 1);
+    const size_t tneed = triplets_workspace_bytes(bucket_rows, T);
     w.tws = c.take<char>(tneed);
     w.tws_bytes = tneed;
     *needed = c.used;
     return w;
+}
+
+int launch_zero_words(void* p, int64_t words, hipStream_t stream)
+{
+    return launch_fill(static_cast<int*>(p), words, 0, stream);
+}
+
+// Source row of every active storage position, -1 elsewhere.
+int mark_active_rows(int nrow, const int* offsets, const int* row_counts, int T, int* rows, hipStream_t s)
+{
+    WA_TRY(launch_fill(rows, T, -1, s));
+    if (T > 0 && nrow > 0) {
+        active_rows_kernel<<<(unsigned)nrow, kWave, 0, s>>>(nrow, offsets, row_counts, T, rows);
+        WA_LAUNCH_CHECK();
+    }
+    return kOk;
+}
+
+int bsr_transpose(int nrow, int ncol, int T, const int* offsets, const int* row_counts, const int* columns,
+                  int* t_offsets, int* t_row_counts, int* t_columns, int* block_map, int* status, void* ws,
+                  size_t bytes, hipStream_t s)
+{
+    WA_REQUIRE(nrow >= 0 && ncol >= 0 && T >= 0, "bsr_transpose: invalid sizes");
+    WA_REQUIRE(offsets && t_offsets && (T == 0 || (columns && t_columns && block_map)),
+               "bsr_transpose: null topology arrays");
+    size_t needed = 0;
+    StageWs w = carve_stage(ws, bytes, 0, ncol, T, 0, true, &needed);
+    WA_REQUIRE(ws && needed <= bytes, "bsr_transpose: workspace too small (%zu < %zu bytes)", bytes, needed);
+    size_t tneed = 0;
+    TripletWs tw = carve_ws(w.tws, w.tws_bytes, ncol, T, true, &tneed);
+    WA_TRY(mark_active_rows(nrow, offsets, row_counts, T, w.rows, s));
+    // triplet i = (row columns[i], column rows[i]); inactive positions have column -1 and are dropped
+    WA_TRY(bucket_sort_rows(1, 0, ncol, nrow, T, nullptr, columns, w.rows, nullptr, 0, nullptr, nullptr, nullptr, tw,
+                            tw.summed_indices, w.urow, s));
+    if (t_row_counts == nullptr) {
+        WA_HIP(hipMemcpyAsync(t_offsets, tw.row_start, sizeof(int) * ((size_t)ncol + 1), hipMemcpyDeviceToDevice, s));
+        if (T > 0) {
+            transpose_emit_kernel<<<(unsigned)ceil_div(T, kBlock), kBlock, 0, s>>>(T, tw.row_start + ncol, tw.keys,
+                                                                                   t_columns, block_map);
+            WA_LAUNCH_CHECK();
+        }
+    } else if (ncol > 0) {
+        transpose_padded_kernel<<<(unsigned)ncol, kWave, 0, s>>>(ncol, tw.row_start, tw.keys, t_offsets, t_row_counts,
+                                                                 t_columns, block_map, status);
+        WA_LAUNCH_CHECK();
+    }
+    return kOk;
+}
+
+size_t transpose_workspace_bytes(int ncol, int T)
+{
+    size_t needed = 0;
+    carve_stage(nullptr, 0, 0, ncol, T, 0, true, &needed);
+    return needed + 256;
 }
 
 }  // namespace wa
@@ -591,8 +845,9 @@ WA_API int warp_amd_bsr_from_triplets(
 {
     return wa::bsr_from_triplets(
         block_size, scalar_size, row_count, col_count, nnz, tpl_nnz, tpl_rows, tpl_columns, tpl_values,
-        scalar_zero_mask, masked_topology, summed_block_offsets, summed_block_indices, bsr_offsets, bsr_row_counts,
-        bsr_columns, bsr_values, bsr_nnz_host, workspace, workspace_bytes, wa::as_stream(stream)
+        scalar_zero_mask, masked_topology ? bsr_offsets : nullptr, masked_topology ? bsr_row_counts : nullptr,
+        masked_topology ? bsr_columns : nullptr, summed_block_offsets, summed_block_indices, bsr_offsets, bsr_columns,
+        bsr_values, bsr_nnz_host, workspace, workspace_bytes, wa::as_stream(stream)
     );
 }
 
@@ -627,8 +882,9 @@ WA_API void wp_bsr_matrix_from_triplets_device(
     }
     wa::bsr_from_triplets(
         block_size, scalar_size, row_count, col_count, nnz, tpl_nnz, tpl_rows, tpl_columns, tpl_values,
-        scalar_zero_mask, masked_topology ? 1 : 0, tpl_block_offsets, tpl_block_indices, bsr_offsets, bsr_row_counts,
-        bsr_columns, nullptr, bsr_nnz, ws, bytes, stream
+        scalar_zero_mask, masked_topology ? bsr_offsets : nullptr, masked_topology ? bsr_row_counts : nullptr,
+        masked_topology ? bsr_columns : nullptr, tpl_block_offsets, tpl_block_indices, bsr_offsets, bsr_columns,
+        nullptr, bsr_nnz, ws, bytes, stream
     );
     if (bsr_nnz && bsr_nnz_event)
         (void)hipEventRecord(static_cast<hipEvent_t>(bsr_nnz_event), stream);
@@ -644,7 +900,7 @@ WA_API void wp_bsr_matrix_from_triplets_device(
 WA_API size_t warp_amd_bsr_compress_workspace_size(int nrow, int T, int block_bytes)
 {
     size_t needed = 0;
-    wa::carve_compress(nullptr, 0, nrow, T, block_bytes, &needed);
+    wa::carve_stage(nullptr, 0, nrow, nrow, T, block_bytes, true, &needed);
     return needed + 256;
 }
 
@@ -669,8 +925,9 @@ WA_API int warp_amd_bsr_compress(int block_size, int scalar_size, int nrow, int 
     WA_REQUIRE(!values || scalar_size == 4 || scalar_size == 8, "bsr_compress: values must be fp32 or fp64");
     hipStream_t s = wa::as_stream(stream);
     const int block_bytes = values ? block_size * scalar_size : 0;
+    const int block_words = block_bytes / 4;
     size_t needed = 0;
-    wa::CompressWs w = wa::carve_compress(workspace, workspace_bytes, nrow, T, block_bytes, &needed);
+    wa::StageWs w = wa::carve_stage(workspace, workspace_bytes, nrow, nrow, T, block_bytes, true, &needed);
     WA_REQUIRE(workspace && needed <= workspace_bytes, "bsr_compress: workspace too small (%zu < %zu bytes)",
                workspace_bytes, needed);
     if (nrow == 0) {
@@ -678,14 +935,10 @@ WA_API int warp_amd_bsr_compress(int block_size, int scalar_size, int nrow, int 
             *nnz_host = 0;
         return wa::kOk;
     }
-    WA_TRY(wa::launch_fill(w.rows, T, -1, s));
-    if (T > 0) {
-        wa::active_rows_kernel<<<(unsigned)nrow, wa::kWave, 0, s>>>(nrow, offsets, row_counts, T, w.rows);
-        WA_LAUNCH_CHECK();
-    }
+    WA_TRY(wa::mark_active_rows(nrow, offsets, row_counts, T, w.rows, s));
     WA_TRY(wa::bsr_from_triplets(block_size, scalar_size, nrow, ncol, T, nullptr, w.rows, columns, values,
-                                 prune_numerical_zeros && values ? zero_mask : 0, 0, nullptr, nullptr, w.c_offsets,
-                                 nullptr, w.c_columns, values ? (void*)w.c_values : nullptr,
+                                 prune_numerical_zeros && values ? zero_mask : 0, nullptr, nullptr, nullptr, nullptr,
+                                 nullptr, w.c_offsets, w.c_columns, values ? (void*)w.c_values : nullptr,
                                  make_compact ? nnz_host : nullptr, w.tws, w.tws_bytes, s));
     if (make_compact) {
         WA_HIP(hipMemcpyAsync(offsets, w.c_offsets, sizeof(int) * ((size_t)nrow + 1), hipMemcpyDeviceToDevice, s));
@@ -694,16 +947,211 @@ WA_API int warp_amd_bsr_compress(int block_size, int scalar_size, int nrow, int 
             if (values)
                 WA_HIP(hipMemcpyAsync(values, w.c_values, (size_t)T * block_bytes, hipMemcpyDeviceToDevice, s));
             wa::clear_tail_kernel<<<(unsigned)wa::ceil_div(T, wa::kBlock), wa::kBlock, 0, s>>>(
-                nrow, offsets, T, block_bytes, columns, static_cast<unsigned char*>(values)
+                nrow, offsets, T, block_words, columns, static_cast<uint32_t*>(values)
             );
             WA_LAUNCH_CHECK();
         }
     } else {
         wa::padded_scatter_kernel<<<(unsigned)nrow, wa::kWave, 0, s>>>(
-            nrow, offsets, row_counts, w.c_offsets, w.c_columns, w.c_values, block_bytes, columns,
-            static_cast<unsigned char*>(values)
+            nrow, offsets, row_counts, w.c_offsets, w.c_columns, w.c_values, block_words, columns,
+            static_cast<uint32_t*>(values), nullptr
         );
         WA_LAUNCH_CHECK();
     }
+    return wa::kOk;
+}
+
+// ---------------------------------------------------------------------------
+// Padded and masked triplet assembly
+// ---------------------------------------------------------------------------
+
+// Workspace of warp_amd_bsr_set_from_triplets_padded / _masked for nrow block
+// rows, T triplets and blocks of block_bytes bytes (0: topology only).
+WA_API size_t warp_amd_bsr_topology_workspace_size(int nrow, int T, int block_bytes)
+{
+    size_t needed = 0;
+    wa::carve_stage(nullptr, 0, nrow, nrow, T, block_bytes, false, &needed);
+    return needed + 256;
+}
+
+// Triplets into the existing row capacity of a padded matrix (reference
+// bsr_set_from_triplets with topology="padded", warp/_src/sparse.py:1335-1420).
+// offsets[0..nrow] is read as capacity and left unchanged; row r receives its
+// sorted unique columns at offsets[r].. and row_counts[r] their number.  A row
+// whose blocks exceed its capacity is left empty and raises *status (sticky) to
+// ROW_CAPACITY_EXCEEDED.  With values, the first `capacity` blocks of `values`
+// are cleared and the summed blocks written (duplicates summed in ascending
+// triplet order, as the compact path).  Invalid and (zero_mask != 0) all-zero
+// triplets are dropped as in warp_amd_bsr_from_triplets.
+WA_API int warp_amd_bsr_set_from_triplets_padded(int block_size, int scalar_size, int nrow, int ncol, int T,
+                                                 const int* tpl_count, const int* rows, const int* cols,
+                                                 const void* tpl_values, uint64_t zero_mask, const int* offsets,
+                                                 int* row_counts, int* columns, void* values, int64_t capacity,
+                                                 int* status, void* workspace, size_t workspace_bytes, void* stream)
+{
+    WA_REQUIRE(nrow >= 0 && ncol >= 0 && T >= 0 && block_size > 0 && capacity >= 0, "bsr_set_from_triplets: invalid sizes");
+    WA_REQUIRE(offsets && row_counts && (T == 0 || (rows && cols && columns)),
+               "bsr_set_from_triplets: null topology arrays");
+    const bool with_values = tpl_values && values;
+    WA_REQUIRE(!with_values || scalar_size == 4 || scalar_size == 8,
+               "bsr_set_from_triplets: values must be fp32 or fp64");
+    hipStream_t s = wa::as_stream(stream);
+    const int block_bytes = with_values ? block_size * scalar_size : 0;
+    const int block_words = block_bytes / 4;
+    size_t needed = 0;
+    wa::StageWs w = wa::carve_stage(workspace, workspace_bytes, nrow, nrow, T, block_bytes, false, &needed);
+    WA_REQUIRE(workspace && needed <= workspace_bytes, "bsr_set_from_triplets: workspace too small (%zu < %zu bytes)",
+               workspace_bytes, needed);
+    if (nrow == 0)
+        return wa::kOk;
+    WA_TRY(wa::bsr_from_triplets(block_size, scalar_size, nrow, ncol, T, tpl_count, rows, cols, tpl_values, zero_mask,
+                                 nullptr, nullptr, nullptr, nullptr, nullptr, w.c_offsets, w.c_columns,
+                                 with_values ? (void*)w.c_values : nullptr, nullptr, w.tws, w.tws_bytes, s));
+    if (with_values)
+        WA_TRY(wa::launch_zero_words(values, capacity * block_words, s));
+    wa::padded_scatter_kernel<<<(unsigned)nrow, wa::kWave, 0, s>>>(
+        nrow, offsets, row_counts, w.c_offsets, w.c_columns, w.c_values, block_words, columns,
+        with_values ? static_cast<uint32_t*>(values) : nullptr, status
+    );
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}
+
+// Triplets into the existing topology of `offsets` / `row_counts` (NULL:
+// compact) / `columns`, which is left unchanged (reference
+// bsr_set_from_triplets with topology="masked", warp/_src/sparse.py:1422-1441):
+// the first `capacity` blocks of `values` are cleared, and every active block
+// (r, c) receives the sum of the valid triplets at (r, c), in ascending
+// triplet order (the reference adds them with atomics, in no fixed order).
+// Triplets outside the topology are dropped; none is pruned as zero.
+WA_API int warp_amd_bsr_set_from_triplets_masked(int block_size, int scalar_size, int nrow, int ncol, int T,
+                                                 const int* tpl_count, const int* rows, const int* cols,
+                                                 const void* tpl_values, const int* offsets, const int* row_counts,
+                                                 const int* columns, void* values, int64_t capacity, void* workspace,
+                                                 size_t workspace_bytes, void* stream)
+{
+    WA_REQUIRE(nrow >= 0 && ncol >= 0 && T >= 0 && block_size > 0 && capacity >= 0, "bsr_set_from_triplets: invalid sizes");
+    WA_REQUIRE(offsets && values && tpl_values && (T == 0 || (rows && cols && columns)),
+               "bsr_set_from_triplets: null arrays");
+    WA_REQUIRE(scalar_size == 4 || scalar_size == 8, "bsr_set_from_triplets: values must be fp32 or fp64");
+    hipStream_t s = wa::as_stream(stream);
+    const int block_bytes = block_size * scalar_size;
+    const int block_words = block_bytes / 4;
+    size_t needed = 0;
+    wa::StageWs w = wa::carve_stage(workspace, workspace_bytes, nrow, nrow, T, block_bytes, false, &needed);
+    WA_REQUIRE(workspace && needed <= workspace_bytes, "bsr_set_from_triplets: workspace too small (%zu < %zu bytes)",
+               workspace_bytes, needed);
+    WA_TRY(wa::launch_zero_words(values, capacity * block_words, s));
+    if (nrow == 0)
+        return wa::kOk;
+    WA_TRY(wa::bsr_from_triplets(block_size, scalar_size, nrow, ncol, T, tpl_count, rows, cols, tpl_values, 0, offsets,
+                                 row_counts, columns, nullptr, nullptr, w.c_offsets, w.c_columns, w.c_values, nullptr,
+                                 w.tws, w.tws_bytes, s));
+    wa::masked_scatter_kernel<<<(unsigned)nrow, wa::kWave, 0, s>>>(nrow, offsets, row_counts, columns, w.c_offsets,
+                                                                   w.c_columns, w.c_values, block_words,
+                                                                   static_cast<uint32_t*>(values));
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}
+
+// ---------------------------------------------------------------------------
+// Transpose
+// ---------------------------------------------------------------------------
+
+// Workspace of warp_amd_bsr_transpose for a source with ncol block columns and
+// T storage positions.
+WA_API size_t warp_amd_bsr_transpose_workspace_size(int ncol, int T) { return wa::transpose_workspace_bytes(ncol, T); }
+
+// Topology of the transpose of an nrow x ncol block matrix (replaces
+// wp_bsr_transpose_device / _host, warp/native/sparse.cu:1400-1492,
+// sparse.cpp:415-519).  The active blocks of the source (positions below T)
+// are reordered by (column, row, position).  t_row_counts NULL: compact
+// output, t_offsets[0..ncol] and t_columns / block_map[0..nnz) written.
+// Otherwise padded output: t_offsets is read as row capacity; t_row_counts,
+// t_columns and block_map over the capacity are written (block_map -1 on
+// unused slots); a row that does not fit is left empty and raises *status.
+// block_map[b] is the source position of transposed block b.
+WA_API int warp_amd_bsr_transpose(int nrow, int ncol, int T, const int* offsets, const int* row_counts,
+                                  const int* columns, int* t_offsets, int* t_row_counts, int* t_columns,
+                                  int* block_map, int* status, void* workspace, size_t workspace_bytes, void* stream)
+{
+    return wa::bsr_transpose(nrow, ncol, T, offsets, row_counts, columns, t_offsets, t_row_counts, t_columns,
+                             block_map, status, workspace, workspace_bytes, wa::as_stream(stream));
+}
+
+// Same name and arguments as the reference native entry point
+// (warp/native/warp.h, wp_bsr_transpose_device; bound by name in
+// warp/_src/context.py).  Runs on the stream set by
+// warp_amd_set_current_stream and allocates its scratch stream-ordered.
+// src_block_indices needs max(nnz, capacity) entries; errors are reported
+// through warp_amd_last_error().
+WA_API void wp_bsr_transpose_device(int row_count, int col_count, int nnz, const int* bsr_offsets,
+                                    const int* bsr_row_counts, const int* bsr_columns, int* transposed_bsr_offsets,
+                                    int* transposed_bsr_row_counts, int* transposed_bsr_columns,
+                                    int* src_block_indices, int* status)
+{
+    hipStream_t stream = wa::as_stream(g_current_stream);
+    const size_t bytes = wa::transpose_workspace_bytes(col_count, nnz);
+    void* ws = nullptr;
+    hipError_t err = hipMallocAsync(&ws, bytes, stream);
+    if (err != hipSuccess) {
+        wa::set_error("wp_bsr_transpose_device: hipMallocAsync failed: %s", hipGetErrorString(err));
+        return;
+    }
+    wa::bsr_transpose(row_count, col_count, nnz, bsr_offsets, bsr_row_counts, bsr_columns, transposed_bsr_offsets,
+                      transposed_bsr_row_counts, transposed_bsr_columns, src_block_indices, status, ws, bytes, stream);
+    (void)hipFreeAsync(ws, stream);
+}
+
+// dst[b] = scale * transpose(src[block_map[b]]) for the first `count`
+// transposed blocks b below *total (a device int, the transposed
+// offsets[nrow]) whose block_map entry is >= 0 (reference
+// _bsr_transpose_values, warp/_src/sparse.py:2604-2633).  Source blocks are
+// br x bc, destination blocks bc x br; scalar_size 4 (fp32) or 8 (fp64).
+WA_API int warp_amd_bsr_transpose_values(int scalar_size, int br, int bc, int64_t count, const int* total,
+                                         const int* block_map, const void* src, double scale, void* dst, void* stream)
+{
+    WA_REQUIRE(br > 0 && bc > 0 && count >= 0, "bsr_transpose_values: invalid sizes");
+    WA_REQUIRE(scalar_size == 4 || scalar_size == 8, "bsr_transpose_values: values must be fp32 or fp64");
+    if (count == 0)
+        return wa::kOk;
+    WA_REQUIRE(total && block_map && src && dst, "bsr_transpose_values: null arrays");
+    hipStream_t s = wa::as_stream(stream);
+    const unsigned grid = (unsigned)wa::ceil_div(count * br * bc, wa::kBlock);
+    if (scalar_size == 4)
+        wa::transpose_values_kernel<float><<<grid, wa::kBlock, 0, s>>>(
+            count, total, br, bc, block_map, static_cast<const float*>(src), (float)scale, static_cast<float*>(dst));
+    else
+        wa::transpose_values_kernel<double><<<grid, wa::kBlock, 0, s>>>(
+            count, total, br, bc, block_map, static_cast<const double*>(src), scale, static_cast<double*>(dst));
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}
+
+// Masked transpose values (reference bsr_set_transpose with
+// topology="masked", warp/_src/sparse.py:2785-2792, 2831-2848): every active
+// block (r, c) of the destination topology takes scale * transpose of source
+// block (c, r) when the source holds one.  Destination values are not cleared
+// here.
+WA_API int warp_amd_bsr_transpose_masked_values(int scalar_size, int br, int bc, int t_nrow, const int* t_offsets,
+                                                const int* t_row_counts, const int* t_columns, int nrow,
+                                                const int* offsets, const int* row_counts, const int* columns,
+                                                const void* src, double scale, void* dst, void* stream)
+{
+    WA_REQUIRE(br > 0 && bc > 0 && t_nrow >= 0 && nrow >= 0, "bsr_transpose_masked_values: invalid sizes");
+    WA_REQUIRE(scalar_size == 4 || scalar_size == 8, "bsr_transpose_masked_values: values must be fp32 or fp64");
+    if (t_nrow == 0)
+        return wa::kOk;
+    WA_REQUIRE(t_offsets && offsets, "bsr_transpose_masked_values: null arrays");
+    hipStream_t s = wa::as_stream(stream);
+    if (scalar_size == 4)
+        wa::transpose_masked_values_kernel<float><<<(unsigned)t_nrow, wa::kWave, 0, s>>>(
+            t_nrow, t_offsets, t_row_counts, t_columns, nrow, offsets, row_counts, columns, br, bc,
+            static_cast<const float*>(src), (float)scale, static_cast<float*>(dst));
+    else
+        wa::transpose_masked_values_kernel<double><<<(unsigned)t_nrow, wa::kWave, 0, s>>>(
+            t_nrow, t_offsets, t_row_counts, t_columns, nrow, offsets, row_counts, columns, br, bc,
+            static_cast<const double*>(src), scale, static_cast<double*>(dst));
+    WA_LAUNCH_CHECK();
     return wa::kOk;
 }

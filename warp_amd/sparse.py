@@ -5,12 +5,16 @@ reference (warp/sparse.py:25-51 -> warp/_src/sparse.py); arrays are torch
 tensors (see :mod:`warp_amd.types` for the layout).  All work runs in the
 hand-written gfx950 library through ctypes; there is no CPU fallback.
 
-Covered here: ``BsrMatrix`` (compact topology), ``bsr_zeros``,
-``bsr_set_zero``, ``bsr_set_from_triplets``, ``bsr_from_triplets``,
-``bsr_mv`` and ``bsr_get_diag``.
+Covered here: ``BsrMatrix`` (compact and padded topologies, the sticky
+status of padded operations), ``bsr_zeros``, ``bsr_set_zero``,
+``bsr_set_from_triplets`` (compact, masked, padded), ``bsr_from_triplets``,
+``bsr_compress``, ``bsr_set_transpose`` / ``bsr_transposed``, ``bsr_assign`` /
+``bsr_copy``, ``bsr_mv`` (plain and transposed) and ``bsr_get_diag``.
 """
 
 from __future__ import annotations
+
+import warnings
 
 import torch
 
@@ -20,6 +24,24 @@ from .types import block_shape_of, type_is_matrix, type_scalar_type, type_size
 
 BSR_STATUS_SUCCESS = 0
 BSR_STATUS_ROW_CAPACITY_EXCEEDED = 1
+
+
+def _bsr_status_message(status: int) -> str:
+    """Text of a sparse status code (sparse.py:873-878)."""
+    if status == BSR_STATUS_SUCCESS:
+        return "success"
+    if status == BSR_STATUS_ROW_CAPACITY_EXCEEDED:
+        return "row capacity exceeded"
+    return f"unknown status {status}"
+
+
+def _warn_masked_arg_deprecated(func_name: str) -> None:
+    warnings.warn(
+        f"The `masked` argument to {func_name}() is deprecated; pass topology='masked' instead.",
+        DeprecationWarning,
+        stacklevel=3,
+    )
+
 
 # Scalar bit masks used to detect numerical-zero blocks (sparse.py:993-1006).
 _zero_value_masks = {
@@ -157,7 +179,46 @@ class BsrMatrix:
         out[pos[keep]] = rows[keep]
         return out
 
+    # -- sticky status of padded operations (sparse.py:265-310) -----------
+    def _ensure_status(self) -> torch.Tensor:
+        status = getattr(self, "_status", None)
+        if status is None or status.device != self.device:
+            status = torch.zeros(1, dtype=torch.int32, device=self.device)
+            self._status = status
+        return status
+
+    def _status_if_any(self) -> torch.Tensor | None:
+        return getattr(self, "_status", None)
+
+    def status_sync(self) -> int:
+        """Status code of the padded operations since the last :meth:`clear_status` (synchronizes).
+
+        The code is sticky: operations can raise it, successful ones do not clear it."""
+        status = self._status_if_any()
+        if status is None:
+            return BSR_STATUS_SUCCESS
+        if status.device.type == "cuda" and torch.cuda.is_current_stream_capturing():
+            raise RuntimeError(
+                "BsrMatrix.status_sync() cannot read sparse status during a live CUDA graph capture "
+                "because it requires a device-to-host readback. Query the status outside the capture."
+            )
+        return int(status[0])
+
+    def status_message(self) -> str:
+        """Human-readable text of :meth:`status_sync`."""
+        return _bsr_status_message(self.status_sync())
+
+    def clear_status(self) -> None:
+        """Reset the sparse status code to success."""
+        status = self._status_if_any()
+        if status is not None:
+            status.zero_()
+
     # -- operators -------------------------------------------------------
+    def transpose(self):
+        """Transposed copy of this matrix (sparse.py:460-462)."""
+        return bsr_transposed(self)
+
     def __matmul__(self, x):
         if isinstance(x, torch.Tensor):
             return bsr_mv(self, x)
@@ -207,6 +268,45 @@ class _BsrScalingExpression:
     def __neg__(self):
         return _BsrScalingExpression(self.mat, -self.scale)
 
+    def __truediv__(self, s):
+        return _BsrScalingExpression(self.mat, self.scale / s)
+
+    def eval(self) -> BsrMatrix:
+        """Materialize ``scale * A`` as a new matrix (sparse.py:1684-1685)."""
+        return bsr_copy(self)
+
+    def transpose(self):
+        """Transposed copy, still scaled (sparse.py:1776-1778)."""
+        return _BsrScalingExpression(self.mat.transpose(), self.scale)
+
+    @property
+    def nnz(self):
+        return self.mat.nnz
+
+    @property
+    def offsets(self):
+        return self.mat.offsets
+
+    @property
+    def row_counts(self):
+        return self.mat.row_counts
+
+    @property
+    def columns(self):
+        return self.mat.columns
+
+    @property
+    def scalar_type(self):
+        return self.mat.scalar_type
+
+    @property
+    def block_shape(self):
+        return self.mat.block_shape
+
+    @property
+    def block_size(self):
+        return self.mat.block_size
+
     @property
     def nrow(self):
         return self.mat.nrow
@@ -233,7 +333,7 @@ def _extract_matrix_and_scale(A):
         return A, 1.0
     if isinstance(A, _BsrScalingExpression):
         return A.mat, A.scale
-    raise ValueError(f"Expected a BsrMatrix, got {type(A)}")
+    raise ValueError("Argument cannot be interpreted as a BsrMatrix")
 
 
 def _bsr_ensure_fits(bsr: BsrMatrix, nnz: int | None = None, capacity: int | None = None) -> None:
@@ -252,6 +352,88 @@ def _bsr_ensure_fits(bsr: BsrMatrix, nnz: int | None = None, capacity: int | Non
         bsr.values = torch.empty((capacity, *bsr.values.shape[1:]), dtype=bsr.values.dtype, device=bsr.values.device)
 
 
+def _bsr_resize(bsr: BsrMatrix, rows_of_blocks=None, cols_of_blocks=None, preserve_offsets: bool = False) -> None:
+    """New dimensions; with ``preserve_offsets`` retained rows keep their capacity and added rows get none (sparse.py:693-716)."""
+    old_nrow = bsr.nrow
+    old_offsets = bsr.offsets
+    if rows_of_blocks is not None:
+        bsr.nrow = int(rows_of_blocks)
+    if cols_of_blocks is not None:
+        bsr.ncol = int(cols_of_blocks)
+    if bsr.offsets.numel() < bsr.nrow + 1:
+        bsr.offsets = torch.empty(bsr.nrow + 1, dtype=torch.int32, device=old_offsets.device)
+    if preserve_offsets and bsr.nrow > old_nrow:
+        if bsr.offsets is not old_offsets:
+            bsr.offsets[: old_nrow + 1].copy_(old_offsets[: old_nrow + 1])
+        bsr.offsets[old_nrow + 1 : bsr.nrow + 1] = bsr.offsets[old_nrow]
+
+
+def _bsr_ensure_independent_row_counts(bsr: BsrMatrix, preserve_row_counts: bool = True) -> None:
+    """Give ``bsr`` its own ``row_counts`` of size ``nrow`` (sparse.py:724-742)."""
+    old = bsr.row_counts
+    if old is not None and old.numel() >= bsr.nrow:
+        if not preserve_row_counts:
+            old.zero_()
+        return
+    device = bsr.offsets.device
+    if not preserve_row_counts or bsr.nrow == 0:
+        bsr.row_counts = torch.zeros(bsr.nrow, dtype=torch.int32, device=device)
+    else:
+        bsr.row_counts = (bsr.offsets[1 : bsr.nrow + 1] - bsr.offsets[: bsr.nrow]).to(torch.int32)
+
+
+def _check_row_capacity(bsr: BsrMatrix, topology: str, row_capacity, nnz_capacity):
+    if nnz_capacity is not None:
+        nnz_capacity = int(nnz_capacity)
+        if nnz_capacity < 0:
+            raise ValueError("nnz_capacity must be nonnegative")
+    if row_capacity is not None:
+        if topology != "padded":
+            raise ValueError("row_capacity requires topology='padded'")
+        if isinstance(row_capacity, torch.Tensor):
+            if row_capacity.ndim != 1 or row_capacity.shape[0] != bsr.nrow:
+                raise ValueError(f"row_capacity array must have shape ({bsr.nrow},), got {tuple(row_capacity.shape)}")
+            if row_capacity.dtype != torch.int32:
+                raise TypeError("row_capacity array must have dtype int32")
+            if row_capacity.device != bsr.offsets.device:
+                raise ValueError(
+                    f"row_capacity array must reside on the destination matrix device, got {row_capacity.device} and {bsr.offsets.device}"
+                )
+        else:
+            row_capacity = int(row_capacity)
+            if row_capacity < 0:
+                raise ValueError("row_capacity must be nonnegative")
+    return row_capacity, nnz_capacity
+
+
+def _bsr_set_zero_topology(bsr: BsrMatrix, topology: str, row_capacity, nnz_capacity, preserve_row_capacity=False):
+    """Empty topology: compact, or padded with the given / kept row capacity (sparse.py:580-636)."""
+    row_capacity, nnz_capacity = _check_row_capacity(bsr, topology, row_capacity, nnz_capacity)
+    if topology == "padded":
+        if row_capacity is not None:
+            if isinstance(row_capacity, torch.Tensor):
+                bsr.offsets[:1].zero_()
+                torch.cumsum(row_capacity, 0, out=bsr.offsets[1 : bsr.nrow + 1])
+                nnz = None
+            else:
+                bsr.offsets[: bsr.nrow + 1] = (
+                    torch.arange(bsr.nrow + 1, dtype=torch.int32, device=bsr.offsets.device) * row_capacity
+                )
+                nnz = bsr.nrow * row_capacity
+        elif preserve_row_capacity:
+            nnz = bsr.nnz
+        else:
+            bsr.offsets.zero_()
+            nnz = 0
+        _bsr_ensure_independent_row_counts(bsr, preserve_row_counts=False)
+        bsr._ensure_status()
+    else:
+        bsr.offsets.zero_()
+        bsr.row_counts = None
+        nnz = 0
+    bsr.notify_nnz_changed(nnz=nnz, nnz_capacity=nnz_capacity)
+
+
 def bsr_zeros(
     rows_of_blocks: int,
     cols_of_blocks: int,
@@ -267,29 +449,12 @@ def bsr_zeros(
     ``topology="padded"`` (implied by ``row_capacity``) reserves
     ``row_capacity`` slots per row (an int, or an int32 tensor of per-row
     capacities) with ``row_counts`` zero; such matrices are filled in place
-    and coalesced by :func:`bsr_compress`."""
+    (``bsr_set_from_triplets(..., topology="padded")``, ``bsr_compress``,
+    ``bsr_set_transpose``).  ``nnz_capacity`` preallocates block storage."""
     if topology is None:
         topology = "padded" if row_capacity is not None else "compact"
     elif topology not in ("compact", "padded"):
         raise ValueError(f"Unsupported topology policy: {topology}")
-    if nnz_capacity is not None:
-        nnz_capacity = int(nnz_capacity)
-        if nnz_capacity < 0:
-            raise ValueError("nnz_capacity must be nonnegative")
-    if row_capacity is not None:
-        if topology != "padded":
-            raise ValueError("row_capacity requires topology='padded'")
-        if isinstance(row_capacity, torch.Tensor):
-            if row_capacity.ndim != 1 or row_capacity.shape[0] != int(rows_of_blocks):
-                raise ValueError(
-                    f"row_capacity array must have shape ({int(rows_of_blocks)},), got {tuple(row_capacity.shape)}"
-                )
-            if row_capacity.dtype != torch.int32:
-                raise TypeError("row_capacity array must have dtype int32")
-        else:
-            row_capacity = int(row_capacity)
-            if row_capacity < 0:
-                raise ValueError("row_capacity must be nonnegative")
 
     if not type_is_matrix(block_type):
         if type_is_vector_like(block_type):
@@ -308,25 +473,11 @@ def bsr_zeros(
         bsr.values = torch.empty((0, *block_type._shape_), dtype=torch_dtype, device=dev)
     else:
         bsr.values = torch.empty(0, dtype=torch_dtype, device=dev)
+    _bsr_set_zero_topology(bsr, topology, row_capacity, nnz_capacity)
     if topology == "padded":
-        # row r owns [offsets[r], offsets[r+1]), with row_counts[r] active blocks (none yet)
-        if isinstance(row_capacity, torch.Tensor):
-            bsr.offsets[1:] = torch.cumsum(row_capacity.to(dev), 0).to(torch.int32)
-            nnz = int(bsr.offsets[bsr.nrow]) if nnz_capacity is None else nnz_capacity
-        elif row_capacity is not None:
-            bsr.offsets = torch.arange(bsr.nrow + 1, dtype=torch.int32, device=dev) * row_capacity
-            nnz = bsr.nrow * row_capacity
-        else:
-            nnz = 0
-        bsr.row_counts = torch.zeros(bsr.nrow, dtype=torch.int32, device=dev)
-        capacity = max(nnz, nnz_capacity or 0)
-        _bsr_ensure_fits(bsr, nnz=capacity)
-        bsr.columns[:capacity].fill_(-1)
-        bsr.values[:capacity].zero_()
-        bsr.nnz = nnz
-        return bsr
-    _bsr_ensure_fits(bsr, nnz=int(nnz_capacity or 0))
-    bsr.nnz = 0
+        # unused slots: column -1 and zero values
+        bsr.columns.fill_(-1)
+        bsr.values.zero_()
     return bsr
 
 
@@ -334,19 +485,33 @@ def type_is_vector_like(t) -> bool:
     return wt.type_is_vector(t)
 
 
-def bsr_set_zero(bsr: BsrMatrix, rows_of_blocks: int | None = None, cols_of_blocks: int | None = None, topology=None):
-    """Remove all blocks (sparse.py:769-810, compact policy)."""
-    if rows_of_blocks is not None:
-        bsr.nrow = int(rows_of_blocks)
-    if cols_of_blocks is not None:
-        bsr.ncol = int(cols_of_blocks)
-    if bsr.offsets.numel() < bsr.nrow + 1:
-        bsr.offsets = torch.zeros(bsr.nrow + 1, dtype=torch.int32, device=bsr.offsets.device)
-    else:
-        bsr.offsets.zero_()
-    bsr.row_counts = None
-    bsr.nnz = 0
+def bsr_set_zero(
+    bsr: BsrMatrix,
+    rows_of_blocks: int | None = None,
+    cols_of_blocks: int | None = None,
+    *,
+    topology: str = "compact",
+    row_capacity=None,
+    nnz_capacity: int | None = None,
+):
+    """Set a matrix to zero, possibly resizing it (sparse.py:769-810).
+
+    ``"compact"`` drops all blocks; ``"padded"`` keeps (or resets, with
+    ``row_capacity``) the row capacity and empties every row; ``"masked"``
+    keeps the topology and zeroes the values."""
+    if topology not in ("compact", "padded", "masked"):
+        raise ValueError(f"Unsupported topology policy: {topology}")
+    if row_capacity is not None and topology != "padded":
+        raise ValueError("row_capacity requires topology='padded'")
+    if topology == "masked":
+        if rows_of_blocks is not None or cols_of_blocks is not None:
+            raise ValueError("Cannot resize a matrix with topology='masked'")
+        bsr.values.zero_()
+        return
+    preserve = topology == "padded" and row_capacity is None
+    _bsr_resize(bsr, rows_of_blocks, cols_of_blocks, preserve_offsets=preserve)
     bsr._nnz_pending = False
+    _bsr_set_zero_topology(bsr, topology, row_capacity, nnz_capacity, preserve_row_capacity=preserve)
 
 
 def bsr_set_from_triplets(
@@ -363,9 +528,16 @@ def bsr_set_from_triplets(
 
     Triplets with out-of-range indices are dropped; with
     ``prune_numerical_zeros`` (default) blocks whose scalars are all +-0 are
-    dropped too.  Duplicates are summed in ascending triplet order.  After the
-    call ``dest.nnz`` is the triplet count (an upper bound) until
-    :meth:`BsrMatrix.nnz_sync`.
+    dropped too.  Duplicates are summed in ascending triplet order.
+
+    ``topology="compact"`` (default) rebuilds the topology; ``dest.nnz`` is
+    then the triplet count (an upper bound) until :meth:`BsrMatrix.nnz_sync`.
+    ``"padded"`` writes each row into its existing capacity
+    ``offsets[r]:offsets[r+1]`` and sets ``row_counts``; a row that does not
+    fit is left empty and ``dest.status_sync()`` reports
+    ``BSR_STATUS_ROW_CAPACITY_EXCEEDED``.  ``"masked"`` (or the deprecated
+    ``masked=True``) keeps the topology of ``dest`` and drops triplets outside
+    it; no zero block is pruned.
     """
     if topology is None:
         topology = "masked" if masked else "compact"
@@ -417,25 +589,85 @@ def bsr_set_from_triplets(
         if prune_numerical_zeros and not values.is_contiguous():
             raise ValueError("Values array should be contiguous for numerical zero pruning")
 
+    if masked:
+        _warn_masked_arg_deprecated("bsr_set_from_triplets")
+
     nnz = rows.shape[0]
     if nnz == 0:
-        bsr_set_zero(dest)
+        bsr_set_zero(dest, topology=topology)
         return
 
-    if topology != "compact":
-        raise NotImplementedError(f"'{topology}' topology policy is not provided by warp_amd")
-
     _require_gpu(dest.device, "bsr_set_from_triplets")
-    _bsr_ensure_fits(dest, nnz=nnz)
-
     scalar_type = dest.scalar_type
     zero_mask = _zero_value_masks.get(scalar_type, 0) if prune_numerical_zeros and values is not None else 0
     if values is not None:
         values = values.contiguous()
     rows = rows.contiguous()
     columns = columns.contiguous()
-
     lib = _native.lib()
+
+    if topology == "padded":
+        # into the existing row capacity; overflowing rows raise the sticky status
+        _bsr_ensure_independent_row_counts(dest, preserve_row_counts=False)
+        status = dest._ensure_status()
+        block_bytes = dest.block_size * scalar_type.size if values is not None else 0
+        ws_bytes = lib.warp_amd_bsr_topology_workspace_size(dest.nrow, nnz, block_bytes)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dest.device)
+        _native.call(
+            "warp_amd_bsr_set_from_triplets_padded",
+            dest.block_size,
+            scalar_type.size if values is not None else 0,
+            dest.nrow,
+            dest.ncol,
+            nnz,
+            _native.ptr(count),
+            _native.ptr(rows),
+            _native.ptr(columns),
+            _native.ptr(values),
+            zero_mask,
+            _native.ptr(dest.offsets),
+            _native.ptr(dest.row_counts),
+            _native.ptr(dest.columns),
+            _native.ptr(dest.values) if values is not None else None,
+            int(dest.values.shape[0]),
+            _native.ptr(status),
+            _native.ptr(ws),
+            ws_bytes,
+            _native.stream_of(dest.device),
+        )
+        return
+
+    if topology == "masked":
+        # existing topology kept; blocks outside it are dropped
+        if values is None:
+            dest.values.zero_()
+            return
+        block_bytes = dest.block_size * scalar_type.size
+        ws_bytes = lib.warp_amd_bsr_topology_workspace_size(dest.nrow, nnz, block_bytes)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dest.device)
+        _native.call(
+            "warp_amd_bsr_set_from_triplets_masked",
+            dest.block_size,
+            scalar_type.size,
+            dest.nrow,
+            dest.ncol,
+            nnz,
+            _native.ptr(count),
+            _native.ptr(rows),
+            _native.ptr(columns),
+            _native.ptr(values),
+            _native.ptr(dest.offsets),
+            _native.ptr(dest.row_counts),
+            _native.ptr(dest.columns),
+            _native.ptr(dest.values),
+            int(dest.values.shape[0]),
+            _native.ptr(ws),
+            ws_bytes,
+            _native.stream_of(dest.device),
+        )
+        return
+
+    _bsr_ensure_fits(dest, nnz=nnz)
     ws_bytes = lib.warp_amd_bsr_from_triplets_workspace_size(dest.nrow, nnz)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dest.device)
     nnz_buf, nnz_event = dest._setup_nnz_transfer()
@@ -550,6 +782,312 @@ def bsr_compress(src: BsrMatrix, prune_numerical_zeros: bool = True, inplace: bo
         src._nnz_pending = True
         src.row_counts = None
     return src
+
+
+def _positions(A: BsrMatrix) -> int:
+    """Storage positions that can hold active blocks of ``A``."""
+    return max(0, min(int(A.nnz), A.columns.numel(), A.values.shape[0]))
+
+
+def bsr_set_transpose(dest: BsrMatrix, src, masked: bool = False, topology: str | None = None):
+    """Assign the transpose of ``src`` (a matrix or ``scale * matrix``) to ``dest`` (sparse.py:2636-2850).
+
+    ``topology="compact"`` (default) rebuilds ``dest``: transposed rows list
+    their blocks by ascending source row, nothing merged.  ``"masked"`` keeps
+    the topology of ``dest`` (blocks missing from ``src`` become zero).
+    ``"padded"`` writes each transposed row into the existing capacity of
+    ``dest`` and raises ``BSR_STATUS_ROW_CAPACITY_EXCEEDED`` for rows that do
+    not fit (left empty).
+    """
+    src, src_scale = _extract_matrix_and_scale(src)
+    if topology is None:
+        topology = "masked" if masked else "compact"
+    elif topology not in ("compact", "masked", "padded"):
+        raise ValueError(f"Unsupported topology policy: {topology}")
+    elif masked and topology != "masked":
+        raise ValueError("Cannot pass masked=True with a non-masked topology policy")
+    if masked:
+        _warn_masked_arg_deprecated("bsr_set_transpose")
+    if dest.values.device != src.values.device:
+        raise ValueError(
+            f"All arguments must reside on the same device, got {dest.values.device} and {src.values.device}"
+        )
+    if dest.scalar_type != src.scalar_type:
+        raise ValueError(f"All arguments must have the same scalar type, got {dest.scalar_type} and {src.scalar_type}")
+    transpose_block_shape = src.block_shape[::-1]
+    if dest.block_shape != transpose_block_shape:
+        raise ValueError(f"Destination block shape must be {transpose_block_shape}, got {dest.block_shape}")
+    _require_gpu(dest.device, "bsr_set_transpose")
+    if dest.scalar_type.torch_dtype not in (torch.float32, torch.float64):
+        raise TypeError("bsr_set_transpose supports float32 and float64 matrices")
+    lib = _native.lib()
+    stream = _native.stream_of(dest.device)
+    br, bc = src.block_shape
+    ssize = src.scalar_type.size
+
+    if topology in ("masked", "padded") and (dest.nrow != src.ncol or dest.ncol != src.nrow):
+        raise ValueError(
+            f"Destination matrix must have {src.ncol} rows and {src.nrow} columns, got {dest.nrow} and {dest.ncol}"
+        )
+
+    if topology == "masked":
+        dest.values.zero_()
+        _native.call(
+            "warp_amd_bsr_transpose_masked_values", ssize, br, bc, dest.nrow, _native.ptr(dest.offsets),
+            _native.ptr(dest.row_counts), _native.ptr(dest.columns), src.nrow, _native.ptr(src.offsets),
+            _native.ptr(src.row_counts), _native.ptr(src.columns), _native.ptr(src.values), float(src_scale),
+            _native.ptr(dest.values), stream,
+        )
+        return
+
+    if topology == "padded":
+        status = dest._ensure_status()
+        T = _positions(src)
+        if T == 0:
+            bsr_set_zero(dest, topology="padded")
+            return
+        _bsr_ensure_independent_row_counts(dest)
+        dest_slots = dest.columns.numel()
+        block_map = torch.empty(max(dest_slots, T), dtype=torch.int32, device=dest.device)
+        ws_bytes = lib.warp_amd_bsr_transpose_workspace_size(src.ncol, T)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dest.device)
+        _native.call(
+            "warp_amd_bsr_transpose", src.nrow, src.ncol, T, _native.ptr(src.offsets), _native.ptr(src.row_counts),
+            _native.ptr(src.columns), _native.ptr(dest.offsets), _native.ptr(dest.row_counts),
+            _native.ptr(dest.columns), _native.ptr(block_map), _native.ptr(status), _native.ptr(ws), ws_bytes, stream,
+        )
+        count = min(dest_slots, dest.values.shape[0])
+    else:
+        _bsr_resize(dest, rows_of_blocks=src.ncol, cols_of_blocks=src.nrow)
+        T = _positions(src)
+        if T == 0:
+            bsr_set_zero(dest)
+            return
+        _bsr_ensure_fits(dest, nnz=T)
+        block_map = torch.empty(T, dtype=torch.int32, device=dest.device)
+        ws_bytes = lib.warp_amd_bsr_transpose_workspace_size(src.ncol, T)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dest.device)
+        _native.call(
+            "warp_amd_bsr_transpose", src.nrow, src.ncol, T, _native.ptr(src.offsets), _native.ptr(src.row_counts),
+            _native.ptr(src.columns), _native.ptr(dest.offsets), None, _native.ptr(dest.columns),
+            _native.ptr(block_map), None, _native.ptr(ws), ws_bytes, stream,
+        )
+        dest.row_counts = None
+        dest._copy_nnz_async()
+        dest._nnz_pending = True
+        count = T
+    _native.call(
+        "warp_amd_bsr_transpose_values", ssize, br, bc, count, _native.ptr(dest.offsets[dest.nrow :]),
+        _native.ptr(block_map), _native.ptr(src.values), float(src_scale), _native.ptr(dest.values), stream,
+    )
+
+
+def bsr_transposed(A) -> BsrMatrix:
+    """New compact matrix holding the transpose of ``A`` (sparse.py:2853-2870)."""
+    if A.block_shape == (1, 1):
+        block_type = A.scalar_type
+    else:
+        block_type = wt.matrix(shape=A.block_shape[::-1], dtype=A.scalar_type)
+    transposed = bsr_zeros(A.ncol, A.nrow, block_type, device=A.device)
+    bsr_set_transpose(transposed, A)
+    return transposed
+
+
+def bsr_scale(x, alpha) -> BsrMatrix:
+    """``x := alpha * x``; returns the matrix (sparse.py, ``bsr_scale``)."""
+    x, scale = _extract_matrix_and_scale(x)
+    alpha = float(alpha) * scale
+    if alpha != 1.0 and x.nnz > 0:
+        count = min(int(x.nnz), x.values.shape[0])
+        if alpha == 0.0:
+            x.values.zero_()
+        else:
+            _require_gpu(x.device, "bsr_scale")
+            _native.call(
+                "warp_amd_scale_values", x.scalar_type.size, count * x.block_size, _native.ptr(x.values), alpha,
+                _native.stream_of(x.device),
+            )
+    return x
+
+
+def _split_ratios(src_shape, dest_shape):
+    """(src_sub, dest_sub) pieces per block along one axis, or None when neither divides the other."""
+    if src_shape >= dest_shape:
+        src_sub, dest_sub = src_shape // dest_shape, 1
+    else:
+        src_sub, dest_sub = 1, dest_shape // src_shape
+    if src_sub * dest_shape != src_shape * dest_sub:
+        return None
+    return src_sub, dest_sub
+
+
+def _reblock(src: BsrMatrix, dest_block_shape):
+    rows = _split_ratios(src.block_shape[0], dest_block_shape[0])
+    if rows is None:
+        raise ValueError(
+            f"Incompatible dest and src block shapes; block rows must evenly divide one another (Got {dest_block_shape[0]}, {src.block_shape[0]})"
+        )
+    cols = _split_ratios(src.block_shape[1], dest_block_shape[1])
+    if cols is None:
+        raise ValueError(
+            f"Incompatible dest and src block shapes; block columns must evenly divide one another (Got {dest_block_shape[1]}, {src.block_shape[1]})"
+        )
+    (src_subrows, dest_subrows), (src_subcols, dest_subcols) = rows, cols
+    dest_nrow = (src.nrow * src_subrows) // dest_subrows
+    dest_ncol = (src.ncol * src_subcols) // dest_subcols
+    if src.nrow * src_subrows != dest_nrow * dest_subrows or src.ncol * src_subcols != dest_ncol * dest_subcols:
+        raise ValueError(
+            f"The requested block shape {tuple(dest_block_shape)} does not evenly divide the source matrix of total size {src.shape}"
+        )
+    return src_subrows, src_subcols, dest_subrows, dest_subcols, dest_nrow, dest_ncol
+
+
+def bsr_assign(dest: BsrMatrix, src, structure_only: bool = False, masked: bool = False, topology: str | None = None):
+    """Copy ``src`` (a matrix or ``scale * matrix``) into ``dest`` (sparse.py:2258-2452).
+
+    ``dest`` may use another scalar type or a block shape whose dimensions
+    divide, or are divided by, those of ``src``.  ``"compact"`` (default)
+    rebuilds the topology of ``dest``; ``"masked"`` keeps it (source blocks
+    outside it are dropped, missing ones become zero); ``"padded"`` copies
+    each source row into the existing row capacity of ``dest`` (same block
+    shape), raising ``BSR_STATUS_ROW_CAPACITY_EXCEEDED`` for rows that do not
+    fit."""
+    src, src_scale = _extract_matrix_and_scale(src)
+    if dest.values.device != src.values.device:
+        raise ValueError("Source and destination matrices must reside on the same device")
+    if topology is None:
+        topology = "masked" if masked else "compact"
+    elif topology not in ("compact", "masked", "padded"):
+        raise ValueError(f"Unsupported topology policy: {topology}")
+    elif masked and topology != "masked":
+        raise ValueError("Cannot pass masked=True with a non-masked topology policy")
+    if masked:
+        _warn_masked_arg_deprecated("bsr_assign")
+    src_subrows, src_subcols, dest_subrows, dest_subcols, dest_nrow, dest_ncol = _reblock(src, dest.block_shape)
+    _require_gpu(dest.device, "bsr_assign")
+    stream = _native.stream_of(dest.device)
+
+    if topology == "padded":
+        status = dest._ensure_status()
+        if dest_nrow != dest.nrow or dest_ncol != dest.ncol:
+            raise ValueError(
+                f"Incompatible destination matrix size, expected ({dest_nrow}, {dest_ncol}), got ({dest.nrow}, {dest.ncol})"
+            )
+        if dest is src:
+            if not structure_only and src_scale != 1.0:
+                bsr_scale(dest, src_scale)
+            return
+        if dest.block_shape != src.block_shape:
+            raise NotImplementedError("bsr_assign(topology='padded') with a change of block shape is not provided")
+        _bsr_ensure_independent_row_counts(dest)
+        with_values = not structure_only
+        if with_values:
+            for m in (src, dest):
+                if m.scalar_type.torch_dtype not in (torch.float32, torch.float64):
+                    raise TypeError("bsr_assign supports float32 and float64 values")
+        _native.call(
+            "warp_amd_bsr_assign_padded", src.scalar_type.size, dest.scalar_type.size, dest.nrow, src.block_size,
+            _native.ptr(src.offsets), _native.ptr(src.row_counts), _native.ptr(src.columns),
+            _native.ptr(src.values) if with_values else None, float(src_scale), _native.ptr(dest.offsets),
+            _native.ptr(dest.row_counts), _native.ptr(dest.columns),
+            _native.ptr(dest.values) if with_values else None, _native.ptr(status), stream,
+        )
+        return
+
+    if topology == "masked":
+        if dest_nrow != dest.nrow or dest_ncol != dest.ncol:
+            raise ValueError(
+                f"Incompatible destination matrix size, expected ({dest_nrow}, {dest_ncol}), got ({dest.nrow}, {dest.ncol})"
+            )
+    else:
+        _bsr_resize(dest, rows_of_blocks=dest_nrow, cols_of_blocks=dest_ncol)
+
+    if dest is src and topology != "masked":
+        if not structure_only and src_scale != 1.0:
+            bsr_scale(dest, src_scale)
+        return
+
+    T = _positions(src)
+    if topology != "masked":
+        n = T * src_subrows * src_subcols
+        dest_rows = torch.empty(n, dtype=torch.int32, device=dest.device)
+        dest_cols = torch.empty(n, dtype=torch.int32, device=dest.device)
+        _native.call(
+            "warp_amd_bsr_assign_list_blocks", T, src_subrows, src_subcols, dest_subrows, dest_subcols, src.nrow,
+            _native.ptr(src.offsets), _native.ptr(src.row_counts), _native.ptr(src.columns), _native.ptr(dest_rows),
+            _native.ptr(dest_cols), stream,
+        )
+        if n == 0:
+            bsr_set_zero(dest)
+        else:
+            _bsr_ensure_fits(dest, nnz=n)
+            lib = _native.lib()
+            ws_bytes = lib.warp_amd_bsr_from_triplets_workspace_size(dest.nrow, n)
+            ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dest.device)
+            nnz_buf, nnz_event = dest._setup_nnz_transfer()
+            _native.call(
+                "warp_amd_bsr_from_triplets", dest.block_size, 0, dest.nrow, dest.ncol, n, None,
+                _native.ptr(dest_rows), _native.ptr(dest_cols), None, 0, 0, None, None, _native.ptr(dest.offsets),
+                None, _native.ptr(dest.columns), None, nnz_buf.data_ptr(), _native.ptr(ws), ws_bytes, stream,
+            )
+            nnz_event.record(torch.cuda.current_stream(dest.device))
+            dest._nnz_pending = True
+            dest.row_counts = None
+    if not structure_only:
+        dest.values.zero_()
+        _assign_values(dest, src, src_scale, T, src_subrows, src_subcols, dest_subrows, dest_subcols)
+
+
+def _assign_values(dest, src, src_scale, T, src_subrows, src_subcols, dest_subrows, dest_subcols):
+    for m in (src, dest):
+        if m.scalar_type.torch_dtype not in (torch.float32, torch.float64):
+            raise TypeError("bsr_assign supports float32 and float64 values")
+    sbr, sbc = src.block_shape
+    dbr, dbc = dest.block_shape
+    _native.call(
+        "warp_amd_bsr_assign_copy_blocks", src.scalar_type.size, dest.scalar_type.size, T, src_subrows, src_subcols,
+        dest_subrows, dest_subcols, src.nrow, _native.ptr(src.offsets), _native.ptr(src.row_counts),
+        _native.ptr(src.columns), sbr, sbc, _native.ptr(src.values), float(src_scale), _native.ptr(dest.offsets),
+        _native.ptr(dest.row_counts), _native.ptr(dest.columns), dbr, dbc, _native.ptr(dest.values),
+        _native.stream_of(dest.device),
+    )
+
+
+def bsr_copy(A, scalar_type=None, block_shape=None, structure_only: bool = False, topology: str = "compact"):
+    """Copy of ``A`` (a matrix or ``scale * matrix``), possibly with another scalar type or block shape (sparse.py:2455-2601).
+
+    ``topology="padded"`` keeps the row capacity and ``row_counts`` of ``A``
+    (same block shape only)."""
+    src, src_scale = _extract_matrix_and_scale(A)
+    if topology not in ("compact", "padded"):
+        raise ValueError(f"Unsupported topology policy: {topology}")
+    if scalar_type is None:
+        scalar_type = src.scalar_type
+    if block_shape is None:
+        block_shape = src.block_shape
+    block_shape = tuple(block_shape)
+    block_type = scalar_type if block_shape == (1, 1) else wt.matrix(shape=block_shape, dtype=scalar_type)
+    if topology == "padded":
+        if block_shape != src.block_shape:
+            raise NotImplementedError("bsr_copy(topology='padded') with a change of block shape is not provided")
+        copy = bsr_zeros(src.nrow, src.ncol, block_type, device=src.device, topology="padded")
+        _bsr_ensure_fits(copy, nnz=src.nnz)
+        copy.offsets[: src.nrow + 1].copy_(src.offsets[: src.nrow + 1])
+        if src.row_counts is None:
+            copy.row_counts = (src.offsets[1 : src.nrow + 1] - src.offsets[: src.nrow]).to(torch.int32)
+        else:
+            copy.row_counts = src.row_counts[: src.nrow].clone()
+        n = min(int(src.nnz), src.columns.numel(), src.values.shape[0])
+        if n > 0:
+            copy.columns[:n].copy_(src.columns[:n])
+            if not structure_only:
+                copy.values[:n].copy_(src.values[:n])
+                if src_scale != 1.0:
+                    bsr_scale(copy, src_scale)
+        return copy
+    copy = bsr_zeros(src.nrow, src.ncol, block_type, device=src.device)
+    bsr_assign(copy, A, structure_only=structure_only, topology="compact")
+    return copy
 
 
 def _vec_view(array: torch.Tensor, torch_dtype: torch.dtype, expected_scalar_count: int) -> torch.Tensor:
