@@ -220,6 +220,77 @@ int warp_amd_bsr_assign_padded(int src_scalar_size, int dest_scalar_size, int nr
                                double scale, const int* dest_offsets, int* dest_row_counts, int* dest_columns,
                                void* dest_values, int* status, void* stream);
 
+/* rows[p] = block row of storage position p < T, -1 where p holds no active
+ * block (replaces the kernel of BsrMatrix.uncompress_rows,
+ * warp/_src/sparse.py:164-193). */
+int warp_amd_bsr_uncompress_rows(int nrow, const int* offsets, const int* row_counts, int T, int* rows, void* stream);
+
+/* dest[block (rows[i], cols[i])] += scale * src[i] for i < n; missing blocks
+ * and rows[i] < 0 skipped (replaces _bsr_axpy_add_block,
+ * warp/_src/sparse.py:3145-3163).  At most one source block per destination
+ * block per call. */
+int warp_amd_bsr_add_blocks(int scalar_size, int block_size, int64_t n, const int* rows, const int* cols,
+                            const void* src, double scale, const int* dest_offsets, const int* dest_row_counts,
+                            const int* dest_columns, void* dest_values, void* stream);
+
+/* y := alpha x + beta y into y's row capacity (bsr_axpy with
+ * topology="padded", warp/_src/sparse.py:3395-3478 and its kernels
+ * _bsr_axpy_padded_count/_fill/_finalize, 3191-3311): sorted union per row,
+ * merged in place from the back; a row that does not fit is left unchanged
+ * with row_counts 0 and raises *status to 1. */
+int warp_amd_bsr_axpy_padded(int scalar_size, int block_size, int nrow, double alpha, double beta,
+                             const int* x_offsets, const int* x_row_counts, const int* x_columns,
+                             const void* x_values, const int* y_offsets, int* y_row_counts, int* y_columns,
+                             void* y_values, int* status, void* stream);
+
+/* Symbolic sparse product x @ y (replaces bsr_mm_count_coeffs and its scan,
+ * warp/_src/sparse.py:3620-3680, 4455-4475): mm_offsets[0..T] = exclusive
+ * scan over the T x positions of the product blocks each lists, row_min as
+ * the reference, *total_host = mm_offsets[T] (pinned, asynchronous,
+ * optional).  deep_ranges = 1: a row with more products than the span of the
+ * columns it reaches lists the whole span (the reference's compact rule). */
+size_t warp_amd_bsr_mm_count_workspace_size(int T);
+int warp_amd_bsr_mm_count(int T, int x_nrow, int y_nrow, int y_ncol, int deep_ranges, const int* x_offsets,
+                          const int* x_row_counts, const int* x_columns, const int* y_offsets,
+                          const int* y_row_counts, const int* y_columns, int* row_min, int* mm_offsets,
+                          int* total_host, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Product triplets (replaces _bsr_mm_list_coeffs, warp/_src/sparse.py:3683-3735):
+ * rows/cols[base + i], i < n, = (row, column) of product block i; (-1, -1)
+ * beyond mm_offsets[T]. */
+int warp_amd_bsr_mm_list(int n, int base, int T, int x_nrow, const int* x_offsets, const int* x_row_counts,
+                         const int* x_columns, const int* y_offsets, const int* y_columns, const int* row_min,
+                         const int* mm_offsets, int* rows, int* cols, void* stream);
+
+/* Product values into an existing topology (replaces _bsr_mm_compute_values,
+ * warp/_src/sparse.py:3764-3800): z(r, c) += alpha * sum over x row r's
+ * active blocks k, in storage order, of x_k @ y(x_columns[k], c).  x blocks
+ * br x bk, y bk x bc (br * bc <= 64); products and sums rounded one by one
+ * (no fused multiply-add). */
+int warp_amd_bsr_mm_values(int scalar_size, int br, int bk, int bc, double alpha, int T, int z_nrow,
+                           const int* z_offsets, const int* z_row_counts, const int* z_columns, void* z_values,
+                           int y_nrow, const int* x_offsets, const int* x_row_counts, const int* x_columns,
+                           const void* x_values, const int* y_offsets, const int* y_row_counts, const int* y_columns,
+                           const void* y_values, void* stream);
+
+/* Block-diagonal topology: offsets[r] = min(r, ndiag), columns[r] = r for
+ * r < ndiag (replaces _bsr_set_diag_kernel of bsr_set_diag,
+ * warp/_src/sparse.py:2939-2995). */
+int warp_amd_bsr_diag_topology(int nrow, int ndiag, int* offsets, int* columns, void* stream);
+
+/* y = alpha x + beta y over n scalars (replaces fem array_axpy,
+ * warp/_src/fem/linalg.py:425-461). */
+int warp_amd_axpby(int scalar_size, int64_t n, double alpha, const void* x, double beta, void* y, void* stream);
+
+/* Dirichlet projector normalization (replaces normalize_dirichlet_projector's
+ * kernels, warp/_src/fem/dirichlet.py:12-66, 127-199): each diagonal block
+ * (block_dim <= 8) becomes sum v v^T over its eigenpairs with eigenvalue
+ * above 1e-6 trace, the row's fixed value (NULL: none) sum (v.b)/lambda v.
+ * FP64 Jacobi eigen decomposition. */
+int warp_amd_normalize_dirichlet_projector(int scalar_size, int block_dim, int nrow, const int* offsets,
+                                           const int* row_counts, const int* columns, void* values,
+                                           void* fixed_values, void* stream);
+
 /* x[0..n) *= alpha, fp32 (4) or fp64 (8) (bsr_scale's value kernel). */
 int warp_amd_scale_values(int scalar_size, int64_t n, void* x, double alpha, void* stream);
 

@@ -243,6 +243,117 @@ def ref_bsr_transpose(nrow, ncol, offsets, columns, row_counts=None, T=None, cap
     return dict(row_counts=t_cnt, columns=t_cols[:slots], block_map=bmap[:slots], status=int(status[0]))
 
 
+def _row_ids(offsets):
+    offsets = np.asarray(offsets, np.int64)
+    return np.repeat(np.arange(offsets.size - 1), np.diff(offsets))
+
+
+def _find_blocks(offsets, columns, ncol, rows, cols):
+    """Position of blocks (rows[i], cols[i]) in a compact row-sorted matrix, -1 if absent."""
+    keys = _row_ids(offsets) * np.int64(ncol) + np.asarray(columns, np.int64)
+    q = np.asarray(rows, np.int64) * np.int64(ncol) + np.asarray(cols, np.int64)
+    pos = np.searchsorted(keys, q)
+    ok = pos < keys.size
+    ok[ok] = keys[pos[ok]] == q[ok]
+    return np.where(ok, pos, -1)
+
+
+def bsr_mm(x, y, z=None, alpha=1.0, beta=0.0, deep_ranges=True, z_topology=None):
+    """z := alpha x @ y + beta z on compact matrices given as dicts (nrow, ncol,
+    offsets, columns, values[nnz, br, bc]) with sorted unique rows.
+
+    Topology (reference bsr_mm, sparse.py:4200-4640 with bsr_mm_count_coeffs /
+    _bsr_mm_list_coeffs, 3620-3735): every product block (row r of x, column c
+    of the y rows it reaches), plus z's blocks when beta != 0; with deep_ranges
+    a row whose product count exceeds the span of the columns it reaches lists
+    that whole span instead.  z_topology = (offsets, columns) fixes the
+    topology instead (masked product).
+
+    Values (_bsr_mm_compute_values, 3764-3800, and the add of the old z): each
+    block starts at 0 (+ beta z), then acc = sum over the row's x blocks in
+    storage order of x_k @ y, each entry x[i,0] y[0,j] + x[i,1] y[1,j] + ...
+    left to right, every product and sum rounded in the values' dtype; finally
+    z += alpha acc."""
+    dt = x["values"].dtype
+    xo, xc, xv = np.asarray(x["offsets"], np.int64), np.asarray(x["columns"], np.int64), x["values"]
+    yo, yc, yv = np.asarray(y["offsets"], np.int64), np.asarray(y["columns"], np.int64), y["values"]
+    nrow, ncol = x["nrow"], y["ncol"]
+    br, bk, bc = xv.shape[1], xv.shape[2], yv.shape[2]
+    if z_topology is not None:
+        out_off = np.asarray(z_topology[0], np.int64)
+        out_cols = np.asarray(z_topology[1], np.int64)
+    else:
+        cand_rows, cand_cols = [], []
+        if beta != 0.0 and z is not None:
+            cand_rows.append(_row_ids(z["offsets"]))
+            cand_cols.append(np.asarray(z["columns"], np.int64))
+        for r in range(nrow):
+            segs = [yc[yo[c]:yo[c + 1]] for c in xc[xo[r]:xo[r + 1]]]
+            segs = [sg for sg in segs if sg.size]
+            if not segs:
+                continue
+            count = sum(sg.size for sg in segs)
+            lo, hi = min(sg[0] for sg in segs), max(sg[-1] for sg in segs)
+            cols = np.arange(lo, hi + 1) if deep_ranges and count > hi - lo + 1 else np.concatenate(segs)
+            cand_rows.append(np.full(cols.size, r))
+            cand_cols.append(cols)
+        rr = np.concatenate(cand_rows) if cand_rows else np.zeros(0, np.int64)
+        cc = np.concatenate(cand_cols) if cand_cols else np.zeros(0, np.int64)
+        keys = np.unique(rr * np.int64(ncol) + cc)
+        out_cols = keys % ncol
+        out_off = np.concatenate([[0], np.cumsum(np.bincount(keys // ncol, minlength=nrow))])
+    out_rows = _row_ids(out_off)
+    nb = out_cols.size
+    vals = np.zeros((nb, br, bc), dt)
+    if beta != 0.0 and z is not None:
+        pos = _find_blocks(z["offsets"], z["columns"], ncol, out_rows, out_cols)
+        has = pos >= 0
+        vals[has] = vals[has] + np.asarray(beta, dt) * z["values"][pos[has]]
+    acc = np.zeros((nb, br, bc), dt)
+    xlen = np.diff(xo)
+    step = 0
+    while nb and step < (xlen[out_rows].max() if nb else 0):
+        live = np.flatnonzero(xlen[out_rows] > step)
+        k = xo[out_rows[live]] + step
+        yb = _find_blocks(yo, yc, ncol, xc[k], out_cols[live])
+        ok = yb >= 0
+        live, k, yb = live[ok], k[ok], yb[ok]
+        a, b = xv[k], yv[yb]
+        prod = np.empty((live.size, br, bc), dt)
+        for i in range(br):
+            for j in range(bc):
+                t = a[:, i, 0] * b[:, 0, j]
+                for m in range(1, bk):
+                    t = t + a[:, i, m] * b[:, m, j]
+                prod[:, i, j] = t
+        acc[live] = acc[live] + prod
+        step += 1
+    vals = vals + np.asarray(alpha, dt) * acc
+    return dict(offsets=out_off.astype(np.int32), columns=out_cols.astype(np.int32), values=vals)
+
+
+def bsr_axpy(x, y, alpha=1.0, beta=1.0):
+    """y := alpha x + beta y on compact matrices (dicts as for bsr_mm):
+    union topology; blocks 0 + beta y, then + alpha x, each op rounded in the
+    values' dtype (reference bsr_axpy, sparse.py:3480-3617, with
+    _bsr_axpy_add_block)."""
+    dt = y["values"].dtype
+    nrow, ncol = y["nrow"], y["ncol"]
+    ry, rx = _row_ids(y["offsets"]), _row_ids(x["offsets"])
+    keys = np.unique(np.concatenate([ry * np.int64(ncol) + np.asarray(y["columns"], np.int64),
+                                     rx * np.int64(ncol) + np.asarray(x["columns"], np.int64)]))
+    cols = keys % ncol
+    off = np.concatenate([[0], np.cumsum(np.bincount(keys // ncol, minlength=nrow))])
+    rows = keys // ncol
+    vals = np.zeros((keys.size, *y["values"].shape[1:]), dt)
+    py = _find_blocks(y["offsets"], y["columns"], ncol, rows, cols)
+    px = _find_blocks(x["offsets"], x["columns"], ncol, rows, cols)
+    hy, hx = py >= 0, px >= 0
+    vals[hy] = vals[hy] + np.asarray(beta, dt) * y["values"][py[hy]]
+    vals[hx] = vals[hx] + np.asarray(alpha, dt) * x["values"][px[hx]]
+    return dict(offsets=off.astype(np.int32), columns=cols.astype(np.int32), values=vals)
+
+
 def bsr_mv(offsets, columns, values, x, y=None, alpha=1.0, beta=0.0, row_counts=None, group=0):
     """y = alpha A x + beta y.  group 0: reference row-kernel order
     (sparse.py:4627-4665); group G > 0: the GPU lane-group order."""

@@ -326,3 +326,70 @@ def test_transpose_oracle_dense_and_known_answers():
     o = oracle.bsr_transpose(2, 2, np.array([0, 2, 4]), np.array([0, 1, 0, 1]), capacity=np.array([0, 1, 3]))
     assert o["status"] == 1
     np.testing.assert_array_equal(o["row_counts"], [0, 2])
+
+
+def _rand_bsr(rng, nrow, ncol, bshape, density, dtype=np.float32):
+    D = rng.random((nrow, ncol)) < density
+    return dict(nrow=nrow, ncol=ncol, offsets=np.concatenate([[0], np.cumsum(D.sum(1))]), columns=np.nonzero(D)[1],
+                values=rng.standard_normal((int(D.sum()), *bshape)).astype(dtype))
+
+
+def _bsr_dense(m):
+    br, bc = m["values"].shape[1:]
+    D = np.zeros((m["nrow"] * br, m["ncol"] * bc))
+    rows = np.repeat(np.arange(m["nrow"]), np.diff(m["offsets"]))
+    for k, (i, j) in enumerate(zip(rows, m["columns"])):
+        D[i * br:(i + 1) * br, j * bc:(j + 1) * bc] += m["values"][k]
+    return D
+
+
+@pytest.mark.parametrize("shapes", [(1, 1, 1), (3, 3, 3), (2, 3, 4)])
+def test_mm_and_axpy_oracle_match_dense(shapes):
+    """oracle.bsr_mm / bsr_axpy reproduce the dense products and sums; the
+    product topology covers exactly the boolean product pattern when no row is deep."""
+    br, bk, bc = shapes
+    rng = np.random.default_rng(sum(shapes))
+    x = _rand_bsr(rng, 30, 20, (br, bk), 0.15)
+    y = _rand_bsr(rng, 20, 40, (bk, bc), 0.1)
+    z = _rand_bsr(rng, 30, 40, (br, bc), 0.05)
+    o = oracle.bsr_mm(x, y, z, alpha=-0.5, beta=2.0)
+    ref = -0.5 * _bsr_dense(x) @ _bsr_dense(y) + 2.0 * _bsr_dense(z)
+    np.testing.assert_allclose(_bsr_dense(dict(nrow=30, ncol=40, **o)), ref, rtol=1e-5, atol=1e-5)
+    o0 = oracle.bsr_mm(x, y, deep_ranges=False)
+    Bx = np.zeros((30, 20), bool)
+    Bx[np.repeat(np.arange(30), np.diff(x["offsets"])), x["columns"]] = True
+    By = np.zeros((20, 40), bool)
+    By[np.repeat(np.arange(20), np.diff(y["offsets"])), y["columns"]] = True
+    pattern = (Bx.astype(int) @ By.astype(int)) > 0
+    np.testing.assert_array_equal(np.diff(o0["offsets"]), pattern.sum(1))
+    np.testing.assert_array_equal(o0["columns"], np.nonzero(pattern)[1])
+    w = _rand_bsr(rng, 30, 40, (br, bc), 0.1)
+    a = oracle.bsr_axpy(w, z, alpha=1.5, beta=-2.0)
+    np.testing.assert_allclose(_bsr_dense(dict(nrow=30, ncol=40, **a)), 1.5 * _bsr_dense(w) - 2.0 * _bsr_dense(z), rtol=1e-6,
+                               atol=1e-6)
+
+
+def test_mm_oracle_known_answers():
+    """Reference rules and known answers: a row with more products than the span of
+    its columns lists the whole span (bsr_mm_count_coeffs, sparse.py:3620-3680);
+    test_sparse.py:965-990 (2x2 dense product has 4 blocks); test_sparse.py:298-310
+    (gapped CSR times B)."""
+    one = lambda n: np.ones((n, 1, 1), np.float32)  # noqa: E731
+    x = dict(nrow=1, ncol=2, offsets=np.array([0, 2]), columns=np.array([0, 1]), values=one(2))
+    y = dict(nrow=2, ncol=3, offsets=np.array([0, 2, 4]), columns=np.array([0, 2, 0, 2]), values=one(4))
+    deep = oracle.bsr_mm(x, y)
+    np.testing.assert_array_equal(deep["columns"], [0, 1, 2])           # 4 products > span 3
+    np.testing.assert_array_equal(deep["values"].ravel(), [2, 0, 2])
+    np.testing.assert_array_equal(oracle.bsr_mm(x, y, deep_ranges=False)["columns"], [0, 2])
+    y3 = dict(nrow=2, ncol=5, offsets=np.array([0, 2, 4]), columns=np.array([0, 4, 0, 4]), values=one(4))
+    np.testing.assert_array_equal(oracle.bsr_mm(x, y3)["columns"], [0, 4])  # 4 products, span 5: not deep
+    A = dict(nrow=2, ncol=2, offsets=np.array([0, 2, 4]), columns=np.array([0, 1, 0, 1]),
+             values=np.array([1, 2, 3, 4], np.float32).reshape(4, 1, 1))
+    assert oracle.bsr_mm(A, A)["columns"].size == 4
+    G = dict(nrow=2, ncol=3, offsets=np.array([0, 2, 4]), columns=np.array([0, 2, 1, 2]),
+             values=np.array([1, 2, 3, 4], np.float32).reshape(4, 1, 1))
+    B = dict(nrow=3, ncol=2, offsets=np.array([0, 1, 2, 3]), columns=np.array([0, 1, 0]),
+             values=np.array([5, 7, 11], np.float32).reshape(3, 1, 1))
+    o = oracle.bsr_mm(G, B, deep_ranges=False)
+    np.testing.assert_array_equal(_bsr_dense(dict(nrow=2, ncol=2, **o)), [[27, 0], [44, 21]])
+    np.testing.assert_array_equal(o["columns"], [0, 0, 1])

@@ -831,3 +831,238 @@ def test_bsr_assign_reblock_vs_dense(device, src_shape, dest_shape, dest_dtype):
     for k in range(min(B.nrow, B.ncol)):
         keep[k * br:(k + 1) * br, k * bc:(k + 1) * bc] = True
     np.testing.assert_array_equal(Dd, np.where(keep, D, 0.0))
+
+
+# ---------------------------------------------------------------------------
+# bsr_axpy / bsr_mm / block-diagonal matrices
+# ---------------------------------------------------------------------------
+
+def _from_triplets_np(device, nrow, ncol, rows, cols, vals):
+    btype = wa.types.type_to_scalar(_t(vals[:1], "cpu").dtype)
+    if vals.ndim == 3 and vals.shape[1:] != (1, 1):
+        btype = wa.matrix(vals.shape[1:], btype)
+    A = sparse.bsr_zeros(nrow, ncol, btype, device=device)
+    v = vals.reshape(-1) if vals.ndim == 3 and vals.shape[1:] == (1, 1) else vals
+    sparse.bsr_set_from_triplets(A, _i(rows, device), _i(cols, device), _t(v, device))
+    return A
+
+
+def _as_dict(A):
+    n = A.nnz_sync()
+    br, bc = A.block_shape
+    return dict(nrow=A.nrow, ncol=A.ncol, offsets=A.offsets.cpu().numpy(), columns=A.columns[:n].cpu().numpy(),
+                values=A.values[:n].cpu().numpy().reshape(n, br, bc))
+
+
+SHAPES = [((1, 1), np.float32), ((3, 3), np.float64), ((1, 3), np.float32)]
+
+
+@pytest.mark.parametrize("bshape,dtype", SHAPES)
+def test_bsr_axpy_reference_scenario(device, bshape, dtype):
+    """test_sparse.py:678-728: alpha/beta sweep, aliasing, masked, shape check."""
+    rng = np.random.default_rng(123)
+    nrow, ncol, nnz = 2, 3, 6
+    mk = lambda: _from_triplets_np(device, nrow, ncol, rng.integers(0, nrow, nnz), rng.integers(0, ncol, nnz),  # noqa: E731
+                                   rng.random((nnz, *bshape)).astype(dtype))
+    x, y = mk(), mk()
+    work = sparse.bsr_axpy_work_arrays()
+    for alpha, beta in zip([-1.0, 0.0, 1.0], [2.0, -1.0, 0.0]):
+        ref = alpha * _active_dense(x)[0] + beta * _active_dense(y)[0]
+        sparse.bsr_axpy(x, y, alpha, beta, work_arrays=work)
+        np.testing.assert_allclose(_active_dense(y)[0], ref, rtol=1e-5, atol=1e-6)
+    ref = 3.0 * _active_dense(y)[0]
+    y += y * 2.0
+    np.testing.assert_allclose(_active_dense(y)[0], ref, rtol=1e-6)
+    n = y.nnz_sync()
+    y_mask = sparse.bsr_from_triplets(nrow, ncol, y.uncompress_rows()[:1], y.columns[:1], y.values[:1])
+    sparse.bsr_axpy(y, y_mask, topology="masked")
+    assert y_mask.nnz_sync() == 1
+    np.testing.assert_allclose(y_mask.values[:1].cpu().numpy(), 2.0 * y.values[:1].cpu().numpy(), rtol=1e-6)
+    assert n > 0
+    y.ncol = y.ncol + 1
+    with pytest.raises(ValueError, match="Matrices must have the same number of rows and columns"):
+        sparse.bsr_axpy(x, y)
+
+
+@pytest.mark.parametrize("bshape,dtype", SHAPES)
+def test_bsr_mm_reference_scenario(device, bshape, dtype):
+    """test_sparse.py:733-828: alpha/beta sweep, reused topology, masked product into a
+    diagonal, operators, aliasing, shape checks; test_sparse.py:831-885 (A A^T)."""
+    rng = np.random.default_rng(123)
+    nnz = 6
+    xs, ys = bshape, bshape[::-1]
+    zs = (xs[0], ys[1])
+    x = _from_triplets_np(device, 3, 2, rng.integers(0, 3, nnz), rng.integers(0, 2, nnz), rng.random((nnz, *xs)).astype(dtype))
+    y = _from_triplets_np(device, 2, 3, rng.integers(0, 2, nnz), rng.integers(0, 3, nnz), rng.random((nnz, *ys)).astype(dtype))
+    z = _from_triplets_np(device, 3, 3, rng.integers(0, 3, nnz), rng.integers(0, 3, nnz), rng.random((nnz, *zs)).astype(dtype))
+    tol = dict(rtol=1e-5, atol=1e-5)
+    work = sparse.bsr_mm_work_arrays()
+    for alpha, beta in zip([-1.0, 0.0, 2.0], [2.0, -1.0, 0.0]):
+        ref = alpha * (_active_dense(x)[0] @ _active_dense(y)[0]) + beta * _active_dense(z)[0]
+        sparse.bsr_mm(x, y, z, alpha, beta, work_arrays=work)
+        np.testing.assert_allclose(_active_dense(z)[0], ref, **tol)
+    sparse.bsr_mm(x, y, z, alpha, beta, work_arrays=work, reuse_topology=True)
+    np.testing.assert_allclose(_active_dense(z)[0], ref, **tol)
+
+    zd = sparse.bsr_diag(rows_of_blocks=z.nrow, block_type=z.dtype, device=device)
+    sparse.bsr_mm(x, y, zd, topology="masked")
+    full = _active_dense(x)[0] @ _active_dense(y)[0]
+    keep = np.zeros_like(full, bool)
+    for k in range(3):
+        keep[k * zs[0]:(k + 1) * zs[0], k * zs[1]:(k + 1) * zs[1]] = True
+    np.testing.assert_allclose(_active_dense(zd)[0], np.where(keep, full, 0.0), **tol)
+    prod = (alpha * x) @ y
+    np.testing.assert_allclose(_active_dense(prod)[0], alpha * full, **tol)
+
+    if xs[0] == xs[1]:
+        a, b = -1.0, 2.0
+        ref = a * (_active_dense(zd)[0] @ _active_dense(x)[0]) + b * _active_dense(x)[0]
+        sparse.bsr_mm(zd, x, x, a, b)
+        np.testing.assert_allclose(_active_dense(x)[0], ref, **tol)
+        ref = a * (_active_dense(zd)[0] @ _active_dense(zd)[0]) + b * _active_dense(zd)[0]
+        sparse.bsr_mm(zd, zd, zd, a, b)
+        np.testing.assert_allclose(_active_dense(zd)[0], ref, **tol)
+    else:
+        with pytest.raises(ValueError, match="Incompatible block sizes"):
+            sparse.bsr_mm(z, y)
+    y.ncol = y.ncol * 2
+    with pytest.raises(ValueError, match="Incompatible number of rows/columns"):
+        sparse.bsr_mm(y, z)
+    y.ncol = y.ncol // 2
+    AAt = sparse.bsr_mm(y, sparse.bsr_transposed(y))
+    Dy = _active_dense(y)[0]
+    np.testing.assert_allclose(_active_dense(AAt)[0], Dy @ Dy.T, **tol)
+
+
+def test_bsr_mm_deep_and_max_new_nnz(device):
+    """test_sparse.py:907-1003: products of a wide dense block matrix with its
+    transpose (deep rows), and the max_new_nnz cap."""
+    rng = np.random.default_rng(123)
+    nrow, ncol = 2, 200
+    rows, cols = np.meshgrid(np.arange(nrow), np.arange(ncol))
+    vals = rng.random((nrow * ncol, 3, 3)).astype(np.float32)
+    A = _from_triplets_np(device, nrow, ncol, rows.ravel(), cols.ravel(), vals)
+    At = sparse.bsr_transposed(A)
+    D = _active_dense(A)[0]
+    np.testing.assert_allclose(_active_dense(sparse.bsr_mm(A, At))[0], D @ D.T, rtol=1e-4)
+    np.testing.assert_allclose(_active_dense(sparse.bsr_mm(At, A))[0], D.T @ D, rtol=1e-4, atol=1e-5)
+    B = _from_triplets_np(device, 2, 2, [0, 0, 1, 1], [0, 1, 0, 1], np.array([1, 2, 3, 4], np.float32).reshape(4, 1, 1))
+    C = sparse.bsr_zeros(2, 2, wa.float32, device=device)
+    sparse.bsr_mm(B, B, C, max_new_nnz=4)
+    assert C.nnz_sync() == 4
+    sparse.bsr_set_zero(C)
+    assert C.nnz_sync() == 0
+    with pytest.warns(RuntimeWarning, match="exceeded"):
+        sparse.bsr_mm(B, B, C, max_new_nnz=2)
+    assert C.nnz_sync() == 2
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_bsr_mm_and_axpy_vs_oracle(device, oracle_mod, dtype):
+    """Compact product and sum bit-exact against the oracle (topology, including the
+    deep-row span rule, and values in the stated summation order)."""
+    rng = np.random.default_rng(41)
+
+    def rand(nrow, ncol, nnz, bshape):
+        return _from_triplets_np(device, nrow, ncol, rng.integers(0, nrow, nnz), rng.integers(0, ncol, nnz),
+                                 rng.standard_normal((nnz, *bshape)).astype(dtype))
+    x, y, z = rand(300, 200, 3000, (3, 3)), rand(200, 250, 2500, (3, 3)), rand(300, 250, 1500, (3, 3))
+    ox, oy, oz = _as_dict(x), _as_dict(y), _as_dict(z)
+    sparse.bsr_mm(x, y, z, alpha=-0.75, beta=1.5)
+    ref = oracle_mod.bsr_mm(ox, oy, oz, alpha=-0.75, beta=1.5)
+    got = _as_dict(z)
+    np.testing.assert_array_equal(got["offsets"], ref["offsets"])
+    np.testing.assert_array_equal(got["columns"], ref["columns"])
+    assert got["values"].tobytes() == ref["values"].tobytes()
+    # a gapped band matrix squared has deep rows (more products than the span of
+    # columns reached): the span is listed whole, odd offsets as structural zeros
+    n = 64
+    r = np.repeat(np.arange(n), 4)
+    c = np.clip(r + np.tile(np.arange(0, 8, 2), n), 0, n - 1)
+    band = _from_triplets_np(device, n, n, r, c, rng.standard_normal((r.size, 1, 1)).astype(dtype))
+    ob = _as_dict(band)
+    sq = sparse.bsr_mm(band, band)
+    ref = oracle_mod.bsr_mm(ob, ob)
+    got = _as_dict(sq)
+    assert (np.diff(got["offsets"]) > 4).any()      # spans wider than the products' columns
+    np.testing.assert_array_equal(got["offsets"], ref["offsets"])
+    np.testing.assert_array_equal(got["columns"], ref["columns"])
+    assert got["values"].tobytes() == ref["values"].tobytes()
+    w = rand(300, 250, 2000, (3, 3))
+    ow, oz = _as_dict(w), _as_dict(z)
+    sparse.bsr_axpy(w, z, alpha=0.5, beta=-2.0)
+    ref = oracle_mod.bsr_axpy(ow, oz, alpha=0.5, beta=-2.0)
+    got = _as_dict(z)
+    np.testing.assert_array_equal(got["columns"], ref["columns"])
+    assert got["values"].tobytes() == ref["values"].tobytes()
+
+
+def test_bsr_mm_axpy_padded_and_masked_vs_oracle(device, oracle_mod):
+    """Padded product / sum into row capacity (rows that fit equal the oracle's exact
+    union, bit-exact; the others raise the status) and the masked product."""
+    rng = np.random.default_rng(43)
+
+    def rand(nrow, ncol, nnz):
+        return _from_triplets_np(device, nrow, ncol, rng.integers(0, nrow, nnz), rng.integers(0, ncol, nnz),
+                                 rng.standard_normal((nnz, 3, 3)).astype(np.float32))
+    x, y, z0 = rand(200, 150, 1500), rand(150, 180, 1200), rand(200, 180, 600)
+    ox, oy, oz = _as_dict(x), _as_dict(y), _as_dict(z0)
+    ref = oracle_mod.bsr_mm(ox, oy, oz, alpha=2.0, beta=-1.0, deep_ranges=False)
+    cnt = np.diff(ref["offsets"])
+    cap = np.maximum(cnt + rng.integers(-2, 4, 200), np.diff(oz["offsets"])).astype(np.int32)
+    z = sparse.bsr_zeros(200, 180, wa.mat33, device=device, row_capacity=_t(cap, device))
+    sparse.bsr_assign(z, z0, topology="padded")     # the capacity holds z0's rows
+    assert z.status_sync() == 0
+    sparse.bsr_mm(x, y, z, alpha=2.0, beta=-1.0, topology="padded")
+    fits = cnt <= cap
+    assert z.status_sync() == (0 if fits.all() else 1)
+    np.testing.assert_array_equal(z.row_counts.cpu().numpy(), np.where(fits, cnt, 0))
+    off = np.concatenate([[0], np.cumsum(cap)])
+    gc, gv = z.columns.cpu().numpy(), z.values.cpu().numpy()
+    for r in np.flatnonzero(fits):
+        k0, k1 = ref["offsets"][r], ref["offsets"][r + 1]
+        np.testing.assert_array_equal(gc[off[r]:off[r] + cnt[r]], ref["columns"][k0:k1])
+        assert gv[off[r]:off[r] + cnt[r]].tobytes() == ref["values"][k0:k1].tobytes()
+
+    # padded axpy: y := 0.5 w + 2 y into y's capacity
+    w = rand(200, 180, 800)
+    ow = _as_dict(w)
+    refa = oracle_mod.bsr_axpy(ow, oz, alpha=0.5, beta=2.0)
+    cnta = np.diff(refa["offsets"])
+    capa = (cnta + rng.integers(-1, 3, 200)).clip(np.diff(oz["offsets"])).astype(np.int32)
+    ya = sparse.bsr_zeros(200, 180, wa.mat33, device=device, row_capacity=_t(capa, device))
+    sparse.bsr_assign(ya, z0, topology="padded")
+    sparse.bsr_axpy(w, ya, alpha=0.5, beta=2.0, topology="padded")
+    fitsa = cnta <= capa
+    assert ya.status_sync() == (0 if fitsa.all() else 1)
+    offa = np.concatenate([[0], np.cumsum(capa)])
+    gc, gv = ya.columns.cpu().numpy(), ya.values.cpu().numpy()
+    for r in range(200):
+        if fitsa[r]:
+            k0, k1 = refa["offsets"][r], refa["offsets"][r + 1]
+            np.testing.assert_array_equal(gc[offa[r]:offa[r] + cnta[r]], refa["columns"][k0:k1])
+            assert gv[offa[r]:offa[r] + cnta[r]].tobytes() == refa["values"][k0:k1].tobytes()
+    np.testing.assert_array_equal(ya.row_counts.cpu().numpy(), np.where(fitsa, cnta, 0))
+
+    # masked product keeps z0's topology
+    zm = sparse.bsr_copy(z0)
+    sparse.bsr_mm(x, y, zm, alpha=1.5, beta=0.5, topology="masked")
+    refm = oracle_mod.bsr_mm(ox, oy, oz, alpha=1.5, beta=0.5, z_topology=(oz["offsets"], oz["columns"]))
+    got = _as_dict(zm)
+    np.testing.assert_array_equal(got["columns"], oz["columns"])
+    assert got["values"].tobytes() == refm["values"].tobytes()
+
+
+def test_bsr_diag_and_identity(device):
+    """bsr_diag / bsr_identity / bsr_set_diag (sparse.py:2939-3093)."""
+    d = torch.arange(12, dtype=torch.float32, device=device).reshape(4, 3)
+    blocks = torch.diag_embed(d)
+    D = sparse.bsr_diag(blocks)
+    assert D.nrow == D.ncol == 4 and D.nnz_sync() == 4
+    np.testing.assert_array_equal(D.offsets.cpu().numpy(), [0, 1, 2, 3, 4])
+    np.testing.assert_array_equal(D.columns[:4].cpu().numpy(), [0, 1, 2, 3])
+    np.testing.assert_array_equal(_active_dense(D)[0], np.diag(np.arange(12.0)))
+    I = sparse.bsr_identity(5, wa.mat33, device=device)
+    np.testing.assert_array_equal(_active_dense(I)[0], np.eye(15))
+    R = sparse.bsr_diag(2.0, rows_of_blocks=3, cols_of_blocks=5, device=device)
+    np.testing.assert_array_equal(_active_dense(R)[0], 2.0 * np.eye(3, 5))

@@ -88,6 +88,24 @@ _SIGNATURES = {
         c_int, [c_int, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_f64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]
     ),
     "warp_amd_scale_values": (c_int, [c_int, c_i64, c_ptr, c_f64, c_ptr]),
+    "warp_amd_bsr_uncompress_rows": (c_int, [c_int, c_ptr, c_ptr, c_int, c_ptr, c_ptr]),
+    "warp_amd_bsr_add_blocks": (
+        c_int, [c_int, c_int, c_i64, c_ptr, c_ptr, c_ptr, c_f64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]
+    ),
+    "warp_amd_bsr_axpy_padded": (
+        c_int, [c_int, c_int, c_int, c_f64, c_f64] + [c_ptr] * 10
+    ),
+    "warp_amd_bsr_mm_count_workspace_size": (c_size, [c_int]),
+    "warp_amd_bsr_mm_count": (c_int, [c_int] * 5 + [c_ptr] * 9 + [c_ptr, c_size, c_ptr]),
+    "warp_amd_bsr_mm_list": (c_int, [c_int] * 4 + [c_ptr] * 9 + [c_ptr]),
+    "warp_amd_bsr_mm_values": (
+        c_int,
+        [c_int, c_int, c_int, c_int, c_f64, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_int]
+        + [c_ptr] * 8 + [c_ptr],
+    ),
+    "warp_amd_bsr_diag_topology": (c_int, [c_int, c_int, c_ptr, c_ptr, c_ptr]),
+    "warp_amd_axpby": (c_int, [c_int, c_i64, c_f64, c_ptr, c_f64, c_ptr, c_ptr]),
+    "warp_amd_normalize_dirichlet_projector": (c_int, [c_int, c_int, c_int] + [c_ptr] * 6),
     "warp_amd_bsr_residual_workspace_size": (c_size, [c_int]),
     "warp_amd_bsr_residual_f32_f64": (c_int, [c_int, c_int, c_ptr, c_ptr, c_ptr, c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
     "warp_amd_axpy_f32_to_f64": (c_int, [c_i64, c_f64, c_ptr, c_ptr, c_ptr]),

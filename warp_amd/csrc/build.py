@@ -29,7 +29,8 @@ COMPILE_FLAGS = [
     "-std=c++17",
     "-fPIC",
     "-fvisibility=hidden",
-    "-ffp-contract=fast",
+    # FMA contraction everywhere, except inside blocks marked "#pragma clang fp contract(off)"
+    "-ffp-contract=fast-honor-pragmas",
     # keep the compiler from emitting packed-FP32 VALU instructions
     "-Xclang",
     "-target-feature",

@@ -104,6 +104,8 @@ int bsr_mv_lanes(int nrow, int br, int bc, int64_t nnz_hint);
 // Exclusive scan of n int32 values: out[i] = sum(in[0..i)), out[n] = total.
 // `in` and `out` may alias.  Workspace from scan_workspace_bytes(n).
 size_t scan_workspace_bytes(int64_t n);
+// p[0..n) = v on `stream` (a kernel, never a memset node)
+int launch_fill_i32(int* p, int64_t n, int v, hipStream_t stream);
 int exclusive_scan_i32(const int* in, int* out, int64_t n, void* ws, hipStream_t stream);
 
 // ---------------------------------------------------------------------------

@@ -145,7 +145,23 @@ __global__ __launch_bounds__(kScanBlock) void scan_tile_kernel(
 
 __global__ void scan_empty_kernel(int* out) { *out = 0; }
 
+__global__ __launch_bounds__(256) void fill_i32_kernel(int* p, int64_t n, int v)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
 }  // namespace
+
+int launch_fill_i32(int* p, int64_t n, int v, hipStream_t stream)
+{
+    if (n <= 0)
+        return kOk;
+    fill_i32_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, stream>>>(p, n, v);
+    WA_LAUNCH_CHECK();
+    return kOk;
+}
 
 size_t scan_workspace_bytes(int64_t n)
 {

@@ -163,6 +163,436 @@ __global__ __launch_bounds__(kWave) void assign_padded_kernel(
             dst[(size_t)dbeg * block_size + e] = (DS)(scale * src[(size_t)beg * block_size + e]);
 }
 
+// Active end of row r, clipped to its capacity.
+__device__ __forceinline__ int row_end(const int* offsets, const int* row_counts, int r)
+{
+    const int cap_end = offsets[r + 1];
+    if (!row_counts)
+        return cap_end;
+    const int e = offsets[r] + row_counts[r];
+    return e < cap_end ? e : cap_end;
+}
+
+// rows[p] = row of storage position p, -1 for inactive positions.
+__global__ __launch_bounds__(kBlock) void uncompress_rows_kernel(int nrow, const int* __restrict__ offsets,
+                                                                const int* __restrict__ row_counts, int T,
+                                                                int* __restrict__ rows)
+{
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p < T)
+        rows[p] = row_of_position(offsets, row_counts, nrow, p);
+}
+
+// dst[block (rows[i], cols[i])] += scale * src[i], one thread per scalar;
+// blocks absent from the destination are skipped (reference
+// _bsr_axpy_add_block).  Every destination block must receive at most one
+// source block per launch.
+template <typename S>
+__global__ __launch_bounds__(kBlock) void add_blocks_kernel(int64_t n, int bs, const int* __restrict__ rows,
+                                                           const int* __restrict__ cols, const S* __restrict__ src,
+                                                           S scale, const int* __restrict__ d_offsets,
+                                                           const int* __restrict__ d_row_counts,
+                                                           const int* __restrict__ d_columns, S* __restrict__ dst)
+{
+#pragma clang fp contract(off)
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n * bs)
+        return;
+    const int64_t i = t / bs;
+    const int e = (int)(t - i * bs);
+    const int r = rows[i];
+    if (r < 0)
+        return;
+    const int b = find_block(d_offsets, d_row_counts, d_columns, r, cols[i]);
+    if (b < 0)
+        return;
+    dst[(size_t)b * bs + e] += scale * src[(size_t)i * bs + e];
+}
+
+// y := alpha x + beta y written into y's own row capacity, one wave per row
+// (reference _bsr_axpy_padded_count / _fill / _finalize): the union of the
+// two sorted rows is merged from the back, in place; a row whose union does
+// not fit is left unchanged with row_counts 0 and raises *status.
+template <typename S>
+__global__ __launch_bounds__(kWave) void axpy_padded_kernel(
+    int nrow, int bs, S alpha, S beta, const int* __restrict__ x_offsets, const int* __restrict__ x_row_counts,
+    const int* __restrict__ x_columns, const S* __restrict__ x_values, const int* __restrict__ y_offsets,
+    int* __restrict__ y_row_counts, int* __restrict__ y_columns, S* __restrict__ y_values, int* __restrict__ status)
+{
+#pragma clang fp contract(off)
+    const int r = blockIdx.x;
+    if (r >= nrow)
+        return;
+    const int lane = threadIdx.x;
+    const int xb = x_offsets[r];
+    const int xe = row_end(x_offsets, x_row_counts, r);
+    const int yb = y_offsets[r];
+    const int ye = row_end(y_offsets, y_row_counts, r);
+    int i = xb, j = yb, count = 0;
+    while (i < xe && j < ye) {
+        const int xc = x_columns[i];
+        const int yc = y_columns[j];
+        ++count;
+        i += xc <= yc ? 1 : 0;
+        j += yc <= xc ? 1 : 0;
+    }
+    count += (xe - i) + (ye - j);
+    if (yb + count > y_offsets[r + 1]) {
+        if (lane == 0) {
+            y_row_counts[r] = 0;
+            atomicMax(status, kStatusRowCapacityExceeded);
+        }
+        return;
+    }
+    i = xe - 1;
+    j = ye - 1;
+    for (int w = yb + count - 1; w >= yb; --w) {
+        bool use_x = false, use_y = false;
+        int col;
+        if (i >= xb && j >= yb) {
+            const int xc = x_columns[i];
+            const int yc = y_columns[j];
+            use_x = xc >= yc;
+            use_y = yc >= xc;
+            col = use_x ? xc : yc;
+        } else if (i >= xb) {
+            use_x = true;
+            col = x_columns[i];
+        } else {
+            use_y = true;
+            col = y_columns[j];
+        }
+        for (int e = lane; e < bs; e += kWave) {
+            S v = S(0);
+            if (use_x)
+                v += alpha * x_values[(size_t)i * bs + e];
+            if (use_y)
+                v += beta * y_values[(size_t)j * bs + e];
+            y_values[(size_t)w * bs + e] = v;
+        }
+        if (lane == 0)
+            y_columns[w] = col;
+        i -= use_x ? 1 : 0;
+        j -= use_y ? 1 : 0;
+    }
+    if (lane == 0)
+        y_row_counts[r] = count;
+}
+
+// Sparse product x @ y, symbolic count (reference bsr_mm_count_coeffs): for
+// each active x block k of row r, counts[k] = number of active blocks in y
+// row x_columns[k].  With deep_ranges, a row whose product count exceeds the
+// span [min_col, max_col] of the columns it reaches is listed as that whole
+// span instead: row_min[r] = min_col and the span size goes to the row's last
+// x block (other blocks 0).  row_min[r] = -1 otherwise.  counts is zeroed
+// beforehand.
+__global__ __launch_bounds__(kBlock) void mm_count_kernel(int x_nrow, int y_nrow, int y_ncol, int deep_ranges,
+                                                         const int* __restrict__ x_offsets,
+                                                         const int* __restrict__ x_row_counts,
+                                                         const int* __restrict__ x_columns,
+                                                         const int* __restrict__ y_offsets,
+                                                         const int* __restrict__ y_row_counts,
+                                                         const int* __restrict__ y_columns, int* __restrict__ row_min,
+                                                         int* __restrict__ counts)
+{
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= x_nrow)
+        return;
+    const int xb = x_offsets[r];
+    const int xe = row_end(x_offsets, x_row_counts, r);
+    int64_t total = 0;
+    int min_col = y_ncol, max_col = 0;
+    for (int k = xb; k < xe; ++k) {
+        const int c = x_columns[k];
+        int n = 0;
+        if (c >= 0 && c < y_nrow) {
+            const int b = y_offsets[c];
+            const int e = row_end(y_offsets, y_row_counts, c);
+            n = e - b;
+            if (n > 0) {
+                min_col = min(min_col, y_columns[b]);
+                max_col = max(max_col, y_columns[e - 1]);
+            }
+        }
+        counts[k] = n;
+        total += n;
+    }
+    const int span = max(0, max_col - min_col + 1);
+    if (deep_ranges && total > span) {
+        row_min[r] = min_col;
+        for (int k = xb; k < xe - 1; ++k)
+            counts[k] = 0;
+        counts[xe - 1] = span;
+    } else {
+        row_min[r] = -1;
+    }
+}
+
+// Product list (reference _bsr_mm_list_coeffs): entry i < n of the product
+// triplets, at base + i, is (row, column) of the product block numbered i in
+// x-block order (mm_offsets = exclusive scan of the counts over the T x
+// positions); entries at or beyond the real total get (-1, -1).
+__global__ __launch_bounds__(kBlock) void mm_list_kernel(int n, int base, int T, int x_nrow,
+                                                        const int* __restrict__ x_offsets,
+                                                        const int* __restrict__ x_row_counts,
+                                                        const int* __restrict__ x_columns,
+                                                        const int* __restrict__ y_offsets,
+                                                        const int* __restrict__ y_columns,
+                                                        const int* __restrict__ row_min,
+                                                        const int* __restrict__ mm_offsets, int* __restrict__ rows,
+                                                        int* __restrict__ cols)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    if (i >= mm_offsets[T]) {
+        rows[base + i] = -1;
+        cols[base + i] = -1;
+        return;
+    }
+    // last x position k with mm_offsets[k] <= i
+    int lo = 0, hi = T - 1;
+    while (lo < hi) {
+        const int mid = lo + (hi - lo + 1) / 2;
+        if (mm_offsets[mid] <= i)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int k = lo;
+    const int pos = i - mm_offsets[k];
+    const int row = row_of_position(x_offsets, x_row_counts, x_nrow, k);
+    const int rmin = row_min[row];
+    rows[base + i] = row;
+    cols[base + i] = rmin < 0 ? y_columns[y_offsets[x_columns[k]] + pos] : rmin + pos;
+}
+
+// Product values (reference _bsr_mm_compute_values): for every active z
+// block (r, c), acc = sum over the active x blocks k of row r, in storage
+// order, of x_k @ y(x_columns[k], c) for those y blocks that exist; then
+// z += alpha * acc.  One thread per z block.  Each block product entry is
+// x[i][0] y[0][j] + x[i][1] y[1][j] + ... evaluated left to right with every
+// product and sum rounded (no fused multiply-add), so that the CPU oracle
+// reproduces the bits.
+template <typename S, int BR, int BK, int BC>
+__global__ __launch_bounds__(kBlock) void mm_values_kernel(
+    int T, int z_nrow, S alpha, const int* __restrict__ z_offsets, const int* __restrict__ z_row_counts,
+    const int* __restrict__ z_columns, S* __restrict__ z_values, int y_nrow, const int* __restrict__ x_offsets,
+    const int* __restrict__ x_row_counts, const int* __restrict__ x_columns, const S* __restrict__ x_values,
+    const int* __restrict__ y_offsets, const int* __restrict__ y_row_counts, const int* __restrict__ y_columns,
+    const S* __restrict__ y_values)
+{
+#pragma clang fp contract(off)
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= T)
+        return;
+    const int r = row_of_position(z_offsets, z_row_counts, z_nrow, p);
+    if (r < 0)
+        return;
+    const int col = z_columns[p];
+    S acc[BR * BC];
+#pragma unroll
+    for (int e = 0; e < BR * BC; ++e)
+        acc[e] = S(0);
+    const int xb = x_offsets[r];
+    const int xe = row_end(x_offsets, x_row_counts, r);
+    for (int k = xb; k < xe; ++k) {
+        const int xc = x_columns[k];
+        if (xc < 0 || xc >= y_nrow)
+            continue;
+        const int b = find_block(y_offsets, y_row_counts, y_columns, xc, col);
+        if (b < 0)
+            continue;
+        const S* xv = x_values + (size_t)k * BR * BK;
+        const S* yv = y_values + (size_t)b * BK * BC;
+#pragma unroll
+        for (int i = 0; i < BR; ++i)
+#pragma unroll
+            for (int j = 0; j < BC; ++j) {
+                S prod = xv[i * BK] * yv[j];
+#pragma unroll
+                for (int m = 1; m < BK; ++m)
+                    prod = prod + xv[i * BK + m] * yv[m * BC + j];
+                acc[i * BC + j] = acc[i * BC + j] + prod;
+            }
+    }
+    S* zv = z_values + (size_t)p * BR * BC;
+#pragma unroll
+    for (int e = 0; e < BR * BC; ++e)
+        zv[e] = zv[e] + alpha * acc[e];
+}
+
+// Same, with block dimensions known at run time (br * bc <= 64).
+template <typename S>
+__global__ __launch_bounds__(kBlock) void mm_values_any_kernel(
+    int T, int z_nrow, S alpha, int BR, int BK, int BC, const int* __restrict__ z_offsets,
+    const int* __restrict__ z_row_counts, const int* __restrict__ z_columns, S* __restrict__ z_values, int y_nrow,
+    const int* __restrict__ x_offsets, const int* __restrict__ x_row_counts, const int* __restrict__ x_columns,
+    const S* __restrict__ x_values, const int* __restrict__ y_offsets, const int* __restrict__ y_row_counts,
+    const int* __restrict__ y_columns, const S* __restrict__ y_values)
+{
+#pragma clang fp contract(off)
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= T)
+        return;
+    const int r = row_of_position(z_offsets, z_row_counts, z_nrow, p);
+    if (r < 0)
+        return;
+    const int col = z_columns[p];
+    S acc[64];
+    for (int e = 0; e < BR * BC; ++e)
+        acc[e] = S(0);
+    const int xb = x_offsets[r];
+    const int xe = row_end(x_offsets, x_row_counts, r);
+    for (int k = xb; k < xe; ++k) {
+        const int xc = x_columns[k];
+        if (xc < 0 || xc >= y_nrow)
+            continue;
+        const int b = find_block(y_offsets, y_row_counts, y_columns, xc, col);
+        if (b < 0)
+            continue;
+        const S* xv = x_values + (size_t)k * BR * BK;
+        const S* yv = y_values + (size_t)b * BK * BC;
+        for (int i = 0; i < BR; ++i)
+            for (int j = 0; j < BC; ++j) {
+                S prod = xv[i * BK] * yv[j];
+                for (int m = 1; m < BK; ++m)
+                    prod = prod + xv[i * BK + m] * yv[m * BC + j];
+                acc[i * BC + j] = acc[i * BC + j] + prod;
+            }
+    }
+    S* zv = z_values + (size_t)p * BR * BC;
+    for (int e = 0; e < BR * BC; ++e)
+        zv[e] = zv[e] + alpha * acc[e];
+}
+
+// Block-diagonal topology: row r < ndiag holds column r.
+__global__ __launch_bounds__(kBlock) void diag_topology_kernel(int nrow, int ndiag, int* __restrict__ offsets,
+                                                              int* __restrict__ columns)
+{
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    if (r > nrow)
+        return;
+    offsets[r] = r < ndiag ? r : ndiag;
+    if (r < ndiag)
+        columns[r] = r;
+}
+
+// y = alpha x + beta y (reference fem array_axpy), contraction off.
+template <typename S>
+__global__ __launch_bounds__(kBlock) void axpby_kernel(int64_t n, S alpha, const S* __restrict__ x, S beta,
+                                                      S* __restrict__ y)
+{
+#pragma clang fp contract(off)
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n)
+        y[i] = alpha * x[i] + beta * y[i];
+}
+
+constexpr int kMaxEigen = 8;
+
+// Eigen decomposition of a symmetric n x n matrix (n <= kMaxEigen) by cyclic
+// Jacobi rotations in FP64: on return a holds the eigenvalues on its diagonal
+// and row k of v the unit eigenvector of eigenvalue a[k][k].
+__device__ void symmetric_eigen(int n, double (&a)[kMaxEigen][kMaxEigen], double (&v)[kMaxEigen][kMaxEigen])
+{
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+            v[i][j] = i == j ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 64; ++sweep) {
+        double off = 0.0, diag = 0.0;
+        for (int i = 0; i < n; ++i) {
+            diag += a[i][i] * a[i][i];
+            for (int j = i + 1; j < n; ++j)
+                off += a[i][j] * a[i][j];
+        }
+        if (off <= 1e-30 * diag || off == 0.0)
+            break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = a[p][q];
+                if (apq == 0.0)
+                    continue;
+                // rotation angle zeroing a[p][q] (Golub & Van Loan, sym.schur2)
+                const double tau = (a[q][q] - a[p][p]) / (2.0 * apq);
+                const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                const double c = 1.0 / sqrt(1.0 + t * t);
+                const double sn = t * c;
+                for (int k = 0; k < n; ++k) {
+                    const double akp = a[k][p], akq = a[k][q];
+                    a[k][p] = c * akp - sn * akq;
+                    a[k][q] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double apk = a[p][k], aqk = a[q][k];
+                    a[p][k] = c * apk - sn * aqk;
+                    a[q][k] = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double vpk = v[p][k], vqk = v[q][k];
+                    v[p][k] = c * vpk - sn * vqk;
+                    v[q][k] = sn * vpk + c * vqk;
+                }
+            }
+    }
+}
+
+// Dirichlet projector normalization, one thread per block row (reference
+// fem _normalize_dirichlet_projector(_and_values)_kernel and
+// _normalize_projector_and_value, warp/_src/fem/dirichlet.py:127-199): the
+// diagonal block P becomes the orthogonal projector sum v v^T over its
+// eigenpairs with eigenvalue > eps = 1e-6 trace(P), and the fixed value b
+// (when given) becomes sum (v . b) / lambda v over the same pairs.
+template <typename S>
+__global__ __launch_bounds__(kBlock) void normalize_projector_kernel(int nrow, int n, const int* __restrict__ offsets,
+                                                                    const int* __restrict__ row_counts,
+                                                                    const int* __restrict__ columns,
+                                                                    S* __restrict__ values, S* __restrict__ fixed)
+{
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= nrow)
+        return;
+    const int b = find_block(offsets, row_counts, columns, r, r);
+    if (b < 0)
+        return;
+    S* P = values + (size_t)b * n * n;
+    double a[kMaxEigen][kMaxEigen], v[kMaxEigen][kMaxEigen];
+    double trace = 0.0;
+    for (int i = 0; i < n; ++i) {
+        trace += (double)P[i * n + i];
+        for (int j = 0; j < n; ++j)
+            a[i][j] = 0.5 * ((double)P[i * n + j] + (double)P[j * n + i]);
+    }
+    const double eps = trace * 1.0e-6;
+    symmetric_eigen(n, a, v);
+    double out[kMaxEigen][kMaxEigen], bn[kMaxEigen];
+    for (int i = 0; i < n; ++i) {
+        bn[i] = 0.0;
+        for (int j = 0; j < n; ++j)
+            out[i][j] = 0.0;
+    }
+    for (int k = 0; k < n; ++k) {
+        const double lam = a[k][k];
+        if (!(lam > eps))
+            continue;
+        double vb = 0.0;
+        if (fixed)
+            for (int i = 0; i < n; ++i)
+                vb += v[k][i] * (double)fixed[(size_t)r * n + i];
+        for (int i = 0; i < n; ++i) {
+            bn[i] += vb / lam * v[k][i];
+            for (int j = 0; j < n; ++j)
+                out[i][j] += v[k][i] * v[k][j];
+        }
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+            P[i * n + j] = (S)out[i][j];
+    if (fixed)
+        for (int i = 0; i < n; ++i)
+            fixed[(size_t)r * n + i] = (S)bn[i];
+}
+
 template <typename S> __global__ __launch_bounds__(kBlock) void scale_kernel(int64_t n, S* __restrict__ x, S alpha)
 {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -300,6 +730,254 @@ WA_API int warp_amd_scale_values(int scalar_size, int64_t n, void* x, double alp
         wa::scale_kernel<float><<<grid, wa::kBlock, 0, s>>>(n, static_cast<float*>(x), (float)alpha);
     else
         wa::scale_kernel<double><<<grid, wa::kBlock, 0, s>>>(n, static_cast<double*>(x), alpha);
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}
+
+// ---------------------------------------------------------------------------
+// Sums and products of BSR matrices (bsr_axpy, bsr_mm)
+// ---------------------------------------------------------------------------
+
+// rows[p] = block row of storage position p < T, -1 where p holds no active
+// block (replaces BsrMatrix.uncompress_rows' kernel _bsr_get_block_row,
+// warp/_src/sparse.py:164-193).
+WA_API int warp_amd_bsr_uncompress_rows(int nrow, const int* offsets, const int* row_counts, int T, int* rows,
+                                        void* stream)
+{
+    WA_REQUIRE(nrow >= 0 && T >= 0, "uncompress_rows: invalid sizes");
+    if (T == 0)
+        return wa::kOk;
+    WA_REQUIRE(offsets && rows, "uncompress_rows: null arrays");
+    wa::uncompress_rows_kernel<<<(unsigned)wa::ceil_div(T, wa::kBlock), wa::kBlock, 0, wa::as_stream(stream)>>>(
+        nrow, offsets, row_counts, T, rows);
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}
+
+// dst[block (rows[i], cols[i])] += scale * src[i] for i < n (rows[i] < 0 or
+// blocks missing from the destination topology are skipped); replaces
+// _bsr_axpy_add_block (warp/_src/sparse.py:3145-3163).  Each destination
+// block may receive at most one source block per call.
+WA_API int warp_amd_bsr_add_blocks(int scalar_size, int block_size, int64_t n, const int* rows, const int* cols,
+                                   const void* src, double scale, const int* dest_offsets,
+                                   const int* dest_row_counts, const int* dest_columns, void* dest_values,
+                                   void* stream)
+{
+    WA_REQUIRE(n >= 0 && block_size > 0, "bsr_add_blocks: invalid sizes");
+    WA_REQUIRE(scalar_size == 4 || scalar_size == 8, "bsr_add_blocks: values must be fp32 or fp64");
+    if (n == 0)
+        return wa::kOk;
+    WA_REQUIRE(rows && cols && src && dest_offsets && dest_values, "bsr_add_blocks: null arrays");
+    const unsigned grid = (unsigned)wa::ceil_div(n * block_size, wa::kBlock);
+    hipStream_t s = wa::as_stream(stream);
+    if (scalar_size == 4)
+        wa::add_blocks_kernel<float><<<grid, wa::kBlock, 0, s>>>(
+            n, block_size, rows, cols, static_cast<const float*>(src), (float)scale, dest_offsets, dest_row_counts,
+            dest_columns, static_cast<float*>(dest_values));
+    else
+        wa::add_blocks_kernel<double><<<grid, wa::kBlock, 0, s>>>(
+            n, block_size, rows, cols, static_cast<const double*>(src), scale, dest_offsets, dest_row_counts,
+            dest_columns, static_cast<double*>(dest_values));
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}
+
+// y := alpha x + beta y into y's existing row capacity (bsr_axpy with
+// topology="padded", warp/_src/sparse.py:3395-3478): per row, the sorted union
+// of x's and y's active columns, values alpha x + beta y; a row whose union
+// exceeds its capacity is left unchanged with row_counts 0 and raises *status.
+// x and y must not alias.
+WA_API int warp_amd_bsr_axpy_padded(int scalar_size, int block_size, int nrow, double alpha, double beta,
+                                    const int* x_offsets, const int* x_row_counts, const int* x_columns,
+                                    const void* x_values, const int* y_offsets, int* y_row_counts, int* y_columns,
+                                    void* y_values, int* status, void* stream)
+{
+    WA_REQUIRE(nrow >= 0 && block_size > 0, "bsr_axpy: invalid sizes");
+    WA_REQUIRE(scalar_size == 4 || scalar_size == 8, "bsr_axpy: values must be fp32 or fp64");
+    if (nrow == 0)
+        return wa::kOk;
+    WA_REQUIRE(x_offsets && y_offsets && y_row_counts && x_values && y_values && status, "bsr_axpy: null arrays");
+    hipStream_t s = wa::as_stream(stream);
+    if (scalar_size == 4)
+        wa::axpy_padded_kernel<float><<<(unsigned)nrow, wa::kWave, 0, s>>>(
+            nrow, block_size, (float)alpha, (float)beta, x_offsets, x_row_counts, x_columns,
+            static_cast<const float*>(x_values), y_offsets, y_row_counts, y_columns, static_cast<float*>(y_values),
+            status);
+    else
+        wa::axpy_padded_kernel<double><<<(unsigned)nrow, wa::kWave, 0, s>>>(
+            nrow, block_size, alpha, beta, x_offsets, x_row_counts, x_columns, static_cast<const double*>(x_values),
+            y_offsets, y_row_counts, y_columns, static_cast<double*>(y_values), status);
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}
+
+// Workspace of warp_amd_bsr_mm_count for T x positions.
+WA_API size_t warp_amd_bsr_mm_count_workspace_size(int T)
+{
+    return wa::align_up((size_t)T * sizeof(int), 256) + wa::scan_workspace_bytes((int64_t)T + 1) + 256;
+}
+
+// Symbolic product x @ y (replaces bsr_mm_count_coeffs and its scan,
+// warp/_src/sparse.py:3620-3680, 4455-4475): mm_offsets[0..T] = exclusive
+// scan over the T x positions of the number of product blocks each lists
+// (see deep_ranges below), row_min[0..x_nrow) as in the reference, and
+// *total_host (pinned, optional, asynchronous) = mm_offsets[T].  deep_ranges =
+// 1 lists a row reaching fewer distinct columns than it has products as its
+// whole column span (the reference's compact-topology rule); 0 lists every
+// product.
+WA_API int warp_amd_bsr_mm_count(int T, int x_nrow, int y_nrow, int y_ncol, int deep_ranges, const int* x_offsets,
+                                 const int* x_row_counts, const int* x_columns, const int* y_offsets,
+                                 const int* y_row_counts, const int* y_columns, int* row_min, int* mm_offsets,
+                                 int* total_host, void* workspace, size_t workspace_bytes, void* stream)
+{
+    WA_REQUIRE(T >= 0 && x_nrow >= 0 && y_nrow >= 0 && y_ncol >= 0, "bsr_mm: invalid sizes");
+    WA_REQUIRE(x_offsets && y_offsets && row_min && mm_offsets, "bsr_mm: null arrays");
+    WA_REQUIRE(workspace && workspace_bytes >= warp_amd_bsr_mm_count_workspace_size(T) - 256,
+               "bsr_mm: workspace too small");
+    hipStream_t s = wa::as_stream(stream);
+    int* counts = static_cast<int*>(workspace);
+    void* scan_ws = static_cast<char*>(workspace) + wa::align_up((size_t)T * sizeof(int), 256);
+    WA_TRY(wa::launch_fill_i32(counts, T, 0, s));
+    if (x_nrow > 0) {
+        wa::mm_count_kernel<<<(unsigned)wa::ceil_div(x_nrow, wa::kBlock), wa::kBlock, 0, s>>>(
+            x_nrow, y_nrow, y_ncol, deep_ranges, x_offsets, x_row_counts, x_columns, y_offsets, y_row_counts,
+            y_columns, row_min, counts);
+        WA_LAUNCH_CHECK();
+    }
+    WA_TRY(wa::exclusive_scan_i32(counts, mm_offsets, T, scan_ws, s));
+    if (total_host)
+        WA_HIP(hipMemcpyAsync(total_host, mm_offsets + T, sizeof(int), hipMemcpyDeviceToHost, s));
+    return wa::kOk;
+}
+
+// Product triplets (replaces _bsr_mm_list_coeffs, warp/_src/sparse.py:3683-3735):
+// rows/cols[base + i] for i < n = (row, column) of product block i (in x-block
+// order, from warp_amd_bsr_mm_count's mm_offsets / row_min); (-1, -1) for
+// i >= mm_offsets[T].
+WA_API int warp_amd_bsr_mm_list(int n, int base, int T, int x_nrow, const int* x_offsets, const int* x_row_counts,
+                                const int* x_columns, const int* y_offsets, const int* y_columns, const int* row_min,
+                                const int* mm_offsets, int* rows, int* cols, void* stream)
+{
+    WA_REQUIRE(n >= 0 && base >= 0 && T >= 0 && x_nrow >= 0, "bsr_mm: invalid sizes");
+    if (n == 0)
+        return wa::kOk;
+    WA_REQUIRE(x_offsets && y_offsets && row_min && mm_offsets && rows && cols, "bsr_mm: null arrays");
+    wa::mm_list_kernel<<<(unsigned)wa::ceil_div(n, wa::kBlock), wa::kBlock, 0, wa::as_stream(stream)>>>(
+        n, base, T, x_nrow, x_offsets, x_row_counts, x_columns, y_offsets, y_columns, row_min, mm_offsets, rows, cols);
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}
+
+// Product values into an existing topology (replaces _bsr_mm_compute_values,
+// warp/_src/sparse.py:3764-3800): for each active z block (r, c) among the
+// first T positions, z += alpha * sum over x row r's active blocks k (storage
+// order) of x_k @ y(x_columns[k], c).  Blocks: x br x bk, y bk x bc, z br x bc
+// (br * bc <= 64); fp32 or fp64.  No fused multiply-add: products and sums
+// are rounded one by one, in the order stated.
+WA_API int warp_amd_bsr_mm_values(int scalar_size, int br, int bk, int bc, double alpha, int T, int z_nrow,
+                                  const int* z_offsets, const int* z_row_counts, const int* z_columns,
+                                  void* z_values, int y_nrow, const int* x_offsets, const int* x_row_counts,
+                                  const int* x_columns, const void* x_values, const int* y_offsets,
+                                  const int* y_row_counts, const int* y_columns, const void* y_values, void* stream)
+{
+    WA_REQUIRE(br > 0 && bk > 0 && bc > 0 && br * bc <= 64 && T >= 0 && z_nrow >= 0, "bsr_mm: invalid sizes");
+    WA_REQUIRE(scalar_size == 4 || scalar_size == 8, "bsr_mm: values must be fp32 or fp64");
+    if (T == 0 || z_nrow == 0)
+        return wa::kOk;
+    WA_REQUIRE(z_offsets && z_columns && z_values && x_offsets && y_offsets, "bsr_mm: null arrays");
+    const unsigned grid = (unsigned)wa::ceil_div(T, wa::kBlock);
+    hipStream_t s = wa::as_stream(stream);
+#define WA_MM(S, BR, BK, BC)                                                                                          \
+    wa::mm_values_kernel<S, BR, BK, BC><<<grid, wa::kBlock, 0, s>>>(                                                  \
+        T, z_nrow, (S)alpha, z_offsets, z_row_counts, z_columns, static_cast<S*>(z_values), y_nrow, x_offsets,       \
+        x_row_counts, x_columns, static_cast<const S*>(x_values), y_offsets, y_row_counts, y_columns,                 \
+        static_cast<const S*>(y_values))
+#define WA_MM_ANY(S)                                                                                                  \
+    wa::mm_values_any_kernel<S><<<grid, wa::kBlock, 0, s>>>(                                                          \
+        T, z_nrow, (S)alpha, br, bk, bc, z_offsets, z_row_counts, z_columns, static_cast<S*>(z_values), y_nrow,      \
+        x_offsets, x_row_counts, x_columns, static_cast<const S*>(x_values), y_offsets, y_row_counts, y_columns,      \
+        static_cast<const S*>(y_values))
+    const bool s1 = br == 1 && bk == 1 && bc == 1;
+    const bool s3 = br == 3 && bk == 3 && bc == 3;
+    if (scalar_size == 4) {
+        if (s1)
+            WA_MM(float, 1, 1, 1);
+        else if (s3)
+            WA_MM(float, 3, 3, 3);
+        else
+            WA_MM_ANY(float);
+    } else {
+        if (s1)
+            WA_MM(double, 1, 1, 1);
+        else if (s3)
+            WA_MM(double, 3, 3, 3);
+        else
+            WA_MM_ANY(double);
+    }
+#undef WA_MM
+#undef WA_MM_ANY
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}
+
+// Block-diagonal topology of an nrow-row matrix: offsets[r] = min(r, ndiag),
+// columns[r] = r for r < ndiag (replaces _bsr_set_diag_kernel of bsr_set_diag,
+// warp/_src/sparse.py:2939-2995).
+WA_API int warp_amd_bsr_diag_topology(int nrow, int ndiag, int* offsets, int* columns, void* stream)
+{
+    WA_REQUIRE(nrow >= 0 && ndiag >= 0 && ndiag <= nrow, "bsr_set_diag: invalid sizes");
+    WA_REQUIRE(offsets && (ndiag == 0 || columns), "bsr_set_diag: null arrays");
+    wa::diag_topology_kernel<<<(unsigned)wa::ceil_div((int64_t)nrow + 1, wa::kBlock), wa::kBlock, 0,
+                               wa::as_stream(stream)>>>(nrow, ndiag, offsets, columns);
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}
+
+// y = alpha x + beta y over n scalars, fp32 or fp64 (reference fem array_axpy,
+// warp/_src/fem/linalg.py:425-461).
+WA_API int warp_amd_axpby(int scalar_size, int64_t n, double alpha, const void* x, double beta, void* y, void* stream)
+{
+    WA_REQUIRE(n >= 0, "axpby: invalid size");
+    WA_REQUIRE(scalar_size == 4 || scalar_size == 8, "axpby: values must be fp32 or fp64");
+    if (n == 0)
+        return wa::kOk;
+    WA_REQUIRE(x && y, "axpby: null arrays");
+    const unsigned grid = (unsigned)wa::ceil_div(n, wa::kBlock);
+    hipStream_t s = wa::as_stream(stream);
+    if (scalar_size == 4)
+        wa::axpby_kernel<float><<<grid, wa::kBlock, 0, s>>>(n, (float)alpha, static_cast<const float*>(x), (float)beta,
+                                                            static_cast<float*>(y));
+    else
+        wa::axpby_kernel<double><<<grid, wa::kBlock, 0, s>>>(n, alpha, static_cast<const double*>(x), beta,
+                                                             static_cast<double*>(y));
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}
+
+// Normalize the diagonal blocks (block_dim x block_dim, block_dim <= 8) of a
+// Dirichlet projector into orthogonal projectors, and the fixed values
+// (nrow x block_dim, may be NULL) accordingly (replaces
+// normalize_dirichlet_projector's kernels, warp/_src/fem/dirichlet.py:12-66,
+// 127-199).  The eigen decomposition runs in FP64 (Jacobi rotations) where the
+// reference runs a QR iteration in the matrix scalar type.
+WA_API int warp_amd_normalize_dirichlet_projector(int scalar_size, int block_dim, int nrow, const int* offsets,
+                                                  const int* row_counts, const int* columns, void* values,
+                                                  void* fixed_values, void* stream)
+{
+    WA_REQUIRE(nrow >= 0 && block_dim > 0 && block_dim <= wa::kMaxEigen, "normalize_dirichlet_projector: invalid sizes");
+    WA_REQUIRE(scalar_size == 4 || scalar_size == 8, "normalize_dirichlet_projector: values must be fp32 or fp64");
+    if (nrow == 0)
+        return wa::kOk;
+    WA_REQUIRE(offsets && values, "normalize_dirichlet_projector: null arrays");
+    const unsigned grid = (unsigned)wa::ceil_div(nrow, wa::kBlock);
+    hipStream_t s = wa::as_stream(stream);
+    if (scalar_size == 4)
+        wa::normalize_projector_kernel<float><<<grid, wa::kBlock, 0, s>>>(
+            nrow, block_dim, offsets, row_counts, columns, static_cast<float*>(values), static_cast<float*>(fixed_values));
+    else
+        wa::normalize_projector_kernel<double><<<grid, wa::kBlock, 0, s>>>(
+            nrow, block_dim, offsets, row_counts, columns, static_cast<double*>(values),
+            static_cast<double*>(fixed_values));
     WA_LAUNCH_CHECK();
     return wa::kOk;
 }

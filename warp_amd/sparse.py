@@ -162,21 +162,28 @@ class BsrMatrix:
 
     def uncompress_rows(self, out: torch.Tensor | None = None) -> torch.Tensor:
         """Row index of every stored block slot, ``-1`` outside the active blocks (sparse.py:164-193)."""
-        starts = self.offsets[: self.nrow].to(torch.int64)
-        if self.row_counts is None:
-            counts = self.offsets[1 : self.nrow + 1].to(torch.int64) - starts
-        else:
-            counts = self.row_counts[: self.nrow].to(torch.int64)
-        rows = torch.repeat_interleave(torch.arange(self.nrow, device=self.device, dtype=torch.int32), counts)
-        # position of each active block: its row start plus its rank within the row
-        first = torch.repeat_interleave(torch.cumsum(counts, 0) - counts, counts)
-        pos = torch.repeat_interleave(starts, counts) + torch.arange(rows.numel(), device=self.device) - first
         if out is None:
-            out = torch.full((self.nnz,), -1, dtype=torch.int32, device=self.device)
-        else:
-            out[: self.nnz].fill_(-1)
-        keep = pos < self.nnz
-        out[pos[keep]] = rows[keep]
+            out = torch.empty(self.nnz, dtype=torch.int32, device=self.device)
+        n = min(int(self.nnz), out.numel())
+        if self.device.type != "cuda":
+            # host-side bookkeeping of CPU-resident matrices (no device work involved)
+            starts = self.offsets[: self.nrow].to(torch.int64)
+            if self.row_counts is None:
+                counts = self.offsets[1 : self.nrow + 1].to(torch.int64) - starts
+            else:
+                counts = torch.minimum(self.row_counts[: self.nrow].to(torch.int64),
+                                       self.offsets[1 : self.nrow + 1].to(torch.int64) - starts)
+            rows = torch.repeat_interleave(torch.arange(self.nrow, dtype=torch.int32), counts)
+            first = torch.repeat_interleave(torch.cumsum(counts, 0) - counts, counts)
+            pos = torch.repeat_interleave(starts, counts) + torch.arange(rows.numel()) - first
+            out[:n].fill_(-1)
+            keep = pos < n
+            out[pos[keep]] = rows[keep]
+            return out
+        _native.call(
+            "warp_amd_bsr_uncompress_rows", self.nrow, _native.ptr(self.offsets), _native.ptr(self.row_counts), n,
+            _native.ptr(out), _native.stream_of(self.device),
+        )
         return out
 
     # -- sticky status of padded operations (sparse.py:265-310) -----------
@@ -214,21 +221,45 @@ class BsrMatrix:
         if status is not None:
             status.zero_()
 
-    # -- operators -------------------------------------------------------
+    # -- operators (sparse.py:411-462) -------------------------------------
     def transpose(self):
         """Transposed copy of this matrix (sparse.py:460-462)."""
         return bsr_transposed(self)
 
+    def __add__(self, y):
+        return bsr_axpy(y, bsr_copy(self))
+
+    def __iadd__(self, y):
+        return bsr_axpy(y, self)
+
+    def __radd__(self, x):
+        return bsr_axpy(x, bsr_copy(self))
+
+    def __sub__(self, y):
+        return bsr_axpy(y, bsr_copy(self), alpha=-1.0)
+
+    def __rsub__(self, x):
+        return bsr_axpy(x, bsr_copy(self), beta=-1.0)
+
+    def __isub__(self, y):
+        return bsr_axpy(y, self, alpha=-1.0)
+
+    def __imul__(self, y):
+        return bsr_scale(self, y)
+
     def __matmul__(self, x):
         if isinstance(x, torch.Tensor):
             return bsr_mv(self, x)
-        return NotImplemented
+        return bsr_mm(self, x)
 
     def __rmatmul__(self, x):
         """x @ A = A^T x (reference BsrMatrix.__rmatmul__)."""
         if isinstance(x, torch.Tensor):
             return bsr_mv(self, x, transpose=True)
-        return NotImplemented
+        return bsr_mm(x, self)
+
+    def __imatmul__(self, y):
+        return bsr_mm(self, y, self)
 
     def __mul__(self, s):
         return _BsrScalingExpression(self, s)
@@ -253,12 +284,24 @@ class _BsrScalingExpression:
     def __matmul__(self, x):
         if isinstance(x, torch.Tensor):
             return bsr_mv(self, x)
-        return NotImplemented
+        return bsr_mm(self.mat, x, alpha=self.scale)
 
     def __rmatmul__(self, x):
         if isinstance(x, torch.Tensor):
             return bsr_mv(self, x, transpose=True)
-        return NotImplemented
+        return bsr_mm(x, self.mat, alpha=self.scale)
+
+    def __add__(self, y):
+        return bsr_axpy(y, bsr_copy(self.mat), alpha=self.scale)
+
+    def __radd__(self, x):
+        return bsr_axpy(x, bsr_copy(self.mat), beta=self.scale)
+
+    def __sub__(self, y):
+        return bsr_axpy(y, bsr_copy(self.mat), alpha=-self.scale)
+
+    def __rsub__(self, x):
+        return bsr_axpy(x, bsr_copy(self.mat), beta=-self.scale)
 
     def __mul__(self, s):
         return _BsrScalingExpression(self.mat, self.scale * s)
@@ -1088,6 +1131,389 @@ def bsr_copy(A, scalar_type=None, block_shape=None, structure_only: bool = False
     copy = bsr_zeros(src.nrow, src.ncol, block_type, device=src.device)
     bsr_assign(copy, A, structure_only=structure_only, topology="compact")
     return copy
+
+
+class bsr_axpy_work_arrays:
+    """Reusable temporaries of :func:`bsr_axpy` (sparse.py:3314-3337)."""
+
+    def __init__(self):
+        self._buffers = {}
+
+    def _get(self, name, n, dtype, device):
+        buf = self._buffers.get(name)
+        if buf is None or buf.numel() < n or buf.dtype != dtype or buf.device != device:
+            buf = torch.empty(max(n, 1), dtype=dtype, device=device)
+            self._buffers[name] = buf
+        return buf[:n]
+
+
+class bsr_mm_work_arrays(bsr_axpy_work_arrays):
+    """Reusable temporaries of :func:`bsr_mm` (sparse.py:3992-4064).  The product
+    topology is recomputed on every call (``reuse_topology`` gives the same result)."""
+
+
+def _check_same_block_type(x: BsrMatrix, y: BsrMatrix):
+    if x.values.device != y.values.device:
+        raise ValueError(f"All arguments must reside on the same device, got {x.values.device} and {y.values.device}")
+    if x.scalar_type != y.scalar_type or x.block_shape != y.block_shape:
+        raise ValueError(
+            f"Matrices must have the same block type, got ({x.block_shape}, {x.scalar_type}) and ({y.block_shape}, {y.scalar_type})"
+        )
+    if x.nrow != y.nrow or x.ncol != y.ncol:
+        raise ValueError(
+            f"Matrices must have the same number of rows and columns, got ({x.nrow}, {x.ncol}) and ({y.nrow}, {y.ncol})"
+        )
+
+
+def _add_blocks(dest: BsrMatrix, rows, cols, values, n: int, scale: float):
+    """dest[block (rows[i], cols[i])] += scale * values[i] for i < n."""
+    _native.call(
+        "warp_amd_bsr_add_blocks", dest.scalar_type.size, dest.block_size, n, _native.ptr(rows), _native.ptr(cols),
+        _native.ptr(values), float(scale), _native.ptr(dest.offsets), _native.ptr(dest.row_counts),
+        _native.ptr(dest.columns), _native.ptr(dest.values), _native.stream_of(dest.device),
+    )
+
+
+def _topology_from_triplets(dest: BsrMatrix, rows, cols, n: int):
+    """Compact topology of the (row, col) pairs into ``dest`` (offsets, columns, pending nnz)."""
+    _bsr_ensure_fits(dest, nnz=n)
+    lib = _native.lib()
+    ws_bytes = lib.warp_amd_bsr_from_triplets_workspace_size(dest.nrow, n)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dest.device)
+    nnz_buf, nnz_event = dest._setup_nnz_transfer()
+    _native.call(
+        "warp_amd_bsr_from_triplets", dest.block_size, 0, dest.nrow, dest.ncol, n, None, _native.ptr(rows),
+        _native.ptr(cols), None, 0, 0, None, None, _native.ptr(dest.offsets), None, _native.ptr(dest.columns), None,
+        nnz_buf.data_ptr(), _native.ptr(ws), ws_bytes, _native.stream_of(dest.device),
+    )
+    nnz_event.record(torch.cuda.current_stream(dest.device))
+    dest._nnz_pending = True
+    dest.row_counts = None
+
+
+def bsr_axpy(x, y: BsrMatrix | None = None, alpha=1.0, beta=1.0, masked: bool = False, work_arrays=None,
+             topology: str | None = None) -> BsrMatrix:
+    """``y := alpha * x + beta * y``; returns ``y`` (sparse.py:3340-3617).
+
+    ``"compact"`` (default) gives ``y`` the union topology; ``"masked"`` keeps
+    the topology of ``y`` (blocks of ``x`` outside it are dropped);
+    ``"padded"`` merges each row into the existing row capacity of ``y`` and
+    raises ``BSR_STATUS_ROW_CAPACITY_EXCEEDED`` (row left unchanged, emptied)
+    where the union does not fit.  ``x`` and ``y`` may alias.  Blocks present
+    in both get ``beta * y + alpha * x``."""
+    x, x_scale = _extract_matrix_and_scale(x)
+    alpha = float(alpha) * x_scale
+    beta = float(beta)
+    if topology is None:
+        topology = "masked" if masked else "compact"
+    elif topology not in ("compact", "masked", "padded"):
+        raise ValueError(f"Unsupported topology policy: {topology}")
+    elif masked and topology != "masked":
+        raise ValueError("Cannot pass masked=True with a non-masked topology policy")
+    if masked:
+        _warn_masked_arg_deprecated("bsr_axpy")
+    if y is None:
+        if topology != "compact":
+            raise ValueError("Left-hand-side 'y' matrix must be provided for this topology policy")
+        y = bsr_zeros(x.nrow, x.ncol, block_type=x.dtype, device=x.values.device)
+        beta = 0.0
+    x_nnz, y_nnz = int(x.nnz), int(y.nnz)
+
+    if topology == "padded":
+        _check_same_block_type(x, y)
+        status = y._ensure_status()
+        if beta == 0.0:
+            if x is y:
+                return bsr_scale(y, alpha)
+            _bsr_ensure_independent_row_counts(y)
+            _require_gpu(y.device, "bsr_axpy")
+            _native.call(
+                "warp_amd_bsr_assign_padded", x.scalar_type.size, y.scalar_type.size, y.nrow, y.block_size,
+                _native.ptr(x.offsets), _native.ptr(x.row_counts), _native.ptr(x.columns), _native.ptr(x.values),
+                alpha, _native.ptr(y.offsets), _native.ptr(y.row_counts), _native.ptr(y.columns),
+                _native.ptr(y.values), _native.ptr(status), _native.stream_of(y.device),
+            )
+            return y
+        if alpha == 0.0 or x_nnz == 0:
+            return bsr_scale(y, beta)
+        if x is y:
+            return bsr_scale(y, alpha + beta)
+        _bsr_ensure_independent_row_counts(y)
+        _require_gpu(y.device, "bsr_axpy")
+        _native.call(
+            "warp_amd_bsr_axpy_padded", y.scalar_type.size, y.block_size, y.nrow, alpha, beta,
+            _native.ptr(x.offsets), _native.ptr(x.row_counts), _native.ptr(x.columns), _native.ptr(x.values),
+            _native.ptr(y.offsets), _native.ptr(y.row_counts), _native.ptr(y.columns), _native.ptr(y.values),
+            _native.ptr(status), _native.stream_of(y.device),
+        )
+        return y
+
+    if beta == 0.0 or y_nnz == 0:
+        bsr_assign(src=x, dest=y, topology=topology)
+        return bsr_scale(y, alpha)
+    if alpha == 0.0 or x_nnz == 0:
+        return bsr_scale(y, beta)
+    if x is y:
+        return bsr_scale(y, alpha + beta)
+    _check_same_block_type(x, y)
+    _require_gpu(y.device, "bsr_axpy")
+    if y.scalar_type.torch_dtype not in (torch.float32, torch.float64):
+        raise TypeError("bsr_axpy supports float32 and float64 matrices")
+    device = y.values.device
+    xT, yT = _positions(x), _positions(y)
+
+    if topology == "masked":
+        bsr_scale(y, beta)
+        rows = torch.empty(xT, dtype=torch.int32, device=device)
+        x.uncompress_rows(out=rows)
+        _add_blocks(y, rows, x.columns, x.values, xT, alpha)
+        return y
+
+    work = work_arrays if work_arrays is not None else bsr_axpy_work_arrays()
+    n = xT + yT
+    rows = work._get("rows", n, torch.int32, device)
+    cols = work._get("cols", n, torch.int32, device)
+    y.uncompress_rows(out=rows[:yT])
+    cols[:yT].copy_(y.columns[:yT])
+    x.uncompress_rows(out=rows[yT:])
+    cols[yT:].copy_(x.columns[:xT])
+    old_values = work._get("old_y_values", yT * y.block_size, y.values.dtype, device)
+    old_values.copy_(y.values[:yT].reshape(-1))
+    _topology_from_triplets(y, rows, cols, n)
+    y.values.zero_()
+    _add_blocks(y, rows, cols, old_values, yT, beta)
+    _add_blocks(y, rows[yT:], cols[yT:], x.values, xT, alpha)
+    return y
+
+
+def bsr_mm(x, y, z: BsrMatrix | None = None, alpha=1.0, beta=0.0, masked: bool = False, work_arrays=None,
+           reuse_topology: bool = False, tile_size: int = 0, max_new_nnz: int | None = None,
+           topology: str | None = None) -> BsrMatrix:
+    """``z := alpha * x @ y + beta * z``; returns ``z`` (sparse.py:4200-4640).
+
+    The compact topology lists every product block, except that a row whose
+    product count exceeds the column span it reaches takes that whole span
+    (the reference's rule for deep products, so results match its topology).
+    ``"masked"`` keeps the topology of ``z``; ``"padded"`` writes the exact
+    product topology (plus ``z``'s blocks when ``beta != 0``) into the row
+    capacity of ``z``, raising the row-capacity status where it does not fit.
+    Each block is ``sum_k x[r, k] @ y[k, c]`` over the row's blocks of ``x`` in
+    storage order; ``tile_size`` is accepted and does not change the order.
+    ``max_new_nnz`` caps the product blocks listed (extra ones are dropped with
+    a warning, as the reference does).  Operands may alias ``z``."""
+    x, x_scale = _extract_matrix_and_scale(x)
+    y, y_scale = _extract_matrix_and_scale(y)
+    alpha = float(alpha) * x_scale * y_scale
+    beta = float(beta)
+    if topology is None:
+        topology = "masked" if masked else "compact"
+    elif topology not in ("compact", "masked", "padded"):
+        raise ValueError(f"Unsupported topology policy: {topology}")
+    elif masked and topology != "masked":
+        raise ValueError("Cannot pass masked=True with a non-masked topology policy")
+    if masked:
+        _warn_masked_arg_deprecated("bsr_mm")
+    if topology != "compact" and reuse_topology:
+        raise ValueError("reuse_topology is only supported with topology='compact'")
+    if z is None:
+        if topology != "compact":
+            raise ValueError("Left-hand-side 'z' matrix must be provided for this topology policy")
+        zshape = (x.block_shape[0], y.block_shape[1])
+        ztype = x.scalar_type if zshape == (1, 1) else wt.matrix(shape=zshape, dtype=x.scalar_type)
+        z = bsr_zeros(x.nrow, y.ncol, block_type=ztype, device=x.values.device)
+        beta = 0.0
+    if x.values.device != y.values.device or x.values.device != z.values.device:
+        raise ValueError(
+            f"All arguments must reside on the same device, got {x.values.device}, {y.values.device} and {z.values.device}"
+        )
+    if x.scalar_type != y.scalar_type or x.scalar_type != z.scalar_type:
+        raise ValueError(
+            f"Matrices must have the same scalar type, got {x.scalar_type}, {y.scalar_type} and {z.scalar_type}"
+        )
+    if x.block_shape[0] != z.block_shape[0] or y.block_shape[1] != z.block_shape[1] or x.block_shape[1] != y.block_shape[0]:
+        raise ValueError(
+            f"Incompatible block sizes for matrix multiplication, got ({x.block_shape}, {y.block_shape}) and ({z.block_shape})"
+        )
+    if x.nrow != z.nrow or z.ncol != y.ncol or x.ncol != y.nrow:
+        raise ValueError(
+            f"Incompatible number of rows/columns for matrix multiplication, got ({x.nrow}, {x.ncol}) and ({y.nrow}, {y.ncol})"
+        )
+    if z.scalar_type.torch_dtype not in (torch.float32, torch.float64):
+        raise TypeError("bsr_mm supports float32 and float64 matrices")
+    _require_gpu(z.device, "bsr_mm")
+    device = z.values.device
+    stream = _native.stream_of(device)
+
+    if alpha == 0.0 or x.nnz == 0 or y.nnz == 0:
+        if topology == "padded":
+            z._ensure_status()
+        if topology == "masked" and beta == 0.0:
+            z.values.zero_()
+            return z
+        return bsr_scale(z, beta)
+
+    if topology == "masked":
+        if z is x or z is y:
+            raise ValueError("topology='masked' is not supported for aliased inputs")
+        if beta == 0.0:
+            z.values.zero_()
+        else:
+            bsr_scale(z, beta)
+        _mm_values(x, y, z, alpha, _positions(z))
+        return z
+
+    # operands aliasing the output are read from copies
+    if x is z:
+        x = bsr_copy(x, topology="padded" if x.row_counts is not None else "compact")
+    if y is z:
+        y = bsr_copy(y, topology="padded" if y.row_counts is not None else "compact")
+
+    if topology == "compact" and max_new_nnz is None:
+        z.nnz_sync()
+    work = work_arrays if work_arrays is not None else bsr_mm_work_arrays()
+    padded = topology == "padded"
+    keep_z = beta != 0.0
+    zT = _positions(z) if keep_z else 0
+    xT = _positions(x)
+    lib = _native.lib()
+    row_min = work._get("row_min", x.nrow + 1, torch.int32, device)
+    mm_offsets = work._get("mm_offsets", xT + 1, torch.int32, device)
+    ws_bytes = lib.warp_amd_bsr_mm_count_workspace_size(xT)
+    ws = work._get("count_ws", ws_bytes, torch.uint8, device)
+    total_buf = torch.zeros(1, dtype=torch.int32, pin_memory=True)
+    _native.call(
+        "warp_amd_bsr_mm_count", xT, x.nrow, y.nrow, y.ncol, 0 if padded else 1, _native.ptr(x.offsets),
+        _native.ptr(x.row_counts), _native.ptr(x.columns), _native.ptr(y.offsets), _native.ptr(y.row_counts),
+        _native.ptr(y.columns), _native.ptr(row_min), _native.ptr(mm_offsets),
+        total_buf.data_ptr() if max_new_nnz is None else None, _native.ptr(ws), ws_bytes, stream,
+    )
+    if max_new_nnz is None:
+        torch.cuda.current_stream(device).synchronize()
+        n_new = int(total_buf[0])
+        if n_new == 0:
+            return bsr_scale(z, beta)
+    else:
+        n_new = int(max_new_nnz)
+        if not torch.cuda.is_current_stream_capturing():
+            total = int(mm_offsets[xT : xT + 1].cpu()[0])
+            if total > n_new:
+                warnings.warn(
+                    f"Number of potential `bsr_mm` blocks ({total}) exceeded `max_new_nnz` ({n_new})",
+                    RuntimeWarning, stacklevel=2,
+                )
+    n = zT + n_new
+    rows = work._get("rows", n, torch.int32, device)
+    cols = work._get("cols", n, torch.int32, device)
+    old_values = None
+    if zT > 0:
+        z.uncompress_rows(out=rows[:zT])
+        cols[:zT].copy_(z.columns[:zT])
+        old_values = work._get("old_z_values", zT * z.block_size, z.values.dtype, device)
+        old_values.copy_(z.values[:zT].reshape(-1))
+    _native.call(
+        "warp_amd_bsr_mm_list", n_new, zT, xT, x.nrow, _native.ptr(x.offsets), _native.ptr(x.row_counts),
+        _native.ptr(x.columns), _native.ptr(y.offsets), _native.ptr(y.columns), _native.ptr(row_min),
+        _native.ptr(mm_offsets), _native.ptr(rows), _native.ptr(cols), stream,
+    )
+    if padded:
+        status = z._ensure_status()
+        _bsr_ensure_independent_row_counts(z)
+        tws = lib.warp_amd_bsr_topology_workspace_size(z.nrow, n, 0)
+        tw = work._get("topology_ws", tws, torch.uint8, device)
+        _native.call(
+            "warp_amd_bsr_set_from_triplets_padded", z.block_size, 0, z.nrow, z.ncol, n, None, _native.ptr(rows),
+            _native.ptr(cols), None, 0, _native.ptr(z.offsets), _native.ptr(z.row_counts), _native.ptr(z.columns),
+            None, 0, _native.ptr(status), _native.ptr(tw), tws, stream,
+        )
+        zpos = _positions(z)
+    else:
+        _topology_from_triplets(z, rows, cols, n)
+        zpos = z.nnz_sync() if max_new_nnz is None else n
+        _bsr_ensure_fits(z, nnz=zpos)
+    z.values.zero_()
+    if zT > 0:
+        _add_blocks(z, rows, cols, old_values, zT, beta)
+    _mm_values(x, y, z, alpha, zpos)
+    return z
+
+
+def _mm_values(x: BsrMatrix, y: BsrMatrix, z: BsrMatrix, alpha: float, T: int):
+    br, bk = x.block_shape
+    bc = y.block_shape[1]
+    _native.call(
+        "warp_amd_bsr_mm_values", z.scalar_type.size, br, bk, bc, float(alpha), T, z.nrow, _native.ptr(z.offsets),
+        _native.ptr(z.row_counts), _native.ptr(z.columns), _native.ptr(z.values), y.nrow, _native.ptr(x.offsets),
+        _native.ptr(x.row_counts), _native.ptr(x.columns), _native.ptr(x.values), _native.ptr(y.offsets),
+        _native.ptr(y.row_counts), _native.ptr(y.columns), _native.ptr(y.values), _native.stream_of(z.device),
+    )
+
+
+def bsr_set_diag(A: BsrMatrix, diag=None, rows_of_blocks: int | None = None, cols_of_blocks: int | None = None):
+    """Make ``A`` block diagonal (sparse.py:2939-2995).  ``diag``: a tensor of
+    blocks (one per diagonal block), one block value for all (a number or
+    nested list, not a tensor), or None (values left as they are)."""
+    if rows_of_blocks is None and cols_of_blocks is not None:
+        rows_of_blocks = cols_of_blocks
+    if cols_of_blocks is None and rows_of_blocks is not None:
+        cols_of_blocks = rows_of_blocks
+    if isinstance(diag, torch.Tensor) and rows_of_blocks is None:
+        rows_of_blocks = cols_of_blocks = diag.shape[0]
+    if rows_of_blocks is not None:
+        _bsr_resize(A, rows_of_blocks, cols_of_blocks)
+    nnz = min(A.nrow, A.ncol)
+    _bsr_ensure_fits(A, nnz=nnz)
+    _require_gpu(A.device, "bsr_set_diag")
+    _native.call("warp_amd_bsr_diag_topology", A.nrow, nnz, _native.ptr(A.offsets), _native.ptr(A.columns),
+                 _native.stream_of(A.device))
+    A.row_counts = None
+    A._nnz_pending = False
+    A.nnz = nnz
+    if isinstance(diag, torch.Tensor):
+        A.values[:nnz].copy_(diag[:nnz].reshape(A.values[:nnz].shape))
+    elif diag is not None:
+        A.values[:nnz] = torch.as_tensor(diag, dtype=A.values.dtype).to(A.values.device)
+
+
+def bsr_diag(diag=None, rows_of_blocks: int | None = None, cols_of_blocks: int | None = None, block_type=None,
+             device=None) -> BsrMatrix:
+    """New block-diagonal matrix (sparse.py:2998-3054)."""
+    if rows_of_blocks is None and cols_of_blocks is not None:
+        rows_of_blocks = cols_of_blocks
+    if cols_of_blocks is None and rows_of_blocks is not None:
+        cols_of_blocks = rows_of_blocks
+    if isinstance(diag, torch.Tensor):
+        if rows_of_blocks is None:
+            rows_of_blocks = cols_of_blocks = diag.shape[0]
+        device = diag.device
+        if block_type is None:
+            block_type = (wt.type_to_scalar(diag.dtype) if diag.ndim == 1
+                          else wt.matrix(shape=tuple(diag.shape[1:]), dtype=wt.type_to_scalar(diag.dtype)))
+    elif rows_of_blocks is None:
+        raise ValueError(
+            "rows_of_blocks and/or cols_of_blocks must be provided for constructing a diagonal matrix with uniform diagonal"
+        )
+    if block_type is None:
+        if diag is None:
+            raise ValueError("Either `diag` or `block_type` needs to be provided")
+        d = torch.as_tensor(diag)
+        block_type = (wt.type_to_scalar(d.dtype) if d.ndim == 0
+                      else wt.matrix(shape=tuple(d.shape), dtype=wt.type_to_scalar(d.dtype)))
+    A = bsr_zeros(rows_of_blocks, cols_of_blocks, block_type=block_type, device=device)
+    bsr_set_diag(A, diag)
+    return A
+
+
+def bsr_set_identity(A: BsrMatrix, rows_of_blocks: int | None = None) -> None:
+    """Make ``A`` the identity (sparse.py:3057-3071)."""
+    identity = 1.0 if A.block_shape == (1, 1) else torch.eye(A.block_shape[0]).tolist()
+    bsr_set_diag(A, diag=identity, rows_of_blocks=rows_of_blocks, cols_of_blocks=rows_of_blocks)
+
+
+def bsr_identity(rows_of_blocks: int, block_type, device=None) -> BsrMatrix:
+    """New square identity matrix (sparse.py:3074-3093)."""
+    A = bsr_zeros(rows_of_blocks, rows_of_blocks, block_type=block_type, device=device)
+    bsr_set_identity(A)
+    return A
 
 
 def _vec_view(array: torch.Tensor, torch_dtype: torch.dtype, expected_scalar_count: int) -> torch.Tensor:
