@@ -312,6 +312,43 @@ def main() -> int:
                                   "alg_GBps": round((T5 * 40 + u5 * 40 + 4 * (nn5 + 1)) / t_c / 1e6, 1)}
         del A5, cols_p, vals_p, counts
 
+        # transpose of the elasticity matrix and its Dirichlet projection (the fem steps
+        # between assembly and solve): x = 0 face, identity projector blocks, fixed value 0
+        from warp_amd import fem
+
+        ms_t = []
+        for rep in range(4):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            At = sparse.bsr_transposed(A)
+            nt = At.nnz_sync()
+            if rep:
+                ms_t.append((time.perf_counter() - t0) * 1e3)
+            del At
+        t_t = float(np.median(ms_t))
+        t_bytes = nnzb * (8 + 2 * 36) + 8 * (nn + 1)
+        log(f"[bench] transpose: {nt} blocks in {t_t:.2f} ms ({t_bytes / t_t / 1e6:.0f} GB/s algorithmic)")
+        result["transpose"] = {"blocks": nt, "ms": round(t_t, 3), "alg_GBps": round(t_bytes / t_t / 1e6, 1)}
+        p1 = n + 1
+        face = torch.nonzero(torch.arange(nn, device=dev) % p1 == 0).reshape(-1).to(torch.int32)
+        eye = torch.eye(3, device=dev).expand(face.numel(), 3, 3).contiguous()
+        ms_d = []
+        for rep in range(3):
+            Ad = sparse.bsr_copy(A)
+            P = sparse.bsr_from_triplets(nn, nn, face, face, eye)
+            bd = b.clone()
+            fixed = torch.zeros_like(b)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fem.project_linear_system(Ad, bd, P, fixed)
+            nd = Ad.nnz_sync()
+            if rep:
+                ms_d.append((time.perf_counter() - t0) * 1e3)
+            del Ad, P
+        t_d = float(np.median(ms_d))
+        log(f"[bench] Dirichlet projection (normalize + rhs + (I-P)A(I-P)+P): {nd} blocks in {t_d:.2f} ms")
+        result["dirichlet_projection"] = {"face_nodes": int(face.numel()), "blocks": nd, "ms": round(t_d, 3)}
+
         # FP32 Jacobi-CG + FP64 iterative refinement to 1e-10 on 256^3 variable-coefficient Poisson,
         # solution checked on the host by the oracle's FP64 residual
         if args.ir_n > 0:
