@@ -4,21 +4,29 @@
 // declared at warp/native/warp.h:563-602) and the value accumulation kernel
 // _bsr_accumulate_triplet_values (warp/_src/sparse.py:1073-1090, 1122-1141).
 //
-// Algorithm (an MSD radix sort whose first digit is the whole row index):
-//   1. classify: drop out-of-range / masked-out / all-zero triplets, count
-//      triplets per row (atomic histogram);
-//   2. exclusive scan of the histogram -> bucket start of every row;
-//   3. scatter key = (col << 32) | triplet_index into the row's bucket;
-//   4. sort every bucket by key (LDS bitonic sort, one wave per row of <= 256
-//      triplets, one workgroup per larger row, with global rank merges beyond
-//      4096): this yields exactly the order of a stable sort by (row, col),
-//      whatever order the scatter produced;
-//   5. per-row unique count -> exclusive scan -> bsr offsets;
-//   6. per row: columns and inclusive run ends (summed_block_offsets);
-//   7. per unique block: values = v[i0] + v[i1] + ... sequentially in
-//      ascending triplet order (bit-identical to the reference CUDA order).
+// Algorithm (an MSD radix sort whose digits are a coarse row bucket, the row,
+// then (column, triplet index)):
+//   1. classify, per tile of triplets: drop out-of-range / masked-out /
+//      all-zero triplets and count the tile's triplets per coarse bucket of
+//      rows in LDS;
+//   2. exclusive scan of the (bucket, tile) counts;
+//   3. scatter key = (col << 32) | triplet_index of every triplet into its
+//      bucket's segment for the tile (LDS-atomic ranks, no global atomics);
+//   4. per bucket: count its rows in LDS -> row starts, and move its keys into
+//      row order;
+//   5. sort every row by key (bitonic in registers across one wavefront for
+//      <= 64 keys, in LDS by one workgroup beyond, with global rank merges past
+//      4096 keys): exactly the order of a stable sort by (row, col), whatever
+//      order the scatters produced; unique count per row -> scan -> offsets;
+//   6. per row (<= 64 keys: sorted again in registers): columns, inclusive run
+//      ends (summed_block_offsets) when asked for, and per unique block
+//      values = v[i0] + v[i1] + ... sequentially in ascending triplet order
+//      (bit-identical to the reference CUDA order).
+// Matrices of more than 4096 * 4096 rows use a single-level row histogram and
+// scatter with global atomics for steps 1-4.
 #include "common.h"
 
+#include <algorithm>
 #include <climits>
 
 namespace wa {
@@ -29,31 +37,80 @@ constexpr int kSmallCap = 256;    // rows sorted by one wave
 constexpr int kLargeCap = 4096;   // rows sorted in one workgroup's LDS
 constexpr uint64_t kPadKey = ~uint64_t(0);
 
+// Where each row's keys live in the key array: at start[r], count[r] of them
+// (count NULL: up to start[r + 1]).
+struct RowRanges {
+    const int* start;
+    const int* count;
+    __device__ __forceinline__ int base(int r) const { return start[r]; }
+    __device__ __forceinline__ int size(int r) const { return count ? count[r] : start[r + 1] - start[r]; }
+};
+
+// Two-level row bucketing: rows are grouped into coarse buckets of 1 << shift
+// consecutive rows (at most kMaxBuckets buckets, at most kMaxBucketRows rows
+// each); the triplets are cut into tiles of `tile` consecutive triplets.
+constexpr int kMaxBuckets = 4096;
+constexpr int kMaxBucketRows = 4096;
+constexpr int kMaxTiles = 4096;
+constexpr int kMinTile = 16384;
+
+struct Partition {
+    int shift = 0;
+    int nb = 1;       // coarse buckets
+    int tile = kMinTile;
+    int ntiles = 1;
+    bool tiled = false;  // false: single-level bucketing with global atomics
+};
+
+Partition partition_of(int nrow, int T)
+{
+    Partition p;
+    while (((int64_t)nrow + (1ll << p.shift) - 1) >> p.shift > kMaxBuckets)
+        ++p.shift;
+    p.nb = (int)(((int64_t)nrow + (1ll << p.shift) - 1) >> p.shift);
+    if (p.nb < 1)
+        p.nb = 1;
+    p.tiled = (1 << p.shift) <= kMaxBucketRows && T > 0 && nrow > 0;
+    const int64_t per = (T + kMaxTiles - 1) / kMaxTiles;
+    p.tile = (int)(per > kMinTile ? (per + kBlock - 1) / kBlock * kBlock : kMinTile);
+    p.ntiles = (int)((T + (int64_t)p.tile - 1) / p.tile);
+    if (p.ntiles < 1)
+        p.ntiles = 1;
+    return p;
+}
+
 struct TripletWs {
     int* trow;
     int* hist;
     int* row_start;
     uint64_t* keys;
     uint64_t* alt;
+    int* brow;
+    int* tile_offsets;
     int* large_rows;
     int* counters;
     void* scan_ws;
     int* summed_offsets;
     int* summed_indices;
+    Partition part;
 };
 
 TripletWs carve_ws(void* base, size_t bytes, int nrow, int T, bool own_summed, size_t* needed)
 {
     Carve c(base, bytes);
     TripletWs w;
+    w.part = partition_of(nrow, T);
+    const int64_t ntile_hist = w.part.tiled ? (int64_t)w.part.nb * w.part.ntiles + 1 : 1;
     w.trow = c.take<int>((size_t)T);
     w.hist = c.take<int>((size_t)nrow + 1);
     w.row_start = c.take<int>((size_t)nrow + 1);
     w.keys = c.take<uint64_t>((size_t)T);
     w.alt = c.take<uint64_t>((size_t)T);
+    w.brow = w.part.tiled ? c.take<int>((size_t)T) : nullptr;
+    w.tile_offsets = c.take<int>((size_t)ntile_hist);
     w.large_rows = c.take<int>((size_t)nrow + 1);
     w.counters = c.take<int>(64);
-    w.scan_ws = c.take<char>(scan_workspace_bytes((int64_t)nrow + 1));
+    w.scan_ws = c.take<char>(scan_workspace_bytes(std::max<int64_t>((int64_t)nrow + 1, ntile_hist)));
     w.summed_offsets = own_summed ? c.take<int>((size_t)T) : nullptr;
     w.summed_indices = own_summed ? c.take<int>((size_t)T) : nullptr;
     *needed = c.used;
@@ -130,6 +187,119 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(
     keys[pos] = ((uint64_t)(uint32_t)cols[i] << 32) | (uint32_t)i;
 }
 
+// Tiled classification (one workgroup per tile): validity as classify_kernel,
+// trow[t] = row or -1, and the tile's histogram of coarse buckets counted in
+// LDS, stored bucket-major: tile_hist[b * ntiles + tile].
+template <typename U>
+__global__ __launch_bounds__(kBlock) void tile_classify_kernel(
+    int T, const int* tpl_count, int nrow, int ncol, const int* rows, const int* cols, const U* values,
+    int block_size, U zero_mask, const int* mask_offsets, const int* mask_row_counts, const int* mask_columns,
+    Partition part, int* trow, int* tile_hist
+)
+{
+    __shared__ int hist[kMaxBuckets];
+    for (int b = threadIdx.x; b < part.nb; b += kBlock)
+        hist[b] = 0;
+    __syncthreads();
+    const int limit = tpl_count ? *tpl_count : T;
+    const int64_t t0 = (int64_t)blockIdx.x * part.tile;
+    const int64_t t1 = t0 + part.tile < T ? t0 + part.tile : T;
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += kBlock) {
+        const int r = rows[t];
+        const int c = cols[t];
+        bool valid = t < limit && r >= 0 && r < nrow && c >= 0 && c < ncol;
+        if (valid && mask_offsets)
+            valid = in_mask(r, c, mask_offsets, mask_row_counts, mask_columns);
+        if (valid && values)
+            valid = !block_is_zero(values + (size_t)t * block_size, block_size, zero_mask);
+        trow[t] = valid ? r : -1;
+        if (valid)
+            atomicAdd(&hist[r >> part.shift], 1);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < part.nb; b += kBlock)
+        tile_hist[(size_t)b * part.ntiles + blockIdx.x] = hist[b];
+}
+
+// Tiled scatter into coarse buckets (one workgroup per tile): each valid
+// triplet's key (col << 32 | t) and row go to its bucket's segment of the
+// tile, ranks taken with LDS atomics from the scanned tile histograms.
+__global__ __launch_bounds__(kBlock) void tile_scatter_kernel(
+    int T, const int* __restrict__ trow, const int* __restrict__ cols, Partition part,
+    const int* __restrict__ tile_offsets, uint64_t* __restrict__ bkeys, int* __restrict__ brow
+)
+{
+    __shared__ int cursor[kMaxBuckets];
+    for (int b = threadIdx.x; b < part.nb; b += kBlock)
+        cursor[b] = tile_offsets[(size_t)b * part.ntiles + blockIdx.x];
+    __syncthreads();
+    const int64_t t0 = (int64_t)blockIdx.x * part.tile;
+    const int64_t t1 = t0 + part.tile < T ? t0 + part.tile : T;
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += kBlock) {
+        const int r = trow[t];
+        if (r < 0)
+            continue;
+        const int pos = atomicAdd(&cursor[r >> part.shift], 1);
+        bkeys[pos] = ((uint64_t)(uint32_t)cols[t] << 32) | (uint32_t)t;
+        brow[pos] = r;
+    }
+}
+
+// One workgroup per coarse bucket: the bucket's rows get their key ranges
+// (row_start, counted and scanned in LDS) and their keys, in row order.
+__global__ __launch_bounds__(kBlock) void bucket_rows_kernel(
+    int nrow, Partition part, const int* __restrict__ tile_offsets, const uint64_t* __restrict__ bkeys,
+    const int* __restrict__ brow, int* __restrict__ row_start, uint64_t* __restrict__ keys
+)
+{
+    __shared__ int cnt[kMaxBucketRows];
+    __shared__ int part_sum[kBlock];
+    const int b = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int r0 = b << part.shift;
+    const int nr = min(1 << part.shift, nrow - r0);
+    const int k0 = tile_offsets[(size_t)b * part.ntiles];
+    const int k1 = tile_offsets[(size_t)(b + 1) * part.ntiles];
+    for (int j = tid; j < nr; j += kBlock)
+        cnt[j] = 0;
+    __syncthreads();
+    for (int i = k0 + tid; i < k1; i += kBlock)
+        atomicAdd(&cnt[brow[i] - r0], 1);
+    __syncthreads();
+    // exclusive scan of cnt[0..nr): contiguous chunks per thread, then the chunk sums
+    const int chunk = (nr + kBlock - 1) / kBlock;
+    const int j0 = tid * chunk;
+    const int j1 = min(j0 + chunk, nr);
+    int sum = 0;
+    for (int j = j0; j < j1; ++j)
+        sum += cnt[j];
+    part_sum[tid] = sum;
+    __syncthreads();
+    if (tid == 0) {
+        int run = 0;
+        for (int i = 0; i < kBlock; ++i) {
+            const int v = part_sum[i];
+            part_sum[i] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    int run = k0 + part_sum[tid];
+    for (int j = j0; j < j1; ++j) {
+        const int v = cnt[j];
+        cnt[j] = run;
+        row_start[r0 + j] = run;
+        run += v;
+    }
+    if (b == part.nb - 1 && tid == 0)
+        row_start[nrow] = k1;
+    __syncthreads();
+    for (int i = k0 + tid; i < k1; i += kBlock) {
+        const int pos = atomicAdd(&cnt[brow[i] - r0], 1);
+        keys[pos] = bkeys[i];
+    }
+}
+
 // In-LDS bitonic sort of `n` (power of two) keys by `nthreads` threads.
 __device__ __forceinline__ void lds_bitonic(uint64_t* s, int n, int tid, int nthreads)
 {
@@ -179,41 +349,77 @@ __device__ void emit_sorted_row(
         urow[row] = *lds_count;
 }
 
-__global__ __launch_bounds__(kWave) void sort_small_rows_kernel(
-    int nrow, const int* row_start, uint64_t* keys, int* summed_indices, int* urow, int* large_rows, int* counters
+// 64-bit key exchange with lane ^ mask.
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int mask)
+{
+    const int lo = __shfl_xor((int)(uint32_t)v, mask);
+    const int hi = __shfl_xor((int)(uint32_t)(v >> 32), mask);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+// Bitonic sort of one key per lane across the 64 lanes of a wavefront, in
+// registers (no LDS): ascending by lane.
+__device__ __forceinline__ uint64_t wave_sort64(uint64_t key, int lane)
+{
+#pragma unroll
+    for (int k = 2; k <= kWave; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const uint64_t other = shfl_xor_u64(key, j);
+            const bool up = (lane & k) == 0;
+            const bool lower = (lane & j) == 0;
+            key = (lower == up) ? (key < other ? key : other) : (key < other ? other : key);
+        }
+    return key;
+}
+
+// Rows of at most one key per lane: a row's keys, sorted in registers.
+__device__ __forceinline__ uint64_t load_sorted_row(const uint64_t* keys, int base, int size, int lane)
+{
+    const uint64_t key = lane < size ? keys[base + lane] : kPadKey;
+    return wave_sort64(key, lane);
+}
+
+// Rows are handled one per wavefront, kRowsPerBlock wavefronts per workgroup.
+constexpr int kRowsPerBlock = 4;
+
+// Unique column count of every row of at most kWave keys (sorted in
+// registers; with write_back the sorted keys and triplet indices are stored
+// in place).  Larger rows go to large_rows for the LDS / global-merge sort.
+__global__ __launch_bounds__(kWave * kRowsPerBlock) void count_small_rows_kernel(
+    int nrow, RowRanges rr, uint64_t* __restrict__ keys, int* __restrict__ summed_indices,
+    int write_back, int* __restrict__ urow, int* __restrict__ large_rows, int* __restrict__ counters
 )
 {
-    __shared__ uint64_t s[kSmallCap];
-    __shared__ int count;
-    const int row = blockIdx.x;
-    const int tid = threadIdx.x;
-    const int base = row_start[row];
-    const int size = row_start[row + 1] - base;
-    if (size > kSmallCap) {
-        if (tid == 0) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int row = blockIdx.x * kRowsPerBlock + (int)(threadIdx.x / kWave);
+    if (row >= nrow)
+        return;
+    const int base = rr.base(row);
+    const int size = rr.size(row);
+    if (size > kWave) {
+        if (lane == 0) {
             const int slot = atomicAdd(&counters[0], 1);
             large_rows[slot] = row;
         }
         return;
     }
-    if (size == 0) {
-        if (tid == 0)
-            urow[row] = 0;
-        return;
+    const uint64_t key = load_sorted_row(keys, base, size, lane);
+    const int col = (int)(key >> 32);
+    const int prev = __shfl_up(col, 1);
+    const bool head = lane < size && (lane == 0 || prev != col);
+    const int n = __popcll(__ballot(head));
+    if (lane == 0)
+        urow[row] = n;
+    if (write_back && lane < size) {
+        keys[base + lane] = key;
+        summed_indices[base + lane] = (int)(uint32_t)key;
     }
-    const int n = pow2_at_least(size, 2);
-    for (int k = tid; k < n; k += kWave)
-        s[k] = k < size ? keys[base + k] : kPadKey;
-    if (tid == 0)
-        count = 0;
-    __syncthreads();
-    lds_bitonic(s, n, tid, kWave);
-    emit_sorted_row(s, size, base, row, tid, kWave, keys, summed_indices, urow, &count);
 }
 
 // Rows above kSmallCap: one workgroup each (grid-stride over the row list).
 __global__ __launch_bounds__(kBlock) void sort_large_rows_kernel(
-    const int* row_start, uint64_t* keys, uint64_t* alt, int* summed_indices, int* urow, const int* large_rows,
+    RowRanges rr, uint64_t* keys, uint64_t* alt, int* summed_indices, int* urow, const int* large_rows,
     const int* counters
 )
 {
@@ -223,8 +429,8 @@ __global__ __launch_bounds__(kBlock) void sort_large_rows_kernel(
     const int tid = threadIdx.x;
     for (int li = blockIdx.x; li < nlarge; li += gridDim.x) {
         const int row = large_rows[li];
-        const int base = row_start[row];
-        const int size = row_start[row + 1] - base;
+        const int base = rr.base(row);
+        const int size = rr.size(row);
         if (tid == 0)
             count = 0;
         if (size <= kLargeCap) {
@@ -297,59 +503,150 @@ __global__ __launch_bounds__(kBlock) void sort_large_rows_kernel(
     }
 }
 
-// One wave per row: columns and inclusive run ends of the row's unique blocks.
-__global__ __launch_bounds__(kWave) void finalize_rows_kernel(
-    const int* row_start, const uint64_t* keys, const int* bsr_offsets, int* bsr_columns, int* summed_offsets
+// Rows above kWave keys (sorted in memory by the LDS / merge sort), one wave
+// per listed row: columns and inclusive run ends of the row's unique blocks.
+__global__ __launch_bounds__(kWave) void finalize_large_rows_kernel(
+    RowRanges rr, const uint64_t* __restrict__ keys, const int* __restrict__ bsr_offsets,
+    int* __restrict__ bsr_columns, int* __restrict__ summed_offsets, const int* __restrict__ large_rows,
+    const int* __restrict__ counters
 )
 {
-    const int row = blockIdx.x;
     const int lane = threadIdx.x;
-    const int rs = row_start[row];
-    const int re = row_start[row + 1];
-    const int out0 = bsr_offsets[row];
-    int runs = 0;
-    for (int p0 = rs; p0 < re; p0 += kWave) {
-        const int p = p0 + lane;
-        const bool active = p < re;
-        int col = -1, prev = -1, next = -1;
-        if (active) {
-            col = (int)(keys[p] >> 32);
-            prev = p > rs ? (int)(keys[p - 1] >> 32) : -1;
-            next = p + 1 < re ? (int)(keys[p + 1] >> 32) : -1;
+    const int nlarge = counters[0];
+    for (int li = blockIdx.x; li < nlarge; li += gridDim.x) {
+        const int row = large_rows[li];
+        const int rs = rr.base(row);
+        const int re = rs + rr.size(row);
+        const int out0 = bsr_offsets[row];
+        int runs = 0;
+        for (int p0 = rs; p0 < re; p0 += kWave) {
+            const int p = p0 + lane;
+            const bool active = p < re;
+            int col = -1, prev = -1, next = -1;
+            if (active) {
+                col = (int)(keys[p] >> 32);
+                prev = p > rs ? (int)(keys[p - 1] >> 32) : -1;
+                next = p + 1 < re ? (int)(keys[p + 1] >> 32) : -1;
+            }
+            const bool head = active && col != prev;
+            const bool tail = active && col != next;
+            const uint64_t mask = __ballot(head);
+            const uint64_t below = lane == 63 ? ~uint64_t(0) : ((uint64_t(1) << (lane + 1)) - 1);
+            const int run = runs + __popcll(mask & below) - 1;
+            if (head)
+                bsr_columns[out0 + run] = col;
+            if (tail)
+                summed_offsets[out0 + run] = p + 1;
+            runs += __popcll(mask);
         }
-        const bool head = active && col != prev;
-        const bool tail = active && col != next;
-        const uint64_t mask = __ballot(head);
-        const uint64_t below = lane == 63 ? ~uint64_t(0) : ((uint64_t(1) << (lane + 1)) - 1);
-        const int run = runs + __popcll(mask & below) - 1;
-        if (head)
-            bsr_columns[out0 + run] = col;
-        if (tail)
-            summed_offsets[out0 + run] = p + 1;
-        runs += __popcll(mask);
     }
 }
 
-// Sequential run sums per (unique block, scalar); one wave per row.
+// Sequential run sums per (unique block, scalar) of the listed large rows.
 template <typename S>
-__global__ __launch_bounds__(kWave) void accumulate_values_kernel(
-    const int* bsr_offsets, const int* summed_offsets, const int* summed_indices, const S* tpl_values, int block_size,
-    S* bsr_values
+__global__ __launch_bounds__(kWave) void accumulate_large_rows_kernel(
+    RowRanges rr, const int* __restrict__ bsr_offsets, const int* __restrict__ summed_offsets,
+    const int* __restrict__ summed_indices, const S* __restrict__ tpl_values, int block_size,
+    S* __restrict__ bsr_values, const int* __restrict__ large_rows, const int* __restrict__ counters
 )
 {
-    const int row = blockIdx.x;
-    const int u0 = bsr_offsets[row];
-    const int u1 = bsr_offsets[row + 1];
-    const int64_t tasks = (int64_t)(u1 - u0) * block_size;
-    for (int64_t t = threadIdx.x; t < tasks; t += kWave) {
-        const int u = u0 + (int)(t / block_size);
-        const int s = (int)(t % block_size);
-        const int beg = u == 0 ? 0 : summed_offsets[u - 1];
-        const int end = summed_offsets[u];
-        S acc = tpl_values[(size_t)summed_indices[beg] * block_size + s];
-        for (int k = beg + 1; k < end; ++k)
-            acc += tpl_values[(size_t)summed_indices[k] * block_size + s];
-        bsr_values[(size_t)u * block_size + s] = acc;
+    const int nlarge = counters[0];
+    for (int li = blockIdx.x; li < nlarge; li += gridDim.x) {
+        const int row = large_rows[li];
+        const int u0 = bsr_offsets[row];
+        const int u1 = bsr_offsets[row + 1];
+        const int64_t tasks = (int64_t)(u1 - u0) * block_size;
+        for (int64_t t = threadIdx.x; t < tasks; t += kWave) {
+            const int u = u0 + (int)(t / block_size);
+            const int s = (int)(t % block_size);
+            const int beg = u == u0 ? rr.base(row) : summed_offsets[u - 1];
+            const int end = summed_offsets[u];
+            S acc = tpl_values[(size_t)summed_indices[beg] * block_size + s];
+            for (int k = beg + 1; k < end; ++k)
+                acc += tpl_values[(size_t)summed_indices[k] * block_size + s];
+            bsr_values[(size_t)u * block_size + s] = acc;
+        }
+    }
+}
+
+// Rows of at most kWave keys, one wave per row: the keys are sorted again in
+// registers, then the row's columns, (optionally) its inclusive run ends and
+// sorted triplet indices, and (optionally) its block sums are written.  A sum
+// adds the run's blocks in ascending triplet order, the first one as the
+// initial value; its loads are issued four at a time ahead of the adds.
+template <typename S, int BS>
+__global__ __launch_bounds__(kWave * kRowsPerBlock) void finalize_small_rows_kernel(
+    int nrow, RowRanges rr, const uint64_t* __restrict__ keys,
+    const int* __restrict__ bsr_offsets, int* __restrict__ bsr_columns, int* __restrict__ summed_offsets,
+    int* __restrict__ summed_indices, const S* __restrict__ tpl_values, int block_size, S* __restrict__ bsr_values
+)
+{
+    const int bs = BS > 0 ? BS : block_size;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int row = blockIdx.x * kRowsPerBlock + (int)(threadIdx.x / kWave);
+    if (row >= nrow)
+        return;
+    const int base = rr.base(row);
+    const int size = rr.size(row);
+    if (size > kWave)
+        return;
+    const uint64_t key = load_sorted_row(keys, base, size, lane);
+    const int col = (int)(key >> 32);
+    const int idx = (int)(uint32_t)key;
+    const int prev = __shfl_up(col, 1);
+    const int next = __shfl_down(col, 1);
+    const bool head = lane < size && (lane == 0 || prev != col);
+    const bool tail = lane < size && (lane == size - 1 || next != col);
+    const uint64_t heads = __ballot(head);
+    const uint64_t below = lane == 63 ? ~uint64_t(0) : ((uint64_t(1) << (lane + 1)) - 1);
+    const int run = __popcll(heads & below) - 1;
+    const int out0 = bsr_offsets[row];
+    if (head)
+        bsr_columns[out0 + run] = col;
+    if (summed_offsets && tail)
+        summed_offsets[out0 + run] = base + lane + 1;
+    if (summed_indices && lane < size)
+        summed_indices[base + lane] = idx;
+    if (!bsr_values)
+        return;
+    // Cross-lane reads (ds_bpermute) only in wave-uniform control flow: every
+    // loop below runs the same trip count on all lanes.
+    const int nu = __popcll(heads);
+    const uint64_t above = heads & ~below;
+    int run_len = head ? (above ? __builtin_ctzll(above) : size) - lane : 0;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1)
+        run_len = max(run_len, __shfl_xor(run_len, o));
+    for (int t0 = 0; t0 < nu * bs; t0 += kWave) {
+        const int t = t0 + lane;
+        const bool act = t < nu * bs;
+        const int u = act ? t / bs : 0;
+        const int sc = t - u * bs;
+        uint64_t m = heads;
+        for (int i = 0; i < u; ++i)
+            m &= m - 1;
+        const int beg = m ? __builtin_ctzll(m) : 0;
+        m &= m - 1;
+        const int end = m ? __builtin_ctzll(m) : size;
+        const int i0 = __shfl(idx, beg);
+        S acc = act ? tpl_values[(size_t)i0 * bs + sc] : S(0);
+        for (int k = 1; k < run_len; k += 4) {
+            int iq[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                iq[q] = __shfl(idx, min(beg + k + q, kWave - 1));
+            S v[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (act && beg + k + q < end)
+                    v[q] = tpl_values[(size_t)iq[q] * bs + sc];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (act && beg + k + q < end)
+                    acc += v[q];
+        }
+        if (act)
+            bsr_values[(size_t)(out0 + u) * bs + sc] = acc;
     }
 }
 
@@ -371,22 +668,145 @@ size_t triplets_workspace_bytes(int nrow, int T)
     return needed + 256;
 }
 
+// Sort the keys of every row (rows given by rr) by (col, triplet index) and
+// count its unique columns into urow[0..nrow).  Rows of more than kWave keys
+// are listed in w.large_rows (count in w.counters[0]) and sorted in place in
+// memory, with their sorted triplet indices in summed_indices.  Smaller rows
+// are sorted in registers and, unless write_back, left unsorted in memory
+// (the finalize pass sorts them again).
+int sort_rows(int nrow, RowRanges rr, const TripletWs& w, int* summed_indices, int* urow, bool write_back,
+              hipStream_t stream)
+{
+    WA_TRY(launch_fill(w.counters, 64, 0, stream));
+    if (nrow > 0) {
+        count_small_rows_kernel<<<(unsigned)ceil_div(nrow, kRowsPerBlock), kWave * kRowsPerBlock, 0, stream>>>(
+            nrow, rr, w.keys, summed_indices, write_back ? 1 : 0, urow, w.large_rows, w.counters
+        );
+        WA_LAUNCH_CHECK();
+        const int large_grid = device_cu_count() * 2;
+        sort_large_rows_kernel<<<(unsigned)large_grid, kBlock, 0, stream>>>(
+            rr, w.keys, w.alt, summed_indices, urow, w.large_rows, w.counters
+        );
+        WA_LAUNCH_CHECK();
+    }
+    return kOk;
+}
+
+// Steps 5-7 for rows sorted by sort_rows: exclusive scan of the unique counts
+// (urow, in bsr_offsets) into bsr_offsets, then columns, run ends / sorted
+// indices (when the caller asked for them: own_summed false) and value sums.
+int finalize_rows(int nrow, RowRanges rr, const TripletWs& w, bool own_summed, int* summed_offsets,
+                  int* summed_indices, int* bsr_offsets, int* bsr_columns, const void* tpl_values, int scalar_size,
+                  int block_size, void* bsr_values, int* bsr_nnz_host, hipStream_t stream)
+{
+    const bool sum_values = tpl_values && bsr_values;
+    WA_TRY(exclusive_scan_i32(bsr_offsets, bsr_offsets, nrow, w.scan_ws, stream));
+    if (bsr_nnz_host)
+        WA_HIP(hipMemcpyAsync(bsr_nnz_host, bsr_offsets + nrow, sizeof(int), hipMemcpyDeviceToHost, stream));
+    if (nrow == 0)
+        return kOk;
+    // rows of <= kWave keys: sorted again in registers, finalized and summed in one pass
+    const unsigned grid = (unsigned)ceil_div(nrow, kRowsPerBlock);
+    const int block = kWave * kRowsPerBlock;
+    int* so = own_summed ? nullptr : summed_offsets;
+    int* si = own_summed ? nullptr : summed_indices;
+    if (sum_values && scalar_size == 4 && block_size == 9)
+        finalize_small_rows_kernel<float, 9><<<grid, block, 0, stream>>>(
+            nrow, rr, w.keys, bsr_offsets, bsr_columns, so, si, (const float*)tpl_values, block_size,
+            (float*)bsr_values);
+    else if (sum_values && scalar_size == 4 && block_size == 1)
+        finalize_small_rows_kernel<float, 1><<<grid, block, 0, stream>>>(
+            nrow, rr, w.keys, bsr_offsets, bsr_columns, so, si, (const float*)tpl_values, block_size,
+            (float*)bsr_values);
+    else if (sum_values && scalar_size == 4)
+        finalize_small_rows_kernel<float, 0><<<grid, block, 0, stream>>>(
+            nrow, rr, w.keys, bsr_offsets, bsr_columns, so, si, (const float*)tpl_values, block_size,
+            (float*)bsr_values);
+    else if (sum_values)
+        finalize_small_rows_kernel<double, 0><<<grid, block, 0, stream>>>(
+            nrow, rr, w.keys, bsr_offsets, bsr_columns, so, si, (const double*)tpl_values, block_size,
+            (double*)bsr_values);
+    else
+        finalize_small_rows_kernel<float, 1><<<grid, block, 0, stream>>>(
+            nrow, rr, w.keys, bsr_offsets, bsr_columns, so, si, nullptr, block_size, nullptr);
+    WA_LAUNCH_CHECK();
+    // rows of more than kWave keys (sorted in memory), one wave per listed row
+    const int large_grid = device_cu_count() * 4;
+    finalize_large_rows_kernel<<<(unsigned)large_grid, kWave, 0, stream>>>(
+        rr, w.keys, bsr_offsets, bsr_columns, summed_offsets, w.large_rows, w.counters);
+    WA_LAUNCH_CHECK();
+    if (sum_values) {
+        if (scalar_size == 4)
+            accumulate_large_rows_kernel<float><<<(unsigned)large_grid, kWave, 0, stream>>>(
+                rr, bsr_offsets, summed_offsets, summed_indices, (const float*)tpl_values, block_size,
+                (float*)bsr_values, w.large_rows, w.counters);
+        else
+            accumulate_large_rows_kernel<double><<<(unsigned)large_grid, kWave, 0, stream>>>(
+                rr, bsr_offsets, summed_offsets, summed_indices, (const double*)tpl_values, block_size,
+                (double*)bsr_values, w.large_rows, w.counters);
+        WA_LAUNCH_CHECK();
+    }
+    return kOk;
+}
+
 // Steps 1-4: bucket the valid triplets by row and sort every bucket by
-// (col, triplet index).  Leaves the sorted keys at w.keys[w.row_start[r] ...],
-// the sorted triplet indices in summed_indices and the unique column count of
-// every row in urow[0..nrow).
+// (col, triplet index); the unique column count of every row goes to
+// urow[0..nrow).  Rows of more than kWave keys are listed in w.large_rows
+// (count in w.counters[0]) and left sorted in place at
+// w.keys[w.row_start[r] ...] with their sorted triplet indices in
+// summed_indices.  Smaller rows are sorted in registers and, unless
+// write_back, left unsorted in memory (the finalize pass sorts them again).
 int bucket_sort_rows(
     int block_size, int scalar_size, int nrow, int ncol, int T, const int* tpl_count, const int* rows, const int* cols,
     const void* tpl_values, uint64_t zero_mask, const int* mo, const int* mrc, const int* mc, const TripletWs& w,
-    int* summed_indices, int* urow, hipStream_t stream
+    int* summed_indices, int* urow, bool write_back, hipStream_t stream
 )
 {
     const bool prune = tpl_values && zero_mask != 0;
     if (prune)
         WA_REQUIRE(scalar_size == 1 || scalar_size == 2 || scalar_size == 4 || scalar_size == 8,
                    "bsr_from_triplets: unsupported scalar size %d", scalar_size);
+    if (w.part.tiled) {
+        // coarse buckets through per-tile LDS histograms (no global atomics),
+        // then each bucket's rows in LDS
+        const Partition& pt = w.part;
+        const unsigned tiles = (unsigned)pt.ntiles;
+        switch (prune ? scalar_size : 0) {
+        case 1:
+            tile_classify_kernel<uint8_t><<<tiles, kBlock, 0, stream>>>(
+                T, tpl_count, nrow, ncol, rows, cols, (const uint8_t*)tpl_values, block_size, (uint8_t)zero_mask, mo,
+                mrc, mc, pt, w.trow, w.tile_offsets);
+            break;
+        case 2:
+            tile_classify_kernel<uint16_t><<<tiles, kBlock, 0, stream>>>(
+                T, tpl_count, nrow, ncol, rows, cols, (const uint16_t*)tpl_values, block_size, (uint16_t)zero_mask,
+                mo, mrc, mc, pt, w.trow, w.tile_offsets);
+            break;
+        case 4:
+            tile_classify_kernel<uint32_t><<<tiles, kBlock, 0, stream>>>(
+                T, tpl_count, nrow, ncol, rows, cols, (const uint32_t*)tpl_values, block_size, (uint32_t)zero_mask,
+                mo, mrc, mc, pt, w.trow, w.tile_offsets);
+            break;
+        case 8:
+            tile_classify_kernel<uint64_t><<<tiles, kBlock, 0, stream>>>(
+                T, tpl_count, nrow, ncol, rows, cols, (const uint64_t*)tpl_values, block_size, (uint64_t)zero_mask,
+                mo, mrc, mc, pt, w.trow, w.tile_offsets);
+            break;
+        default:
+            tile_classify_kernel<uint32_t><<<tiles, kBlock, 0, stream>>>(
+                T, tpl_count, nrow, ncol, rows, cols, nullptr, block_size, 0u, mo, mrc, mc, pt, w.trow, w.tile_offsets);
+            break;
+        }
+        WA_LAUNCH_CHECK();
+        WA_TRY(exclusive_scan_i32(w.tile_offsets, w.tile_offsets, (int64_t)pt.nb * pt.ntiles, w.scan_ws, stream));
+        tile_scatter_kernel<<<tiles, kBlock, 0, stream>>>(T, w.trow, cols, pt, w.tile_offsets, w.alt, w.brow);
+        WA_LAUNCH_CHECK();
+        bucket_rows_kernel<<<(unsigned)pt.nb, kBlock, 0, stream>>>(nrow, pt, w.tile_offsets, w.alt, w.brow,
+                                                                   w.row_start, w.keys);
+        WA_LAUNCH_CHECK();
+        return sort_rows(nrow, RowRanges{w.row_start, nullptr}, w, summed_indices, urow, write_back, stream);
+    }
     WA_TRY(launch_fill(w.hist, (int64_t)nrow + 1, 0, stream));
-    WA_TRY(launch_fill(w.counters, 64, 0, stream));
     if (T > 0) {
         const unsigned grid = (unsigned)ceil_div(T, kBlock);
         switch (prune ? scalar_size : 0) {
@@ -429,18 +849,7 @@ int bucket_sort_rows(
         );
         WA_LAUNCH_CHECK();
     }
-    if (nrow > 0) {
-        sort_small_rows_kernel<<<(unsigned)nrow, kWave, 0, stream>>>(
-            nrow, w.row_start, w.keys, summed_indices, urow, w.large_rows, w.counters
-        );
-        WA_LAUNCH_CHECK();
-        const int large_grid = device_cu_count() * 2;
-        sort_large_rows_kernel<<<(unsigned)large_grid, kBlock, 0, stream>>>(
-            w.row_start, w.keys, w.alt, summed_indices, urow, w.large_rows, w.counters
-        );
-        WA_LAUNCH_CHECK();
-    }
-    return kOk;
+    return sort_rows(nrow, RowRanges{w.row_start, nullptr}, w, summed_indices, urow, write_back, stream);
 }
 
 // Triplets -> compact BSR.  `mo`, `mrc`, `mc` (offsets, row counts, columns of
@@ -470,32 +879,12 @@ int bsr_from_triplets(
     if (sum_values)
         WA_REQUIRE(scalar_size == 4 || scalar_size == 8, "bsr_from_triplets: value accumulation needs fp32 or fp64");
 
-    // unique counts land in bsr_offsets[0..nrow), scanned in place below
+    // unique counts land in bsr_offsets[0..nrow), scanned in place by finalize_rows
     WA_TRY(bucket_sort_rows(block_size, scalar_size, nrow, ncol, T, tpl_count, rows, cols, tpl_values, zero_mask, mo,
-                            mrc, mc, w, summed_indices, bsr_offsets, stream));
-    WA_TRY(exclusive_scan_i32(bsr_offsets, bsr_offsets, nrow, w.scan_ws, stream));
-    if (bsr_nnz_host)
-        WA_HIP(hipMemcpyAsync(bsr_nnz_host, bsr_offsets + nrow, sizeof(int), hipMemcpyDeviceToHost, stream));
-    if (nrow > 0) {
-        finalize_rows_kernel<<<(unsigned)nrow, kWave, 0, stream>>>(
-            w.row_start, w.keys, bsr_offsets, bsr_columns, summed_offsets
-        );
-        WA_LAUNCH_CHECK();
-        if (sum_values) {
-            if (scalar_size == 4)
-                accumulate_values_kernel<float><<<(unsigned)nrow, kWave, 0, stream>>>(
-                    bsr_offsets, summed_offsets, summed_indices, (const float*)tpl_values, block_size,
-                    (float*)bsr_values
-                );
-            else
-                accumulate_values_kernel<double><<<(unsigned)nrow, kWave, 0, stream>>>(
-                    bsr_offsets, summed_offsets, summed_indices, (const double*)tpl_values, block_size,
-                    (double*)bsr_values
-                );
-            WA_LAUNCH_CHECK();
-        }
-    }
-    return kOk;
+                            mrc, mc, w, summed_indices, bsr_offsets, false, stream));
+    return finalize_rows(nrow, RowRanges{w.row_start, nullptr}, w, own_summed, summed_offsets, summed_indices,
+                         bsr_offsets, bsr_columns, sum_values ? tpl_values : nullptr, scalar_size, block_size,
+                         sum_values ? bsr_values : nullptr, bsr_nnz_host, stream);
 }
 
 
@@ -526,6 +915,60 @@ __global__ __launch_bounds__(kWave) void active_rows_kernel(int nrow, const int*
     end = end < T ? end : T;
     for (int k = beg + (int)threadIdx.x; k < end; k += kWave)
         rows[k] = r;
+}
+
+// The active blocks of row r become keys (column << 32 | position) at
+// keys[offsets[r] ...] in storage order (ballot compaction, a wave per row),
+// rcount[r] their number.  Dropped, as by the triplet classification:
+// columns outside [0, ncol) and, with values, all-zero blocks.  Positions
+// beyond T or the row's capacity are never active.
+template <typename U>
+__global__ __launch_bounds__(kWave * kRowsPerBlock) void compress_keys_kernel(
+    int nrow, int ncol, int T, const int* __restrict__ offsets, const int* __restrict__ row_counts,
+    const int* __restrict__ columns, const U* __restrict__ values, int block_size, U zero_mask,
+    uint64_t* __restrict__ keys, int* __restrict__ rcount)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int row = blockIdx.x * kRowsPerBlock + (int)(threadIdx.x / kWave);
+    if (row >= nrow)
+        return;
+    const int beg = offsets[row];
+    const int cap_end = offsets[row + 1];
+    int end = row_counts ? beg + row_counts[row] : cap_end;
+    end = end < cap_end ? end : cap_end;
+    end = end < T ? end : T;
+    const uint64_t below = (uint64_t(1) << lane) - 1;
+    int count = 0;
+    for (int p0 = beg; p0 < end; p0 += kWave) {
+        const int p = p0 + lane;
+        bool valid = p < end;
+        const int c = valid ? columns[p] : -1;
+        valid = valid && c >= 0 && c < ncol;
+        if (valid && values)
+            valid = !block_is_zero(values + (size_t)p * block_size, block_size, zero_mask);
+        const uint64_t mask = __ballot(valid);
+        if (valid)
+            keys[beg + count + __popcll(mask & below)] = ((uint64_t)(uint32_t)c << 32) | (uint32_t)p;
+        count += __popcll(mask);
+    }
+    if (lane == 0)
+        rcount[row] = count;
+}
+
+// Compact result written back over the matrix: columns / values of the
+// nnz = c_offsets[nrow] coalesced blocks, then column -1 and zero values up to
+// position T (values moved as 32-bit words).
+__global__ __launch_bounds__(kBlock) void compact_copy_kernel(int nrow, const int* __restrict__ c_offsets, int T,
+                                                             int block_words, const int* __restrict__ c_columns,
+                                                             const uint32_t* __restrict__ c_values,
+                                                             int* __restrict__ columns, uint32_t* __restrict__ values)
+{
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int nnz = c_offsets[nrow];
+    if (t < T)
+        columns[t] = t < nnz ? c_columns[t] : -1;
+    if (values && t < (int64_t)T * block_words)
+        values[t] = t < (int64_t)nnz * block_words ? c_values[t] : 0u;
 }
 
 // Padded result: row r's compact blocks go to the start of its capacity
@@ -602,20 +1045,6 @@ __global__ __launch_bounds__(kWave) void masked_scatter_kernel(int nrow, const i
     }
 }
 
-// Compact result: positions at or beyond nnz = offsets[nrow] get column -1 and zero values.
-__global__ __launch_bounds__(kBlock) void clear_tail_kernel(int nrow, const int* __restrict__ offsets, int T,
-                                                           int block_words, int* __restrict__ columns,
-                                                           uint32_t* __restrict__ values)
-{
-    const int nnz = offsets[nrow];
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= T || i < nnz)
-        return;
-    columns[i] = -1;
-    if (values)
-        for (int b = 0; b < block_words; ++b)
-            values[i * block_words + b] = 0u;
-}
 
 // ---------------------------------------------------------------------------
 // Transpose (reference wp_bsr_transpose_device, warp/native/sparse.cu:1400-1492,
@@ -802,7 +1231,7 @@ int bsr_transpose(int nrow, int ncol, int T, const int* offsets, const int* row_
     WA_TRY(mark_active_rows(nrow, offsets, row_counts, T, w.rows, s));
     // triplet i = (row columns[i], column rows[i]); inactive positions have column -1 and are dropped
     WA_TRY(bucket_sort_rows(1, 0, ncol, nrow, T, nullptr, columns, w.rows, nullptr, 0, nullptr, nullptr, nullptr, tw,
-                            tw.summed_indices, w.urow, s));
+                            tw.summed_indices, w.urow, true, s));
     if (t_row_counts == nullptr) {
         WA_HIP(hipMemcpyAsync(t_offsets, tw.row_start, sizeof(int) * ((size_t)ncol + 1), hipMemcpyDeviceToDevice, s));
         if (T > 0) {
@@ -935,20 +1364,34 @@ WA_API int warp_amd_bsr_compress(int block_size, int scalar_size, int nrow, int 
             *nnz_host = 0;
         return wa::kOk;
     }
-    WA_TRY(wa::mark_active_rows(nrow, offsets, row_counts, T, w.rows, s));
-    WA_TRY(wa::bsr_from_triplets(block_size, scalar_size, nrow, ncol, T, nullptr, w.rows, columns, values,
-                                 prune_numerical_zeros && values ? zero_mask : 0, nullptr, nullptr, nullptr, nullptr,
-                                 nullptr, w.c_offsets, w.c_columns, values ? (void*)w.c_values : nullptr,
-                                 make_compact ? nnz_host : nullptr, w.tws, w.tws_bytes, s));
+    // the candidates are already grouped by row: keys are built in place per row,
+    // with no row bucketing, then sorted, counted and summed by the assembly core
+    size_t tneed = 0;
+    wa::TripletWs tw = wa::carve_ws(w.tws, w.tws_bytes, nrow, T, true, &tneed);
+    int* rcount = w.urow;
+    const bool prune = prune_numerical_zeros && values && zero_mask != 0;
+    const unsigned grid = (unsigned)wa::ceil_div(nrow, wa::kRowsPerBlock);
+    const int block = wa::kWave * wa::kRowsPerBlock;
+    if (prune && scalar_size == 8)
+        wa::compress_keys_kernel<uint64_t><<<grid, block, 0, s>>>(nrow, ncol, T, offsets, row_counts, columns,
+                                                                  static_cast<const uint64_t*>(values), block_size,
+                                                                  zero_mask, tw.keys, rcount);
+    else
+        wa::compress_keys_kernel<uint32_t><<<grid, block, 0, s>>>(
+            nrow, ncol, T, offsets, row_counts, columns, prune ? static_cast<const uint32_t*>(values) : nullptr,
+            block_size, (uint32_t)zero_mask, tw.keys, rcount);
+    WA_LAUNCH_CHECK();
+    const wa::RowRanges rr{offsets, rcount};
+    WA_TRY(wa::sort_rows(nrow, rr, tw, tw.summed_indices, w.c_offsets, false, s));
+    WA_TRY(wa::finalize_rows(nrow, rr, tw, true, tw.summed_offsets, tw.summed_indices, w.c_offsets, w.c_columns,
+                             values, scalar_size, block_size, values ? (void*)w.c_values : nullptr,
+                             make_compact ? nnz_host : nullptr, s));
     if (make_compact) {
         WA_HIP(hipMemcpyAsync(offsets, w.c_offsets, sizeof(int) * ((size_t)nrow + 1), hipMemcpyDeviceToDevice, s));
         if (T > 0) {
-            WA_HIP(hipMemcpyAsync(columns, w.c_columns, sizeof(int) * (size_t)T, hipMemcpyDeviceToDevice, s));
-            if (values)
-                WA_HIP(hipMemcpyAsync(values, w.c_values, (size_t)T * block_bytes, hipMemcpyDeviceToDevice, s));
-            wa::clear_tail_kernel<<<(unsigned)wa::ceil_div(T, wa::kBlock), wa::kBlock, 0, s>>>(
-                nrow, offsets, T, block_words, columns, static_cast<uint32_t*>(values)
-            );
+            const int64_t n = (int64_t)T * (block_words > 1 ? block_words : 1);
+            wa::compact_copy_kernel<<<(unsigned)wa::ceil_div(n, wa::kBlock), wa::kBlock, 0, s>>>(
+                nrow, w.c_offsets, T, block_words, w.c_columns, w.c_values, columns, static_cast<uint32_t*>(values));
             WA_LAUNCH_CHECK();
         }
     } else {
