@@ -127,6 +127,31 @@ enum OutFlags { kOutRelu = 1, kOutPool = 2, kOutF32 = 4, kOutAccumulate = 8 };
 // w >> 1) in x, y).  Its 10 x 10 x 6 halo is staged once in LDS (the
 // upsample + skip add applied there, once per voxel), then every wave reads
 // its B operands from LDS for all 27 taps.
+// Persistent tile loops: workgroup b is dispatched to XCD b % kXcds (the
+// MI355X command processor deals workgroups to its 8 XCDs round-robin), so
+// each XCD walks its own contiguous chunk of tiles: tiles that share halo
+// voxels then run on the same XCD and meet in its L2.  Any other mapping
+// only costs locality, never correctness.
+constexpr int kXcds = 8;
+
+struct TileWalk {
+    int t, end, step;
+    __device__ __forceinline__ explicit TileWalk(int ntiles)
+    {
+        if (gridDim.x >= kXcds && gridDim.x % kXcds == 0) {
+            const int xcd = blockIdx.x % kXcds;
+            const int chunk = (ntiles + kXcds - 1) / kXcds;
+            t = xcd * chunk + blockIdx.x / kXcds;
+            end = min(ntiles, (xcd + 1) * chunk);
+            step = gridDim.x / kXcds;
+        } else {
+            t = blockIdx.x;
+            end = ntiles;
+            step = gridDim.x;
+        }
+    }
+};
+
 constexpr int kTX = 8, kTY = 8, kTZ = 4;
 constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2;
 constexpr int kHalo = kHX * kHY * kHZ;
@@ -232,7 +257,8 @@ __global__ __launch_bounds__(kBlock) void conv3_mfma_kernel(
     const int mh = m / 2;
     // persistent loop over tiles; the channel split varies fastest so that
     // concurrent workgroups share one input tile in L2
-    for (int tile_id = blockIdx.x; tile_id < ntiles; tile_id += gridDim.x) {
+    for (TileWalk walk(ntiles); walk.t < walk.end; walk.t += walk.step) {
+    const int tile_id = walk.t;
     const int cb0 = (tile_id % nsplit) * CBG;
     const int t2 = tile_id / nsplit;
     const int X0 = (t2 % tiles_x) * kTX;
@@ -410,7 +436,8 @@ __global__ __launch_bounds__(kBlock) void conv3_co1_kernel(int m, const unsigned
     __shared__ unsigned tile[kHalo * SW];
     const int tiles_x = (m + kTX - 1) / kTX;
     const int ntiles = tiles_x * tiles_x * (m / kTZ);
-    for (int tile_id = blockIdx.x; tile_id < ntiles; tile_id += gridDim.x) {
+    for (TileWalk walk(ntiles); walk.t < walk.end; walk.t += walk.step) {
+    const int tile_id = walk.t;
     const int X0 = (tile_id % tiles_x) * kTX;
     const int Y0 = ((tile_id / tiles_x) % tiles_x) * kTY;
     const int Z0 = (tile_id / (tiles_x * tiles_x)) * kTZ;
