@@ -7,6 +7,8 @@
 // built by the triplet assembly (one triplet per destination sub-block).
 #include "common.h"
 
+#include <climits>
+
 namespace wa {
 namespace {
 
@@ -51,80 +53,123 @@ __device__ __forceinline__ int find_block(const int* offsets, const int* row_cou
     return lo < end && columns[lo] == col ? lo : -1;
 }
 
+// Active end of row r, clipped to its capacity.
+__device__ __forceinline__ int row_end(const int* offsets, const int* row_counts, int r)
+{
+    const int cap_end = offsets[r + 1];
+    if (!row_counts)
+        return cap_end;
+    const int e = offsets[r] + row_counts[r];
+    return e < cap_end ? e : cap_end;
+}
+
+// Row-parallel launches: one wavefront per row, kRowsPerBlock rows per
+// workgroup; a row's storage positions are walked directly (no search for the
+// row of a position).
+constexpr int kRowsPerBlock = 4;
+
+__device__ __forceinline__ int wave_row() { return blockIdx.x * kRowsPerBlock + (int)(threadIdx.x / kWave); }
+
 struct Reblock {
     int src_subrows, src_subcols, dest_subrows, dest_subcols;
 };
 
-// One thread per (source position, sub-row, sub-column): the destination block
-// coordinates of that sub-block, or (-1, -1).
-__global__ __launch_bounds__(kBlock) void assign_list_blocks_kernel(int T, Reblock rb, int nrow,
-                                                                   const int* __restrict__ offsets,
-                                                                   const int* __restrict__ row_counts,
-                                                                   const int* __restrict__ columns,
-                                                                   int* __restrict__ dest_rows,
-                                                                   int* __restrict__ dest_cols)
+// One wavefront per source row, a lane per (position, sub-row, sub-column):
+// the destination block coordinates of that sub-block at entry
+// (p * src_subcols + subcol) * src_subrows + subrow.  Entries of inactive
+// positions were set to -1 beforehand.
+__global__ __launch_bounds__(kWave * kRowsPerBlock) void assign_list_blocks_kernel(int T, Reblock rb, int nrow,
+                                                                                  const int* __restrict__ offsets,
+                                                                                  const int* __restrict__ row_counts,
+                                                                                  const int* __restrict__ columns,
+                                                                                  int* __restrict__ dest_rows,
+                                                                                  int* __restrict__ dest_cols)
 {
+    const int row = wave_row();
+    if (row >= nrow)
+        return;
     const int per = rb.src_subrows * rb.src_subcols;
-    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= (int64_t)T * per)
-        return;
-    const int p = (int)(t / per);
-    const int rem = (int)(t - (int64_t)p * per);
-    const int subrow = rem % rb.src_subrows;
-    const int subcol = rem / rb.src_subrows;
-    const int row = row_of_position(offsets, row_counts, nrow, p);
-    const int col = row < 0 ? -1 : columns[p];
-    if (row < 0 || col < 0) {
-        dest_rows[t] = -1;
-        dest_cols[t] = -1;
-        return;
+    const int beg = offsets[row];
+    const int end = min(row_end(offsets, row_counts, row), T);
+    for (int t = (int)(threadIdx.x % kWave); t < (end - beg) * per; t += kWave) {
+        const int p = beg + t / per;
+        const int rem = t % per;
+        const int subcol = rem % rb.src_subcols;
+        const int subrow = rem / rb.src_subcols;
+        const int col = columns[p];
+        if (col < 0)
+            continue;
+        const int64_t o = ((int64_t)p * rb.src_subcols + subcol) * rb.src_subrows + subrow;
+        dest_rows[o] = (row * rb.src_subrows + subrow) / rb.dest_subrows;
+        dest_cols[o] = (col * rb.src_subcols + subcol) / rb.dest_subcols;
     }
-    dest_rows[t] = (row * rb.src_subrows + subrow) / rb.dest_subrows;
-    dest_cols[t] = (col * rb.src_subcols + subcol) / rb.dest_subcols;
 }
 
-// One thread per (source position, sub-row, sub-column): copies the sub-block
-// into its destination block, converted to the destination scalar type after
-// the scale is applied in the source type.
+// One wavefront per source row, a lane per (position, sub-row, sub-column):
+// copies the sub-block into its destination block, converted to the
+// destination scalar type after the scale is applied in the source type.
 template <typename SS, typename DS>
-__global__ __launch_bounds__(kBlock) void assign_copy_blocks_kernel(
+__global__ __launch_bounds__(kWave * kRowsPerBlock) void assign_copy_blocks_kernel(
     int T, Reblock rb, int nrow, const int* __restrict__ offsets, const int* __restrict__ row_counts,
     const int* __restrict__ columns, int src_br, int src_bc, const SS* __restrict__ src, SS scale,
     const int* __restrict__ d_offsets, const int* __restrict__ d_row_counts, const int* __restrict__ d_columns,
     int dst_br, int dst_bc, DS* __restrict__ dst)
 {
+    const int row = wave_row();
+    if (row >= nrow)
+        return;
     const int per = rb.src_subrows * rb.src_subcols;
-    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= (int64_t)T * per)
+    const int beg = offsets[row];
+    const int end = min(row_end(offsets, row_counts, row), T);
+    if (per == 1 && rb.dest_subrows == 1 && rb.dest_subcols == 1) {
+        // same block shape: lanes over (block, scalar), scalars fastest, so
+        // loads and stores are contiguous; the destination block usually sits
+        // at the same rank in its row (copies of a sorted topology)
+        const int bs = src_br * src_bc;
+        const int dbeg = d_offsets[row];
+        const int dend = row_end(d_offsets, d_row_counts, row);
+        for (int t = (int)(threadIdx.x % kWave); t < (end - beg) * bs; t += kWave) {
+            const int k = t / bs;
+            const int e = t - k * bs;
+            const int p = beg + k;
+            const int col = columns[p];
+            if (col < 0)
+                continue;
+            int b = dbeg + k;
+            if (!(b < dend && d_columns[b] == col))
+                b = find_block(d_offsets, d_row_counts, d_columns, row, col);
+            if (b >= 0)
+                dst[(size_t)b * bs + e] = (DS)(scale * src[(size_t)p * bs + e]);
+        }
         return;
-    const int p = (int)(t / per);
-    const int rem = (int)(t - (int64_t)p * per);
-    const int subrow = rem % rb.src_subrows;
-    const int subcol = rem / rb.src_subrows;
-    const int row = row_of_position(offsets, row_counts, nrow, p);
-    if (row < 0)
-        return;
-    const int col = columns[p];
-    if (col < 0)
-        return;
-    const int dsr = row * rb.src_subrows + subrow;
-    const int dsc = col * rb.src_subcols + subcol;
-    const int drow = dsr / rb.dest_subrows;
-    const int dcol = dsc / rb.dest_subcols;
-    const int b = find_block(d_offsets, d_row_counts, d_columns, drow, dcol);
-    if (b < 0)
-        return;
-    const int rps = src_br / rb.src_subrows;
-    const int cps = src_bc / rb.src_subcols;
-    const int di0 = (dsr - rb.dest_subrows * drow) * rps;
-    const int dj0 = (dsc - rb.dest_subcols * dcol) * cps;
-    const int si0 = subrow * rps;
-    const int sj0 = subcol * cps;
-    const SS* s = src + (size_t)p * src_br * src_bc;
-    DS* d = dst + (size_t)b * dst_br * dst_bc;
-    for (int i = 0; i < rps; ++i)
-        for (int j = 0; j < cps; ++j)
-            d[(di0 + i) * dst_bc + dj0 + j] = (DS)(scale * s[(si0 + i) * src_bc + sj0 + j]);
+    }
+    for (int t = (int)(threadIdx.x % kWave); t < (end - beg) * per; t += kWave) {
+        const int p = beg + t / per;
+        const int rem = t % per;
+        const int subcol = rem % rb.src_subcols;
+        const int subrow = rem / rb.src_subcols;
+        const int col = columns[p];
+        if (col < 0)
+            continue;
+        const int dsr = row * rb.src_subrows + subrow;
+        const int dsc = col * rb.src_subcols + subcol;
+        const int drow = dsr / rb.dest_subrows;
+        const int dcol = dsc / rb.dest_subcols;
+        const int b = find_block(d_offsets, d_row_counts, d_columns, drow, dcol);
+        if (b < 0)
+            continue;
+        const int rps = src_br / rb.src_subrows;
+        const int cps = src_bc / rb.src_subcols;
+        const int di0 = (dsr - rb.dest_subrows * drow) * rps;
+        const int dj0 = (dsc - rb.dest_subcols * dcol) * cps;
+        const int si0 = subrow * rps;
+        const int sj0 = subcol * cps;
+        const SS* s = src + (size_t)p * src_br * src_bc;
+        DS* d = dst + (size_t)b * dst_br * dst_bc;
+        for (int i = 0; i < rps; ++i)
+            for (int j = 0; j < cps; ++j)
+                d[(di0 + i) * dst_bc + dj0 + j] = (DS)(scale * s[(si0 + i) * src_bc + sj0 + j]);
+    }
 }
 
 // Padded same-shape assignment, one wave per row: the row's active source
@@ -163,50 +208,53 @@ __global__ __launch_bounds__(kWave) void assign_padded_kernel(
             dst[(size_t)dbeg * block_size + e] = (DS)(scale * src[(size_t)beg * block_size + e]);
 }
 
-// Active end of row r, clipped to its capacity.
-__device__ __forceinline__ int row_end(const int* offsets, const int* row_counts, int r)
+// rows[p] = r for the active positions p < T of row r (one wavefront per row;
+// the other positions were set to -1 beforehand).
+__global__ __launch_bounds__(kWave * kRowsPerBlock) void uncompress_rows_kernel(int nrow,
+                                                                               const int* __restrict__ offsets,
+                                                                               const int* __restrict__ row_counts,
+                                                                               int T, int* __restrict__ rows)
 {
-    const int cap_end = offsets[r + 1];
-    if (!row_counts)
-        return cap_end;
-    const int e = offsets[r] + row_counts[r];
-    return e < cap_end ? e : cap_end;
+    const int row = wave_row();
+    if (row >= nrow)
+        return;
+    const int end = min(row_end(offsets, row_counts, row), T);
+    for (int p = offsets[row] + (int)(threadIdx.x % kWave); p < end; p += kWave)
+        rows[p] = row;
 }
 
-// rows[p] = row of storage position p, -1 for inactive positions.
-__global__ __launch_bounds__(kBlock) void uncompress_rows_kernel(int nrow, const int* __restrict__ offsets,
-                                                                const int* __restrict__ row_counts, int T,
-                                                                int* __restrict__ rows)
-{
-    const int p = blockIdx.x * kBlock + threadIdx.x;
-    if (p < T)
-        rows[p] = row_of_position(offsets, row_counts, nrow, p);
-}
-
-// dst[block (rows[i], cols[i])] += scale * src[i], one thread per scalar;
-// blocks absent from the destination are skipped (reference
-// _bsr_axpy_add_block).  Every destination block must receive at most one
-// source block per launch.
-template <typename S>
-__global__ __launch_bounds__(kBlock) void add_blocks_kernel(int64_t n, int bs, const int* __restrict__ rows,
+// dst[block (rows[i], cols[i])] += scale * src[i], one thread per source
+// block (one search, then the block's scalars); blocks absent from the
+// destination are skipped (reference _bsr_axpy_add_block).  Every destination
+// block must receive at most one source block per launch.
+template <typename S, int BS>
+__global__ __launch_bounds__(kBlock) void add_blocks_kernel(int n, int block_size, const int* __restrict__ rows,
                                                            const int* __restrict__ cols, const S* __restrict__ src,
                                                            S scale, const int* __restrict__ d_offsets,
                                                            const int* __restrict__ d_row_counts,
                                                            const int* __restrict__ d_columns, S* __restrict__ dst)
 {
 #pragma clang fp contract(off)
-    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= n * bs)
+    const int bs = BS > 0 ? BS : block_size;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n)
         return;
-    const int64_t i = t / bs;
-    const int e = (int)(t - i * bs);
     const int r = rows[i];
     if (r < 0)
         return;
     const int b = find_block(d_offsets, d_row_counts, d_columns, r, cols[i]);
     if (b < 0)
         return;
-    dst[(size_t)b * bs + e] += scale * src[(size_t)i * bs + e];
+    const S* s = src + (size_t)i * bs;
+    S* d = dst + (size_t)b * bs;
+    if constexpr (BS > 0) {
+#pragma unroll
+        for (int e = 0; e < BS; ++e)
+            d[e] += scale * s[e];
+    } else {
+        for (int e = 0; e < bs; ++e)
+            d[e] += scale * s[e];
+    }
 }
 
 // y := alpha x + beta y written into y's own row capacity, one wave per row
@@ -605,7 +653,7 @@ void launch_copy(unsigned grid, hipStream_t s, int T, Reblock rb, int nrow, cons
                  const int* columns, int src_br, int src_bc, const void* src, double scale, const int* d_offsets,
                  const int* d_row_counts, const int* d_columns, int dst_br, int dst_bc, void* dst)
 {
-    assign_copy_blocks_kernel<SS, DS><<<grid, kBlock, 0, s>>>(
+    assign_copy_blocks_kernel<SS, DS><<<grid, kWave * kRowsPerBlock, 0, s>>>(
         T, rb, nrow, offsets, row_counts, columns, src_br, src_bc, static_cast<const SS*>(src), (SS)scale, d_offsets,
         d_row_counts, d_columns, dst_br, dst_bc, static_cast<DS*>(dst));
 }
@@ -629,9 +677,14 @@ WA_API int warp_amd_bsr_assign_list_blocks(int T, int src_subrows, int src_subco
     if (n == 0)
         return wa::kOk;
     WA_REQUIRE(offsets && columns && dest_rows && dest_cols, "bsr_assign: null arrays");
-    wa::assign_list_blocks_kernel<<<(unsigned)wa::ceil_div(n, wa::kBlock), wa::kBlock, 0, wa::as_stream(stream)>>>(
-        T, wa::Reblock{src_subrows, src_subcols, dest_subrows, dest_subcols}, nrow, offsets, row_counts, columns,
-        dest_rows, dest_cols);
+    hipStream_t s = wa::as_stream(stream);
+    WA_TRY(wa::launch_fill_i32(dest_rows, n, -1, s));
+    WA_TRY(wa::launch_fill_i32(dest_cols, n, -1, s));
+    if (nrow == 0)
+        return wa::kOk;
+    wa::assign_list_blocks_kernel<<<(unsigned)wa::ceil_div(nrow, wa::kRowsPerBlock), wa::kWave * wa::kRowsPerBlock, 0,
+                                    s>>>(T, wa::Reblock{src_subrows, src_subcols, dest_subrows, dest_subcols}, nrow,
+                                         offsets, row_counts, columns, dest_rows, dest_cols);
     WA_LAUNCH_CHECK();
     return wa::kOk;
 }
@@ -659,7 +712,9 @@ WA_API int warp_amd_bsr_assign_copy_blocks(int src_scalar_size, int dest_scalar_
     if (n == 0)
         return wa::kOk;
     WA_REQUIRE(offsets && columns && src_values && dest_offsets && dest_values, "bsr_assign: null arrays");
-    const unsigned grid = (unsigned)wa::ceil_div(n, wa::kBlock);
+    if (nrow == 0)
+        return wa::kOk;
+    const unsigned grid = (unsigned)wa::ceil_div(nrow, wa::kRowsPerBlock);
     const wa::Reblock rb{src_subrows, src_subcols, dest_subrows, dest_subcols};
     hipStream_t s = wa::as_stream(stream);
 #define WA_COPY(SS, DS)                                                                                               \
@@ -748,8 +803,12 @@ WA_API int warp_amd_bsr_uncompress_rows(int nrow, const int* offsets, const int*
     if (T == 0)
         return wa::kOk;
     WA_REQUIRE(offsets && rows, "uncompress_rows: null arrays");
-    wa::uncompress_rows_kernel<<<(unsigned)wa::ceil_div(T, wa::kBlock), wa::kBlock, 0, wa::as_stream(stream)>>>(
-        nrow, offsets, row_counts, T, rows);
+    hipStream_t s = wa::as_stream(stream);
+    WA_TRY(wa::launch_fill_i32(rows, T, -1, s));
+    if (nrow == 0)
+        return wa::kOk;
+    wa::uncompress_rows_kernel<<<(unsigned)wa::ceil_div(nrow, wa::kRowsPerBlock), wa::kWave * wa::kRowsPerBlock, 0,
+                                 s>>>(nrow, offsets, row_counts, T, rows);
     WA_LAUNCH_CHECK();
     return wa::kOk;
 }
@@ -763,21 +822,33 @@ WA_API int warp_amd_bsr_add_blocks(int scalar_size, int block_size, int64_t n, c
                                    const int* dest_row_counts, const int* dest_columns, void* dest_values,
                                    void* stream)
 {
-    WA_REQUIRE(n >= 0 && block_size > 0, "bsr_add_blocks: invalid sizes");
+    WA_REQUIRE(n >= 0 && n <= INT_MAX && block_size > 0, "bsr_add_blocks: invalid sizes");
     WA_REQUIRE(scalar_size == 4 || scalar_size == 8, "bsr_add_blocks: values must be fp32 or fp64");
     if (n == 0)
         return wa::kOk;
     WA_REQUIRE(rows && cols && src && dest_offsets && dest_values, "bsr_add_blocks: null arrays");
-    const unsigned grid = (unsigned)wa::ceil_div(n * block_size, wa::kBlock);
+    const unsigned grid = (unsigned)wa::ceil_div(n, wa::kBlock);
     hipStream_t s = wa::as_stream(stream);
-    if (scalar_size == 4)
-        wa::add_blocks_kernel<float><<<grid, wa::kBlock, 0, s>>>(
-            n, block_size, rows, cols, static_cast<const float*>(src), (float)scale, dest_offsets, dest_row_counts,
-            dest_columns, static_cast<float*>(dest_values));
-    else
-        wa::add_blocks_kernel<double><<<grid, wa::kBlock, 0, s>>>(
-            n, block_size, rows, cols, static_cast<const double*>(src), scale, dest_offsets, dest_row_counts,
-            dest_columns, static_cast<double*>(dest_values));
+#define WA_ADD(S, BS)                                                                                                 \
+    wa::add_blocks_kernel<S, BS><<<grid, wa::kBlock, 0, s>>>((int)n, block_size, rows, cols,                          \
+                                                            static_cast<const S*>(src), (S)scale, dest_offsets,      \
+                                                            dest_row_counts, dest_columns, static_cast<S*>(dest_values))
+    if (scalar_size == 4) {
+        if (block_size == 9)
+            WA_ADD(float, 9);
+        else if (block_size == 1)
+            WA_ADD(float, 1);
+        else
+            WA_ADD(float, 0);
+    } else {
+        if (block_size == 9)
+            WA_ADD(double, 9);
+        else if (block_size == 1)
+            WA_ADD(double, 1);
+        else
+            WA_ADD(double, 0);
+    }
+#undef WA_ADD
     WA_LAUNCH_CHECK();
     return wa::kOk;
 }
@@ -888,12 +959,12 @@ WA_API int warp_amd_bsr_mm_values(int scalar_size, int br, int bk, int bc, doubl
     const unsigned grid = (unsigned)wa::ceil_div(T, wa::kBlock);
     hipStream_t s = wa::as_stream(stream);
 #define WA_MM(S, BR, BK, BC)                                                                                          \
-    wa::mm_values_kernel<S, BR, BK, BC><<<grid, wa::kBlock, 0, s>>>(                                                  \
+    wa::mm_values_kernel<S, BR, BK, BC><<<grid, wa::kBlock, 0, s>>>(                               \
         T, z_nrow, (S)alpha, z_offsets, z_row_counts, z_columns, static_cast<S*>(z_values), y_nrow, x_offsets,       \
         x_row_counts, x_columns, static_cast<const S*>(x_values), y_offsets, y_row_counts, y_columns,                 \
         static_cast<const S*>(y_values))
 #define WA_MM_ANY(S)                                                                                                  \
-    wa::mm_values_any_kernel<S><<<grid, wa::kBlock, 0, s>>>(                                                          \
+    wa::mm_values_any_kernel<S><<<grid, wa::kBlock, 0, s>>>(                                       \
         T, z_nrow, (S)alpha, br, bk, bc, z_offsets, z_row_counts, z_columns, static_cast<S*>(z_values), y_nrow,      \
         x_offsets, x_row_counts, x_columns, static_cast<const S*>(x_values), y_offsets, y_row_counts, y_columns,      \
         static_cast<const S*>(y_values))
