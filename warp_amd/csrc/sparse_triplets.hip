@@ -50,27 +50,38 @@ struct RowRanges {
 // consecutive rows (at most kMaxBuckets buckets, at most kMaxBucketRows rows
 // each); the triplets are cut into tiles of `tile` consecutive triplets.
 constexpr int kMaxBuckets = 4096;
-constexpr int kMaxBucketRows = 4096;
+constexpr int kMaxBucketRows = 8192;
 constexpr int kMaxTiles = 4096;
 constexpr int kMinTile = 16384;
+// With at most kStagedBuckets buckets, the scatter stages each sub-tile of
+// kSubTile triplets in LDS ordered by bucket and writes every bucket's run
+// contiguously (~kSubTile / buckets keys per run) instead of key by key.
+constexpr int kStagedBuckets = 256;
+constexpr int kSubTile = 4096;
 
 struct Partition {
     int shift = 0;
     int nb = 1;       // coarse buckets
     int tile = kMinTile;
     int ntiles = 1;
-    bool tiled = false;  // false: single-level bucketing with global atomics
+    bool tiled = false;   // false: single-level bucketing with global atomics
+    bool staged = false;  // LDS-staged, run-contiguous scatter (nb <= kStagedBuckets)
 };
 
 Partition partition_of(int nrow, int T)
 {
     Partition p;
-    while (((int64_t)nrow + (1ll << p.shift) - 1) >> p.shift > kMaxBuckets)
+    const auto buckets = [&](int shift) { return ((int64_t)nrow + (1ll << shift) - 1) >> shift; };
+    while (buckets(p.shift) > kStagedBuckets && (1 << (p.shift + 1)) <= kMaxBucketRows)
         ++p.shift;
-    p.nb = (int)(((int64_t)nrow + (1ll << p.shift) - 1) >> p.shift);
+    p.staged = buckets(p.shift) <= kStagedBuckets;
+    while (buckets(p.shift) > kMaxBuckets)
+        ++p.shift;
+    p.nb = (int)buckets(p.shift);
     if (p.nb < 1)
         p.nb = 1;
     p.tiled = (1 << p.shift) <= kMaxBucketRows && T > 0 && nrow > 0;
+    p.staged = p.staged && p.tiled;
     const int64_t per = (T + kMaxTiles - 1) / kMaxTiles;
     p.tile = (int)(per > kMinTile ? (per + kBlock - 1) / kBlock * kBlock : kMinTile);
     p.ntiles = (int)((T + (int64_t)p.tile - 1) / p.tile);
@@ -245,57 +256,169 @@ __global__ __launch_bounds__(kBlock) void tile_scatter_kernel(
     }
 }
 
-// One workgroup per coarse bucket: the bucket's rows get their key ranges
-// (row_start, counted and scanned in LDS) and their keys, in row order.
-__global__ __launch_bounds__(kBlock) void bucket_rows_kernel(
-    int nrow, Partition part, const int* __restrict__ tile_offsets, const uint64_t* __restrict__ bkeys,
-    const int* __restrict__ brow, int* __restrict__ row_start, uint64_t* __restrict__ keys
+// Tiled scatter with LDS staging (one workgroup per tile, nb <= 256 buckets):
+// each sub-tile of kSubTile triplets is ranked per bucket in LDS, placed in
+// LDS in bucket order, then copied out so that consecutive threads write
+// consecutive keys of one bucket's run.
+__global__ __launch_bounds__(kBlock) void tile_scatter_staged_kernel(
+    int T, const int* __restrict__ trow, const int* __restrict__ cols, Partition part,
+    const int* __restrict__ tile_offsets, uint64_t* __restrict__ bkeys, int* __restrict__ brow
 )
 {
-    __shared__ int cnt[kMaxBucketRows];
-    __shared__ int part_sum[kBlock];
-    const int b = blockIdx.x;
+    constexpr int PER = kSubTile / kBlock;
+    __shared__ int cursor[kStagedBuckets];
+    __shared__ int lcnt[kStagedBuckets];
+    __shared__ int loff[kStagedBuckets];
+    __shared__ uint64_t skey[kSubTile];
+    __shared__ int srow[kSubTile];
     const int tid = threadIdx.x;
-    const int r0 = b << part.shift;
-    const int nr = min(1 << part.shift, nrow - r0);
-    const int k0 = tile_offsets[(size_t)b * part.ntiles];
-    const int k1 = tile_offsets[(size_t)(b + 1) * part.ntiles];
-    for (int j = tid; j < nr; j += kBlock)
-        cnt[j] = 0;
-    __syncthreads();
-    for (int i = k0 + tid; i < k1; i += kBlock)
-        atomicAdd(&cnt[brow[i] - r0], 1);
-    __syncthreads();
-    // exclusive scan of cnt[0..nr): contiguous chunks per thread, then the chunk sums
-    const int chunk = (nr + kBlock - 1) / kBlock;
-    const int j0 = tid * chunk;
-    const int j1 = min(j0 + chunk, nr);
-    int sum = 0;
-    for (int j = j0; j < j1; ++j)
-        sum += cnt[j];
-    part_sum[tid] = sum;
-    __syncthreads();
-    if (tid == 0) {
-        int run = 0;
-        for (int i = 0; i < kBlock; ++i) {
-            const int v = part_sum[i];
-            part_sum[i] = run;
-            run += v;
+    for (int b = tid; b < part.nb; b += kBlock)
+        cursor[b] = tile_offsets[(size_t)b * part.ntiles + blockIdx.x];
+    const int64_t t0 = (int64_t)blockIdx.x * part.tile;
+    const int64_t t1 = t0 + part.tile < T ? t0 + part.tile : T;
+    for (int64_t sub = t0; sub < t1; sub += kSubTile) {
+        for (int b = tid; b < kStagedBuckets; b += kBlock)
+            lcnt[b] = 0;
+        __syncthreads();
+        int r[PER], c[PER], rank[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int64_t t = sub + k * kBlock + tid;
+            r[k] = t < t1 ? trow[t] : -1;
+            c[k] = r[k] >= 0 ? cols[t] : 0;
+            rank[k] = r[k] >= 0 ? atomicAdd(&lcnt[r[k] >> part.shift], 1) : 0;
         }
+        __syncthreads();
+        if (tid < kWave) {
+            // exclusive scan of the 256 bucket counts by one wavefront (4 per lane)
+            int v[kStagedBuckets / kWave];
+            int sum = 0;
+#pragma unroll
+            for (int j = 0; j < kStagedBuckets / kWave; ++j) {
+                v[j] = lcnt[tid * (kStagedBuckets / kWave) + j];
+                sum += v[j];
+            }
+            int incl = sum;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                const int y = __shfl_up(incl, o);
+                if (tid >= o)
+                    incl += y;
+            }
+            int run = incl - sum;
+#pragma unroll
+            for (int j = 0; j < kStagedBuckets / kWave; ++j) {
+                loff[tid * (kStagedBuckets / kWave) + j] = run;
+                run += v[j];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            if (r[k] < 0)
+                continue;
+            const int pos = loff[r[k] >> part.shift] + rank[k];
+            const int64_t t = sub + k * kBlock + tid;
+            skey[pos] = ((uint64_t)(uint32_t)c[k] << 32) | (uint32_t)t;
+            srow[pos] = r[k];
+        }
+        __syncthreads();
+        const int n = loff[kStagedBuckets - 1] + lcnt[kStagedBuckets - 1];
+        for (int i = tid; i < n; i += kBlock) {
+            const int row = srow[i];
+            const int b = row >> part.shift;
+            const int g = cursor[b] + (i - loff[b]);
+            bkeys[g] = skey[i];
+            brow[g] = row;
+        }
+        __syncthreads();
+        for (int b = tid; b < part.nb; b += kBlock)
+            cursor[b] += lcnt[b];
+    }
+}
+
+// Row pass over the bucket-ordered keys, one workgroup per chunk of kChunk
+// keys: a chunk's rows lie in a window of kRowWindow rows from its first
+// bucket, counted in LDS and added to the global histogram with one atomic per
+// (chunk, row) (rows beyond the window: one atomic per key).
+constexpr int kChunk = 16384;
+constexpr int kRowWindow = 8192;
+constexpr int kXcds = 8;
+
+// Workgroup b is dispatched to XCD b % kXcds (round-robin over the MI355X's 8
+// XCDs); numbering the chunks so that each XCD takes a contiguous range keeps
+// the chunks of one bucket -- which write into the same key range -- in one
+// L2.  The grid is a multiple of kXcds; surplus workgroups find no chunk.
+__device__ __forceinline__ int xcd_chunk()
+{
+    return (int)((blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds);
+}
+
+__global__ __launch_bounds__(kBlock) void chunk_row_hist_kernel(int nrow, Partition part,
+                                                               const int* __restrict__ total,
+                                                               const int* __restrict__ brow, int* __restrict__ hist)
+{
+    __shared__ int cnt[kRowWindow];
+    const int n = *total;
+    const int i0 = xcd_chunk() * kChunk;
+    if (i0 >= n)
+        return;
+    const int i1 = min(i0 + kChunk, n);
+    const int rlo = (brow[i0] >> part.shift) << part.shift;
+    for (int d = threadIdx.x; d < kRowWindow; d += kBlock)
+        cnt[d] = 0;
+    __syncthreads();
+    for (int i = i0 + threadIdx.x; i < i1; i += kBlock) {
+        const int r = brow[i];
+        const int d = r - rlo;
+        if (d < kRowWindow)
+            atomicAdd(&cnt[d], 1);
+        else
+            atomicAdd(&hist[r], 1);
     }
     __syncthreads();
-    int run = k0 + part_sum[tid];
-    for (int j = j0; j < j1; ++j) {
-        const int v = cnt[j];
-        cnt[j] = run;
-        row_start[r0 + j] = run;
-        run += v;
-    }
-    if (b == part.nb - 1 && tid == 0)
-        row_start[nrow] = k1;
+    for (int d = threadIdx.x; d < kRowWindow && rlo + d < nrow; d += kBlock)
+        if (cnt[d])
+            atomicAdd(&hist[rlo + d], cnt[d]);
+}
+
+// Scatter of the bucket-ordered keys into row order, one workgroup per chunk:
+// each (chunk, row) reserves its run with one global atomic on fill[row]
+// (zeroed beforehand), keys then take LDS-atomic ranks within the run.  The
+// writes of a chunk stay within its buckets' key ranges.
+__global__ __launch_bounds__(kBlock) void chunk_row_scatter_kernel(int nrow, Partition part,
+                                                                  const int* __restrict__ total,
+                                                                  const uint64_t* __restrict__ bkeys,
+                                                                  const int* __restrict__ brow,
+                                                                  const int* __restrict__ row_start,
+                                                                  int* __restrict__ fill, uint64_t* __restrict__ keys)
+{
+    __shared__ int cnt[kRowWindow];
+    const int n = *total;
+    const int i0 = xcd_chunk() * kChunk;
+    if (i0 >= n)
+        return;
+    const int i1 = min(i0 + kChunk, n);
+    const int rlo = (brow[i0] >> part.shift) << part.shift;
+    for (int d = threadIdx.x; d < kRowWindow; d += kBlock)
+        cnt[d] = 0;
     __syncthreads();
-    for (int i = k0 + tid; i < k1; i += kBlock) {
-        const int pos = atomicAdd(&cnt[brow[i] - r0], 1);
+    for (int i = i0 + threadIdx.x; i < i1; i += kBlock) {
+        const int d = brow[i] - rlo;
+        if (d < kRowWindow)
+            atomicAdd(&cnt[d], 1);
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < kRowWindow && rlo + d < nrow; d += kBlock) {
+        const int c = cnt[d];
+        if (c)
+            cnt[d] = row_start[rlo + d] + atomicAdd(&fill[rlo + d], c);
+    }
+    __syncthreads();
+    for (int i = i0 + threadIdx.x; i < i1; i += kBlock) {
+        const int r = brow[i];
+        const int d = r - rlo;
+        const int pos = d < kRowWindow ? atomicAdd(&cnt[d], 1) : row_start[r] + atomicAdd(&fill[r], 1);
         keys[pos] = bkeys[i];
     }
 }
@@ -799,10 +922,22 @@ int bucket_sort_rows(
         }
         WA_LAUNCH_CHECK();
         WA_TRY(exclusive_scan_i32(w.tile_offsets, w.tile_offsets, (int64_t)pt.nb * pt.ntiles, w.scan_ws, stream));
-        tile_scatter_kernel<<<tiles, kBlock, 0, stream>>>(T, w.trow, cols, pt, w.tile_offsets, w.alt, w.brow);
+        if (pt.staged)
+            tile_scatter_staged_kernel<<<tiles, kBlock, 0, stream>>>(T, w.trow, cols, pt, w.tile_offsets, w.alt,
+                                                                     w.brow);
+        else
+            tile_scatter_kernel<<<tiles, kBlock, 0, stream>>>(T, w.trow, cols, pt, w.tile_offsets, w.alt, w.brow);
         WA_LAUNCH_CHECK();
-        bucket_rows_kernel<<<(unsigned)pt.nb, kBlock, 0, stream>>>(nrow, pt, w.tile_offsets, w.alt, w.brow,
-                                                                   w.row_start, w.keys);
+        // rows within the buckets: chunked LDS histograms -> scan -> chunked scatter
+        const int* total = w.tile_offsets + (size_t)pt.nb * pt.ntiles;
+        const unsigned chunks = (unsigned)(ceil_div(ceil_div(T, kChunk), kXcds) * kXcds);
+        WA_TRY(launch_fill(w.hist, (int64_t)nrow + 1, 0, stream));
+        chunk_row_hist_kernel<<<chunks, kBlock, 0, stream>>>(nrow, pt, total, w.brow, w.hist);
+        WA_LAUNCH_CHECK();
+        WA_TRY(exclusive_scan_i32(w.hist, w.row_start, nrow, w.scan_ws, stream));
+        WA_TRY(launch_fill(w.hist, (int64_t)nrow + 1, 0, stream));
+        chunk_row_scatter_kernel<<<chunks, kBlock, 0, stream>>>(nrow, pt, total, w.alt, w.brow, w.row_start, w.hist,
+                                                                w.keys);
         WA_LAUNCH_CHECK();
         return sort_rows(nrow, RowRanges{w.row_start, nullptr}, w, summed_indices, urow, write_back, stream);
     }
