@@ -283,6 +283,18 @@ __global__ __launch_bounds__(kBlock) void conv3_mfma_kernel(
 #pragma unroll
         for (int g = 0; g < 4; ++g)
             acc[cb][g] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // CI <= 2: the halo offset of every tap this lane reads (-1: padding tap >= 27)
+    constexpr int NT = CI == 2 ? 2 : 4;  // taps per lane and k-block
+    int toff[CI <= 2 ? KB : 1][NT];
+    if constexpr (CI <= 2) {
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+            for (int i = 0; i < NT; ++i) {
+                const int tap = CI == 2 ? kb * 8 + 2 * q + i : kb * 16 + 4 * q + i;
+                toff[kb][i] = tap < 27 ? ((tap / 9) * kHY + ly + (tap / 3) % 3) * kHX + lx + tap % 3 : -1;
+            }
+    }
 
     for (int t = 0; t < TL; ++t) {
         const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
@@ -298,31 +310,22 @@ __global__ __launch_bounds__(kBlock) void conv3_mfma_kernel(
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 if constexpr (CI <= 2) {
-                    // k = 16 kb + 4 q + j -> (tap, channel) = (k / CI, k % CI)
+                    // k = 16 kb + 4 q + j -> (tap, channel) = (k / CI, k % CI); halo
+                    // offsets of the taps precomputed in toff
                     bf16x4 v;
+                    const int gz = g * (kHX * kHY);
+                    if constexpr (CI == 2) {
 #pragma unroll
-                    for (int j = 0; j < 4; j += 2) {
-                        const int k = kb * 16 + 4 * q + j;
-                        if constexpr (CI == 2) {
-                            const int tap = k >> 1;  // channels 0, 1 of one tap
-                            unsigned u = 0;
-                            if (tap < 27) {
-                                const int hv = ((g + tap / 9) * kHY + (ly + (tap / 3) % 3)) * kHX + (lx + tap % 3);
-                                u = tile[hv];
-                            }
-                            v[j] = (short)(u & 0xffffu);
-                            v[j + 1] = (short)(u >> 16);
-                        } else {
+                        for (int i = 0; i < 2; ++i) {
+                            const unsigned u = toff[kb][i] >= 0 ? tile[gz + toff[kb][i]] : 0u;
+                            v[2 * i] = (short)(u & 0xffffu);  // channels 0, 1 of one tap
+                            v[2 * i + 1] = (short)(u >> 16);
+                        }
+                    } else {
 #pragma unroll
-                            for (int jj = 0; jj < 2; ++jj) {
-                                const int tap = k + jj;
-                                unsigned u = 0;
-                                if (tap < 27) {
-                                    const int hv = ((g + tap / 9) * kHY + (ly + (tap / 3) % 3)) * kHX + (lx + tap % 3);
-                                    u = tile[hv];
-                                }
-                                v[j + jj] = (short)(u & 0xffffu);
-                            }
+                        for (int i = 0; i < 4; ++i) {
+                            const unsigned u = toff[kb][i] >= 0 ? tile[gz + toff[kb][i]] : 0u;
+                            v[i] = (short)(u & 0xffffu);
                         }
                     }
                     b[g] = v;
