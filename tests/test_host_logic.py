@@ -157,3 +157,43 @@ def test_padded_bsr_zeros_and_uncompress_rows():
         sparse.bsr_compress(A * 2.0)
     with pytest.raises(RuntimeError, match="ROCm GPU devices only"):
         sparse.bsr_compress(A)
+
+
+def test_row_and_block_index_helpers():
+    # Known answers for sparse.py:1794-1900 on a 3-row matrix with an empty middle row.
+    off = torch.tensor([0, 2, 2, 5], dtype=torch.int32)
+    cols = torch.tensor([0, 3, 1, 2, 4], dtype=torch.int32)
+    assert [sparse.bsr_row_index(off, 3, i) for i in range(-1, 7)] == [-1, 0, 0, 2, 2, 2, -1, -1]
+    pairs = [(0, 0), (0, 3), (0, 1), (1, 0), (2, 1), (2, 2), (2, 4), (2, 3), (-1, 0)]
+    assert [sparse.bsr_block_index(r, c, off, cols) for r, c in pairs] == [0, 1, -1, -1, 2, 3, 4, -1, -1]
+    # Capacity-aware overloads ignore slack positions past row_counts.
+    rc = torch.tensor([1, 0, 2], dtype=torch.int32)
+    np.testing.assert_array_equal(sparse.bsr_row_index(off, 3, torch.arange(6), rc).numpy(), [0, -1, 2, 2, -1, -1])
+    got = sparse.bsr_block_index(torch.tensor([0, 0, 2, 2]), torch.tensor([0, 3, 2, 4]), off, cols, rc)
+    np.testing.assert_array_equal(got.numpy(), [0, -1, 3, -1])
+    # Brute force on a random compact pattern: every (row, col) pair.
+    g = torch.Generator().manual_seed(3)
+    nrow, ncol = 17, 13
+    mask = torch.rand(nrow, ncol, generator=g) < 0.3
+    counts = mask.sum(1)
+    off = torch.zeros(nrow + 1, dtype=torch.int32)
+    off[1:] = torch.cumsum(counts, 0)
+    cols = torch.nonzero(mask)[:, 1].to(torch.int32)
+    rr, cc = torch.meshgrid(torch.arange(nrow), torch.arange(ncol), indexing="ij")
+    got = sparse.bsr_block_index(rr.flatten(), cc.flatten(), off, cols)
+    expect = torch.full((nrow * ncol,), -1, dtype=torch.int32)
+    expect[mask.flatten()] = torch.arange(int(counts.sum()), dtype=torch.int32)
+    np.testing.assert_array_equal(got.numpy(), expect.numpy())
+    rows = sparse.bsr_row_index(off, nrow, torch.arange(int(counts.sum())))
+    np.testing.assert_array_equal(rows.numpy(), torch.nonzero(mask)[:, 0].numpy())
+    assert sparse.bsr_block_index(0, 0, torch.zeros(3, dtype=torch.int32), torch.zeros(0, dtype=torch.int32)) == -1
+
+
+def test_public_names_match_reference_module():
+    names = set(sparse.__all__)
+    for name in names:
+        assert hasattr(sparse, name), name
+    assert sparse.bsr_matrix_t(wa.mat33) is sparse.BsrMatrix
+    assert sparse.bsr_matrix_t(wa.float64) is sparse.BsrMatrix
+    with pytest.raises(ValueError, match="matrix or scalar"):
+        sparse.bsr_matrix_t(str)

@@ -1,9 +1,12 @@
 """warp_amd: the sparse linear-algebra path of Warp, rebuilt for AMD MI355X (gfx950).
 
-``warp_amd.sparse`` mirrors ``warp.sparse`` (BSR assembly from triplets,
-``bsr_mv``, ``bsr_get_diag``) and ``warp_amd.optim.linear`` mirrors the CG
-part of ``warp.optim.linear``.  Arrays are torch tensors on a ROCm device;
-the compute runs in the hand-written HIP library ``_lib/libwarp_amd.so``.
+``warp_amd.sparse`` mirrors the public names of ``warp.sparse``: BSR assembly
+from triplets (compact, padded and masked topologies), ``bsr_compress``,
+transposes, ``bsr_assign`` / ``bsr_copy``, ``bsr_axpy``, ``bsr_mm``, ``bsr_mv``,
+diagonal helpers and the row / block index lookups.  ``warp_amd.optim.linear``
+mirrors the CG part of ``warp.optim.linear``; ``warp_amd.fem`` holds the
+Dirichlet projection of ``warp.fem``.  Arrays are torch tensors on a ROCm
+device; the compute runs in the hand-written HIP library ``_lib/libwarp_amd.so``.
 """
 
 from . import types

@@ -22,6 +22,35 @@ from . import _native
 from . import types as wt
 from .types import block_shape_of, type_is_matrix, type_scalar_type, type_size
 
+__all__ = [
+    "BSR_STATUS_ROW_CAPACITY_EXCEEDED",
+    "BSR_STATUS_SUCCESS",
+    "BsrMatrix",
+    "bsr_assign",
+    "bsr_axpy",
+    "bsr_axpy_work_arrays",
+    "bsr_block_index",
+    "bsr_compress",
+    "bsr_copy",
+    "bsr_diag",
+    "bsr_from_triplets",
+    "bsr_get_diag",
+    "bsr_identity",
+    "bsr_matrix_t",
+    "bsr_mm",
+    "bsr_mm_work_arrays",
+    "bsr_mv",
+    "bsr_row_index",
+    "bsr_scale",
+    "bsr_set_diag",
+    "bsr_set_from_triplets",
+    "bsr_set_identity",
+    "bsr_set_transpose",
+    "bsr_set_zero",
+    "bsr_transposed",
+    "bsr_zeros",
+]
+
 BSR_STATUS_SUCCESS = 0
 BSR_STATUS_ROW_CAPACITY_EXCEEDED = 1
 
@@ -1514,6 +1543,65 @@ def bsr_identity(rows_of_blocks: int, block_type, device=None) -> BsrMatrix:
     A = bsr_zeros(rows_of_blocks, rows_of_blocks, block_type=block_type, device=device)
     bsr_set_identity(A)
     return A
+
+
+def bsr_matrix_t(dtype):
+    """Matrix type for blocks of ``dtype`` (sparse.py:493-520): a scalar or a
+    matrix type; every block type shares the one :class:`BsrMatrix` class here."""
+    if not type_is_matrix(dtype):
+        try:
+            wt.type_to_scalar(dtype)
+        except TypeError:
+            raise ValueError(f"BsrMatrix block type must be either warp matrix or scalar; got {dtype!r}") from None
+    return BsrMatrix
+
+
+def bsr_row_index(offsets: torch.Tensor, row_count: int, block_index, row_counts: torch.Tensor | None = None):
+    """Row holding storage position ``block_index`` (a number or an integer
+    tensor), -1 when the position holds no (active) block (sparse.py:1794-1828).
+    Evaluated on tensors, not inside kernels."""
+    idx = torch.as_tensor(block_index, device=offsets.device).to(torch.int64)
+    off = offsets[: row_count + 1].to(torch.int64)
+    rows = torch.searchsorted(off, idx, right=True) - 1
+    rows = torch.where((idx >= 0) & (idx < off[row_count]), rows, torch.full_like(rows, -1))
+    if row_counts is not None:
+        safe = rows.clamp(min=0)
+        beyond = idx >= off[safe] + row_counts.to(torch.int64)[safe]
+        rows = torch.where((rows >= 0) & beyond, torch.full_like(rows, -1), rows)
+    rows = rows.to(torch.int32)
+    return int(rows) if rows.ndim == 0 else rows
+
+
+def bsr_block_index(row, col, bsr_offsets: torch.Tensor, bsr_columns: torch.Tensor,
+                    bsr_row_counts: torch.Tensor | None = None):
+    """Storage position of block (``row``, ``col``), -1 when absent
+    (sparse.py:1852-1900); rows must hold sorted columns.  ``row`` / ``col``
+    are numbers or integer tensors of one shape.  Evaluated on tensors."""
+    dev = bsr_offsets.device
+    r = torch.as_tensor(row, device=dev).to(torch.int64)
+    c = torch.as_tensor(col, device=dev).to(torch.int64)
+    nrow = bsr_offsets.numel() - 1
+    rs = r.clamp(0, max(nrow - 1, 0))
+    beg = bsr_offsets.to(torch.int64)[rs]
+    if bsr_row_counts is None:
+        end = bsr_offsets.to(torch.int64)[rs + 1]
+    else:
+        end = beg + bsr_row_counts.to(torch.int64)[rs]
+    # lower bound of c in columns[beg:end) by bisection, elementwise
+    cols = bsr_columns.to(torch.int64)
+    if cols.numel() == 0:
+        out = torch.full_like(r, -1).to(torch.int32)
+        return int(out) if out.ndim == 0 else out
+    lo, hi = beg.clone(), end.clone()
+    for _ in range(max(1, int(cols.numel()).bit_length())):
+        active = lo < hi
+        mid = torch.where(active, (lo + hi) // 2, lo)
+        less = active & (cols[mid.clamp(max=cols.numel() - 1)] < c)
+        lo = torch.where(less, mid + 1, lo)
+        hi = torch.where(active & ~less, mid, hi)
+    found = (r >= 0) & (r < nrow) & (lo < end) & (cols[lo.clamp(max=cols.numel() - 1)] == c)
+    out = torch.where(found, lo, torch.full_like(lo, -1)).to(torch.int32)
+    return int(out) if out.ndim == 0 else out
 
 
 def _vec_view(array: torch.Tensor, torch_dtype: torch.dtype, expected_scalar_count: int) -> torch.Tensor:
