@@ -261,6 +261,37 @@ def main() -> int:
         log(f"[bench] block-Jacobi-CG: {itb} iterations, {t_cgb * 1e3:.1f} ms")
         result["cg_block_jacobi"] = {"iterations": itb, "ms": round(t_cgb * 1e3, 2)}
 
+        # 2D Q1 Poisson, 128 x 128 cells (16,129 interior dofs): Jacobi-CG, tol 1e-6, maxiter 2000
+        rp, cp, vp, bp, ndp = problems.poisson2d_q1(128, 0)
+        Ap = sparse.bsr_from_triplets(ndp, ndp, torch.from_numpy(rp).to(dev), torch.from_numpy(cp).to(dev),
+                                      torch.from_numpy(vp).to(dev))
+        nnzp = Ap.nnz_sync()
+        bpt = torch.from_numpy(bp).to(dev)
+        xp = torch.zeros_like(bpt)
+        yp = torch.empty_like(bpt)
+        for _ in range(20):
+            sparse.bsr_mv(Ap, xp, yp)
+        e0.record()
+        for _ in range(500):
+            sparse.bsr_mv(Ap, bpt, yp)
+        e1.record()
+        torch.cuda.synchronize()
+        us_mv = e0.elapsed_time(e1) / 500 * 1e3
+        Mp = linear.preconditioner(Ap, "diag")
+        linear.cg(Ap, bpt, xp, tol=1e-6, maxiter=20, M=Mp)
+        xp.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        itp, errp, atolp = linear.cg(Ap, bpt, xp, tol=1e-6, maxiter=2000, M=Mp)
+        torch.cuda.synchronize()
+        t_p = time.perf_counter() - t0
+        log(f"[bench] Poisson 2D Q1 (n={ndp}, nnz={nnzp}): bsr_mv {us_mv:.1f} us, Jacobi-CG {itp} iterations "
+            f"{t_p * 1e3:.1f} ms")
+        result["poisson2d_cg"] = {"dofs": ndp, "nnz": nnzp, "bsr_mv_us": round(us_mv, 2), "iterations": itp,
+                                  "ms": round(t_p * 1e3, 2), "us_per_iter": round(t_p * 1e6 / max(itp, 1), 2),
+                                  "residual": errp, "atol": atolp}
+        del Ap, Mp
+
         # assembly of the shuffled element-loop stream
         r3, c3, v3, nn3 = problems.elasticity_triplets_shuffled(n, device=dev)
         A3 = sparse.bsr_zeros(nn3, nn3, wa.mat33, device=dev)

@@ -1066,3 +1066,102 @@ def test_bsr_diag_and_identity(device):
     np.testing.assert_array_equal(_active_dense(I)[0], np.eye(15))
     R = sparse.bsr_diag(2.0, rows_of_blocks=3, cols_of_blocks=5, device=device)
     np.testing.assert_array_equal(_active_dense(R)[0], 2.0 * np.eye(3, 5))
+
+
+def test_bsr_get_set_diag_reference_scenario(device):
+    """test_bsr_get_set_diag (test_sparse.py:486-551)."""
+    rng = np.random.default_rng(123)
+    nrow = ncol = 4
+    rows = torch.tensor([0, 1, 2, 3, 2, 1], dtype=torch.int32, device=device)
+    cols = torch.tensor([1, 1, 1, 3, 2, 2], dtype=torch.int32, device=device)
+    vals_np = rng.random(size=(6, 3, 3))
+    vals = torch.tensor(vals_np, dtype=torch.float32, device=device)
+    A = sparse.bsr_zeros(nrow, ncol, wa.mat33, device=device)
+    sparse.bsr_set_from_triplets(A, rows, cols, vals)
+    diag = sparse.bsr_get_diag(A)
+    d = diag.cpu().numpy()
+    np.testing.assert_array_equal(d[0], np.zeros((3, 3)))
+    for r, k in ((1, 1), (2, 4), (3, 3)):
+        np.testing.assert_allclose(d[r], vals_np[k], atol=1e-5)
+    sentinel = torch.full((nrow, 3, 3), 7.0, dtype=torch.float32, device=device)
+    sparse.bsr_get_diag(A, out=sentinel)
+    np.testing.assert_allclose(sentinel.cpu().numpy(), d, atol=1e-5)
+    # set_diag / get_diag round trips
+    D = sparse.bsr_diag(diag)
+    sparse.bsr_get_diag(D, out=diag)
+    np.testing.assert_array_equal(diag.cpu().numpy(), d)
+    s_np = rng.random(size=nrow)
+    Ds = sparse.bsr_diag(torch.tensor(s_np, dtype=torch.float32, device=device))
+    np.testing.assert_allclose(sparse.bsr_get_diag(Ds).cpu().numpy(), s_np, atol=1e-6)
+    np.testing.assert_allclose(sparse.bsr_get_diag(2.0 * Ds).cpu().numpy(), 2.0 * s_np, atol=1e-6)
+    # uniform block diagonals
+    with pytest.raises(ValueError, match="BsrMatrix block type must be either warp matrix or scalar"):
+        sparse.bsr_diag(diag=vals_np[0, 0].tolist(), rows_of_blocks=nrow, cols_of_blocks=nrow + 1, device=device)
+    U = sparse.bsr_diag(diag=vals_np[0].astype(np.float32).tolist(), rows_of_blocks=nrow, cols_of_blocks=nrow + 1,
+                        device=device)
+    assert U.values.shape[0] == nrow
+    np.testing.assert_allclose(U.values.cpu().numpy(), np.broadcast_to(vals_np[0], (nrow, 3, 3)), atol=1e-6)
+    Us = sparse.bsr_diag(diag=float(s_np[0]), rows_of_blocks=nrow, cols_of_blocks=nrow + 1, device=device)
+    assert Us.values.shape[0] == nrow
+    np.testing.assert_allclose(Us.values.cpu().numpy(), np.full(nrow, s_np[0]), atol=1e-6)
+    I4 = sparse.bsr_identity(nrow, block_type=wa.matrix((4, 4), wa.float32), device=device)
+    np.testing.assert_allclose(I4.values.cpu().numpy(), np.broadcast_to(np.eye(4), (nrow, 4, 4)), atol=1e-6)
+    I1 = sparse.bsr_identity(nrow, block_type=wa.float64, device=device)
+    np.testing.assert_array_equal(I1.values.cpu().numpy(), np.ones(nrow))
+
+
+def test_bsr_split_merge_reference_scenario(device):
+    """test_bsr_split_merge (test_sparse.py:554-600)."""
+    rng = np.random.default_rng(123)
+    nrow, ncol, n = 4, 8, 20
+    rows = torch.tensor(rng.integers(0, high=nrow, size=n), dtype=torch.int32, device=device)
+    cols = torch.tensor(rng.integers(0, high=ncol, size=n), dtype=torch.int32, device=device)
+    vals = torch.tensor(rng.random(size=(n, 4, 2)), dtype=torch.float32, device=device)
+    A = sparse.bsr_zeros(nrow, ncol, wa.matrix((4, 2), wa.float32), device=device)
+    sparse.bsr_set_from_triplets(A, rows, cols, vals)
+    ref = _dense_of(A)
+    for shape, size in (((2, 2), 4), ((1, 1), 1), ((4, 4), 16), ((8, 8), 64)):
+        B = sparse.bsr_copy(A, block_shape=shape)
+        assert B.block_size == size
+        np.testing.assert_allclose(_dense_of(B), ref, atol=1e-4)
+    for bad in ((3, 3), (5, 5)):
+        with pytest.raises(ValueError, match="Incompatible dest and src block shapes"):
+            sparse.bsr_copy(A, block_shape=bad)
+    with pytest.raises(
+        ValueError,
+        match=r"The requested block shape \(32, 32\) does not evenly divide the source matrix of total size \(16, 16\)",
+    ):
+        sparse.bsr_copy(A, block_shape=(32, 32))
+
+
+def test_bsr_assign_masked_reference_scenario(device):
+    """test_bsr_assign_masked (test_sparse.py:603-628): coarse diagonal by masked assign."""
+    rng = np.random.default_rng(123)
+    nrow, ncol, n = 16, 8, 20
+    rows = torch.tensor(rng.integers(0, high=nrow, size=n), dtype=torch.int32, device=device)
+    cols = torch.tensor(rng.integers(0, high=ncol, size=n), dtype=torch.int32, device=device)
+    vals = torch.tensor(rng.random(size=(n, 1, 2)), dtype=torch.float32, device=device)
+    A = sparse.bsr_from_triplets(nrow, ncol, rows, cols, vals)
+    coarse = sparse.bsr_copy(A, block_shape=(4, 4))
+    ref = _dense_of(sparse.bsr_diag(sparse.bsr_get_diag(coarse)))
+    masked = sparse.bsr_diag(rows_of_blocks=4, block_type=coarse.dtype, device=device)
+    sparse.bsr_assign(src=A, dest=masked, topology="masked")
+    np.testing.assert_allclose(_dense_of(masked), ref, atol=1e-4)
+    # the masked destination keeps its diagonal topology
+    np.testing.assert_array_equal(masked.columns[:4].cpu().numpy(), [0, 1, 2, 3])
+
+
+def test_bsr_copy_scale_reference_scenario(device):
+    """TestSparse.test_bsr_copy_scale (test_sparse.py:1393-1415)."""
+    nrow, bsize = 6, 2
+    D = sparse.bsr_diag(diag=(np.eye(bsize) * 2.0).tolist(), rows_of_blocks=nrow, device=device)
+    C = sparse.bsr_copy(D, scalar_type=wa.float64)
+    assert C.scalar_type is wa.float64 and C.block_shape == (bsize, bsize)
+    assert C.values.dtype == torch.float64
+    sparse.bsr_scale(x=C, alpha=0.5)
+    np.testing.assert_allclose(_dense_of(C), np.eye(nrow * bsize), atol=1e-4)
+    sparse.bsr_scale(x=C, alpha=0.0)
+    assert C.nrow == nrow and C.ncol == nrow and C.nnz == D.nnz
+    pruned = sparse.bsr_from_triplets(C.nrow, C.ncol, C.uncompress_rows(), C.columns[: C.nnz], C.values[: C.nnz],
+                                      prune_numerical_zeros=True)
+    assert pruned.nnz_sync() == 0

@@ -1525,6 +1525,8 @@ def bsr_diag(diag=None, rows_of_blocks: int | None = None, cols_of_blocks: int |
         if diag is None:
             raise ValueError("Either `diag` or `block_type` needs to be provided")
         d = torch.as_tensor(diag)
+        if d.ndim == 1:
+            raise ValueError(f"BsrMatrix block type must be either warp matrix or scalar; got a vector of length {d.shape[0]}")
         block_type = (wt.type_to_scalar(d.dtype) if d.ndim == 0
                       else wt.matrix(shape=tuple(d.shape), dtype=wt.type_to_scalar(d.dtype)))
     A = bsr_zeros(rows_of_blocks, cols_of_blocks, block_type=block_type, device=device)
