@@ -172,14 +172,50 @@ __device__ __forceinline__ void write_block_partials(const double (&mine)[NV], d
     }
 }
 
+// Fixed-order sums of NV rows of `count` partials (row v at partials + v *
+// stride) by one workgroup; results in every thread.  Each thread adds its
+// entries k = tid, tid + BLOCK, ... in ascending order, but loads them in
+// batches of kBatch per row first: a finalize kernel is one workgroup, so its
+// time is the number of dependent L2 round trips, not bandwidth.
+template <int BLOCK, int NV>
+__device__ __forceinline__ void sum_partials_rows(const double* partials, size_t stride, int count, double* lds,
+                                                  double (&out)[NV])
+{
+    constexpr int kBatch = 16 / NV > 4 ? 16 / NV : 4;
+    double a[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+        a[v] = 0.0;
+    int k = threadIdx.x;
+    for (; k + (kBatch - 1) * BLOCK < count; k += kBatch * BLOCK) {
+        double t[NV][kBatch];
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u)
+                t[v][u] = partials[(size_t)v * stride + k + u * BLOCK];
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u)
+                a[v] += t[v][u];
+    }
+    for (; k < count; k += BLOCK)
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+            a[v] += partials[(size_t)v * stride + k];
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+        out[v] = block_sum<BLOCK>(a[v], lds);
+}
+
 // Fixed-order sum of `count` partials by one workgroup (result in all threads).
 template <int BLOCK>
 __device__ __forceinline__ double sum_partials(const double* partials, int count, double* lds)
 {
-    double a = 0.0;
-    for (int k = threadIdx.x; k < count; k += BLOCK)
-        a += partials[k];
-    return block_sum<BLOCK>(a, lds);
+    double out[1];
+    sum_partials_rows<BLOCK, 1>(partials, 0, count, lds, out);
+    return out[0];
 }
 
 }  // namespace wa

@@ -200,10 +200,15 @@ __global__ __launch_bounds__(kFinalizeBlock) void finalize_fcg_kernel(const doub
                                                                      double* scale, double* inv_scale)
 {
     __shared__ double lds[kFinalizeBlock / kWave];
-    const double t0 = sum_partials<kFinalizeBlock>(partials, nparts, lds);
-    double t1 = 0.0;
-    if (out1)
-        t1 = sum_partials<kFinalizeBlock>(partials + nparts, nparts, lds);
+    double t0, t1 = 0.0;
+    if (out1) {
+        double t[2];
+        sum_partials_rows<kFinalizeBlock, 2>(partials, (size_t)nparts, nparts, lds, t);
+        t0 = t[0];
+        t1 = t[1];
+    } else {
+        t0 = sum_partials<kFinalizeBlock>(partials, nparts, lds);
+    }
     if (threadIdx.x == 0) {
         *out0 = t0;
         if (out1)
