@@ -337,12 +337,18 @@ __global__ __launch_bounds__(kBlock) void tile_scatter_staged_kernel(
     }
 }
 
-// Row pass over the bucket-ordered keys, one workgroup per chunk of kChunk
-// keys: a chunk's rows lie in a window of kRowWindow rows from its first
-// bucket, counted in LDS and added to the global histogram with one atomic per
-// (chunk, row) (rows beyond the window: one atomic per key).
+// Row pass over the bucket-ordered keys, one workgroup of kChunkThreads per
+// chunk of kChunk keys: a chunk's rows lie in a window of kRowWindow rows from
+// its first bucket, counted in LDS and added to the global histogram with one
+// atomic per (chunk, row) (rows beyond the window: one atomic per key).  Each
+// thread loads its kChunkPer keys up front and issues its atomics in unrolled
+// batches: these kernels are bound by latency (a few thousand workgroups, one
+// global round trip each), not by their ~270 MB of traffic.
 constexpr int kChunk = 16384;
+constexpr int kChunkThreads = 1024;
+constexpr int kChunkPer = kChunk / kChunkThreads;
 constexpr int kRowWindow = 8192;
+constexpr int kWindowPer = kRowWindow / kChunkThreads;
 constexpr int kXcds = 8;
 
 // Workgroup b is dispatched to XCD b % kXcds (round-robin over the MI355X's 8
@@ -354,9 +360,10 @@ __device__ __forceinline__ int xcd_chunk()
     return (int)((blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds);
 }
 
-__global__ __launch_bounds__(kBlock) void chunk_row_hist_kernel(int nrow, Partition part,
-                                                               const int* __restrict__ total,
-                                                               const int* __restrict__ brow, int* __restrict__ hist)
+__global__ __launch_bounds__(kChunkThreads) void chunk_row_hist_kernel(int nrow, Partition part,
+                                                                      const int* __restrict__ total,
+                                                                      const int* __restrict__ brow,
+                                                                      int* __restrict__ hist)
 {
     __shared__ int cnt[kRowWindow];
     const int n = *total;
@@ -365,33 +372,47 @@ __global__ __launch_bounds__(kBlock) void chunk_row_hist_kernel(int nrow, Partit
         return;
     const int i1 = min(i0 + kChunk, n);
     const int rlo = (brow[i0] >> part.shift) << part.shift;
-    for (int d = threadIdx.x; d < kRowWindow; d += kBlock)
-        cnt[d] = 0;
+    int r[kChunkPer];
+#pragma unroll
+    for (int u = 0; u < kChunkPer; ++u) {
+        const int i = i0 + (int)threadIdx.x + u * kChunkThreads;
+        r[u] = i < i1 ? brow[i] : -1;
+    }
+#pragma unroll
+    for (int q = 0; q < kWindowPer; ++q)
+        cnt[threadIdx.x + q * kChunkThreads] = 0;
     __syncthreads();
-    for (int i = i0 + threadIdx.x; i < i1; i += kBlock) {
-        const int r = brow[i];
-        const int d = r - rlo;
+#pragma unroll
+    for (int u = 0; u < kChunkPer; ++u) {
+        if (r[u] < 0)
+            continue;
+        const int d = r[u] - rlo;
         if (d < kRowWindow)
             atomicAdd(&cnt[d], 1);
         else
-            atomicAdd(&hist[r], 1);
+            atomicAdd(&hist[r[u]], 1);
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < kRowWindow && rlo + d < nrow; d += kBlock)
-        if (cnt[d])
-            atomicAdd(&hist[rlo + d], cnt[d]);
+#pragma unroll
+    for (int q = 0; q < kWindowPer; ++q) {
+        const int d = threadIdx.x + q * kChunkThreads;
+        const int c = cnt[d];
+        if (c && rlo + d < nrow)
+            atomicAdd(&hist[rlo + d], c);
+    }
 }
 
 // Scatter of the bucket-ordered keys into row order, one workgroup per chunk:
 // each (chunk, row) reserves its run with one global atomic on fill[row]
 // (zeroed beforehand), keys then take LDS-atomic ranks within the run.  The
 // writes of a chunk stay within its buckets' key ranges.
-__global__ __launch_bounds__(kBlock) void chunk_row_scatter_kernel(int nrow, Partition part,
-                                                                  const int* __restrict__ total,
-                                                                  const uint64_t* __restrict__ bkeys,
-                                                                  const int* __restrict__ brow,
-                                                                  const int* __restrict__ row_start,
-                                                                  int* __restrict__ fill, uint64_t* __restrict__ keys)
+__global__ __launch_bounds__(kChunkThreads) void chunk_row_scatter_kernel(int nrow, Partition part,
+                                                                         const int* __restrict__ total,
+                                                                         const uint64_t* __restrict__ bkeys,
+                                                                         const int* __restrict__ brow,
+                                                                         const int* __restrict__ row_start,
+                                                                         int* __restrict__ fill,
+                                                                         uint64_t* __restrict__ keys)
 {
     __shared__ int cnt[kRowWindow];
     const int n = *total;
@@ -400,27 +421,52 @@ __global__ __launch_bounds__(kBlock) void chunk_row_scatter_kernel(int nrow, Par
         return;
     const int i1 = min(i0 + kChunk, n);
     const int rlo = (brow[i0] >> part.shift) << part.shift;
-    for (int d = threadIdx.x; d < kRowWindow; d += kBlock)
-        cnt[d] = 0;
+    // the keys themselves are loaded again at the end (from L2): holding them
+    // across the atomics would cost the registers of a second resident workgroup
+    int r[kChunkPer];
+#pragma unroll
+    for (int u = 0; u < kChunkPer; ++u) {
+        const int i = i0 + (int)threadIdx.x + u * kChunkThreads;
+        r[u] = i < i1 ? brow[i] : -1;
+    }
+#pragma unroll
+    for (int q = 0; q < kWindowPer; ++q)
+        cnt[threadIdx.x + q * kChunkThreads] = 0;
     __syncthreads();
-    for (int i = i0 + threadIdx.x; i < i1; i += kBlock) {
-        const int d = brow[i] - rlo;
-        if (d < kRowWindow)
+#pragma unroll
+    for (int u = 0; u < kChunkPer; ++u) {
+        const int d = r[u] - rlo;
+        if (r[u] >= 0 && d < kRowWindow)
             atomicAdd(&cnt[d], 1);
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < kRowWindow && rlo + d < nrow; d += kBlock) {
-        const int c = cnt[d];
-        if (c)
-            cnt[d] = row_start[rlo + d] + atomicAdd(&fill[rlo + d], c);
+    int c[kWindowPer], base[kWindowPer];
+#pragma unroll
+    for (int q = 0; q < kWindowPer; ++q) {
+        const int d = threadIdx.x + q * kChunkThreads;
+        c[q] = rlo + d < nrow ? cnt[d] : 0;
+        base[q] = 0;
+        if (c[q])
+            base[q] = row_start[rlo + d] + atomicAdd(&fill[rlo + d], c[q]);
     }
+#pragma unroll
+    for (int q = 0; q < kWindowPer; ++q)
+        if (c[q])
+            cnt[threadIdx.x + q * kChunkThreads] = base[q];
     __syncthreads();
-    for (int i = i0 + threadIdx.x; i < i1; i += kBlock) {
-        const int r = brow[i];
-        const int d = r - rlo;
-        const int pos = d < kRowWindow ? atomicAdd(&cnt[d], 1) : row_start[r] + atomicAdd(&fill[r], 1);
-        keys[pos] = bkeys[i];
+    int pos[kChunkPer];
+#pragma unroll
+    for (int u = 0; u < kChunkPer; ++u) {
+        pos[u] = -1;
+        if (r[u] >= 0) {
+            const int d = r[u] - rlo;
+            pos[u] = d < kRowWindow ? atomicAdd(&cnt[d], 1) : row_start[r[u]] + atomicAdd(&fill[r[u]], 1);
+        }
     }
+#pragma unroll
+    for (int u = 0; u < kChunkPer; ++u)
+        if (pos[u] >= 0)
+            keys[pos[u]] = bkeys[i0 + (int)threadIdx.x + u * kChunkThreads];
 }
 
 // In-LDS bitonic sort of `n` (power of two) keys by `nthreads` threads.
@@ -932,12 +978,12 @@ int bucket_sort_rows(
         const int* total = w.tile_offsets + (size_t)pt.nb * pt.ntiles;
         const unsigned chunks = (unsigned)(ceil_div(ceil_div(T, kChunk), kXcds) * kXcds);
         WA_TRY(launch_fill(w.hist, (int64_t)nrow + 1, 0, stream));
-        chunk_row_hist_kernel<<<chunks, kBlock, 0, stream>>>(nrow, pt, total, w.brow, w.hist);
+        chunk_row_hist_kernel<<<chunks, kChunkThreads, 0, stream>>>(nrow, pt, total, w.brow, w.hist);
         WA_LAUNCH_CHECK();
         WA_TRY(exclusive_scan_i32(w.hist, w.row_start, nrow, w.scan_ws, stream));
         WA_TRY(launch_fill(w.hist, (int64_t)nrow + 1, 0, stream));
-        chunk_row_scatter_kernel<<<chunks, kBlock, 0, stream>>>(nrow, pt, total, w.alt, w.brow, w.row_start, w.hist,
-                                                                w.keys);
+        chunk_row_scatter_kernel<<<chunks, kChunkThreads, 0, stream>>>(nrow, pt, total, w.alt, w.brow, w.row_start,
+                                                                       w.hist, w.keys);
         WA_LAUNCH_CHECK();
         return sort_rows(nrow, RowRanges{w.row_start, nullptr}, w, summed_indices, urow, write_back, stream);
     }
