@@ -423,10 +423,58 @@ void launch_tiles(int ntiles, hipStream_t s, Args... args)
     Kernel<<<grid, kBlock, 0, s>>>(args...);
 }
 
+// Halo of one tile held in registers between its loads and its LDS stores
+// (CI >= 16: 16-byte channel octets, kHalo * CI / 8 of them per tile).
+template <int CI, int IN> struct HaloRegs {
+    static constexpr int QPV = CI / 8;
+    static constexpr int TOTAL = kHalo * QPV;
+    static constexpr int ITERS = (TOTAL + kBlock - 1) / kBlock;
+    bf16x8 v[ITERS];
+    bf16x8 cv[IN == kInUpAdd ? ITERS : 1];
+
+    __device__ __forceinline__ void load(int m, int X0, int Y0, int Z0, const unsigned short* __restrict__ x,
+                                         const unsigned short* __restrict__ coarse)
+    {
+        const int mh = m / 2;
+#pragma unroll
+        for (int u = 0; u < ITERS; ++u) {
+            const int c = u * kBlock + threadIdx.x;
+            const int hv = c / QPV, od = c % QPV;
+            const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
+            const int gx = X0 - 1 + hx, gy = Y0 - 1 + hy, gz = Z0 - 1 + hz;
+            v[u] = zero8();
+            if constexpr (IN == kInUpAdd)
+                cv[u] = zero8();
+            if (c < TOTAL && gx >= 0 && gx < m && gy >= 0 && gy < m && gz >= 0 && gz < m) {
+                v[u] = load8(x + ((size_t)(gz * m + gy) * m + gx) * CI + 8 * od);
+                if constexpr (IN == kInUpAdd)
+                    cv[u] = load8(coarse + ((size_t)((gz >> 1) * mh + (gy >> 1)) * mh + (gx >> 1)) * CI + 8 * od);
+            }
+        }
+    }
+
+    __device__ __forceinline__ void store(unsigned* tile) const
+    {
+        constexpr int SW = lds_stride_words(CI);
+#pragma unroll
+        for (int u = 0; u < ITERS; ++u) {
+            const int c = u * kBlock + threadIdx.x;
+            if (c < TOTAL) {
+                bf16x8 w = v[u];
+                if constexpr (IN == kInUpAdd)
+                    w = add8(w, cv[u]);
+                *reinterpret_cast<bf16x8*>(&tile[(c / QPV) * SW + 4 * (c % QPV)]) = w;
+            }
+        }
+    }
+};
+
 // One output channel (the network's last layer): MFMA would waste 15 of its
 // 16 rows, so each thread computes one voxel with packed BF16 dot products,
 // v_dot2_f32_bf16, against weights wpk[tap][ci / 2] (BF16 pairs), reading
-// the LDS-staged halo.  Output FP32; flags: ReLU, accumulate.
+// the LDS-staged halo.  Output FP32; flags: ReLU, accumulate.  The halo of
+// the workgroup's next tile is loaded into registers while the dot products
+// of the current one run, so the global round trips overlap the arithmetic.
 template <int CI, int IN>
 __global__ __launch_bounds__(kBlock) void conv3_co1_kernel(int m, const unsigned short* __restrict__ x,
                                                           const unsigned short* __restrict__ coarse,
@@ -439,25 +487,47 @@ __global__ __launch_bounds__(kBlock) void conv3_co1_kernel(int m, const unsigned
     __shared__ unsigned tile[kHalo * SW];
     const int tiles_x = (m + kTX - 1) / kTX;
     const int ntiles = tiles_x * tiles_x * (m / kTZ);
-    for (TileWalk walk(ntiles); walk.t < walk.end; walk.t += walk.step) {
-    const int tile_id = walk.t;
-    const int X0 = (tile_id % tiles_x) * kTX;
-    const int Y0 = ((tile_id / tiles_x) % tiles_x) * kTY;
-    const int Z0 = (tile_id / (tiles_x * tiles_x)) * kTZ;
+    TileWalk walk(ntiles);
+    auto origin = [&](int t, int& X0, int& Y0, int& Z0) {
+        X0 = (t % tiles_x) * kTX;
+        Y0 = ((t / tiles_x) % tiles_x) * kTY;
+        Z0 = (t / (tiles_x * tiles_x)) * kTZ;
+    };
+    HaloRegs<CI, IN> halo;
+    if (walk.t < walk.end) {
+        int X0, Y0, Z0;
+        origin(walk.t, X0, Y0, Z0);
+        halo.load(m, X0, Y0, Z0, x, coarse);
+    }
+    const float b0 = bias ? bias[0] : 0.0f;
+    for (; walk.t < walk.end; walk.t += walk.step) {
+    int X0, Y0, Z0;
+    origin(walk.t, X0, Y0, Z0);
     __syncthreads();  // the previous tile's LDS reads are done
-    stage_tile<CI, IN>(tile, m, X0, Y0, Z0, x, coarse);
+    halo.store(tile);
     __syncthreads();
+    if (walk.t + walk.step < walk.end) {
+        int X1, Y1, Z1;
+        origin(walk.t + walk.step, X1, Y1, Z1);
+        halo.load(m, X1, Y1, Z1, x, coarse);
+    }
 
     const int lx = threadIdx.x & 7, ly = (threadIdx.x >> 3) & 7, lz = threadIdx.x >> 6;  // 8 x 8 x 4
-    float acc = bias ? bias[0] : 0.0f;
+    float acc = b0;
+    // taps three rows at a time: a full unroll holds every LDS read at once
+    // (159 VGPRs, 3 waves per SIMD) and measured slower than this (88 VGPRs)
+#pragma unroll 3
+    for (int t9 = 0; t9 < 9; ++t9) {
 #pragma unroll
-    for (int t = 0; t < 27; ++t) {
-        const int hv = ((lz + t / 9) * kHY + (ly + (t / 3) % 3)) * kHX + (lx + t % 3);
-        const unsigned* src = &tile[hv * SW];
+        for (int t3 = 0; t3 < 3; ++t3) {
+            const int t = t9 * 3 + t3;
+            const int hv = ((lz + t9 / 3) * kHY + (ly + t9 % 3)) * kHX + (lx + t3);
+            const unsigned* src = &tile[hv * SW];
 #pragma unroll
-        for (int j = 0; j < CI / 2; ++j)
-            acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, src[j]),
-                                                  __builtin_bit_cast(bf16x2, wpk[t * (CI / 2) + j]), acc, false);
+            for (int j = 0; j < CI / 2; ++j)
+                acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, src[j]),
+                                                      __builtin_bit_cast(bf16x2, wpk[t * (CI / 2) + j]), acc, false);
+        }
     }
     const int vx = X0 + lx, vy = Y0 + ly, vz = Z0 + lz;
     if (vx < m && vy < m && vz < m) {
