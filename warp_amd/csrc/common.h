@@ -175,13 +175,13 @@ __device__ __forceinline__ void write_block_partials(const double (&mine)[NV], d
 // Fixed-order sums of NV rows of `count` partials (row v at partials + v *
 // stride) by one workgroup; results in every thread.  Each thread adds its
 // entries k = tid, tid + BLOCK, ... in ascending order, but loads them in
-// batches of kBatch per row first: a finalize kernel is one workgroup, so its
+// batches of kBatch per row first (64 VGPRs of loads in flight): a finalize kernel is one workgroup, so its
 // time is the number of dependent L2 round trips, not bandwidth.
 template <int BLOCK, int NV>
 __device__ __forceinline__ void sum_partials_rows(const double* partials, size_t stride, int count, double* lds,
                                                   double (&out)[NV])
 {
-    constexpr int kBatch = 16 / NV > 4 ? 16 / NV : 4;
+    constexpr int kBatch = 32 / NV > 4 ? 32 / NV : 4;
     double a[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v)

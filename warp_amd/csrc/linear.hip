@@ -206,7 +206,13 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(
     write_block_partials<kBlock, 1>(acc, ws.partials, lds);
 }
 
-// One block row per thread.
+// Block rows grid-strided over at most kMaxUpdateGrid workgroups, so the
+// finalize kernel after it sums few partials (a one-workgroup sum of 65 K
+// partials cost 17 us per iteration at 16.7 M rows).  The SpMV keeps one
+// workgroup per 256 rows: capping its grid the same way measured slower
+// (405 -> 416-430 us per iteration at 256^3).
+constexpr int kMaxUpdateGrid = 4096;
+
 template <typename S, int BR, int PREC>
 __global__ __launch_bounds__(kBlock) void cg_update_kernel(
     int nrow, const S* __restrict__ prec, S* __restrict__ x, S* __restrict__ r, S* __restrict__ z,
@@ -217,9 +223,8 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(
     if (ws.state->done)
         return;
     const S alpha = (S)ws.state->alpha;
-    const int row = blockIdx.x * kBlock + threadIdx.x;
     double acc[2] = {0.0, 0.0};
-    if (row < nrow) {
+    for (int row = blockIdx.x * kBlock + threadIdx.x; row < nrow; row += gridDim.x * kBlock) {
         S rv[BR], zv[BR];
 #pragma unroll
         for (int s = 0; s < BR; ++s) {
@@ -415,7 +420,7 @@ template <typename S> struct CgLauncher {
     }
     template <int BR, int PREC> int update_t()
     {
-        const unsigned grid = (unsigned)ceil_div(pb.nrow, kBlock);
+        const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(pb.nrow, kBlock), kMaxUpdateGrid);
         cg_update_kernel<S, BR, PREC><<<grid, kBlock, 0, stream>>>(
             pb.nrow, (const S*)pb.prec, (S*)pb.x, (S*)ws.r, (S*)ws.z, p_cur(), (const S*)ws.Ap, ws
         );
