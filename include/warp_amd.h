@@ -225,6 +225,27 @@ int warp_amd_bsr_assign_padded(int src_scalar_size, int dest_scalar_size, int nr
  * warp/_src/sparse.py:164-193). */
 int warp_amd_bsr_uncompress_rows(int nrow, const int* offsets, const int* row_counts, int T, int* rows, void* stream);
 
+/* dst[i] = (dst type)(scale * src[i]) for i < n, scale in the source type:
+ * the values of a same-shape compact assignment (replaces the block copy of
+ * _bsr_assign_copy_blocks, warp/_src/sparse.py:2258-2452, when the
+ * destination topology is the source's own). */
+int warp_amd_copy_scaled_values(int src_scalar_size, int dst_scalar_size, int64_t n, const void* src, double scale,
+                                void* dst, void* stream);
+
+/* x[i] = 0 + beta * x[i] for i < n (fused multiply-add off): the values a
+ * compact union-topology rebuild gives the blocks it keeps, so sums whose
+ * topology is already present skip the rebuild bit for bit.  No reference
+ * counterpart (the reference always rebuilds, sparse.py:3340-3617,
+ * 4200-4640). */
+int warp_amd_rescale_values(int scalar_size, int64_t n, void* x, double beta, void* stream);
+
+/* *missing = number of pairs (rows[i], cols[i]), rows[i] >= 0, i < n, that
+ * have no block in the matrix (offsets, row_counts, columns; sorted rows),
+ * found by binary search as bsr_block_index does (sparse.py:1852-1900).
+ * *missing is cleared by the call. */
+int warp_amd_bsr_count_missing_blocks(int64_t n, const int* rows, const int* cols, const int* offsets,
+                                      const int* row_counts, const int* columns, int* missing, void* stream);
+
 /* dest[block (rows[i], cols[i])] += scale * src[i] for i < n; missing blocks
  * and rows[i] < 0 skipped (replaces _bsr_axpy_add_block,
  * warp/_src/sparse.py:3145-3163).  At most one source block per destination
@@ -263,11 +284,11 @@ int warp_amd_bsr_mm_list(int n, int base, int T, int x_nrow, const int* x_offset
                          const int* mm_offsets, int* rows, int* cols, void* stream);
 
 /* Product values into an existing topology (replaces _bsr_mm_compute_values,
- * warp/_src/sparse.py:3764-3800): z(r, c) += alpha * sum over x row r's
- * active blocks k, in storage order, of x_k @ y(x_columns[k], c).  x blocks
- * br x bk, y bk x bc (br * bc <= 64); products and sums rounded one by one
- * (no fused multiply-add). */
-int warp_amd_bsr_mm_values(int scalar_size, int br, int bk, int bc, double alpha, int T, int z_nrow,
+ * warp/_src/sparse.py:3764-3800): z(r, c) = (0 + beta * z(r, c)) + alpha * sum
+ * over x row r's active blocks k, in storage order, of x_k @ y(x_columns[k],
+ * c).  x blocks br x bk, y bk x bc (br * bc <= 64); products and sums rounded
+ * one by one (no fused multiply-add). */
+int warp_amd_bsr_mm_values(int scalar_size, int br, int bk, int bc, double alpha, double beta, int T, int z_nrow,
                            const int* z_offsets, const int* z_row_counts, const int* z_columns, void* z_values,
                            int y_nrow, const int* x_offsets, const int* x_row_counts, const int* x_columns,
                            const void* x_values, const int* y_offsets, const int* y_row_counts, const int* y_columns,

@@ -1165,3 +1165,87 @@ def test_bsr_copy_scale_reference_scenario(device):
     pruned = sparse.bsr_from_triplets(C.nrow, C.ncol, C.uncompress_rows(), C.columns[: C.nnz], C.values[: C.nnz],
                                       prune_numerical_zeros=True)
     assert pruned.nnz_sync() == 0
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_bsr_mm_axpy_into_own_topology_vs_oracle(device, oracle_mod, dtype):
+    """Compact product / sum whose blocks all exist in the destination already
+    (the union topology is the destination's own): the rebuild is skipped and the
+    result must still equal the oracle's compact one bit for bit -- topology,
+    values, and kept zero blocks scaled by a negative beta (0 + beta * 0 is
+    +0.0 in the rebuild, not -0.0)."""
+    rng = np.random.default_rng(43)
+
+    def rand(nrow, ncol, nnz, bshape):
+        return _from_triplets_np(device, nrow, ncol, rng.integers(0, nrow, nnz), rng.integers(0, ncol, nnz),
+                                 rng.standard_normal((nnz, *bshape)).astype(dtype))
+
+    x, y = rand(120, 90, 600, (3, 3)), rand(90, 110, 500, (3, 3))
+    z = sparse.bsr_mm(x, y)                        # z holds exactly the product topology
+    nz = z.nnz_sync()
+    z.values[:nz] = torch.as_tensor(rng.standard_normal((nz, 3, 3)).astype(dtype), device=device)
+    ox, oy, oz = _as_dict(x), _as_dict(y), _as_dict(z)
+    sparse.bsr_mm(x, y, z, alpha=-0.75, beta=1.0)
+    ref = oracle_mod.bsr_mm(ox, oy, oz, alpha=-0.75, beta=1.0)
+    got = _as_dict(z)
+    np.testing.assert_array_equal(got["offsets"], ref["offsets"])
+    np.testing.assert_array_equal(got["columns"], ref["columns"])
+    assert got["values"].tobytes() == ref["values"].tobytes()
+    # a diagonal projector times the matrix (the Dirichlet projection's products)
+    d = rng.integers(0, 120, 30)
+    P = _from_triplets_np(device, 120, 120, d, d, rng.standard_normal((d.size, 3, 3)).astype(dtype))
+    A = rand(120, 120, 900, (3, 3))
+    C = sparse.bsr_copy(A)
+    oP, oA, oC = _as_dict(P), _as_dict(A), _as_dict(C)
+    sparse.bsr_mm(P, A, C, alpha=-1.0, beta=1.0)
+    ref = oracle_mod.bsr_mm(oP, oA, oC, alpha=-1.0, beta=1.0)
+    got = _as_dict(C)
+    np.testing.assert_array_equal(got["columns"], ref["columns"])
+    assert got["values"].tobytes() == ref["values"].tobytes()
+    oC = _as_dict(C)
+    sparse.bsr_mm(C, P, A, alpha=-1.0, beta=1.0)   # P block-diagonal: C P lies inside A's topology
+    ref = oracle_mod.bsr_mm(oC, oP, oA, alpha=-1.0, beta=1.0)
+    got = _as_dict(A)
+    np.testing.assert_array_equal(got["offsets"], ref["offsets"])
+    np.testing.assert_array_equal(got["columns"], ref["columns"])
+    assert got["values"].tobytes() == ref["values"].tobytes()
+    # sum of a matrix into the union topology it already has
+    w = rand(120, 110, 400, (3, 3))
+    u = sparse.bsr_axpy(w, sparse.bsr_copy(z), alpha=1.0, beta=1.0)   # union of w and z
+    nu = u.nnz_sync()
+    u.values[:nu] = torch.as_tensor(rng.standard_normal((nu, 3, 3)).astype(dtype), device=device)
+    u.values[:nu:7] = 0.0
+    ow, ou = _as_dict(w), _as_dict(u)
+    sparse.bsr_axpy(w, u, alpha=0.5, beta=-2.0)
+    ref = oracle_mod.bsr_axpy(ow, ou, alpha=0.5, beta=-2.0)
+    got = _as_dict(u)
+    np.testing.assert_array_equal(got["offsets"], ref["offsets"])
+    np.testing.assert_array_equal(got["columns"], ref["columns"])
+    assert got["values"].tobytes() == ref["values"].tobytes()
+
+
+def test_bsr_copy_same_shape_is_exact(device):
+    """A compact same-shape copy keeps the source's topology and copies the
+    values as (dest type)(scale * value), scale in the source type; a padded
+    source is compacted."""
+    rng = np.random.default_rng(47)
+    rows, cols = rng.integers(0, 80, 700), rng.integers(0, 60, 700)
+    A = _from_triplets_np(device, 80, 60, rows, cols, rng.standard_normal((700, 3, 3)).astype(np.float32))
+    n = A.nnz_sync()
+    A.values[3] = -0.0
+    C = sparse.bsr_copy(A)
+    assert C.nnz_sync() == n and C.row_counts is None
+    np.testing.assert_array_equal(C.offsets.cpu().numpy(), A.offsets[:81].cpu().numpy())
+    np.testing.assert_array_equal(C.columns[:n].cpu().numpy(), A.columns[:n].cpu().numpy())
+    assert C.values[:n].cpu().numpy().tobytes() == A.values[:n].cpu().numpy().tobytes()
+    D = sparse.bsr_copy(A * 2.5, scalar_type=wa.float64)
+    expect = (np.float32(2.5) * A.values[:n].cpu().numpy()).astype(np.float64)
+    assert D.values.dtype == torch.float64 and D.values[:n].cpu().numpy().tobytes() == expect.tobytes()
+    # padded source: active blocks only, compacted
+    P = sparse.bsr_copy(A, topology="padded")
+    P.row_counts = torch.clamp(P.row_counts - 1, min=0)
+    Q = sparse.bsr_copy(P)
+    dq, _ = _active_dense(Q)
+    dp, _ = _active_dense(P)
+    np.testing.assert_array_equal(dq, dp)
+    assert Q.row_counts is None and Q.nnz_sync() == int(P.row_counts.sum())

@@ -89,6 +89,9 @@ _SIGNATURES = {
     ),
     "warp_amd_scale_values": (c_int, [c_int, c_i64, c_ptr, c_f64, c_ptr]),
     "warp_amd_bsr_uncompress_rows": (c_int, [c_int, c_ptr, c_ptr, c_int, c_ptr, c_ptr]),
+    "warp_amd_rescale_values": (c_int, [c_int, c_i64, c_ptr, c_f64, c_ptr]),
+    "warp_amd_copy_scaled_values": (c_int, [c_int, c_int, c_i64, c_ptr, c_f64, c_ptr, c_ptr]),
+    "warp_amd_bsr_count_missing_blocks": (c_int, [c_i64] + [c_ptr] * 7),
     "warp_amd_bsr_add_blocks": (
         c_int, [c_int, c_int, c_i64, c_ptr, c_ptr, c_ptr, c_f64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]
     ),
@@ -100,7 +103,7 @@ _SIGNATURES = {
     "warp_amd_bsr_mm_list": (c_int, [c_int] * 4 + [c_ptr] * 9 + [c_ptr]),
     "warp_amd_bsr_mm_values": (
         c_int,
-        [c_int, c_int, c_int, c_int, c_f64, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_int]
+        [c_int, c_int, c_int, c_int, c_f64, c_f64, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_int]
         + [c_ptr] * 8 + [c_ptr],
     ),
     "warp_amd_bsr_diag_topology": (c_int, [c_int, c_int, c_ptr, c_ptr, c_ptr]),

@@ -418,13 +418,13 @@ __global__ __launch_bounds__(kBlock) void mm_list_kernel(int n, int base, int T,
 // Product values (reference _bsr_mm_compute_values): for every active z
 // block (r, c), acc = sum over the active x blocks k of row r, in storage
 // order, of x_k @ y(x_columns[k], c) for those y blocks that exist; then
-// z += alpha * acc.  One thread per z block.  Each block product entry is
+// z = (0 + beta * z) + alpha * acc (beta 1 leaves z's bits, -0 aside).  One thread per z block.  Each block product entry is
 // x[i][0] y[0][j] + x[i][1] y[1][j] + ... evaluated left to right with every
 // product and sum rounded (no fused multiply-add), so that the CPU oracle
 // reproduces the bits.
 template <typename S, int BR, int BK, int BC>
 __global__ __launch_bounds__(kBlock) void mm_values_kernel(
-    int T, int z_nrow, S alpha, const int* __restrict__ z_offsets, const int* __restrict__ z_row_counts,
+    int T, int z_nrow, S alpha, S beta, const int* __restrict__ z_offsets, const int* __restrict__ z_row_counts,
     const int* __restrict__ z_columns, S* __restrict__ z_values, int y_nrow, const int* __restrict__ x_offsets,
     const int* __restrict__ x_row_counts, const int* __restrict__ x_columns, const S* __restrict__ x_values,
     const int* __restrict__ y_offsets, const int* __restrict__ y_row_counts, const int* __restrict__ y_columns,
@@ -467,13 +467,13 @@ __global__ __launch_bounds__(kBlock) void mm_values_kernel(
     S* zv = z_values + (size_t)p * BR * BC;
 #pragma unroll
     for (int e = 0; e < BR * BC; ++e)
-        zv[e] = zv[e] + alpha * acc[e];
+        zv[e] = (S(0) + beta * zv[e]) + alpha * acc[e];
 }
 
 // Same, with block dimensions known at run time (br * bc <= 64).
 template <typename S>
 __global__ __launch_bounds__(kBlock) void mm_values_any_kernel(
-    int T, int z_nrow, S alpha, int BR, int BK, int BC, const int* __restrict__ z_offsets,
+    int T, int z_nrow, S alpha, S beta, int BR, int BK, int BC, const int* __restrict__ z_offsets,
     const int* __restrict__ z_row_counts, const int* __restrict__ z_columns, S* __restrict__ z_values, int y_nrow,
     const int* __restrict__ x_offsets, const int* __restrict__ x_row_counts, const int* __restrict__ x_columns,
     const S* __restrict__ x_values, const int* __restrict__ y_offsets, const int* __restrict__ y_row_counts,
@@ -511,7 +511,7 @@ __global__ __launch_bounds__(kBlock) void mm_values_any_kernel(
     }
     S* zv = z_values + (size_t)p * BR * BC;
     for (int e = 0; e < BR * BC; ++e)
-        zv[e] = zv[e] + alpha * acc[e];
+        zv[e] = (S(0) + beta * zv[e]) + alpha * acc[e];
 }
 
 // Block-diagonal topology: row r < ndiag holds column r.
@@ -648,6 +648,49 @@ template <typename S> __global__ __launch_bounds__(kBlock) void scale_kernel(int
         x[i] *= alpha;
 }
 
+// dst[i] = (DS)(scale * src[i]): the same-shape block copy of an assignment
+// whose destination topology is the source's own.
+template <typename SS, typename DS>
+__global__ __launch_bounds__(kBlock) void copy_scaled_kernel(int64_t n, const SS* __restrict__ src, SS scale,
+                                                            DS* __restrict__ dst)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n)
+        dst[i] = (DS)(scale * src[i]);
+}
+
+// x = 0 + beta * x: the bits a union-topology rebuild gives a kept block
+// (zeroed, then beta * old added), -0 included.
+template <typename S> __global__ __launch_bounds__(kBlock) void rescale_kernel(int64_t n, S* __restrict__ x, S beta)
+{
+#pragma clang fp contract(off)
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        const S zero = S(0);
+        x[i] = zero + beta * x[i];
+    }
+}
+
+// Number of pairs (rows[i], cols[i]), rows[i] >= 0, with no block in the
+// matrix, added to *missing (one atomic per workgroup).
+__global__ __launch_bounds__(kBlock) void count_missing_kernel(int64_t n, const int* __restrict__ rows,
+                                                              const int* __restrict__ cols,
+                                                              const int* __restrict__ offsets,
+                                                              const int* __restrict__ row_counts,
+                                                              const int* __restrict__ columns, int* __restrict__ missing)
+{
+    __shared__ int lds_count;
+    if (threadIdx.x == 0)
+        lds_count = 0;
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n && rows[i] >= 0 && find_block(offsets, row_counts, columns, rows[i], cols[i]) < 0)
+        atomicAdd(&lds_count, 1);
+    __syncthreads();
+    if (threadIdx.x == 0 && lds_count)
+        atomicAdd(missing, lds_count);
+}
+
 template <typename SS, typename DS>
 void launch_copy(unsigned grid, hipStream_t s, int T, Reblock rb, int nrow, const int* offsets, const int* row_counts,
                  const int* columns, int src_br, int src_bc, const void* src, double scale, const int* d_offsets,
@@ -772,6 +815,65 @@ WA_API int warp_amd_bsr_assign_padded(int src_scalar_size, int dest_scalar_size,
 }
 
 // x[0..n) *= alpha (reference bsr_scale's value kernel); fp32 or fp64.
+WA_API int warp_amd_copy_scaled_values(int src_scalar_size, int dst_scalar_size, int64_t n, const void* src,
+                                       double scale, void* dst, void* stream)
+{
+    WA_REQUIRE(n >= 0, "copy_scaled_values: invalid size");
+    WA_REQUIRE((src_scalar_size == 4 || src_scalar_size == 8) && (dst_scalar_size == 4 || dst_scalar_size == 8),
+               "copy_scaled_values: values must be fp32 or fp64");
+    if (n == 0)
+        return wa::kOk;
+    WA_REQUIRE(src && dst, "copy_scaled_values: null array");
+    const unsigned grid = (unsigned)wa::ceil_div(n, wa::kBlock);
+    hipStream_t s = wa::as_stream(stream);
+#define WA_COPY(SS, DS)                                                                                               \
+    wa::copy_scaled_kernel<SS, DS><<<grid, wa::kBlock, 0, s>>>(n, static_cast<const SS*>(src), (SS)scale,             \
+                                                               static_cast<DS*>(dst))
+    if (src_scalar_size == 4 && dst_scalar_size == 4)
+        WA_COPY(float, float);
+    else if (src_scalar_size == 4)
+        WA_COPY(float, double);
+    else if (dst_scalar_size == 4)
+        WA_COPY(double, float);
+    else
+        WA_COPY(double, double);
+#undef WA_COPY
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}
+
+WA_API int warp_amd_rescale_values(int scalar_size, int64_t n, void* x, double beta, void* stream)
+{
+    WA_REQUIRE(n >= 0, "rescale_values: invalid size");
+    WA_REQUIRE(scalar_size == 4 || scalar_size == 8, "rescale_values: values must be fp32 or fp64");
+    if (n == 0)
+        return wa::kOk;
+    WA_REQUIRE(x != nullptr, "rescale_values: null array");
+    const unsigned grid = (unsigned)wa::ceil_div(n, wa::kBlock);
+    hipStream_t s = wa::as_stream(stream);
+    if (scalar_size == 4)
+        wa::rescale_kernel<float><<<grid, wa::kBlock, 0, s>>>(n, static_cast<float*>(x), (float)beta);
+    else
+        wa::rescale_kernel<double><<<grid, wa::kBlock, 0, s>>>(n, static_cast<double*>(x), beta);
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}
+
+WA_API int warp_amd_bsr_count_missing_blocks(int64_t n, const int* rows, const int* cols, const int* offsets,
+                                             const int* row_counts, const int* columns, int* missing, void* stream)
+{
+    WA_REQUIRE(n >= 0 && missing, "count_missing_blocks: invalid arguments");
+    hipStream_t s = wa::as_stream(stream);
+    WA_TRY(wa::launch_fill_i32(missing, 1, 0, s));
+    if (n == 0)
+        return wa::kOk;
+    WA_REQUIRE(rows && cols && offsets && columns, "count_missing_blocks: null array");
+    wa::count_missing_kernel<<<(unsigned)wa::ceil_div(n, wa::kBlock), wa::kBlock, 0, s>>>(n, rows, cols, offsets,
+                                                                                         row_counts, columns, missing);
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}
+
 WA_API int warp_amd_scale_values(int scalar_size, int64_t n, void* x, double alpha, void* stream)
 {
     WA_REQUIRE(n >= 0, "scale_values: invalid size");
@@ -944,8 +1046,10 @@ WA_API int warp_amd_bsr_mm_list(int n, int base, int T, int x_nrow, const int* x
 // first T positions, z += alpha * sum over x row r's active blocks k (storage
 // order) of x_k @ y(x_columns[k], c).  Blocks: x br x bk, y bk x bc, z br x bc
 // (br * bc <= 64); fp32 or fp64.  No fused multiply-add: products and sums
-// are rounded one by one, in the order stated.
-WA_API int warp_amd_bsr_mm_values(int scalar_size, int br, int bk, int bc, double alpha, int T, int z_nrow,
+// are rounded one by one, in the order stated.  The old block enters as
+// 0 + beta * z (beta 1: a topology rebuild already scaled it).
+WA_API int warp_amd_bsr_mm_values(int scalar_size, int br, int bk, int bc, double alpha, double beta, int T,
+                                  int z_nrow,
                                   const int* z_offsets, const int* z_row_counts, const int* z_columns,
                                   void* z_values, int y_nrow, const int* x_offsets, const int* x_row_counts,
                                   const int* x_columns, const void* x_values, const int* y_offsets,
@@ -960,12 +1064,14 @@ WA_API int warp_amd_bsr_mm_values(int scalar_size, int br, int bk, int bc, doubl
     hipStream_t s = wa::as_stream(stream);
 #define WA_MM(S, BR, BK, BC)                                                                                          \
     wa::mm_values_kernel<S, BR, BK, BC><<<grid, wa::kBlock, 0, s>>>(                               \
-        T, z_nrow, (S)alpha, z_offsets, z_row_counts, z_columns, static_cast<S*>(z_values), y_nrow, x_offsets,       \
+        T, z_nrow, (S)alpha, (S)beta, z_offsets, z_row_counts, z_columns, static_cast<S*>(z_values), y_nrow,         \
+        x_offsets,                                                                                                    \
         x_row_counts, x_columns, static_cast<const S*>(x_values), y_offsets, y_row_counts, y_columns,                 \
         static_cast<const S*>(y_values))
 #define WA_MM_ANY(S)                                                                                                  \
     wa::mm_values_any_kernel<S><<<grid, wa::kBlock, 0, s>>>(                                       \
-        T, z_nrow, (S)alpha, br, bk, bc, z_offsets, z_row_counts, z_columns, static_cast<S*>(z_values), y_nrow,      \
+        T, z_nrow, (S)alpha, (S)beta, br, bk, bc, z_offsets, z_row_counts, z_columns, static_cast<S*>(z_values),     \
+        y_nrow,                                                                                                       \
         x_offsets, x_row_counts, x_columns, static_cast<const S*>(x_values), y_offsets, y_row_counts, y_columns,      \
         static_cast<const S*>(y_values))
     const bool s1 = br == 1 && bk == 1 && bc == 1;

@@ -1080,6 +1080,25 @@ def bsr_assign(dest: BsrMatrix, src, structure_only: bool = False, masked: bool 
         return
 
     T = _positions(src)
+    same_shape = src_subrows == src_subcols == dest_subrows == dest_subcols == 1
+    if topology != "masked" and same_shape and src.row_counts is None:
+        # compact source, same block shape: the rebuilt topology is the
+        # source's own (sorted unique rows), so it is copied instead
+        _bsr_ensure_fits(dest, nnz=T)
+        dest.offsets[: dest.nrow + 1].copy_(src.offsets[: src.nrow + 1])
+        dest.columns[:T].copy_(src.columns[:T])
+        dest.row_counts = None
+        dest._nnz_pending = False
+        if not structure_only:
+            for m in (src, dest):
+                if m.scalar_type.torch_dtype not in (torch.float32, torch.float64):
+                    raise TypeError("bsr_assign supports float32 and float64 values")
+            dest.values[T:].zero_()
+            _native.call(
+                "warp_amd_copy_scaled_values", src.scalar_type.size, dest.scalar_type.size, T * src.block_size,
+                _native.ptr(src.values), float(src_scale), _native.ptr(dest.values), stream,
+            )
+        return
     if topology != "masked":
         n = T * src_subrows * src_subcols
         dest_rows = torch.empty(n, dtype=torch.int32, device=dest.device)
@@ -1203,6 +1222,27 @@ def _add_blocks(dest: BsrMatrix, rows, cols, values, n: int, scale: float):
     )
 
 
+def _rescale_kept(dest: BsrMatrix, beta: float, n: int):
+    """Values of the first ``n`` positions := 0 + beta * values, the bits a
+    compact union rebuild gives the blocks it keeps."""
+    _native.call("warp_amd_rescale_values", dest.scalar_type.size, n * dest.block_size, _native.ptr(dest.values),
+                 float(beta), _native.stream_of(dest.device))
+
+
+def _union_is_own_topology(dest: BsrMatrix, rows, cols, n: int) -> bool:
+    """True when ``dest`` is compact and holds a block at every (rows[i],
+    cols[i]), i < n (rows[i] < 0 ignored): the compact union topology is then
+    ``dest``'s own and the rebuild can be skipped.  Reads one count back, so
+    never under graph capture (then False)."""
+    if dest.row_counts is not None or torch.cuda.is_current_stream_capturing():
+        return False
+    missing = torch.empty(1, dtype=torch.int32, device=dest.device)
+    _native.call("warp_amd_bsr_count_missing_blocks", n, _native.ptr(rows), _native.ptr(cols),
+                 _native.ptr(dest.offsets), None, _native.ptr(dest.columns), _native.ptr(missing),
+                 _native.stream_of(dest.device))
+    return int(missing.item()) == 0
+
+
 def _topology_from_triplets(dest: BsrMatrix, rows, cols, n: int):
     """Compact topology of the (row, col) pairs into ``dest`` (offsets, columns, pending nnz)."""
     _bsr_ensure_fits(dest, nnz=n)
@@ -1302,10 +1342,16 @@ def bsr_axpy(x, y: BsrMatrix | None = None, alpha=1.0, beta=1.0, masked: bool = 
     n = xT + yT
     rows = work._get("rows", n, torch.int32, device)
     cols = work._get("cols", n, torch.int32, device)
-    y.uncompress_rows(out=rows[:yT])
-    cols[:yT].copy_(y.columns[:yT])
     x.uncompress_rows(out=rows[yT:])
     cols[yT:].copy_(x.columns[:xT])
+    if _union_is_own_topology(y, rows[yT:], cols[yT:], xT):
+        # every block of x is already in y: same values as the rebuild below
+        # (zero, + beta * old, + alpha * x), without sorting y's topology again
+        _rescale_kept(y, beta, yT)
+        _add_blocks(y, rows[yT:], cols[yT:], x.values, xT, alpha)
+        return y
+    y.uncompress_rows(out=rows[:yT])
+    cols[:yT].copy_(y.columns[:yT])
     old_values = work._get("old_y_values", yT * y.block_size, y.values.dtype, device)
     old_values.copy_(y.values[:yT].reshape(-1))
     _topology_from_triplets(y, rows, cols, n)
@@ -1386,9 +1432,7 @@ def bsr_mm(x, y, z: BsrMatrix | None = None, alpha=1.0, beta=0.0, masked: bool =
             raise ValueError("topology='masked' is not supported for aliased inputs")
         if beta == 0.0:
             z.values.zero_()
-        else:
-            bsr_scale(z, beta)
-        _mm_values(x, y, z, alpha, _positions(z))
+        _mm_values(x, y, z, alpha, _positions(z), beta if beta != 0.0 else 1.0)
         return z
 
     # operands aliasing the output are read from copies
@@ -1433,17 +1477,22 @@ def bsr_mm(x, y, z: BsrMatrix | None = None, alpha=1.0, beta=0.0, masked: bool =
     n = zT + n_new
     rows = work._get("rows", n, torch.int32, device)
     cols = work._get("cols", n, torch.int32, device)
+    _native.call(
+        "warp_amd_bsr_mm_list", n_new, zT, xT, x.nrow, _native.ptr(x.offsets), _native.ptr(x.row_counts),
+        _native.ptr(x.columns), _native.ptr(y.offsets), _native.ptr(y.columns), _native.ptr(row_min),
+        _native.ptr(mm_offsets), _native.ptr(rows), _native.ptr(cols), stream,
+    )
+    if not padded and keep_z and max_new_nnz is None and _union_is_own_topology(z, rows[zT:], cols[zT:], n_new):
+        # every product block is already in z: the compact rebuild would give
+        # z's own topology back; same values (0 + beta * old, then the sums)
+        _mm_values(x, y, z, alpha, zT, beta)
+        return z
     old_values = None
     if zT > 0:
         z.uncompress_rows(out=rows[:zT])
         cols[:zT].copy_(z.columns[:zT])
         old_values = work._get("old_z_values", zT * z.block_size, z.values.dtype, device)
         old_values.copy_(z.values[:zT].reshape(-1))
-    _native.call(
-        "warp_amd_bsr_mm_list", n_new, zT, xT, x.nrow, _native.ptr(x.offsets), _native.ptr(x.row_counts),
-        _native.ptr(x.columns), _native.ptr(y.offsets), _native.ptr(y.columns), _native.ptr(row_min),
-        _native.ptr(mm_offsets), _native.ptr(rows), _native.ptr(cols), stream,
-    )
     if padded:
         status = z._ensure_status()
         _bsr_ensure_independent_row_counts(z)
@@ -1466,11 +1515,13 @@ def bsr_mm(x, y, z: BsrMatrix | None = None, alpha=1.0, beta=0.0, masked: bool =
     return z
 
 
-def _mm_values(x: BsrMatrix, y: BsrMatrix, z: BsrMatrix, alpha: float, T: int):
+def _mm_values(x: BsrMatrix, y: BsrMatrix, z: BsrMatrix, alpha: float, T: int, beta: float = 1.0):
+    """z = (0 + beta * z) + alpha * (x @ y) over z's first T positions."""
     br, bk = x.block_shape
     bc = y.block_shape[1]
     _native.call(
-        "warp_amd_bsr_mm_values", z.scalar_type.size, br, bk, bc, float(alpha), T, z.nrow, _native.ptr(z.offsets),
+        "warp_amd_bsr_mm_values", z.scalar_type.size, br, bk, bc, float(alpha), float(beta), T, z.nrow,
+        _native.ptr(z.offsets),
         _native.ptr(z.row_counts), _native.ptr(z.columns), _native.ptr(z.values), y.nrow, _native.ptr(x.offsets),
         _native.ptr(x.row_counts), _native.ptr(x.columns), _native.ptr(x.values), _native.ptr(y.offsets),
         _native.ptr(y.row_counts), _native.ptr(y.columns), _native.ptr(y.values), _native.stream_of(z.device),
