@@ -208,3 +208,19 @@ def test_cg_functor_reuse(device):
     np.testing.assert_array_equal(x.cpu().numpy(), x2.cpu().numpy())
     with pytest.raises(ValueError, match="Incompatible b.shape"):
         solver(b=torch.zeros(nd + 1, device=device))
+
+
+def test_cg_zero_rhs_and_identity(device):
+    """b = 0 converges before the first iteration with x = 0; the identity
+    converges in one iteration (textbook special cases)."""
+    n = 50
+    idx = torch.arange(n, dtype=torch.int32)
+    A = _bsr_from(device, n, idx, idx, torch.ones(n))
+    b = torch.zeros(n, device=device)
+    x = torch.zeros(n, device=device)
+    it, err, atol = linear.cg(A, b, x, tol=1e-6, maxiter=100)
+    assert it == 0 and err == 0.0 and torch.count_nonzero(x) == 0
+    b = torch.randn(n, device=device)
+    it, err, atol = linear.cg(A, b, x, tol=1e-6, maxiter=100, check_every=1)
+    assert it == 1 and err <= atol
+    torch.testing.assert_close(x, b, rtol=1e-6, atol=0)

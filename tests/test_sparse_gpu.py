@@ -1249,3 +1249,50 @@ def test_bsr_copy_same_shape_is_exact(device):
     dp, _ = _active_dense(P)
     np.testing.assert_array_equal(dq, dp)
     assert Q.row_counts is None and Q.nnz_sync() == int(P.row_counts.sum())
+
+
+def test_degenerate_inputs(device, oracle_mod):
+    """Empty and one-row cases: no triplets, a device count of zero, every
+    triplet out of range, one row of many duplicates, products and sums with an
+    empty operand, and bsr_mv of an empty matrix."""
+    e32 = torch.zeros(0, dtype=torch.int32, device=device)
+    A = sparse.bsr_from_triplets(7, 5, e32, e32, torch.zeros((0, 3, 3), device=device))
+    assert A.nnz_sync() == 0
+    np.testing.assert_array_equal(A.offsets.cpu().numpy(), np.zeros(8, np.int32))
+    x = torch.randn(15, device=device)
+    y = torch.full((21,), 3.0, device=device)
+    sparse.bsr_mv(A, x, y, alpha=1.0, beta=0.5)
+    np.testing.assert_array_equal(y.cpu().numpy(), np.full(21, 1.5, np.float32))
+    # a device count of zero over non-empty arrays
+    r = torch.tensor([0, 1, 2], dtype=torch.int32, device=device)
+    B = sparse.bsr_zeros(3, 3, wa.float32, device=device)
+    sparse.bsr_set_from_triplets(B, r, r, torch.ones(3, device=device), count=torch.zeros(1, dtype=torch.int32,
+                                                                                          device=device))
+    assert B.nnz_sync() == 0
+    # every triplet outside the matrix
+    bad = torch.tensor([-1, 3, 7, -5], dtype=torch.int32, device=device)
+    C = sparse.bsr_from_triplets(3, 3, bad, torch.tensor([0, 0, 1, 1], dtype=torch.int32, device=device),
+                                 torch.ones(4, device=device))
+    assert C.nnz_sync() == 0
+    # one row, 5000 duplicates of one block plus a second column: sums in triplet order
+    rng = np.random.default_rng(53)
+    rows = np.zeros(5000, np.int32)
+    cols = (rng.random(5000) < 0.3).astype(np.int32)
+    vals = rng.standard_normal((5000, 3, 3)).astype(np.float32)
+    D = _from_triplets_np(device, 1, 2, rows, cols, vals)
+    o = oracle_mod.bsr_from_triplets(1, 2, rows, cols, vals)
+    assert D.nnz_sync() == 2
+    np.testing.assert_array_equal(D.columns[:2].cpu().numpy(), o["columns"])
+    assert D.values[:2].cpu().numpy().tobytes() == o["values"].tobytes()
+    # products and sums with an empty operand
+    Z = sparse.bsr_zeros(1, 2, wa.mat33, device=device)
+    P = sparse.bsr_mm(D, sparse.bsr_zeros(2, 2, wa.mat33, device=device))
+    assert P.nnz_sync() == 0
+    S = sparse.bsr_axpy(Z, sparse.bsr_copy(D), alpha=2.0, beta=1.0)
+    assert S.nnz_sync() == 2 and S.values[:2].cpu().numpy().tobytes() == o["values"].tobytes()
+    T_ = sparse.bsr_transposed(A)
+    assert (T_.nrow, T_.ncol) == (5, 7) and T_.nnz_sync() == 0
+    # zero-row matrix
+    E = sparse.bsr_zeros(0, 4, wa.float32, device=device)
+    out = sparse.bsr_mv(E, torch.randn(4, device=device))
+    assert out.numel() == 0

@@ -1744,6 +1744,14 @@ def bsr_mv(
             int(A.nnz), _native.ptr(x_view), _native.ptr(y_view), alpha, beta, _native.stream_of(A.values.device),
         )
         return y
+    if int(A.nnz) == 0 and nrow > 0:
+        # no blocks (the matrix arrays may be empty): y = beta * y, y unread when beta is 0
+        if beta == 0.0:
+            y_view.zero_()
+        elif beta != 1.0:
+            _native.call("warp_amd_scale_values", y_view.element_size(), y_view.numel(), _native.ptr(y_view), beta,
+                         _native.stream_of(y_view.device))
+        return y
     fn = "warp_amd_bsr_mv_ex_" + sfx
     # tile_size > 0: lanes per block row; < 0: one lane per row (row-sequential
     # sums, the reference row kernel's order); 0: chosen from the mean row length
