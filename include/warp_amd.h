@@ -446,6 +446,43 @@ int warp_amd_cg_step_p_f32(int64_t n, const float* z, float* p, const double* rr
                            const double* rz_new, const double* rz_old, void* stream);
 int warp_amd_cg_step_p_f64(int64_t n, const double* z, double* p, const double* rr, const double* atol_sq,
                            const double* rz_new, const double* rz_old, void* stream);
+/* Batched solves (LinearOperator.batch_offsets, linear.py:33-131): scalar dofs
+ * [batch_offsets[b], batch_offsets[b + 1]) form subproblem b < nb; dofs
+ * outside [batch_offsets[0], batch_offsets[nb]) are inactive.
+ * warp_amd_seg_dot_plan computes chunk_start[0..nb] once per layout
+ * (workspace: warp_amd_seg_dot_plan_workspace_size); warp_amd_seg_dot_* then
+ * gives out[b] = sum over subproblem b of a[i] b[i] in FP64, each subproblem
+ * cut into chunks of 1024 dofs summed in a fixed order (replaces the segmented
+ * TiledDot, linear.py:309-577); partials holds warp_amd_seg_dot_max_chunks(n,
+ * nb) doubles, max_chunks being that value. */
+int warp_amd_seg_dot_max_chunks(int64_t n, int nb);
+size_t warp_amd_seg_dot_plan_workspace_size(int nb);
+int warp_amd_seg_dot_plan(int nb, const int* batch_offsets, int* chunk_start, void* workspace, size_t workspace_bytes,
+                          void* stream);
+int warp_amd_seg_dot_f32(int64_t n, int nb, const int* batch_offsets, const int* chunk_start, int max_chunks,
+                         const float* a, const float* b, double* partials, double* out, void* stream);
+int warp_amd_seg_dot_f64(int64_t n, int nb, const int* batch_offsets, const int* chunk_start, int max_chunks,
+                         const double* a, const double* b, double* partials, double* out, void* stream);
+/* atol_sq[b] = max(rtol * sqrt(bb[b]), atol)^2 per subproblem
+ * (_initialize_tolerance, linear.py:2051-2083). */
+int warp_amd_cg_tolerance_batched(int nb, const double* bb, double tol, double atol, int is_f64, double* atol_sq,
+                                  void* stream);
+/* _cg_kernel_1 / _cg_kernel_2 with batches (linear.py:2332-2373): entry e of
+ * dpe scalars belongs to the subproblem of its first dof (_find_entry_batch,
+ * linear.py:2034-2048) and uses that subproblem's alpha / beta; inactive
+ * entries are left unchanged. */
+int warp_amd_cg_step_xr_batched_f32(int64_t nent, int dpe, int nb, const int* batch_offsets, float* x, float* r,
+                                    const float* p, const float* Ap, const double* rr, const double* atol_sq,
+                                    const double* rz, const double* pAp, void* stream);
+int warp_amd_cg_step_xr_batched_f64(int64_t nent, int dpe, int nb, const int* batch_offsets, double* x, double* r,
+                                    const double* p, const double* Ap, const double* rr, const double* atol_sq,
+                                    const double* rz, const double* pAp, void* stream);
+int warp_amd_cg_step_p_batched_f32(int64_t nent, int dpe, int nb, const int* batch_offsets, const float* z, float* p,
+                                   const double* rr, const double* atol_sq, const double* rz_new,
+                                   const double* rz_old, void* stream);
+int warp_amd_cg_step_p_batched_f64(int64_t nent, int dpe, int nb, const int* batch_offsets, const double* z,
+                                   double* p, const double* rr, const double* atol_sq, const double* rz_new,
+                                   const double* rz_old, void* stream);
 /* z = alpha (d o x) + beta y (_diag_mv_kernel, linear.py:2271-2287) */
 int warp_amd_diag_mv_f32(int64_t n, const float* d, const float* x, const float* y, float* z, float alpha, float beta,
                          void* stream);

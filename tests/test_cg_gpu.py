@@ -224,3 +224,101 @@ def test_cg_zero_rhs_and_identity(device):
     it, err, atol = linear.cg(A, b, x, tol=1e-6, maxiter=100, check_every=1)
     assert it == 1 and err <= atol
     torch.testing.assert_close(x, b, rtol=1e-6, atol=0)
+
+
+def _batch_offsets(sizes, device):
+    return torch.tensor(np.concatenate([[0], np.cumsum(sizes)]), dtype=torch.int32, device=device)
+
+
+def _spd_np(n, seed, dtype):
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = (q * rng.uniform(1.0, 10.0, n)) @ q.T
+    return A.astype(dtype), rng.standard_normal(n).astype(dtype)
+
+
+@pytest.mark.parametrize("sizes", [[20, 20, 20, 20], [8, 15, 10, 12]])
+def test_batched_cg_dense(device, sizes):
+    """test_batched_cg / test_batched_nonuniform (test_linear_solvers.py:451-489):
+    block-diagonal SPD system, one subproblem per block; each block's residual
+    within 32 tol |b_i| (FP32)."""
+    rows = sum(sizes)
+    A = np.zeros((rows, rows), np.float32)
+    b = np.zeros(rows, np.float32)
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    for i, n in enumerate(sizes):
+        A[off[i]:off[i + 1], off[i]:off[i + 1]], b[off[i]:off[i + 1]] = _spd_np(n, i, np.float32)
+    op = linear.aslinearoperator(torch.tensor(A, device=device), batch_offsets=_batch_offsets(sizes, device),
+                                 max_batch_length=max(sizes))
+    assert op.batch_count == len(sizes) and op.max_batch_length == max(sizes)
+    x = torch.zeros(rows, device=device)
+    it, err, atol = linear.cg(op, torch.tensor(b, device=device), x, tol=1e-5, maxiter=1000)
+    assert err <= atol
+    xn = x.cpu().numpy().astype(np.float64)
+    for i in range(len(sizes)):
+        sl = slice(off[i], off[i + 1])
+        res = np.linalg.norm(A[sl, sl].astype(np.float64) @ xn[sl] - b[sl])
+        assert res <= 32 * 1e-5 * np.linalg.norm(b[sl].astype(np.float64)), i
+
+
+def test_batched_cg_vector_entries_and_inactive_tail(device):
+    """test_batched_vector_offsets / _inactive_tail / _vector_inactive_tail
+    (test_linear_solvers.py:492-577): diagonal operators, entries of two scalars
+    assigned to the subproblem of their first dof, inactive dofs untouched."""
+    d = torch.tensor([2.0, 2.0, 5.0, 5.0], device=device)
+    b = torch.tensor([[2.0, 4.0], [10.0, 15.0]], device=device)
+    x = torch.zeros_like(b)
+    op = linear.aslinearoperator(d, batch_offsets=_batch_offsets([2, 2], device), max_batch_length=2)
+    linear.cg(op, b, x, tol=1e-7, maxiter=4, check_every=1, use_cuda_graph=False)
+    np.testing.assert_allclose(x.cpu().numpy(), [[1.0, 2.0], [2.0, 3.0]], rtol=1e-5, atol=1e-5)
+    # one subproblem over the first three dofs of five
+    d = torch.tensor([2.0, 3.0, 4.0, 5.0, 6.0], dtype=torch.float64, device=device)
+    b = torch.tensor([2.0, 6.0, 12.0, 100.0, 200.0], dtype=torch.float64, device=device)
+    x = torch.tensor([0.0, 0.0, 0.0, 7.0, 8.0], dtype=torch.float64, device=device)
+    op = linear.aslinearoperator(d, batch_offsets=_batch_offsets([3], device), max_batch_length=3)
+    it, err, atol = linear.cg(op, b, x, tol=1e-8, maxiter=20, check_every=1)
+    assert it > 1 and err <= atol
+    np.testing.assert_allclose(x.cpu().numpy(), [1.0, 2.0, 3.0, 7.0, 8.0], rtol=1e-7, atol=1e-7)
+    # two-scalar entries, the last entry inactive
+    d = torch.tensor([2.0, 3.0, 4.0, 5.0, 6.0, 7.0], dtype=torch.float64, device=device)
+    b = torch.tensor([[2.0, 6.0], [12.0, 20.0], [100.0, 200.0]], dtype=torch.float64, device=device)
+    x = torch.tensor([[0.0, 0.0], [0.0, 0.0], [7.0, 8.0]], dtype=torch.float64, device=device)
+    op = linear.aslinearoperator(d, batch_offsets=_batch_offsets([2, 2], device), max_batch_length=2)
+    _, err, atol = linear.cg(op, b, x, tol=1e-8, maxiter=8, check_every=1, use_cuda_graph=False)
+    assert err <= atol
+    np.testing.assert_allclose(x.cpu().numpy(), [[1.0, 2.0], [3.0, 4.0], [7.0, 8.0]], rtol=1e-7, atol=1e-7)
+
+
+def test_batched_cg_bsr_matches_single_solves(device):
+    """Three 2D Poisson systems stacked block-diagonally in one BSR matrix,
+    one subproblem each, Jacobi-preconditioned: every subproblem's solution
+    equals its own single-system solve (the subproblems only differ in the FP64
+    dot grouping) and the device-side results hold one value per subproblem."""
+    mats = []
+    for k, n in enumerate((12, 16, 20)):
+        rows, cols, vals, b, nd = problems.poisson2d_q1(n, k)
+        mats.append((rows, cols, vals, b, nd))
+    off = np.concatenate([[0], np.cumsum([m[4] for m in mats])])
+    R = np.concatenate([m[0] + off[i] * (m[0] >= 0) for i, m in enumerate(mats)]).astype(np.int32)
+    C = np.concatenate([m[1] + off[i] * (m[1] >= 0) for i, m in enumerate(mats)]).astype(np.int32)
+    R = np.where(np.concatenate([m[0] for m in mats]) < 0, -1, R)
+    C = np.where(np.concatenate([m[1] for m in mats]) < 0, -1, C)
+    V = np.concatenate([m[2] for m in mats])
+    B = np.concatenate([m[3] for m in mats])
+    ntot = int(off[-1])
+    A = _bsr_from(device, ntot, torch.from_numpy(R), torch.from_numpy(C), torch.from_numpy(V))
+    M = linear.preconditioner(A, "diag")
+    op = linear.aslinearoperator(A, batch_offsets=torch.tensor(off, dtype=torch.int32, device=device))
+    x = torch.zeros(ntot, device=device)
+    it, err, atol = linear.cg(op, torch.from_numpy(B).to(device), x, tol=1e-6, maxiter=2000, M=M)
+    assert err <= atol
+    for i, (rows, cols, vals, b, nd) in enumerate(mats):
+        Ai = _bsr_from(device, nd, torch.from_numpy(rows), torch.from_numpy(cols), torch.from_numpy(vals))
+        xi = torch.zeros(nd, device=device)
+        linear.cg(Ai, torch.from_numpy(b).to(device), xi, tol=1e-6, maxiter=2000, M=linear.preconditioner(Ai, "diag"))
+        xs = x[off[i]:off[i + 1]].cpu().numpy()
+        ref = xi.cpu().numpy()
+        assert np.linalg.norm(xs - ref) <= 1e-4 * np.linalg.norm(ref), i
+    x.zero_()
+    its, rr, at = linear.cg(op, torch.from_numpy(B).to(device), x, tol=1e-6, maxiter=400, M=M, check_every=0)
+    assert rr.shape == (3,) and at.shape == (3,) and bool((rr <= at).all())

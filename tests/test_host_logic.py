@@ -197,3 +197,18 @@ def test_public_names_match_reference_module():
     assert sparse.bsr_matrix_t(wa.float64) is sparse.BsrMatrix
     with pytest.raises(ValueError, match="matrix or scalar"):
         sparse.bsr_matrix_t(str)
+
+
+def test_batched_host_loop_per_batch_tolerance():
+    """test_batched_host_loop_per_batch_tolerance (test_linear_solvers.py:423-448):
+    the loop stops when every subproblem meets its own tolerance."""
+    r_norm_sq = torch.tensor([25.0, 4.0])
+    atol_sq = torch.tensor([100.0, 1.0])
+    cycles = []
+
+    def do_cycle():
+        cycles.append(1)
+        r_norm_sq.copy_(torch.tensor([25.0, 0.25]))
+
+    niter, err, atol = linear._host_loop(do_cycle, 1, r_norm_sq, atol_sq, 3, None, 1)
+    assert niter == 1 and len(cycles) == 1 and err <= atol

@@ -120,9 +120,14 @@ def test_triplets_known_answers(device):
     assert A.nnz == 4
     np.testing.assert_array_equal(A.offsets.cpu().numpy(), [0, 1, 2])
     np.testing.assert_array_equal(A.uncompress_rows().cpu().numpy(), [0, 1, -1, -1])
+    copied = sparse.bsr_copy(A, scalar_type=wa.float64)   # before the sync: stale upper bound
     assert A.nnz_sync() == 2
     assert A.columns.numel() == 4 and A.values.shape[0] == 4
     np.testing.assert_array_equal(A.values[:2].cpu().numpy(), [3.0, 7.0])
+    assert copied.nnz_sync() == 2
+    np.testing.assert_array_equal(copied.columns[:2].cpu().numpy(), [0, 1])
+    np.testing.assert_array_equal(copied.values[:2].cpu().numpy(), np.array([3.0, 7.0]))
+    assert copied.values.dtype == torch.float64
     # topology only (test_sparse.py:164-172)
     T = sparse.bsr_zeros(2, 3, wa.float32, device=device)
     sparse.bsr_set_from_triplets(T, _t([1, 0, 1], device, torch.int32), _t([2, 1, 2], device, torch.int32), values=None)

@@ -168,6 +168,10 @@ _SIGNATURES = {
     "warp_amd_cg_state_offset": (c_size, [c_int]),
     "warp_amd_dot_workspace_size": (c_size, [c_i64]),
     "warp_amd_cg_tolerance": (c_int, [c_ptr, c_f64, c_f64, c_int, c_ptr, c_ptr]),
+    "warp_amd_seg_dot_max_chunks": (c_int, [c_i64, c_int]),
+    "warp_amd_seg_dot_plan_workspace_size": (c_size, [c_int]),
+    "warp_amd_seg_dot_plan": (c_int, [c_int, c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
+    "warp_amd_cg_tolerance_batched": (c_int, [c_int, c_ptr, c_f64, c_f64, c_int, c_ptr, c_ptr]),
 }
 
 for _sfx, _s in (("f32", c_f32), ("f64", c_f64)):
@@ -186,6 +190,9 @@ for _sfx, _s in (("f32", c_f32), ("f64", c_f64)):
             f"warp_amd_dot_{_sfx}": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
             f"warp_amd_cg_step_xr_{_sfx}": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]),
             f"warp_amd_cg_step_p_{_sfx}": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]),
+            f"warp_amd_seg_dot_{_sfx}": (c_int, [c_i64, c_int, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]),
+            f"warp_amd_cg_step_xr_batched_{_sfx}": (c_int, [c_i64, c_int, c_int] + [c_ptr] * 10),
+            f"warp_amd_cg_step_p_batched_{_sfx}": (c_int, [c_i64, c_int, c_int] + [c_ptr] * 8),
             f"warp_amd_diag_mv_{_sfx}": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr, _s, _s, c_ptr]),
             f"warp_amd_dense_mv_{_sfx}": (c_int, [c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, _s, _s, c_ptr]),
         }

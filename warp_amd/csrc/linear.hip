@@ -655,6 +655,145 @@ __global__ void tolerance_kernel(const double* bb, double rtol, double atol, dou
     *atol_sq = a * a;
 }
 
+// ---------------------------------------------------------------------------
+// Batched solves (linear.py:33-131, 2010-2373 with batch_offsets): scalar dofs
+// [off[b], off[b + 1]) form subproblem b < nb; dofs below off[0] or at or
+// beyond off[nb] are inactive.  Dots are per subproblem; alpha / beta and
+// the convergence test are per subproblem too.
+// ---------------------------------------------------------------------------
+
+// Dofs per workgroup of a segmented dot.  Chunks never cross a subproblem
+// boundary, so each subproblem's partials are contiguous and summed in a
+// fixed order: the results do not depend on timing.
+constexpr int kSegChunk = 4 * kBlock;
+
+__global__ __launch_bounds__(kBlock) void seg_chunk_counts_kernel(int nb, const int* __restrict__ off,
+                                                                 int* __restrict__ counts)
+{
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    if (b < nb) {
+        const int len = off[b + 1] - off[b];
+        counts[b] = len > 0 ? (len + kSegChunk - 1) / kSegChunk : 0;
+    }
+}
+
+// Subproblem of chunk w: the last b with chunk_start[b] <= w.
+__device__ __forceinline__ int chunk_batch(int w, int nb, const int* chunk_start)
+{
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (chunk_start[mid] <= w)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void seg_dot_partials_kernel(int nb, const int* __restrict__ off,
+                                                                 const int* __restrict__ chunk_start,
+                                                                 const S* __restrict__ a, const S* __restrict__ c,
+                                                                 double* __restrict__ partials)
+{
+    __shared__ double lds[kBlock / kWave];
+    const int w = blockIdx.x;
+    if (w >= chunk_start[nb])
+        return;  // workgroup-uniform
+    const int bt = chunk_batch(w, nb, chunk_start);
+    const int64_t beg = off[bt] + (int64_t)(w - chunk_start[bt]) * kSegChunk;
+    const int64_t end = min(beg + (int64_t)kSegChunk, (int64_t)off[bt + 1]);
+    double acc = 0.0;
+    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
+        acc += (double)a[i] * (double)c[i];
+    const double sum = block_sum<kBlock>(acc, lds);
+    if (threadIdx.x == 0)
+        partials[w] = sum;
+}
+
+// out[b] = sum of subproblem b's partials, one workgroup per subproblem.
+__global__ __launch_bounds__(kBlock) void seg_dot_finalize_kernel(const int* __restrict__ chunk_start,
+                                                                 const double* __restrict__ partials,
+                                                                 double* __restrict__ out)
+{
+    __shared__ double lds[kBlock / kWave];
+    const int bt = blockIdx.x;
+    const int c0 = chunk_start[bt], c1 = chunk_start[bt + 1];
+    double acc = 0.0;
+    for (int k = c0 + (int)threadIdx.x; k < c1; k += kBlock)
+        acc += partials[k];
+    const double sum = block_sum<kBlock>(acc, lds);
+    if (threadIdx.x == 0)
+        out[bt] = sum;
+}
+
+// Subproblem of entry e (dpe scalars; the entry's first dof decides, as the
+// reference's _find_entry_batch does), -1 when inactive.
+__device__ __forceinline__ int entry_batch(int64_t e, int dpe, int nb, const int* off)
+{
+    const int64_t dof = e * dpe;
+    if (dof < off[0] || dof >= off[nb])
+        return -1;
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (off[mid] <= dof)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void step_xr_batched_kernel(
+    int64_t nent, int dpe, int nb, const int* __restrict__ off, S* __restrict__ x, S* __restrict__ r,
+    const S* __restrict__ p, const S* __restrict__ Ap, const double* rr, const double* atol_sq, const double* rz,
+    const double* pAp
+)
+{
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nent; e += (int64_t)gridDim.x * kBlock) {
+        const int bt = entry_batch(e, dpe, nb, off);
+        if (bt < 0)
+            continue;
+        const S alpha = (S)(rr[bt] > atol_sq[bt] ? rz[bt] / pAp[bt] : 0.0);
+        for (int k = 0; k < dpe; ++k) {
+            const int64_t i = e * dpe + k;
+            x[i] = fma(alpha, p[i], x[i]);
+            r[i] = fma(-alpha, Ap[i], r[i]);
+        }
+    }
+}
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void step_p_batched_kernel(
+    int64_t nent, int dpe, int nb, const int* __restrict__ off, const S* __restrict__ z, S* __restrict__ p,
+    const double* rr, const double* atol_sq, const double* rz_new, const double* rz_old
+)
+{
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nent; e += (int64_t)gridDim.x * kBlock) {
+        const int bt = entry_batch(e, dpe, nb, off);
+        if (bt < 0)
+            continue;
+        const S beta = (S)(rr[bt] > atol_sq[bt] ? rz_new[bt] / rz_old[bt] : 0.0);
+        for (int k = 0; k < dpe; ++k) {
+            const int64_t i = e * dpe + k;
+            p[i] = fma(beta, p[i], z[i]);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void tolerance_batched_kernel(int nb, const double* bb, double rtol,
+                                                                  double atol, double* atol_sq)
+{
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    if (b < nb) {
+        const double a = fmax(rtol * sqrt(bb[b]), atol);
+        atol_sq[b] = a * a;
+    }
+}
+
 // z = alpha * (d o x) + beta * y (linear.py:2271-2287)
 template <typename S>
 __global__ __launch_bounds__(kBlock) void diag_mv_kernel(
@@ -777,6 +916,51 @@ WA_API size_t warp_amd_cg_state_offset(int which)
         WA_LAUNCH_CHECK();                                                                                            \
         return wa::kOk;                                                                                               \
     }                                                                                                                 \
+    WA_API int warp_amd_seg_dot_##SUFFIX(                                                                             \
+        int64_t n, int nb, const int* batch_offsets, const int* chunk_start, int max_chunks, const S* a, const S* b,  \
+        double* partials, double* out, void* stream                                                                   \
+    )                                                                                                                 \
+    {                                                                                                                 \
+        WA_REQUIRE(nb > 0 && max_chunks >= 0 && batch_offsets && chunk_start && partials && out,                      \
+                   "seg_dot: invalid arguments");                                                                     \
+        hipStream_t s = wa::as_stream(stream);                                                                        \
+        if (max_chunks > 0) {                                                                                         \
+            wa::seg_dot_partials_kernel<S><<<(unsigned)max_chunks, wa::kBlock, 0, s>>>(nb, batch_offsets,            \
+                                                                                      chunk_start, a, b, partials);   \
+            WA_LAUNCH_CHECK();                                                                                        \
+        }                                                                                                             \
+        wa::seg_dot_finalize_kernel<<<(unsigned)nb, wa::kBlock, 0, s>>>(chunk_start, partials, out);                  \
+        WA_LAUNCH_CHECK();                                                                                            \
+        return wa::kOk;                                                                                               \
+    }                                                                                                                 \
+    WA_API int warp_amd_cg_step_xr_batched_##SUFFIX(                                                                  \
+        int64_t nent, int dpe, int nb, const int* batch_offsets, S* x, S* r, const S* p, const S* Ap,                 \
+        const double* rr, const double* atol_sq, const double* rz, const double* pAp, void* stream                    \
+    )                                                                                                                 \
+    {                                                                                                                 \
+        WA_REQUIRE(nb > 0 && dpe > 0 && batch_offsets, "cg_step_xr_batched: invalid arguments");                     \
+        if (nent <= 0)                                                                                                \
+            return wa::kOk;                                                                                           \
+        wa::step_xr_batched_kernel<S><<<wa::vec_grid(nent), wa::kBlock, 0, wa::as_stream(stream)>>>(                 \
+            nent, dpe, nb, batch_offsets, x, r, p, Ap, rr, atol_sq, rz, pAp                                          \
+        );                                                                                                            \
+        WA_LAUNCH_CHECK();                                                                                            \
+        return wa::kOk;                                                                                               \
+    }                                                                                                                 \
+    WA_API int warp_amd_cg_step_p_batched_##SUFFIX(                                                                   \
+        int64_t nent, int dpe, int nb, const int* batch_offsets, const S* z, S* p, const double* rr,                  \
+        const double* atol_sq, const double* rz_new, const double* rz_old, void* stream                               \
+    )                                                                                                                 \
+    {                                                                                                                 \
+        WA_REQUIRE(nb > 0 && dpe > 0 && batch_offsets, "cg_step_p_batched: invalid arguments");                      \
+        if (nent <= 0)                                                                                                \
+            return wa::kOk;                                                                                           \
+        wa::step_p_batched_kernel<S><<<wa::vec_grid(nent), wa::kBlock, 0, wa::as_stream(stream)>>>(                  \
+            nent, dpe, nb, batch_offsets, z, p, rr, atol_sq, rz_new, rz_old                                           \
+        );                                                                                                            \
+        WA_LAUNCH_CHECK();                                                                                            \
+        return wa::kOk;                                                                                               \
+    }                                                                                                                 \
     WA_API int warp_amd_diag_mv_##SUFFIX(                                                                             \
         int64_t n, const S* d, const S* x, const S* y, S* z, S alpha, S beta, void* stream                            \
     )                                                                                                                 \
@@ -805,6 +989,46 @@ WA_DEFINE_CG(f32, float)
 WA_DEFINE_CG(f64, double)
 
 WA_API size_t warp_amd_dot_workspace_size(int64_t n) { return sizeof(double) * wa::vec_grid(n); }
+
+// Chunks a segmented dot over n active dofs in nb subproblems can use.
+WA_API int warp_amd_seg_dot_max_chunks(int64_t n, int nb)
+{
+    return (int)(wa::ceil_div(n, wa::kSegChunk) + nb);
+}
+
+WA_API size_t warp_amd_seg_dot_plan_workspace_size(int nb)
+{
+    return wa::align_up(sizeof(int) * (size_t)(nb + 1), 256) + wa::scan_workspace_bytes((int64_t)nb + 1);
+}
+
+// chunk_start[0..nb] = exclusive scan of the chunks per subproblem
+// (chunk_start[nb] = total); computed once per batch layout.
+WA_API int warp_amd_seg_dot_plan(int nb, const int* batch_offsets, int* chunk_start, void* workspace,
+                                 size_t workspace_bytes, void* stream)
+{
+    WA_REQUIRE(nb > 0 && batch_offsets && chunk_start, "seg_dot_plan: invalid arguments");
+    WA_REQUIRE(workspace && workspace_bytes >= warp_amd_seg_dot_plan_workspace_size(nb),
+               "seg_dot_plan: workspace too small");
+    hipStream_t s = wa::as_stream(stream);
+    int* counts = static_cast<int*>(workspace);
+    void* scan_ws = static_cast<char*>(workspace) + wa::align_up(sizeof(int) * (size_t)(nb + 1), 256);
+    wa::seg_chunk_counts_kernel<<<(unsigned)wa::ceil_div(nb, wa::kBlock), wa::kBlock, 0, s>>>(nb, batch_offsets,
+                                                                                            counts);
+    WA_LAUNCH_CHECK();
+    return wa::exclusive_scan_i32(counts, chunk_start, nb, scan_ws, s);
+}
+
+WA_API int warp_amd_cg_tolerance_batched(int nb, const double* bb, double tol, double atol, int is_f64,
+                                         double* atol_sq, void* stream)
+{
+    WA_REQUIRE(nb > 0 && bb && atol_sq, "cg_tolerance_batched: invalid arguments");
+    double rtol_v, atol_v;
+    wa::get_tolerances(is_f64 != 0, tol, atol, &rtol_v, &atol_v);
+    wa::tolerance_batched_kernel<<<(unsigned)wa::ceil_div(nb, wa::kBlock), wa::kBlock, 0, wa::as_stream(stream)>>>(
+        nb, bb, rtol_v, atol_v, atol_sq);
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}
 
 WA_API int warp_amd_cg_tolerance(const double* bb, double tol, double atol, int is_f64, double* atol_sq, void* stream)
 {

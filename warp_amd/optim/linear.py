@@ -292,8 +292,6 @@ class LinearSolverState:
         self._device = self._A.device
         self._scalar_type = self._A.scalar_type
         self._batch_count = self._A.batch_count
-        if self._batch_count != 1:
-            raise NotImplementedError("batched solves are not provided by warp_amd")
         if maxiter is None or maxiter == 0:
             maxiter = _scalar_dof_count(b) // self._batch_count
         self._maxiter = int(maxiter)
@@ -377,11 +375,12 @@ class CG(LinearSolverState):
         self._ws_bytes = lib.warp_amd_cg_workspace_size(n, self._b.element_size())
         self._ws = torch.zeros(self._ws_bytes, dtype=torch.uint8, device=self._b.device)
         self._dot_ws = None
+        self._batch_plan = None
 
     # -- native route ------------------------------------------------------
     def _native_ok(self, A: LinearOperator, b, x, M) -> bool:
         bsr = A._bsr
-        if bsr is None or bsr.row_counts is not None:
+        if bsr is None or bsr.row_counts is not None or A.batch_offsets is not None:
             return False
         br, bc = bsr.block_shape
         if br != bc or br not in (1, 3):
@@ -392,9 +391,14 @@ class CG(LinearSolverState):
             return False
         return b.is_contiguous() and x.is_contiguous() and b.dtype == x.dtype == bsr.scalar_type.torch_dtype
 
+    def _run_batched(self, A, b, x, M):
+        return _batched_cg(self, A, b, x, M)
+
     def _run(self, A, b, x, M):
         if self._native_ok(A, b, x, M):
             return self._run_native(A, b, x, M)
+        if A.batch_offsets is not None:
+            return self._run_batched(A, b, x, M)
         return self._run_generic(A, b, x, M)
 
     def _run_native(self, A, b, x, M):
@@ -558,16 +562,130 @@ class CG(LinearSolverState):
         return _host_loop(do_cycle, cycle_size, rr, atol_sq, maxiter, self._callback, check_every)
 
 
+def _batched_cg(state, A, b, x, M):
+    """CG over the subproblems of ``A.batch_offsets`` (linear.py:893-1033 with
+    batches): per-subproblem dots, tolerances, alpha / beta and freezing;
+    inactive dofs are left unchanged."""
+    dev = b.device
+    sfx = _sfx(b.dtype)
+    n = _scalar_dof_count(b)
+    nent = b.shape[0] if b.dim() > 0 else 1
+    dpe = n // max(nent, 1)
+    stream = _native.stream_of(dev)
+    lib = _native.lib()
+    offs = A.batch_offsets
+    if offs.dtype != torch.int32 or offs.device != dev or not offs.is_contiguous():
+        raise TypeError("batch_offsets must be a contiguous int32 array on the solver's device")
+    nb = offs.numel() - 1
+    if nb < 1:
+        raise ValueError("batch_offsets must hold at least two entries")
+    plan = state._batch_plan
+    if plan is None or plan[0] is not offs:
+        chunk_start = torch.empty(nb + 1, dtype=torch.int32, device=dev)
+        ws_bytes = lib.warp_amd_seg_dot_plan_workspace_size(nb)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+        _native.call("warp_amd_seg_dot_plan", nb, _native.ptr(offs), _native.ptr(chunk_start), _native.ptr(ws),
+                     ws_bytes, stream)
+        max_chunks = lib.warp_amd_seg_dot_max_chunks(n, nb)
+        partials = torch.empty(max(max_chunks, 1), dtype=torch.float64, device=dev)
+        vecs = torch.zeros((4, n), dtype=b.dtype, device=dev)
+        scal = torch.zeros((6, nb), dtype=torch.float64, device=dev)
+        plan = state._batch_plan = (offs, chunk_start, max_chunks, partials, vecs, scal)
+    _, chunk_start, max_chunks, partials, vecs, scal = plan
+    r, z, p, Ap = vecs[0], vecs[1], vecs[2], vecs[3]
+    bf = b.reshape(-1)
+    xf = x.view(-1)
+    bb, atol_sq, rr, rz0, rz1, pAp = (scal[i] for i in range(6))
+    rz = [rz0, rz1]
+
+    def dot(a, c, out):
+        _native.call(f"warp_amd_seg_dot_{sfx}", n, nb, _native.ptr(offs), _native.ptr(chunk_start), max_chunks,
+                     _native.ptr(a), _native.ptr(c), _native.ptr(partials), _native.ptr(out), stream)
+
+    nan = float("nan")
+    dot(bf, bf, bb)
+    _native.call("warp_amd_cg_tolerance_batched", nb, _native.ptr(bb),
+                 nan if state._tol is None else float(state._tol), nan if state._atol is None else float(state._atol),
+                 1 if b.dtype == torch.float64 else 0, _native.ptr(atol_sq), stream)
+    Ap.zero_()
+    z.zero_()
+    cur = [0]
+
+    def update_rr_rz(slot):
+        if M is None:
+            dot(r, r, rr)
+            dot(r, r, rz[slot])
+        else:
+            M.matvec(r, z, z, 1.0, 0.0)
+            dot(r, r, rr)
+            dot(r, z, rz[slot])
+
+    A.matvec(xf, bf, r, -1.0, 1.0)
+    update_rr_rz(cur[0])
+    p.copy_(r if M is None else z)
+
+    def do_iteration(update_direction=True):
+        A.matvec(p, Ap, Ap, 1.0, 0.0)
+        dot(p, Ap, pAp)
+        _native.call(f"warp_amd_cg_step_xr_batched_{sfx}", nent, dpe, nb, _native.ptr(offs), _native.ptr(xf),
+                     _native.ptr(r), _native.ptr(p), _native.ptr(Ap), _native.ptr(rr), _native.ptr(atol_sq),
+                     _native.ptr(rz[cur[0]]), _native.ptr(pAp), stream)
+        if update_direction:
+            nxt = 1 - cur[0]
+            update_rr_rz(nxt)
+            zz = r if M is None else z
+            _native.call(f"warp_amd_cg_step_p_batched_{sfx}", nent, dpe, nb, _native.ptr(offs), _native.ptr(zz),
+                         _native.ptr(p), _native.ptr(rr), _native.ptr(atol_sq), _native.ptr(rz[nxt]),
+                         _native.ptr(rz[cur[0]]), stream)
+            cur[0] = nxt
+
+    restart = state._restart
+    maxiter = state._maxiter
+    check_every = state._check_every
+    if restart is None:
+        cycle_size = 1
+
+        def do_cycle():
+            do_iteration()
+    else:
+        cycle_size = max(1, min(restart, maxiter))
+        if check_every > 0:
+            check_every = max(check_every, cycle_size)
+
+        def do_cycle():
+            for _ in range(cycle_size - 1):
+                do_iteration()
+            do_iteration(update_direction=False)
+            A.matvec(xf, bf, r, -1.0, 1.0)
+            update_rr_rz(cur[0])
+            p.copy_(r if M is None else z)
+
+    if check_every <= 0:
+        for _ in range(0, maxiter, cycle_size):
+            do_cycle()
+        iters = torch.full((1,), maxiter, dtype=torch.int32, device=dev)
+        return iters, rr.clone(), atol_sq.clone()
+    return _host_loop(do_cycle, cycle_size, rr, atol_sq, maxiter, state._callback, check_every)
+
+
 def _host_loop(do_cycle, cycle_size, rr, atol_sq, maxiter, callback, check_every):
-    """Residual checks every ``check_every`` iterations (linear.py:2182-2243)."""
-    atol_sq_h = float(atol_sq.item())
-    atol = math.sqrt(atol_sq_h)
+    """Residual checks every ``check_every`` iterations (linear.py:2182-2243).
+    ``rr`` / ``atol_sq`` hold one value per subproblem: the loop stops when
+    every subproblem meets its own tolerance, and reports the largest residual
+    and tolerance."""
+    atol_sq_h = atol_sq.detach().to("cpu", torch.float64).reshape(-1)
+    atol = math.sqrt(float(atol_sq_h.max()))
     check_every = max(check_every, cycle_size)
     cur_iter = 0
-    err_sq = float(rr.item())
+
+    def read():
+        rr_h = rr.detach().to("cpu", torch.float64).reshape(-1)
+        return float(rr_h.max()), bool((rr_h <= atol_sq_h).all())
+
+    err_sq, done = read()
     if callback is not None:
         callback(cur_iter, math.sqrt(err_sq), atol)
-    if err_sq <= atol_sq_h:
+    if done:
         return cur_iter, math.sqrt(err_sq), atol
     while True:
         do_cycle()
@@ -575,12 +693,12 @@ def _host_loop(do_cycle, cycle_size, rr, atol_sq, maxiter, callback, check_every
         if cur_iter >= maxiter:
             break
         if (cur_iter % check_every) < cycle_size:
-            err_sq = float(rr.item())
-            if err_sq <= atol_sq_h:
+            err_sq, done = read()
+            if done:
                 break
             if callback is not None:
                 callback(cur_iter, math.sqrt(err_sq), atol)
-    err = math.sqrt(float(rr.item()))
+    err = math.sqrt(read()[0])
     if callback is not None:
         callback(cur_iter, err, atol)
     return cur_iter, err, atol
