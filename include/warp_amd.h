@@ -412,17 +412,24 @@ size_t warp_amd_cg_state_offset(int which);
  * *residual_norm, *abs_tolerance (host).  check_every == 0: no host checks;
  * the convergence flag stays on the device and results are read from the
  * workspace (warp_amd_cg_state_offset).  restart > 0: recompute r = b - A x
- * every `restart` iterations.  x: initial guess in, solution out. */
+ * every `restart` iterations.  use_graph (use_cuda_graph of
+ * linear.py:1035-1049): for systems of at most 1 MiB per vector, whose
+ * iterations are launch-bound, run on a private stream and replay the cycles
+ * between two residual checks as one HIP graph; larger systems and callers
+ * whose stream is being captured launch directly (same iterates either
+ * way).  x: initial guess in, solution out. */
 int warp_amd_cg_solve_f32(
     int nrow, int block_size, const int* offsets, const int* columns, const float* values, int64_t nnz_hint,
     int prec_kind, const float* prec, const float* b, float* x, double tol, double atol, int maxiter,
-    int check_every, int restart, warp_amd_cg_callback callback, void* callback_user, void* workspace,
-    size_t workspace_bytes, int* iterations, double* residual_norm, double* abs_tolerance, void* stream);
+    int check_every, int restart, int use_graph, warp_amd_cg_callback callback, void* callback_user,
+    void* workspace, size_t workspace_bytes, int* iterations, double* residual_norm, double* abs_tolerance,
+    void* stream);
 int warp_amd_cg_solve_f64(
     int nrow, int block_size, const int* offsets, const int* columns, const double* values, int64_t nnz_hint,
     int prec_kind, const double* prec, const double* b, double* x, double tol, double atol, int maxiter,
-    int check_every, int restart, warp_amd_cg_callback callback, void* callback_user, void* workspace,
-    size_t workspace_bytes, int* iterations, double* residual_norm, double* abs_tolerance, void* stream);
+    int check_every, int restart, int use_graph, warp_amd_cg_callback callback, void* callback_user,
+    void* workspace, size_t workspace_bytes, int* iterations, double* residual_norm, double* abs_tolerance,
+    void* stream);
 
 /* Generic CG building blocks for caller-supplied operators.
  * *out = sum a[i] b[i] accumulated in FP64, deterministic (replaces TiledDot

@@ -322,3 +322,28 @@ def test_batched_cg_bsr_matches_single_solves(device):
     x.zero_()
     its, rr, at = linear.cg(op, torch.from_numpy(B).to(device), x, tol=1e-6, maxiter=400, M=M, check_every=0)
     assert rr.shape == (3,) and at.shape == (3,) and bool((rr <= at).all())
+
+
+@pytest.mark.parametrize("n,restart,check_every", [(64, None, 10), (64, 5, 10), (64, 7, 10), (64, None, 0),
+                                                   (600, None, 10)])
+def test_cg_graph_replay_matches_eager(device, n, restart, check_every):
+    """use_cuda_graph replays the cycles between two checks as one HIP graph
+    on a private stream (small systems); the iterates are the eager ones bit
+    for bit, replay after replay: fused direction pass with device-side buffer
+    parity at n = 64, restart cycles that divide check_every (graphed) or not
+    (eager), device-only convergence (groups of 16 cycles), and a large system
+    that always launches directly (n = 600)."""
+    rows, cols, vals, b, nd = problems.poisson2d_q1(n, 3)
+    A = _bsr_from(device, nd, torch.from_numpy(rows), torch.from_numpy(cols), torch.from_numpy(vals))
+    M = linear.preconditioner(A, "diag")
+    bt = torch.from_numpy(b).to(device)
+    out = []
+    for graph in (False, True, True):
+        x = torch.zeros_like(bt)
+        it, err, atol = linear.cg(A, bt, x, tol=1e-6, maxiter=3000, M=M, restart=restart, check_every=check_every,
+                                  use_cuda_graph=graph)
+        if check_every == 0:
+            it, err = int(it.cpu()[0]), float(err.cpu()[0])
+        out.append((it, err, x.cpu().numpy().tobytes()))
+    assert out[0] == out[1] == out[2]
+    assert out[0][0] > 10

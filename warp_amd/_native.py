@@ -185,7 +185,7 @@ for _sfx, _s in (("f32", c_f32), ("f64", c_f64)):
             f"warp_amd_cg_solve_{_sfx}": (
                 c_int,
                 [c_int, c_int, c_ptr, c_ptr, c_ptr, c_i64, c_int, c_ptr, c_ptr, c_ptr, c_f64, c_f64, c_int, c_int, c_int,
-                 CG_CALLBACK, c_ptr, c_ptr, c_size, ctypes.POINTER(c_int), ctypes.POINTER(c_f64), ctypes.POINTER(c_f64), c_ptr],
+                 c_int, CG_CALLBACK, c_ptr, c_ptr, c_size, ctypes.POINTER(c_int), ctypes.POINTER(c_f64), ctypes.POINTER(c_f64), c_ptr],
             ),
             f"warp_amd_dot_{_sfx}": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
             f"warp_amd_cg_step_xr_{_sfx}": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]),

@@ -84,6 +84,16 @@ CgWs carve_cg(void* base, size_t bytes, int64_t nrow, int64_t n, size_t scalar_b
 
 constexpr int kFinalizeBlock = 1024;
 
+// Direction buffers.  Unfused, p stays in ws.p.  Fused, iteration k
+// (state->iter) reads buffer k & 1 and writes the other: the choice is made
+// on the device, so a captured iteration replays correctly any number of
+// times.
+template <typename S> __device__ __forceinline__ S* dir_buffer(const CgWs& ws, int which)
+{
+    return static_cast<S*>(which ? ws.p2 : ws.p);
+}
+
+
 // Vectors up to this size fold the direction pass into the SpMV (measured on
 // MI355X: 2D Poisson, 16,129 dofs, 18 -> 15 us per iteration; 823,875-dof
 // elasticity and 2M-dof FP64 systems slower by 2-12 us per iteration).
@@ -121,10 +131,11 @@ template <typename S, int BR, int G, int PREC, int kCgRpg = rows_per_group_for(B
 __global__ __launch_bounds__(kBlock) void cg_init_kernel(
     int nrow, const int* __restrict__ offsets, const int* __restrict__ columns, const S* __restrict__ values,
     const S* __restrict__ prec, const S* __restrict__ b, const S* __restrict__ x, S* __restrict__ r, S* __restrict__ z,
-    S* __restrict__ p, CgWs ws
+    int fused, CgWs ws
 )
 {
     __shared__ double lds[kBlock / kWave];
+    S* __restrict__ p = dir_buffer<S>(ws, fused ? (ws.state->iter & 1) : 0);
     const int row0 = (blockIdx.x * (kBlock / G) + threadIdx.x / G) * kCgRpg;
     const int lane = threadIdx.x % G;
     double acc3[3] = {0.0, 0.0, 0.0};
@@ -169,17 +180,20 @@ template <typename S> struct NextDirection {
 template <typename S, int BR, int G, bool FUSED, int kCgRpg = rows_per_group_for(BR, BR)>
 __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(
     int nrow, const int* __restrict__ offsets, const int* __restrict__ columns, const S* __restrict__ values,
-    const S* __restrict__ z, const S* __restrict__ p, S* __restrict__ p_next, S* __restrict__ Ap, CgWs ws
+    const S* __restrict__ z, S* __restrict__ Ap, CgWs ws
 )
 {
     __shared__ double lds[kBlock / kWave];
     if (ws.state->done)
         return;
+    const int par = FUSED ? (ws.state->iter & 1) : 0;
+    const S* __restrict__ p = dir_buffer<S>(ws, par);
     const int row0 = (blockIdx.x * (kBlock / G) + threadIdx.x / G) * kCgRpg;
     const int lane = threadIdx.x % G;
     double acc[1] = {0.0};
     if (row0 < nrow) {
         if constexpr (FUSED) {
+            S* __restrict__ p_next = dir_buffer<S>(ws, par ^ 1);
             const NextDirection<S> dir{z, p, (S)ws.state->beta};
             const RowAcc<S, BR * kCgRpg> ap = group_rows_product_f<S, BR, BR, G, kCgRpg, S>(
                 row0, nrow, lane, offsets, nullptr, columns, values, dir
@@ -216,12 +230,14 @@ constexpr int kMaxUpdateGrid = 4096;
 template <typename S, int BR, int PREC>
 __global__ __launch_bounds__(kBlock) void cg_update_kernel(
     int nrow, const S* __restrict__ prec, S* __restrict__ x, S* __restrict__ r, S* __restrict__ z,
-    const S* __restrict__ p, const S* __restrict__ Ap, CgWs ws
+    const S* __restrict__ Ap, int fused, CgWs ws
 )
 {
     __shared__ double lds[kBlock / kWave];
     if (ws.state->done)
         return;
+    // the direction this iteration's SpMV used (fused: the buffer it wrote)
+    const S* __restrict__ p = dir_buffer<S>(ws, fused ? ((ws.state->iter & 1) ^ 1) : 0);
     const S alpha = (S)ws.state->alpha;
     double acc[2] = {0.0, 0.0};
     for (int row = blockIdx.x * kBlock + threadIdx.x; row < nrow; row += gridDim.x * kBlock) {
@@ -371,12 +387,7 @@ template <typename S> struct CgLauncher {
     hipStream_t stream;
     // Direction pass folded into the SpMV (small systems).
     bool fused = false;
-    // Which direction buffer holds the current p (fused form).  Kernels are
-    // enqueued in order, so the host tracks the alternation.
-    bool flip = false;
 
-    S* p_cur() const { return (S*)(flip ? ws.p2 : ws.p); }
-    S* p_other() const { return (S*)(flip ? ws.p : ws.p2); }
     const S* z_vec() const { return pb.prec_kind != kPrecNone ? (const S*)ws.z : (const S*)ws.r; }
 
     template <int BR, int G> unsigned rows_grid() const
@@ -397,7 +408,7 @@ template <typename S> struct CgLauncher {
         const unsigned grid = rows_grid<BR, G>();
         cg_init_kernel<S, BR, G, PREC><<<grid, kBlock, 0, stream>>>(
             pb.nrow, pb.offsets, pb.columns, (const S*)pb.values, (const S*)pb.prec, (const S*)pb.b,
-            (const S*)pb.x, (S*)ws.r, (S*)ws.z, p_cur(), ws
+            (const S*)pb.x, (S*)ws.r, (S*)ws.z, fused ? 1 : 0, ws
         );
         WA_LAUNCH_CHECK();
         return finalize<kFinInit>(grid, first);
@@ -405,16 +416,14 @@ template <typename S> struct CgLauncher {
     template <int BR, int G> int spmv_t()
     {
         const unsigned grid = rows_grid<BR, G>();
-        if (fused) {
+        if (fused)
             cg_spmv_kernel<S, BR, G, true><<<grid, kBlock, 0, stream>>>(
-                pb.nrow, pb.offsets, pb.columns, (const S*)pb.values, z_vec(), p_cur(), p_other(), (S*)ws.Ap, ws
+                pb.nrow, pb.offsets, pb.columns, (const S*)pb.values, z_vec(), (S*)ws.Ap, ws
             );
-            flip = !flip;
-        } else {
+        else
             cg_spmv_kernel<S, BR, G, false><<<grid, kBlock, 0, stream>>>(
-                pb.nrow, pb.offsets, pb.columns, (const S*)pb.values, nullptr, p_cur(), nullptr, (S*)ws.Ap, ws
+                pb.nrow, pb.offsets, pb.columns, (const S*)pb.values, nullptr, (S*)ws.Ap, ws
             );
-        }
         WA_LAUNCH_CHECK();
         return finalize<kFinSpmv>(grid, 0);
     }
@@ -422,7 +431,7 @@ template <typename S> struct CgLauncher {
     {
         const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(pb.nrow, kBlock), kMaxUpdateGrid);
         cg_update_kernel<S, BR, PREC><<<grid, kBlock, 0, stream>>>(
-            pb.nrow, (const S*)pb.prec, (S*)pb.x, (S*)ws.r, (S*)ws.z, p_cur(), (const S*)ws.Ap, ws
+            pb.nrow, (const S*)pb.prec, (S*)pb.x, (S*)ws.r, (S*)ws.z, (const S*)ws.Ap, fused ? 1 : 0, ws
         );
         WA_LAUNCH_CHECK();
         return finalize<kFinUpdate>(grid, 0);
@@ -487,7 +496,7 @@ template <typename S> struct CgLauncher {
     int direction()
     {
         const int64_t n = (int64_t)pb.nrow * pb.br;
-        cg_direction_kernel<S><<<(unsigned)ceil_div(n, kBlock), kBlock, 0, stream>>>(n, z_vec(), p_cur(), ws.state);
+        cg_direction_kernel<S><<<(unsigned)ceil_div(n, kBlock), kBlock, 0, stream>>>(n, z_vec(), (S*)ws.p, ws.state);
         WA_LAUNCH_CHECK();
         return kOk;
     }
@@ -503,11 +512,111 @@ template <typename S> struct CgLauncher {
     }
 };
 
+// Runs the solver's cycles on a private stream and, with graphs on, replays
+// each group of cycles between two residual checks as one HIP graph (the
+// reference captures its iterations in CUDA graphs, linear.py:2205-2215).  The caller's stream may be the legacy
+// default stream, which cannot be captured, hence the private stream; it
+// starts after the caller's pending work and the caller's stream waits for
+// it at the end.  A failed capture falls back to eager launches.
+struct CycleRunner {
+    hipStream_t caller;
+    hipStream_t ps = nullptr;
+    hipEvent_t ev = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool graph_ok = false;
+
+    CycleRunner(hipStream_t s, bool use_graph) : caller(s)
+    {
+        if (!use_graph)
+            return;
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) {
+            (void)hipGetLastError();
+            return;  // already inside the caller's capture: launch into it
+        }
+        if (hipStreamCreateWithFlags(&ps, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess ||
+            hipEventRecord(ev, caller) != hipSuccess || hipStreamWaitEvent(ps, ev, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            release();
+            return;
+        }
+        graph_ok = true;
+    }
+    ~CycleRunner()
+    {
+        if (ps) {
+            // everything enqueued here precedes the caller's later work; the
+            // graph is destroyed only once it has run
+            if (hipEventRecord(ev, ps) == hipSuccess)
+                (void)hipStreamWaitEvent(caller, ev, 0);
+            (void)hipStreamSynchronize(ps);
+        }
+        release();
+    }
+    void release()
+    {
+        if (exec)
+            (void)hipGraphExecDestroy(exec);
+        if (ps)
+            (void)hipStreamDestroy(ps);
+        if (ev)
+            (void)hipEventDestroy(ev);
+        exec = nullptr;
+        ps = nullptr;
+        ev = nullptr;
+    }
+    hipStream_t stream() const { return ps ? ps : caller; }
+
+    // `count` cycles of `body`; with graph = true they are one replayable
+    // graph (always the same count: the group between two residual checks).
+    template <typename F> int run(F&& body, int count, bool graph)
+    {
+        auto eager = [&]() -> int {
+            for (int c = 0; c < count; ++c)
+                WA_TRY(body());
+            return kOk;
+        };
+        if (!graph || !graph_ok)
+            return eager();
+        if (exec) {
+            WA_HIP(hipGraphLaunch(exec, ps));
+            return kOk;
+        }
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(ps, hipStreamCaptureModeRelaxed) != hipSuccess) {
+            (void)hipGetLastError();
+            graph_ok = false;
+            return eager();
+        }
+        const int st = eager();
+        const hipError_t e = hipStreamEndCapture(ps, &g);
+        if (st != kOk) {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return st;
+        }
+        hipError_t ie = e;
+        if (e == hipSuccess && g)
+            ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        if (g)
+            (void)hipGraphDestroy(g);
+        if (e != hipSuccess || !g || ie != hipSuccess) {
+            (void)hipGetLastError();
+            exec = nullptr;
+            graph_ok = false;
+            return eager();  // nothing of the capture ran: run the cycles eagerly
+        }
+        WA_HIP(hipGraphLaunch(exec, ps));
+        return kOk;
+    }
+};
+
 template <typename S>
 int cg_solve_impl(
-    const CgProblem& pb, double tol, double atol, int maxiter, int check_every, int restart, CgCallback cb,
-    void* cb_user, void* workspace, size_t workspace_bytes, int* iters_out, double* rnorm_out, double* atol_out,
-    hipStream_t stream
+    const CgProblem& pb, double tol, double atol, int maxiter, int check_every, int restart, int use_graph,
+    CgCallback cb, void* cb_user, void* workspace, size_t workspace_bytes, int* iters_out, double* rnorm_out,
+    double* atol_out, hipStream_t caller_stream
 )
 {
     WA_REQUIRE(pb.br == 1 || pb.br == 3, "cg_solve: block size must be 1 or 3, got %d", pb.br);
@@ -522,6 +631,12 @@ int cg_solve_impl(
     if (maxiter <= 0)
         maxiter = (int)n;
 
+    // graphs only where launches bound the iteration: measured on MI355X,
+    // replaying the cycles between two checks as one graph gave 2D Poisson
+    // (16 K dofs) 19 -> 17 us per iteration but the 3x3 elasticity system
+    // (824 K dofs) 78.6 -> 79.6 us, and one graph per iteration was slower for both
+    CycleRunner runner(caller_stream, use_graph != 0 && (size_t)n * sizeof(S) <= kFusedDirectionMaxBytes);
+    const hipStream_t stream = runner.stream();
     CgLauncher<S> L{pb, ws, stream};
     L.fused = (size_t)n * sizeof(S) <= kFusedDirectionMaxBytes;
 
@@ -538,18 +653,24 @@ int cg_solve_impl(
     WA_LAUNCH_CHECK();
     WA_TRY(L.init(1));
 
-    auto run_cycle = [&]() -> int {
+    auto cycle_body = [&]() -> int {
         for (int k = 0; k < cycle; ++k)
             WA_TRY(L.iteration(restart > 0 && k == cycle - 1));
         if (restart > 0)
             WA_TRY(L.init(0));
         return kOk;
     };
+    auto run_cycle = [&]() -> int { return runner.run(cycle_body, 1, false); };
 
     if (check_every <= 0) {
-        // Device-side convergence only: enqueue every cycle, no host read-back.
-        for (int it = 0; it < maxiter; it += cycle)
-            WA_TRY(run_cycle());
+        // Device-side convergence only: enqueue every cycle, no host read-back
+        // (groups of kDeviceGroup cycles, one graph replay each).
+        constexpr int kDeviceGroup = 16;
+        const int total = (int)ceil_div(maxiter, cycle);
+        for (int c = 0; c < total; c += kDeviceGroup) {
+            const int k = std::min(kDeviceGroup, total - c);
+            WA_TRY(runner.run(cycle_body, k, k == kDeviceGroup));
+        }
         if (iters_out || rnorm_out || atol_out) {
             CgState h;
             WA_TRY(read_state(ws.state, &h, stream));
@@ -569,7 +690,26 @@ int cg_solve_impl(
     int cur_iter = 0;
     if (cb)
         cb(cur_iter, std::sqrt(h.rr), atol_out_v, cb_user);
-    if (!(h.rr <= h.atol_sq)) {
+    if (!(h.rr <= h.atol_sq) && check_every % cycle == 0) {
+        // the cycles between two checks run as one group (one graph replay)
+        const int per_check = check_every / cycle;
+        for (;;) {
+            const int left = (int)ceil_div(maxiter - cur_iter, cycle);
+            const int k = std::min(per_check, left);
+            WA_TRY(runner.run(cycle_body, k, k == per_check));
+            cur_iter += k * cycle;
+            if (cur_iter >= maxiter)
+                break;
+            WA_TRY(read_state(ws.state, &h, stream));
+            if (h.rr <= h.atol_sq)
+                break;
+            if (cb)
+                cb(cur_iter, std::sqrt(h.rr), atol_out_v, cb_user);
+        }
+        WA_TRY(read_state(ws.state, &h, stream));
+        if (cb)
+            cb(cur_iter, std::sqrt(h.rr), atol_out_v, cb_user);
+    } else if (!(h.rr <= h.atol_sq)) {
         for (;;) {
             WA_TRY(run_cycle());
             cur_iter += cycle;
@@ -870,14 +1010,15 @@ WA_API size_t warp_amd_cg_state_offset(int which)
     WA_API int warp_amd_cg_solve_##SUFFIX(                                                                            \
         int nrow, int block_size, const int* offsets, const int* columns, const S* values, int64_t nnz_hint,          \
         int prec_kind, const S* prec, const S* b, S* x, double tol, double atol, int maxiter, int check_every,        \
-        int restart, wa::CgCallback callback, void* callback_user, void* workspace, size_t workspace_bytes,           \
+        int restart, int use_graph, wa::CgCallback callback, void* callback_user, void* workspace,                    \
+        size_t workspace_bytes,                                                                                       \
         int* iterations, double* residual_norm, double* abs_tolerance, void* stream                                   \
     )                                                                                                                 \
     {                                                                                                                 \
         wa::CgProblem pb{nrow, block_size, offsets, columns, values, nnz_hint, prec_kind, prec, b, x};                \
         return wa::cg_solve_impl<S>(                                                                                  \
-            pb, tol, atol, maxiter, check_every, restart, callback, callback_user, workspace, workspace_bytes,       \
-            iterations, residual_norm, abs_tolerance, wa::as_stream(stream)                                           \
+            pb, tol, atol, maxiter, check_every, restart, use_graph, callback, callback_user, workspace,              \
+            workspace_bytes, iterations, residual_norm, abs_tolerance, wa::as_stream(stream)                          \
         );                                                                                                            \
     }                                                                                                                 \
     WA_API int warp_amd_dot_##SUFFIX(                                                                                 \

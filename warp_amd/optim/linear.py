@@ -442,6 +442,7 @@ class CG(LinearSolverState):
             self._maxiter,
             int(self._check_every),
             int(self._restart or 0),
+            1 if self._use_cuda_graph else 0,
             cb,
             None,
             _native.ptr(self._ws),
