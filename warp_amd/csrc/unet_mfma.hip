@@ -423,11 +423,11 @@ void launch_tiles(int ntiles, hipStream_t s, Args... args)
     Kernel<<<grid, kBlock, 0, s>>>(args...);
 }
 
-// Halo of one tile held in registers between its loads and its LDS stores
-// (CI >= 16: 16-byte channel octets, kHalo * CI / 8 of them per tile).
-template <int CI, int IN> struct HaloRegs {
+// Halo (HX x HY x HZ voxels) of one tile held in registers between its loads
+// and its LDS stores (CI >= 16: 16-byte channel octets, CI / 8 per voxel).
+template <int CI, int IN, int HX = kHX, int HY = kHY, int HZ = kHZ> struct HaloRegs {
     static constexpr int QPV = CI / 8;
-    static constexpr int TOTAL = kHalo * QPV;
+    static constexpr int TOTAL = HX * HY * HZ * QPV;
     static constexpr int ITERS = (TOTAL + kBlock - 1) / kBlock;
     bf16x8 v[ITERS];
     bf16x8 cv[IN == kInUpAdd ? ITERS : 1];
@@ -440,7 +440,7 @@ template <int CI, int IN> struct HaloRegs {
         for (int u = 0; u < ITERS; ++u) {
             const int c = u * kBlock + threadIdx.x;
             const int hv = c / QPV, od = c % QPV;
-            const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
+            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
             const int gx = X0 - 1 + hx, gy = Y0 - 1 + hy, gz = Z0 - 1 + hz;
             v[u] = zero8();
             if constexpr (IN == kInUpAdd)
@@ -539,6 +539,97 @@ __global__ __launch_bounds__(kBlock) void conv3_co1_kernel(int m, const unsigned
     }  // tiles
 }
 
+// The same layer on 8 x 8 x 8 tiles, each thread computing two z-adjacent
+// voxels: their 3 x 3 x 4 halo voxels are read from LDS once for both (36
+// reads instead of 54) and a tile's halo is 1.95x its voxels instead of 2.34x.
+// Every voxel still adds its taps in the order t = 0..26, so the results are
+// those of conv3_co1_kernel bit for bit.  Measured at 128^3: 62.6 us against
+// 64.0 (the layer is not bound by its LDS reads).
+constexpr int kTZ2 = 8;
+constexpr int kHZ2 = kTZ2 + 2;
+
+template <int CI, int IN>
+__global__ __launch_bounds__(kBlock) void conv3_co1_z2_kernel(int m, const unsigned short* __restrict__ x,
+                                                             const unsigned short* __restrict__ coarse,
+                                                             const unsigned* __restrict__ wpk,
+                                                             const float* __restrict__ bias, int flags,
+                                                             float* __restrict__ yf)
+{
+    constexpr int SW = lds_stride_words(CI);
+    static_assert(CI >= 16 && CI % 16 == 0, "one-output kernel expects a multiple of 16 input channels");
+    __shared__ unsigned tile[kHX * kHY * kHZ2 * SW];
+    const int tiles_x = (m + kTX - 1) / kTX;
+    const int tiles_z = (m + kTZ2 - 1) / kTZ2;
+    const int ntiles = tiles_x * tiles_x * tiles_z;
+    TileWalk walk(ntiles);
+    auto origin = [&](int t, int& X0, int& Y0, int& Z0) {
+        X0 = (t % tiles_x) * kTX;
+        Y0 = ((t / tiles_x) % tiles_x) * kTY;
+        Z0 = (t / (tiles_x * tiles_x)) * kTZ2;
+    };
+    HaloRegs<CI, IN, kHX, kHY, kHZ2> halo;
+    if (walk.t < walk.end) {
+        int X0, Y0, Z0;
+        origin(walk.t, X0, Y0, Z0);
+        halo.load(m, X0, Y0, Z0, x, coarse);
+    }
+    const float b0 = bias ? bias[0] : 0.0f;
+    const int lx = threadIdx.x & 7, ly = (threadIdx.x >> 3) & 7, lz = threadIdx.x >> 6;  // 8 x 8 x 4 pairs
+    for (; walk.t < walk.end; walk.t += walk.step) {
+    int X0, Y0, Z0;
+    origin(walk.t, X0, Y0, Z0);
+    __syncthreads();  // the previous tile's LDS reads are done
+    halo.store(tile);
+    __syncthreads();
+    if (walk.t + walk.step < walk.end) {
+        int X1, Y1, Z1;
+        origin(walk.t + walk.step, X1, Y1, Z1);
+        halo.load(m, X1, Y1, Z1, x, coarse);
+    }
+    float acc0 = b0, acc1 = b0;
+#pragma unroll 1
+    for (int dz = 0; dz < 4; ++dz) {
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const int hv = ((2 * lz + dz) * kHY + (ly + dy)) * kHX + (lx + dx);
+                const unsigned* src = &tile[hv * SW];
+                unsigned v[CI / 2];
+#pragma unroll
+                for (int j = 0; j < CI / 2; ++j)
+                    v[j] = src[j];
+                if (dz < 3) {
+                    const unsigned* w = wpk + ((dz * 3 + dy) * 3 + dx) * (CI / 2);
+#pragma unroll
+                    for (int j = 0; j < CI / 2; ++j)
+                        acc0 = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, v[j]),
+                                                               __builtin_bit_cast(bf16x2, w[j]), acc0, false);
+                }
+                if (dz > 0) {
+                    const unsigned* w = wpk + (((dz - 1) * 3 + dy) * 3 + dx) * (CI / 2);
+#pragma unroll
+                    for (int j = 0; j < CI / 2; ++j)
+                        acc1 = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, v[j]),
+                                                               __builtin_bit_cast(bf16x2, w[j]), acc1, false);
+                }
+            }
+    }
+    const int vx = X0 + lx, vy = Y0 + ly;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int vz = Z0 + 2 * lz + h;
+        float acc = h ? acc1 : acc0;
+        if (vx < m && vy < m && vz < m) {
+            if (flags & kOutRelu)
+                acc = fmaxf(acc, 0.0f);
+            float* d = yf + (size_t)(vz * m + vy) * m + vx;
+            *d = (flags & kOutAccumulate) ? *d + acc : acc;
+        }
+    }
+    }  // tiles
+}
+
 // wpk[tap][c / 2] = (W(tap, c, 0), W(tap, c + 1, 0)) as a BF16 pair (low, high)
 __global__ void pack_co1_kernel(int ci, int transpose_flip, const float* __restrict__ W, unsigned* __restrict__ out)
 {
@@ -552,17 +643,27 @@ __global__ void pack_co1_kernel(int ci, int transpose_flip, const float* __restr
     out[idx] = (unsigned)f2bf(w0) | ((unsigned)f2bf(w1) << 16);
 }
 
+constexpr bool kCo1PairTiles = true;
+
 template <int CI>
 int launch_co1(int m, int in_mode, const unsigned short* x, const unsigned short* coarse, const void* wpk,
                const float* bias, int flags, float* yf, hipStream_t s)
 {
     const int txy = (m + kTX - 1) / kTX;
-    const int ntiles = txy * txy * (m / kTZ);
     const unsigned* w = static_cast<const unsigned*>(wpk);
-    if (in_mode == kInUpAdd)
-        launch_tiles<conv3_co1_kernel<CI, kInUpAdd>>(ntiles, s, m, x, coarse, w, bias, flags, yf);
-    else
-        launch_tiles<conv3_co1_kernel<CI, kInPlain>>(ntiles, s, m, x, coarse, w, bias, flags, yf);
+    if (kCo1PairTiles) {
+        const int ntiles = txy * txy * ((m + kTZ2 - 1) / kTZ2);
+        if (in_mode == kInUpAdd)
+            launch_tiles<conv3_co1_z2_kernel<CI, kInUpAdd>>(ntiles, s, m, x, coarse, w, bias, flags, yf);
+        else
+            launch_tiles<conv3_co1_z2_kernel<CI, kInPlain>>(ntiles, s, m, x, coarse, w, bias, flags, yf);
+    } else {
+        const int ntiles = txy * txy * (m / kTZ);
+        if (in_mode == kInUpAdd)
+            launch_tiles<conv3_co1_kernel<CI, kInUpAdd>>(ntiles, s, m, x, coarse, w, bias, flags, yf);
+        else
+            launch_tiles<conv3_co1_kernel<CI, kInPlain>>(ntiles, s, m, x, coarse, w, bias, flags, yf);
+    }
     WA_LAUNCH_CHECK();
     return kOk;
 }
