@@ -347,3 +347,47 @@ def test_cg_graph_replay_matches_eager(device, n, restart, check_every):
         out.append((it, err, x.cpu().numpy().tobytes()))
     assert out[0] == out[1] == out[2]
     assert out[0][0] > 10
+
+
+def _seg_dot(device, a, b, offs):
+    from warp_amd import _native
+
+    lib = _native.lib()
+    n, nb = a.numel(), offs.numel() - 1
+    sfx = "f64" if a.dtype == torch.float64 else "f32"
+    chunk = torch.empty(nb + 1, dtype=torch.int32, device=device)
+    wsb = lib.warp_amd_seg_dot_plan_workspace_size(nb)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=device)
+    s = torch.cuda.current_stream(device).cuda_stream
+    _native.call("warp_amd_seg_dot_plan", nb, offs.data_ptr(), chunk.data_ptr(), ws.data_ptr(), wsb, s)
+    mc = lib.warp_amd_seg_dot_max_chunks(n, nb)
+    partials = torch.empty(max(mc, 1), dtype=torch.float64, device=device)
+    out = torch.full((nb,), float("nan"), dtype=torch.float64, device=device)
+    _native.call(f"warp_amd_seg_dot_{sfx}", n, nb, offs.data_ptr(), chunk.data_ptr(), mc, a.data_ptr(), b.data_ptr(),
+                 partials.data_ptr(), out.data_ptr(), s)
+    return out.cpu().numpy()
+
+
+def test_segmented_dot_known_answers(device):
+    """test_tiled_dot_large_single_batch / _batched_tree known answers
+    (test_linear_solvers.py:605-680): inactive tail excluded, empty and odd-sized
+    subproblems, and per-subproblem sums against FP64 sums of the same data."""
+    length = 100_000
+    active = length - 2
+    a = torch.full((length,), 2.0, device=device)
+    a[active:] = 100.0
+    out = _seg_dot(device, a, a, _batch_offsets([active], device))
+    assert out[0] == 4 * active
+    sizes = [0, 1, 511, 512, 513, 65_537, 262_145]
+    n = sum(sizes) + 17
+    offs = _batch_offsets(sizes, device)
+    two = torch.full((n,), 2.0, device=device)
+    three = torch.full((n,), 3.0, device=device)
+    np.testing.assert_array_equal(_seg_dot(device, two, three, offs), 6.0 * np.array(sizes))
+    rng = np.random.default_rng(123)
+    x = rng.standard_normal(n)
+    y = rng.standard_normal(n)
+    got = _seg_dot(device, torch.tensor(x, device=device), torch.tensor(y, device=device), offs)
+    bo = np.concatenate([[0], np.cumsum(sizes)])
+    ref = np.array([math.fsum(x[s:e] * y[s:e]) for s, e in zip(bo[:-1], bo[1:])])
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12)
