@@ -38,7 +38,7 @@ def cg_refined(
     b: torch.Tensor,
     x: torch.Tensor,
     tol: float = 1.0e-10,
-    inner_tol: float = 1.0e-4,
+    inner_tol: float = 3.0e-5,
     M: str | None = "diag",
     max_outer: int = 50,
     inner_maxiter: int = 0,
@@ -46,6 +46,11 @@ def cg_refined(
     callback=None,
 ) -> RefinementResult:
     """Solve A x = b to an FP64 relative residual ``tol`` (x: FP64 initial guess in, solution out).
+
+    ``inner_tol``: relative tolerance of each FP32 inner solve.  On the 256^3
+    variable-coefficient 7-point system (log-k sigma 2.3) to 1e-10, 3e-5 took
+    3 outer / 4,560 inner iterations; 1e-5: 3 / 5,380; 1e-4: 4 / 5,490; 3e-4:
+    4 / 4,930; 1e-3: 5 / 5,550.
 
     ``A``: compact BsrMatrix with FP32 values and 1x1 or 3x3 blocks.  ``b``,
     ``x``: FP64 tensors with ``A.shape[0]`` entries.  ``M``: "diag",
