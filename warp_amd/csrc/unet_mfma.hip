@@ -470,81 +470,15 @@ template <int CI, int IN, int HX = kHX, int HY = kHY, int HZ = kHZ> struct HaloR
 };
 
 // One output channel (the network's last layer): MFMA would waste 15 of its
-// 16 rows, so each thread computes one voxel with packed BF16 dot products,
-// v_dot2_f32_bf16, against weights wpk[tap][ci / 2] (BF16 pairs), reading
-// the LDS-staged halo.  Output FP32; flags: ReLU, accumulate.  The halo of
-// the workgroup's next tile is loaded into registers while the dot products
-// of the current one run, so the global round trips overlap the arithmetic.
-template <int CI, int IN>
-__global__ __launch_bounds__(kBlock) void conv3_co1_kernel(int m, const unsigned short* __restrict__ x,
-                                                          const unsigned short* __restrict__ coarse,
-                                                          const unsigned* __restrict__ wpk,
-                                                          const float* __restrict__ bias, int flags,
-                                                          float* __restrict__ yf)
-{
-    constexpr int SW = lds_stride_words(CI);
-    static_assert(CI >= 16 && CI % 16 == 0, "one-output kernel expects a multiple of 16 input channels");
-    __shared__ unsigned tile[kHalo * SW];
-    const int tiles_x = (m + kTX - 1) / kTX;
-    const int ntiles = tiles_x * tiles_x * (m / kTZ);
-    TileWalk walk(ntiles);
-    auto origin = [&](int t, int& X0, int& Y0, int& Z0) {
-        X0 = (t % tiles_x) * kTX;
-        Y0 = ((t / tiles_x) % tiles_x) * kTY;
-        Z0 = (t / (tiles_x * tiles_x)) * kTZ;
-    };
-    HaloRegs<CI, IN> halo;
-    if (walk.t < walk.end) {
-        int X0, Y0, Z0;
-        origin(walk.t, X0, Y0, Z0);
-        halo.load(m, X0, Y0, Z0, x, coarse);
-    }
-    const float b0 = bias ? bias[0] : 0.0f;
-    for (; walk.t < walk.end; walk.t += walk.step) {
-    int X0, Y0, Z0;
-    origin(walk.t, X0, Y0, Z0);
-    __syncthreads();  // the previous tile's LDS reads are done
-    halo.store(tile);
-    __syncthreads();
-    if (walk.t + walk.step < walk.end) {
-        int X1, Y1, Z1;
-        origin(walk.t + walk.step, X1, Y1, Z1);
-        halo.load(m, X1, Y1, Z1, x, coarse);
-    }
-
-    const int lx = threadIdx.x & 7, ly = (threadIdx.x >> 3) & 7, lz = threadIdx.x >> 6;  // 8 x 8 x 4
-    float acc = b0;
-    // taps three rows at a time: a full unroll holds every LDS read at once
-    // (159 VGPRs, 3 waves per SIMD) and measured slower than this (88 VGPRs)
-#pragma unroll 3
-    for (int t9 = 0; t9 < 9; ++t9) {
-#pragma unroll
-        for (int t3 = 0; t3 < 3; ++t3) {
-            const int t = t9 * 3 + t3;
-            const int hv = ((lz + t9 / 3) * kHY + (ly + t9 % 3)) * kHX + (lx + t3);
-            const unsigned* src = &tile[hv * SW];
-#pragma unroll
-            for (int j = 0; j < CI / 2; ++j)
-                acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, src[j]),
-                                                      __builtin_bit_cast(bf16x2, wpk[t * (CI / 2) + j]), acc, false);
-        }
-    }
-    const int vx = X0 + lx, vy = Y0 + ly, vz = Z0 + lz;
-    if (vx < m && vy < m && vz < m) {
-        if (flags & kOutRelu)
-            acc = fmaxf(acc, 0.0f);
-        float* d = yf + (size_t)(vz * m + vy) * m + vx;
-        *d = (flags & kOutAccumulate) ? *d + acc : acc;
-    }
-    }  // tiles
-}
-
-// The same layer on 8 x 8 x 8 tiles, each thread computing two z-adjacent
-// voxels: their 3 x 3 x 4 halo voxels are read from LDS once for both (36
-// reads instead of 54) and a tile's halo is 1.95x its voxels instead of 2.34x.
-// Every voxel still adds its taps in the order t = 0..26, so the results are
-// those of conv3_co1_kernel bit for bit.  Measured at 128^3: 62.6 us against
-// 64.0 (the layer is not bound by its LDS reads).
+// 16 rows, so each thread computes voxels with packed BF16 dot products,
+// v_dot2_f32_bf16, against weights wpk[tap][ci / 2] (BF16 pairs), reading the
+// LDS-staged halo; every voxel adds its taps in the order t = 0..26.  Output
+// FP32; flags: ReLU, accumulate.  Tiles are 8 x 8 x 8 with two z-adjacent
+// voxels per thread: their 3 x 3 x 4 halo voxels are read from LDS once for
+// both (36 reads instead of 54) and a tile's halo is 1.95x its voxels (2.34x
+// for 8 x 8 x 4 tiles of one voxel per thread: 64.0 us against 62.6 at
+// 128^3).  The halo of the workgroup's next tile is loaded into registers
+// while the dot products of the current one run.
 constexpr int kTZ2 = 8;
 constexpr int kHZ2 = kTZ2 + 2;
 
@@ -643,27 +577,17 @@ __global__ void pack_co1_kernel(int ci, int transpose_flip, const float* __restr
     out[idx] = (unsigned)f2bf(w0) | ((unsigned)f2bf(w1) << 16);
 }
 
-constexpr bool kCo1PairTiles = true;
-
 template <int CI>
 int launch_co1(int m, int in_mode, const unsigned short* x, const unsigned short* coarse, const void* wpk,
                const float* bias, int flags, float* yf, hipStream_t s)
 {
     const int txy = (m + kTX - 1) / kTX;
     const unsigned* w = static_cast<const unsigned*>(wpk);
-    if (kCo1PairTiles) {
-        const int ntiles = txy * txy * ((m + kTZ2 - 1) / kTZ2);
-        if (in_mode == kInUpAdd)
-            launch_tiles<conv3_co1_z2_kernel<CI, kInUpAdd>>(ntiles, s, m, x, coarse, w, bias, flags, yf);
-        else
-            launch_tiles<conv3_co1_z2_kernel<CI, kInPlain>>(ntiles, s, m, x, coarse, w, bias, flags, yf);
-    } else {
-        const int ntiles = txy * txy * (m / kTZ);
-        if (in_mode == kInUpAdd)
-            launch_tiles<conv3_co1_kernel<CI, kInUpAdd>>(ntiles, s, m, x, coarse, w, bias, flags, yf);
-        else
-            launch_tiles<conv3_co1_kernel<CI, kInPlain>>(ntiles, s, m, x, coarse, w, bias, flags, yf);
-    }
+    const int ntiles = txy * txy * ((m + kTZ2 - 1) / kTZ2);
+    if (in_mode == kInUpAdd)
+        launch_tiles<conv3_co1_z2_kernel<CI, kInUpAdd>>(ntiles, s, m, x, coarse, w, bias, flags, yf);
+    else
+        launch_tiles<conv3_co1_z2_kernel<CI, kInPlain>>(ntiles, s, m, x, coarse, w, bias, flags, yf);
     WA_LAUNCH_CHECK();
     return kOk;
 }
