@@ -348,6 +348,21 @@ int warp_amd_bsr_mv_transpose_f64(
     const int* columns, const double* values, int64_t nnz_hint, const double* x, double* y, double alpha,
     double beta, void* stream);
 
+/* y = alpha * A^T x + beta * y without atomics, deterministic (the variant
+ * of the reference's atomic bsr_mv_transpose_kernel, warp/_src/sparse.py:
+ * 4718-4747, for structurally symmetric patterns): A compact, square, blocks
+ * block_size x block_size (1 or 3), every block (r, c) with a mirror (c, r).
+ * Output row c sums, over its own blocks (c, r), the mirror block (r, c)^T
+ * x[r] (found by binary search); per warp_amd_bsr_mv_lanes lanes, an xor
+ * butterfly.  Blocks without a mirror contribute nothing: check the pattern
+ * first (warp_amd_bsr_count_missing_blocks of the transposed pairs). */
+int warp_amd_bsr_mv_transpose_gather_f32(int nrow, int block_size, const int* offsets, const int* columns,
+                                         const float* values, int64_t nnz_hint, const float* x, float* y,
+                                         float alpha, float beta, void* stream);
+int warp_amd_bsr_mv_transpose_gather_f64(int nrow, int block_size, const int* offsets, const int* columns,
+                                         const double* values, int64_t nnz_hint, const double* x, double* y,
+                                         double alpha, double beta, void* stream);
+
 /* Same with explicit kernel shape: `lanes` per block row (power of two in
  * [1, 64]; the reference's bsr_mv `tile_size` plays this role) and
  * `rows_per_group` consecutive rows per lane group (1, 2 or 4); <= 0 picks the

@@ -1301,3 +1301,51 @@ def test_degenerate_inputs(device, oracle_mod):
     E = sparse.bsr_zeros(0, 4, wa.float32, device=device)
     out = sparse.bsr_mv(E, torch.randn(4, device=device))
     assert out.numel() == 0
+
+
+@pytest.mark.parametrize("bshape,dtype", [((1, 1), torch.float32), ((3, 3), torch.float32), ((3, 3), torch.float64)])
+def test_bsr_mv_transpose_symmetric_pattern_is_deterministic(device, bshape, dtype):
+    """A square matrix whose pattern is structurally symmetric (values are not)
+    takes the gather form of A^T x: no atomics, the same bits on every run, and
+    the bits of bsr_mv on the explicit transpose (same lanes, same order)."""
+    rng = np.random.default_rng(19)
+    n, T = 400, 3000
+    r = rng.integers(0, n, T)
+    c = rng.integers(0, n, T)
+    rows = np.concatenate([r, c]).astype(np.int32)
+    cols = np.concatenate([c, r]).astype(np.int32)
+    vals = rng.standard_normal((2 * T, *bshape))
+    A = sparse.bsr_zeros(n, n, wa.matrix(bshape, wa.types.type_to_scalar(dtype)) if bshape != (1, 1)
+                         else wa.types.type_to_scalar(dtype), device=device)
+    v = vals.reshape(-1) if bshape == (1, 1) else vals
+    sparse.bsr_set_from_triplets(A, _t(rows, device), _t(cols, device), _t(v, device, dtype))
+    A.nnz_sync()
+    assert sparse._mirror_symmetric(A)
+    AT = sparse.bsr_transposed(A)
+    x = _t(rng.standard_normal(n * bshape[0]), device, dtype)
+    y0 = rng.standard_normal(n * bshape[1])
+    D = _dense_of(A)
+    for alpha, beta in [(1.0, 0.0), (-0.5, 2.0), (2.0, 1.0)]:
+        outs = []
+        for _ in range(3):
+            y = _t(y0, device, dtype)
+            sparse.bsr_mv(A, x, y, alpha, beta, transpose=True)
+            outs.append(y.cpu().numpy().tobytes())
+        assert outs[0] == outs[1] == outs[2]
+        yt = _t(y0, device, dtype)
+        sparse.bsr_mv(AT, x, yt, alpha, beta)
+        assert outs[0] == yt.cpu().numpy().tobytes()
+        ref = alpha * D.T @ x.cpu().numpy().astype(np.float64) + beta * y0
+        tol = dict(rtol=1e-4, atol=1e-4) if dtype == torch.float32 else dict(rtol=1e-10, atol=1e-10)
+        np.testing.assert_allclose(np.frombuffer(outs[0], dtype=np.float32 if dtype == torch.float32
+                                                 else np.float64).astype(np.float64), ref, **tol)
+    # one block without its mirror: the atomic scatter (still correct to rounding)
+    rows2 = np.append(rows, 0).astype(np.int32)
+    cols2 = np.append(cols, n - 1).astype(np.int32)
+    rows2[np.where((rows2 == n - 1) & (cols2 == 0))] = 1
+    B = sparse.bsr_zeros(n, n, A.dtype, device=device)
+    v2 = np.concatenate([v, v[:1]])
+    sparse.bsr_set_from_triplets(B, _t(rows2, device), _t(cols2, device), _t(v2, device, dtype))
+    DB = _dense_of(B)
+    if not np.array_equal(DB != 0, (DB != 0).T):
+        assert not sparse._mirror_symmetric(B)

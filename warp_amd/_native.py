@@ -179,6 +179,7 @@ for _sfx, _s in (("f32", c_f32), ("f64", c_f64)):
         {
             f"warp_amd_bsr_mv_{_sfx}": (c_int, [c_int, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_i64, c_ptr, c_ptr, _s, _s, c_ptr]),
             f"warp_amd_bsr_mv_transpose_{_sfx}": (c_int, [c_int, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_i64, c_ptr, c_ptr, _s, _s, c_ptr]),
+            f"warp_amd_bsr_mv_transpose_gather_{_sfx}": (c_int, [c_int, c_int, c_ptr, c_ptr, c_ptr, c_i64, c_ptr, c_ptr, _s, _s, c_ptr]),
             f"warp_amd_bsr_mv_ex_{_sfx}": (c_int, [c_int, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_i64, c_ptr, c_ptr, _s, _s, c_int, c_int, c_ptr]),
             f"warp_amd_bsr_get_diag_{_sfx}": (c_int, [c_int, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, _s, c_ptr, c_ptr]),
             f"warp_amd_bsr_jacobi_{_sfx}": (c_int, [c_int, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_int, c_ptr, c_ptr]),

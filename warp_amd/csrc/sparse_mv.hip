@@ -326,6 +326,83 @@ __global__ __launch_bounds__(kBlock) void bsr_mv_transpose_kernel(
     }
 }
 
+// y = alpha A^T x + beta y without atomics, for a square compact matrix
+// whose pattern is structurally symmetric (every block (r, c) has a mirror
+// (c, r)): output block row c gathers, over row c's blocks (c, r) in storage
+// order, the mirror block (r, c) -- found by binary search in row r -- times
+// x[r].  G lanes per output row: lane g takes blocks g, g + G, ... with fused
+// multiply-adds, then an xor butterfly; the result depends only on G.
+template <typename S, int B, int G>
+__global__ __launch_bounds__(kBlock) void bsr_mv_transpose_gather_kernel(
+    int nrow, const int* __restrict__ offsets, const int* __restrict__ columns, const S* __restrict__ values,
+    const S* __restrict__ x, S* __restrict__ y, S alpha, S beta
+)
+{
+    const int c = blockIdx.x * (kBlock / G) + threadIdx.x / G;
+    const int lane = threadIdx.x % G;
+    if (c >= nrow)
+        return;
+    const int beg = offsets[c];
+    const int end = offsets[c + 1];
+    S acc[B];
+#pragma unroll
+    for (int j = 0; j < B; ++j)
+        acc[j] = S(0);
+    for (int k = beg + lane; k < end; k += G) {
+        const int r = columns[k];
+        if (r < 0 || r >= nrow)
+            continue;
+        const int rend = offsets[r + 1];
+        int lo = offsets[r], hi = rend;
+        while (lo < hi) {
+            const int mid = lo + (hi - lo) / 2;
+            if (columns[mid] < c)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        if (lo == rend || columns[lo] != c)
+            continue;  // no mirror block (the caller checks that there is always one)
+        const S* blk = values + (size_t)lo * (B * B);  // block (r, c): B x B, row-major
+#pragma unroll
+        for (int j = 0; j < B; ++j)
+#pragma unroll
+            for (int i = 0; i < B; ++i)
+                acc[j] = fma(blk[i * B + j], x[(size_t)r * B + i], acc[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+        const S v = group_sum<G>(acc[j]);
+        if (lane == 0) {
+            S out = v * alpha;
+            if (beta != S(0))
+                out = fma(beta, y[(size_t)c * B + j], out);
+            y[(size_t)c * B + j] = out;
+        }
+    }
+}
+
+template <typename S>
+int bsr_mv_transpose_gather_impl(int nrow, int b, const int* offsets, const int* columns, const S* values,
+                                 int64_t nnz_hint, const S* x, S* y, S alpha, S beta, hipStream_t stream)
+{
+    const int g = bsr_mv_lanes(nrow, b, b, nnz_hint);
+#define WA_GATHER_CASE(B, G)                                                                                         \
+    if (b == B && g == G) {                                                                                           \
+        bsr_mv_transpose_gather_kernel<S, B, G><<<(unsigned)ceil_div(nrow, kBlock / G), kBlock, 0, stream>>>(        \
+            nrow, offsets, columns, values, x, y, alpha, beta);                                                       \
+        WA_LAUNCH_CHECK();                                                                                            \
+        return kOk;                                                                                                   \
+    }
+    WA_GATHER_CASE(1, 1) WA_GATHER_CASE(1, 2) WA_GATHER_CASE(1, 4) WA_GATHER_CASE(1, 8)
+    WA_GATHER_CASE(1, 16) WA_GATHER_CASE(1, 32) WA_GATHER_CASE(1, 64)
+    WA_GATHER_CASE(3, 1) WA_GATHER_CASE(3, 2) WA_GATHER_CASE(3, 4) WA_GATHER_CASE(3, 8)
+    WA_GATHER_CASE(3, 16) WA_GATHER_CASE(3, 32) WA_GATHER_CASE(3, 64)
+#undef WA_GATHER_CASE
+    set_error("bsr_mv_transpose_gather: block size must be 1 or 3, got %d", b);
+    return kErrArg;
+}
+
 // Any block shape: one thread per (block row, stored block).
 template <typename S>
 __global__ __launch_bounds__(kBlock) void bsr_mv_transpose_generic_kernel(
@@ -460,6 +537,19 @@ WA_DEFINE_SPARSE_MV(f64, double)
     {                                                                                                                 \
         return wa::bsr_mv_transpose_impl<S>(nrow, ncol, block_rows, block_cols, offsets, row_counts, columns,       \
                                             values, nnz_hint, x, y, alpha, beta, wa::as_stream(stream));             \
+    }                                                                                                                 \
+    WA_API int warp_amd_bsr_mv_transpose_gather_##SUFFIX(                                                             \
+        int nrow, int block_size, const int* offsets, const int* columns, const S* values, int64_t nnz_hint,         \
+        const S* x, S* y, S alpha, S beta, void* stream                                                               \
+    )                                                                                                                 \
+    {                                                                                                                 \
+        WA_REQUIRE(nrow >= 0 && offsets && y, "bsr_mv_transpose_gather: invalid arguments");                         \
+        if (nrow == 0)                                                                                                \
+            return wa::kOk;                                                                                           \
+        WA_REQUIRE(columns && values && x, "bsr_mv_transpose_gather: null arrays");                                  \
+        return wa::bsr_mv_transpose_gather_impl<S>(nrow, block_size, offsets, columns, values, nnz_hint, x, y, alpha, \
+                                                   beta, wa::as_stream(stream));                                      \
     }
+
 WA_DEFINE_MV_TRANSPOSE(f32, float)
 WA_DEFINE_MV_TRANSPOSE(f64, double)

@@ -1657,6 +1657,24 @@ def bsr_block_index(row, col, bsr_offsets: torch.Tensor, bsr_columns: torch.Tens
     return int(out) if out.ndim == 0 else out
 
 
+def _mirror_symmetric(A: BsrMatrix) -> bool:
+    """True when ``A`` is compact and square with 1x1 or 3x3 blocks and every
+    block (r, c) has a mirror (c, r): the transposed product can then gather
+    without atomics.  Reads one count back, so never under graph capture."""
+    br, bc = A.block_shape
+    if (A.row_counts is not None or A.nrow != A.ncol or br != bc or br not in (1, 3)
+            or torch.cuda.is_current_stream_capturing()):
+        return False
+    T = min(A.nnz_sync(), A.columns.numel())  # exact: stale tail slots hold no valid column
+    if T == 0:
+        return False
+    rows = A.uncompress_rows()
+    missing = torch.empty(1, dtype=torch.int32, device=A.device)
+    _native.call("warp_amd_bsr_count_missing_blocks", T, _native.ptr(A.columns), _native.ptr(rows),
+                 _native.ptr(A.offsets), None, _native.ptr(A.columns), _native.ptr(missing), _native.stream_of(A.device))
+    return int(missing.item()) == 0
+
+
 def _vec_view(array: torch.Tensor, torch_dtype: torch.dtype, expected_scalar_count: int) -> torch.Tensor:
     """Flat scalar view of a vector array (sparse.py:4750-4789)."""
     if array.numel() != expected_scalar_count:
@@ -1737,6 +1755,14 @@ def bsr_mv(
         raise TypeError(f"bsr_mv supports float32 and float64 matrices, got {torch_dtype}")
     sfx = "f64" if torch_dtype == torch.float64 else "f32"
     if transpose:
+        if _mirror_symmetric(A):
+            # every block has its mirror: gather, deterministic
+            _native.call(
+                "warp_amd_bsr_mv_transpose_gather_" + sfx, A.nrow, A.block_shape[0], _native.ptr(A.offsets),
+                _native.ptr(A.columns), _native.ptr(A.values), int(A.nnz), _native.ptr(x_view), _native.ptr(y_view),
+                alpha, beta, _native.stream_of(A.values.device),
+            )
+            return y
         # atomic scatter, as the reference: the last bits of y vary between runs
         _native.call(
             "warp_amd_bsr_mv_transpose_" + sfx, A.nrow, A.ncol, A.block_shape[0], A.block_shape[1],
