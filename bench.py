@@ -360,6 +360,22 @@ def main() -> int:
         t_bytes = nnzb * (8 + 2 * 36) + 8 * (nn + 1)
         log(f"[bench] transpose: {nt} blocks in {t_t:.2f} ms ({t_bytes / t_t / 1e6:.0f} GB/s algorithmic)")
         result["transpose"] = {"blocks": nt, "ms": round(t_t, 3), "alg_GBps": round(t_bytes / t_t / 1e6, 1)}
+        # y = A^T x on the same matrix: its pattern is symmetric, so the deterministic
+        # gather (after the mirror check, which reads one count back)
+        yT = torch.empty_like(y)
+        sparse.bsr_mv(A, x, yT, transpose=True)
+        ms_mt = []
+        for rep in range(6):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sparse.bsr_mv(A, x, yT, transpose=True)
+            torch.cuda.synchronize()
+            ms_mt.append((time.perf_counter() - t0) * 1e3)
+        sym = sparse._mirror_symmetric(A)
+        log(f"[bench] bsr_mv transpose=True: {float(np.median(ms_mt)):.3f} ms per call "
+            f"({'gather, deterministic' if sym else 'atomic scatter'}, mirror check included)")
+        result["transpose"]["bsr_mv_transpose_ms"] = round(float(np.median(ms_mt)), 3)
+        result["transpose"]["bsr_mv_transpose_path"] = "gather" if sym else "atomic"
         p1 = n + 1
         face = torch.nonzero(torch.arange(nn, device=dev) % p1 == 0).reshape(-1).to(torch.int32)
         eye = torch.eye(3, device=dev).expand(face.numel(), 3, 3).contiguous()
