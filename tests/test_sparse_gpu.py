@@ -111,6 +111,54 @@ def test_triplets_long_rows(device, oracle_mod, per_row):
     _check_against_oracle(oracle_mod, A, nrow, ncol, rows, cols, vals)
 
 
+ROW_CLASS_SIZES = [0, 1, 2, 63, 64, 65, 66, 100, 127, 128, 129, 200, 255, 256, 257, 300]
+
+
+@pytest.mark.parametrize("bshape,dtype", [((), np.float32), ((3, 3), np.float32), ((2, 2), np.float64)])
+def test_triplets_row_size_classes(device, oracle_mod, bshape, dtype):
+    """Rows on both sides of each sort-path boundary (one key per lane: 64
+    keys; one wave's registers: 256 keys), heavy and no duplication, in
+    shuffled order; then the transpose of the result, whose rows cross the
+    same boundaries."""
+    rng = np.random.default_rng(4242 + len(bshape))
+    ncol = 4000
+    rows, cols = [], []
+    for r, n in enumerate(ROW_CLASS_SIZES * 2):
+        rows.append(np.full(n, r, np.int32))
+        # first copy of the list: about three triplets per unique column; second: all unique
+        span = max(1, n // 3) if r < len(ROW_CLASS_SIZES) else ncol
+        cols.append(rng.choice(span, size=n, replace=span < n).astype(np.int32))
+    rows, cols = np.concatenate(rows), np.concatenate(cols)
+    perm = rng.permutation(rows.size)
+    rows, cols = rows[perm], cols[perm]
+    nrow = 2 * len(ROW_CLASS_SIZES)
+    vals = rng.standard_normal((rows.size, *bshape)).astype(dtype)
+    print(f"[triplets] row classes block={bshape} dtype={np.dtype(dtype).name} T={rows.size}", flush=True)
+    A = _assemble(device, nrow, ncol, rows, cols, vals)
+    _check_against_oracle(oracle_mod, A, nrow, ncol, rows, cols, vals)
+    A = _assemble(device, nrow, ncol, rows, cols, None)
+    _check_against_oracle(oracle_mod, A, nrow, ncol, rows, cols, None)
+    # columns of a transpose source holding up to ~300 entries each
+    tr = np.repeat(np.arange(300, dtype=np.int32), 8)
+    tc = rng.integers(0, 12, tr.size).astype(np.int32)
+    tc[: ROW_CLASS_SIZES[-1]] = 11
+    tv = rng.standard_normal((tr.size, *bshape)).astype(dtype)
+    S = _assemble(device, 300, 12, tr, tc, tv)
+    nnz = S.nnz_sync()
+    off, scol = S.offsets.cpu().numpy(), S.columns[:nnz].cpu().numpy()
+    dest = sparse.bsr_transposed(S)
+    ref = oracle_mod.bsr_transpose(300, 12, off, scol)
+    n = dest.nnz_sync()
+    assert n == ref["columns"].size
+    np.testing.assert_array_equal(dest.offsets.cpu().numpy(), ref["offsets"])
+    np.testing.assert_array_equal(dest.columns.cpu().numpy()[:n], ref["columns"])
+    sv = S.values[:nnz].cpu().numpy()
+    want = sv[ref["block_map"]]
+    if want.ndim == 3:
+        want = want.transpose(0, 2, 1)
+    np.testing.assert_array_equal(dest.values.cpu().numpy()[:n], want)
+
+
 def test_triplets_known_answers(device):
     # test_bsr_stale_compact_tail (test_sparse.py:225-252)
     A = sparse.bsr_from_triplets(

@@ -33,8 +33,9 @@ namespace wa {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kSmallCap = 256;    // rows sorted by one wave
-constexpr int kLargeCap = 4096;   // rows sorted in one workgroup's LDS
+constexpr int kMediumRegs = 4;                     // keys per lane of a medium row
+constexpr int kMediumCap = kMediumRegs * kWave;    // rows sorted in one wave's registers
+constexpr int kLargeCap = 4096;                    // rows sorted in one workgroup's LDS
 constexpr uint64_t kPadKey = ~uint64_t(0);
 
 // Where each row's keys live in the key array: at start[r], count[r] of them
@@ -98,7 +99,8 @@ struct TripletWs {
     uint64_t* alt;
     int* brow;
     int* tile_offsets;
-    int* large_rows;
+    int* large_rows;   // rows of more than kMediumCap keys (count in counters[0])
+    int* medium_rows;  // rows of kWave + 1 .. kMediumCap keys (count in counters[1])
     int* counters;
     void* scan_ws;
     int* summed_offsets;
@@ -120,6 +122,7 @@ TripletWs carve_ws(void* base, size_t bytes, int nrow, int T, bool own_summed, s
     w.brow = w.part.tiled ? c.take<int>((size_t)T) : nullptr;
     w.tile_offsets = c.take<int>((size_t)ntile_hist);
     w.large_rows = c.take<int>((size_t)nrow + 1);
+    w.medium_rows = c.take<int>((size_t)nrow + 1);
     w.counters = c.take<int>(64);
     w.scan_ws = c.take<char>(scan_workspace_bytes(std::max<int64_t>((int64_t)nrow + 1, ntile_hist)));
     w.summed_offsets = own_summed ? c.take<int>((size_t)T) : nullptr;
@@ -552,12 +555,56 @@ __device__ __forceinline__ uint64_t load_sorted_row(const uint64_t* keys, int ba
 // Rows are handled one per wavefront, kRowsPerBlock wavefronts per workgroup.
 constexpr int kRowsPerBlock = 4;
 
+// Lists the rows of more than kWave keys: those of at most kMediumCap keys in
+// medium_rows (count in counters[1]), larger ones in large_rows (count in
+// counters[0]).  One atomic per workgroup and list: a per-row append to one
+// counter serialises ~10^5 device-scope atomics on a single address.
+__global__ __launch_bounds__(kBlock) void classify_rows_kernel(
+    int nrow, RowRanges rr, int* __restrict__ large_rows, int* __restrict__ medium_rows, int* __restrict__ counters
+)
+{
+    __shared__ int s_cnt[2][kBlock / kWave];
+    __shared__ int s_base[2];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    const int size = row < nrow ? rr.size(row) : 0;
+    const bool medium = size > kWave && size <= kMediumCap;
+    const bool large = size > kMediumCap;
+    const uint64_t mm = __ballot(medium);
+    const uint64_t ml = __ballot(large);
+    if (lane == 0) {
+        s_cnt[0][wv] = __popcll(mm);
+        s_cnt[1][wv] = __popcll(ml);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        int total = 0;
+        for (int w = 0; w < kBlock / kWave; ++w)
+            total += s_cnt[threadIdx.x][w];
+        // list 0 (medium) counts in counters[1], list 1 (large) in counters[0]
+        s_base[threadIdx.x] = total ? atomicAdd(&counters[threadIdx.x == 0 ? 1 : 0], total) : 0;
+    }
+    __syncthreads();
+    int off_m = s_base[0];
+    int off_l = s_base[1];
+    for (int w = 0; w < wv; ++w) {
+        off_m += s_cnt[0][w];
+        off_l += s_cnt[1][w];
+    }
+    const uint64_t below = (uint64_t(1) << lane) - 1;
+    if (medium)
+        medium_rows[off_m + __popcll(mm & below)] = row;
+    if (large)
+        large_rows[off_l + __popcll(ml & below)] = row;
+}
+
 // Unique column count of every row of at most kWave keys (sorted in
 // registers; with write_back the sorted keys and triplet indices are stored
-// in place).  Larger rows go to large_rows for the LDS / global-merge sort.
+// in place).  Larger rows are listed by classify_rows_kernel.
 __global__ __launch_bounds__(kWave * kRowsPerBlock) void count_small_rows_kernel(
     int nrow, RowRanges rr, uint64_t* __restrict__ keys, int* __restrict__ summed_indices,
-    int write_back, int* __restrict__ urow, int* __restrict__ large_rows, int* __restrict__ counters
+    int write_back, int* __restrict__ urow
 )
 {
     const int lane = threadIdx.x & (kWave - 1);
@@ -566,13 +613,8 @@ __global__ __launch_bounds__(kWave * kRowsPerBlock) void count_small_rows_kernel
         return;
     const int base = rr.base(row);
     const int size = rr.size(row);
-    if (size > kWave) {
-        if (lane == 0) {
-            const int slot = atomicAdd(&counters[0], 1);
-            large_rows[slot] = row;
-        }
+    if (size > kWave)
         return;
-    }
     const uint64_t key = load_sorted_row(keys, base, size, lane);
     const int col = (int)(key >> 32);
     const int prev = __shfl_up(col, 1);
@@ -586,7 +628,253 @@ __global__ __launch_bounds__(kWave * kRowsPerBlock) void count_small_rows_kernel
     }
 }
 
-// Rows above kSmallCap: one workgroup each (grid-stride over the row list).
+// Medium rows (kWave + 1 .. kMediumCap keys) live in R = 2 (up to 2 * kWave
+// keys) or R = kMediumRegs registers per lane of one wave, element
+// i = q * kWave + lane.
+
+// Bitonic sort of an R x kWave register tile (kPadKey past the row's end),
+// ascending by element index.  Every trip count is a compile-time constant,
+// so the network unrolls with static register indices: exchanges at distance
+// >= kWave swap registers inside a lane, shorter ones go across lanes.
+template <int R, int N>
+__device__ __forceinline__ void wave_sort_regs(uint64_t (&k)[N], int lane)
+{
+    static_assert(R <= N, "register tile too small");
+#pragma unroll
+    for (int size = 2; size <= R * kWave; size <<= 1) {
+#pragma unroll
+        for (int j = size >> 1; j > 0; j >>= 1) {
+            if (j >= kWave) {
+                const int jq = j / kWave;
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    if (q & jq)
+                        continue;
+                    const bool up = ((q * kWave) & size) == 0;
+                    const uint64_t a = k[q];
+                    const uint64_t b = k[q | jq];
+                    const bool keep = (a < b) == up;
+                    k[q] = keep ? a : b;
+                    k[q | jq] = keep ? b : a;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const uint64_t other = shfl_xor_u64(k[q], j);
+                    const bool up = ((q * kWave + lane) & size) == 0;
+                    const bool lower = (lane & j) == 0;
+                    // the lower element of an ascending pair keeps the smaller key
+                    const bool take_other = (other < k[q]) == (lower == up);
+                    k[q] = take_other ? other : k[q];
+                }
+            }
+        }
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void load_medium_row(const uint64_t* keys, int base, int size, int lane, uint64_t (&k)[R])
+{
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int i = q * kWave + lane;
+        k[q] = i < size ? keys[base + i] : kPadKey;
+    }
+}
+
+// Column of element i - 1 (q, lane) of a register tile (-1 before the first).
+template <int R>
+__device__ __forceinline__ int prev_col(const int (&c)[R], int q, int lane)
+{
+    const int up = __shfl_up(c[q], 1);
+    const int wrap = __shfl(c[q > 0 ? q - 1 : 0], kWave - 1);
+    return lane > 0 ? up : (q > 0 ? wrap : -1);
+}
+
+// Column of element i + 1 (q, lane) of a register tile (-1 after the last).
+template <int R>
+__device__ __forceinline__ int next_col(const int (&c)[R], int q, int lane)
+{
+    const int down = __shfl_down(c[q], 1);
+    const int wrap = __shfl(c[q < R - 1 ? q + 1 : q], 0);
+    return lane < kWave - 1 ? down : (q < R - 1 ? wrap : -1);
+}
+
+// Orders the LDS writes of a wave before its own later LDS reads (no
+// workgroup barrier: the waves of a workgroup work on different rows).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    // a wave's LDS instructions execute in issue order: only the compiler's
+    // reordering has to be fenced (and no global loads in flight waited for)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One medium row, loaded into k (kPadKey past its end): sorted in registers,
+// stored back in place with its sorted triplet indices; returns its unique
+// column count.
+template <int R>
+__device__ __forceinline__ int sort_medium_row(uint64_t (&k)[kMediumRegs], uint64_t* keys, int* summed_indices,
+                                               int base, int size, int lane)
+{
+    wave_sort_regs<R>(k, lane);
+    int c[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q)
+        c[q] = (int)(k[q] >> 32);
+    int nu = 0;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int i = q * kWave + lane;
+        const int prev = prev_col<R>(c, q, lane);
+        const bool head = i < size && (i == 0 || prev != c[q]);
+        nu += __popcll(__ballot(head));
+        if (i < size) {
+            keys[base + i] = k[q];
+            summed_indices[base + i] = (int)(uint32_t)k[q];
+        }
+    }
+    return nu;
+}
+
+// Medium rows, one wave per listed row (grid-stride over the list).
+__global__ __launch_bounds__(kWave * kRowsPerBlock) void sort_medium_rows_kernel(
+    RowRanges rr, uint64_t* __restrict__ keys, int* __restrict__ summed_indices, int* __restrict__ urow,
+    const int* __restrict__ medium_rows, const int* __restrict__ counters
+)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int nmed = counters[1];
+    const int nwaves = gridDim.x * kRowsPerBlock;
+    for (int li = blockIdx.x * kRowsPerBlock + (int)(threadIdx.x / kWave); li < nmed; li += nwaves) {
+        const int row = medium_rows[li];
+        const int base = rr.base(row);
+        const int size = rr.size(row);
+        uint64_t k[kMediumRegs];
+        load_medium_row<kMediumRegs>(keys, base, size, lane, k);
+        const int nu = size <= 2 * kWave ? sort_medium_row<2>(k, keys, summed_indices, base, size, lane)
+                                         : sort_medium_row<kMediumRegs>(k, keys, summed_indices, base, size, lane);
+        if (lane == 0)
+            urow[row] = nu;
+    }
+}
+
+// Topology of one sorted medium row: columns and (optionally) inclusive run
+// ends to global memory, run starts and sorted triplet indices to the wave's
+// LDS.  Returns the number of unique columns.
+template <int R>
+__device__ __forceinline__ int medium_row_runs(const uint64_t* keys, int base, int size, int out0, int lane,
+                                               int* bsr_columns, int* summed_offsets, int* idx_of, int* beg_of)
+{
+    const uint64_t below = lane == 63 ? ~uint64_t(0) : ((uint64_t(1) << (lane + 1)) - 1);
+    uint64_t k[R];
+    load_medium_row<R>(keys, base, size, lane, k);
+    int c[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q)
+        c[q] = (int)(k[q] >> 32);
+    int runs = 0;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int i = q * kWave + lane;
+        const bool act = i < size;
+        const int prev = prev_col<R>(c, q, lane);
+        const int next = next_col<R>(c, q, lane);
+        const bool head = act && (i == 0 || prev != c[q]);
+        const bool tail = act && (i == size - 1 || next != c[q]);
+        const uint64_t mask = __ballot(head);
+        const int r = runs + __popcll(mask & below) - 1;
+        if (head) {
+            bsr_columns[out0 + r] = c[q];
+            beg_of[r] = i;
+        }
+        if (summed_offsets && tail)
+            summed_offsets[out0 + r] = base + i + 1;
+        if (act)
+            idx_of[i] = (int)(uint32_t)k[q];
+        runs += __popcll(mask);
+    }
+    return runs;
+}
+
+// Loads of one run sum issued ahead of its adds.
+constexpr int kRunBatch = 8;
+
+// Medium rows (sorted in memory by sort_medium_rows_kernel), one wave per
+// listed row: columns, optionally inclusive run ends, and optionally the
+// block sums.  The run starts and sorted triplet indices go through the
+// wave's LDS.  A sum adds its run's blocks in ascending triplet order, the
+// first as the initial value; a lane works on two (block, scalar) sums at
+// once and issues kRunBatch loads of each before adding.
+template <typename S, int BS>
+__global__ __launch_bounds__(kWave * kRowsPerBlock) void finalize_medium_rows_kernel(
+    RowRanges rr, const uint64_t* __restrict__ keys, const int* __restrict__ bsr_offsets,
+    int* __restrict__ bsr_columns, int* __restrict__ summed_offsets, const S* __restrict__ tpl_values,
+    int block_size, S* __restrict__ bsr_values, const int* __restrict__ medium_rows, const int* __restrict__ counters
+)
+{
+    __shared__ int s_idx[kRowsPerBlock][kMediumCap];
+    __shared__ int s_beg[kRowsPerBlock][kMediumCap + 1];
+    const int bs = BS > 0 ? BS : block_size;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = (int)(threadIdx.x / kWave);
+    int* idx_of = s_idx[wv];
+    int* beg_of = s_beg[wv];
+    const int nmed = counters[1];
+    const int nwaves = gridDim.x * kRowsPerBlock;
+    for (int li = blockIdx.x * kRowsPerBlock + wv; li < nmed; li += nwaves) {
+        const int row = medium_rows[li];
+        const int base = rr.base(row);
+        const int size = rr.size(row);
+        const int out0 = bsr_offsets[row];
+        const int runs =
+            size <= 2 * kWave
+                ? medium_row_runs<2>(keys, base, size, out0, lane, bsr_columns, summed_offsets, idx_of, beg_of)
+                : medium_row_runs<kMediumRegs>(keys, base, size, out0, lane, bsr_columns, summed_offsets, idx_of,
+                                               beg_of);
+        if (bsr_values) {
+            if (lane == 0)
+                beg_of[runs] = size;
+            wave_lds_sync();
+            const int ntask = runs * bs;
+            for (int t0 = lane; t0 < ntask; t0 += 2 * kWave) {
+                int beg[2], end[2], sc[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int t = t0 + h * kWave;
+                    const int u = t < ntask ? t / bs : 0;
+                    sc[h] = t - u * bs;
+                    beg[h] = t < ntask ? beg_of[u] : 0;
+                    end[h] = t < ntask ? beg_of[u + 1] : 0;
+                }
+                S acc[2] = {S(0), S(0)};
+                for (int o = 0; beg[0] + o < end[0] || beg[1] + o < end[1]; o += kRunBatch) {
+                    S v[2][kRunBatch];
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+#pragma unroll
+                        for (int q = 0; q < kRunBatch; ++q)
+                            if (beg[h] + o + q < end[h])
+                                v[h][q] = tpl_values[(size_t)idx_of[beg[h] + o + q] * bs + sc[h]];
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+#pragma unroll
+                        for (int q = 0; q < kRunBatch; ++q)
+                            if (beg[h] + o + q < end[h])
+                                acc[h] = (o + q == 0) ? v[h][q] : acc[h] + v[h][q];
+                }
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+                    if (t0 + h * kWave < ntask)
+                        bsr_values[(size_t)(out0 + (t0 + h * kWave) / bs) * bs + sc[h]] = acc[h];
+            }
+            wave_lds_sync();
+        }
+    }
+}
+
+// Rows above kMediumCap: one workgroup each (grid-stride over the row list).
 __global__ __launch_bounds__(kBlock) void sort_large_rows_kernel(
     RowRanges rr, uint64_t* keys, uint64_t* alt, int* summed_indices, int* urow, const int* large_rows,
     const int* counters
@@ -672,8 +960,8 @@ __global__ __launch_bounds__(kBlock) void sort_large_rows_kernel(
     }
 }
 
-// Rows above kWave keys (sorted in memory by the LDS / merge sort), one wave
-// per listed row: columns and inclusive run ends of the row's unique blocks.
+// Rows above kMediumCap keys (sorted in memory by the LDS / merge sort), one
+// wave per listed row: columns and inclusive run ends of the row's unique blocks.
 __global__ __launch_bounds__(kWave) void finalize_large_rows_kernel(
     RowRanges rr, const uint64_t* __restrict__ keys, const int* __restrict__ bsr_offsets,
     int* __restrict__ bsr_columns, int* __restrict__ summed_offsets, const int* __restrict__ large_rows,
@@ -837,19 +1125,38 @@ size_t triplets_workspace_bytes(int nrow, int T)
     return needed + 256;
 }
 
+// Grid of the grid-strided medium-row kernels (one wave per listed row): the
+// list length is only known on the device, so the grid covers the worst case
+// up to 16 workgroups per compute unit.
+unsigned medium_grid(int nrow)
+{
+    const int64_t cap = (int64_t)device_cu_count() * 16;
+    const int64_t need = ceil_div(nrow, kRowsPerBlock);
+    return (unsigned)std::max<int64_t>(1, std::min(need, cap));
+}
+
 // Sort the keys of every row (rows given by rr) by (col, triplet index) and
 // count its unique columns into urow[0..nrow).  Rows of more than kWave keys
-// are listed in w.large_rows (count in w.counters[0]) and sorted in place in
-// memory, with their sorted triplet indices in summed_indices.  Smaller rows
-// are sorted in registers and, unless write_back, left unsorted in memory
-// (the finalize pass sorts them again).
+// are listed in w.medium_rows (up to kMediumCap keys, count in
+// w.counters[1]) or w.large_rows (count in w.counters[0]) and sorted in place
+// in memory, with their sorted triplet indices in summed_indices.  Smaller
+// rows are sorted in registers and, unless write_back, left unsorted in
+// memory (the finalize pass sorts them again).
 int sort_rows(int nrow, RowRanges rr, const TripletWs& w, int* summed_indices, int* urow, bool write_back,
               hipStream_t stream)
 {
     WA_TRY(launch_fill(w.counters, 64, 0, stream));
     if (nrow > 0) {
+        classify_rows_kernel<<<(unsigned)ceil_div(nrow, kBlock), kBlock, 0, stream>>>(
+            nrow, rr, w.large_rows, w.medium_rows, w.counters
+        );
+        WA_LAUNCH_CHECK();
         count_small_rows_kernel<<<(unsigned)ceil_div(nrow, kRowsPerBlock), kWave * kRowsPerBlock, 0, stream>>>(
-            nrow, rr, w.keys, summed_indices, write_back ? 1 : 0, urow, w.large_rows, w.counters
+            nrow, rr, w.keys, summed_indices, write_back ? 1 : 0, urow
+        );
+        WA_LAUNCH_CHECK();
+        sort_medium_rows_kernel<<<medium_grid(nrow), kWave * kRowsPerBlock, 0, stream>>>(
+            rr, w.keys, summed_indices, urow, w.medium_rows, w.counters
         );
         WA_LAUNCH_CHECK();
         const int large_grid = device_cu_count() * 2;
@@ -899,7 +1206,29 @@ int finalize_rows(int nrow, RowRanges rr, const TripletWs& w, bool own_summed, i
         finalize_small_rows_kernel<float, 1><<<grid, block, 0, stream>>>(
             nrow, rr, w.keys, bsr_offsets, bsr_columns, so, si, nullptr, block_size, nullptr);
     WA_LAUNCH_CHECK();
-    // rows of more than kWave keys (sorted in memory), one wave per listed row
+    // rows of kWave + 1 .. kMediumCap keys (sorted in memory), one wave per listed row
+    const unsigned mgrid = medium_grid(nrow);
+    if (sum_values && scalar_size == 4 && block_size == 9)
+        finalize_medium_rows_kernel<float, 9><<<mgrid, block, 0, stream>>>(
+            rr, w.keys, bsr_offsets, bsr_columns, so, (const float*)tpl_values, block_size, (float*)bsr_values,
+            w.medium_rows, w.counters);
+    else if (sum_values && scalar_size == 4 && block_size == 1)
+        finalize_medium_rows_kernel<float, 1><<<mgrid, block, 0, stream>>>(
+            rr, w.keys, bsr_offsets, bsr_columns, so, (const float*)tpl_values, block_size, (float*)bsr_values,
+            w.medium_rows, w.counters);
+    else if (sum_values && scalar_size == 4)
+        finalize_medium_rows_kernel<float, 0><<<mgrid, block, 0, stream>>>(
+            rr, w.keys, bsr_offsets, bsr_columns, so, (const float*)tpl_values, block_size, (float*)bsr_values,
+            w.medium_rows, w.counters);
+    else if (sum_values)
+        finalize_medium_rows_kernel<double, 0><<<mgrid, block, 0, stream>>>(
+            rr, w.keys, bsr_offsets, bsr_columns, so, (const double*)tpl_values, block_size, (double*)bsr_values,
+            w.medium_rows, w.counters);
+    else
+        finalize_medium_rows_kernel<float, 1><<<mgrid, block, 0, stream>>>(
+            rr, w.keys, bsr_offsets, bsr_columns, so, nullptr, block_size, nullptr, w.medium_rows, w.counters);
+    WA_LAUNCH_CHECK();
+    // rows of more than kMediumCap keys (sorted in memory), one wave per listed row
     const int large_grid = device_cu_count() * 4;
     finalize_large_rows_kernel<<<(unsigned)large_grid, kWave, 0, stream>>>(
         rr, w.keys, bsr_offsets, bsr_columns, summed_offsets, w.large_rows, w.counters);
@@ -920,8 +1249,8 @@ int finalize_rows(int nrow, RowRanges rr, const TripletWs& w, bool own_summed, i
 
 // Steps 1-4: bucket the valid triplets by row and sort every bucket by
 // (col, triplet index); the unique column count of every row goes to
-// urow[0..nrow).  Rows of more than kWave keys are listed in w.large_rows
-// (count in w.counters[0]) and left sorted in place at
+// urow[0..nrow).  Rows of more than kWave keys are listed in w.medium_rows /
+// w.large_rows (counts in w.counters[1] / [0]) and left sorted in place at
 // w.keys[w.row_start[r] ...] with their sorted triplet indices in
 // summed_indices.  Smaller rows are sorted in registers and, unless
 // write_back, left unsorted in memory (the finalize pass sorts them again).
