@@ -66,8 +66,8 @@ __device__ __forceinline__ RowAcc<S, BR> group_row_product(
 // of each of the RPG rows row0 .. row0+RPG-1, issuing the loads of all rows
 // before the FMAs (more bytes in flight per wave).  Each row is summed in the
 // same order as group_row_product, so results are identical to it.  On return
-// acc.v[j*BR + s] holds (A x)[(row0+j)*BR + s] in every lane of the group;
-// rows at or beyond nrow contribute zeros.
+// acc.v[j*BR + s] holds (A x)[(row0+j*rstride)*BR + s] in every lane of the
+// group; rows at or beyond nrow contribute zeros.
 //
 // V is the stored value type and S the vector / accumulation type (V = float,
 // S = double gives the FP64 residual of an FP32 matrix).
@@ -77,7 +77,7 @@ __device__ __forceinline__ RowAcc<S, BR> group_row_product(
 template <typename S, int BR, int BC, int G, int RPG, typename V, typename XF>
 __device__ __forceinline__ RowAcc<S, BR * RPG> group_rows_product_f(
     int row0, int nrow, int lane, const int* __restrict__ offsets, const int* __restrict__ row_counts,
-    const int* __restrict__ columns, const V* __restrict__ values, XF x
+    const int* __restrict__ columns, const V* __restrict__ values, XF x, int rstride = 1
 )
 {
     RowAcc<S, BR * RPG> acc;
@@ -88,7 +88,7 @@ __device__ __forceinline__ RowAcc<S, BR * RPG> group_rows_product_f(
     int nmax = 0;
 #pragma unroll
     for (int j = 0; j < RPG; ++j) {
-        const int r = row0 + j;
+        const int r = row0 + j * rstride;
         beg[j] = 0;
         cnt[j] = 0;
         if (r < nrow) {
@@ -142,11 +142,11 @@ template <typename S> struct LoadX {
 template <typename S, int BR, int BC, int G, int RPG, typename V = S>
 __device__ __forceinline__ RowAcc<S, BR * RPG> group_rows_product(
     int row0, int nrow, int lane, const int* __restrict__ offsets, const int* __restrict__ row_counts,
-    const int* __restrict__ columns, const V* __restrict__ values, const S* __restrict__ x
+    const int* __restrict__ columns, const V* __restrict__ values, const S* __restrict__ x, int rstride = 1
 )
 {
     return group_rows_product_f<S, BR, BC, G, RPG, V>(row0, nrow, lane, offsets, row_counts, columns, values,
-                                                      LoadX<S>{x});
+                                                      LoadX<S>{x}, rstride);
 }
 
 // acc.v[i] for a lane-dependent i.  Written as an explicit select chain on

@@ -10,27 +10,38 @@ namespace {
 
 constexpr int kBlock = 256;
 
-// RPG consecutive block rows per group of G lanes (see group_rows_product).
+// Interleaved rows: a wave's kWave / G groups take its kWave / G * RPG rows
+// round-robin (group g: rows g, g + kWave / G, ...), so the j-th block load of
+// all groups reads consecutive rows' blocks.  Each row's sum is unchanged.
+constexpr bool kInterleaveRows = true;
+
+// RPG block rows per group of G lanes (see group_rows_product).
 template <typename S, int BR, int BC, int G, int RPG>
 __global__ __launch_bounds__(kBlock) void bsr_mv_rows_kernel(
     int nrow, const int* __restrict__ offsets, const int* __restrict__ row_counts, const int* __restrict__ columns,
     const S* __restrict__ values, const S* __restrict__ x, S* __restrict__ y, S alpha, S beta
 )
 {
-    const int row0 = (blockIdx.x * (kBlock / G) + threadIdx.x / G) * RPG;
+    static_assert(G <= kWave, "a row group lies within one wave");
+    constexpr int kStride = kInterleaveRows ? kWave / G : 1;
+    const int wave = (int)((blockIdx.x * kBlock + threadIdx.x) / kWave);
+    const int gw = (int)(threadIdx.x % kWave) / G;
+    const int row0 = kInterleaveRows ? wave * (kWave / G) * RPG + gw
+                                     : (int)(blockIdx.x * (kBlock / G) + threadIdx.x / G) * RPG;
     const int lane = threadIdx.x % G;
     if (row0 >= nrow)
         return;
     RowAcc<S, BR * RPG> acc;
     if (alpha != S(0)) {
-        acc = group_rows_product<S, BR, BC, G, RPG>(row0, nrow, lane, offsets, row_counts, columns, values, x);
+        acc = group_rows_product<S, BR, BC, G, RPG>(row0, nrow, lane, offsets, row_counts, columns, values, x,
+                                                    kStride);
     } else {
 #pragma unroll
         for (int s = 0; s < BR * RPG; ++s)
             acc.v[s] = S(0);
     }
-    for (int e = lane; e < BR * RPG && row0 + e / BR < nrow; e += G) {
-        const size_t yi = (size_t)row0 * BR + e;
+    for (int e = lane; e < BR * RPG && row0 + (e / BR) * kStride < nrow; e += G) {
+        const size_t yi = (size_t)(row0 + (e / BR) * kStride) * BR + e % BR;
         S v = pick(acc, e) * alpha;
         if (beta != S(0))
             v = fma(beta, y[yi], v);
