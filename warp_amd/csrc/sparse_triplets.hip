@@ -999,28 +999,46 @@ __global__ __launch_bounds__(kWave) void finalize_large_rows_kernel(
     }
 }
 
-// Sequential run sums per (unique block, scalar) of the listed large rows.
+// Sequential run sums per (unique block, scalar) of the listed large rows,
+// one wave per listed row (kRowsPerBlock per workgroup).  A sum adds its
+// run's blocks in ascending triplet order, the first as the initial value;
+// the sorted triplet indices and then the values are loaded kRunBatch ahead.
 template <typename S>
-__global__ __launch_bounds__(kWave) void accumulate_large_rows_kernel(
+__global__ __launch_bounds__(kWave * kRowsPerBlock) void accumulate_large_rows_kernel(
     RowRanges rr, const int* __restrict__ bsr_offsets, const int* __restrict__ summed_offsets,
     const int* __restrict__ summed_indices, const S* __restrict__ tpl_values, int block_size,
     S* __restrict__ bsr_values, const int* __restrict__ large_rows, const int* __restrict__ counters
 )
 {
     const int nlarge = counters[0];
-    for (int li = blockIdx.x; li < nlarge; li += gridDim.x) {
+    const int nwaves = gridDim.x * kRowsPerBlock;
+    const int lane = threadIdx.x & (kWave - 1);
+    for (int li = blockIdx.x * kRowsPerBlock + (int)(threadIdx.x / kWave); li < nlarge; li += nwaves) {
         const int row = large_rows[li];
         const int u0 = bsr_offsets[row];
         const int u1 = bsr_offsets[row + 1];
         const int64_t tasks = (int64_t)(u1 - u0) * block_size;
-        for (int64_t t = threadIdx.x; t < tasks; t += kWave) {
+        for (int64_t t = lane; t < tasks; t += kWave) {
             const int u = u0 + (int)(t / block_size);
             const int s = (int)(t % block_size);
             const int beg = u == u0 ? rr.base(row) : summed_offsets[u - 1];
             const int end = summed_offsets[u];
-            S acc = tpl_values[(size_t)summed_indices[beg] * block_size + s];
-            for (int k = beg + 1; k < end; ++k)
-                acc += tpl_values[(size_t)summed_indices[k] * block_size + s];
+            S acc = S(0);
+            for (int k = beg; k < end; k += kRunBatch) {
+                int iq[kRunBatch];
+#pragma unroll
+                for (int q = 0; q < kRunBatch; ++q)
+                    iq[q] = k + q < end ? summed_indices[k + q] : 0;
+                S v[kRunBatch];
+#pragma unroll
+                for (int q = 0; q < kRunBatch; ++q)
+                    if (k + q < end)
+                        v[q] = tpl_values[(size_t)iq[q] * block_size + s];
+#pragma unroll
+                for (int q = 0; q < kRunBatch; ++q)
+                    if (k + q < end)
+                        acc = (k + q == beg) ? v[q] : acc + v[q];
+            }
             bsr_values[(size_t)u * block_size + s] = acc;
         }
     }
@@ -1235,11 +1253,11 @@ int finalize_rows(int nrow, RowRanges rr, const TripletWs& w, bool own_summed, i
     WA_LAUNCH_CHECK();
     if (sum_values) {
         if (scalar_size == 4)
-            accumulate_large_rows_kernel<float><<<(unsigned)large_grid, kWave, 0, stream>>>(
+            accumulate_large_rows_kernel<float><<<mgrid, block, 0, stream>>>(
                 rr, bsr_offsets, summed_offsets, summed_indices, (const float*)tpl_values, block_size,
                 (float*)bsr_values, w.large_rows, w.counters);
         else
-            accumulate_large_rows_kernel<double><<<(unsigned)large_grid, kWave, 0, stream>>>(
+            accumulate_large_rows_kernel<double><<<mgrid, block, 0, stream>>>(
                 rr, bsr_offsets, summed_offsets, summed_indices, (const double*)tpl_values, block_size,
                 (double*)bsr_values, w.large_rows, w.counters);
         WA_LAUNCH_CHECK();
