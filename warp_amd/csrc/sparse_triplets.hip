@@ -1177,7 +1177,7 @@ int sort_rows(int nrow, RowRanges rr, const TripletWs& w, int* summed_indices, i
             rr, w.keys, summed_indices, urow, w.medium_rows, w.counters
         );
         WA_LAUNCH_CHECK();
-        const int large_grid = device_cu_count() * 2;
+        const int large_grid = device_cu_count() * 4;  // 32 KB of LDS each: up to 5 per compute unit
         sort_large_rows_kernel<<<(unsigned)large_grid, kBlock, 0, stream>>>(
             rr, w.keys, w.alt, summed_indices, urow, w.large_rows, w.counters
         );
