@@ -961,16 +961,18 @@ __global__ __launch_bounds__(kBlock) void sort_large_rows_kernel(
 }
 
 // Rows above kMediumCap keys (sorted in memory by the LDS / merge sort), one
-// wave per listed row: columns and inclusive run ends of the row's unique blocks.
-__global__ __launch_bounds__(kWave) void finalize_large_rows_kernel(
+// wave per listed row (kRowsPerBlock per workgroup): columns and inclusive run
+// ends of the row's unique blocks.
+__global__ __launch_bounds__(kWave * kRowsPerBlock) void finalize_large_rows_kernel(
     RowRanges rr, const uint64_t* __restrict__ keys, const int* __restrict__ bsr_offsets,
     int* __restrict__ bsr_columns, int* __restrict__ summed_offsets, const int* __restrict__ large_rows,
     const int* __restrict__ counters
 )
 {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1);
     const int nlarge = counters[0];
-    for (int li = blockIdx.x; li < nlarge; li += gridDim.x) {
+    const int nwaves = gridDim.x * kRowsPerBlock;
+    for (int li = blockIdx.x * kRowsPerBlock + (int)(threadIdx.x / kWave); li < nlarge; li += nwaves) {
         const int row = large_rows[li];
         const int rs = rr.base(row);
         const int re = rs + rr.size(row);
@@ -1247,8 +1249,7 @@ int finalize_rows(int nrow, RowRanges rr, const TripletWs& w, bool own_summed, i
             rr, w.keys, bsr_offsets, bsr_columns, so, nullptr, block_size, nullptr, w.medium_rows, w.counters);
     WA_LAUNCH_CHECK();
     // rows of more than kMediumCap keys (sorted in memory), one wave per listed row
-    const int large_grid = device_cu_count() * 4;
-    finalize_large_rows_kernel<<<(unsigned)large_grid, kWave, 0, stream>>>(
+    finalize_large_rows_kernel<<<mgrid, block, 0, stream>>>(
         rr, w.keys, bsr_offsets, bsr_columns, summed_offsets, w.large_rows, w.counters);
     WA_LAUNCH_CHECK();
     if (sum_values) {
