@@ -632,37 +632,38 @@ __global__ __launch_bounds__(kWave * kRowsPerBlock) void count_small_rows_kernel
 // keys) or R = kMediumRegs registers per lane of one wave, element
 // i = q * kWave + lane.
 
-// Bitonic sort of an R x kWave register tile (kPadKey past the row's end),
-// ascending by element index.  Every trip count is a compile-time constant,
-// so the network unrolls with static register indices: exchanges at distance
-// >= kWave swap registers inside a lane, shorter ones go across lanes.
-template <int R, int N>
-__device__ __forceinline__ void wave_sort_regs(uint64_t (&k)[N], int lane)
+// Bitonic sort of a lane-major register tile, element i = lane * R + q
+// (kPadKey past the row's end), ascending by element index.  Every trip count
+// is a compile-time constant, so the network unrolls with static register
+// indices: exchanges at distance < R stay inside a lane, longer ones go across
+// lanes (distance / R apart), which leaves fewer cross-lane steps than a
+// register-major tile.
+template <int R>
+__device__ __forceinline__ void wave_sort_lane_major(uint64_t (&k)[R], int lane)
 {
-    static_assert(R <= N, "register tile too small");
 #pragma unroll
     for (int size = 2; size <= R * kWave; size <<= 1) {
 #pragma unroll
         for (int j = size >> 1; j > 0; j >>= 1) {
-            if (j >= kWave) {
-                const int jq = j / kWave;
+            if (j < R) {
 #pragma unroll
                 for (int q = 0; q < R; ++q) {
-                    if (q & jq)
+                    if (q & j)
                         continue;
-                    const bool up = ((q * kWave) & size) == 0;
+                    const bool up = ((lane * R + q) & size) == 0;
                     const uint64_t a = k[q];
-                    const uint64_t b = k[q | jq];
+                    const uint64_t b = k[q | j];
                     const bool keep = (a < b) == up;
                     k[q] = keep ? a : b;
-                    k[q | jq] = keep ? b : a;
+                    k[q | j] = keep ? b : a;
                 }
             } else {
+                const int lj = j / R;
+                const bool lower = (lane & lj) == 0;
 #pragma unroll
                 for (int q = 0; q < R; ++q) {
-                    const uint64_t other = shfl_xor_u64(k[q], j);
-                    const bool up = ((q * kWave + lane) & size) == 0;
-                    const bool lower = (lane & j) == 0;
+                    const uint64_t other = shfl_xor_u64(k[q], lj);
+                    const bool up = ((lane * R + q) & size) == 0;
                     // the lower element of an ascending pair keeps the smaller key
                     const bool take_other = (other < k[q]) == (lower == up);
                     k[q] = take_other ? other : k[q];
@@ -711,24 +712,27 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One medium row, loaded into k (kPadKey past its end): sorted in registers,
-// stored back in place with its sorted triplet indices; returns its unique
-// column count.
+// One medium row (at most R * kWave keys): loaded lane-major, sorted in
+// registers, stored back in place with its sorted triplet indices; returns its
+// unique column count.
 template <int R>
-__device__ __forceinline__ int sort_medium_row(uint64_t (&k)[kMediumRegs], uint64_t* keys, int* summed_indices,
-                                               int base, int size, int lane)
+__device__ __forceinline__ int sort_medium_row(uint64_t* keys, int* summed_indices, int base, int size, int lane)
 {
-    wave_sort_regs<R>(k, lane);
-    int c[R];
+    uint64_t k[R];
 #pragma unroll
-    for (int q = 0; q < R; ++q)
-        c[q] = (int)(k[q] >> 32);
+    for (int q = 0; q < R; ++q) {
+        const int i = lane * R + q;
+        k[q] = i < size ? keys[base + i] : kPadKey;
+    }
+    wave_sort_lane_major<R>(k, lane);
+    const int prev_lane_last = __shfl_up((int)(k[R - 1] >> 32), 1);
     int nu = 0;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-        const int i = q * kWave + lane;
-        const int prev = prev_col<R>(c, q, lane);
-        const bool head = i < size && (i == 0 || prev != c[q]);
+        const int i = lane * R + q;
+        const int c = (int)(k[q] >> 32);
+        const int prev = q > 0 ? (int)(k[q - 1] >> 32) : prev_lane_last;
+        const bool head = i < size && (i == 0 || prev != c);
         nu += __popcll(__ballot(head));
         if (i < size) {
             keys[base + i] = k[q];
@@ -751,10 +755,8 @@ __global__ __launch_bounds__(kWave * kRowsPerBlock) void sort_medium_rows_kernel
         const int row = medium_rows[li];
         const int base = rr.base(row);
         const int size = rr.size(row);
-        uint64_t k[kMediumRegs];
-        load_medium_row<kMediumRegs>(keys, base, size, lane, k);
-        const int nu = size <= 2 * kWave ? sort_medium_row<2>(k, keys, summed_indices, base, size, lane)
-                                         : sort_medium_row<kMediumRegs>(k, keys, summed_indices, base, size, lane);
+        const int nu = size <= 2 * kWave ? sort_medium_row<2>(keys, summed_indices, base, size, lane)
+                                         : sort_medium_row<kMediumRegs>(keys, summed_indices, base, size, lane);
         if (lane == 0)
             urow[row] = nu;
     }
