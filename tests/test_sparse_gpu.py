@@ -111,13 +111,14 @@ def test_triplets_long_rows(device, oracle_mod, per_row):
     _check_against_oracle(oracle_mod, A, nrow, ncol, rows, cols, vals)
 
 
-ROW_CLASS_SIZES = [0, 1, 2, 63, 64, 65, 66, 100, 127, 128, 129, 200, 255, 256, 257, 300]
+ROW_CLASS_SIZES = [0, 1, 2, 63, 64, 65, 66, 100, 127, 128, 129, 200, 255, 256, 257, 300, 511, 512, 513, 600]
 
 
 @pytest.mark.parametrize("bshape,dtype", [((), np.float32), ((3, 3), np.float32), ((2, 2), np.float64)])
 def test_triplets_row_size_classes(device, oracle_mod, bshape, dtype):
     """Rows on both sides of each sort-path boundary (one key per lane: 64
-    keys; one wave's registers: 256 keys), heavy and no duplication, in
+    keys; two, four and eight keys per lane: 128, 256 and 512), heavy and no
+    duplication, in
     shuffled order; then the transpose of the result, whose rows cross the
     same boundaries."""
     rng = np.random.default_rng(4242 + len(bshape))

@@ -33,7 +33,7 @@ namespace wa {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMediumRegs = 4;                     // keys per lane of a medium row
+constexpr int kMediumRegs = 8;                     // keys per lane of the longest medium rows
 constexpr int kMediumCap = kMediumRegs * kWave;    // rows sorted in one wave's registers
 constexpr int kLargeCap = 4096;                    // rows sorted in one workgroup's LDS
 constexpr uint64_t kPadKey = ~uint64_t(0);
@@ -628,9 +628,8 @@ __global__ __launch_bounds__(kWave * kRowsPerBlock) void count_small_rows_kernel
     }
 }
 
-// Medium rows (kWave + 1 .. kMediumCap keys) live in R = 2 (up to 2 * kWave
-// keys) or R = kMediumRegs registers per lane of one wave, element
-// i = q * kWave + lane.
+// Medium rows (kWave + 1 .. kMediumCap keys) live in R = 2, 4 or 8 registers
+// per lane of one wave (the smallest R with R * kWave >= the row's keys).
 
 // Bitonic sort of a lane-major register tile, element i = lane * R + q
 // (kPadKey past the row's end), ascending by element index.  Every trip count
@@ -755,8 +754,9 @@ __global__ __launch_bounds__(kWave * kRowsPerBlock) void sort_medium_rows_kernel
         const int row = medium_rows[li];
         const int base = rr.base(row);
         const int size = rr.size(row);
-        const int nu = size <= 2 * kWave ? sort_medium_row<2>(keys, summed_indices, base, size, lane)
-                                         : sort_medium_row<kMediumRegs>(keys, summed_indices, base, size, lane);
+        const int nu = size <= 2 * kWave   ? sort_medium_row<2>(keys, summed_indices, base, size, lane)
+                       : size <= 4 * kWave ? sort_medium_row<4>(keys, summed_indices, base, size, lane)
+                                           : sort_medium_row<kMediumRegs>(keys, summed_indices, base, size, lane);
         if (lane == 0)
             urow[row] = nu;
     }
@@ -833,6 +833,8 @@ __global__ __launch_bounds__(kWave * kRowsPerBlock) void finalize_medium_rows_ke
         const int runs =
             size <= 2 * kWave
                 ? medium_row_runs<2>(keys, base, size, out0, lane, bsr_columns, summed_offsets, idx_of, beg_of)
+            : size <= 4 * kWave
+                ? medium_row_runs<4>(keys, base, size, out0, lane, bsr_columns, summed_offsets, idx_of, beg_of)
                 : medium_row_runs<kMediumRegs>(keys, base, size, out0, lane, bsr_columns, summed_offsets, idx_of,
                                                beg_of);
         if (bsr_values) {
