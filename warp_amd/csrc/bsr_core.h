@@ -104,9 +104,12 @@ __device__ __forceinline__ RowAcc<S, BR * RPG> group_rows_product_f(
 #pragma unroll
         for (int j = 0; j < RPG; ++j) {
             ok[j] = k < cnt[j];
-            // inactive slots read block 0 (always valid when any row has blocks)
+            // Inactive slots read storage slot 0 (in bounds whenever any row
+            // has blocks) and gather x at column 0: slot 0 of a padded matrix
+            // may be an unused slot whose column is -1.
             const int b = ok[j] ? beg[j] + k : 0;
-            col[j] = columns[b];
+            const int cb = columns[b];
+            col[j] = ok[j] ? cb : 0;
             const V* blk = values + (size_t)b * (BR * BC);
 #pragma unroll
             for (int q = 0; q < BR * BC; ++q)

@@ -9,20 +9,27 @@
 // every dot product is accumulated in FP64 with a deterministic two-level
 // reduction, and alpha/beta are formed in FP64.
 //
-// One iteration = three fused kernels on a device-resident state, the first
-// two followed by a one-workgroup finalize kernel that sums their per-
-// workgroup partials in a fixed order:
-//   spmv_pap : Ap = A p, pAp = p.Ap            -> alpha = rz / pAp  (0 once converged)
-//   update   : x += alpha p, r -= alpha Ap, z = M r, rr = r.r, rz = r.z -> beta, done
-//   direction: p = z + beta p
+// One iteration = three fused kernels on a device-resident state and no
+// one-workgroup finalize kernel: each reduction leaves one FP64 partial per
+// workgroup, and every workgroup of the next kernel sums all of them in the
+// same fixed order in its prologue (bit-identical in every workgroup; the
+// grids are capped so that these sums stay a few KB of L2 reads):
+//   spmv     : Ap = A p, partials of p.Ap
+//   update   : alpha = rz / pAp (0 once converged); x += alpha p,
+//              r -= alpha Ap, z = M r, partials of r.r and r.z
+//   direction: beta = rz_new / rz_old, done; p = z + beta p
+// A field of the state is never written by a kernel whose workgroups read
+// it: workgroup 0 of each kernel records what the next one needs (r.z of the
+// previous iterate, `done` as the update saw it, the iteration count).
 // Small systems (launch-bound: every kernel costs a few microseconds however
-// little it does) fold the direction pass into the next SpMV: it evaluates
-// p' = z + beta p entry by entry where the product gathers it (the same fma,
-// so the iterates are bit-identical) and writes p' to the other of two
-// direction buffers, since workgroups still gather p while others write p'.
-// beta is 0 after (re)initialisation, which makes p' = z.  Large systems keep
-// the separate pass: gathering two vectors costs the HBM-bound SpMV more
-// than the pass it saves.
+// little it does) fold the direction pass into the next SpMV: its prologue
+// sums the update's partials into beta, and it evaluates p' = z + beta p
+// entry by entry where the product gathers it (the same fma, so the iterates
+// are bit-identical) and writes p' to the other of two direction buffers,
+// since workgroups still gather p while others write p'.  r.z of the
+// previous iterate is +inf after (re)initialisation, which makes beta 0 and
+// p' = z.  Large systems keep the separate pass: gathering two vectors costs
+// the HBM-bound SpMV more than the pass it saves.
 // The convergence flag `done` (r.r <= atol^2) makes later kernels no-ops, the
 // device-side equivalent of the reference's frozen alpha/beta.
 #include "bsr_core.h"
@@ -34,45 +41,61 @@ namespace wa {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMaxGrid = 2048;
+constexpr int kMaxGrid = 2048;  // generic vector kernels
 
 struct CgState {
     double bb;
-    double rr;
-    double rz;
+    double rr;       // r.r of the current iterate (host-visible)
+    double rz;       // r.z of the current iterate
+    double rz_prev;  // r.z of the previous iterate; +inf after (re)initialisation, so beta = 0
     double pAp;
     double atol_sq;
     double alpha;
     double beta;
     double rtol;
     double atol;
-    int iter;
-    int done;
+    int iter;       // iterations done (host-visible)
+    int iter_copy;  // the SpMV's copy of iter (fused direction buffers), read by the update
+    int done;       // r.r <= atol^2 for the current iterate
+    int done_prev;  // `done` as the update saw it, read by the direction pass
 };
 
 enum PrecKind { kPrecNone = 0, kPrecDiag = 1, kPrecBlock = 2 };
 
+// Grid caps.  Every reduction of an iteration leaves one FP64 partial per
+// workgroup, and the next kernel's workgroups each sum all of them in the
+// same fixed order in their prologue (so no one-workgroup finalize kernel
+// sits between two work kernels): the SpMV walks its row tiles with at most
+// kMaxSpmvGrid workgroups, the update and direction passes grid-stride with
+// at most kMaxVecGrid.
+constexpr int kMaxSpmvGrid = 2048;
+constexpr int kMaxVecGrid = 1024;
+
 struct CgWs {
     CgState* state;
-    double* partials;  // 3 * max_blocks
-    int max_blocks;
+    double* init_partials;  // 3 * max_init_blocks (b.b, r.r, r.z of the initial residual)
+    double* spmv_partials;  // kMaxSpmvGrid (p.Ap)
+    double* upd_partials;   // 2 * kMaxVecGrid (r.r, r.z)
+    int max_init_blocks;
     void* r;
     void* z;
-    void* p;   // direction buffers: iterations alternate which one holds p
+    void* p;   // direction buffers: iterations alternate which one holds p (fused path)
     void* p2;
     void* Ap;
 };
 
-// Largest grid of any CG kernel: one workgroup per 256/G block rows, G <= 64.
+// Largest row grid (the initial residual): one workgroup per 256/G block rows, G <= 64.
 int cg_max_blocks(int64_t nrow) { return (int)std::max<int64_t>(1, ceil_div(nrow, 4)); }
 
 CgWs carve_cg(void* base, size_t bytes, int64_t nrow, int64_t n, size_t scalar_bytes, size_t* needed)
 {
     Carve c(base, bytes);
     CgWs w;
-    w.max_blocks = cg_max_blocks(nrow);
+    w.max_init_blocks = cg_max_blocks(nrow);
     w.state = c.take<CgState>(1);
-    w.partials = c.take<double>((size_t)3 * w.max_blocks);
+    w.init_partials = c.take<double>((size_t)3 * w.max_init_blocks);
+    w.spmv_partials = c.take<double>((size_t)kMaxSpmvGrid);
+    w.upd_partials = c.take<double>((size_t)2 * kMaxVecGrid);
     w.r = c.take<char>((size_t)n * scalar_bytes);
     w.z = c.take<char>((size_t)n * scalar_bytes);
     w.p = c.take<char>((size_t)n * scalar_bytes);
@@ -84,15 +107,41 @@ CgWs carve_cg(void* base, size_t bytes, int64_t nrow, int64_t n, size_t scalar_b
 
 constexpr int kFinalizeBlock = 1024;
 
-// Direction buffers.  Unfused, p stays in ws.p.  Fused, iteration k
-// (state->iter) reads buffer k & 1 and writes the other: the choice is made
-// on the device, so a captured iteration replays correctly any number of
-// times.
+// Fixed-order sum of NV rows of `count` <= MAXP partials (row v at parts + v *
+// stride) by one workgroup of kBlock threads, result in every thread.  Every
+// workgroup of a kernel that calls it gets bit-identical sums: thread t adds
+// parts[t], parts[t + kBlock], ... in order, then the same block reduction.
+template <int NV, int MAXP>
+__device__ __forceinline__ void sum_parts(const double* __restrict__ parts, size_t stride, int count, double* lds,
+                                          double (&out)[NV])
+{
+    constexpr int kPer = MAXP / kBlock;
+    static_assert(MAXP % kBlock == 0, "partial capacity is a multiple of the block");
+    double t[NV][kPer];
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int k = (int)threadIdx.x + u * kBlock;
+            t[v][u] = k < count ? parts[(size_t)v * stride + k] : 0.0;
+        }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        double a = 0.0;
+#pragma unroll
+        for (int u = 0; u < kPer; ++u)
+            a += t[v][u];
+        out[v] = block_sum<kBlock>(a, lds);
+    }
+}
+
+// Direction buffers.  Unfused, p stays in ws.p.  Fused, iteration k reads
+// buffer k & 1 and writes the other: the choice is made on the device, so a
+// captured iteration replays correctly any number of times.
 template <typename S> __device__ __forceinline__ S* dir_buffer(const CgWs& ws, int which)
 {
     return static_cast<S*>(which ? ws.p2 : ws.p);
 }
-
 
 // Vectors up to this size fold the direction pass into the SpMV (measured on
 // MI355X: 2D Poisson, 16,129 dofs, 18 -> 15 us per iteration; 823,875-dof
@@ -123,7 +172,6 @@ __device__ __forceinline__ void apply_prec(const S* __restrict__ prec, int row, 
             z[s] = r[s];
     }
 }
-
 
 // r = b - A x ; z = M r ; p = z ; partials of (b.b, r.r, r.z).  The grid
 // covers the matrix: kCgRpg consecutive block rows per group of G lanes.
@@ -164,7 +212,42 @@ __global__ __launch_bounds__(kBlock) void cg_init_kernel(
             }
         }
     }
-    write_block_partials<kBlock, 3>(acc3, ws.partials, lds);
+    write_block_partials<kBlock, 3>(acc3, ws.init_partials, lds);
+}
+
+// One workgroup: sums the initial residual's partials and (re)initialises
+// the state; on the fused path also leaves r.r and r.z as the update's
+// partials (first entry, zeros after it), which the first SpMV sums back
+// exactly.
+__global__ __launch_bounds__(kFinalizeBlock) void cg_init_finalize_kernel(CgWs ws, int nparts, int upd_parts,
+                                                                          int has_prec, int first, int fused)
+{
+    __shared__ double lds[kFinalizeBlock / kWave];
+    CgState* state = ws.state;
+    double tot[3];
+    sum_partials_rows<kFinalizeBlock, 3>(ws.init_partials, (size_t)nparts, nparts, lds, tot);
+    const double rz = has_prec ? tot[2] : tot[1];
+    if (fused) {
+        for (int k = threadIdx.x; k < upd_parts; k += kFinalizeBlock) {
+            ws.upd_partials[k] = k == 0 ? tot[1] : 0.0;
+            ws.upd_partials[upd_parts + k] = k == 0 ? rz : 0.0;
+        }
+    }
+    if (threadIdx.x != 0)
+        return;
+    if (first) {
+        state->bb = tot[0];
+        const double a = fmax(state->rtol * sqrt(tot[0]), state->atol);
+        state->atol_sq = a * a;
+        state->iter = 0;
+    }
+    state->rr = tot[1];
+    state->rz = rz;
+    state->rz_prev = __builtin_inf();
+    state->done = !(tot[1] > state->atol_sq);
+    state->done_prev = state->done;
+    state->alpha = 0.0;
+    state->beta = 0.0;
 }
 
 // p' = z + beta p, entry i.
@@ -175,26 +258,54 @@ template <typename S> struct NextDirection {
     __device__ __forceinline__ S operator()(size_t i) const { return fma(beta, p[i], z[i]); }
 };
 
-// FUSED: p' = z + beta p evaluated on the fly and written to p_next (the
-// product is A p'); otherwise the product is A p and z / p_next are unused.
+// Ap = A p and p.Ap partials, the grid walking tiles of kBlock / G * kCgRpg
+// block rows.  FUSED: the prologue first sums the update's partials into
+// r.r and r.z of the current iterate, hence beta and `done` (workgroup 0
+// records them), and the product is A p' with p' = z + beta p evaluated
+// where it is gathered and written to the other direction buffer.
+// Unfused: `done` and p come from the direction pass.
 template <typename S, int BR, int G, bool FUSED, int kCgRpg = rows_per_group_for(BR, BR)>
 __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(
     int nrow, const int* __restrict__ offsets, const int* __restrict__ columns, const S* __restrict__ values,
-    const S* __restrict__ z, S* __restrict__ Ap, CgWs ws
+    const S* __restrict__ z, S* __restrict__ Ap, int upd_parts, CgWs ws
 )
 {
     __shared__ double lds[kBlock / kWave];
-    if (ws.state->done)
-        return;
-    const int par = FUSED ? (ws.state->iter & 1) : 0;
+    CgState* state = ws.state;
+    S beta = S(0);
+    int par = 0;
+    if constexpr (FUSED) {
+        double t[2];
+        sum_parts<2, kMaxVecGrid>(ws.upd_partials, (size_t)upd_parts, upd_parts, lds, t);
+        const bool live = t[0] > state->atol_sq;
+        const double bt = live ? t[1] / state->rz_prev : 0.0;
+        par = state->iter & 1;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            state->rr = t[0];
+            state->rz = t[1];
+            state->beta = bt;
+            state->done = !live;
+            state->iter_copy = state->iter;
+        }
+        if (!live)
+            return;
+        beta = (S)bt;
+    } else {
+        if (state->done)
+            return;
+    }
     const S* __restrict__ p = dir_buffer<S>(ws, par);
-    const int row0 = (blockIdx.x * (kBlock / G) + threadIdx.x / G) * kCgRpg;
+    S* __restrict__ p_next = dir_buffer<S>(ws, par ^ 1);
+    const NextDirection<S> dir{z, p, beta};
+    constexpr int kTileRows = (kBlock / G) * kCgRpg;
+    const int ntiles = (nrow + kTileRows - 1) / kTileRows;
     const int lane = threadIdx.x % G;
     double acc[1] = {0.0};
-    if (row0 < nrow) {
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int row0 = (tile * (kBlock / G) + (int)threadIdx.x / G) * kCgRpg;
+        if (row0 >= nrow)
+            continue;
         if constexpr (FUSED) {
-            S* __restrict__ p_next = dir_buffer<S>(ws, par ^ 1);
-            const NextDirection<S> dir{z, p, (S)ws.state->beta};
             const RowAcc<S, BR * kCgRpg> ap = group_rows_product_f<S, BR, BR, G, kCgRpg, S>(
                 row0, nrow, lane, offsets, nullptr, columns, values, dir
             );
@@ -217,36 +328,73 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(
             }
         }
     }
-    write_block_partials<kBlock, 1>(acc, ws.partials, lds);
+    write_block_partials<kBlock, 1>(acc, ws.spmv_partials, lds);
 }
 
-// Block rows grid-strided over at most kMaxUpdateGrid workgroups, so the
-// finalize kernel after it sums few partials (a one-workgroup sum of 65 K
-// partials cost 17 us per iteration at 16.7 M rows).  The SpMV keeps one
-// workgroup per 256 rows: capping its grid the same way measured slower
-// (405 -> 416-430 us per iteration at 256^3).
-constexpr int kMaxUpdateGrid = 4096;
-
+// x += alpha p, r -= alpha Ap, z = M r, partials of (r.r, r.z).  The
+// prologue sums the SpMV's partials into p.Ap and forms alpha; workgroup 0
+// records alpha and the values the next kernel must read unchanged (r.z of
+// this iterate, `done`, the iteration count).  Each thread loads its first
+// rows before that sum, so the L2 round trips of the prologue overlap the
+// HBM loads.
 template <typename S, int BR, int PREC>
 __global__ __launch_bounds__(kBlock) void cg_update_kernel(
     int nrow, const S* __restrict__ prec, S* __restrict__ x, S* __restrict__ r, S* __restrict__ z,
-    const S* __restrict__ Ap, int fused, CgWs ws
+    const S* __restrict__ Ap, int fused, int spmv_parts, CgWs ws
 )
 {
     __shared__ double lds[kBlock / kWave];
-    if (ws.state->done)
+    CgState* state = ws.state;
+    const int done = state->done;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        state->rz_prev = state->rz;
+        state->done_prev = done;
+        if (!done)
+            state->iter = (fused ? state->iter_copy : state->iter) + 1;
+    }
+    if (done)
         return;
     // the direction this iteration's SpMV used (fused: the buffer it wrote)
-    const S* __restrict__ p = dir_buffer<S>(ws, fused ? ((ws.state->iter & 1) ^ 1) : 0);
-    const S alpha = (S)ws.state->alpha;
+    const S* __restrict__ p = dir_buffer<S>(ws, fused ? ((state->iter_copy & 1) ^ 1) : 0);
+    const int stride = (int)gridDim.x * kBlock;
+    int row = (int)(blockIdx.x * kBlock + threadIdx.x);
+    S pv[BR], apv[BR], xv[BR], rv0[BR];
+    if (row < nrow) {
+#pragma unroll
+        for (int s = 0; s < BR; ++s) {
+            const size_t i = (size_t)row * BR + s;
+            pv[s] = p[i];
+            apv[s] = Ap[i];
+            xv[s] = x[i];
+            rv0[s] = r[i];
+        }
+    }
+    double t[1];
+    sum_parts<1, kMaxSpmvGrid>(ws.spmv_partials, 0, spmv_parts, lds, t);
+    const double a = state->rr > state->atol_sq ? state->rz / t[0] : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        state->pAp = t[0];
+        state->alpha = a;
+    }
+    const S alpha = (S)a;
     double acc[2] = {0.0, 0.0};
-    for (int row = blockIdx.x * kBlock + threadIdx.x; row < nrow; row += gridDim.x * kBlock) {
+    for (bool first = true; row < nrow; row += stride, first = false) {
+        if (!first) {
+#pragma unroll
+            for (int s = 0; s < BR; ++s) {
+                const size_t i = (size_t)row * BR + s;
+                pv[s] = p[i];
+                apv[s] = Ap[i];
+                xv[s] = x[i];
+                rv0[s] = r[i];
+            }
+        }
         S rv[BR], zv[BR];
 #pragma unroll
         for (int s = 0; s < BR; ++s) {
             const size_t i = (size_t)row * BR + s;
-            x[i] = fma(alpha, p[i], x[i]);
-            rv[s] = fma(-alpha, Ap[i], r[i]);
+            x[i] = fma(alpha, pv[s], xv[s]);
+            rv[s] = fma(-alpha, apv[s], rv0[s]);
             r[i] = rv[s];
         }
         apply_prec<S, BR, PREC>(prec, row, rv, zv);
@@ -258,62 +406,48 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(
             acc[1] += (double)rv[s] * (double)zv[s];
         }
     }
-    write_block_partials<kBlock, 2>(acc, ws.partials, lds);
+    write_block_partials<kBlock, 2>(acc, ws.upd_partials, lds);
 }
 
-// Finalize kernels: one workgroup sums the partials of the preceding kernel
-// (fixed order) and updates the device state.
-enum FinalizeStep { kFinInit = 0, kFinSpmv = 1, kFinUpdate = 2 };
-
-template <int STEP>
-__global__ __launch_bounds__(kFinalizeBlock) void cg_finalize_kernel(CgWs ws, int nparts, int has_prec, int first)
-{
-    __shared__ double lds[kFinalizeBlock / kWave];
-    CgState* state = ws.state;
-    if (STEP != kFinInit && state->done)
-        return;
-    constexpr int NV = STEP == kFinInit ? 3 : (STEP == kFinSpmv ? 1 : 2);
-    double tot[NV];
-    sum_partials_rows<kFinalizeBlock, NV>(ws.partials, (size_t)nparts, nparts, lds, tot);
-    if (threadIdx.x != 0)
-        return;
-    if (STEP == kFinInit) {
-        if (first) {
-            state->bb = tot[0];
-            const double a = fmax(state->rtol * sqrt(tot[0]), state->atol);
-            state->atol_sq = a * a;
-            state->iter = 0;
-        }
-        state->rr = tot[1];
-        state->rz = has_prec ? tot[NV - 1] : tot[1];
-        state->done = !(tot[1] > state->atol_sq);
-        state->alpha = 0.0;
-        state->beta = 0.0;
-    } else if (STEP == kFinSpmv) {
-        state->pAp = tot[0];
-        state->alpha = state->rr > state->atol_sq ? state->rz / tot[0] : 0.0;
-    } else {
-        const double rz_old = state->rz;
-        const double rz_new = has_prec ? tot[NV - 1] : tot[0];
-        state->rr = tot[0];
-        state->rz = rz_new;
-        state->beta = tot[0] > state->atol_sq ? rz_new / rz_old : 0.0;
-        state->done = !(tot[0] > state->atol_sq);
-        state->iter += 1;
-    }
-}
-
+// p = z + beta p (unfused path).  The prologue sums the update's partials
+// into r.r and r.z of the new iterate, forms beta and `done`; workgroup 0
+// records them.
 template <typename S>
-__global__ __launch_bounds__(kBlock) void cg_direction_kernel(
-    int64_t n, const S* __restrict__ z, S* __restrict__ p, const CgState* state
-)
+__global__ __launch_bounds__(kBlock) void cg_direction_kernel(int64_t n, const S* __restrict__ z, S* __restrict__ p,
+                                                              int upd_parts, CgWs ws)
 {
-    if (state->done)
+    __shared__ double lds[kBlock / kWave];
+    CgState* state = ws.state;
+    if (state->done_prev)
         return;
-    const S beta = (S)state->beta;
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i < n)
+    double t[2];
+    sum_parts<2, kMaxVecGrid>(ws.upd_partials, (size_t)upd_parts, upd_parts, lds, t);
+    const bool live = t[0] > state->atol_sq;
+    const double bt = live ? t[1] / state->rz_prev : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        state->rr = t[0];
+        state->rz = t[1];
+        state->beta = bt;
+        state->done = !live;
+    }
+    const S beta = (S)bt;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
         p[i] = fma(beta, p[i], z[i]);
+}
+
+// Fused path at a residual check: r.r, r.z and `done` of the current iterate
+// from the update's partials (the next SpMV recomputes the same sums).
+__global__ __launch_bounds__(kBlock) void cg_check_kernel(int upd_parts, CgWs ws)
+{
+    __shared__ double lds[kBlock / kWave];
+    CgState* state = ws.state;
+    double t[2];
+    sum_parts<2, kMaxVecGrid>(ws.upd_partials, (size_t)upd_parts, upd_parts, lds, t);
+    if (threadIdx.x == 0) {
+        state->rr = t[0];
+        state->rz = t[1];
+        state->done = !(t[0] > state->atol_sq);
+    }
 }
 
 __global__ void cg_state_init_kernel(CgWs ws, double rtol, double atol)
@@ -323,7 +457,9 @@ __global__ void cg_state_init_kernel(CgWs ws, double rtol, double atol)
     state->atol = atol;
     state->atol_sq = 0.0;
     state->done = 0;
+    state->done_prev = 0;
     state->iter = 0;
+    state->iter_copy = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -394,15 +530,17 @@ template <typename S> struct CgLauncher {
     {
         return (unsigned)ceil_div(pb.nrow, (kBlock / G) * rows_per_group_for(BR, BR));
     }
-
-    template <int STEP> int finalize(unsigned nparts, int first)
+    template <int BR, int G> unsigned spmv_grid() const
     {
-        cg_finalize_kernel<STEP><<<1, kFinalizeBlock, 0, stream>>>(
-            ws, (int)nparts, pb.prec_kind != kPrecNone ? 1 : 0, first
-        );
-        WA_LAUNCH_CHECK();
-        return kOk;
+        return std::min<unsigned>(rows_grid<BR, G>(), (unsigned)kMaxSpmvGrid);
     }
+    static unsigned vec_grid(int64_t count)
+    {
+        return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(count, kBlock), kMaxVecGrid));
+    }
+    // workgroups (= partials) of the update pass: one thread per block row
+    unsigned upd_grid() const { return vec_grid(pb.nrow); }
+
     template <int BR, int G, int PREC> int init_t(int first)
     {
         const unsigned grid = rows_grid<BR, G>();
@@ -411,30 +549,46 @@ template <typename S> struct CgLauncher {
             (const S*)pb.x, (S*)ws.r, (S*)ws.z, fused ? 1 : 0, ws
         );
         WA_LAUNCH_CHECK();
-        return finalize<kFinInit>(grid, first);
+        cg_init_finalize_kernel<<<1, kFinalizeBlock, 0, stream>>>(
+            ws, (int)grid, (int)upd_grid(), pb.prec_kind != kPrecNone ? 1 : 0, first, fused ? 1 : 0
+        );
+        WA_LAUNCH_CHECK();
+        return kOk;
     }
     template <int BR, int G> int spmv_t()
     {
-        const unsigned grid = rows_grid<BR, G>();
+        const unsigned grid = spmv_grid<BR, G>();
         if (fused)
             cg_spmv_kernel<S, BR, G, true><<<grid, kBlock, 0, stream>>>(
-                pb.nrow, pb.offsets, pb.columns, (const S*)pb.values, z_vec(), (S*)ws.Ap, ws
+                pb.nrow, pb.offsets, pb.columns, (const S*)pb.values, z_vec(), (S*)ws.Ap, (int)upd_grid(), ws
             );
         else
             cg_spmv_kernel<S, BR, G, false><<<grid, kBlock, 0, stream>>>(
-                pb.nrow, pb.offsets, pb.columns, (const S*)pb.values, nullptr, (S*)ws.Ap, ws
+                pb.nrow, pb.offsets, pb.columns, (const S*)pb.values, nullptr, (S*)ws.Ap, (int)upd_grid(), ws
             );
         WA_LAUNCH_CHECK();
-        return finalize<kFinSpmv>(grid, 0);
+        return kOk;
     }
     template <int BR, int PREC> int update_t()
     {
-        const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(pb.nrow, kBlock), kMaxUpdateGrid);
-        cg_update_kernel<S, BR, PREC><<<grid, kBlock, 0, stream>>>(
-            pb.nrow, (const S*)pb.prec, (S*)pb.x, (S*)ws.r, (S*)ws.z, (const S*)ws.Ap, fused ? 1 : 0, ws
+        unsigned spmv_parts = 0;
+#define WA_SPMV_PARTS(BR_, G) spmv_parts = spmv_grid<BR_, G>(); break;
+        switch (g()) {
+        case 1: WA_SPMV_PARTS(BR, 1)
+        case 2: WA_SPMV_PARTS(BR, 2)
+        case 4: WA_SPMV_PARTS(BR, 4)
+        case 8: WA_SPMV_PARTS(BR, 8)
+        case 16: WA_SPMV_PARTS(BR, 16)
+        case 32: WA_SPMV_PARTS(BR, 32)
+        default: WA_SPMV_PARTS(BR, 64)
+        }
+#undef WA_SPMV_PARTS
+        cg_update_kernel<S, BR, PREC><<<upd_grid(), kBlock, 0, stream>>>(
+            pb.nrow, (const S*)pb.prec, (S*)pb.x, (S*)ws.r, (S*)ws.z, (const S*)ws.Ap, fused ? 1 : 0, (int)spmv_parts,
+            ws
         );
         WA_LAUNCH_CHECK();
-        return finalize<kFinUpdate>(grid, 0);
+        return kOk;
     }
 
     int g() const { return group_for_square(pb.nnz_hint, pb.nrow, pb.br); }
@@ -496,7 +650,18 @@ template <typename S> struct CgLauncher {
     int direction()
     {
         const int64_t n = (int64_t)pb.nrow * pb.br;
-        cg_direction_kernel<S><<<(unsigned)ceil_div(n, kBlock), kBlock, 0, stream>>>(n, z_vec(), (S*)ws.p, ws.state);
+        cg_direction_kernel<S><<<vec_grid(n), kBlock, 0, stream>>>(n, z_vec(), (S*)ws.p, (int)upd_grid(), ws);
+        WA_LAUNCH_CHECK();
+        return kOk;
+    }
+
+    // Makes the state's r.r and `done` current before the host reads them
+    // (the fused path leaves the last update's sums in its partials).
+    int check()
+    {
+        if (!fused)
+            return kOk;
+        cg_check_kernel<<<1, kBlock, 0, stream>>>((int)upd_grid(), ws);
         WA_LAUNCH_CHECK();
         return kOk;
     }
@@ -660,7 +825,15 @@ int cg_solve_impl(
             WA_TRY(L.init(0));
         return kOk;
     };
-    auto run_cycle = [&]() -> int { return runner.run(cycle_body, 1, false); };
+    // k cycles, then the state made current for the host's read-back
+    auto group_of = [&](int k) {
+        return [&, k]() -> int {
+            for (int c = 0; c < k; ++c)
+                WA_TRY(cycle_body());
+            return L.check();
+        };
+    };
+    auto run_cycle = [&]() -> int { return runner.run(group_of(1), 1, false); };
 
     if (check_every <= 0) {
         // Device-side convergence only: enqueue every cycle, no host read-back
@@ -669,7 +842,7 @@ int cg_solve_impl(
         const int total = (int)ceil_div(maxiter, cycle);
         for (int c = 0; c < total; c += kDeviceGroup) {
             const int k = std::min(kDeviceGroup, total - c);
-            WA_TRY(runner.run(cycle_body, k, k == kDeviceGroup));
+            WA_TRY(runner.run(group_of(k), 1, k == kDeviceGroup));
         }
         if (iters_out || rnorm_out || atol_out) {
             CgState h;
@@ -696,7 +869,7 @@ int cg_solve_impl(
         for (;;) {
             const int left = (int)ceil_div(maxiter - cur_iter, cycle);
             const int k = std::min(per_check, left);
-            WA_TRY(runner.run(cycle_body, k, k == per_check));
+            WA_TRY(runner.run(group_of(k), 1, k == per_check));
             cur_iter += k * cycle;
             if (cur_iter >= maxiter)
                 break;

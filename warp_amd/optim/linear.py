@@ -432,7 +432,7 @@ class CG(LinearSolverState):
             _native.ptr(bsr.offsets),
             _native.ptr(bsr.columns),
             _native.ptr(bsr.values),
-            int(bsr.nnz),
+            bsr._lane_nnz(wait=want_host),
             prec_kind,
             prec_ptr,
             _native.ptr(b),

@@ -177,8 +177,35 @@ class BsrMatrix:
         self._nnz_pending = False
         return self.nnz
 
+    # -- topology version: the cached mirror-symmetry verdict is valid for one
+    # version; assigning offsets / columns / row_counts or any operation of
+    # this module that rewrites them starts a new one
+    def __setattr__(self, name, value):
+        if name in ("offsets", "columns", "row_counts", "nrow", "ncol"):
+            self.__dict__["_topology_version"] = self.__dict__.get("_topology_version", 0) + 1
+        object.__setattr__(self, name, value)
+
+    def _bump_topology(self) -> None:
+        self.__dict__["_topology_version"] = self.__dict__.get("_topology_version", 0) + 1
+
+    def _lane_nnz(self, wait: bool = False) -> int:
+        """nnz used to choose the lanes per block row of the products: exact
+        once the pending ``offsets[nrow]`` transfer has landed (a non-blocking
+        event query), or after waiting for it when ``wait`` is set and no graph
+        capture is running; otherwise the host-side upper bound.  After a
+        compact assembly that bound is the triplet count, which would pick
+        twice the lanes a 27-block row needs."""
+        if getattr(self, "_nnz_pending", False) and self._nnz_event is not None:
+            if self._nnz_event.query():
+                self.nnz = int(self._nnz_buf[0])
+                self._nnz_pending = False
+            elif wait and not torch.cuda.is_current_stream_capturing():
+                self.nnz_sync()
+        return int(self.nnz)
+
     def notify_nnz_changed(self, nnz: int | None = None, nnz_capacity: int | None = None) -> None:
         """Notify that ``offsets`` changed outside this module (sparse.py:311-333)."""
+        self._bump_topology()
         self._nnz_pending = False
         if nnz is None:
             if nnz_capacity is None:
@@ -571,6 +598,7 @@ def bsr_set_zero(
     ``"compact"`` drops all blocks; ``"padded"`` keeps (or resets, with
     ``row_capacity``) the row capacity and empties every row; ``"masked"``
     keeps the topology and zeroes the values."""
+    bsr._bump_topology()
     if topology not in ("compact", "padded", "masked"):
         raise ValueError(f"Unsupported topology policy: {topology}")
     if row_capacity is not None and topology != "padded":
@@ -611,6 +639,7 @@ def bsr_set_from_triplets(
     ``masked=True``) keeps the topology of ``dest`` and drops triplets outside
     it; no zero block is pruned.
     """
+    dest._bump_topology()
     if topology is None:
         topology = "masked" if masked else "compact"
     elif topology not in ("compact", "masked", "padded"):
@@ -811,6 +840,7 @@ def bsr_compress(src: BsrMatrix, prune_numerical_zeros: bool = True, inplace: bo
     elif topology not in ("compact", "padded"):
         raise ValueError(f"Unsupported topology policy: {topology}")
     _require_gpu(src.device, "bsr_compress")
+    src._bump_topology()
     compact = topology == "compact"
     if not compact and src.row_counts is None:
         src.row_counts = (src.offsets[1 : src.nrow + 1] - src.offsets[: src.nrow]).to(torch.int32)
@@ -871,6 +901,7 @@ def bsr_set_transpose(dest: BsrMatrix, src, masked: bool = False, topology: str 
     ``dest`` and raises ``BSR_STATUS_ROW_CAPACITY_EXCEEDED`` for rows that do
     not fit (left empty).
     """
+    dest._bump_topology()
     src, src_scale = _extract_matrix_and_scale(src)
     if topology is None:
         topology = "masked" if masked else "compact"
@@ -1024,6 +1055,7 @@ def bsr_assign(dest: BsrMatrix, src, structure_only: bool = False, masked: bool 
     each source row into the existing row capacity of ``dest`` (same block
     shape), raising ``BSR_STATUS_ROW_CAPACITY_EXCEEDED`` for rows that do not
     fit."""
+    dest._bump_topology()
     src, src_scale = _extract_matrix_and_scale(src)
     if dest.values.device != src.values.device:
         raise ValueError("Source and destination matrices must reside on the same device")
@@ -1286,6 +1318,7 @@ def bsr_axpy(x, y: BsrMatrix | None = None, alpha=1.0, beta=1.0, masked: bool = 
             raise ValueError("Left-hand-side 'y' matrix must be provided for this topology policy")
         y = bsr_zeros(x.nrow, x.ncol, block_type=x.dtype, device=x.values.device)
         beta = 0.0
+    y._bump_topology()
     x_nnz, y_nnz = int(x.nnz), int(y.nnz)
 
     if topology == "padded":
@@ -1397,6 +1430,7 @@ def bsr_mm(x, y, z: BsrMatrix | None = None, alpha=1.0, beta=0.0, masked: bool =
         ztype = x.scalar_type if zshape == (1, 1) else wt.matrix(shape=zshape, dtype=x.scalar_type)
         z = bsr_zeros(x.nrow, y.ncol, block_type=ztype, device=x.values.device)
         beta = 0.0
+    z._bump_topology()
     if x.values.device != y.values.device or x.values.device != z.values.device:
         raise ValueError(
             f"All arguments must reside on the same device, got {x.values.device}, {y.values.device} and {z.values.device}"
@@ -1532,6 +1566,7 @@ def bsr_set_diag(A: BsrMatrix, diag=None, rows_of_blocks: int | None = None, col
     """Make ``A`` block diagonal (sparse.py:2939-2995).  ``diag``: a tensor of
     blocks (one per diagonal block), one block value for all (a number or
     nested list, not a tensor), or None (values left as they are)."""
+    A._bump_topology()
     if rows_of_blocks is None and cols_of_blocks is not None:
         rows_of_blocks = cols_of_blocks
     if cols_of_blocks is None and rows_of_blocks is not None:
@@ -1660,19 +1695,29 @@ def bsr_block_index(row, col, bsr_offsets: torch.Tensor, bsr_columns: torch.Tens
 def _mirror_symmetric(A: BsrMatrix) -> bool:
     """True when ``A`` is compact and square with 1x1 or 3x3 blocks and every
     block (r, c) has a mirror (c, r): the transposed product can then gather
-    without atomics.  Reads one count back, so never under graph capture."""
+    without atomics.  The verdict is cached per topology version (see
+    ``BsrMatrix.__setattr__``); computing it reads one count back, so under a
+    graph capture only a cached verdict can say True."""
     br, bc = A.block_shape
-    if (A.row_counts is not None or A.nrow != A.ncol or br != bc or br not in (1, 3)
-            or torch.cuda.is_current_stream_capturing()):
+    if A.row_counts is not None or A.nrow != A.ncol or br != bc or br not in (1, 3):
+        return False
+    key = (A.__dict__.get("_topology_version", 0), A.offsets.data_ptr(), A.columns.data_ptr())
+    cached = A.__dict__.get("_mirror_cache")
+    if cached is not None and cached[0] == key:
+        return cached[1]
+    if torch.cuda.is_current_stream_capturing():
         return False
     T = min(A.nnz_sync(), A.columns.numel())  # exact: stale tail slots hold no valid column
-    if T == 0:
-        return False
-    rows = A.uncompress_rows()
-    missing = torch.empty(1, dtype=torch.int32, device=A.device)
-    _native.call("warp_amd_bsr_count_missing_blocks", T, _native.ptr(A.columns), _native.ptr(rows),
-                 _native.ptr(A.offsets), None, _native.ptr(A.columns), _native.ptr(missing), _native.stream_of(A.device))
-    return int(missing.item()) == 0
+    verdict = False
+    if T > 0:
+        rows = A.uncompress_rows()
+        missing = torch.empty(1, dtype=torch.int32, device=A.device)
+        _native.call("warp_amd_bsr_count_missing_blocks", T, _native.ptr(A.columns), _native.ptr(rows),
+                     _native.ptr(A.offsets), None, _native.ptr(A.columns), _native.ptr(missing),
+                     _native.stream_of(A.device))
+        verdict = int(missing.item()) == 0
+    A.__dict__["_mirror_cache"] = (key, verdict)
+    return verdict
 
 
 def _vec_view(array: torch.Tensor, torch_dtype: torch.dtype, expected_scalar_count: int) -> torch.Tensor:
@@ -1754,12 +1799,13 @@ def bsr_mv(
     if torch_dtype not in (torch.float32, torch.float64):
         raise TypeError(f"bsr_mv supports float32 and float64 matrices, got {torch_dtype}")
     sfx = "f64" if torch_dtype == torch.float64 else "f32"
+    nnz_hint = A._lane_nnz()
     if transpose:
         if _mirror_symmetric(A):
             # every block has its mirror: gather, deterministic
             _native.call(
                 "warp_amd_bsr_mv_transpose_gather_" + sfx, A.nrow, A.block_shape[0], _native.ptr(A.offsets),
-                _native.ptr(A.columns), _native.ptr(A.values), int(A.nnz), _native.ptr(x_view), _native.ptr(y_view),
+                _native.ptr(A.columns), _native.ptr(A.values), nnz_hint, _native.ptr(x_view), _native.ptr(y_view),
                 alpha, beta, _native.stream_of(A.values.device),
             )
             return y
@@ -1767,10 +1813,10 @@ def bsr_mv(
         _native.call(
             "warp_amd_bsr_mv_transpose_" + sfx, A.nrow, A.ncol, A.block_shape[0], A.block_shape[1],
             _native.ptr(A.offsets), _native.ptr(A.row_counts), _native.ptr(A.columns), _native.ptr(A.values),
-            int(A.nnz), _native.ptr(x_view), _native.ptr(y_view), alpha, beta, _native.stream_of(A.values.device),
+            nnz_hint, _native.ptr(x_view), _native.ptr(y_view), alpha, beta, _native.stream_of(A.values.device),
         )
         return y
-    if int(A.nnz) == 0 and nrow > 0:
+    if nnz_hint == 0 and nrow > 0:
         # no blocks (the matrix arrays may be empty): y = beta * y, y unread when beta is 0
         if beta == 0.0:
             y_view.zero_()
@@ -1797,7 +1843,7 @@ def bsr_mv(
         _native.ptr(A.row_counts),
         _native.ptr(A.columns),
         _native.ptr(A.values),
-        int(A.nnz),
+        nnz_hint,
         _native.ptr(x_view),
         _native.ptr(y_view),
         alpha,
