@@ -79,6 +79,10 @@ def ref_lib() -> ctypes.CDLL | None:
         ]
         L.wp_bsr_transpose_host.restype = None
         L.wp_bsr_transpose_host.argtypes = [_I, _I, _I] + [_P] * 8
+        L.wp_bsr_compress_inplace_host.restype = None
+        L.wp_bsr_compress_inplace_host.argtypes = [
+            _I, _I, _I, _I, _I, ctypes.c_bool, ctypes.c_uint64, ctypes.c_bool, _P, _P, _P, _P, ctypes.c_bool, _P, _P,
+        ]
         _ref = L
     return _ref
 
@@ -155,6 +159,31 @@ def ref_bsr_from_triplets(nrow, ncol, rows, cols, values=None, prune=True, count
     )
     n = int(nnz[0])
     return dict(offsets=off, columns=colsout[:n], summed_offsets=so[:n], summed_indices=si, nnz=n)
+
+
+def ref_bsr_compress_inplace(offsets, row_counts, columns, values, prune=True, make_compact=True):
+    """The reference's own in-place row compression (wp_bsr_compress_inplace_host,
+    sparse.cpp:521-607; rows: sparse_util.h:82-150) on copies of the arrays:
+    per row, zero blocks pruned, a stable insertion sort by column, duplicates
+    summed left to right.  Returns dict(offsets, row_counts, columns, values,
+    nnz) or None when the reference library was not built."""
+    L = ref_lib()
+    if L is None:
+        return None
+    off = np.array(offsets, np.int32, copy=True)
+    nrow = off.shape[0] - 1
+    rc = None if row_counts is None else np.array(row_counts, np.int32, copy=True)
+    cols = np.array(columns, np.int32, copy=True)
+    vals = np.array(values, copy=True, order="C")
+    block_size = int(np.prod(vals.shape[1:])) if vals.ndim > 1 else 1
+    sb = vals.dtype.itemsize
+    scalar_type = 1 if vals.dtype == np.float64 else 0
+    nnz = np.zeros(1, np.int32)
+    L.wp_bsr_compress_inplace_host(
+        nrow, block_size, sb, scalar_type, int(cols.shape[0]), bool(prune), _ZERO_MASK[sb], bool(make_compact),
+        _p(off), _p(rc), _p(cols), _p(vals), True, _p(nnz), None,
+    )
+    return dict(offsets=off, row_counts=rc, columns=cols, values=vals, nnz=int(nnz[0]))
 
 
 def _active_positions(nrow, offsets, row_counts, T):

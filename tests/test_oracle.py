@@ -84,6 +84,55 @@ def test_oracle_matches_reference_live(seed):
     np.testing.assert_array_equal(o["summed_offsets"], r["summed_offsets"])
 
 
+def _padded_candidates(rng, nrow, ncol, dtype, bshape):
+    """Padded matrix of row candidates as fem's row_compress writes them:
+    duplicates, zero blocks, unused (-1) slots and inactive capacity."""
+    cap = rng.integers(0, 40, nrow).astype(np.int32)
+    counts = np.minimum(cap, rng.integers(0, 40, nrow)).astype(np.int32)
+    off = np.concatenate([[0], np.cumsum(cap)]).astype(np.int32)
+    total = int(off[-1])
+    cols = np.full(total, -1, np.int32)
+    vals = np.zeros((total, *bshape), dtype)
+    rows_t, cols_t, vals_t = [], [], []
+    for r in range(nrow):
+        c = rng.integers(-1, min(ncol, 20), counts[r]).astype(np.int32)
+        v = (rng.standard_normal((counts[r], *bshape)) * 10.0 ** rng.integers(-3, 3, (counts[r], *bshape))).astype(dtype)
+        v[rng.random(counts[r]) < 0.1] = 0.0
+        cols[off[r]:off[r] + counts[r]] = c
+        vals[off[r]:off[r] + counts[r]] = v
+        rows_t.append(np.full(counts[r], r, np.int32))
+        cols_t.append(c)
+        vals_t.append(v)
+    trip = (np.concatenate(rows_t), np.concatenate(cols_t), np.concatenate(vals_t))
+    return off, counts, cols, vals, trip
+
+
+@pytest.mark.skipif(oracle.ref_lib() is None, reason="reference CPU library not built (make -C oracle ref)")
+@pytest.mark.parametrize("dtype,bshape", [(np.float32, (3, 3)), (np.float64, ()), (np.float32, ())])
+def test_oracle_matches_reference_inplace_compress(dtype, bshape):
+    """The reference's own wp_bsr_compress_inplace_host (sparse.cpp:521-607)
+    against the oracle assembly of the same candidates in storage order:
+    structure and the left-to-right duplicate sums bit for bit, compact and
+    padded."""
+    rng = np.random.default_rng(11 + len(bshape))
+    nrow, ncol = 200, 30
+    off, counts, cols, vals, (rt, ct, vt) = _padded_candidates(rng, nrow, ncol, dtype, bshape)
+    o = oracle.bsr_from_triplets(nrow, ncol, rt, ct, vt)
+    ref = oracle.ref_bsr_compress_inplace(off, counts, cols, vals, prune=True, make_compact=True)
+    assert ref["nnz"] == o["nnz"]
+    np.testing.assert_array_equal(ref["offsets"], o["offsets"])
+    np.testing.assert_array_equal(ref["columns"][: o["nnz"]], o["columns"])
+    assert ref["values"][: o["nnz"]].tobytes() == o["values"].tobytes()
+    assert (ref["columns"][o["nnz"]:] == -1).all()
+    pad = oracle.ref_bsr_compress_inplace(off, counts, cols, vals, prune=True, make_compact=False)
+    np.testing.assert_array_equal(pad["offsets"], off)
+    np.testing.assert_array_equal(pad["row_counts"], np.diff(o["offsets"]))
+    for r in range(nrow):
+        k0, k1 = o["offsets"][r], o["offsets"][r + 1]
+        np.testing.assert_array_equal(pad["columns"][off[r]:off[r] + k1 - k0], o["columns"][k0:k1])
+        assert pad["values"][off[r]:off[r] + k1 - k0].tobytes() == o["values"][k0:k1].tobytes()
+
+
 def test_value_sums_sequential_order():
     """Values are v[i0] + v[i1] + ... in ascending triplet order, in fp32."""
     rng = np.random.default_rng(5)
