@@ -529,6 +529,18 @@ int warp_amd_bsr_residual_f32_f64(
     int nrow, int block_size, const int* offsets, const int* columns, const float* values, int64_t nnz_hint,
     const double* x, const double* b, double* r64, float* r32, double* rr, void* workspace, size_t workspace_bytes,
     void* stream);
+/* warp_amd_bsr_residual_f32_f64 without a workspace.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  The partials
+ * live in a device array of the library (65,536 doubles, 512 KB, held by
+ * every GPU the library runs on until the process ends): nrow is limited to
+ * 65,536 residual workgroups (16,777,216 scalar rows, 1,048,576 rows of 3x3
+ * blocks at the default lanes per row; larger sizes return an error before
+ * any launch).  Calls that overlap on one GPU race on that array.  The first
+ * call on a device looks up the array's address on that device. */
+int warp_amd_bsr_residual_f32_f64_noalloc(
+    int nrow, int block_size, const int* offsets, const int* columns, const float* values, int64_t nnz_hint,
+    const double* x, const double* b, double* r64, float* r32, double* rr, void* stream);
 /* x64 += alpha * (double) d32 */
 int warp_amd_axpy_f32_to_f64(int64_t n, double alpha, const float* d, double* x, void* stream);
 /* y64 = A x64 (FP32 values promoted, FP64 accumulation); *xy (device) = x.y.

@@ -111,6 +111,7 @@ _SIGNATURES = {
     "warp_amd_normalize_dirichlet_projector": (c_int, [c_int, c_int, c_int] + [c_ptr] * 6),
     "warp_amd_bsr_residual_workspace_size": (c_size, [c_int]),
     "warp_amd_bsr_residual_f32_f64": (c_int, [c_int, c_int, c_ptr, c_ptr, c_ptr, c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
+    "warp_amd_bsr_residual_f32_f64_noalloc": (c_int, [c_int, c_int, c_ptr, c_ptr, c_ptr, c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]),
     "warp_amd_axpy_f32_to_f64": (c_int, [c_i64, c_f64, c_ptr, c_ptr, c_ptr]),
     "warp_amd_bsr_mv_f32_f64": (c_int, [c_int, c_int, c_ptr, c_ptr, c_ptr, c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
     "warp_amd_rms_scales": (c_int, [c_ptr, c_i64, c_ptr, c_ptr, c_ptr]),

@@ -7,6 +7,8 @@
 // matrix promoted in-kernel, solve A d = r in FP32, x += d in FP64.
 #include "bsr_core.h"
 
+#include <mutex>
+
 namespace wa {
 namespace {
 
@@ -431,6 +433,78 @@ WA_API int warp_amd_bsr_residual_f32_f64(
         return wa::residual_dispatch<1>(g, nrow, offsets, columns, values, x, b, r64, r32, partials, rr,
                                         wa::as_stream(stream));
     return wa::residual_dispatch<3>(g, nrow, offsets, columns, values, x, b, r64, r32, partials, rr,
+                                    wa::as_stream(stream));
+}
+
+// ---------------------------------------------------------------------------
+// The residual without a workspace, for callers that cannot allocate (code
+// under graph capture, serving loops): its partials live in `scratch`, a
+// file-scope device array that every GPU the library runs on holds until the
+// process ends.  Calls on one GPU share it, so calls that overlap on one GPU
+// race.  Every partial a call sums is written by that call's residual
+// kernel, so nothing needs clearing between calls.
+// ---------------------------------------------------------------------------
+
+namespace wa {
+
+// 65,536 FP64 partials (512 KB): one per workgroup of the residual kernel,
+// i.e. up to 16,777,216 scalar rows or 1,048,576 block rows of 3x3 blocks at
+// the default lanes per row.
+constexpr int kResidualScratchParts = 65536;
+
+__device__ __align__(256) double scratch[kResidualScratchParts];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (each device holds its own copy at its own address; the host-side
+// name is not a device address).
+int scratch_on_current_device(double** out)
+{
+    int dev = -1;
+    WA_HIP(hipGetDevice(&dev));
+    WA_REQUIRE(dev >= 0 && dev < kMaxDevices, "bsr_residual_noalloc: device id %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        WA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return kOk;
+}
+
+}  // namespace
+}  // namespace wa
+
+// warp_amd_bsr_residual_f32_f64 with the partials in the library's device
+// scratch: allocates nothing.  Sizes whose residual grid exceeds the scratch
+// are rejected before any launch.
+WA_API int warp_amd_bsr_residual_f32_f64_noalloc(
+    int nrow, int block_size, const int* offsets, const int* columns, const float* values, int64_t nnz_hint,
+    const double* x, const double* b, double* r64, float* r32, double* rr, void* stream
+)
+{
+    WA_REQUIRE(block_size == 1 || block_size == 3, "bsr_residual_noalloc: block size must be 1 or 3, got %d",
+               block_size);
+    WA_REQUIRE(nrow > 0, "bsr_residual_noalloc: empty matrix");
+    const wa::BsrShape shape = wa::default_bsr_shape(nrow, block_size, block_size, nnz_hint);
+    const int64_t rows_per_block = (int64_t)(wa::kBlock / shape.lanes) * wa::rows_per_group_for(block_size, block_size);
+    const int64_t parts = wa::ceil_div(nrow, rows_per_block);
+    WA_REQUIRE(parts <= wa::kResidualScratchParts,
+               "bsr_residual_noalloc: %d block rows need %lld partials, more than the %d the scratch holds; "
+               "use warp_amd_bsr_residual_f32_f64 with a workspace",
+               nrow, (long long)parts, wa::kResidualScratchParts);
+    double* partials = nullptr;
+    WA_TRY(wa::scratch_on_current_device(&partials));
+    if (block_size == 1)
+        return wa::residual_dispatch<1>(shape.lanes, nrow, offsets, columns, values, x, b, r64, r32, partials, rr,
+                                        wa::as_stream(stream));
+    return wa::residual_dispatch<3>(shape.lanes, nrow, offsets, columns, values, x, b, r64, r32, partials, rr,
                                     wa::as_stream(stream));
 }
 
