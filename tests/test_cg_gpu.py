@@ -81,6 +81,64 @@ def test_cg_restart_callbacks(device):
         assert np.linalg.norm(A_np @ x.cpu().numpy() - b_np) <= atol
 
 
+def _restart_drift_system(batch_count):
+    """_make_cg_restart_drift_system (test_linear_solvers.py:175-201): a
+    130-dof mass + spring chain with every 17th dof fixed (identity row and
+    column), stacked batch_count times block-diagonally; x0 = 0.1 N(0,1)
+    (seed 7301) and b = mass x0, both zero on the fixed dofs."""
+    dof_count = 130
+    chain_count = dof_count - 1
+    edge_count = chain_count - 1
+    mass = 3.0 / chain_count
+    w = 0.18**2 / (3.0 / edge_count)
+    A1 = mass * np.eye(dof_count)
+    e = np.arange(edge_count)
+    np.add.at(A1, (e, e), w)
+    np.add.at(A1, (e + 1, e + 1), w)
+    A1[e, e + 1] -= w
+    A1[e + 1, e] -= w
+    fixed = np.arange(0, chain_count, 17)
+    A1[fixed, :] = 0.0
+    A1[:, fixed] = 0.0
+    A1[fixed, fixed] = 1.0
+    A = np.kron(np.eye(batch_count), A1).astype(np.float32)
+    rng = np.random.default_rng(7301)
+    x0 = (0.1 * rng.standard_normal((batch_count, dof_count))).astype(np.float32)
+    b = (mass * x0).astype(np.float32)
+    x0[:, fixed] = 0.0
+    b[:, fixed] = 0.0
+    return A, b.reshape(-1), x0.reshape(-1), dof_count
+
+
+@pytest.mark.parametrize("batch_count", [1, 3])
+def test_cg_restart_float32_drift(device, batch_count):
+    """test_cg_restart_float32_drift (test_linear_solvers.py:204-253): FP32 CG
+    restarted every 32 iterations, checked every 32, graph-replayed, to
+    tol 1e-6; unpreconditioned and Jacobi.  Every subproblem's FP64 true
+    residual must reach tol |b_i| (restarts recompute r = b - A x, so the
+    recurrence's drift cannot stop the solve short)."""
+    interval, tol = 32, 1.0e-6
+    A_np, b_np, x0_np, ndof = _restart_drift_system(batch_count)
+    A = torch.tensor(A_np, device=device)
+    b = torch.tensor(b_np, device=device)
+    M = linear.preconditioner(A, "diag")
+    offs = None if batch_count == 1 else _batch_offsets([ndof] * batch_count, device)
+    op = linear.aslinearoperator(A, batch_offsets=offs)
+    for name, prec in (("none", None), ("Jacobi", M)):
+        x = torch.tensor(x0_np, device=device)
+        niter, err, atol = linear.cg(op, b, x, tol=tol, atol=0.0, maxiter=512, M=prec, check_every=interval,
+                                     use_cuda_graph=True, restart=interval)
+        err_t, atol_t = torch.as_tensor(err), torch.as_tensor(atol)
+        assert bool((err_t <= atol_t).all()), (name, err, atol)
+        assert niter % interval == 0 and niter <= 512, (name, niter)
+        xn = x.cpu().numpy().astype(np.float64)
+        for k in range(batch_count):
+            sl = slice(k * ndof, (k + 1) * ndof)
+            res = A_np[sl, sl].astype(np.float64) @ xn[sl] - b_np[sl].astype(np.float64)
+            target = tol * np.linalg.norm(b_np[sl].astype(np.float64))
+            assert np.linalg.norm(res) <= target, f"{name} batch {k}: {np.linalg.norm(res):.3e} > {target:.3e}"
+
+
 def _lanes(A):
     from warp_amd import _native
 

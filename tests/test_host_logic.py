@@ -3,6 +3,8 @@ reference, no CPU fallback, workload generators."""
 
 from __future__ import annotations
 
+import math
+
 import numpy as np
 import pytest
 import torch
@@ -113,6 +115,27 @@ def test_fv7_operator_properties():
     # strictly diagonally dominant on boundary rows, weakly inside: SPD
     assert np.linalg.eigvalsh(A.toarray()).min() > 0
     assert np.all(np.asarray(A.sum(axis=1)).ravel() >= -1e-5)
+
+
+def test_grf_correlation_length():
+    """The field's covariance is exp(-r^2 / (2 l^2)): the empirical
+    correlation along each axis, averaged over four 128^3 fields with l = 8
+    cells, is exp(-1/8) at lag l/2, exp(-1/2) at lag l and exp(-2) at lag 2l
+    (a filter without the square root would give exp(-1/4) = 0.78 at lag l,
+    a correlation length off by 2x exp(-1/32) or exp(-1/8) there)."""
+    n, ell = 128, 8.0
+    corr = {4: [], 8: [], 16: []}
+    for seed in range(4):
+        g = problems.gaussian_random_field(n, 1.0, ell, seed=100 + seed).to(torch.float64)
+        assert abs(float(g.std()) - 1.0) < 1e-6 and abs(float(g.mean())) < 1e-6
+        var = float((g * g).mean())
+        for lag in corr:
+            for axis in range(3):
+                corr[lag].append(float((g * torch.roll(g, lag, dims=axis)).mean()) / var)
+    for lag, vals in corr.items():
+        want = math.exp(-0.5 * (lag / ell) ** 2)
+        got = float(np.mean(vals))
+        assert abs(got - want) < 0.04, (lag, got, want)
 
 
 def test_learned_checkpoint_roundtrip(tmp_path):

@@ -207,6 +207,47 @@ def test_bsr_mv_matches_dense(bshape, dtype):
     assert np.all(np.isfinite(oracle.bsr_mv(o["offsets"], o["columns"], vv, x, nan_y, 1.0, 0.0)))
 
 
+@pytest.mark.parametrize("bshape,dtype", [((1, 1), np.float32), ((3, 3), np.float32), ((3, 3), np.float64),
+                                          ((2, 3), np.float32)])
+def test_bsr_mv_one_lane_group_equals_row_order(bshape, dtype):
+    """With one lane per block row the lane-group order degenerates to the
+    reference row order (blocks in storage order, block columns in order,
+    fused multiply-adds, then alpha, then beta y): bit for bit, on ragged
+    rows (empty ones included), compact and padded, for several alpha/beta
+    pairs.  A G > 1 butterfly regroups the sums, so it is checked only to
+    rounding."""
+    rng = np.random.default_rng(41)
+    nrow, ncol, T = 300, 120, 4000
+    rows = rng.integers(-1, nrow, T).astype(np.int32)
+    rows[rows % 7 == 3] = 5  # one long row
+    cols = rng.integers(0, ncol, T).astype(np.int32)
+    vals = rng.standard_normal((T, *bshape)).astype(dtype)
+    flat = bshape == (1, 1)
+    o = oracle.bsr_from_triplets(nrow, ncol, rows, cols, vals.reshape(T) if flat else vals)
+    assert (np.diff(o["offsets"]) == 0).any()
+    vv = o["values"].reshape(-1, *bshape)
+    x = rng.standard_normal(ncol * bshape[1]).astype(dtype)
+    y = rng.standard_normal(nrow * bshape[0]).astype(dtype)
+    # padded copy: two unused slots after every row
+    cnt = np.diff(o["offsets"]).astype(np.int32)
+    poff = np.concatenate([[0], np.cumsum(cnt + 2)]).astype(np.int32)
+    pcol = np.full(int(poff[-1]), -1, np.int32)
+    pval = np.full((int(poff[-1]), *bshape), np.nan, dtype)
+    for r in range(nrow):
+        pcol[poff[r]:poff[r] + cnt[r]] = o["columns"][o["offsets"][r]:o["offsets"][r + 1]]
+        pval[poff[r]:poff[r] + cnt[r]] = vv[o["offsets"][r]:o["offsets"][r + 1]]
+    for alpha, beta in [(1.0, 0.0), (-0.75, 1.5), (2.0, -1.0)]:
+        row = oracle.bsr_mv(o["offsets"], o["columns"], vv, x, y, alpha, beta, group=0)
+        one = oracle.bsr_mv(o["offsets"], o["columns"], vv, x, y, alpha, beta, group=1)
+        assert one.tobytes() == row.tobytes(), (alpha, beta)
+        pad = oracle.bsr_mv(poff, pcol, pval, x, y, alpha, beta, row_counts=cnt, group=1)
+        assert pad.tobytes() == row.tobytes(), (alpha, beta)
+        eight = oracle.bsr_mv(o["offsets"], o["columns"], vv, x, y, alpha, beta, group=8)
+        tol = 1e-5 if dtype == np.float32 else 1e-13
+        np.testing.assert_allclose(eight, row, rtol=tol, atol=tol * np.abs(row).max())
+        assert eight.tobytes() != row.tobytes() or dtype == np.float64
+
+
 def test_diag_and_jacobi_closed_form():
     # test_bsr_get_set_diag (test_sparse.py:486-510)
     rng = np.random.default_rng(123)
