@@ -220,6 +220,29 @@ int warp_amd_bsr_assign_padded(int src_scalar_size, int dest_scalar_size, int nr
                                double scale, const int* dest_offsets, int* dest_row_counts, int* dest_columns,
                                void* dest_values, int* status, void* stream);
 
+/* Padded assignment with a change of block shape, topology part (bsr_assign
+ * with topology="padded", warp/_src/sparse.py:2201-2255 and 2346-2358,
+ * kernel _bsr_assign_padded_reblock_topology, 2038-2083): per destination
+ * row, the sorted destination columns that the covering source rows' blocks
+ * overlap, written within the row's existing capacity; dest_row_counts set;
+ * a row that does not fit is left empty and raises *status to 1.  Values
+ * follow with warp_amd_bsr_assign_copy_blocks into zeroed storage. */
+int warp_amd_bsr_assign_padded_reblock_topology(int src_subrows, int src_subcols, int dest_subrows, int dest_subcols,
+                                                int src_nrow, int dest_nrow, int dest_ncol, const int* src_offsets,
+                                                const int* src_row_counts, const int* src_columns,
+                                                const int* dest_offsets, int* dest_row_counts, int* dest_columns,
+                                                int* status, void* stream);
+
+/* Row capacity of a reblocked padded copy as offsets[dest_nrow + 1] (bsr_copy
+ * with topology="padded" and another block shape, warp/_src/sparse.py:
+ * 2561-2588, kernel _bsr_copy_reblocked_capacity_counts, 2086-2108): each
+ * destination row gets the capacity of its covering source rows times the
+ * sub-columns per source block. */
+size_t warp_amd_bsr_reblock_capacity_workspace_size(int dest_nrow);
+int warp_amd_bsr_reblock_capacity(int src_subrows, int src_subcols, int dest_subrows, int dest_subcols, int src_nrow,
+                                  int dest_nrow, const int* src_offsets, int* dest_offsets, void* workspace,
+                                  size_t workspace_bytes, void* stream);
+
 /* rows[p] = block row of storage position p < T, -1 where p holds no active
  * block (replaces the kernel of BsrMatrix.uncompress_rows,
  * warp/_src/sparse.py:164-193). */
@@ -293,6 +316,19 @@ int warp_amd_bsr_mm_values(int scalar_size, int br, int bk, int bc, double alpha
                            int y_nrow, const int* x_offsets, const int* x_row_counts, const int* x_columns,
                            const void* x_values, const int* y_offsets, const int* y_row_counts, const int* y_columns,
                            const void* y_values, void* stream);
+
+/* Tiled product values (bsr_mm with tile_size > 0; replaces the tiled-outer
+ * values kernel, warp/_src/sparse.py:3813-3909, and its dispatch, 4101-4144):
+ * `tile` lanes per z block (a power of two in [1, 64]); the x row is read as
+ * the flat list of (x block k, inner index m) pairs, m fastest; lane g adds
+ * the rank-one updates x_k[:, m] y(x_columns[k], c)[m, :] of pairs g, g+tile,
+ * ... (each product rounded, then added); the lanes combine by an xor
+ * butterfly; z(r, c) = (0 + beta * z(r, c)) + alpha * sum. */
+int warp_amd_bsr_mm_values_tiled(int scalar_size, int br, int bk, int bc, double alpha, double beta, int T, int tile,
+                                 int z_nrow, const int* z_offsets, const int* z_row_counts, const int* z_columns,
+                                 void* z_values, int y_nrow, const int* x_offsets, const int* x_row_counts,
+                                 const int* x_columns, const void* x_values, const int* y_offsets,
+                                 const int* y_row_counts, const int* y_columns, const void* y_values, void* stream);
 
 /* Block-diagonal topology: offsets[r] = min(r, ndiag), columns[r] = r for
  * r < ndiag (replaces _bsr_set_diag_kernel of bsr_set_diag,
@@ -375,6 +411,24 @@ int warp_amd_bsr_mv_ex_f64(
     int nrow, int ncol, int block_rows, int block_cols, const int* offsets, const int* row_counts,
     const int* columns, const double* values, int64_t nnz_hint, const double* x, double* y, double alpha,
     double beta, int lanes, int rows_per_group, void* stream);
+
+/* The same product with the matrix's arguments packed once into a host
+ * struct (no reference counterpart: a binding that calls bsr_mv on the same
+ * matrix many times fills it once and passes 6 arguments per call instead of
+ * 16, which is most of the host cost of a small product).  scalar_size 4:
+ * float values / vectors, 8: double; alpha and beta are rounded to the
+ * scalar type.  The struct is read during the call only. */
+typedef struct warp_amd_bsr_mv_args {
+    int nrow, ncol, block_rows, block_cols;
+    const int* offsets;    /* device */
+    const int* row_counts; /* device, NULL = compact */
+    const int* columns;    /* device */
+    const void* values;    /* device */
+    int64_t nnz_hint;
+    int lanes, rows_per_group, scalar_size, reserved;
+} warp_amd_bsr_mv_args;
+int warp_amd_bsr_mv_packed(const warp_amd_bsr_mv_args* args, const void* x, void* y, double alpha, double beta,
+                           void* stream);
 
 /* Lanes per block row G the vector-BSR kernels (bsr_mv and the fused CG
  * kernels) use for this shape and nnz_hint; 1 for the generic kernel.  Each

@@ -287,7 +287,7 @@ def _find_blocks(offsets, columns, ncol, rows, cols):
     return np.where(ok, pos, -1)
 
 
-def bsr_mm(x, y, z=None, alpha=1.0, beta=0.0, deep_ranges=True, z_topology=None):
+def bsr_mm(x, y, z=None, alpha=1.0, beta=0.0, deep_ranges=True, z_topology=None, tile=0):
     """z := alpha x @ y + beta z on compact matrices given as dicts (nrow, ncol,
     offsets, columns, values[nnz, br, bc]) with sorted unique rows.
 
@@ -302,7 +302,16 @@ def bsr_mm(x, y, z=None, alpha=1.0, beta=0.0, deep_ranges=True, z_topology=None)
     block starts at 0 (+ beta z), then acc = sum over the row's x blocks in
     storage order of x_k @ y, each entry x[i,0] y[0,j] + x[i,1] y[1,j] + ...
     left to right, every product and sum rounded in the values' dtype; finally
-    z += alpha acc."""
+    z += alpha acc.
+
+    tile > 0 (the tiled-outer values kernel, sparse.py:3813-3909, with `tile`
+    lanes per block): the row's (x block k, inner index m) pairs, m fastest,
+    are dealt to lanes g = pair % tile; each lane adds its pairs' rank-one
+    updates x_k[:, m] y[m, :] in order (product rounded, then added; pairs
+    without a y block skipped), then the lanes combine by an xor butterfly
+    (a[g] = a[g] + a[g ^ off], off = tile/2 .. 1) and lane 0 holds acc.  The
+    reference's own lane sum (tile_sum) has no fixed order, so this is the
+    GPU kernels' order, checked against dense products for the values."""
     dt = x["values"].dtype
     xo, xc, xv = np.asarray(x["offsets"], np.int64), np.asarray(x["columns"], np.int64), x["values"]
     yo, yc, yv = np.asarray(y["offsets"], np.int64), np.asarray(y["columns"], np.int64), y["values"]
@@ -340,8 +349,25 @@ def bsr_mm(x, y, z=None, alpha=1.0, beta=0.0, deep_ranges=True, z_topology=None)
         vals[has] = vals[has] + np.asarray(beta, dt) * z["values"][pos[has]]
     acc = np.zeros((nb, br, bc), dt)
     xlen = np.diff(xo)
+    if tile > 0 and nb:
+        lanes = np.zeros((nb, tile, br, bc), dt)
+        npairs = xlen[out_rows] * bk
+        for c in range(int(npairs.max())):
+            g, m = c % tile, c % bk
+            live = np.flatnonzero(npairs > c)
+            k = xo[out_rows[live]] + c // bk
+            yb = _find_blocks(yo, yc, ncol, xc[k], out_cols[live])
+            ok = yb >= 0
+            live, k, yb = live[ok], k[ok], yb[ok]
+            upd = xv[k][:, :, m][:, :, None] * yv[yb][:, m, :][:, None, :]
+            lanes[live, g] = lanes[live, g] + upd
+        off = tile // 2
+        while off > 0:
+            lanes = lanes + lanes[:, np.arange(tile) ^ off]
+            off //= 2
+        acc = lanes[:, 0]
     step = 0
-    while nb and step < (xlen[out_rows].max() if nb else 0):
+    while nb and tile <= 0 and step < (xlen[out_rows].max() if nb else 0):
         live = np.flatnonzero(xlen[out_rows] > step)
         k = xo[out_rows[live]] + step
         yb = _find_blocks(yo, yc, ncol, xc[k], out_cols[live])

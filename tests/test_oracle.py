@@ -483,3 +483,30 @@ def test_mm_oracle_known_answers():
     o = oracle.bsr_mm(G, B, deep_ranges=False)
     np.testing.assert_array_equal(_bsr_dense(dict(nrow=2, ncol=2, **o)), [[27, 0], [44, 21]])
     np.testing.assert_array_equal(o["columns"], [0, 0, 1])
+
+
+@pytest.mark.parametrize("shapes", [(1, 1, 1), (3, 3, 3), (2, 4, 3)])
+def test_mm_oracle_tiled_order(shapes):
+    """The tiled product order (lanes over (block, inner index) pairs, rank-one
+    updates, xor butterfly): dense product to rounding for 1..64 lanes; with one
+    lane and one inner index it is the block-by-block order bit for bit (the
+    same rounded products, added in the same order); integer-valued operands
+    are exact whatever the lane count (no term lost or counted twice)."""
+    br, bk, bc = shapes
+    rng = np.random.default_rng(7 + br + bk)
+    x = _rand_bsr(rng, 25, 30, (br, bk), 0.3)
+    y = _rand_bsr(rng, 30, 20, (bk, bc), 0.2)
+    ref = 1.5 * _bsr_dense(x) @ _bsr_dense(y)
+    plain = oracle.bsr_mm(x, y, alpha=1.5)
+    for tile in (1, 2, 8, 64):
+        o = oracle.bsr_mm(x, y, alpha=1.5, tile=tile)
+        np.testing.assert_array_equal(o["columns"], plain["columns"])
+        np.testing.assert_allclose(_bsr_dense(dict(nrow=25, ncol=20, **o)), ref, rtol=2e-5, atol=2e-5)
+    xi = dict(x, values=np.round(x["values"] * 8).astype(np.float32))
+    yi = dict(y, values=np.round(y["values"] * 8).astype(np.float32))
+    refi = _bsr_dense(xi) @ _bsr_dense(yi)
+    for tile in (1, 4, 32):
+        oi = oracle.bsr_mm(xi, yi, tile=tile)
+        np.testing.assert_array_equal(_bsr_dense(dict(nrow=25, ncol=20, **oi)), refi)
+    if bk == 1:
+        assert oracle.bsr_mm(x, y, alpha=1.5, tile=1)["values"].tobytes() == plain["values"].tobytes()

@@ -886,6 +886,55 @@ def test_bsr_assign_reblock_vs_dense(device, src_shape, dest_shape, dest_dtype):
     for k in range(min(B.nrow, B.ncol)):
         keep[k * br:(k + 1) * br, k * bc:(k + 1) * bc] = True
     np.testing.assert_array_equal(Dd, np.where(keep, D, 0.0))
+    # padded copy with the change of block shape: each row's capacity is that
+    # of its covering source rows, its active columns those of the compact copy
+    P = sparse.bsr_copy(-0.5 * A, scalar_type=st, block_shape=dest_shape, topology="padded")
+    assert P.block_shape == tuple(dest_shape) and P.row_counts is not None
+    assert P.status_sync() == sparse.BSR_STATUS_SUCCESS
+    Dp, pcols = _active_dense(P)
+    np.testing.assert_array_equal(Dp, -0.5 * D)
+    _, bcols = _active_dense(B)
+    assert pcols == bcols
+    # capacity: source rows covering destination row r are those holding its
+    # sub-rows; each source block spans src_subcols destination sub-columns
+    src_subrows = max(1, src_shape[0] // br)
+    dest_subrows = max(1, br // src_shape[0])
+    src_subcols = max(1, src_shape[1] // bc)
+    a_cap = np.diff(A.offsets.cpu().numpy())
+    p_cap = np.diff(P.offsets.cpu().numpy())
+    for r in range(P.nrow):
+        first = r * dest_subrows // src_subrows
+        last = min(A.nrow, (r * dest_subrows + dest_subrows + src_subrows - 1) // src_subrows)
+        assert p_cap[r] == a_cap[first:last].sum() * src_subcols, r
+    # padded assign across block shapes into capacities: the longest row one
+    # block short (left empty, status raised), every other row one spare
+    cap = np.array([len(c) for c in bcols], np.int32)
+    small = int(np.argmax(cap))
+    row_cap = cap + 1
+    row_cap[small] = cap[small] - 1
+    Q = sparse.bsr_zeros(B.nrow, B.ncol, B.dtype, device=device, row_capacity=_t(row_cap, device))
+    sparse.bsr_assign(Q, -0.5 * A, topology="padded")
+    assert Q.status_sync() == sparse.BSR_STATUS_ROW_CAPACITY_EXCEEDED
+    Dq, qcols = _active_dense(Q)
+    assert qcols[small] == [] and all(qcols[r] == bcols[r] for r in range(B.nrow) if r != small)
+    rows_keep = np.ones(D.shape[0], bool)
+    rows_keep[small * br:(small + 1) * br] = False
+    np.testing.assert_array_equal(Dq[rows_keep], -0.5 * D[rows_keep])
+    assert not Dq[~rows_keep].any()
+
+
+def test_padded_reblock_copy_reference_scenario(device):
+    """test_padded_bsr_capture_reblock_copy (test_sparse.py:1231-1274), eagerly:
+    one 2x2 block split into 1x1 blocks keeps a padded topology."""
+    src = sparse.bsr_zeros(1, 1, wa.mat22, device=device, row_capacity=1)
+    sparse.bsr_set_from_triplets(src, _i([0], device), _i([0], device),
+                                 _t(np.array([[[1.0, 2.0], [3.0, 4.0]]], np.float32), device), topology="padded")
+    dest = sparse.bsr_copy(src, block_shape=(1, 1), topology="padded")
+    np.testing.assert_array_equal(dest.offsets.cpu().numpy(), [0, 2, 4])
+    np.testing.assert_array_equal(dest.row_counts.cpu().numpy(), [2, 2])
+    np.testing.assert_array_equal(dest.columns.cpu().numpy()[:4], [0, 1, 0, 1])
+    np.testing.assert_array_equal(dest.values.cpu().numpy()[:4], [1.0, 2.0, 3.0, 4.0])
+    assert dest.status_sync() == sparse.BSR_STATUS_SUCCESS
 
 
 # ---------------------------------------------------------------------------
@@ -1050,6 +1099,33 @@ def test_bsr_mm_and_axpy_vs_oracle(device, oracle_mod, dtype):
     got = _as_dict(z)
     np.testing.assert_array_equal(got["columns"], ref["columns"])
     assert got["values"].tobytes() == ref["values"].tobytes()
+
+
+@pytest.mark.parametrize("shapes,dtype", [((1, 1, 1), np.float32), ((3, 3, 3), np.float32), ((3, 3, 3), np.float64),
+                                          ((2, 4, 3), np.float32), ((4, 4, 4), np.float64)])
+def test_bsr_mm_tiled_vs_oracle(device, oracle_mod, shapes, dtype):
+    """bsr_mm(tile_size=G): values bit-exact against the oracle's tiled order
+    (rank-one updates over (block, inner index) pairs dealt to G lanes, xor
+    butterfly) for G = 1, 8, 64, the topology unchanged; tile sizes that are not
+    powers of two <= 64 are refused."""
+    br, bk, bc = shapes
+    rng = np.random.default_rng(53 + br * bk)
+
+    def rand(nrow, ncol, nnz, bshape):
+        return _from_triplets_np(device, nrow, ncol, rng.integers(0, nrow, nnz), rng.integers(0, ncol, nnz),
+                                 rng.standard_normal((nnz, *bshape)).astype(dtype))
+    x, y, z0 = rand(120, 90, 1500, (br, bk)), rand(90, 110, 1200, (bk, bc)), rand(120, 110, 600, (br, bc))
+    ox, oy, oz = _as_dict(x), _as_dict(y), _as_dict(z0)
+    for tile in (1, 8, 64):
+        z = sparse.bsr_copy(z0)
+        sparse.bsr_mm(x, y, z, alpha=-0.75, beta=1.5, tile_size=tile)
+        ref = oracle_mod.bsr_mm(ox, oy, oz, alpha=-0.75, beta=1.5, tile=tile)
+        got = _as_dict(z)
+        np.testing.assert_array_equal(got["offsets"], ref["offsets"])
+        np.testing.assert_array_equal(got["columns"], ref["columns"])
+        assert got["values"].tobytes() == ref["values"].tobytes(), tile
+    with pytest.raises(ValueError, match="power of two"):
+        sparse.bsr_mm(x, y, tile_size=48)
 
 
 def test_bsr_mm_axpy_padded_and_masked_vs_oracle(device, oracle_mod):

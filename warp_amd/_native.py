@@ -29,6 +29,19 @@ c_ptr = ctypes.c_void_p
 
 CG_CALLBACK = ctypes.CFUNCTYPE(c_int, c_int, c_f64, c_f64, c_ptr)
 
+
+class BsrMvArgs(ctypes.Structure):
+    """warp_amd_bsr_mv_args: a matrix's constant arguments of bsr_mv, packed
+    once so that each product passes six arguments (include/warp_amd.h)."""
+
+    _fields_ = [
+        ("nrow", c_int), ("ncol", c_int), ("block_rows", c_int), ("block_cols", c_int),
+        ("offsets", c_ptr), ("row_counts", c_ptr), ("columns", c_ptr), ("values", c_ptr),
+        ("nnz_hint", c_i64),
+        ("lanes", c_int), ("rows_per_group", c_int), ("scalar_size", c_int), ("reserved", c_int),
+    ]
+
+
 # name -> (restype, argtypes)
 _SIGNATURES = {
     "warp_amd_last_error": (ctypes.c_char_p, []),
@@ -87,6 +100,9 @@ _SIGNATURES = {
     "warp_amd_bsr_assign_padded": (
         c_int, [c_int, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_f64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]
     ),
+    "warp_amd_bsr_assign_padded_reblock_topology": (c_int, [c_int] * 7 + [c_ptr] * 8),
+    "warp_amd_bsr_reblock_capacity_workspace_size": (c_size, [c_int]),
+    "warp_amd_bsr_reblock_capacity": (c_int, [c_int] * 6 + [c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
     "warp_amd_scale_values": (c_int, [c_int, c_i64, c_ptr, c_f64, c_ptr]),
     "warp_amd_bsr_uncompress_rows": (c_int, [c_int, c_ptr, c_ptr, c_int, c_ptr, c_ptr]),
     "warp_amd_rescale_values": (c_int, [c_int, c_i64, c_ptr, c_f64, c_ptr]),
@@ -101,6 +117,11 @@ _SIGNATURES = {
     "warp_amd_bsr_mm_count_workspace_size": (c_size, [c_int]),
     "warp_amd_bsr_mm_count": (c_int, [c_int] * 5 + [c_ptr] * 9 + [c_ptr, c_size, c_ptr]),
     "warp_amd_bsr_mm_list": (c_int, [c_int] * 4 + [c_ptr] * 9 + [c_ptr]),
+    "warp_amd_bsr_mm_values_tiled": (
+        c_int,
+        [c_int, c_int, c_int, c_int, c_f64, c_f64, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_int]
+        + [c_ptr] * 8 + [c_ptr],
+    ),
     "warp_amd_bsr_mm_values": (
         c_int,
         [c_int, c_int, c_int, c_int, c_f64, c_f64, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_int]
@@ -173,6 +194,7 @@ _SIGNATURES = {
     "warp_amd_seg_dot_plan_workspace_size": (c_size, [c_int]),
     "warp_amd_seg_dot_plan": (c_int, [c_int, c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
     "warp_amd_cg_tolerance_batched": (c_int, [c_int, c_ptr, c_f64, c_f64, c_int, c_ptr, c_ptr]),
+    "warp_amd_bsr_mv_packed": (c_int, [ctypes.POINTER(BsrMvArgs), c_ptr, c_ptr, c_f64, c_f64, c_ptr]),
 }
 
 for _sfx, _s in (("f32", c_f32), ("f64", c_f64)):

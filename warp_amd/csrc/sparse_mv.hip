@@ -536,6 +536,35 @@ WA_API int warp_amd_bsr_mv_lanes(int nrow, int block_rows, int block_cols, int64
 WA_DEFINE_SPARSE_MV(f32, float)
 WA_DEFINE_SPARSE_MV(f64, double)
 
+// The matrix's arguments of a product, packed once by the caller (host
+// struct; layout of warp_amd_bsr_mv_args in include/warp_amd.h).
+struct WaBsrMvArgs {
+    int nrow, ncol, block_rows, block_cols;
+    const int* offsets;
+    const int* row_counts;
+    const int* columns;
+    const void* values;
+    int64_t nnz_hint;
+    int lanes, rows_per_group, scalar_size, reserved;
+};
+static_assert(sizeof(WaBsrMvArgs) == 72, "warp_amd_bsr_mv_args layout");
+
+WA_API int warp_amd_bsr_mv_packed(const WaBsrMvArgs* a, const void* x, void* y, double alpha, double beta,
+                                  void* stream)
+{
+    WA_REQUIRE(a != nullptr, "bsr_mv_packed: null argument struct");
+    if (a->scalar_size == 4)
+        return wa::bsr_mv_impl<float>(a->nrow, a->ncol, a->block_rows, a->block_cols, a->offsets, a->row_counts,
+                                      a->columns, static_cast<const float*>(a->values), a->nnz_hint,
+                                      static_cast<const float*>(x), static_cast<float*>(y), (float)alpha,
+                                      (float)beta, a->lanes, a->rows_per_group, stream);
+    WA_REQUIRE(a->scalar_size == 8, "bsr_mv_packed: scalar size must be 4 or 8, got %d", a->scalar_size);
+    return wa::bsr_mv_impl<double>(a->nrow, a->ncol, a->block_rows, a->block_cols, a->offsets, a->row_counts,
+                                   a->columns, static_cast<const double*>(a->values), a->nnz_hint,
+                                   static_cast<const double*>(x), static_cast<double*>(y), alpha, beta, a->lanes,
+                                   a->rows_per_group, stream);
+}
+
 // y = alpha * A^T x + beta * y for an nrow x ncol BSR matrix of
 // block_rows x block_cols blocks: x has nrow * block_rows entries, y has
 // ncol * block_cols.  Atomic scatter, as the reference: non-deterministic

@@ -208,6 +208,99 @@ __global__ __launch_bounds__(kWave) void assign_padded_kernel(
             dst[(size_t)dbeg * block_size + e] = (DS)(scale * src[(size_t)beg * block_size + e]);
 }
 
+// First position in [lo, hi) whose column is >= v (columns sorted).
+__device__ __forceinline__ int lower_bound_col(const int* __restrict__ columns, int lo, int hi, int v)
+{
+    while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (columns[mid] < v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Padded topology of a reblocked copy, one thread per destination row
+// (reference _bsr_assign_padded_reblock_topology and
+// _bsr_reblock_row_next_col, warp/_src/sparse.py:2000-2083): the
+// destination columns are the sorted union, over the source rows covering
+// the destination row, of the destination columns each source block
+// overlaps.  Each step takes the smallest destination column above the
+// previous one: per source row, the first block at or past the source column
+// holding that column's first sub-column (binary search) gives the row's
+// candidate.  A row that does not fit its capacity is left empty and raises
+// *status.
+__global__ __launch_bounds__(kBlock) void reblock_topology_kernel(
+    Reblock rb, int src_nrow, int dest_nrow, int dest_ncol, const int* __restrict__ s_off,
+    const int* __restrict__ s_rc, const int* __restrict__ s_col, const int* __restrict__ d_off,
+    int* __restrict__ d_rc, int* __restrict__ d_col, int* __restrict__ status)
+{
+    const int drow = (int)(blockIdx.x * kBlock + threadIdx.x);
+    if (drow >= dest_nrow)
+        return;
+    const int64_t first_sub = (int64_t)drow * rb.dest_subrows;
+    const int r0 = (int)(first_sub / rb.src_subrows);
+    const int r1 = (int)min<int64_t>(src_nrow, (first_sub + rb.dest_subrows + rb.src_subrows - 1) / rb.src_subrows);
+    const int beg = d_off[drow];
+    const int cap = d_off[drow + 1];
+    int b = beg;
+    int prev = -1;
+    bool overflow = false;
+    for (;;) {
+        const int minc = prev + 1;
+        int next = dest_ncol;
+        for (int r = r0; r < r1; ++r) {
+            const int sb = s_off[r];
+            const int se = row_end(s_off, s_rc, r);
+            const int sc = (int)(((int64_t)minc * rb.dest_subcols) / rb.src_subcols);
+            const int k = lower_bound_col(s_col, sb, se, sc);
+            if (k == se)
+                continue;
+            const int64_t sub0 = (int64_t)s_col[k] * rb.src_subcols;
+            const int dbeg = (int)(sub0 / rb.dest_subcols);
+            const int dend = (int)((sub0 + rb.src_subcols + rb.dest_subcols - 1) / rb.dest_subcols);
+            const int dc = max(dbeg, minc);
+            if (dc < dend && dc < next)
+                next = dc;
+        }
+        if (next >= dest_ncol)
+            break;
+        if (b >= cap) {
+            overflow = true;
+            break;
+        }
+        d_col[b++] = next;
+        prev = next;
+    }
+    if (overflow) {
+        d_rc[drow] = 0;
+        if (status)
+            atomicMax(status, kStatusRowCapacityExceeded);
+    } else {
+        d_rc[drow] = b - beg;
+    }
+}
+
+// Row capacity of a reblocked padded copy (reference
+// _bsr_copy_reblocked_capacity_counts, warp/_src/sparse.py:2086-2108): the
+// source capacity of the covering source rows times the sub-columns per
+// source block; cap[drow] (an exclusive scan turns it into offsets).
+__global__ __launch_bounds__(kBlock) void reblock_capacity_kernel(Reblock rb, int src_nrow, int dest_nrow,
+                                                                  const int* __restrict__ s_off, int* __restrict__ cap)
+{
+    const int drow = (int)(blockIdx.x * kBlock + threadIdx.x);
+    if (drow >= dest_nrow)
+        return;
+    const int64_t first_sub = (int64_t)drow * rb.dest_subrows;
+    const int r0 = (int)(first_sub / rb.src_subrows);
+    const int r1 = (int)min<int64_t>(src_nrow, (first_sub + rb.dest_subrows + rb.src_subrows - 1) / rb.src_subrows);
+    int64_t n = 0;
+    for (int r = r0; r < r1; ++r)
+        n += (int64_t)(s_off[r + 1] - s_off[r]) * rb.src_subcols;
+    cap[drow] = (int)n;
+}
+
 // rows[p] = r for the active positions p < T of row r (one wavefront per row;
 // the other positions were set to -1 beforehand).
 __global__ __launch_bounds__(kWave * kRowsPerBlock) void uncompress_rows_kernel(int nrow,
@@ -468,6 +561,75 @@ __global__ __launch_bounds__(kBlock) void mm_values_kernel(
 #pragma unroll
     for (int e = 0; e < BR * BC; ++e)
         zv[e] = (S(0) + beta * zv[e]) + alpha * acc[e];
+}
+
+// Tiled product values (replaces the reference's tiled-outer values kernel,
+// make_bsr_mm_compute_values_tiled_outer, warp/_src/sparse.py:3813-3909):
+// G lanes per z block.  The block's x row is read as the flat list of
+// (x block k, inner index m) pairs, m fastest; lane g takes pairs g, g + G,
+// ... and adds the rank-one update x_k[:, m] y(x_col k, c)[m, :] (each product
+// rounded, then added), skipping pairs whose y block is absent; the G
+// partial blocks are then combined by an xor butterfly (lane g adds lane
+// g ^ off for off = G/2 .. 1).  Finally z = (0 + beta z) + alpha acc.  Block
+// entries br * bc <= MAXE.
+template <typename S, int MAXE>
+__global__ __launch_bounds__(kBlock) void mm_values_tiled_kernel(
+    int T, int G, int z_nrow, S alpha, S beta, int BR, int BK, int BC, const int* __restrict__ z_offsets,
+    const int* __restrict__ z_row_counts, const int* __restrict__ z_columns, S* __restrict__ z_values, int y_nrow,
+    const int* __restrict__ x_offsets, const int* __restrict__ x_row_counts, const int* __restrict__ x_columns,
+    const S* __restrict__ x_values, const int* __restrict__ y_offsets, const int* __restrict__ y_row_counts,
+    const int* __restrict__ y_columns, const S* __restrict__ y_values)
+{
+#pragma clang fp contract(off)
+    const int p = (int)((blockIdx.x * kBlock + threadIdx.x) / G);
+    const int lane = (int)(threadIdx.x % G);
+    const int ne = BR * BC;
+    int r = -1;
+    int col = 0;
+    if (p < T) {
+        r = row_of_position(z_offsets, z_row_counts, z_nrow, p);
+        col = r >= 0 ? z_columns[p] : 0;
+    }
+    S acc[MAXE];
+#pragma unroll
+    for (int e = 0; e < MAXE; ++e)
+        acc[e] = S(0);
+    if (r >= 0) {
+        const int xb = x_offsets[r];
+        const int n = (row_end(x_offsets, x_row_counts, r) - xb) * BK;
+        for (int c = lane; c < n; c += G) {
+            const int kk = c / BK;
+            const int m = c - kk * BK;
+            const int k = xb + kk;
+            const int xc = x_columns[k];
+            if (xc < 0 || xc >= y_nrow)
+                continue;
+            const int b = find_block(y_offsets, y_row_counts, y_columns, xc, col);
+            if (b < 0)
+                continue;
+            const S* xv = x_values + (size_t)k * BR * BK;
+            const S* yv = y_values + (size_t)b * BK * BC;
+#pragma unroll
+            for (int e = 0; e < MAXE; ++e)
+                if (e < ne) {
+                    const int i = e / BC;
+                    const int j = e - i * BC;
+                    acc[e] = acc[e] + xv[i * BK + m] * yv[m * BC + j];
+                }
+        }
+    }
+    // every lane of the wave takes part in the butterfly (uniform trip count)
+    for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int e = 0; e < MAXE; ++e)
+            acc[e] = acc[e] + __shfl_xor(acc[e], off, kWave);
+    if (r >= 0 && lane == 0) {
+        S* zv = z_values + (size_t)p * ne;
+#pragma unroll
+        for (int e = 0; e < MAXE; ++e)
+            if (e < ne)
+                zv[e] = (S(0) + beta * zv[e]) + alpha * acc[e];
+    }
 }
 
 // Same, with block dimensions known at run time (br * bc <= 64).
@@ -814,6 +976,65 @@ WA_API int warp_amd_bsr_assign_padded(int src_scalar_size, int dest_scalar_size,
     return wa::kOk;
 }
 
+// Padded topology of a reblocked assignment (reference bsr_assign with
+// topology="padded" and a change of block shape, warp/_src/sparse.py:
+// 2201-2255, 2346-2358): dest_columns / dest_row_counts of each destination
+// row within its existing capacity (dest_offsets); rows that do not fit are
+// left empty and raise *status.  The values are then copied by
+// warp_amd_bsr_assign_copy_blocks into the zeroed destination.
+WA_API int warp_amd_bsr_assign_padded_reblock_topology(int src_subrows, int src_subcols, int dest_subrows,
+                                                       int dest_subcols, int src_nrow, int dest_nrow, int dest_ncol,
+                                                       const int* src_offsets, const int* src_row_counts,
+                                                       const int* src_columns, const int* dest_offsets,
+                                                       int* dest_row_counts, int* dest_columns, int* status,
+                                                       void* stream)
+{
+    WA_REQUIRE(src_subrows > 0 && src_subcols > 0 && dest_subrows > 0 && dest_subcols > 0 && src_nrow >= 0 &&
+                   dest_nrow >= 0 && dest_ncol >= 0,
+               "bsr_assign: invalid reblocking sizes");
+    if (dest_nrow == 0)
+        return wa::kOk;
+    WA_REQUIRE(src_offsets && dest_offsets && dest_row_counts && (src_nrow == 0 || src_columns),
+               "bsr_assign: null arrays");
+    const wa::Reblock rb{src_subrows, src_subcols, dest_subrows, dest_subcols};
+    wa::reblock_topology_kernel<<<(unsigned)wa::ceil_div(dest_nrow, wa::kBlock), wa::kBlock, 0, wa::as_stream(stream)>>>(
+        rb, src_nrow, dest_nrow, dest_ncol, src_offsets, src_row_counts, src_columns, dest_offsets, dest_row_counts,
+        dest_columns, status);
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}
+
+// Workspace of warp_amd_bsr_reblock_capacity (its offsets scan).
+WA_API size_t warp_amd_bsr_reblock_capacity_workspace_size(int dest_nrow)
+{
+    return wa::scan_workspace_bytes((int64_t)(dest_nrow > 0 ? dest_nrow : 0) + 1);
+}
+
+// Offsets of a reblocked padded copy (reference bsr_copy with
+// topology="padded" and a change of block shape, warp/_src/sparse.py:
+// 2561-2588): each destination row holds the capacity of its covering source
+// rows times the sub-columns per source block; dest_offsets[dest_nrow + 1].
+WA_API int warp_amd_bsr_reblock_capacity(int src_subrows, int src_subcols, int dest_subrows, int dest_subcols,
+                                         int src_nrow, int dest_nrow, const int* src_offsets, int* dest_offsets,
+                                         void* workspace, size_t workspace_bytes, void* stream)
+{
+    WA_REQUIRE(src_subrows > 0 && src_subcols > 0 && dest_subrows > 0 && dest_subcols > 0 && src_nrow >= 0 &&
+                   dest_nrow >= 0,
+               "bsr_copy: invalid reblocking sizes");
+    WA_REQUIRE(dest_offsets, "bsr_copy: null offsets");
+    hipStream_t s = wa::as_stream(stream);
+    if (dest_nrow == 0)
+        return wa::launch_fill_i32(dest_offsets, 1, 0, s);
+    WA_REQUIRE(src_offsets, "bsr_copy: null source offsets");
+    WA_REQUIRE(workspace && workspace_bytes >= warp_amd_bsr_reblock_capacity_workspace_size(dest_nrow),
+               "bsr_copy: workspace too small");
+    const wa::Reblock rb{src_subrows, src_subcols, dest_subrows, dest_subcols};
+    wa::reblock_capacity_kernel<<<(unsigned)wa::ceil_div(dest_nrow, wa::kBlock), wa::kBlock, 0, s>>>(
+        rb, src_nrow, dest_nrow, src_offsets, dest_offsets);
+    WA_LAUNCH_CHECK();
+    return wa::exclusive_scan_i32(dest_offsets, dest_offsets, dest_nrow, workspace, s);
+}
+
 // x[0..n) *= alpha (reference bsr_scale's value kernel); fp32 or fp64.
 WA_API int warp_amd_copy_scaled_values(int src_scalar_size, int dst_scalar_size, int64_t n, const void* src,
                                        double scale, void* dst, void* stream)
@@ -1048,6 +1269,49 @@ WA_API int warp_amd_bsr_mm_list(int n, int base, int T, int x_nrow, const int* x
 // (br * bc <= 64); fp32 or fp64.  No fused multiply-add: products and sums
 // are rounded one by one, in the order stated.  The old block enters as
 // 0 + beta * z (beta 1: a topology rebuild already scaled it).
+// Tiled product values: `tile` lanes per z block (a power of two in
+// [1, 64]), rank-one updates per (x block, inner index) pair and an xor
+// butterfly (replaces bsr_mm's tiled-outer values kernel, warp/_src/
+// sparse.py:3813-3909, selected by tile_size > 0, 4101-4144); otherwise as
+// warp_amd_bsr_mm_values.
+WA_API int warp_amd_bsr_mm_values_tiled(int scalar_size, int br, int bk, int bc, double alpha, double beta, int T,
+                                        int tile, int z_nrow, const int* z_offsets, const int* z_row_counts,
+                                        const int* z_columns, void* z_values, int y_nrow, const int* x_offsets,
+                                        const int* x_row_counts, const int* x_columns, const void* x_values,
+                                        const int* y_offsets, const int* y_row_counts, const int* y_columns,
+                                        const void* y_values, void* stream)
+{
+    WA_REQUIRE(br > 0 && bk > 0 && bc > 0 && br * bc <= 64 && T >= 0 && z_nrow >= 0, "bsr_mm: invalid sizes");
+    WA_REQUIRE(scalar_size == 4 || scalar_size == 8, "bsr_mm: values must be fp32 or fp64");
+    WA_REQUIRE(tile >= 1 && tile <= wa::kWave && (tile & (tile - 1)) == 0,
+               "bsr_mm: tile_size must be a power of two in [1, 64], got %d", tile);
+    if (T == 0 || z_nrow == 0)
+        return wa::kOk;
+    WA_REQUIRE(z_offsets && z_columns && z_values && x_offsets && y_offsets, "bsr_mm: null arrays");
+    const unsigned grid = (unsigned)wa::ceil_div((int64_t)T * tile, wa::kBlock);
+    hipStream_t s = wa::as_stream(stream);
+#define WA_MMT(S, MAXE)                                                                                               \
+    wa::mm_values_tiled_kernel<S, MAXE><<<grid, wa::kBlock, 0, s>>>(                                                  \
+        T, tile, z_nrow, (S)alpha, (S)beta, br, bk, bc, z_offsets, z_row_counts, z_columns, static_cast<S*>(z_values), \
+        y_nrow, x_offsets, x_row_counts, x_columns, static_cast<const S*>(x_values), y_offsets, y_row_counts,          \
+        y_columns, static_cast<const S*>(y_values))
+    const bool small = br * bc <= 9;
+    if (scalar_size == 4) {
+        if (small)
+            WA_MMT(float, 9);
+        else
+            WA_MMT(float, 64);
+    } else {
+        if (small)
+            WA_MMT(double, 9);
+        else
+            WA_MMT(double, 64);
+    }
+#undef WA_MMT
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}
+
 WA_API int warp_amd_bsr_mm_values(int scalar_size, int br, int bk, int bc, double alpha, double beta, int T,
                                   int z_nrow,
                                   const int* z_offsets, const int* z_row_counts, const int* z_columns,

@@ -14,6 +14,7 @@ status of padded operations), ``bsr_zeros``, ``bsr_set_zero``,
 
 from __future__ import annotations
 
+import ctypes
 import warnings
 
 import torch
@@ -1052,9 +1053,9 @@ def bsr_assign(dest: BsrMatrix, src, structure_only: bool = False, masked: bool 
     divide, or are divided by, those of ``src``.  ``"compact"`` (default)
     rebuilds the topology of ``dest``; ``"masked"`` keeps it (source blocks
     outside it are dropped, missing ones become zero); ``"padded"`` copies
-    each source row into the existing row capacity of ``dest`` (same block
-    shape), raising ``BSR_STATUS_ROW_CAPACITY_EXCEEDED`` for rows that do not
-    fit."""
+    each source row (or, across block shapes, the blocks its source rows
+    overlap) into the existing row capacity of ``dest``, raising
+    ``BSR_STATUS_ROW_CAPACITY_EXCEEDED`` for rows that do not fit."""
     dest._bump_topology()
     src, src_scale = _extract_matrix_and_scale(src)
     if dest.values.device != src.values.device:
@@ -1081,9 +1082,11 @@ def bsr_assign(dest: BsrMatrix, src, structure_only: bool = False, masked: bool 
             if not structure_only and src_scale != 1.0:
                 bsr_scale(dest, src_scale)
             return
-        if dest.block_shape != src.block_shape:
-            raise NotImplementedError("bsr_assign(topology='padded') with a change of block shape is not provided")
         _bsr_ensure_independent_row_counts(dest)
+        if dest.block_shape != src.block_shape:
+            _assign_padded_reblock(dest, src, src_scale, structure_only, status, src_subrows, src_subcols,
+                                   dest_subrows, dest_subcols)
+            return
         with_values = not structure_only
         if with_values:
             for m in (src, dest):
@@ -1161,6 +1164,22 @@ def bsr_assign(dest: BsrMatrix, src, structure_only: bool = False, masked: bool 
         _assign_values(dest, src, src_scale, T, src_subrows, src_subcols, dest_subrows, dest_subcols)
 
 
+def _assign_padded_reblock(dest, src, src_scale, structure_only, status, src_subrows, src_subcols, dest_subrows,
+                           dest_subcols):
+    """Padded assignment across block shapes (sparse.py:2201-2255): the
+    destination topology within its row capacity, then the sub-blocks copied
+    into zeroed values."""
+    _native.call(
+        "warp_amd_bsr_assign_padded_reblock_topology", src_subrows, src_subcols, dest_subrows, dest_subcols, src.nrow,
+        dest.nrow, dest.ncol, _native.ptr(src.offsets), _native.ptr(src.row_counts), _native.ptr(src.columns),
+        _native.ptr(dest.offsets), _native.ptr(dest.row_counts), _native.ptr(dest.columns), _native.ptr(status),
+        _native.stream_of(dest.device),
+    )
+    if not structure_only:
+        dest.values.zero_()
+        _assign_values(dest, src, src_scale, _positions(src), src_subrows, src_subcols, dest_subrows, dest_subcols)
+
+
 def _assign_values(dest, src, src_scale, T, src_subrows, src_subcols, dest_subrows, dest_subcols):
     for m in (src, dest):
         if m.scalar_type.torch_dtype not in (torch.float32, torch.float64):
@@ -1179,8 +1198,9 @@ def _assign_values(dest, src, src_scale, T, src_subrows, src_subcols, dest_subro
 def bsr_copy(A, scalar_type=None, block_shape=None, structure_only: bool = False, topology: str = "compact"):
     """Copy of ``A`` (a matrix or ``scale * matrix``), possibly with another scalar type or block shape (sparse.py:2455-2601).
 
-    ``topology="padded"`` keeps the row capacity and ``row_counts`` of ``A``
-    (same block shape only)."""
+    ``topology="padded"`` keeps the row capacity and ``row_counts`` of ``A``;
+    with another block shape, each row gets the capacity of the source rows
+    it covers (times the sub-columns per source block)."""
     src, src_scale = _extract_matrix_and_scale(A)
     if topology not in ("compact", "padded"):
         raise ValueError(f"Unsupported topology policy: {topology}")
@@ -1190,9 +1210,27 @@ def bsr_copy(A, scalar_type=None, block_shape=None, structure_only: bool = False
         block_shape = src.block_shape
     block_shape = tuple(block_shape)
     block_type = scalar_type if block_shape == (1, 1) else wt.matrix(shape=block_shape, dtype=scalar_type)
+    if topology == "padded" and block_shape != src.block_shape:
+        # capacity of the covering source rows, then the padded reblocked assignment (sparse.py:2561-2588)
+        src_subrows, src_subcols, dest_subrows, dest_subcols, nrow, ncol = _reblock(src, block_shape)
+        _require_gpu(src.device, "bsr_copy")
+        copy = bsr_zeros(nrow, ncol, block_type, device=src.device, topology="padded")
+        status = copy._ensure_status()
+        _bsr_ensure_fits(copy, nnz=int(src.nnz) * src_subrows * src_subcols)
+        lib = _native.lib()
+        ws_bytes = lib.warp_amd_bsr_reblock_capacity_workspace_size(nrow)
+        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=src.device)
+        _native.call(
+            "warp_amd_bsr_reblock_capacity", src_subrows, src_subcols, dest_subrows, dest_subcols, src.nrow, nrow,
+            _native.ptr(src.offsets), _native.ptr(copy.offsets), _native.ptr(ws), ws_bytes,
+            _native.stream_of(src.device),
+        )
+        copy.row_counts = torch.zeros(nrow, dtype=torch.int32, device=src.device)
+        copy.columns.fill_(-1)
+        _assign_padded_reblock(copy, src, src_scale, structure_only, status, src_subrows, src_subcols, dest_subrows,
+                               dest_subcols)
+        return copy
     if topology == "padded":
-        if block_shape != src.block_shape:
-            raise NotImplementedError("bsr_copy(topology='padded') with a change of block shape is not provided")
         copy = bsr_zeros(src.nrow, src.ncol, block_type, device=src.device, topology="padded")
         _bsr_ensure_fits(copy, nnz=src.nnz)
         copy.offsets[: src.nrow + 1].copy_(src.offsets[: src.nrow + 1])
@@ -1406,7 +1444,11 @@ def bsr_mm(x, y, z: BsrMatrix | None = None, alpha=1.0, beta=0.0, masked: bool =
     product topology (plus ``z``'s blocks when ``beta != 0``) into the row
     capacity of ``z``, raising the row-capacity status where it does not fit.
     Each block is ``sum_k x[r, k] @ y[k, c]`` over the row's blocks of ``x`` in
-    storage order; ``tile_size`` is accepted and does not change the order.
+    storage order.  ``tile_size > 0`` (a power of two <= 64) selects the tiled
+    order: that many lanes per block each add the rank-one updates
+    ``x[r, k][:, m] y[k, c][m, :]`` of every ``tile_size``-th (block, inner
+    index) pair, combined by an xor butterfly; ``tile_size <= 0`` keeps the
+    block-by-block order.
     ``max_new_nnz`` caps the product blocks listed (extra ones are dropped with
     a warning, as the reference does).  Operands may alias ``z``."""
     x, x_scale = _extract_matrix_and_scale(x)
@@ -1423,6 +1465,9 @@ def bsr_mm(x, y, z: BsrMatrix | None = None, alpha=1.0, beta=0.0, masked: bool =
         _warn_masked_arg_deprecated("bsr_mm")
     if topology != "compact" and reuse_topology:
         raise ValueError("reuse_topology is only supported with topology='compact'")
+    tile = int(tile_size) if tile_size > 0 else 0
+    if tile and (tile & (tile - 1) or tile > 64):
+        raise ValueError(f"tile_size must be a power of two <= 64, got {tile_size}")
     if z is None:
         if topology != "compact":
             raise ValueError("Left-hand-side 'z' matrix must be provided for this topology policy")
@@ -1466,7 +1511,7 @@ def bsr_mm(x, y, z: BsrMatrix | None = None, alpha=1.0, beta=0.0, masked: bool =
             raise ValueError("topology='masked' is not supported for aliased inputs")
         if beta == 0.0:
             z.values.zero_()
-        _mm_values(x, y, z, alpha, _positions(z), beta if beta != 0.0 else 1.0)
+        _mm_values(x, y, z, alpha, _positions(z), beta if beta != 0.0 else 1.0, tile)
         return z
 
     # operands aliasing the output are read from copies
@@ -1519,7 +1564,7 @@ def bsr_mm(x, y, z: BsrMatrix | None = None, alpha=1.0, beta=0.0, masked: bool =
     if not padded and keep_z and max_new_nnz is None and _union_is_own_topology(z, rows[zT:], cols[zT:], n_new):
         # every product block is already in z: the compact rebuild would give
         # z's own topology back; same values (0 + beta * old, then the sums)
-        _mm_values(x, y, z, alpha, zT, beta)
+        _mm_values(x, y, z, alpha, zT, beta, tile)
         return z
     old_values = None
     if zT > 0:
@@ -1545,16 +1590,19 @@ def bsr_mm(x, y, z: BsrMatrix | None = None, alpha=1.0, beta=0.0, masked: bool =
     z.values.zero_()
     if zT > 0:
         _add_blocks(z, rows, cols, old_values, zT, beta)
-    _mm_values(x, y, z, alpha, zpos)
+    _mm_values(x, y, z, alpha, zpos, 1.0, tile)
     return z
 
 
-def _mm_values(x: BsrMatrix, y: BsrMatrix, z: BsrMatrix, alpha: float, T: int, beta: float = 1.0):
-    """z = (0 + beta * z) + alpha * (x @ y) over z's first T positions."""
+def _mm_values(x: BsrMatrix, y: BsrMatrix, z: BsrMatrix, alpha: float, T: int, beta: float = 1.0, tile: int = 0):
+    """z = (0 + beta * z) + alpha * (x @ y) over z's first T positions (tile
+    > 0: the tiled summation order with that many lanes per block)."""
     br, bk = x.block_shape
     bc = y.block_shape[1]
+    head = (("warp_amd_bsr_mm_values_tiled", z.scalar_type.size, br, bk, bc, float(alpha), float(beta), T, tile)
+            if tile else ("warp_amd_bsr_mm_values", z.scalar_type.size, br, bk, bc, float(alpha), float(beta), T))
     _native.call(
-        "warp_amd_bsr_mm_values", z.scalar_type.size, br, bk, bc, float(alpha), float(beta), T, z.nrow,
+        *head, z.nrow,
         _native.ptr(z.offsets),
         _native.ptr(z.row_counts), _native.ptr(z.columns), _native.ptr(z.values), y.nrow, _native.ptr(x.offsets),
         _native.ptr(x.row_counts), _native.ptr(x.columns), _native.ptr(x.values), _native.ptr(y.offsets),
@@ -1733,6 +1781,36 @@ def _vec_view(array: torch.Tensor, torch_dtype: torch.dtype, expected_scalar_cou
     return array
 
 
+class _MvPlan:
+    """A matrix's constant arguments of bsr_mv, packed once (host struct of
+    warp_amd_bsr_mv_packed), with what its x and y must match.  Valid for one
+    topology version, one ``values`` tensor and one nnz."""
+
+    __slots__ = ("version", "values", "nnz", "dtype", "nx", "ny", "device", "dev_index", "args", "args_ref", "fn")
+
+    def __init__(self, A: "BsrMatrix", nnz_hint: int, torch_dtype: torch.dtype, nx: int, ny: int):
+        br, bc = A.block_shape
+        self.version = A.__dict__.get("_topology_version", 0)
+        self.values = A.values
+        self.nnz = A.nnz
+        self.dtype = torch_dtype
+        self.nx, self.ny = nx, ny
+        self.device = A.values.device
+        self.dev_index = A.values.get_device()
+        a = _native.BsrMvArgs()
+        a.nrow, a.ncol, a.block_rows, a.block_cols = A.nrow, A.ncol, br, bc
+        a.offsets = A.offsets.data_ptr()
+        a.row_counts = A.row_counts.data_ptr() if A.row_counts is not None else None
+        a.columns = A.columns.data_ptr()
+        a.values = A.values.data_ptr()
+        a.nnz_hint = int(nnz_hint)
+        a.lanes, a.rows_per_group = 0, 0
+        a.scalar_size = 8 if torch_dtype == torch.float64 else 4
+        self.args = a
+        self.args_ref = ctypes.byref(a)
+        self.fn = _native.lib().warp_amd_bsr_mv_packed
+
+
 def bsr_mv(
     A,
     x: torch.Tensor,
@@ -1751,6 +1829,33 @@ def bsr_mv(
     negative selects one lane per row (sums in storage order, like the
     reference row kernel); 0 chooses from the mean row length.
     """
+    if type(A) is BsrMatrix and y is not None and not transpose and work_buffer is None and tile_size == 0:
+        # repeated products with one matrix: its arguments are packed once
+        # (_MvPlan), and x / y get the same checks in a cheaper form
+        plan = A.__dict__.get("_mv_plan")
+        if (
+            plan is not None
+            and plan.version == A.__dict__.get("_topology_version", 0)
+            and plan.values is A.values
+            and plan.nnz == A.nnz
+            and not A.__dict__.get("_nnz_pending", False)
+            and x.dtype is plan.dtype
+            and y.dtype is plan.dtype
+            and x.numel() == plan.nx
+            and y.numel() == plan.ny
+            and x.ndim <= 2
+            and y.ndim <= 2
+            and x.is_contiguous()
+            and y.is_contiguous()
+            and x.get_device() == plan.dev_index
+            and y.get_device() == plan.dev_index
+        ):
+            xp, yp = x.data_ptr(), y.data_ptr()
+            if xp != yp:
+                st = plan.fn(plan.args_ref, xp, yp, float(alpha), float(beta), _native.stream_of(plan.device))
+                if st:
+                    _native.check(st, "warp_amd_bsr_mv_packed")
+                return y
     A, A_scale = _extract_matrix_and_scale(A)
     alpha = float(alpha) * A_scale
 
@@ -1825,6 +1930,14 @@ def bsr_mv(
                          _native.stream_of(y_view.device))
         return y
     fn = "warp_amd_bsr_mv_ex_" + sfx
+    if tile_size == 0 and type(A) is BsrMatrix and not A.__dict__.get("_nnz_pending", False):
+        plan = _MvPlan(A, nnz_hint, torch_dtype, x_view.numel(), y_view.numel())
+        A.__dict__["_mv_plan"] = plan
+        _native.check(
+            plan.fn(plan.args_ref, x_view.data_ptr(), y_view.data_ptr(), alpha, beta, _native.stream_of(plan.device)),
+            "warp_amd_bsr_mv_packed",
+        )
+        return y
     # tile_size > 0: lanes per block row; < 0: one lane per row (row-sequential
     # sums, the reference row kernel's order); 0: chosen from the mean row length
     if tile_size > 0:
