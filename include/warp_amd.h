@@ -417,7 +417,12 @@ int warp_amd_bsr_mv_ex_f64(
  * matrix many times fills it once and passes 6 arguments per call instead of
  * 16, which is most of the host cost of a small product).  scalar_size 4:
  * float values / vectors, 8: double; alpha and beta are rounded to the
- * scalar type.  The struct is read during the call only. */
+ * scalar type.  The struct is read during the call only.  sweep bit 0
+ * reverses the order in which the workgroups take the block rows (1x1 and
+ * 3x3 blocks; the result is bit-identical either way): a caller that flips
+ * it between consecutive products with one matrix reads first, in each
+ * product, the rows the previous one read last, while they are still in
+ * the GPU's memory-side cache. */
 typedef struct warp_amd_bsr_mv_args {
     int nrow, ncol, block_rows, block_cols;
     const int* offsets;    /* device */
@@ -425,7 +430,7 @@ typedef struct warp_amd_bsr_mv_args {
     const int* columns;    /* device */
     const void* values;    /* device */
     int64_t nnz_hint;
-    int lanes, rows_per_group, scalar_size, reserved;
+    int lanes, rows_per_group, scalar_size, sweep;
 } warp_amd_bsr_mv_args;
 int warp_amd_bsr_mv_packed(const warp_amd_bsr_mv_args* args, const void* x, void* y, double alpha, double beta,
                            void* stream);

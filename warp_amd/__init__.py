@@ -10,7 +10,7 @@ device; the compute runs in the hand-written HIP library ``_lib/libwarp_amd.so``
 """
 
 from . import types
-from .types import float32, float64, int32, mat33, mat33d, matrix, vec3, vec3d, vector
+from .types import float32, float64, int32, mat22, mat33, mat33d, mat44, matrix, vec2, vec3, vec3d, vec4, vector
 
 __version__ = "0.1.0"
 

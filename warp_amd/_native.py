@@ -38,7 +38,7 @@ class BsrMvArgs(ctypes.Structure):
         ("nrow", c_int), ("ncol", c_int), ("block_rows", c_int), ("block_cols", c_int),
         ("offsets", c_ptr), ("row_counts", c_ptr), ("columns", c_ptr), ("values", c_ptr),
         ("nnz_hint", c_i64),
-        ("lanes", c_int), ("rows_per_group", c_int), ("scalar_size", c_int), ("reserved", c_int),
+        ("lanes", c_int), ("rows_per_group", c_int), ("scalar_size", c_int), ("sweep", c_int),
     ]
 
 

@@ -300,8 +300,12 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(
     constexpr int kTileRows = (kBlock / G) * kCgRpg;
     const int ntiles = (nrow + kTileRows - 1) / kTileRows;
     const int lane = threadIdx.x % G;
+    // odd iterations walk the tiles from the last: each product starts with
+    // the rows the previous one read last, still in the memory-side cache
+    const bool reverse = state->iter & 1;
     double acc[1] = {0.0};
-    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int tile = reverse ? ntiles - 1 - t : t;
         const int row0 = (tile * (kBlock / G) + (int)threadIdx.x / G) * kCgRpg;
         if (row0 >= nrow)
             continue;

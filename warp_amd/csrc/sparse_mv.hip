@@ -16,18 +16,25 @@ constexpr int kBlock = 256;
 constexpr bool kInterleaveRows = true;
 
 // RPG block rows per group of G lanes (see group_rows_product).
+// reverse: the waves take their row ranges from the last to the first (each
+// row's sum is unchanged).  Alternated between consecutive products with one
+// matrix, the rows read last by one product are read first by the next, while
+// they are still in the 256 MB memory-side cache (5.99 vs 5.13-5.38 TB/s on
+// the 295 MB 64^3 elasticity product).
 template <typename S, int BR, int BC, int G, int RPG>
 __global__ __launch_bounds__(kBlock) void bsr_mv_rows_kernel(
     int nrow, const int* __restrict__ offsets, const int* __restrict__ row_counts, const int* __restrict__ columns,
-    const S* __restrict__ values, const S* __restrict__ x, S* __restrict__ y, S alpha, S beta
+    const S* __restrict__ values, const S* __restrict__ x, S* __restrict__ y, S alpha, S beta, int reverse
 )
 {
     static_assert(G <= kWave, "a row group lies within one wave");
     constexpr int kStride = kInterleaveRows ? kWave / G : 1;
-    const int wave = (int)((blockIdx.x * kBlock + threadIdx.x) / kWave);
+    constexpr int kWavesPerBlock = kBlock / kWave;
+    int wave = (int)((blockIdx.x * kBlock + threadIdx.x) / kWave);
+    if (reverse)
+        wave = (int)gridDim.x * kWavesPerBlock - 1 - wave;
     const int gw = (int)(threadIdx.x % kWave) / G;
-    const int row0 = kInterleaveRows ? wave * (kWave / G) * RPG + gw
-                                     : (int)(blockIdx.x * (kBlock / G) + threadIdx.x / G) * RPG;
+    const int row0 = kInterleaveRows ? wave * (kWave / G) * RPG + gw : (wave * (kWave / G) + gw) * RPG;
     const int lane = threadIdx.x % G;
     if (row0 >= nrow)
         return;
@@ -82,13 +89,14 @@ __global__ __launch_bounds__(kBlock) void bsr_mv_generic_kernel(
 template <typename S, int BR, int BC, int G, int RPG>
 int launch_rows(
     int nrow, const int* offsets, const int* row_counts, const int* columns, const S* values, const S* x, S* y,
-    S alpha, S beta, hipStream_t stream
+    S alpha, S beta, int reverse, hipStream_t stream
 )
 {
     constexpr int kRowsPerBlock = (kBlock / G) * RPG;
     const int64_t blocks = ceil_div(nrow, kRowsPerBlock);
-    bsr_mv_rows_kernel<S, BR, BC, G, RPG>
-        <<<(unsigned)blocks, kBlock, 0, stream>>>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta);
+    bsr_mv_rows_kernel<S, BR, BC, G, RPG><<<(unsigned)blocks, kBlock, 0, stream>>>(
+        nrow, offsets, row_counts, columns, values, x, y, alpha, beta, reverse
+    );
     WA_LAUNCH_CHECK();
     return kOk;
 }
@@ -96,29 +104,32 @@ int launch_rows(
 template <typename S, int BR, int BC, int G>
 int launch_rpg(
     int rpg, int nrow, const int* offsets, const int* row_counts, const int* columns, const S* values, const S* x,
-    S* y, S alpha, S beta, hipStream_t stream
+    S* y, S alpha, S beta, int reverse, hipStream_t stream
 )
 {
+#define WA_RPG_ARGS nrow, offsets, row_counts, columns, values, x, y, alpha, beta, reverse, stream
     switch (rpg) {
-    case 1: return launch_rows<S, BR, BC, G, 1>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
-    case 2: return launch_rows<S, BR, BC, G, 2>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+    case 1: return launch_rows<S, BR, BC, G, 1>(WA_RPG_ARGS);
+    case 2: return launch_rows<S, BR, BC, G, 2>(WA_RPG_ARGS);
     case 8:
         if constexpr (BR * BC == 1)
-            return launch_rows<S, BR, BC, G, 8>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+            return launch_rows<S, BR, BC, G, 8>(WA_RPG_ARGS);
         [[fallthrough]];
-    default: return launch_rows<S, BR, BC, G, 4>(nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+    default: return launch_rows<S, BR, BC, G, 4>(WA_RPG_ARGS);
     }
+#undef WA_RPG_ARGS
 }
 
 template <typename S, int BR, int BC>
 int launch_lanes(
     int g, int rpg, int nrow, const int* offsets, const int* row_counts, const int* columns, const S* values,
-    const S* x, S* y, S alpha, S beta, hipStream_t stream
+    const S* x, S* y, S alpha, S beta, int reverse, hipStream_t stream
 )
 {
 #define WA_LANES_CASE(G)                                                                                              \
     case G:                                                                                                           \
-        return launch_rpg<S, BR, BC, G>(rpg, nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+        return launch_rpg<S, BR, BC, G>(rpg, nrow, offsets, row_counts, columns, values, x, y, alpha, beta, reverse, \
+                                        stream);
     switch (g) {
         WA_LANES_CASE(1)
         WA_LANES_CASE(2)
@@ -138,7 +149,7 @@ template <typename S>
 int bsr_mv_impl(
     int nrow, int ncol, int br, int bc, const int* offsets, const int* row_counts, const int* columns,
     const S* values, int64_t nnz_hint, const S* x, S* y, S alpha, S beta, int lanes, int rows_per_group,
-    void* stream_ptr
+    void* stream_ptr, int reverse = 0
 )
 {
     WA_REQUIRE(nrow >= 0 && ncol >= 0 && br > 0 && bc > 0, "bsr_mv: invalid shape (%d, %d, %d, %d)", nrow, ncol, br, bc);
@@ -154,9 +165,11 @@ int bsr_mv_impl(
     const int g = lanes > 0 ? lanes : shape.lanes;
     const int rpg = rows_per_group > 0 ? rows_per_group : shape.rows_per_group;
     if (br == 1 && bc == 1)
-        return launch_lanes<S, 1, 1>(g, rpg, nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+        return launch_lanes<S, 1, 1>(g, rpg, nrow, offsets, row_counts, columns, values, x, y, alpha, beta, reverse,
+                                     stream);
     if (br == 3 && bc == 3)
-        return launch_lanes<S, 3, 3>(g, rpg, nrow, offsets, row_counts, columns, values, x, y, alpha, beta, stream);
+        return launch_lanes<S, 3, 3>(g, rpg, nrow, offsets, row_counts, columns, values, x, y, alpha, beta, reverse,
+                                     stream);
     const int64_t threads = (int64_t)nrow * br;
     bsr_mv_generic_kernel<S><<<(unsigned)ceil_div(threads, kBlock), kBlock, 0, stream>>>(
         nrow, br, bc, offsets, row_counts, columns, values, x, y, alpha, beta
@@ -545,7 +558,7 @@ struct WaBsrMvArgs {
     const int* columns;
     const void* values;
     int64_t nnz_hint;
-    int lanes, rows_per_group, scalar_size, reserved;
+    int lanes, rows_per_group, scalar_size, sweep;
 };
 static_assert(sizeof(WaBsrMvArgs) == 72, "warp_amd_bsr_mv_args layout");
 
@@ -557,12 +570,12 @@ WA_API int warp_amd_bsr_mv_packed(const WaBsrMvArgs* a, const void* x, void* y, 
         return wa::bsr_mv_impl<float>(a->nrow, a->ncol, a->block_rows, a->block_cols, a->offsets, a->row_counts,
                                       a->columns, static_cast<const float*>(a->values), a->nnz_hint,
                                       static_cast<const float*>(x), static_cast<float*>(y), (float)alpha,
-                                      (float)beta, a->lanes, a->rows_per_group, stream);
+                                      (float)beta, a->lanes, a->rows_per_group, stream, a->sweep & 1);
     WA_REQUIRE(a->scalar_size == 8, "bsr_mv_packed: scalar size must be 4 or 8, got %d", a->scalar_size);
     return wa::bsr_mv_impl<double>(a->nrow, a->ncol, a->block_rows, a->block_cols, a->offsets, a->row_counts,
                                    a->columns, static_cast<const double*>(a->values), a->nnz_hint,
                                    static_cast<const double*>(x), static_cast<double*>(y), alpha, beta, a->lanes,
-                                   a->rows_per_group, stream);
+                                   a->rows_per_group, stream, a->sweep & 1);
 }
 
 // y = alpha * A^T x + beta * y for an nrow x ncol BSR matrix of

@@ -4,27 +4,32 @@
 // channels-last fields [voxel][channel] on an m^3 grid, voxel = x + m (y + m z),
 // weights W[tap][ci][co], tap = (dz+1)*9 + (dy+1)*3 + (dx+1), zero padding.
 // Activations and weights are BF16 (round to nearest even), products are
-// accumulated in FP32 by v_mfma_f32_16x16x16_bf16:
+// accumulated in FP32 by gfx950's v_mfma_f32_16x16x32_bf16:
 //
-//   C[co][voxel] = sum_k A[co][k] B[k][voxel],   k = (tap, ci)
+//   C[co][voxel] = sum_k A[co][k] B[k][voxel],   k = tap * CI + ci
 //
-// Operand lane layout of the 16x16x16 instruction (64 lanes, l = 16 q + n):
-// A: lane holds row n, k = 4q..4q+3;  B: lane holds column n, k = 4q..4q+3;
-// C: lane holds column n, rows 4q..4q+3.  With A = weights and B = input,
-// a lane's B operand is 4 consecutive channels of one voxel (one 8-byte
-// load) and its C fragment is 4 consecutive output channels of one voxel
-// (one 8-byte store).  Weights are pre-swizzled into A-operand order so each
-// lane reads its 4 values with one 8-byte load.
+// (k zero-padded to a multiple of 32).  Operand lane layout (64 lanes,
+// l = 16 q + n): A: lane holds row n, k = 8q..8q+7 of the 32-wide block;
+// B: lane holds column n, the same 8 k; C: lane holds column n, rows
+// 4q..4q+3.  With A = weights and B = input, a lane's B operand is 8
+// consecutive channels of one voxel and tap (one 16-byte LDS read; for
+// CI = 16 a block pairs two taps, lanes q < 2 reading the first), and its
+// C fragment 4 consecutive output channels of one voxel (one 8-byte store).
+// Weights are pre-swizzled into A-operand order (16 bytes per lane).
 //
 // Work split: a wavefront owns a 4x4x4 voxel cube; column n = (y%4) * 4 +
 // (x%4) and the four MFMA column tiles g = 0..3 are the cube's z slices.  The
 // 2x2x2 average pool of the output is then a sum over tile pairs (g, g+1)
 // plus the lane pairs n^1 (x) and n^4 (y), done in registers.  Four cubes
-// share one workgroup and one LDS copy of their input halo.
+// share one workgroup, one LDS copy of their input halo and one LDS copy of
+// the weights of the workgroup's output channels (loaded once: workgroups
+// are persistent and keep their channel group).
 //
-// Input channel counts 1 and 2 (first layer, backward of the last) fold the
-// taps into K: k = tap * CI + ci, padded with zero weights to 16 * KB.
+// Input channel counts 1 and 2 (first layer, backward of the last) fold
+// several taps into one 8-wide k slot: k = tap * CI + ci.
 #include "common.h"
+
+#include <type_traits>
 
 namespace wa {
 namespace {
@@ -32,6 +37,7 @@ namespace {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 mfma_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kBlock = 256;  // 4 wavefronts, one voxel cube each
@@ -47,52 +53,46 @@ __device__ __forceinline__ unsigned short f2bf(float f)
 
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
-__host__ __device__ constexpr int k_blocks(int ci) { return ci <= 2 ? (27 * ci + 15) / 16 : ci / 16; }
-__host__ __device__ constexpr int tap_loops(int ci) { return ci <= 2 ? 1 : 27; }
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// Two floats -> two BF16 (round to nearest even) in one v_cvt_pk_bf16_f32;
+// lo in the low half.
+__device__ __forceinline__ unsigned pack_bf16x2(float lo, float hi)
+{
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(f32x2{lo, hi}, bf16x2));
+}
+
+// 32-wide k blocks of a CI-channel input (27 taps, zero-padded)
+__host__ __device__ constexpr int k_blocks(int ci) { return (27 * ci + 31) / 32; }
 __host__ __device__ constexpr int c_blocks(int co) { return (co + 15) / 16; }
 
-// A-operand tiles: Wsw[((t * KB + kb) * CB + cb) * 64 + lane] = 4 BF16 of
-// W(tap, ci, co) for co = 16 cb + lane % 16, k = 16 kb + 4 (lane / 16) + j.
+// A-operand tiles: Wsw[(kb * CB + cb) * 64 + lane] = 8 BF16 of W(tap, c, co)
+// for co = 16 cb + lane % 16, k = 32 kb + 8 (lane / 16) + j (j = 0..7),
+// (tap, c) = (k / ci, k % ci); zero where tap >= 27 or co >= cout.
 // transpose_flip: use W'(tap, ci, co) = W[26 - tap][co][ci] with W stored
 // for a (co -> ci) layer, i.e. the backward-data convolution.
-__global__ void swizzle_kernel(int ci, int co, int transpose_flip, const float* __restrict__ W, bf16x4* __restrict__ out,
+__global__ void swizzle_kernel(int ci, int co, int transpose_flip, const float* __restrict__ W, bf16x8* __restrict__ out,
                                int total)
 {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= total)
         return;
-    const int KB = k_blocks(ci), CB = c_blocks(co);
+    const int CB = c_blocks(co);
     const int lane = idx % 64;
     const int cb = (idx / 64) % CB;
-    const int kb = (idx / (64 * CB)) % KB;
-    const int t0 = idx / (64 * CB * KB);
+    const int kb = idx / (64 * CB);
     const int o = cb * 16 + lane % 16;
-    bf16x4 v;
-    for (int j = 0; j < 4; ++j) {
-        const int k = kb * 16 + 4 * (lane / 16) + j;
-        int tap, c;
-        if (ci <= 2) {
-            tap = k / ci;
-            c = k % ci;
-        } else {
-            tap = t0;
-            c = k;
-        }
+    bf16x8 v;
+    for (int j = 0; j < 8; ++j) {
+        const int k = kb * 32 + 8 * (lane / 16) + j;
+        const int tap = k / ci, c = k % ci;
         float w = 0.0f;
-        if (o < co && tap < 27 && c < ci)
+        if (o < co && tap < 27)
             w = transpose_flip ? W[((size_t)(26 - tap) * co + o) * ci + c] : W[((size_t)tap * ci + c) * co + o];
         v[j] = (short)f2bf(w);
     }
     out[idx] = v;
 }
-
-__device__ __forceinline__ bf16x4 zero4()
-{
-    bf16x4 z = {0, 0, 0, 0};
-    return z;
-}
-
-__device__ __forceinline__ bf16x4 load4(const unsigned short* p) { return *reinterpret_cast<const bf16x4*>(p); }
 
 __device__ __forceinline__ bf16x8 zero8()
 {
@@ -104,178 +104,217 @@ __device__ __forceinline__ bf16x8 load8(const unsigned short* p) { return *reint
 
 __device__ __forceinline__ bf16x8 add8(bf16x8 a, bf16x8 b)
 {
-    bf16x8 r;
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-        r[j] = (short)f2bf(bf2f((unsigned short)a[j]) + bf2f((unsigned short)b[j]));
-    return r;
-}
-
-__device__ __forceinline__ bf16x4 add4(bf16x4 a, bf16x4 b)
-{
-    bf16x4 r;
+    unsigned r[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-        r[j] = (short)f2bf(bf2f((unsigned short)a[j]) + bf2f((unsigned short)b[j]));
-    return r;
+        r[j] = pack_bf16x2(bf2f((unsigned short)a[2 * j]) + bf2f((unsigned short)b[2 * j]),
+                           bf2f((unsigned short)a[2 * j + 1]) + bf2f((unsigned short)b[2 * j + 1]));
+    return __builtin_bit_cast(bf16x8, r);
 }
 
 enum InMode { kInPlain = 0, kInUpAdd = 1 };
 enum OutFlags { kOutRelu = 1, kOutPool = 2, kOutF32 = 4, kOutAccumulate = 8 };
 
-// Workgroup tile: 8 x 8 x 4 voxels = four 4x4x4 cubes (wave w: cube (w & 1,
-// w >> 1) in x, y).  Its 10 x 10 x 6 halo is staged once in LDS (the
-// upsample + skip add applied there, once per voxel), then every wave reads
-// its B operands from LDS for all 27 taps.
 // Persistent tile loops: workgroup b is dispatched to XCD b % kXcds (the
 // MI355X command processor deals workgroups to its 8 XCDs round-robin), so
 // each XCD walks its own contiguous chunk of tiles: tiles that share halo
 // voxels then run on the same XCD and meet in its L2.  Any other mapping
-// only costs locality, never correctness.
+// only costs locality, never correctness.  (b, nb): this workgroup's index
+// and the number of workgroups walking the same tiles.
 constexpr int kXcds = 8;
 
 struct TileWalk {
     int t, end, step;
-    __device__ __forceinline__ explicit TileWalk(int ntiles)
+    __device__ __forceinline__ TileWalk(int ntiles, int b, int nb)
     {
-        if (gridDim.x >= kXcds && gridDim.x % kXcds == 0) {
-            const int xcd = blockIdx.x % kXcds;
+        if (nb >= kXcds && nb % kXcds == 0) {
+            const int xcd = b % kXcds;
             const int chunk = (ntiles + kXcds - 1) / kXcds;
-            t = xcd * chunk + blockIdx.x / kXcds;
+            t = xcd * chunk + b / kXcds;
             end = min(ntiles, (xcd + 1) * chunk);
-            step = gridDim.x / kXcds;
+            step = nb / kXcds;
         } else {
-            t = blockIdx.x;
+            t = b;
             end = ntiles;
-            step = gridDim.x;
+            step = nb;
         }
     }
+    __device__ __forceinline__ explicit TileWalk(int ntiles) : TileWalk(ntiles, (int)blockIdx.x, (int)gridDim.x) {}
 };
 
+// Workgroup tile: 8 x 8 x 4 voxels = four 4x4x4 cubes (wave w: cube (w & 1,
+// w >> 1) in x, y) and its 10 x 10 x 6 input halo.
 constexpr int kTX = 8, kTY = 8, kTZ = 4;
 constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2;
 constexpr int kHalo = kHX * kHY * kHZ;
 
-// LDS words (4 bytes) per halo voxel: CI channels of BF16, plus 4 words of
-// padding for CI >= 16 to spread the 8-byte reads over the banks (a
-// multiple of 4 words keeps the 16-byte staging stores aligned).
-__host__ __device__ constexpr int lds_stride_words(int ci) { return ci <= 2 ? 1 : ci / 2 + 4; }
+// Bytes per halo voxel of the MFMA kernel: CI BF16 channels, plus 32 bytes of
+// padding for CI = 64 (spreads the 16-byte operand reads of a wavefront's 16
+// voxels over the banks); one 4-byte word for CI <= 2.  Measured at 128^3
+// (us, padding 0 / 16 / 32): 16 -> 32 at 64^3 17.9 / 19.4 / 18.6, 32 -> 16 at
+// 64^3 19.9 / 22.1 / 26.0, 64 -> 64 at 16^3 17.2 / 15.6 / 14.7, 64 -> 32 at
+// 32^3 19.2 / 17.9 / 17.1.
+__host__ __device__ constexpr int halo_bytes(int ci) { return ci <= 2 ? 4 : 2 * ci + (ci >= 64 ? 32 : 0); }
 
-// Stage the 10 x 10 x 6 input halo of the tile at (X0, Y0, Z0) in LDS
-// (zeros outside the grid; with kInUpAdd the input is x + upsample(coarse),
-// rounded to BF16 once per voxel).
-template <int CI, int IN>
-__device__ __forceinline__ void stage_tile(unsigned* tile, int m, int X0, int Y0, int Z0,
-                                           const unsigned short* __restrict__ x,
-                                           const unsigned short* __restrict__ coarse)
-{
-    constexpr int SW = lds_stride_words(CI);
-    const int mh = m / 2;
-    // All loads of a batch are issued before any is used: the halo is ~10
-    // loads per thread, and issuing them one round trip at a time made the
-    // staging, not the MFMAs, the cost of the layer.
-    if constexpr (CI <= 2) {
-        constexpr int ITERS = (kHalo + kBlock - 1) / kBlock;
-        unsigned v[ITERS];
+// Halo voxel offset of tap t relative to the window corner (dx, dy, dz = -1)
+__host__ __device__ constexpr int tap_offset(int t) { return ((t / 9) * kHY + (t / 3) % 3) * kHX + t % 3; }
+
+// LDS layout of conv3_mfma_kernel: [weights of CBG channel blocks | halo | 16 zero bytes]
+template <int CI, int CBG> struct ConvLds {
+    static constexpr int kWeights = k_blocks(CI) * CBG * 1024;
+    static constexpr int kHaloOff = kWeights;
+    static constexpr int kZeroOff = kHaloOff + kHalo * halo_bytes(CI);
+    static constexpr int kBytes = kZeroOff + 16;
+};
+
+// The 10 x 10 x 6 input halo of the tile at (X0, Y0, Z0), held in registers
+// between its loads and its LDS stores, so that a workgroup's next tile is
+// loaded while the MFMAs of the current one run.  Every load of a tile is
+// issued before the first is used (at most 19 per thread); coordinates
+// outside the grid load a clamped in-range voxel and store zeros (with
+// kInUpAdd the input is x + upsample(coarse), rounded to BF16 once per voxel).
+template <int CI, int IN> struct HaloStage {
+    static constexpr int QPV = CI <= 2 ? 1 : CI / 8;  // 16-byte channel octets (or one word) per voxel
+    static constexpr int TOTAL = kHalo * QPV;
+    static constexpr int ITERS = (TOTAL + kBlock - 1) / kBlock;
+    using T = typename std::conditional<(CI <= 2), unsigned, bf16x8>::type;
+    T v[ITERS];
+    T cv[IN == kInUpAdd ? ITERS : 1];
+    bool in[ITERS];
+
+    __device__ __forceinline__ void load(int m, int X0, int Y0, int Z0, const unsigned short* __restrict__ x,
+                                         const unsigned short* __restrict__ coarse)
+    {
+        const int mh = m / 2;
 #pragma unroll
         for (int u = 0; u < ITERS; ++u) {
-            const int hv = u * kBlock + threadIdx.x;
+            const int c = min(u * kBlock + (int)threadIdx.x, TOTAL - 1);
+            const int hv = c / QPV, od = c % QPV;
             const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
             const int gx = X0 - 1 + hx, gy = Y0 - 1 + hy, gz = Z0 - 1 + hz;
-            v[u] = 0;
-            if (hv < kHalo && gx >= 0 && gx < m && gy >= 0 && gy < m && gz >= 0 && gz < m) {
-                const size_t g = (size_t)(gz * m + gy) * m + gx;
+            in[u] = gx >= 0 && gx < m && gy >= 0 && gy < m && gz >= 0 && gz < m;
+            const int cx = min(max(gx, 0), m - 1), cy = min(max(gy, 0), m - 1), cz = min(max(gz, 0), m - 1);
+            const size_t g = (size_t)(cz * m + cy) * m + cx;
+            if constexpr (CI <= 2) {
                 v[u] = CI == 2 ? reinterpret_cast<const unsigned*>(x)[g] : (unsigned)x[g];
+            } else {
+                v[u] = load8(x + g * CI + 8 * od);
+                if constexpr (IN == kInUpAdd)
+                    cv[u] = load8(coarse + ((size_t)((cz >> 1) * mh + (cy >> 1)) * mh + (cx >> 1)) * CI + 8 * od);
             }
         }
+    }
+
+    __device__ __forceinline__ void store(unsigned char* halo) const
+    {
 #pragma unroll
         for (int u = 0; u < ITERS; ++u) {
-            const int hv = u * kBlock + threadIdx.x;
-            if (hv < kHalo)
-                tile[hv] = v[u];
-        }
-    } else {
-        constexpr int QPV = CI / 8;  // 16-byte channel octets per voxel
-        constexpr int TOTAL = kHalo * QPV;
-        constexpr int ITERS = (TOTAL + kBlock - 1) / kBlock;
-        constexpr int BATCH = ITERS < 4 ? ITERS : 4;
-        for (int base = 0; base < ITERS; base += BATCH) {
-            bf16x8 v[BATCH];
-            bf16x8 cv[BATCH];
-#pragma unroll
-            for (int u = 0; u < BATCH; ++u) {
-                const int c = (base + u) * kBlock + threadIdx.x;
-                const int hv = c / QPV, od = c % QPV;
-                const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
-                const int gx = X0 - 1 + hx, gy = Y0 - 1 + hy, gz = Z0 - 1 + hz;
-                v[u] = zero8();
-                cv[u] = zero8();
-                if (base + u < ITERS && c < TOTAL && gx >= 0 && gx < m && gy >= 0 && gy < m && gz >= 0 && gz < m) {
-                    v[u] = load8(x + ((size_t)(gz * m + gy) * m + gx) * CI + 8 * od);
-                    if constexpr (IN == kInUpAdd)
-                        cv[u] = load8(coarse + ((size_t)((gz >> 1) * mh + (gy >> 1)) * mh + (gx >> 1)) * CI + 8 * od);
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < BATCH; ++u) {
-                const int c = (base + u) * kBlock + threadIdx.x;
-                if (base + u < ITERS && c < TOTAL) {
+            const int c = u * kBlock + threadIdx.x;
+            if (c < TOTAL) {
+                if constexpr (CI <= 2) {
+                    reinterpret_cast<unsigned*>(halo)[c] = in[u] ? v[u] : 0u;
+                } else {
                     bf16x8 w = v[u];
                     if constexpr (IN == kInUpAdd)
                         w = add8(w, cv[u]);
-                    *reinterpret_cast<bf16x8*>(&tile[(c / QPV) * SW + 4 * (c % QPV)]) = w;
+                    *reinterpret_cast<bf16x8*>(halo + (c / QPV) * halo_bytes(CI) + 16 * (c % QPV)) =
+                        in[u] ? w : zero8();
                 }
             }
         }
     }
-}
+};
 
 // x: input [m^3][CI] BF16; with kInUpAdd the input is x[v] + coarse[v / 2]
 // (coarse: [(m/2)^3][CI]), rounded to BF16.  Output: y BF16 [m^3][CO] or, with
 // kOutF32, yf FP32 [m^3][CO] (kOutAccumulate adds to yf); pooled BF16
 // [(m/2)^3][CO] with kOutPool.
-// CBG of the CO / 16 output-channel blocks per workgroup (blockIdx.z picks
-// which), so that coarse grids still launch enough workgroups.
+// A workgroup computes CBG of the CO / 16 output-channel blocks: channel
+// group blockIdx.x % (CB / CBG), whose weights it holds in LDS for all of its
+// tiles; the groups split the chip so that coarse grids still launch enough
+// workgroups.  The k loop is fully unrolled: every LDS address is a
+// compile-time offset from one per-lane base, so the compiler can issue the
+// operand reads of later blocks while the MFMAs of earlier ones run.
 template <int CI, int CO, int IN, int CBG>
 __global__ __launch_bounds__(kBlock) void conv3_mfma_kernel(
     int m, const unsigned short* __restrict__ x, const unsigned short* __restrict__ coarse,
-    const bf16x4* __restrict__ wsw, const float* __restrict__ bias, int flags, unsigned short* __restrict__ y,
+    const bf16x8* __restrict__ wsw, const float* __restrict__ bias, int flags, unsigned short* __restrict__ y,
     float* __restrict__ yf, unsigned short* __restrict__ pooled
 )
 {
     constexpr int KB = k_blocks(CI);
     constexpr int CB = c_blocks(CO);
-    constexpr int TL = tap_loops(CI);
-    constexpr int SW = lds_stride_words(CI);
+    constexpr int HB = halo_bytes(CI);
+    constexpr int SLICE = kHX * kHY;  // halo voxels per z slice
+    using L = ConvLds<CI, CBG>;
     static_assert(CB % CBG == 0, "channel blocks per workgroup must divide the channel blocks");
-    __shared__ unsigned tile[kHalo * SW];
+    static_assert(L::kBytes <= 160 * 1024, "LDS budget");
+    __shared__ __attribute__((aligned(16))) unsigned char smem[L::kBytes];
+    unsigned char* halo = smem + L::kHaloOff;
+
+    constexpr int nsplit = CB / CBG;
+    const int group = (int)blockIdx.x % nsplit;
+    const int cb0 = group * CBG;
+    // the workgroup's weights, once: [kb][cbl][lane] 16 bytes each, every
+    // load issued before the first store
+    {
+        constexpr int kW = KB * CBG * 64;
+        constexpr int kWIters = (kW + kBlock - 1) / kBlock;
+        bf16x8 wv[kWIters];
+#pragma unroll
+        for (int u = 0; u < kWIters; ++u) {
+            const int i = min(u * kBlock + (int)threadIdx.x, kW - 1);
+            const int ln = i % 64, cbl = (i / 64) % CBG, kb = i / (64 * CBG);
+            wv[u] = wsw[(kb * CB + cb0 + cbl) * 64 + ln];
+        }
+#pragma unroll
+        for (int u = 0; u < kWIters; ++u) {
+            const int i = u * kBlock + threadIdx.x;
+            if (i < kW)
+                reinterpret_cast<bf16x8*>(smem)[i] = wv[u];
+        }
+    }
+    if (threadIdx.x == 0)
+        *reinterpret_cast<bf16x8*>(smem + L::kZeroOff) = zero8();
 
     const int tiles_x = (m + kTX - 1) / kTX;
     const int tiles_xy = tiles_x * tiles_x;
-    const int nsplit = CB / CBG;
-    const int ntiles = tiles_xy * (m / kTZ) * nsplit;
+    const int ntiles = tiles_xy * (m / kTZ);
     const int mh = m / 2;
-    // persistent loop over tiles; the channel split varies fastest so that
-    // concurrent workgroups share one input tile in L2
-    for (TileWalk walk(ntiles); walk.t < walk.end; walk.t += walk.step) {
-    const int tile_id = walk.t;
-    const int cb0 = (tile_id % nsplit) * CBG;
-    const int t2 = tile_id / nsplit;
-    const int X0 = (t2 % tiles_x) * kTX;
-    const int Y0 = ((t2 / tiles_x) % tiles_x) * kTY;
-    const int Z0 = (t2 / tiles_xy) * kTZ;
-
-    __syncthreads();  // the previous tile's LDS reads are done
-    stage_tile<CI, IN>(tile, m, X0, Y0, Z0, x, coarse);
-    __syncthreads();
-
     const int lane = threadIdx.x % 64;
     const int w = threadIdx.x / 64;
     const int n = lane % 16, q = lane / 16;
     const int lx = 4 * (w & 1) + (n & 3);  // position in the tile
     const int ly = 4 * (w >> 1) + (n >> 2);
+    const int lane_base = ly * kHX + lx;  // halo voxel of the window corner in slice 0
+    const bf16x8* wl = reinterpret_cast<const bf16x8*>(smem) + lane;
+
+    auto origin = [&](int t, int& X0, int& Y0, int& Z0) {
+        X0 = (t % tiles_x) * kTX;
+        Y0 = ((t / tiles_x) % tiles_x) * kTY;
+        Z0 = (t / tiles_xy) * kTZ;
+    };
+    TileWalk walk(ntiles, (int)blockIdx.x / nsplit, (int)gridDim.x / nsplit);
+    HaloStage<CI, IN> stage;
+    if (walk.t < walk.end) {
+        int X0, Y0, Z0;
+        origin(walk.t, X0, Y0, Z0);
+        stage.load(m, X0, Y0, Z0, x, coarse);
+    }
+    for (; walk.t < walk.end; walk.t += walk.step) {
+    int X0, Y0, Z0;
+    origin(walk.t, X0, Y0, Z0);
     const int vx = X0 + lx, vy = Y0 + ly, vz0 = Z0;
     const bool valid = (X0 + 4 * (w & 1)) < m && (Y0 + 4 * (w >> 1)) < m;  // whole cube in range
+
+    __syncthreads();  // weights stored / the previous tile's LDS reads are done
+    stage.store(halo);
+    __syncthreads();
+    if (walk.t + walk.step < walk.end) {  // the next tile's halo, in flight during the MFMAs
+        int X1, Y1, Z1;
+        origin(walk.t + walk.step, X1, Y1, Z1);
+        stage.load(m, X1, Y1, Z1, x, coarse);
+    }
 
     f32x4 acc[CBG][4];
 #pragma unroll
@@ -283,63 +322,50 @@ __global__ __launch_bounds__(kBlock) void conv3_mfma_kernel(
 #pragma unroll
         for (int g = 0; g < 4; ++g)
             acc[cb][g] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // CI <= 2: the halo offset of every tap this lane reads (-1: padding tap >= 27)
-    constexpr int NT = CI == 2 ? 2 : 4;  // taps per lane and k-block
-    int toff[CI <= 2 ? KB : 1][NT];
-    if constexpr (CI <= 2) {
-#pragma unroll
-        for (int kb = 0; kb < KB; ++kb)
-#pragma unroll
-            for (int i = 0; i < NT; ++i) {
-                const int tap = CI == 2 ? kb * 8 + 2 * q + i : kb * 16 + 4 * q + i;
-                toff[kb][i] = tap < 27 ? ((tap / 9) * kHY + ly + (tap / 3) % 3) * kHX + lx + tap % 3 : -1;
-            }
-    }
 
-    for (int t = 0; t < TL; ++t) {
-        const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-        // halo coordinates of this lane's neighbour for slice g: (lx+1+dx, ly+1+dy, g+1+dz)
-        const int hbase = ((1 + dz) * kHY + (ly + 1 + dy)) * kHX + (lx + 1 + dx);
 #pragma unroll
-        for (int kb = 0; kb < KB; ++kb) {
-            bf16x4 a[CBG];
+    for (int kb = 0; kb < KB; ++kb) {
+        mfma_bf16x8 a[CBG];
 #pragma unroll
-            for (int cb = 0; cb < CBG; ++cb)
-                a[cb] = wsw[((t * KB + kb) * CB + cb0 + cb) * 64 + lane];
-            bf16x4 b[4];
+        for (int cb = 0; cb < CBG; ++cb)
+            a[cb] = __builtin_bit_cast(mfma_bf16x8, wl[(kb * CBG + cb) * 64]);
+        mfma_bf16x8 b[4];
+        if constexpr (CI >= 16) {
+            // k = 32 kb + 8 q .. +7: one tap, 8 consecutive channels (lane-dependent tap only for CI = 16)
+            const int k0 = 32 * kb + 8 * q;
+            const int tap = k0 / CI, c0 = k0 % CI;
+            const bool pad = tap >= 27;
+            const unsigned char* src = pad ? smem + L::kZeroOff : halo + (lane_base + tap_offset(tap)) * HB + 2 * c0;
+            const int gstep = pad ? 0 : SLICE * HB;
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+                b[g] = __builtin_bit_cast(mfma_bf16x8, *reinterpret_cast<const bf16x8*>(src + g * gstep));
+        } else {
+            // k = 32 kb + 8 q + j: tap (k / CI), channel k % CI; one 4-byte halo word per voxel
+            constexpr int TPS = 8 / CI;  // taps per 8-wide slot
+            const unsigned* hw = reinterpret_cast<const unsigned*>(halo);
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
-                if constexpr (CI <= 2) {
-                    // k = 16 kb + 4 q + j -> (tap, channel) = (k / CI, k % CI); halo
-                    // offsets of the taps precomputed in toff
-                    bf16x4 v;
-                    const int gz = g * (kHX * kHY);
+                bf16x8 v;
+#pragma unroll
+                for (int i = 0; i < TPS; ++i) {
+                    const int tap = (32 * kb + 8 * q) / CI + i;
+                    const unsigned u = tap < 27 ? hw[lane_base + tap_offset(tap) + g * SLICE] : 0u;
                     if constexpr (CI == 2) {
-#pragma unroll
-                        for (int i = 0; i < 2; ++i) {
-                            const unsigned u = toff[kb][i] >= 0 ? tile[gz + toff[kb][i]] : 0u;
-                            v[2 * i] = (short)(u & 0xffffu);  // channels 0, 1 of one tap
-                            v[2 * i + 1] = (short)(u >> 16);
-                        }
+                        v[2 * i] = (short)(u & 0xffffu);
+                        v[2 * i + 1] = (short)(u >> 16);
                     } else {
-#pragma unroll
-                        for (int i = 0; i < 4; ++i) {
-                            const unsigned u = toff[kb][i] >= 0 ? tile[gz + toff[kb][i]] : 0u;
-                            v[i] = (short)(u & 0xffffu);
-                        }
+                        v[i] = (short)(u & 0xffffu);
                     }
-                    b[g] = v;
-                } else {
-                    const int hv = hbase + g * (kHX * kHY);
-                    b[g] = *reinterpret_cast<const bf16x4*>(&tile[hv * SW + 8 * kb + 2 * q]);
                 }
+                b[g] = __builtin_bit_cast(mfma_bf16x8, v);
             }
-#pragma unroll
-            for (int cb = 0; cb < CBG; ++cb)
-#pragma unroll
-                for (int g = 0; g < 4; ++g)
-                    acc[cb][g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[cb], b[g], acc[cb][g], 0, 0, 0);
         }
+#pragma unroll
+        for (int cb = 0; cb < CBG; ++cb)
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+                acc[cb][g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[cb], b[g], acc[cb][g], 0, 0, 0);
     }
     if (!valid)
         continue;  // whole wavefront (only when m < 8); still joins the next tile's barriers
@@ -372,14 +398,12 @@ __global__ __launch_bounds__(kBlock) void conv3_mfma_kernel(
                         *d = (flags & kOutAccumulate) ? *d + val[i] : val[i];
                     }
             } else {
-                bf16x4 st;
+                const unsigned st[2] = {pack_bf16x2(val[0], val[1]), pack_bf16x2(val[2], val[3])};
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    st[i] = (short)f2bf(val[i]);
-                    val[i] = bf2f((unsigned short)st[i]);  // pool what was stored
-                }
+                for (int i = 0; i < 4; ++i)  // pool what was stored
+                    val[i] = bf2f((unsigned short)(st[i / 2] >> (16 * (i % 2))));
                 if constexpr (CO % 16 == 0)
-                    *reinterpret_cast<bf16x4*>(y + v * CO + o0) = st;
+                    *reinterpret_cast<bf16x4*>(y + v * CO + o0) = __builtin_bit_cast(bf16x4, st);
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i)
@@ -388,18 +412,19 @@ __global__ __launch_bounds__(kBlock) void conv3_mfma_kernel(
         if (flags & kOutPool) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                bf16x4 st;
+                float sv[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    float sv = psum[h][i];
-                    sv += __shfl_xor(sv, 1);
-                    sv += __shfl_xor(sv, 4);
-                    st[i] = (short)f2bf(sv * 0.125f);
+                    sv[i] = psum[h][i];
+                    sv[i] += __shfl_xor(sv[i], 1);
+                    sv[i] += __shfl_xor(sv[i], 4);
                 }
+                const unsigned st[2] = {pack_bf16x2(sv[0] * 0.125f, sv[1] * 0.125f),
+                                        pack_bf16x2(sv[2] * 0.125f, sv[3] * 0.125f)};
                 if ((n & 1) == 0 && (n & 4) == 0) {
                     const size_t pv = (size_t)(((vz0 >> 1) + h) * mh + (vy >> 1)) * mh + (vx >> 1);
                     if constexpr (CO % 16 == 0)
-                        *reinterpret_cast<bf16x4*>(pooled + pv * CO + o0) = st;
+                        *reinterpret_cast<bf16x4*>(pooled + pv * CO + o0) = __builtin_bit_cast(bf16x4, st);
                 }
             }
         }
@@ -423,144 +448,129 @@ void launch_tiles(int ntiles, hipStream_t s, Args... args)
     Kernel<<<grid, kBlock, 0, s>>>(args...);
 }
 
-// Halo (HX x HY x HZ voxels) of one tile held in registers between its loads
-// and its LDS stores (CI >= 16: 16-byte channel octets, CI / 8 per voxel).
-template <int CI, int IN, int HX = kHX, int HY = kHY, int HZ = kHZ> struct HaloRegs {
-    static constexpr int QPV = CI / 8;
-    static constexpr int TOTAL = HX * HY * HZ * QPV;
-    static constexpr int ITERS = (TOTAL + kBlock - 1) / kBlock;
-    bf16x8 v[ITERS];
-    bf16x8 cv[IN == kInUpAdd ? ITERS : 1];
+// One output channel, streamed through z: a workgroup owns 16 x 16 columns
+// (one per thread) and kCo1Z consecutive output slices, and walks the input
+// planes z' = Z0 - 1 .. Z0 + kCo1Z one at a time.  Only one 18 x 18 plane
+// (its halo included) sits in LDS, double-buffered: the next plane's loads
+// are in flight while the current one is used.  A plane contributes to the
+// outputs z' + 1, z' and z' - 1 (taps dz = 0, 1, 2), accumulated in three
+// rolling registers, so every output still adds its taps in the order
+// t = 0..26, each tap's channels by v_dot2_f32_bf16 in pairs: the same
+// instruction sequence, hence the same bits, as one voxel per thread.  A
+// thread reads 9 halo voxels per plane: 9 reads per output against 27.
+constexpr int kCo1T = 16;             // columns per side of a workgroup
+constexpr int kCo1H = kCo1T + 2;      // halo plane side
+constexpr int kCo1Z = 8;              // output slices per workgroup (128^3: 8 -> 39.3 us, 16 -> 43.2, 32 -> 59.4)
+constexpr int kCo1VB = 48;            // LDS bytes per voxel: 16 BF16 channels + 16 bytes of padding
 
-    __device__ __forceinline__ void load(int m, int X0, int Y0, int Z0, const unsigned short* __restrict__ x,
-                                         const unsigned short* __restrict__ coarse)
-    {
-        const int mh = m / 2;
+template <int IN>
+__global__ __launch_bounds__(kBlock) void conv3_co1_stream_kernel(int m, const unsigned short* __restrict__ x,
+                                                                 const unsigned short* __restrict__ coarse,
+                                                                 const unsigned* __restrict__ wpk,
+                                                                 const float* __restrict__ bias, int flags,
+                                                                 float* __restrict__ yf)
+{
+    constexpr int CI = 16;
+    constexpr int kPlane = kCo1H * kCo1H;                    // 324 voxels
+    constexpr int kOct = kPlane * 2;                         // 16-byte octets per plane
+    constexpr int kIters = (kOct + kBlock - 1) / kBlock;     // 3 loads per thread and plane
+    __shared__ __attribute__((aligned(16))) unsigned char plane[2][kPlane * kCo1VB];
+    const int tiles_x = (m + kCo1T - 1) / kCo1T;
+    const int tiles_z = (m + kCo1Z - 1) / kCo1Z;
+    const int ntiles = tiles_x * tiles_x * tiles_z;
+    const int mh = m / 2;
+    const int tx = threadIdx.x % kCo1T, ty = threadIdx.x / kCo1T;
+    const float b0 = bias ? bias[0] : 0.0f;
+
+    for (TileWalk walk(ntiles); walk.t < walk.end; walk.t += walk.step) {
+    const int X0 = (walk.t % tiles_x) * kCo1T;
+    const int Y0 = ((walk.t / tiles_x) % tiles_x) * kCo1T;
+    const int Z0 = (walk.t / (tiles_x * tiles_x)) * kCo1Z;
+    const int nz = min(kCo1Z, m - Z0);
+    const int np = nz + 2;  // planes z' = Z0 - 1 .. Z0 + nz
+
+    bf16x8 v[kIters], cv[IN == kInUpAdd ? kIters : 1];
+    bool in[kIters];
+    auto load_plane = [&](int p) {
+        const int gz = Z0 - 1 + p;
+        const int cz = min(max(gz, 0), m - 1);
 #pragma unroll
-        for (int u = 0; u < ITERS; ++u) {
-            const int c = u * kBlock + threadIdx.x;
-            const int hv = c / QPV, od = c % QPV;
-            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
-            const int gx = X0 - 1 + hx, gy = Y0 - 1 + hy, gz = Z0 - 1 + hz;
-            v[u] = zero8();
+        for (int u = 0; u < kIters; ++u) {
+            const int c = min(u * kBlock + (int)threadIdx.x, kOct - 1);
+            const int hv = c / 2, od = c % 2;
+            const int gx = X0 - 1 + hv % kCo1H, gy = Y0 - 1 + hv / kCo1H;
+            in[u] = gx >= 0 && gx < m && gy >= 0 && gy < m && gz >= 0 && gz < m;
+            const int cx = min(max(gx, 0), m - 1), cy = min(max(gy, 0), m - 1);
+            v[u] = load8(x + ((size_t)(cz * m + cy) * m + cx) * CI + 8 * od);
             if constexpr (IN == kInUpAdd)
-                cv[u] = zero8();
-            if (c < TOTAL && gx >= 0 && gx < m && gy >= 0 && gy < m && gz >= 0 && gz < m) {
-                v[u] = load8(x + ((size_t)(gz * m + gy) * m + gx) * CI + 8 * od);
-                if constexpr (IN == kInUpAdd)
-                    cv[u] = load8(coarse + ((size_t)((gz >> 1) * mh + (gy >> 1)) * mh + (gx >> 1)) * CI + 8 * od);
-            }
+                cv[u] = load8(coarse + ((size_t)((cz >> 1) * mh + (cy >> 1)) * mh + (cx >> 1)) * CI + 8 * od);
         }
-    }
-
-    __device__ __forceinline__ void store(unsigned* tile) const
-    {
-        constexpr int SW = lds_stride_words(CI);
+    };
+    auto store_plane = [&](unsigned char* dst) {
 #pragma unroll
-        for (int u = 0; u < ITERS; ++u) {
+        for (int u = 0; u < kIters; ++u) {
             const int c = u * kBlock + threadIdx.x;
-            if (c < TOTAL) {
+            if (c < kOct) {
                 bf16x8 w = v[u];
                 if constexpr (IN == kInUpAdd)
                     w = add8(w, cv[u]);
-                *reinterpret_cast<bf16x8*>(&tile[(c / QPV) * SW + 4 * (c % QPV)]) = w;
+                *reinterpret_cast<bf16x8*>(dst + (c / 2) * kCo1VB + 16 * (c % 2)) = in[u] ? w : zero8();
             }
         }
-    }
-};
-
-// One output channel (the network's last layer): MFMA would waste 15 of its
-// 16 rows, so each thread computes voxels with packed BF16 dot products,
-// v_dot2_f32_bf16, against weights wpk[tap][ci / 2] (BF16 pairs), reading the
-// LDS-staged halo; every voxel adds its taps in the order t = 0..26.  Output
-// FP32; flags: ReLU, accumulate.  Tiles are 8 x 8 x 8 with two z-adjacent
-// voxels per thread: their 3 x 3 x 4 halo voxels are read from LDS once for
-// both (36 reads instead of 54) and a tile's halo is 1.95x its voxels (2.34x
-// for 8 x 8 x 4 tiles of one voxel per thread: 64.0 us against 62.6 at
-// 128^3).  The halo of the workgroup's next tile is loaded into registers
-// while the dot products of the current one run.
-constexpr int kTZ2 = 8;
-constexpr int kHZ2 = kTZ2 + 2;
-
-template <int CI, int IN>
-__global__ __launch_bounds__(kBlock) void conv3_co1_z2_kernel(int m, const unsigned short* __restrict__ x,
-                                                             const unsigned short* __restrict__ coarse,
-                                                             const unsigned* __restrict__ wpk,
-                                                             const float* __restrict__ bias, int flags,
-                                                             float* __restrict__ yf)
-{
-    constexpr int SW = lds_stride_words(CI);
-    static_assert(CI >= 16 && CI % 16 == 0, "one-output kernel expects a multiple of 16 input channels");
-    __shared__ unsigned tile[kHX * kHY * kHZ2 * SW];
-    const int tiles_x = (m + kTX - 1) / kTX;
-    const int tiles_z = (m + kTZ2 - 1) / kTZ2;
-    const int ntiles = tiles_x * tiles_x * tiles_z;
-    TileWalk walk(ntiles);
-    auto origin = [&](int t, int& X0, int& Y0, int& Z0) {
-        X0 = (t % tiles_x) * kTX;
-        Y0 = ((t / tiles_x) % tiles_x) * kTY;
-        Z0 = (t / (tiles_x * tiles_x)) * kTZ2;
     };
-    HaloRegs<CI, IN, kHX, kHY, kHZ2> halo;
-    if (walk.t < walk.end) {
-        int X0, Y0, Z0;
-        origin(walk.t, X0, Y0, Z0);
-        halo.load(m, X0, Y0, Z0, x, coarse);
-    }
-    const float b0 = bias ? bias[0] : 0.0f;
-    const int lx = threadIdx.x & 7, ly = (threadIdx.x >> 3) & 7, lz = threadIdx.x >> 6;  // 8 x 8 x 4 pairs
-    for (; walk.t < walk.end; walk.t += walk.step) {
-    int X0, Y0, Z0;
-    origin(walk.t, X0, Y0, Z0);
-    __syncthreads();  // the previous tile's LDS reads are done
-    halo.store(tile);
-    __syncthreads();
-    if (walk.t + walk.step < walk.end) {
-        int X1, Y1, Z1;
-        origin(walk.t + walk.step, X1, Y1, Z1);
-        halo.load(m, X1, Y1, Z1, x, coarse);
-    }
-    float acc0 = b0, acc1 = b0;
-#pragma unroll 1
-    for (int dz = 0; dz < 4; ++dz) {
+
+    // acc[0]: output z' - 1 (its last plane), acc[1]: z', acc[2]: z' + 1 (its first plane)
+    float acc[3] = {b0, b0, b0};
+    load_plane(0);
+    for (int p = 0; p < np; ++p) {
+        unsigned char* cur = plane[p & 1];
+        store_plane(cur);
+        __syncthreads();  // the plane is complete; everyone is done with the slot stored next
+        if (p + 1 < np)
+            load_plane(p + 1);
+        const unsigned* hw = reinterpret_cast<const unsigned*>(cur);
+        int wofs = 0;
 #pragma unroll
         for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
             for (int dx = 0; dx < 3; ++dx) {
-                const int hv = ((2 * lz + dz) * kHY + (ly + dy)) * kHX + (lx + dx);
-                const unsigned* src = &tile[hv * SW];
-                unsigned v[CI / 2];
+                // scalar-load the 24 weight words of (dy, dx) only after the
+                // previous group's products (cache hits): hoisted out of the
+                // loops, all 216 would not fit in SGPRs
+                asm volatile("" : "+s"(wofs) : "v"(acc[0]), "v"(acc[1]), "v"(acc[2]));
+                const unsigned* wp = wpk + wofs;
+                const int hv = (ty + dy) * kCo1H + tx + dx;
+                unsigned vv[CI / 2];
 #pragma unroll
                 for (int j = 0; j < CI / 2; ++j)
-                    v[j] = src[j];
-                if (dz < 3) {
-                    const unsigned* w = wpk + ((dz * 3 + dy) * 3 + dx) * (CI / 2);
+                    vv[j] = hw[hv * (kCo1VB / 4) + j];
+#pragma unroll
+                for (int dz = 0; dz < 3; ++dz) {
+                    // plane z' is tap dz of output z' + 1 - dz
+                    const unsigned* w = wp + ((dz * 3 + dy) * 3 + dx) * (CI / 2);
+                    float a = acc[2 - dz];
 #pragma unroll
                     for (int j = 0; j < CI / 2; ++j)
-                        acc0 = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, v[j]),
-                                                               __builtin_bit_cast(bf16x2, w[j]), acc0, false);
-                }
-                if (dz > 0) {
-                    const unsigned* w = wpk + (((dz - 1) * 3 + dy) * 3 + dx) * (CI / 2);
-#pragma unroll
-                    for (int j = 0; j < CI / 2; ++j)
-                        acc1 = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, v[j]),
-                                                               __builtin_bit_cast(bf16x2, w[j]), acc1, false);
+                        a = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, vv[j]),
+                                                            __builtin_bit_cast(bf16x2, w[j]), a, false);
+                    acc[2 - dz] = a;
                 }
             }
-    }
-    const int vx = X0 + lx, vy = Y0 + ly;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int vz = Z0 + 2 * lz + h;
-        float acc = h ? acc1 : acc0;
-        if (vx < m && vy < m && vz < m) {
+        // output z' - 1 has all 27 taps now (planes z' - 2, z' - 1, z')
+        const int zo = Z0 - 2 + p;
+        const int vx = X0 + tx, vy = Y0 + ty;
+        if (p >= 2 && vx < m && vy < m) {
+            float r = acc[0];
             if (flags & kOutRelu)
-                acc = fmaxf(acc, 0.0f);
-            float* d = yf + (size_t)(vz * m + vy) * m + vx;
-            *d = (flags & kOutAccumulate) ? *d + acc : acc;
+                r = fmaxf(r, 0.0f);
+            float* d = yf + (size_t)(zo * m + vy) * m + vx;
+            *d = (flags & kOutAccumulate) ? *d + r : r;
         }
+        acc[0] = acc[1];
+        acc[1] = acc[2];
+        acc[2] = b0;
     }
+    __syncthreads();  // the last plane's reads are done before the next tile stores
     }  // tiles
 }
 
@@ -581,43 +591,70 @@ template <int CI>
 int launch_co1(int m, int in_mode, const unsigned short* x, const unsigned short* coarse, const void* wpk,
                const float* bias, int flags, float* yf, hipStream_t s)
 {
-    const int txy = (m + kTX - 1) / kTX;
+    static_assert(CI == 16, "the streamed one-output kernel takes 16 input channels");
+    const int txy = (m + kCo1T - 1) / kCo1T;
     const unsigned* w = static_cast<const unsigned*>(wpk);
-    const int ntiles = txy * txy * ((m + kTZ2 - 1) / kTZ2);
+    const int ntiles = txy * txy * ((m + kCo1Z - 1) / kCo1Z);
     if (in_mode == kInUpAdd)
-        launch_tiles<conv3_co1_z2_kernel<CI, kInUpAdd>>(ntiles, s, m, x, coarse, w, bias, flags, yf);
+        launch_tiles<conv3_co1_stream_kernel<kInUpAdd>>(ntiles, s, m, x, coarse, w, bias, flags, yf);
     else
-        launch_tiles<conv3_co1_z2_kernel<CI, kInPlain>>(ntiles, s, m, x, coarse, w, bias, flags, yf);
+        launch_tiles<conv3_co1_stream_kernel<kInPlain>>(ntiles, s, m, x, coarse, w, bias, flags, yf);
     WA_LAUNCH_CHECK();
     return kOk;
+}
+
+template <int CI, int CO, int IN, int CBG>
+void launch_conv_groups(int m, int tiles, const unsigned short* x, const unsigned short* coarse, const bf16x8* w,
+                        const float* bias, int flags, unsigned short* y, float* yf, unsigned short* pooled,
+                        hipStream_t s)
+{
+    constexpr auto kernel = conv3_mfma_kernel<CI, CO, IN, CBG>;
+    constexpr int nsplit = c_blocks(CO) / CBG;
+    static const int per_cu = [] {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, kBlock, 0) != hipSuccess || nb < 1)
+            nb = 1;
+        return nb;
+    }();
+    // every channel group gets the same number of workgroups (each tile is
+    // computed exactly once per group, also with kOutAccumulate)
+    const int slots = per_cu * device_cu_count();
+    int grid = tiles * nsplit < slots ? tiles * nsplit : slots;
+    grid = grid / nsplit * nsplit;
+    if (grid < nsplit)
+        grid = nsplit;
+    kernel<<<(unsigned)grid, kBlock, 0, s>>>(m, x, coarse, w, bias, flags, y, yf, pooled);
+}
+
+template <int CI, int CO, int IN>
+void launch_conv_mode(int m, const unsigned short* x, const unsigned short* coarse, const bf16x8* w,
+                      const float* bias, int flags, unsigned short* y, float* yf, unsigned short* pooled, hipStream_t s)
+{
+    constexpr int CB = c_blocks(CO);
+    const int txy = (m + kTX - 1) / kTX;
+    const int tiles = txy * txy * (m / kTZ);
+    // all channel blocks in one workgroup when the spatial tiles alone fill
+    // the chip and their weights fit in LDS beside the halo; else one block each
+    constexpr bool kCanGroup = CB > 1 && ConvLds<CI, CB>::kBytes <= 160 * 1024;
+    if constexpr (kCanGroup) {
+        if (tiles >= 1024) {
+            launch_conv_groups<CI, CO, IN, CB>(m, tiles, x, coarse, w, bias, flags, y, yf, pooled, s);
+            return;
+        }
+    }
+    launch_conv_groups<CI, CO, IN, 1>(m, tiles, x, coarse, w, bias, flags, y, yf, pooled, s);
 }
 
 template <int CI, int CO>
 int launch_conv(int m, int in_mode, const unsigned short* x, const unsigned short* coarse, const void* wsw,
                 const float* bias, int flags, unsigned short* y, float* yf, unsigned short* pooled, hipStream_t s)
 {
-    constexpr int CB = c_blocks(CO);
-    const int txy = (m + kTX - 1) / kTX;
-    const int tiles = txy * txy * (m / kTZ);
-    // one channel block per workgroup while that is needed to fill the chip
-    const bool split = CB > 1 && tiles < 1024;
-    const int ntiles = tiles * (split ? CB : 1);
-    const bf16x4* w = static_cast<const bf16x4*>(wsw);
+    const bf16x8* w = static_cast<const bf16x8*>(wsw);
     if (in_mode == kInUpAdd) {
-        if constexpr (CI > 2) {
-            if (split)
-                launch_tiles<conv3_mfma_kernel<CI, CO, kInUpAdd, 1>>(ntiles, s, m, x, coarse, w, bias, flags, y, yf,
-                             pooled);
-            else
-                launch_tiles<conv3_mfma_kernel<CI, CO, kInUpAdd, CB>>(ntiles, s, m, x, coarse, w, bias, flags, y, yf,
-                             pooled);
-        }
+        if constexpr (CI > 2)
+            launch_conv_mode<CI, CO, kInUpAdd>(m, x, coarse, w, bias, flags, y, yf, pooled, s);
     } else {
-        if (split)
-            launch_tiles<conv3_mfma_kernel<CI, CO, kInPlain, 1>>(ntiles, s, m, x, coarse, w, bias, flags, y, yf, pooled);
-        else
-            launch_tiles<conv3_mfma_kernel<CI, CO, kInPlain, CB>>(ntiles, s, m, x, coarse, w, bias, flags, y, yf,
-                         pooled);
+        launch_conv_mode<CI, CO, kInPlain>(m, x, coarse, w, bias, flags, y, yf, pooled, s);
     }
     WA_LAUNCH_CHECK();
     return kOk;
@@ -671,7 +708,7 @@ WA_API size_t warp_amd_conv3_mfma_weight_size(int cin, int cout)
 {
     if (cout == 1 && cin >= 16)
         return sizeof(unsigned) * 27 * (size_t)(cin / 2);
-    return sizeof(short) * 4 * 64 * (size_t)wa::tap_loops(cin) * wa::k_blocks(cin) * wa::c_blocks(cout);
+    return sizeof(short) * 8 * 64 * (size_t)wa::k_blocks(cin) * wa::c_blocks(cout);
 }
 
 // FP32 W[27][cin][cout] -> swizzled BF16 A-operand tiles for a (cin -> cout)
@@ -688,9 +725,9 @@ WA_API int warp_amd_conv3_mfma_swizzle(int cin, int cout, int transpose_flip, co
         WA_LAUNCH_CHECK();
         return wa::kOk;
     }
-    const int total = (int)(warp_amd_conv3_mfma_weight_size(cin, cout) / 8);
+    const int total = (int)(warp_amd_conv3_mfma_weight_size(cin, cout) / 16);
     wa::swizzle_kernel<<<(unsigned)wa::ceil_div(total, 256), 256, 0, wa::as_stream(stream)>>>(
-        cin, cout, transpose_flip, W, static_cast<wa::bf16x4*>(wsw), total
+        cin, cout, transpose_flip, W, static_cast<wa::bf16x8*>(wsw), total
     );
     WA_LAUNCH_CHECK();
     return wa::kOk;
@@ -768,3 +805,4 @@ WA_API int warp_amd_unet_pack_input_bf16(int64_t nvox, const double* r, const fl
     WA_LAUNCH_CHECK();
     return wa::kOk;
 }
+

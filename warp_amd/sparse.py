@@ -1784,7 +1784,10 @@ def _vec_view(array: torch.Tensor, torch_dtype: torch.dtype, expected_scalar_cou
 class _MvPlan:
     """A matrix's constant arguments of bsr_mv, packed once (host struct of
     warp_amd_bsr_mv_packed), with what its x and y must match.  Valid for one
-    topology version, one ``values`` tensor and one nnz."""
+    topology version, one ``values`` tensor and one nnz.  ``args.sweep``
+    flips after every product: consecutive products with one matrix traverse
+    its rows in opposite directions (same bits), so each starts with the rows
+    the previous one read last, still in the memory-side cache."""
 
     __slots__ = ("version", "values", "nnz", "dtype", "nx", "ny", "device", "dev_index", "args", "args_ref", "fn")
 
@@ -1855,6 +1858,7 @@ def bsr_mv(
                 st = plan.fn(plan.args_ref, xp, yp, float(alpha), float(beta), _native.stream_of(plan.device))
                 if st:
                     _native.check(st, "warp_amd_bsr_mv_packed")
+                plan.args.sweep ^= 1
                 return y
     A, A_scale = _extract_matrix_and_scale(A)
     alpha = float(alpha) * A_scale
@@ -1937,6 +1941,7 @@ def bsr_mv(
             plan.fn(plan.args_ref, x_view.data_ptr(), y_view.data_ptr(), alpha, beta, _native.stream_of(plan.device)),
             "warp_amd_bsr_mv_packed",
         )
+        plan.args.sweep ^= 1
         return y
     # tile_size > 0: lanes per block row; < 0: one lane per row (row-sequential
     # sums, the reference row kernel's order); 0: chosen from the mean row length
