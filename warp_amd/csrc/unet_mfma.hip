@@ -448,51 +448,41 @@ void launch_tiles(int ntiles, hipStream_t s, Args... args)
     Kernel<<<grid, kBlock, 0, s>>>(args...);
 }
 
-// One output channel, streamed through z: a workgroup owns 16 x 16 columns
-// (one per thread) and kCo1Z consecutive output slices, and walks the input
-// planes z' = Z0 - 1 .. Z0 + kCo1Z one at a time.  Only one 18 x 18 plane
-// (its halo included) sits in LDS, double-buffered: the next plane's loads
-// are in flight while the current one is used.  A plane contributes to the
-// outputs z' + 1, z' and z' - 1 (taps dz = 0, 1, 2), accumulated in three
-// rolling registers, so every output still adds its taps in the order
-// t = 0..26, each tap's channels by v_dot2_f32_bf16 in pairs: the same
-// instruction sequence, hence the same bits, as one voxel per thread.  A
-// thread reads 9 halo voxels per plane: 9 reads per output against 27.
+// One output channel (the network's last layer), streamed through z: a
+// workgroup owns 16 x 16 columns and kCo1Z consecutive output slices, and
+// walks the input planes z' = Z0 - 1 .. Z0 + kCo1Z one at a time.  Per plane
+// one MFMA GEMM forms, for every column (x, y),
+//
+//   P_dz(x, y) = sum_{dy, dx, c} W[dz][dy][dx][c] in[x + dx, y + dy, z'][c],   dz = 0, 1, 2
+//
+// (rows = dz, padded to 16 with zero weights; columns = 16 x positions of one
+// y row; k = (dy, dx, c), 144 deep, padded to 5 blocks of 32), and the
+// plane's three results go to the outputs z' + 1, z' and z' - 1 in three
+// rolling FP32 accumulators per column: out = bias + P_0(z - 1) + P_1(z) +
+// P_2(z + 1).  Only one 18 x 18 plane (its halo included) is in LDS at a
+// time, double-buffered; the loads of the plane two steps ahead are in
+// flight in registers.  (Earlier forms, at 128^3: v_dot2_f32_bf16 per voxel
+// and tap, 39.3 us, bound by its 216 dot products per output; MFMA products
+// per voxel and tap exchanged through LDS, 45.7 us, bound by its three
+// barriers per plane.)
 constexpr int kCo1T = 16;             // columns per side of a workgroup
 constexpr int kCo1H = kCo1T + 2;      // halo plane side
-constexpr int kCo1Z = 8;              // output slices per workgroup (128^3: 8 -> 39.3 us, 16 -> 43.2, 32 -> 59.4)
+constexpr int kCo1Z = 8;              // output slices per workgroup
 constexpr int kCo1VB = 48;            // LDS bytes per voxel: 16 BF16 channels + 16 bytes of padding
+constexpr int kCo1KB = 5;             // k blocks of 32: (dy, dx, c) = 144, padded to 160
 
-template <int IN>
-__global__ __launch_bounds__(kBlock) void conv3_co1_stream_kernel(int m, const unsigned short* __restrict__ x,
-                                                                 const unsigned short* __restrict__ coarse,
-                                                                 const unsigned* __restrict__ wpk,
-                                                                 const float* __restrict__ bias, int flags,
-                                                                 float* __restrict__ yf)
-{
-    constexpr int CI = 16;
-    constexpr int kPlane = kCo1H * kCo1H;                    // 324 voxels
-    constexpr int kOct = kPlane * 2;                         // 16-byte octets per plane
-    constexpr int kIters = (kOct + kBlock - 1) / kBlock;     // 3 loads per thread and plane
-    __shared__ __attribute__((aligned(16))) unsigned char plane[2][kPlane * kCo1VB];
-    const int tiles_x = (m + kCo1T - 1) / kCo1T;
-    const int tiles_z = (m + kCo1Z - 1) / kCo1Z;
-    const int ntiles = tiles_x * tiles_x * tiles_z;
-    const int mh = m / 2;
-    const int tx = threadIdx.x % kCo1T, ty = threadIdx.x / kCo1T;
-    const float b0 = bias ? bias[0] : 0.0f;
-
-    for (TileWalk walk(ntiles); walk.t < walk.end; walk.t += walk.step) {
-    const int X0 = (walk.t % tiles_x) * kCo1T;
-    const int Y0 = ((walk.t / tiles_x) % tiles_x) * kCo1T;
-    const int Z0 = (walk.t / (tiles_x * tiles_x)) * kCo1Z;
-    const int nz = min(kCo1Z, m - Z0);
-    const int np = nz + 2;  // planes z' = Z0 - 1 .. Z0 + nz
-
+// One halo plane in registers between its loads and its LDS stores.
+template <int IN> struct Co1Plane {
+    static constexpr int CI = 16;
+    static constexpr int kOct = kCo1H * kCo1H * 2;            // 16-byte octets per plane
+    static constexpr int kIters = (kOct + kBlock - 1) / kBlock;  // 3 per thread
     bf16x8 v[kIters], cv[IN == kInUpAdd ? kIters : 1];
     bool in[kIters];
-    auto load_plane = [&](int p) {
-        const int gz = Z0 - 1 + p;
+
+    __device__ __forceinline__ void load(int m, int X0, int Y0, int gz, const unsigned short* __restrict__ x,
+                                         const unsigned short* __restrict__ coarse)
+    {
+        const int mh = m / 2;
         const int cz = min(max(gz, 0), m - 1);
 #pragma unroll
         for (int u = 0; u < kIters; ++u) {
@@ -505,8 +495,10 @@ __global__ __launch_bounds__(kBlock) void conv3_co1_stream_kernel(int m, const u
             if constexpr (IN == kInUpAdd)
                 cv[u] = load8(coarse + ((size_t)((cz >> 1) * mh + (cy >> 1)) * mh + (cx >> 1)) * CI + 8 * od);
         }
-    };
-    auto store_plane = [&](unsigned char* dst) {
+    }
+
+    __device__ __forceinline__ void store(unsigned char* dst) const
+    {
 #pragma unroll
         for (int u = 0; u < kIters; ++u) {
             const int c = u * kBlock + threadIdx.x;
@@ -517,60 +509,100 @@ __global__ __launch_bounds__(kBlock) void conv3_co1_stream_kernel(int m, const u
                 *reinterpret_cast<bf16x8*>(dst + (c / 2) * kCo1VB + 16 * (c % 2)) = in[u] ? w : zero8();
             }
         }
-    };
-
-    // acc[0]: output z' - 1 (its last plane), acc[1]: z', acc[2]: z' + 1 (its first plane)
-    float acc[3] = {b0, b0, b0};
-    load_plane(0);
-    for (int p = 0; p < np; ++p) {
-        unsigned char* cur = plane[p & 1];
-        store_plane(cur);
-        __syncthreads();  // the plane is complete; everyone is done with the slot stored next
-        if (p + 1 < np)
-            load_plane(p + 1);
-        const unsigned* hw = reinterpret_cast<const unsigned*>(cur);
-        int wofs = 0;
-#pragma unroll
-        for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-            for (int dx = 0; dx < 3; ++dx) {
-                // scalar-load the 24 weight words of (dy, dx) only after the
-                // previous group's products (cache hits): hoisted out of the
-                // loops, all 216 would not fit in SGPRs
-                asm volatile("" : "+s"(wofs) : "v"(acc[0]), "v"(acc[1]), "v"(acc[2]));
-                const unsigned* wp = wpk + wofs;
-                const int hv = (ty + dy) * kCo1H + tx + dx;
-                unsigned vv[CI / 2];
-#pragma unroll
-                for (int j = 0; j < CI / 2; ++j)
-                    vv[j] = hw[hv * (kCo1VB / 4) + j];
-#pragma unroll
-                for (int dz = 0; dz < 3; ++dz) {
-                    // plane z' is tap dz of output z' + 1 - dz
-                    const unsigned* w = wp + ((dz * 3 + dy) * 3 + dx) * (CI / 2);
-                    float a = acc[2 - dz];
-#pragma unroll
-                    for (int j = 0; j < CI / 2; ++j)
-                        a = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, vv[j]),
-                                                            __builtin_bit_cast(bf16x2, w[j]), a, false);
-                    acc[2 - dz] = a;
-                }
-            }
-        // output z' - 1 has all 27 taps now (planes z' - 2, z' - 1, z')
-        const int zo = Z0 - 2 + p;
-        const int vx = X0 + tx, vy = Y0 + ty;
-        if (p >= 2 && vx < m && vy < m) {
-            float r = acc[0];
-            if (flags & kOutRelu)
-                r = fmaxf(r, 0.0f);
-            float* d = yf + (size_t)(zo * m + vy) * m + vx;
-            *d = (flags & kOutAccumulate) ? *d + r : r;
-        }
-        acc[0] = acc[1];
-        acc[1] = acc[2];
-        acc[2] = b0;
     }
-    __syncthreads();  // the last plane's reads are done before the next tile stores
+};
+
+template <int IN>
+__global__ __launch_bounds__(kBlock) void conv3_co1_stream_kernel(int m, const unsigned short* __restrict__ x,
+                                                                 const unsigned short* __restrict__ coarse,
+                                                                 const unsigned* __restrict__ wpk,
+                                                                 const float* __restrict__ bias, int flags,
+                                                                 float* __restrict__ yf)
+{
+    constexpr int CI = 16;
+    constexpr int kPlane = kCo1H * kCo1H;            // 324 voxels
+    constexpr int kRowsPerWave = kCo1T / (kBlock / kWave);  // 4 y rows of 16 columns per wave
+    __shared__ __attribute__((aligned(16))) unsigned char plane[2][kPlane * kCo1VB];
+    const int tiles_x = (m + kCo1T - 1) / kCo1T;
+    const int tiles_z = (m + kCo1Z - 1) / kCo1Z;
+    const int ntiles = tiles_x * tiles_x * tiles_z;
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    const int ln = lane % 16, lq = lane / 16;  // ln: column (x), lq: k slot
+    const float b0 = bias ? bias[0] : 0.0f;
+    // A operands (constant): row ln = dz (< 3), k = 32 kb + 8 lq .. + 7 -> (dy, dx) = k / 16, channels k % 16
+    mfma_bf16x8 wa[kCo1KB];
+    // B operand offsets within a plane (bytes, relative to the column's corner voxel); -1: padding slot
+    int boff[kCo1KB];
+#pragma unroll
+    for (int kb = 0; kb < kCo1KB; ++kb) {
+        const int k0 = 32 * kb + 8 * lq;
+        const int t9 = k0 / 16, c0 = k0 % 16;
+        unsigned wv[4] = {0u, 0u, 0u, 0u};
+        if (ln < 3 && t9 < 9) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                wv[j] = wpk[(ln * 9 + t9) * (CI / 2) + c0 / 2 + j];
+        }
+        wa[kb] = __builtin_bit_cast(mfma_bf16x8, wv);
+        boff[kb] = t9 < 9 ? ((t9 / 3) * kCo1H + t9 % 3) * kCo1VB + 2 * c0 : -1;
+    }
+
+    for (TileWalk walk(ntiles); walk.t < walk.end; walk.t += walk.step) {
+    const int X0 = (walk.t % tiles_x) * kCo1T;
+    const int Y0 = ((walk.t / tiles_x) % tiles_x) * kCo1T;
+    const int Z0 = (walk.t / (tiles_x * tiles_x)) * kCo1Z;
+    const int nz = min(kCo1Z, m - Z0);
+    const int np = nz + 2;  // planes z' = Z0 - 1 .. Z0 + nz (at least 3)
+
+    // acc[j][0]: output z' - 1 of row j (its last plane), [1]: z', [2]: z' + 1 (its first plane);
+    // meaningful in the lanes lq == 0 (C rows 0..2 = dz)
+    float acc[kRowsPerWave][3];
+#pragma unroll
+    for (int j = 0; j < kRowsPerWave; ++j)
+        acc[j][0] = acc[j][1] = acc[j][2] = b0;
+    auto step = [&](Co1Plane<IN>& regs, int p) {
+        unsigned char* cur = plane[p & 1];
+        regs.store(cur);
+        __syncthreads();  // the plane is complete; everyone is done with the slot stored next
+        if (p + 2 < np)
+            regs.load(m, X0, Y0, Z0 + 1 + p, x, coarse);  // plane p + 2
+        const int zo = Z0 - 2 + p;  // output completed by this plane
+#pragma unroll
+        for (int j = 0; j < kRowsPerWave; ++j) {
+            const int ty = wave * kRowsPerWave + j;
+            const unsigned char* col = cur + (ty * kCo1H + ln) * kCo1VB;
+            f32x4 c4 = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kb = 0; kb < kCo1KB; ++kb) {
+                const bf16x8 bv = boff[kb] >= 0 ? *reinterpret_cast<const bf16x8*>(col + boff[kb]) : zero8();
+                c4 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[kb], __builtin_bit_cast(mfma_bf16x8, bv), c4, 0, 0, 0);
+            }
+            // plane z' is tap dz of output z' + 1 - dz
+            acc[j][2] += c4[0];
+            acc[j][1] += c4[1];
+            acc[j][0] += c4[2];
+            const int vx = X0 + ln, vy = Y0 + ty;
+            if (lq == 0 && p >= 2 && vx < m && vy < m) {
+                float r = acc[j][0];
+                if (flags & kOutRelu)
+                    r = fmaxf(r, 0.0f);
+                float* dst = yf + (size_t)(zo * m + vy) * m + vx;
+                *dst = (flags & kOutAccumulate) ? *dst + r : r;
+            }
+            acc[j][0] = acc[j][1];
+            acc[j][1] = acc[j][2];
+            acc[j][2] = b0;
+        }
+    };
+    Co1Plane<IN> ra, rb;
+    ra.load(m, X0, Y0, Z0 - 1, x, coarse);
+    rb.load(m, X0, Y0, Z0, x, coarse);
+    for (int p = 0; p < np; p += 2) {
+        step(ra, p);
+        if (p + 1 < np)
+            step(rb, p + 1);
+    }
+    __syncthreads();  // the last planes' reads are done before the next tile stores
     }  // tiles
 }
 
