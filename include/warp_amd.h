@@ -717,6 +717,19 @@ int warp_amd_bf16_to_f32(int64_t count, const void* in, float* out, void* stream
 /* BF16 network input [nvox][2] = (r * inv_scale, standardized log k) */
 int warp_amd_unet_pack_input_bf16(int64_t nvox, const double* r, const float* logk_std, const double* inv_scale,
                                   void* out, void* stream);
+/* Weight and bias gradients of a BF16 (cin -> cout) convolution on MFMA (no
+ * reference counterpart; training of the learned preconditioner):
+ * dW[27][cin][cout] = sum_v x~[v + d_t][ci] dy[v][co], db[cout] = sum_v dy[v][co],
+ * both overwritten.  x: the layer's BF16 input (in_mode 1: x + upsample(coarse),
+ * rounded as the forward rounds it); dy: FP32 [m^3][cout], rounded to BF16; FP32
+ * accumulation; per-workgroup partials in the workspace added in a fixed order.
+ * Channel pairs: 2-16, 16-32, 32-64, 64-64, 64-32, 32-16, 16-1. */
+size_t warp_amd_conv3_mfma_wgrad_workspace_size(int m, int cin, int cout);
+int warp_amd_conv3_mfma_wgrad(int m, int cin, int cout, int in_mode, const void* x, const void* coarse,
+                              const float* dy, float* dW, float* db, void* workspace, size_t workspace_bytes,
+                              void* stream);
+/* out = y > 0 ? dy : 0 for a BF16 y (ReLU backward of a BF16 forward); dy, out FP32 */
+int warp_amd_relu_mask_bf16(int64_t count, const void* y, const float* dy, float* out, void* stream);
 
 #ifdef __cplusplus
 }

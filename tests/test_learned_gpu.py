@@ -40,8 +40,8 @@ def test_energy_loss_kernel_vs_oracle(device):
     np.testing.assert_allclose(g.cpu().numpy(), g_ref, rtol=1e-12, atol=1e-15)
 
 
-@pytest.mark.parametrize("nu", [0, 1])
-def test_trainer_gradients_vs_oracle(device, nu):
+@pytest.mark.parametrize("nu,precision,tol", [(0, "fp32", 2e-3), (1, "fp32", 2e-3), (1, "bf16", 5e-2)])
+def test_trainer_gradients_vs_oracle(device, nu, precision, tol):
     n = 32
     system = learned.FV7System(n, 1.5, 2.0, seed=11, device=device)
     A = _csr(system)
@@ -53,7 +53,7 @@ def test_trainer_gradients_vs_oracle(device, nu):
         if k.endswith(".b"):
             p[k] = rng.standard_normal(p[k].shape) * 0.05
     net = learned.UNet(n, device, params={k: v.astype(np.float32) for k, v in p.items()})
-    trainer = learned.EnergyTrainer(net, smooth_steps=nu)
+    trainer = learned.EnergyTrainer(net, smooth_steps=nu, precision=precision)
     loss = trainer.step(system, torch.tensor(r, device=device), torch.tensor(e, device=device), lr=0.0)
     torch.cuda.synchronize()
     dinv = system.dinv.cpu().numpy()
@@ -61,11 +61,12 @@ def test_trainer_gradients_vs_oracle(device, nu):
     # the oracle sees the FP32 parameters and FP32-rounded logk the kernels see
     p32 = {k: v.astype(np.float32).astype(np.float64) for k, v in p.items()}
     L_ref, g_ref = ounet.composite_grads(p32, A, dinv, lk, r, e, nu, trainer.omega, 1.0 / system.dmean, n)
-    assert float(loss.item()) == pytest.approx(L_ref, rel=1e-4)
+    assert float(loss.item()) == pytest.approx(L_ref, rel=1e-4 if precision == "fp32" else 2e-2)
     for k in p:
         got = net.grads[k].double().cpu().numpy().reshape(g_ref[k].shape)
         rel = np.linalg.norm(got - g_ref[k]) / max(np.linalg.norm(g_ref[k]), 1e-30)
-        assert rel < 2e-3, (k, rel)
+        print(f"[trainer {precision}] grad {k}: rel. error {rel:.2e}")
+        assert rel < tol, (k, rel)
 
 
 def test_zero_network_stagnates_then_falls_back(device):
@@ -84,9 +85,11 @@ def test_zero_network_stagnates_then_falls_back(device):
     assert out["true_relative_residual"] < 2e-8
 
 
-def test_training_lowers_the_energy_loss(device):
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_training_lowers_the_energy_loss(device, precision):
     """A short run of the training pipeline (plain-CG pairs, then the FCG's own
-    iterates) lowers the energy loss of held-out pairs."""
+    iterates) lowers the energy loss of held-out pairs, with the network's
+    training arithmetic in FP32 kernels or on BF16 MFMA."""
     n = 32
     systems = [learned.FV7System(n, 1.5, 2.0, seed=100 + i, device=device) for i in range(4)]
     held = learned.FV7System(n, 1.5, 2.0, seed=999, device=device)
@@ -96,7 +99,7 @@ def test_training_lowers_the_energy_loss(device):
     trainer = learned.EnergyTrainer(net, smooth_steps=1)
     before = np.mean([trainer.loss(held, r, e) for r, e in pairs])
     result = learned.train_learned_preconditioner(net, systems, smooth_steps=1, steps=300, dagger_rounds=1,
-                                                  dagger_steps=100, seed=0)
+                                                  dagger_steps=100, seed=0, precision=precision)
     assert result["pcg_pairs"] == 40 and result["fcg_pairs"] > 0
     after = np.mean([trainer.loss(held, r, e) for r, e in pairs])
     assert after < 0.8 * before, (before, after)

@@ -151,3 +151,102 @@ def test_unet_forward_bf16_vs_oracle(device):
     # and the BF16 network stays close to the FP64 one
     rel64 = np.linalg.norm(got - ref64[..., 0]) / np.linalg.norm(ref64)
     assert rel64 < 5e-2, rel64
+
+
+WGRAD_PAIRS = [(2, 16), (16, 32), (32, 64), (64, 64), (64, 32), (32, 16), (16, 1)]
+
+
+def _wgrad(device, m, ci, co, x_bf, dy, coarse_bf=None):
+    nbytes = _native.lib().warp_amd_conv3_mfma_wgrad_workspace_size(m, ci, co)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=device)
+    dW = torch.full((27 * ci * co,), float("nan"), dtype=torch.float32, device=device)
+    db = torch.full((co,), float("nan"), dtype=torch.float32, device=device)
+    dyt = torch.tensor(np.asarray(dy, np.float32).reshape(-1), device=device)
+    _native.call("warp_amd_conv3_mfma_wgrad", m, ci, co, 1 if coarse_bf is not None else 0, _native.ptr(x_bf),
+                 _native.ptr(coarse_bf) if coarse_bf is not None else None, _native.ptr(dyt), _native.ptr(dW),
+                 _native.ptr(db), _native.ptr(ws), ws.numel(), _native.stream_of(device))
+    torch.cuda.synchronize()
+    return dW.cpu().numpy().reshape(27, ci, co).astype(np.float64), db.cpu().numpy().astype(np.float64)
+
+
+@pytest.mark.parametrize("ci,co", WGRAD_PAIRS)
+@pytest.mark.parametrize("m", [8, 12])
+def test_wgrad_vs_oracle(device, ci, co, m):
+    """Weight / bias gradients on MFMA against the FP64 oracle fed the same
+    BF16-rounded input and output gradient; FP32 accumulation error bounded by
+    2e-5 of the sum of absolute products."""
+    rng = np.random.default_rng(500 * ci + co + m)
+    x = ounet.bf16(rng.standard_normal((m, m, m, ci)))
+    dy = rng.standard_normal((m, m, m, co))
+    dyb = ounet.bf16(dy)
+    W0 = np.zeros((27, ci, co))
+    _, dW_ref, db_ref = ounet.conv3_bwd(x, W0, None, dyb, relu=False)
+    _, dW_abs, db_abs = ounet.conv3_bwd(np.abs(x), W0, None, np.abs(dyb), relu=False)
+    dW, db = _wgrad(device, m, ci, co, _bf16_tensor(x, device), dy)
+    assert np.all(np.abs(dW - dW_ref) <= 2e-5 * dW_abs + 1e-30), float(np.max(np.abs(dW - dW_ref) / (dW_abs + 1e-30)))
+    assert np.all(np.abs(db - db_ref) <= 2e-5 * db_abs + 1e-30)
+
+
+@pytest.mark.parametrize("ci,co", [(64, 32), (32, 16), (16, 1)])
+def test_wgrad_upsample_add_input(device, ci, co):
+    m = 8
+    rng = np.random.default_rng(41 * ci + co)
+    skip = ounet.bf16(rng.standard_normal((m, m, m, ci)))
+    coarse = ounet.bf16(rng.standard_normal((m // 2, m // 2, m // 2, ci)))
+    x = ounet.bf16(skip + ounet.up2(coarse))
+    dy = rng.standard_normal((m, m, m, co))
+    dyb = ounet.bf16(dy)
+    W0 = np.zeros((27, ci, co))
+    _, dW_ref, db_ref = ounet.conv3_bwd(x, W0, None, dyb, relu=False)
+    _, dW_abs, _ = ounet.conv3_bwd(np.abs(x), W0, None, np.abs(dyb), relu=False)
+    dW, db = _wgrad(device, m, ci, co, _bf16_tensor(skip, device), dy, coarse_bf=_bf16_tensor(coarse, device))
+    assert np.all(np.abs(dW - dW_ref) <= 2e-5 * dW_abs + 1e-30)
+    np.testing.assert_allclose(db, db_ref, rtol=1e-5, atol=1e-6)
+
+
+def test_wgrad_deterministic(device):
+    """Partial sums added in a fixed order: two calls give the same bits."""
+    m, ci, co = 16, 32, 64
+    rng = np.random.default_rng(9)
+    xb = _bf16_tensor(rng.standard_normal((m, m, m, ci)), device)
+    dy = rng.standard_normal((m, m, m, co))
+    a = _wgrad(device, m, ci, co, xb, dy)
+    b = _wgrad(device, m, ci, co, xb, dy)
+    assert a[0].tobytes() == b[0].tobytes() and a[1].tobytes() == b[1].tobytes()
+
+
+def test_unet_backward_bf16_vs_oracle(device):
+    """Every parameter gradient of the MFMA training backward against the FP64
+    oracle's.  Each operator is pinned above with the oracle fed the same
+    BF16 operands; through the whole chain the BF16 activations, ReLU masks
+    and output gradients compound, the first layer's gradients most (4-6 %
+    measured): each within 10 %, the whole gradient's direction within 0.5 %
+    (the FP32 backward stays within 1e-4)."""
+    from warp_amd.optim import learned
+
+    n = 32
+    rng = np.random.default_rng(23)
+    p = ounet.init_params(6)
+    for k in p:
+        if k.endswith(".b"):
+            p[k] = rng.standard_normal(p[k].shape) * 0.05
+    inp = rng.standard_normal((n, n, n, 2))
+    dout = rng.standard_normal((n, n, n, 1))
+    _, cache = ounet.forward(p, inp)
+    g_ref = ounet.backward(p, cache, dout)
+    net = learned.UNet(n, device, params={k: v.astype(np.float32) for k, v in p.items()})
+    net.forward_bf16(_bf16_tensor(inp, device))
+    net.backward_bf16(torch.tensor(dout.reshape(-1), dtype=torch.float32, device=device))
+    torch.cuda.synchronize()
+    rels, got_all, ref_all = {}, [], []
+    for k in p:
+        got = net.grads[k].double().cpu().numpy().reshape(g_ref[k].shape)
+        rels[k] = np.linalg.norm(got - g_ref[k]) / max(np.linalg.norm(g_ref[k]), 1e-30)
+        print(f"[unet bf16] grad {k}: rel. error {rels[k]:.2e}")
+        got_all.append(got.reshape(-1))
+        ref_all.append(g_ref[k].reshape(-1))
+    g, r = np.concatenate(got_all), np.concatenate(ref_all)
+    cos = float(g @ r / (np.linalg.norm(g) * np.linalg.norm(r)))
+    print(f"[unet bf16] cosine of the whole gradient {cos:.5f}")
+    assert all(v < 0.1 for v in rels.values()), rels
+    assert cos > 0.995, cos

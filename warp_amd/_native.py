@@ -183,6 +183,12 @@ _SIGNATURES = {
         c_int,
         [c_int, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr],
     ),
+    "warp_amd_conv3_mfma_wgrad_workspace_size": (c_size, [c_int, c_int, c_int]),
+    "warp_amd_conv3_mfma_wgrad": (
+        c_int,
+        [c_int, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr],
+    ),
+    "warp_amd_relu_mask_bf16": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr]),
     "warp_amd_f32_to_bf16": (c_int, [c_i64, c_ptr, c_ptr, c_ptr]),
     "warp_amd_bf16_to_f32": (c_int, [c_i64, c_ptr, c_ptr, c_ptr]),
     "warp_amd_unet_pack_input_bf16": (c_int, [c_i64, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]),

@@ -719,6 +719,298 @@ __global__ void pack_input_bf16_kernel(int64_t nvox, const double* __restrict__ 
     reinterpret_cast<unsigned*>(out)[i] = lo | (hi << 16);
 }
 
+// ---------------------------------------------------------------------------
+// Weight and bias gradients on MFMA (training):
+//
+//   dW[t][ci][co] = sum_v x~[v + d_t][ci] dy[v][co],   db[co] = sum_v dy[v][co]
+//
+// with x~ the layer input as the forward saw it (x, or x + upsample(coarse)
+// rounded to BF16) and dy the FP32 output gradient, rounded to BF16 for the
+// MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulation).  GEMM per workgroup
+// tile of 8 x 8 x 4 voxels: C[g][co] += A[g][k] B[k][co], k = 32 voxels (four
+// x rows of 8), rows g = (tap, input channel) with g = t * CI + ci, and row
+// 27 * CI a row of ones whose C row is db.  A lane's A operand is the 8
+// voxels of one x row of the input shifted by its row's tap; three copies of
+// the halo pre-shifted by dx = -1, 0, 1 (copy s holds halo x = s .. s + 7)
+// keep that read 16-byte aligned.  Its B operand is the same 8 voxels of dy
+// for one output channel (dy staged channel-major).  Rows are cut in blocks
+// of 16: for CI >= 16 one block per tap and 16-channel group (workgroups
+// split the channel groups) plus the ones block in group 0; for CI <= 2 the
+// 27 * CI + 1 rows in order.  The four waves of a workgroup take every fourth
+// row block.  Workgroups are persistent over their tiles, sum in registers,
+// and write one partial [27 * CI + 1][CO_pad] each; wgrad_sum_kernel adds
+// the partials in a fixed order.
+constexpr int kWgXS = 6 * 10 * 8 * 2 + 16;  // bytes per (copy, channel) of the shifted halo (+16 padding)
+constexpr int kWgDS = 32 * 8 * 2 + 16;       // bytes per output channel of the dy tile (+16 padding)
+constexpr int kWgMaxSlots = 256;             // workgroups per channel group
+
+__host__ __device__ constexpr int wg_channels(int ci) { return ci >= 16 ? 16 : ci; }
+__host__ __device__ constexpr int wg_groups(int ci) { return ci >= 16 ? ci / 16 : 1; }
+__host__ __device__ constexpr int wg_row_blocks(int ci) { return ci >= 16 ? 28 : (27 * ci + 1 + 15) / 16; }
+__host__ __device__ constexpr int wg_rows(int ci) { return 27 * ci + 1; }
+
+template <int CI, int CO, int IN>
+__global__ __launch_bounds__(kBlock) void conv3_wgrad_mfma_kernel(int m, const unsigned short* __restrict__ x,
+                                                                  const unsigned short* __restrict__ coarse,
+                                                                  const float* __restrict__ dy, int nslots,
+                                                                  float* __restrict__ partial)
+{
+    constexpr int CH = wg_channels(CI);
+    constexpr int NCG = wg_groups(CI);
+    constexpr int NRB = wg_row_blocks(CI);
+    constexpr int RBW = (NRB + 3) / 4;  // row blocks per wave
+    constexpr int CBN = c_blocks(CO);
+    constexpr int COP = 16 * CBN;
+    constexpr int ROWS = wg_rows(CI);
+    constexpr int kXBytes = 3 * CH * kWgXS;
+    constexpr int kDyOff = kXBytes;
+    constexpr int kBytes = kDyOff + COP * kWgDS;
+    static_assert(kBytes <= 160 * 1024, "LDS budget");
+    __shared__ __attribute__((aligned(16))) unsigned char smem[kBytes];
+    unsigned char* xs = smem;
+    unsigned char* dys = smem + kDyOff;
+
+    const int cg = (int)blockIdx.x % NCG;  // channel group: input channels 16 cg .. (CI >= 16)
+    const int slot = (int)blockIdx.x / NCG;
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    const int ln = lane % 16, kq = lane / 16;
+    const int tiles_x = (m + kTX - 1) / kTX;
+    const int tiles_xy = tiles_x * tiles_x;
+    const int ntiles = tiles_xy * (m / kTZ);
+    const int mh = m / 2;
+
+    // padded output channels stay zero in the dy tile
+    for (int i = threadIdx.x; i < (COP - CO) * (kWgDS / 4); i += kBlock)
+        reinterpret_cast<unsigned*>(dys + CO * kWgDS)[i] = 0u;
+
+    // the lane's row in each of its row blocks: kind 0 = weight row (A from the
+    // shifted halo), 1 = the ones row, 2 = padding; byte offset of its A rows
+    int kind[RBW], aoff[RBW], grow[RBW];
+#pragma unroll
+    for (int j = 0; j < RBW; ++j) {
+        const int rb = wave + 4 * j;
+        int g = -1, k = 2;
+        if (rb < NRB) {
+            if constexpr (CI >= 16) {
+                if (rb < 27) {
+                    g = rb * CI + 16 * cg + ln;
+                    k = 0;
+                } else if (cg == 0 && ln == 0) {
+                    g = 27 * CI;
+                    k = 1;
+                }
+            } else {
+                const int gg = 16 * rb + ln;
+                if (gg < 27 * CI) {
+                    g = gg;
+                    k = 0;
+                } else if (gg == 27 * CI) {
+                    g = gg;
+                    k = 1;
+                }
+            }
+        }
+        kind[j] = k;
+        grow[j] = g;
+        aoff[j] = 0;
+        if (k == 0) {
+            const int t = g / CI, c = (g % CI) - (CI >= 16 ? 16 * cg : 0);
+            const int dz = t / 9, dyy = (t / 3) % 3, s = t % 3;  // halo offsets (d + 1)
+            aoff[j] = (s * CH + c) * kWgXS + (dz * 10 + dyy) * 16;
+        }
+    }
+    f32x4 acc[RBW][CBN];
+#pragma unroll
+    for (int j = 0; j < RBW; ++j)
+#pragma unroll
+        for (int cb = 0; cb < CBN; ++cb)
+            acc[j][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const bf16x8 ones = {0x3f80, 0x3f80, 0x3f80, 0x3f80, 0x3f80, 0x3f80, 0x3f80, 0x3f80};
+
+    for (TileWalk walk(ntiles, slot, nslots); walk.t < walk.end; walk.t += walk.step) {
+        const int t2 = walk.t;
+        const int X0 = (t2 % tiles_x) * kTX;
+        const int Y0 = ((t2 / tiles_x) % tiles_x) * kTY;
+        const int Z0 = (t2 / tiles_xy) * kTZ;
+        __syncthreads();  // the previous tile's reads are done
+        // shifted halo copies of the group's CH input channels
+        if constexpr (CI >= 16) {
+            constexpr int ITEMS = kHalo * 2;  // (voxel, channel octet)
+            for (int it = threadIdx.x; it < ITEMS; it += kBlock) {
+                const int hv = it >> 1, od = it & 1;
+                const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
+                const int gx = X0 - 1 + hx, gy = Y0 - 1 + hy, gz = Z0 - 1 + hz;
+                bf16x8 v = zero8();
+                if (gx >= 0 && gx < m && gy >= 0 && gy < m && gz >= 0 && gz < m) {
+                    const size_t g = (size_t)(gz * m + gy) * m + gx;
+                    v = load8(x + g * CI + 16 * cg + 8 * od);
+                    if constexpr (IN == kInUpAdd)
+                        v = add8(v, load8(coarse + ((size_t)((gz >> 1) * mh + (gy >> 1)) * mh + (gx >> 1)) * CI +
+                                          16 * cg + 8 * od));
+                }
+#pragma unroll
+                for (int sft = 0; sft < 3; ++sft) {
+                    const int xl = hx - sft;
+                    if (xl >= 0 && xl < 8) {
+#pragma unroll
+                        for (int c = 0; c < 8; ++c)
+                            *reinterpret_cast<unsigned short*>(xs + (sft * CH + 8 * od + c) * kWgXS +
+                                                               (hz * 10 + hy) * 16 + 2 * xl) = (unsigned short)v[c];
+                    }
+                }
+            }
+        } else {
+            for (int hv = threadIdx.x; hv < kHalo; hv += kBlock) {
+                const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
+                const int gx = X0 - 1 + hx, gy = Y0 - 1 + hy, gz = Z0 - 1 + hz;
+                unsigned u = 0u;
+                if (gx >= 0 && gx < m && gy >= 0 && gy < m && gz >= 0 && gz < m) {
+                    const size_t g = (size_t)(gz * m + gy) * m + gx;
+                    u = CI == 2 ? reinterpret_cast<const unsigned*>(x)[g] : (unsigned)x[g];
+                }
+#pragma unroll
+                for (int sft = 0; sft < 3; ++sft) {
+                    const int xl = hx - sft;
+                    if (xl >= 0 && xl < 8) {
+#pragma unroll
+                        for (int c = 0; c < CI; ++c)
+                            *reinterpret_cast<unsigned short*>(xs + (sft * CH + c) * kWgXS + (hz * 10 + hy) * 16 +
+                                                               2 * xl) = (unsigned short)(u >> (16 * c));
+                    }
+                }
+            }
+        }
+        // dy tile, channel-major BF16: dys[co][z * 8 + y][x]
+        {
+            constexpr int QPV = CO >= 4 ? CO / 4 : 1;  // 4-channel quads per voxel
+            constexpr int ITEMS = kTX * kTY * kTZ * QPV;
+            for (int it = threadIdx.x; it < ITEMS; it += kBlock) {
+                const int vl = it / QPV, qd = it % QPV;
+                const int lx = vl % 8, ly = (vl / 8) % 8, lz = vl / 64;
+                const int gx = X0 + lx, gy = Y0 + ly, gz = Z0 + lz;
+                const bool in = gx < m && gy < m;
+                const size_t g = in ? (size_t)(gz * m + gy) * m + gx : 0;
+                unsigned char* row = dys + (lz * 8 + ly) * 16 + 2 * lx;
+                if constexpr (CO >= 4) {
+                    const float4 d = in ? *reinterpret_cast<const float4*>(dy + g * CO + 4 * qd)
+                                        : float4{0.f, 0.f, 0.f, 0.f};
+                    const unsigned p0 = pack_bf16x2(d.x, d.y), p1 = pack_bf16x2(d.z, d.w);
+                    *reinterpret_cast<unsigned short*>(row + (4 * qd + 0) * kWgDS) = (unsigned short)p0;
+                    *reinterpret_cast<unsigned short*>(row + (4 * qd + 1) * kWgDS) = (unsigned short)(p0 >> 16);
+                    *reinterpret_cast<unsigned short*>(row + (4 * qd + 2) * kWgDS) = (unsigned short)p1;
+                    *reinterpret_cast<unsigned short*>(row + (4 * qd + 3) * kWgDS) = (unsigned short)(p1 >> 16);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < CO; ++c) {
+                        const float d = in ? dy[g * CO + c] : 0.0f;
+                        *reinterpret_cast<unsigned short*>(row + c * kWgDS) = (unsigned short)pack_bf16x2(d, 0.0f);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kb = 0; kb < 8; ++kb) {
+            const int R = 4 * kb + kq;  // x row of the tile: y = R % 8, z = R / 8
+            const int rofs = ((R / 8) * 10 + (R % 8)) * 16;
+            mfma_bf16x8 b[CBN];
+#pragma unroll
+            for (int cb = 0; cb < CBN; ++cb)
+                b[cb] = __builtin_bit_cast(mfma_bf16x8,
+                                           *reinterpret_cast<const bf16x8*>(dys + (16 * cb + ln) * kWgDS + R * 16));
+#pragma unroll
+            for (int j = 0; j < RBW; ++j) {
+                if (wave + 4 * j >= NRB)
+                    continue;  // wave-uniform
+                bf16x8 av = zero8();
+                if (kind[j] == 0)
+                    av = *reinterpret_cast<const bf16x8*>(xs + aoff[j] + rofs);
+                else if (kind[j] == 1)
+                    av = ones;
+                const mfma_bf16x8 a = __builtin_bit_cast(mfma_bf16x8, av);
+#pragma unroll
+                for (int cb = 0; cb < CBN; ++cb)
+                    acc[j][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[cb], acc[j][cb], 0, 0, 0);
+            }
+        }
+    }
+    // partial[slot][row][co]: lane holds rows 4 kq + i of each row block, column ln
+#pragma unroll
+    for (int j = 0; j < RBW; ++j) {
+        const int rb = wave + 4 * j;
+        if (rb >= NRB)
+            continue;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            // the row held in C row 4 kq + i of this block (same mapping as the A rows)
+            const int r = 4 * kq + i;
+            int g = -1;
+            if constexpr (CI >= 16) {
+                if (rb < 27)
+                    g = rb * CI + 16 * cg + r;
+                else if (cg == 0 && r == 0)
+                    g = 27 * CI;
+            } else {
+                g = 16 * rb + r < ROWS ? 16 * rb + r : -1;
+            }
+            if (g >= 0) {
+#pragma unroll
+                for (int cb = 0; cb < CBN; ++cb)
+                    partial[((size_t)slot * ROWS + g) * COP + 16 * cb + ln] = acc[j][cb][i];
+            }
+        }
+    }
+}
+
+// dW[row * CO + co] (rows < 27 * CI) and db[co] (row 27 * CI) = sum over the slots in order
+__global__ __launch_bounds__(kBlock) void wgrad_sum_kernel(int rows, int co, int cop, int nslots,
+                                                           const float* __restrict__ partial, float* __restrict__ dW,
+                                                           float* __restrict__ db)
+{
+    const int idx = blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= rows * co)
+        return;
+    const int g = idx / co, c = idx % co;
+    float s = 0.0f;
+    for (int k = 0; k < nslots; ++k)
+        s += partial[((size_t)k * rows + g) * cop + c];
+    if (g < rows - 1)
+        dW[(size_t)g * co + c] = s;
+    else
+        db[c] = s;
+}
+
+template <int CI, int CO>
+int launch_wgrad(int m, int in_mode, const unsigned short* x, const unsigned short* coarse, const float* dy,
+                 float* dW, float* db, float* partial, hipStream_t s)
+{
+    const int txy = (m + kTX - 1) / kTX;
+    const int tiles = txy * txy * (m / kTZ);
+    const int nslots = tiles < kWgMaxSlots ? tiles : kWgMaxSlots;
+    const unsigned grid = (unsigned)(nslots * wg_groups(CI));
+    if (in_mode == kInUpAdd) {
+        if constexpr (CI >= 16)
+            conv3_wgrad_mfma_kernel<CI, CO, kInUpAdd><<<grid, kBlock, 0, s>>>(m, x, coarse, dy, nslots, partial);
+    } else {
+        conv3_wgrad_mfma_kernel<CI, CO, kInPlain><<<grid, kBlock, 0, s>>>(m, x, coarse, dy, nslots, partial);
+    }
+    WA_LAUNCH_CHECK();
+    const int rows = wg_rows(CI);
+    wgrad_sum_kernel<<<(unsigned)ceil_div(rows * CO, kBlock), kBlock, 0, s>>>(rows, CO, 16 * c_blocks(CO), nslots,
+                                                                           partial, dW, db);
+    WA_LAUNCH_CHECK();
+    return kOk;
+}
+
+// BF16 ReLU mask of an FP32 gradient: out = y > 0 ? dy : 0 (y BF16)
+__global__ __launch_bounds__(kBlock) void relu_mask_bf16_kernel(int64_t count, const unsigned short* __restrict__ y,
+                                                                const float* __restrict__ dy, float* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < count)
+        out[i] = bf2f(y[i]) > 0.0f ? dy[i] : 0.0f;
+}
+
 }  // namespace
 }  // namespace wa
 
@@ -838,3 +1130,70 @@ WA_API int warp_amd_unet_pack_input_bf16(int64_t nvox, const double* r, const fl
     return wa::kOk;
 }
 
+namespace {
+bool wgrad_pair_supported(int ci, int co)
+{
+    static const int pairs[][2] = {{2, 16}, {16, 32}, {32, 64}, {64, 64}, {64, 32}, {32, 16}, {16, 1}};
+    for (const auto& p : pairs)
+        if (p[0] == ci && p[1] == co)
+            return true;
+    return false;
+}
+}  // namespace
+
+// Bytes of workspace of warp_amd_conv3_mfma_wgrad on an m^3 grid.
+WA_API size_t warp_amd_conv3_mfma_wgrad_workspace_size(int m, int cin, int cout)
+{
+    if (m <= 0)
+        return 0;
+    const int64_t txy = wa::ceil_div(m, wa::kTX);
+    const int64_t tiles = txy * txy * (m / wa::kTZ);
+    const int64_t slots = tiles < wa::kWgMaxSlots ? tiles : wa::kWgMaxSlots;
+    return sizeof(float) * (size_t)slots * wa::wg_rows(cin) * 16 * wa::c_blocks(cout);
+}
+
+// Weight and bias gradients of a BF16 (cin -> cout) 3x3x3 convolution on an
+// m^3 grid (m a multiple of 4): dW[27][cin][cout] = sum_v x~[v + d_t][ci]
+// dy[v][co], db[cout] = sum_v dy[v][co], both overwritten.  x: BF16 layer
+// input [m^3][cin] (in_mode 1: x + upsample(coarse), as the forward formed
+// it); dy: FP32 [m^3][cout], rounded to BF16; FP32 accumulation, partial
+// sums per workgroup added in a fixed order (deterministic).
+WA_API int warp_amd_conv3_mfma_wgrad(int m, int cin, int cout, int in_mode, const void* x, const void* coarse,
+                                     const float* dy, float* dW, float* db, void* workspace, size_t workspace_bytes,
+                                     void* stream)
+{
+    WA_REQUIRE(wgrad_pair_supported(cin, cout), "conv3_mfma_wgrad: unsupported channel pair %d -> %d", cin, cout);
+    WA_REQUIRE(m >= 4 && m % 4 == 0, "conv3_mfma_wgrad: grid side %d must be a positive multiple of 4", m);
+    WA_REQUIRE(in_mode == 0 || (in_mode == 1 && cin >= 16 && coarse), "conv3_mfma_wgrad: bad input mode");
+    WA_REQUIRE(x && dy && dW && db, "conv3_mfma_wgrad: null array");
+    WA_REQUIRE(workspace && workspace_bytes >= warp_amd_conv3_mfma_wgrad_workspace_size(m, cin, cout),
+               "conv3_mfma_wgrad: workspace too small");
+    const unsigned short* xi = static_cast<const unsigned short*>(x);
+    const unsigned short* ci = static_cast<const unsigned short*>(coarse);
+    float* pw = static_cast<float*>(workspace);
+    hipStream_t s = wa::as_stream(stream);
+#define WA_WGRAD_CASE(A, B)                                                                                      \
+    if (cin == A && cout == B)                                                                                   \
+        return wa::launch_wgrad<A, B>(m, in_mode, xi, ci, dy, dW, db, pw, s);
+    WA_WGRAD_CASE(2, 16)
+    WA_WGRAD_CASE(16, 32)
+    WA_WGRAD_CASE(32, 64)
+    WA_WGRAD_CASE(64, 64)
+    WA_WGRAD_CASE(64, 32)
+    WA_WGRAD_CASE(32, 16)
+    WA_WGRAD_CASE(16, 1)
+#undef WA_WGRAD_CASE
+    return wa::kErrUnsupported;
+}
+
+// out = y > 0 ? dy : 0 with y BF16 (the ReLU of a BF16 forward), dy and out FP32
+WA_API int warp_amd_relu_mask_bf16(int64_t count, const void* y, const float* dy, float* out, void* stream)
+{
+    if (count <= 0)
+        return wa::kOk;
+    wa::relu_mask_bf16_kernel<<<(unsigned)wa::ceil_div(count, wa::kBlock), wa::kBlock, 0, wa::as_stream(stream)>>>(
+        count, static_cast<const unsigned short*>(y), dy, out
+    );
+    WA_LAUNCH_CHECK();
+    return wa::kOk;
+}

@@ -181,6 +181,7 @@ class UNet:
         a3 = H("a3", 3, 64)
         b2, b1 = H("b2", 2, 32), H("b1", 1, 16)
         out = self.buf("out16", 0, 1)
+        self._inp_bf16 = inp_bf16
         self._mconv("e0", inp_bf16, y=a0, pooled=p0)
         self._mconv("e1", p0, y=a1, pooled=p1)
         self._mconv("e2", p1, y=a2, pooled=p2)
@@ -189,6 +190,110 @@ class UNet:
         self._mconv("d1", a1, y=b1, coarse=b2)
         self._mconv("d0", a0, yf=out, coarse=b1)
         return out
+
+    # -- BF16 MFMA training ------------------------------------------------------
+    def sync_bf16_flipped(self) -> None:
+        """Swizzled BF16 backward-data weights (each layer's flipped transpose)."""
+        s = self._s()
+        self._wsw_t = {}
+        for name, ci, co, _, _ in LAYERS:
+            if name == "e0":
+                continue  # no gradient with respect to the network input
+            nbytes = self._lib.warp_amd_conv3_mfma_weight_size(co, ci)
+            t = torch.empty(nbytes, dtype=torch.uint8, device=self.dev)
+            _native.call("warp_amd_conv3_mfma_swizzle", co, ci, 1, _native.ptr(self.params[name + ".W"]),
+                         _native.ptr(t), s)
+            self._wsw_t[name] = t
+
+    def _wgrad_bf16(self, name, x, dy, coarse=None):
+        """dW, db of a layer on MFMA from its BF16 input (x + upsample(coarse) for decoders) and FP32 dy."""
+        ci, co, _, lvl = LAYER_INDEX[name]
+        n = self.side(lvl)
+        ws_bytes = self._lib.warp_amd_conv3_mfma_wgrad_workspace_size(n, ci, co)
+        t = self._buf.get("wgrad_mfma_ws")
+        if t is None or t.numel() < ws_bytes:
+            t = torch.empty(ws_bytes, dtype=torch.uint8, device=self.dev)
+            self._buf["wgrad_mfma_ws"] = t
+        _native.call(
+            "warp_amd_conv3_mfma_wgrad", n, ci, co, 1 if coarse is not None else 0, _native.ptr(x),
+            _native.ptr(coarse) if coarse is not None else None, _native.ptr(dy), _native.ptr(self.grads[name + ".W"]),
+            _native.ptr(self.grads[name + ".b"]), _native.ptr(t), t.numel(), self._s(),
+        )
+
+    def _bwd_data_bf16(self, name, dy, dx):
+        """dx = the input gradient of a layer (FP32) from its FP32 output gradient dy, on MFMA."""
+        ci, co, _, lvl = LAYER_INDEX[name]
+        n = self.side(lvl)
+        s = self._s()
+        h = self._hbuf("gy_" + name, lvl, co)
+        _native.call("warp_amd_f32_to_bf16", h.numel(), _native.ptr(dy), _native.ptr(h), s)
+        _native.call("warp_amd_conv3_mfma", n, co, ci, 0, _native.ptr(h), None, _native.ptr(self._wsw_t[name]), None,
+                     4, None, _native.ptr(dx), None, s)
+
+    def _relu_bf16(self, y_bf16, dy, out):
+        _native.call("warp_amd_relu_mask_bf16", dy.numel(), _native.ptr(y_bf16), _native.ptr(dy), _native.ptr(out),
+                     self._s())
+        return out
+
+    def backward_bf16(self, dout: torch.Tensor) -> None:
+        """Gradients of all parameters (into self.grads) for dL/dout after :meth:`forward_bf16`.
+
+        Weight and bias gradients on MFMA from the BF16 layer inputs the
+        forward formed (decoder inputs skip + upsample(coarse) re-formed the
+        same way) and the output gradients rounded to BF16; input gradients
+        by the MFMA convolution with flipped weights (FP32 out); ReLU masks
+        from the stored BF16 activations; pooling / upsampling adjoints and
+        the skip sums in FP32."""
+        if getattr(self, "_wsw_t", None) is None:
+            self.sync_bf16_flipped()
+        H, B = self._hbuf, self.buf
+        s = self._s()
+        a0, p0 = H("a0", 0, 16), H("p0", 1, 16)
+        a1, p1 = H("a1", 1, 32), H("p1", 2, 32)
+        a2, p2 = H("a2", 2, 64), H("p2", 3, 64)
+        a3 = H("a3", 3, 64)
+        b2, b1 = H("b2", 2, 32), H("b1", 1, 16)
+        # d0 (linear): input a0 + up(b1)
+        self._wgrad_bf16("d0", a0, dout, coarse=b1)
+        da0 = B("da0", 0, 16)
+        self._bwd_data_bf16("d0", dout, da0)
+        # d1: input a1 + up(b2), output b1
+        g1 = B("g1", 1, 16)
+        _native.call("warp_amd_up2_bwd_f32", self.side(1), 16, _native.ptr(da0), _native.ptr(g1), s)
+        dy = self._relu_bf16(b1, g1, B("m1", 1, 16))
+        self._wgrad_bf16("d1", a1, dy, coarse=b2)
+        da1 = B("da1", 1, 32)
+        self._bwd_data_bf16("d1", dy, da1)
+        # d2: input a2 + up(a3), output b2
+        g2 = B("g2", 2, 32)
+        _native.call("warp_amd_up2_bwd_f32", self.side(2), 32, _native.ptr(da1), _native.ptr(g2), s)
+        dy = self._relu_bf16(b2, g2, B("m2", 2, 32))
+        self._wgrad_bf16("d2", a2, dy, coarse=a3)
+        da2 = B("da2", 2, 64)
+        self._bwd_data_bf16("d2", dy, da2)
+        # e3: input p2, output a3
+        g3 = B("g3", 3, 64)
+        _native.call("warp_amd_up2_bwd_f32", self.side(3), 64, _native.ptr(da2), _native.ptr(g3), s)
+        dy = self._relu_bf16(a3, g3, B("m3", 3, 64))
+        self._wgrad_bf16("e3", p2, dy)
+        dp2 = B("dp2", 3, 64)
+        self._bwd_data_bf16("e3", dy, dp2)
+        _native.call("warp_amd_pool2_bwd_f32", self.side(2), 64, _native.ptr(dp2), _native.ptr(da2), 1, s)
+        # e2: input p1, output a2
+        dy = self._relu_bf16(a2, da2, B("m2b", 2, 64))
+        self._wgrad_bf16("e2", p1, dy)
+        dp1 = B("dp1", 2, 32)
+        self._bwd_data_bf16("e2", dy, dp1)
+        _native.call("warp_amd_pool2_bwd_f32", self.side(1), 32, _native.ptr(dp1), _native.ptr(da1), 1, s)
+        # e1: input p0, output a1
+        dy = self._relu_bf16(a1, da1, B("m1b", 1, 32))
+        self._wgrad_bf16("e1", p0, dy)
+        dp0 = B("dp0", 1, 16)
+        self._bwd_data_bf16("e1", dy, dp0)
+        _native.call("warp_amd_pool2_bwd_f32", self.side(0), 16, _native.ptr(dp0), _native.ptr(da0), 1, s)
+        # e0: input (BF16 network input), output a0
+        dy = self._relu_bf16(a0, da0, B("m0", 0, 16))
+        self._wgrad_bf16("e0", self._inp_bf16, dy)
 
     def _layer_grads(self, name, x, y, dy, masked):
         """Weight / bias gradients of one layer; returns dy masked by the ReLU."""
@@ -282,6 +387,7 @@ class UNet:
 
     def adam_step(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8) -> None:
         self._wsw = None
+        self._wsw_t = None
         for k, p in self.params.items():
             _native.call("warp_amd_adam_step_f32", p.numel(), _native.ptr(self.grads[k]), _native.ptr(self.m[k]),
                          _native.ptr(self.v[k]), _native.ptr(p), lr, beta1, beta2, float(self.t), eps, self._s())
@@ -424,10 +530,15 @@ class EnergyTrainer:
     (g <- g - omega A D^-1 g), the network backward and Adam.
     """
 
-    def __init__(self, net: UNet, smooth_steps: int = 1, omega: float = 2.0 / 3.0):
+    def __init__(self, net: UNet, smooth_steps: int = 1, omega: float = 2.0 / 3.0, precision: str = "fp32"):
+        if precision not in ("fp32", "bf16"):
+            raise ValueError("precision must be 'fp32' or 'bf16'")
         self.net = net
         self.nu = int(smooth_steps)
         self.omega = float(omega)
+        # "bf16": the network runs forward and backward on MFMA (BF16 operands,
+        # FP32 accumulation, FP32 master weights and Adam); "fp32": FP32 kernels
+        self.precision = precision
         dev = net.dev
         N = net.n**3
         self.N = N
@@ -442,7 +553,7 @@ class EnergyTrainer:
         self.gt = torch.zeros(N, **f64)
         self.h = torch.zeros(N, **f64)
         self.sc = torch.zeros(16, **f64)  # 1 scale, 2 inv, 7 r1.r1, 10 r.z, 11 z.Az, 12 r.e, 13 loss, 14 g.g
-        self.inp = torch.empty(N * 2, dtype=torch.float32, device=dev)
+        self.inp = torch.empty(N * 2, dtype=torch.int16 if precision == "bf16" else torch.float32, device=dev)
         self.dout = torch.empty(N, dtype=torch.float32, device=dev)
         lib = _native.lib()
         self.ws_mv = torch.empty(lib.warp_amd_bsr_residual_workspace_size(N), dtype=torch.uint8, device=dev)
@@ -482,9 +593,14 @@ class EnergyTrainer:
                 self.z, self.zt = self.zt, self.z
             self._residual(system, r, self.z, self.r1, 7)
         _native.call("warp_amd_rms_scales", self._sp(7), self.N, self._sp(1), self._sp(2), s)
-        _native.call("warp_amd_unet_pack_input", self.N, _native.ptr(self.r1), _native.ptr(system.logk_std),
-                     self._sp(2), _native.ptr(self.inp), s)
-        out = self.net.forward(self.inp)
+        if self.precision == "bf16":
+            _native.call("warp_amd_unet_pack_input_bf16", self.N, _native.ptr(self.r1), _native.ptr(system.logk_std),
+                         self._sp(2), _native.ptr(self.inp), s)
+            out = self.net.forward_bf16(self.inp)
+        else:
+            _native.call("warp_amd_unet_pack_input", self.N, _native.ptr(self.r1), _native.ptr(system.logk_std),
+                         self._sp(2), _native.ptr(self.inp), s)
+            out = self.net.forward(self.inp)
         _native.call("warp_amd_unet_unpack_add", self.N, _native.ptr(out), self._sp(1), gain, _native.ptr(self.z), s)
         for _ in range(self.nu):
             self._sweep(system, r, self.z, self.zt)
@@ -544,7 +660,10 @@ class EnergyTrainer:
             self.g, self.gt = self.gt, self.g
         _native.call("warp_amd_scale_to_f32", self.N, _native.ptr(self.g), self._sp(1), 1.0 / system.dmean,
                      _native.ptr(self.dout), s)
-        self.net.backward(self.dout)
+        if self.precision == "bf16":
+            self.net.backward_bf16(self.dout)
+        else:
+            self.net.backward(self.dout)
         return self.sc[13:14]
 
     def loss(self, system: FV7System, r32: torch.Tensor, e32: torch.Tensor) -> float:
@@ -588,7 +707,7 @@ def train_learned_preconditioner(net: UNet, systems: list, smooth_steps: int = 1
                                  pairs_per_solve: int = 10, steps: int = 2000, lr: float = 1e-3,
                                  dagger_rounds: int = 2, dagger_steps: int = 1000, dagger_lr: float = 5e-4,
                                  dagger_iters: int = 40, dagger_every: int = 8, seed: int = 0, log=None,
-                                 batch: int = 1) -> dict:
+                                 batch: int = 1, precision: str = "fp32") -> dict:
     """Train ``net`` as the preconditioner of :class:`LearnedFCG` on ``systems``.
 
     1. (r, e) pairs from one FP64 Jacobi-CG solve per system
@@ -600,7 +719,7 @@ def train_learned_preconditioner(net: UNet, systems: list, smooth_steps: int = 1
        step the trained network stalls FCG after a few iterations.
     Returns dict(pcg_pairs, fcg_pairs, loss_history, seconds)."""
     t0 = time.time()
-    trainer = EnergyTrainer(net, smooth_steps, omega)
+    trainer = EnergyTrainer(net, smooth_steps, omega, precision=precision)
     pairs = []
     x_stars = []
     for si, sysm in enumerate(systems):
@@ -995,6 +1114,8 @@ def main(argv=None) -> int:
     t.add_argument("--dagger-every", type=int, default=8)
     t.add_argument("--eval-seeds", type=int, default=5)
     t.add_argument("--eval-seed0", type=int, default=4000, help="validation seeds (held out from training and test)")
+    t.add_argument("--precision", choices=("fp32", "bf16"), default="fp32",
+                   help="network arithmetic in training: FP32 kernels or BF16 MFMA (FP32 accumulation)")
     t.add_argument("--out", required=True)
     e = sub.add_parser("evaluate", help="learned FCG vs Jacobi-CG on held-out seeds")
     e.add_argument("--ckpt", required=True)
@@ -1020,12 +1141,12 @@ def main(argv=None) -> int:
                                             dagger_rounds=args.dagger_rounds, dagger_steps=args.dagger_steps,
                                             seed=args.init_seed, log=log, lr=args.lr, dagger_lr=args.dagger_lr,
                                             batch=args.batch, dagger_iters=args.dagger_iters,
-                                            dagger_every=args.dagger_every)
+                                            dagger_every=args.dagger_every, precision=args.precision)
         meta = dict(n=args.n, sigma=args.sigma, corr=args.corr, train_seed0=args.seed0, train_seeds=args.train_seeds,
                     smooth_steps=args.smooth_steps, omega=2.0 / 3.0, steps=args.steps, dagger_rounds=args.dagger_rounds,
                     dagger_steps=args.dagger_steps, init_seed=args.init_seed, lr=args.lr, dagger_lr=args.dagger_lr,
                     batch=args.batch, dagger_iters=args.dagger_iters, dagger_every=args.dagger_every,
-                    pcg_pairs=info["pcg_pairs"],
+                    precision=args.precision, pcg_pairs=info["pcg_pairs"],
                     fcg_pairs=info["fcg_pairs"], train_seconds=info["seconds"])
         save_checkpoint(args.out, net, meta)
         log(f"[train] wrote {args.out} after {time.time() - t0:.0f}s")
