@@ -159,12 +159,13 @@ __host__ __device__ constexpr int halo_bytes(int ci) { return ci <= 2 ? 4 : 2 * 
 // Halo voxel offset of tap t relative to the window corner (dx, dy, dz = -1)
 __host__ __device__ constexpr int tap_offset(int t) { return ((t / 9) * kHY + (t / 3) % 3) * kHX + t % 3; }
 
-// LDS layout of conv3_mfma_kernel: [weights of CBG channel blocks | halo | 16 zero bytes]
+// LDS layout of conv3_mfma_kernel: [weights of CBG channel blocks | halo | 16 zero bytes | bias]
 template <int CI, int CBG> struct ConvLds {
     static constexpr int kWeights = k_blocks(CI) * CBG * 1024;
     static constexpr int kHaloOff = kWeights;
     static constexpr int kZeroOff = kHaloOff + kHalo * halo_bytes(CI);
-    static constexpr int kBytes = kZeroOff + 16;
+    static constexpr int kBiasOff = kZeroOff + 16;
+    static constexpr int kBytes = kBiasOff + 64 * CBG;
 };
 
 // The 10 x 10 x 6 input halo of the tile at (X0, Y0, Z0), held in registers
@@ -288,6 +289,14 @@ __global__ __launch_bounds__(kBlock) void conv3_mfma_kernel(
     const int ly = 4 * (w >> 1) + (n >> 2);
     const int lane_base = ly * kHX + lx;  // halo voxel of the window corner in slice 0
     const bf16x8* wl = reinterpret_cast<const bf16x8*>(smem) + lane;
+    // the workgroup's bias values, in LDS: a global load used in the tile
+    // loop makes the epilogue wait for every load in flight (the counter is in
+    // order), the next tile's halo included
+    float* bias_lds = reinterpret_cast<float*>(smem + L::kBiasOff);
+    if (threadIdx.x < 16 * CBG) {
+        const int o = cb0 * 16 + (int)threadIdx.x;
+        bias_lds[threadIdx.x] = (bias && o < CO) ? bias[o] : 0.0f;
+    }
 
     auto origin = [&](int t, int& X0, int& Y0, int& Z0) {
         X0 = (t % tiles_x) * kTX;
@@ -375,10 +384,7 @@ __global__ __launch_bounds__(kBlock) void conv3_mfma_kernel(
 #pragma unroll
     for (int cb = 0; cb < CBG; ++cb) {
         const int o0 = (cb0 + cb) * 16 + 4 * q;
-        float bsv[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-            bsv[i] = (bias && o0 + i < CO) ? bias[o0 + i] : 0.0f;
+        const float4 bsv = *reinterpret_cast<const float4*>(bias_lds + cb * 16 + 4 * q);
         float psum[2][4];
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
@@ -386,7 +392,7 @@ __global__ __launch_bounds__(kBlock) void conv3_mfma_kernel(
             float val[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                val[i] = acc[cb][g][i] + bsv[i];
+                val[i] = acc[cb][g][i] + (i == 0 ? bsv.x : i == 1 ? bsv.y : i == 2 ? bsv.z : bsv.w);
                 if (relu)
                     val[i] = fmaxf(val[i], 0.0f);
             }
