@@ -518,6 +518,11 @@ def extra_learned(seeds: int, dev) -> dict:
                   "the timed FCG solve includes its initial residual, every iteration and the FP64 check",
         "median_fcg_warmup_ms": round(med("fcg_warmup_ms"), 2),
         "checkpoint": os.path.relpath(ckpt, ROOT),
+        "training": {"precision": str(meta.get("precision", "fp32")),
+                     "seconds": round(float(meta.get("train_seconds", 0.0)), 1),
+                     "pairs": {"plain_cg": int(meta.get("pcg_pairs", 0)), "fcg_iterates": int(meta.get("fcg_pairs", 0))},
+                     "adam_steps": int(meta.get("steps", 0)) + int(meta.get("dagger_rounds", 0)) *
+                     int(meta.get("dagger_steps", 0))},
         "unet": {"us_per_application": round(unet_s * 1e6, 1), "gflop_per_application": round(flops / 1e9, 2),
                  "roofline": {"bound": "mfma", "achieved": round(tf, 1), "peak": 2500.0, "unit": "TFLOP/s",
                               "frac": round(tf / 2500.0, 4), "basis": "sum of 2*27*ci*co*voxels over the 7 layers"}},
