@@ -229,6 +229,52 @@ def test_triplets_count_and_reference_named_entry(device, oracle_mod):
     np.testing.assert_array_equal(si[: int(o["summed_offsets"][-1])].cpu().numpy(), o["summed_indices"])
 
 
+@pytest.mark.parametrize("bshape,dtype,prune", [((3, 3), np.float32, True), ((3, 3), np.float64, False),
+                                                 ((), np.float32, True), ((4, 4), np.float32, True)])
+def test_triplets_value_sums_with_summed_arrays(device, oracle_mod, bshape, dtype, prune):
+    """Value sums through the C ABI with the caller's summed offsets / triplet
+    indices asked for: columns, bitwise sums and both summed arrays equal the
+    oracle's; a device count below T, out-of-range rows and zero blocks
+    included."""
+    from warp_amd import _native
+
+    rng = np.random.default_rng(5150 + len(bshape) + (dtype == np.float64))
+    nrow, ncol, T, count = 3000, 2500, 60000, 57001
+    rows = rng.integers(0, nrow, T).astype(np.int32)
+    cols = rng.integers(0, ncol, T).astype(np.int32)
+    rows[rng.random(T) < 0.02] = nrow + 5  # out of range: dropped
+    vals = rng.standard_normal((T, *bshape)).astype(dtype)
+    vals[rng.random(T) < 0.05] = 0.0        # zero blocks: pruned when asked
+    bsz = int(np.prod(bshape)) if bshape else 1
+    sb = np.dtype(dtype).itemsize
+    mask = {4: 0x7FFFFFFF, 8: 0x7FFFFFFFFFFFFFFF}[sb] if prune else 0
+    o = oracle_mod.bsr_from_triplets(nrow, ncol, rows, cols, vals, prune=prune, count=count)
+    lib = _native.lib()
+    d_rows, d_cols, d_vals = _t(rows, device, torch.int32), _t(cols, device, torch.int32), _t(vals, device)
+    d_count = torch.tensor([count], dtype=torch.int32, device=device)
+    ws_bytes = lib.warp_amd_bsr_from_triplets_workspace_size(nrow, T)
+    off = torch.zeros(nrow + 1, dtype=torch.int32, device=device)
+    colsout = torch.full((T,), -1, dtype=torch.int32, device=device)
+    so = torch.full((T,), -1, dtype=torch.int32, device=device)
+    si = torch.full((T,), -1, dtype=torch.int32, device=device)
+    out = torch.full((T, *bshape), float("nan"), dtype=d_vals.dtype, device=device)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=device)
+    print(f"[triplets] C ABI block={bshape} dtype={np.dtype(dtype).name} prune={prune}", flush=True)
+    _native.call(
+        "warp_amd_bsr_from_triplets", bsz, sb, nrow, ncol, T, _native.ptr(d_count), _native.ptr(d_rows),
+        _native.ptr(d_cols), _native.ptr(d_vals), mask, 0, _native.ptr(so), _native.ptr(si), _native.ptr(off),
+        None, _native.ptr(colsout), _native.ptr(out), None, _native.ptr(ws), ws_bytes, _native.stream_of(device),
+    )
+    torch.cuda.synchronize()
+    nnz = o["nnz"]
+    valid = int(o["summed_offsets"][-1])
+    np.testing.assert_array_equal(off.cpu().numpy(), o["offsets"])
+    np.testing.assert_array_equal(colsout[:nnz].cpu().numpy(), o["columns"])
+    np.testing.assert_array_equal(so[:nnz].cpu().numpy(), o["summed_offsets"])
+    np.testing.assert_array_equal(si[:valid].cpu().numpy(), o["summed_indices"])
+    assert out[:nnz].cpu().numpy().tobytes() == o["values"].tobytes(), "summed values differ bitwise"
+
+
 def test_triplets_elasticity_pattern(device, oracle_mod):
     """Element-loop triplets (8x8 blocks per hex), shuffled, random values: the assembly workload at n=8."""
     rows, cols, vals, nn = problems.elasticity_triplets_shuffled(8, device="cpu")

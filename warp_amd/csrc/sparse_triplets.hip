@@ -1054,7 +1054,8 @@ __global__ __launch_bounds__(kWave * kRowsPerBlock) void accumulate_large_rows_k
 // registers, then the row's columns, (optionally) its inclusive run ends and
 // sorted triplet indices, and (optionally) its block sums are written.  A sum
 // adds the run's blocks in ascending triplet order, the first one as the
-// initial value; its loads are issued four at a time ahead of the adds.
+// initial value; the first four blocks of the runs of four (block, scalar)
+// pairs per lane are loaded together ahead of the adds.
 template <typename S, int BS>
 __global__ __launch_bounds__(kWave * kRowsPerBlock) void finalize_small_rows_kernel(
     int nrow, RowRanges rr, const uint64_t* __restrict__ keys,
@@ -1090,44 +1091,82 @@ __global__ __launch_bounds__(kWave * kRowsPerBlock) void finalize_small_rows_ker
         summed_indices[base + lane] = idx;
     if (!bsr_values)
         return;
-    // Cross-lane reads (ds_bpermute) only in wave-uniform control flow: every
-    // loop below runs the same trip count on all lanes.
+    // Run starts by run number (per-wave LDS; entry nu = size), so that any
+    // lane finds the run of its (block, scalar) pair without a scan.
+    __shared__ int run_start[kRowsPerBlock][kWave + 1];
+    int* rs = run_start[threadIdx.x / kWave];
     const int nu = __popcll(heads);
+    if (head)
+        rs[run] = lane;
+    if (lane == 0)
+        rs[nu] = size;
+    __builtin_amdgcn_wave_barrier();
     const uint64_t above = heads & ~below;
     int run_len = head ? (above ? __builtin_ctzll(above) : size) - lane : 0;
 #pragma unroll
     for (int o = 1; o < kWave; o <<= 1)
         run_len = max(run_len, __shfl_xor(run_len, o));
-    for (int t0 = 0; t0 < nu * bs; t0 += kWave) {
-        const int t = t0 + lane;
-        const bool act = t < nu * bs;
-        const int u = act ? t / bs : 0;
-        const int sc = t - u * bs;
-        uint64_t m = heads;
-        for (int i = 0; i < u; ++i)
-            m &= m - 1;
-        const int beg = m ? __builtin_ctzll(m) : 0;
-        m &= m - 1;
-        const int end = m ? __builtin_ctzll(m) : size;
-        const int i0 = __shfl(idx, beg);
-        S acc = act ? tpl_values[(size_t)i0 * bs + sc] : S(0);
-        for (int k = 1; k < run_len; k += 4) {
-            int iq[4];
+    // kFinSlots (block, scalar) pairs per lane at a time, the first kFinDepth
+    // blocks of each of their runs loaded together before any add (one
+    // memory round trip for the rows of C3); longer runs continue four deep.
+    // Cross-lane reads (ds_bpermute) only in wave-uniform control flow.
+    constexpr int kFinSlots = 4;
+    constexpr int kFinDepth = 4;
+    for (int t0 = 0; t0 < nu * bs; t0 += kFinSlots * kWave) {
+        int beg[kFinSlots], end[kFinSlots], sc[kFinSlots], u[kFinSlots];
+        bool act[kFinSlots];
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                iq[q] = __shfl(idx, min(beg + k + q, kWave - 1));
-            S v[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (act && beg + k + q < end)
-                    v[q] = tpl_values[(size_t)iq[q] * bs + sc];
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (act && beg + k + q < end)
-                    acc += v[q];
+        for (int h = 0; h < kFinSlots; ++h) {
+            const int t = t0 + h * kWave + lane;
+            act[h] = t < nu * bs;
+            u[h] = act[h] ? t / bs : 0;
+            sc[h] = t - u[h] * bs;
+            beg[h] = rs[u[h]];
+            end[h] = rs[u[h] + 1];
         }
-        if (act)
-            bsr_values[(size_t)(out0 + u) * bs + sc] = acc;
+        int iq[kFinSlots][kFinDepth];
+#pragma unroll
+        for (int h = 0; h < kFinSlots; ++h)
+#pragma unroll
+            for (int q = 0; q < kFinDepth; ++q)
+                iq[h][q] = __shfl(idx, min(beg[h] + q, kWave - 1));
+        S v[kFinSlots][kFinDepth];
+#pragma unroll
+        for (int h = 0; h < kFinSlots; ++h)
+#pragma unroll
+            for (int q = 0; q < kFinDepth; ++q)
+                v[h][q] = (act[h] && beg[h] + q < end[h]) ? tpl_values[(size_t)iq[h][q] * bs + sc[h]] : S(0);
+        S acc[kFinSlots];
+#pragma unroll
+        for (int h = 0; h < kFinSlots; ++h) {
+            acc[h] = v[h][0];
+#pragma unroll
+            for (int q = 1; q < kFinDepth; ++q)
+                if (beg[h] + q < end[h])
+                    acc[h] += v[h][q];
+        }
+        for (int k = kFinDepth; k < run_len; k += 4) {
+#pragma unroll
+            for (int h = 0; h < kFinSlots; ++h) {
+                int jq[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    jq[q] = __shfl(idx, min(beg[h] + k + q, kWave - 1));
+                S w[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (act[h] && beg[h] + k + q < end[h])
+                        w[q] = tpl_values[(size_t)jq[q] * bs + sc[h]];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (act[h] && beg[h] + k + q < end[h])
+                        acc[h] += w[q];
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < kFinSlots; ++h)
+            if (act[h])
+                bsr_values[(size_t)(out0 + u[h]) * bs + sc[h]] = acc[h];
     }
 }
 
@@ -1148,6 +1187,7 @@ size_t triplets_workspace_bytes(int nrow, int T)
     carve_ws(nullptr, 0, nrow, T, true, &needed);
     return needed + 256;
 }
+
 
 // Grid of the grid-strided medium-row kernels (one wave per listed row): the
 // list length is only known on the device, so the grid covers the worst case
@@ -1797,6 +1837,7 @@ WA_API size_t warp_amd_bsr_from_triplets_workspace_size(int row_count, int nnz)
 {
     return wa::triplets_workspace_bytes(row_count, nnz);
 }
+
 
 WA_API int warp_amd_bsr_from_triplets(
     int block_size, int scalar_size, int row_count, int col_count, int nnz, const int* tpl_nnz, const int* tpl_rows,
