@@ -646,13 +646,13 @@ def main() -> int:
     t0 = time.perf_counter()
     torch.cuda._sleep(2_000_000)
     events[0].record(stream)
-    for b in range(blocks):
-        for _ in range(bounds[b], bounds[b + 1]):
+    for grp in range(blocks):
+        for _ in range(bounds[grp], bounds[grp + 1]):
             sparse.bsr_mv(A, x, y)
-        events[b + 1].record(stream)
+        events[grp + 1].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    per_ms = [events[b].elapsed_time(events[b + 1]) / (bounds[b + 1] - bounds[b]) for b in range(blocks)]
+    per_ms = [events[g].elapsed_time(events[g + 1]) / (bounds[g + 1] - bounds[g]) for g in range(blocks)]
     ms_kernel = float(np.median(per_ms))
     ms_mean = events[0].elapsed_time(events[blocks]) / steps
     ms_step = ms_kernel
