@@ -405,10 +405,21 @@ __global__ __launch_bounds__(kChunkThreads) void chunk_row_hist_kernel(int nrow,
     }
 }
 
-// Scatter of the bucket-ordered keys into row order, one workgroup per chunk:
-// each (chunk, row) reserves its run with one global atomic on fill[row]
-// (zeroed beforehand), keys then take LDS-atomic ranks within the run.  The
-// writes of a chunk stay within its buckets' key ranges.
+// Scatter of the bucket-ordered keys into row order, one workgroup per chunk
+// of kSChunk keys: the chunk's rows are counted in LDS (a window of
+// kSWindow rows from its first bucket), each (chunk, row) run is reserved
+// with one global atomic on fill[row] (zeroed beforehand), and the keys are
+// placed in LDS in destination order -- runs in row order, keys within a run
+// by LDS-atomic rank -- then written out by consecutive threads, so a run
+// leaves as one contiguous store instead of one scattered store per key
+// (rows beyond the window: one atomic per key).  On 16.8 M keys in 7.2 M
+// blocks, keys scattered one by one from registers wrote 672 MB for 134 MB of
+// keys (283 us); staged, 211 MB (114 us).
+constexpr int kSChunk = 8192;
+constexpr int kSPer = kSChunk / kChunkThreads;
+constexpr int kSWindow = 4096;
+constexpr int kSWinPer = kSWindow / kChunkThreads;
+
 __global__ __launch_bounds__(kChunkThreads) void chunk_row_scatter_kernel(int nrow, Partition part,
                                                                          const int* __restrict__ total,
                                                                          const uint64_t* __restrict__ bkeys,
@@ -417,59 +428,89 @@ __global__ __launch_bounds__(kChunkThreads) void chunk_row_scatter_kernel(int nr
                                                                          int* __restrict__ fill,
                                                                          uint64_t* __restrict__ keys)
 {
-    __shared__ int cnt[kRowWindow];
+    __shared__ int gbase[kSWindow];  // global start of the chunk's run of each window row
+    __shared__ int lrank[kSWindow];  // keys per window row, then the running rank
+    __shared__ int loff[kSWindow];   // start of each window row's run in the staged keys
+    __shared__ uint64_t skey[kSChunk];
+    __shared__ int spos[kSChunk];
+    __shared__ int wsum[kChunkThreads / kWave];
     const int n = *total;
-    const int i0 = xcd_chunk() * kChunk;
+    const int i0 = xcd_chunk() * kSChunk;
     if (i0 >= n)
         return;
-    const int i1 = min(i0 + kChunk, n);
+    const int i1 = min(i0 + kSChunk, n);
     const int rlo = (brow[i0] >> part.shift) << part.shift;
-    // the keys themselves are loaded again at the end (from L2): holding them
-    // across the atomics would cost the registers of a second resident workgroup
-    int r[kChunkPer];
+    int r[kSPer];
+    uint64_t k[kSPer];
 #pragma unroll
-    for (int u = 0; u < kChunkPer; ++u) {
+    for (int u = 0; u < kSPer; ++u) {
         const int i = i0 + (int)threadIdx.x + u * kChunkThreads;
         r[u] = i < i1 ? brow[i] : -1;
+        k[u] = i < i1 ? bkeys[i] : 0;
     }
 #pragma unroll
-    for (int q = 0; q < kWindowPer; ++q)
-        cnt[threadIdx.x + q * kChunkThreads] = 0;
+    for (int q = 0; q < kSWinPer; ++q)
+        lrank[threadIdx.x + q * kChunkThreads] = 0;
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < kChunkPer; ++u) {
+    for (int u = 0; u < kSPer; ++u) {
         const int d = r[u] - rlo;
-        if (r[u] >= 0 && d < kRowWindow)
-            atomicAdd(&cnt[d], 1);
+        if (r[u] >= 0 && d < kSWindow)
+            atomicAdd(&lrank[d], 1);
     }
     __syncthreads();
-    int c[kWindowPer], base[kWindowPer];
+    // thread t owns window rows t * kSWinPer ..: their counts, global runs and
+    // the exclusive scan of the counts over the window (thread order = row order)
+    int c[kSWinPer];
+    int tsum = 0;
 #pragma unroll
-    for (int q = 0; q < kWindowPer; ++q) {
-        const int d = threadIdx.x + q * kChunkThreads;
-        c[q] = rlo + d < nrow ? cnt[d] : 0;
-        base[q] = 0;
-        if (c[q])
-            base[q] = row_start[rlo + d] + atomicAdd(&fill[rlo + d], c[q]);
+    for (int q = 0; q < kSWinPer; ++q) {
+        const int d = (int)threadIdx.x * kSWinPer + q;
+        c[q] = rlo + d < nrow ? lrank[d] : 0;
+        tsum += c[q];
     }
+    const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+    int incl = tsum;
 #pragma unroll
-    for (int q = 0; q < kWindowPer; ++q)
-        if (c[q])
-            cnt[threadIdx.x + q * kChunkThreads] = base[q];
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int y = __shfl_up(incl, o);
+        if (lane >= o)
+            incl += y;
+    }
+    if (lane == kWave - 1)
+        wsum[wv] = incl;
     __syncthreads();
-    int pos[kChunkPer];
+    int run = incl - tsum;
+    for (int w = 0; w < wv; ++w)
+        run += wsum[w];
 #pragma unroll
-    for (int u = 0; u < kChunkPer; ++u) {
-        pos[u] = -1;
-        if (r[u] >= 0) {
-            const int d = r[u] - rlo;
-            pos[u] = d < kRowWindow ? atomicAdd(&cnt[d], 1) : row_start[r[u]] + atomicAdd(&fill[r[u]], 1);
+    for (int q = 0; q < kSWinPer; ++q) {
+        const int d = (int)threadIdx.x * kSWinPer + q;
+        loff[d] = run;
+        run += c[q];
+        gbase[d] = c[q] ? row_start[rlo + d] + atomicAdd(&fill[rlo + d], c[q]) : 0;
+        lrank[d] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kSPer; ++u) {
+        if (r[u] < 0)
+            continue;
+        const int d = r[u] - rlo;
+        if (d < kSWindow) {
+            const int rk = atomicAdd(&lrank[d], 1);
+            skey[loff[d] + rk] = k[u];
+            spos[loff[d] + rk] = gbase[d] + rk;
+        } else {
+            keys[row_start[r[u]] + atomicAdd(&fill[r[u]], 1)] = k[u];
         }
     }
-#pragma unroll
-    for (int u = 0; u < kChunkPer; ++u)
-        if (pos[u] >= 0)
-            keys[pos[u]] = bkeys[i0 + (int)threadIdx.x + u * kChunkThreads];
+    __syncthreads();
+    int staged = 0;
+    for (int w = 0; w < kChunkThreads / kWave; ++w)
+        staged += wsum[w];
+    for (int i = threadIdx.x; i < staged; i += kChunkThreads)
+        keys[spos[i]] = skey[i];
 }
 
 // In-LDS bitonic sort of `n` (power of two) keys by `nthreads` threads.
@@ -1374,8 +1415,9 @@ int bucket_sort_rows(
         WA_LAUNCH_CHECK();
         WA_TRY(exclusive_scan_i32(w.hist, w.row_start, nrow, w.scan_ws, stream));
         WA_TRY(launch_fill(w.hist, (int64_t)nrow + 1, 0, stream));
-        chunk_row_scatter_kernel<<<chunks, kChunkThreads, 0, stream>>>(nrow, pt, total, w.alt, w.brow, w.row_start,
-                                                                       w.hist, w.keys);
+        const unsigned schunks = (unsigned)(ceil_div(ceil_div(T, kSChunk), kXcds) * kXcds);
+        chunk_row_scatter_kernel<<<schunks, kChunkThreads, 0, stream>>>(nrow, pt, total, w.alt, w.brow, w.row_start,
+                                                                        w.hist, w.keys);
         WA_LAUNCH_CHECK();
         return sort_rows(nrow, RowRanges{w.row_start, nullptr}, w, summed_indices, urow, write_back, stream);
     }
