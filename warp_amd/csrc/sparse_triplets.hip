@@ -586,6 +586,21 @@ __device__ __forceinline__ uint64_t wave_sort64(uint64_t key, int lane)
     return key;
 }
 
+// Bitonic sort of one 32-bit value per lane across the wavefront (ascending by lane).
+__device__ __forceinline__ unsigned wave_sort32(unsigned v, int lane)
+{
+#pragma unroll
+    for (int k = 2; k <= kWave; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const unsigned other = (unsigned)__shfl_xor((int)v, j);
+            const bool up = (lane & k) == 0;
+            const bool lower = (lane & j) == 0;
+            v = (lower == up) ? (v < other ? v : other) : (v < other ? other : v);
+        }
+    return v;
+}
+
 // Rows of at most one key per lane: a row's keys, sorted in registers.
 __device__ __forceinline__ uint64_t load_sorted_row(const uint64_t* keys, int base, int size, int lane)
 {
@@ -656,17 +671,30 @@ __global__ __launch_bounds__(kWave * kRowsPerBlock) void count_small_rows_kernel
     const int size = rr.size(row);
     if (size > kWave)
         return;
-    const uint64_t key = load_sorted_row(keys, base, size, lane);
-    const int col = (int)(key >> 32);
-    const int prev = __shfl_up(col, 1);
+    if (write_back) {
+        const uint64_t key = load_sorted_row(keys, base, size, lane);
+        const int col = (int)(key >> 32);
+        const int prev = __shfl_up(col, 1);
+        const bool head = lane < size && (lane == 0 || prev != col);
+        const int n = __popcll(__ballot(head));
+        if (lane == 0)
+            urow[row] = n;
+        if (lane < size) {
+            keys[base + lane] = key;
+            summed_indices[base + lane] = (int)(uint32_t)key;
+        }
+        return;
+    }
+    // the count needs the columns only: sort their 32 bits (half the exchanges
+    // of the 64-bit keys), past the row's end an all-ones column above every
+    // real one
+    const unsigned* khi = reinterpret_cast<const unsigned*>(keys);
+    const unsigned col = wave_sort32(lane < size ? khi[2 * (size_t)(base + lane) + 1] : 0xffffffffu, lane);
+    const unsigned prev = (unsigned)__shfl_up((int)col, 1);
     const bool head = lane < size && (lane == 0 || prev != col);
     const int n = __popcll(__ballot(head));
     if (lane == 0)
         urow[row] = n;
-    if (write_back && lane < size) {
-        keys[base + lane] = key;
-        summed_indices[base + lane] = (int)(uint32_t)key;
-    }
 }
 
 // Medium rows (kWave + 1 .. kMediumCap keys) live in R = 2, 4 or 8 registers
