@@ -107,6 +107,8 @@ size_t scan_workspace_bytes(int64_t n);
 // p[0..n) = v on `stream` (a kernel, never a memset node)
 int launch_fill_i32(int* p, int64_t n, int v, hipStream_t stream);
 int exclusive_scan_i32(const int* in, int* out, int64_t n, void* ws, hipStream_t stream);
+// The same with `in` (distinct from `out`) left zeroed: out = scan(in), in = 0.
+int exclusive_scan_i32_clear(int* in, int* out, int64_t n, void* ws, hipStream_t stream);
 
 // ---------------------------------------------------------------------------
 // Device helpers

@@ -56,6 +56,10 @@ namespace {
 constexpr int kScanBlock = 256;
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kScanBlock * kScanItems;
+// up to this many tiles (1,048,576 items) each tile pass workgroup adds the
+// raw tile sums before its own (at most 4 KB of L2 reads) instead of a third
+// launch scanning them
+constexpr int64_t kScanRawTiles = 1024;
 
 __global__ __launch_bounds__(kScanBlock) void scan_reduce_kernel(const int* in, int64_t n, int* partials)
 {
@@ -101,18 +105,35 @@ __global__ __launch_bounds__(1024) void scan_partials_kernel(int* partials, int6
         *total = carry;
 }
 
+// RAW: `partials` hold the tiles' plain sums; each workgroup adds those of
+// the tiles before it (few tiles: this replaces the one-workgroup scan of the
+// partials) and the last one writes the total to out[n].  Otherwise
+// `partials` hold the scanned tile offsets.
+// CLEAR: zero `in` once the tile is loaded (in != out).
+template <bool RAW, bool CLEAR>
 __global__ __launch_bounds__(kScanBlock) void scan_tile_kernel(
-    const int* in, int* out, int64_t n, const int* partials
+    int* in, int* out, int64_t n, const int* partials
 )
 {
     __shared__ int lds[kScanTile];
     __shared__ int wsum[kScanBlock];
     const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    int prefix = 0;
+    if constexpr (RAW) {
+        int a = 0;
+        for (int k = threadIdx.x; k < (int)blockIdx.x; k += kScanBlock)
+            a += partials[k];
+        prefix = block_sum<kScanBlock>(a, wsum);
+    } else {
+        prefix = partials[blockIdx.x];
+    }
     // load the tile (coalesced), each thread then owns kScanItems consecutive items
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         int64_t i = base + (int64_t)k * kScanBlock + threadIdx.x;
         lds[k * kScanBlock + threadIdx.x] = i < n ? in[i] : 0;
+        if (CLEAR && i < n)
+            in[i] = 0;
     }
     __syncthreads();
     int local[kScanItems];
@@ -130,7 +151,9 @@ __global__ __launch_bounds__(kScanBlock) void scan_tile_kernel(
         wsum[threadIdx.x] += t;
         __syncthreads();
     }
-    const int excl = wsum[threadIdx.x] - run + partials[blockIdx.x];
+    const int excl = wsum[threadIdx.x] - run + prefix;
+    if (RAW && blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanBlock - 1)
+        out[n] = excl + run;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k)
         lds[threadIdx.x * kScanItems + k] = excl + local[k];
@@ -169,7 +192,10 @@ size_t scan_workspace_bytes(int64_t n)
     return align_up((size_t)tiles * sizeof(int), 256);
 }
 
-int exclusive_scan_i32(const int* in, int* out, int64_t n, void* ws, hipStream_t stream)
+namespace {
+
+template <bool CLEAR>
+int scan_i32(int* in, int* out, int64_t n, void* ws, hipStream_t stream)
 {
     if (n <= 0) {
         // a kernel rather than a memset node, so the call can be graph-captured
@@ -182,11 +208,30 @@ int exclusive_scan_i32(const int* in, int* out, int64_t n, void* ws, hipStream_t
     // The reduce pass reads `in` before the tile pass overwrites it, so in-place use is safe.
     scan_reduce_kernel<<<(unsigned)tiles, kScanBlock, 0, stream>>>(in, n, partials);
     WA_LAUNCH_CHECK();
+    if (tiles <= kScanRawTiles) {
+        scan_tile_kernel<true, CLEAR><<<(unsigned)tiles, kScanBlock, 0, stream>>>(in, out, n, partials);
+        WA_LAUNCH_CHECK();
+        return kOk;
+    }
     scan_partials_kernel<<<1, 1024, 0, stream>>>(partials, tiles, out + n);
     WA_LAUNCH_CHECK();
-    scan_tile_kernel<<<(unsigned)tiles, kScanBlock, 0, stream>>>(in, out, n, partials);
+    scan_tile_kernel<false, CLEAR><<<(unsigned)tiles, kScanBlock, 0, stream>>>(in, out, n, partials);
     WA_LAUNCH_CHECK();
     return kOk;
+}
+
+}  // namespace
+
+int exclusive_scan_i32(const int* in, int* out, int64_t n, void* ws, hipStream_t stream)
+{
+    // read-only use of `in` (CLEAR is off), which may alias `out`
+    return scan_i32<false>(const_cast<int*>(in), out, n, ws, stream);
+}
+
+int exclusive_scan_i32_clear(int* in, int* out, int64_t n, void* ws, hipStream_t stream)
+{
+    WA_REQUIRE(in != out, "exclusive_scan_i32_clear: input and output must differ");
+    return scan_i32<true>(in, out, n, ws, stream);
 }
 
 }  // namespace wa

@@ -91,6 +91,9 @@ Partition partition_of(int nrow, int T)
     return p;
 }
 
+// row-length list counters (two used) in the workspace
+constexpr int kListCounters = 64;
+
 struct TripletWs {
     int* trow;
     int* hist;
@@ -123,7 +126,7 @@ TripletWs carve_ws(void* base, size_t bytes, int nrow, int T, bool own_summed, s
     w.tile_offsets = c.take<int>((size_t)ntile_hist);
     w.large_rows = c.take<int>((size_t)nrow + 1);
     w.medium_rows = c.take<int>((size_t)nrow + 1);
-    w.counters = c.take<int>(64);
+    w.counters = c.take<int>(kListCounters);
     w.scan_ws = c.take<char>(scan_workspace_bytes(std::max<int64_t>((int64_t)nrow + 1, ntile_hist)));
     w.summed_offsets = own_summed ? c.take<int>((size_t)T) : nullptr;
     w.summed_indices = own_summed ? c.take<int>((size_t)T) : nullptr;
@@ -218,17 +221,35 @@ __global__ __launch_bounds__(kBlock) void tile_classify_kernel(
     const int limit = tpl_count ? *tpl_count : T;
     const int64_t t0 = (int64_t)blockIdx.x * part.tile;
     const int64_t t1 = t0 + part.tile < T ? t0 + part.tile : T;
-    for (int64_t t = t0 + threadIdx.x; t < t1; t += kBlock) {
-        const int r = rows[t];
-        const int c = cols[t];
-        bool valid = t < limit && r >= 0 && r < nrow && c >= 0 && c < ncol;
-        if (valid && mask_offsets)
-            valid = in_mask(r, c, mask_offsets, mask_row_counts, mask_columns);
-        if (valid && values)
-            valid = !block_is_zero(values + (size_t)t * block_size, block_size, zero_mask);
-        trow[t] = valid ? r : -1;
-        if (valid)
-            atomicAdd(&hist[r >> part.shift], 1);
+    // kClassifyBatch triplets per thread per step, their rows, columns and the
+    // first scalar of each block loaded together (one memory round trip); the
+    // rest of a block is read only when its first scalar is zero
+    constexpr int kClassifyBatch = 4;
+    for (int64_t tb = t0 + threadIdx.x; tb < t1; tb += kClassifyBatch * kBlock) {
+        int r[kClassifyBatch], c[kClassifyBatch];
+        U v0[kClassifyBatch];
+#pragma unroll
+        for (int q = 0; q < kClassifyBatch; ++q) {
+            const int64_t t = tb + (int64_t)q * kBlock;
+            const bool in = t < t1;
+            r[q] = in ? rows[t] : -1;
+            c[q] = in ? cols[t] : -1;
+            v0[q] = (in && values) ? values[(size_t)t * block_size] : U(0);
+        }
+#pragma unroll
+        for (int q = 0; q < kClassifyBatch; ++q) {
+            const int64_t t = tb + (int64_t)q * kBlock;
+            if (t >= t1)
+                break;
+            bool valid = t < limit && r[q] >= 0 && r[q] < nrow && c[q] >= 0 && c[q] < ncol;
+            if (valid && mask_offsets)
+                valid = in_mask(r[q], c[q], mask_offsets, mask_row_counts, mask_columns);
+            if (valid && values && (v0[q] & zero_mask) == U(0))
+                valid = !block_is_zero(values + (size_t)t * block_size, block_size, zero_mask);
+            trow[t] = valid ? r[q] : -1;
+            if (valid)
+                atomicAdd(&hist[r[q] >> part.shift], 1);
+        }
     }
     __syncthreads();
     for (int b = threadIdx.x; b < part.nb; b += kBlock)
@@ -240,9 +261,13 @@ __global__ __launch_bounds__(kBlock) void tile_classify_kernel(
 // tile, ranks taken with LDS atomics from the scanned tile histograms.
 __global__ __launch_bounds__(kBlock) void tile_scatter_kernel(
     int T, const int* __restrict__ trow, const int* __restrict__ cols, Partition part,
-    const int* __restrict__ tile_offsets, uint64_t* __restrict__ bkeys, int* __restrict__ brow
+    const int* __restrict__ tile_offsets, uint64_t* __restrict__ bkeys, int* __restrict__ brow,
+    int* __restrict__ zero, int64_t nzero
 )
 {
+    // zero[0 .. nzero): the row histogram of the next pass (no fill launch)
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nzero; i += (int64_t)gridDim.x * kBlock)
+        zero[i] = 0;
     __shared__ int cursor[kMaxBuckets];
     for (int b = threadIdx.x; b < part.nb; b += kBlock)
         cursor[b] = tile_offsets[(size_t)b * part.ntiles + blockIdx.x];
@@ -262,31 +287,40 @@ __global__ __launch_bounds__(kBlock) void tile_scatter_kernel(
 // Tiled scatter with LDS staging (one workgroup per tile, nb <= 256 buckets):
 // each sub-tile of kSubTile triplets is ranked per bucket in LDS, placed in
 // LDS in bucket order, then copied out so that consecutive threads write
-// consecutive keys of one bucket's run.
-__global__ __launch_bounds__(kBlock) void tile_scatter_staged_kernel(
+// consecutive keys of one bucket's run.  1024 threads per workgroup, four
+// triplets each per sub-tile: the sub-tile's chain of LDS phases is bound by
+// latency, which more waves per sub-tile hide (C3: 194 us with 256 threads,
+// 104 with 1024).
+constexpr int kStagedThreads = 1024;
+__global__ __launch_bounds__(kStagedThreads) void tile_scatter_staged_kernel(
     int T, const int* __restrict__ trow, const int* __restrict__ cols, Partition part,
-    const int* __restrict__ tile_offsets, uint64_t* __restrict__ bkeys, int* __restrict__ brow
+    const int* __restrict__ tile_offsets, uint64_t* __restrict__ bkeys, int* __restrict__ brow,
+    int* __restrict__ zero, int64_t nzero
 )
 {
-    constexpr int PER = kSubTile / kBlock;
+    // zero[0 .. nzero): the row histogram of the next pass (no fill launch)
+    for (int64_t i = (int64_t)blockIdx.x * kStagedThreads + threadIdx.x; i < nzero;
+         i += (int64_t)gridDim.x * kStagedThreads)
+        zero[i] = 0;
+    constexpr int PER = kSubTile / kStagedThreads;
     __shared__ int cursor[kStagedBuckets];
     __shared__ int lcnt[kStagedBuckets];
     __shared__ int loff[kStagedBuckets];
     __shared__ uint64_t skey[kSubTile];
     __shared__ int srow[kSubTile];
     const int tid = threadIdx.x;
-    for (int b = tid; b < part.nb; b += kBlock)
+    for (int b = tid; b < part.nb; b += kStagedThreads)
         cursor[b] = tile_offsets[(size_t)b * part.ntiles + blockIdx.x];
     const int64_t t0 = (int64_t)blockIdx.x * part.tile;
     const int64_t t1 = t0 + part.tile < T ? t0 + part.tile : T;
     for (int64_t sub = t0; sub < t1; sub += kSubTile) {
-        for (int b = tid; b < kStagedBuckets; b += kBlock)
+        for (int b = tid; b < kStagedBuckets; b += kStagedThreads)
             lcnt[b] = 0;
         __syncthreads();
         int r[PER], c[PER], rank[PER];
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
-            const int64_t t = sub + k * kBlock + tid;
+            const int64_t t = sub + k * kStagedThreads + tid;
             r[k] = t < t1 ? trow[t] : -1;
             c[k] = r[k] >= 0 ? cols[t] : 0;
             rank[k] = r[k] >= 0 ? atomicAdd(&lcnt[r[k] >> part.shift], 1) : 0;
@@ -321,13 +355,13 @@ __global__ __launch_bounds__(kBlock) void tile_scatter_staged_kernel(
             if (r[k] < 0)
                 continue;
             const int pos = loff[r[k] >> part.shift] + rank[k];
-            const int64_t t = sub + k * kBlock + tid;
+            const int64_t t = sub + k * kStagedThreads + tid;
             skey[pos] = ((uint64_t)(uint32_t)c[k] << 32) | (uint32_t)t;
             srow[pos] = r[k];
         }
         __syncthreads();
         const int n = loff[kStagedBuckets - 1] + lcnt[kStagedBuckets - 1];
-        for (int i = tid; i < n; i += kBlock) {
+        for (int i = tid; i < n; i += kStagedThreads) {
             const int row = srow[i];
             const int b = row >> part.shift;
             const int g = cursor[b] + (i - loff[b]);
@@ -335,7 +369,7 @@ __global__ __launch_bounds__(kBlock) void tile_scatter_staged_kernel(
             brow[g] = row;
         }
         __syncthreads();
-        for (int b = tid; b < part.nb; b += kBlock)
+        for (int b = tid; b < part.nb; b += kStagedThreads)
             cursor[b] += lcnt[b];
     }
 }
@@ -426,8 +460,12 @@ __global__ __launch_bounds__(kChunkThreads) void chunk_row_scatter_kernel(int nr
                                                                          const int* __restrict__ brow,
                                                                          const int* __restrict__ row_start,
                                                                          int* __restrict__ fill,
-                                                                         uint64_t* __restrict__ keys)
+                                                                         uint64_t* __restrict__ keys,
+                                                                         int* __restrict__ counters)
 {
+    // the row-length list counters of the next pass start at zero (no fill launch)
+    if (blockIdx.x == 0 && threadIdx.x < kListCounters)
+        counters[threadIdx.x] = 0;
     __shared__ int gbase[kSWindow];  // global start of the chunk's run of each window row
     __shared__ int lrank[kSWindow];  // keys per window row, then the running rank
     __shared__ int loff[kSWindow];   // start of each window row's run in the staged keys
@@ -1276,9 +1314,10 @@ unsigned medium_grid(int nrow)
 // rows are sorted in registers and, unless write_back, left unsorted in
 // memory (the finalize pass sorts them again).
 int sort_rows(int nrow, RowRanges rr, const TripletWs& w, int* summed_indices, int* urow, bool write_back,
-              hipStream_t stream)
+              hipStream_t stream, bool counters_zeroed = false)
 {
-    WA_TRY(launch_fill(w.counters, 64, 0, stream));
+    if (!counters_zeroed)
+        WA_TRY(launch_fill(w.counters, kListCounters, 0, stream));
     if (nrow > 0) {
         classify_rows_kernel<<<(unsigned)ceil_div(nrow, kBlock), kBlock, 0, stream>>>(
             nrow, rr, w.large_rows, w.medium_rows, w.counters
@@ -1429,25 +1468,27 @@ int bucket_sort_rows(
         }
         WA_LAUNCH_CHECK();
         WA_TRY(exclusive_scan_i32(w.tile_offsets, w.tile_offsets, (int64_t)pt.nb * pt.ntiles, w.scan_ws, stream));
+        // both scatters also zero the row histogram (w.hist, nrow + 1 entries)
         if (pt.staged)
-            tile_scatter_staged_kernel<<<tiles, kBlock, 0, stream>>>(T, w.trow, cols, pt, w.tile_offsets, w.alt,
-                                                                     w.brow);
+            tile_scatter_staged_kernel<<<tiles, kStagedThreads, 0, stream>>>(T, w.trow, cols, pt, w.tile_offsets, w.alt,
+                                                                             w.brow, w.hist, (int64_t)nrow + 1);
         else
-            tile_scatter_kernel<<<tiles, kBlock, 0, stream>>>(T, w.trow, cols, pt, w.tile_offsets, w.alt, w.brow);
+            tile_scatter_kernel<<<tiles, kBlock, 0, stream>>>(T, w.trow, cols, pt, w.tile_offsets, w.alt, w.brow, w.hist,
+                                                              (int64_t)nrow + 1);
         WA_LAUNCH_CHECK();
         // rows within the buckets: chunked LDS histograms -> scan -> chunked scatter
         const int* total = w.tile_offsets + (size_t)pt.nb * pt.ntiles;
         const unsigned chunks = (unsigned)(ceil_div(ceil_div(T, kChunk), kXcds) * kXcds);
-        WA_TRY(launch_fill(w.hist, (int64_t)nrow + 1, 0, stream));
         chunk_row_hist_kernel<<<chunks, kChunkThreads, 0, stream>>>(nrow, pt, total, w.brow, w.hist);
         WA_LAUNCH_CHECK();
-        WA_TRY(exclusive_scan_i32(w.hist, w.row_start, nrow, w.scan_ws, stream));
-        WA_TRY(launch_fill(w.hist, (int64_t)nrow + 1, 0, stream));
+        // row starts; the histogram is left zeroed for the scatter's run reservations
+        WA_TRY(exclusive_scan_i32_clear(w.hist, w.row_start, nrow, w.scan_ws, stream));
         const unsigned schunks = (unsigned)(ceil_div(ceil_div(T, kSChunk), kXcds) * kXcds);
         chunk_row_scatter_kernel<<<schunks, kChunkThreads, 0, stream>>>(nrow, pt, total, w.alt, w.brow, w.row_start,
-                                                                        w.hist, w.keys);
+                                                                        w.hist, w.keys, w.counters);
         WA_LAUNCH_CHECK();
-        return sort_rows(nrow, RowRanges{w.row_start, nullptr}, w, summed_indices, urow, write_back, stream);
+        return sort_rows(nrow, RowRanges{w.row_start, nullptr}, w, summed_indices, urow, write_back, stream,
+                         /*counters_zeroed=*/true);
     }
     WA_TRY(launch_fill(w.hist, (int64_t)nrow + 1, 0, stream));
     if (T > 0) {

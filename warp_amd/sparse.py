@@ -1784,12 +1784,14 @@ def _vec_view(array: torch.Tensor, torch_dtype: torch.dtype, expected_scalar_cou
 class _MvPlan:
     """A matrix's constant arguments of bsr_mv, packed once (host struct of
     warp_amd_bsr_mv_packed), with what its x and y must match.  Valid for one
-    topology version, one ``values`` tensor and one nnz.  ``args.sweep``
-    flips after every product: consecutive products with one matrix traverse
-    its rows in opposite directions (same bits), so each starts with the rows
-    the previous one read last, still in the memory-side cache."""
+    topology version, one ``values`` tensor and one nnz.  Two copies of the
+    struct differ in ``sweep`` only and are used in turn: consecutive products
+    with one matrix traverse its rows in opposite directions (same bits), so
+    each starts with the rows the previous one read last, still in the
+    memory-side cache."""
 
-    __slots__ = ("version", "values", "nnz", "dtype", "nx", "ny", "device", "dev_index", "args", "args_ref", "fn")
+    __slots__ = ("version", "values", "nnz", "dtype", "nx", "ny", "device", "dev_index", "args", "refs", "turn",
+                 "fn")
 
     def __init__(self, A: "BsrMatrix", nnz_hint: int, torch_dtype: torch.dtype, nx: int, ny: int):
         br, bc = A.block_shape
@@ -1809,8 +1811,12 @@ class _MvPlan:
         a.nnz_hint = int(nnz_hint)
         a.lanes, a.rows_per_group = 0, 0
         a.scalar_size = 8 if torch_dtype == torch.float64 else 4
-        self.args = a
-        self.args_ref = ctypes.byref(a)
+        a.sweep = 0
+        b = _native.BsrMvArgs.from_buffer_copy(a)
+        b.sweep = 1
+        self.args = (a, b)
+        self.refs = (ctypes.byref(a), ctypes.byref(b))
+        self.turn = 0
         self.fn = _native.lib().warp_amd_bsr_mv_packed
 
 
@@ -1855,10 +1861,11 @@ def bsr_mv(
         ):
             xp, yp = x.data_ptr(), y.data_ptr()
             if xp != yp:
-                st = plan.fn(plan.args_ref, xp, yp, float(alpha), float(beta), _native.stream_of(plan.device))
+                k = plan.turn
+                plan.turn = k ^ 1
+                st = plan.fn(plan.refs[k], xp, yp, float(alpha), float(beta), _native.stream_of(plan.device))
                 if st:
                     _native.check(st, "warp_amd_bsr_mv_packed")
-                plan.args.sweep ^= 1
                 return y
     A, A_scale = _extract_matrix_and_scale(A)
     alpha = float(alpha) * A_scale
@@ -1937,11 +1944,12 @@ def bsr_mv(
     if tile_size == 0 and type(A) is BsrMatrix and not A.__dict__.get("_nnz_pending", False):
         plan = _MvPlan(A, nnz_hint, torch_dtype, x_view.numel(), y_view.numel())
         A.__dict__["_mv_plan"] = plan
+        k = plan.turn
+        plan.turn = k ^ 1
         _native.check(
-            plan.fn(plan.args_ref, x_view.data_ptr(), y_view.data_ptr(), alpha, beta, _native.stream_of(plan.device)),
+            plan.fn(plan.refs[k], x_view.data_ptr(), y_view.data_ptr(), alpha, beta, _native.stream_of(plan.device)),
             "warp_amd_bsr_mv_packed",
         )
-        plan.args.sweep ^= 1
         return y
     # tile_size > 0: lanes per block row; < 0: one lane per row (row-sequential
     # sums, the reference row kernel's order); 0: chosen from the mean row length
