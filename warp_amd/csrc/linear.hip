@@ -267,7 +267,7 @@ template <typename S> struct NextDirection {
 template <typename S, int BR, int G, bool FUSED, int kCgRpg = rows_per_group_for(BR, BR)>
 __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(
     int nrow, const int* __restrict__ offsets, const int* __restrict__ columns, const S* __restrict__ values,
-    const S* __restrict__ z, S* __restrict__ Ap, int upd_parts, CgWs ws
+    const S* __restrict__ z, S* __restrict__ Ap, int upd_parts, CgWs ws, double* __restrict__ parts
 )
 {
     __shared__ double lds[kBlock / kWave];
@@ -332,7 +332,21 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(
             }
         }
     }
-    write_block_partials<kBlock, 1>(acc, ws.spmv_partials, lds);
+    write_block_partials<kBlock, 1>(acc, parts, lds);
+}
+
+// Wide SpMV grids (one workgroup per tile, more than kMaxSpmvGrid): their
+// partials summed into spmv_partials[0] in a fixed order, read by the update
+// as a single partial.
+__global__ __launch_bounds__(kFinalizeBlock) void cg_wide_spmv_finalize_kernel(const double* __restrict__ parts,
+                                                                              int nparts, CgWs ws)
+{
+    __shared__ double lds[kFinalizeBlock / kWave];
+    if (ws.state->done)
+        return;
+    const double t = sum_partials<kFinalizeBlock>(parts, nparts, lds);
+    if (threadIdx.x == 0)
+        ws.spmv_partials[0] = t;
 }
 
 // x += alpha p, r -= alpha Ap, z = M r, partials of (r.r, r.z).  The
@@ -534,10 +548,19 @@ template <typename S> struct CgLauncher {
     {
         return (unsigned)ceil_div(pb.nrow, (kBlock / G) * rows_per_group_for(BR, BR));
     }
+    // Very large unfused systems: one workgroup per tile, the partials then
+    // reduced by a one-workgroup kernel (256^3 7-point operator, 65,536 tiles:
+    // FP32 inner CG iteration 423 -> 400 us); otherwise at most kMaxSpmvGrid
+    // persistent workgroups whose partials the update sums in its prologue
+    // (C2, 17,164 tiles: 66 us per iteration, 71 with the wide grid).
+    static constexpr unsigned kWideSpmvTiles = 16u * kMaxSpmvGrid;
+    template <int BR, int G> bool wide_spmv() const { return !fused && rows_grid<BR, G>() > kWideSpmvTiles; }
     template <int BR, int G> unsigned spmv_grid() const
     {
-        return std::min<unsigned>(rows_grid<BR, G>(), (unsigned)kMaxSpmvGrid);
+        return wide_spmv<BR, G>() ? rows_grid<BR, G>() : std::min<unsigned>(rows_grid<BR, G>(), (unsigned)kMaxSpmvGrid);
     }
+    // partials the update reads
+    template <int BR, int G> unsigned spmv_parts() const { return wide_spmv<BR, G>() ? 1u : spmv_grid<BR, G>(); }
     static unsigned vec_grid(int64_t count)
     {
         return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(count, kBlock), kMaxVecGrid));
@@ -562,21 +585,31 @@ template <typename S> struct CgLauncher {
     template <int BR, int G> int spmv_t()
     {
         const unsigned grid = spmv_grid<BR, G>();
-        if (fused)
+        if (fused) {
             cg_spmv_kernel<S, BR, G, true><<<grid, kBlock, 0, stream>>>(
-                pb.nrow, pb.offsets, pb.columns, (const S*)pb.values, z_vec(), (S*)ws.Ap, (int)upd_grid(), ws
+                pb.nrow, pb.offsets, pb.columns, (const S*)pb.values, z_vec(), (S*)ws.Ap, (int)upd_grid(), ws,
+                ws.spmv_partials
             );
-        else
-            cg_spmv_kernel<S, BR, G, false><<<grid, kBlock, 0, stream>>>(
-                pb.nrow, pb.offsets, pb.columns, (const S*)pb.values, nullptr, (S*)ws.Ap, (int)upd_grid(), ws
-            );
+            WA_LAUNCH_CHECK();
+            return kOk;
+        }
+        // the wide grid's partials go to the initial residual's partial buffer
+        // (unused between restarts; it holds one per workgroup of any row grid)
+        double* parts = wide_spmv<BR, G>() ? ws.init_partials : ws.spmv_partials;
+        cg_spmv_kernel<S, BR, G, false><<<grid, kBlock, 0, stream>>>(
+            pb.nrow, pb.offsets, pb.columns, (const S*)pb.values, nullptr, (S*)ws.Ap, (int)upd_grid(), ws, parts
+        );
         WA_LAUNCH_CHECK();
+        if (wide_spmv<BR, G>()) {
+            cg_wide_spmv_finalize_kernel<<<1, kFinalizeBlock, 0, stream>>>(parts, (int)grid, ws);
+            WA_LAUNCH_CHECK();
+        }
         return kOk;
     }
     template <int BR, int PREC> int update_t()
     {
         unsigned spmv_parts = 0;
-#define WA_SPMV_PARTS(BR_, G) spmv_parts = spmv_grid<BR_, G>(); break;
+#define WA_SPMV_PARTS(BR_, G) spmv_parts = this->template spmv_parts<BR_, G>(); break;
         switch (g()) {
         case 1: WA_SPMV_PARTS(BR, 1)
         case 2: WA_SPMV_PARTS(BR, 2)
