@@ -498,6 +498,23 @@ def extra_learned(seeds: int, dev) -> dict:
     e1.record()
     torch.cuda.synchronize()
     unet_s = e0.elapsed_time(e1) / 20 * 1e-3
+    # the same forward replayed from a HIP graph, as inside the captured FCG iterations
+    graph = torch.cuda.CUDAGraph()
+    cap = torch.cuda.Stream(dev)
+    cap.wait_stream(torch.cuda.current_stream(dev))
+    with torch.cuda.stream(cap):
+        with torch.cuda.graph(graph, stream=cap):
+            net.forward_bf16(inp)
+    torch.cuda.current_stream(dev).wait_stream(cap)
+    for _ in range(3):
+        graph.replay()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(20):
+        graph.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    unet_graph_s = e0.elapsed_time(e1) / 20 * 1e-3
     flops = unet_flops(128)
     tf = flops / unet_s / 1e12
     # CPU baseline: the oracle's serial FP64 Jacobi-CG on the first held-out system
@@ -511,7 +528,8 @@ def extra_learned(seeds: int, dev) -> dict:
     spmv64 = int(sysm.A.nnz) * 12 + 4 * (N5 + 1) + 16 * N5
     iter64 = cg_iter_bytes(spmv64, N5, 8)
     ms_it64 = med("jacobi64_ms") / max(med("jacobi64_it"), 1)
-    log(f"[bench] U-Net forward 128^3 (BF16 MFMA, 7 launches): {unet_s * 1e6:.0f} us, {flops / 1e9:.1f} GFLOP, "
+    log(f"[bench] U-Net forward 128^3 (BF16 MFMA, 7 launches): {unet_s * 1e6:.0f} us eager, "
+        f"{unet_graph_s * 1e6:.0f} us replayed from a graph, {flops / 1e9:.1f} GFLOP, "
         f"{tf:.0f} TFLOP/s ({tf / 2500 * 100:.1f}% of 2.5 PFLOP/s); FP64 Jacobi-CG {ms_it64 * 1e3:.0f} us/iteration; "
         f"CPU 1 core {cpu_ms:.0f} ms/iteration")
     return {
@@ -536,7 +554,9 @@ def extra_learned(seeds: int, dev) -> dict:
                      "pairs": {"plain_cg": int(meta.get("pcg_pairs", 0)), "fcg_iterates": int(meta.get("fcg_pairs", 0))},
                      "adam_steps": int(meta.get("steps", 0)) + int(meta.get("dagger_rounds", 0)) *
                      int(meta.get("dagger_steps", 0))},
-        "unet": {"us_per_application": round(unet_s * 1e6, 1), "gflop_per_application": round(flops / 1e9, 2),
+        "unet": {"us_per_application": round(unet_s * 1e6, 1),
+                 "us_per_application_graph_replay": round(unet_graph_s * 1e6, 1),
+                 "gflop_per_application": round(flops / 1e9, 2),
                  "roofline": {"bound": "mfma", "achieved": round(tf, 1), "peak": 2500.0, "unit": "TFLOP/s",
                               "frac": round(tf / 2500.0, 4), "basis": "sum of 2*27*ci*co*voxels over the 7 layers"}},
         "jacobi64_roofline": hbm_roofline(iter64, ms_it64 * 1e-3, "FP64 PCG iteration, FP64 matrix values"),

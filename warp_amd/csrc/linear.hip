@@ -336,17 +336,21 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(
 }
 
 // Wide SpMV grids (one workgroup per tile, more than kMaxSpmvGrid): their
-// partials summed into spmv_partials[0] in a fixed order, read by the update
-// as a single partial.
+// partials reduced to kWideParts, workgroup b summing the b-th contiguous
+// range in a fixed order into spmv_partials[b]; the update sums those.
+constexpr int kWideParts = 64;
 __global__ __launch_bounds__(kFinalizeBlock) void cg_wide_spmv_finalize_kernel(const double* __restrict__ parts,
                                                                               int nparts, CgWs ws)
 {
     __shared__ double lds[kFinalizeBlock / kWave];
     if (ws.state->done)
         return;
-    const double t = sum_partials<kFinalizeBlock>(parts, nparts, lds);
+    const int chunk = (nparts + kWideParts - 1) / kWideParts;
+    const int lo = min((int)blockIdx.x * chunk, nparts);
+    const int hi = min(lo + chunk, nparts);
+    const double t = sum_partials<kFinalizeBlock>(parts + lo, hi - lo, lds);
     if (threadIdx.x == 0)
-        ws.spmv_partials[0] = t;
+        ws.spmv_partials[blockIdx.x] = t;
 }
 
 // x += alpha p, r -= alpha Ap, z = M r, partials of (r.r, r.z).  The
@@ -549,8 +553,8 @@ template <typename S> struct CgLauncher {
         return (unsigned)ceil_div(pb.nrow, (kBlock / G) * rows_per_group_for(BR, BR));
     }
     // Very large unfused systems: one workgroup per tile, the partials then
-    // reduced by a one-workgroup kernel (256^3 7-point operator, 65,536 tiles:
-    // FP32 inner CG iteration 423 -> 400 us); otherwise at most kMaxSpmvGrid
+    // reduced to kWideParts by a small kernel (256^3 7-point operator, 65,536 tiles:
+    // FP32 inner CG iteration 423 -> 391 us); otherwise at most kMaxSpmvGrid
     // persistent workgroups whose partials the update sums in its prologue
     // (C2, 17,164 tiles: 66 us per iteration, 71 with the wide grid).
     static constexpr unsigned kWideSpmvTiles = 16u * kMaxSpmvGrid;
@@ -560,7 +564,10 @@ template <typename S> struct CgLauncher {
         return wide_spmv<BR, G>() ? rows_grid<BR, G>() : std::min<unsigned>(rows_grid<BR, G>(), (unsigned)kMaxSpmvGrid);
     }
     // partials the update reads
-    template <int BR, int G> unsigned spmv_parts() const { return wide_spmv<BR, G>() ? 1u : spmv_grid<BR, G>(); }
+    template <int BR, int G> unsigned spmv_parts() const
+    {
+        return wide_spmv<BR, G>() ? (unsigned)kWideParts : spmv_grid<BR, G>();
+    }
     static unsigned vec_grid(int64_t count)
     {
         return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(count, kBlock), kMaxVecGrid));
@@ -601,7 +608,7 @@ template <typename S> struct CgLauncher {
         );
         WA_LAUNCH_CHECK();
         if (wide_spmv<BR, G>()) {
-            cg_wide_spmv_finalize_kernel<<<1, kFinalizeBlock, 0, stream>>>(parts, (int)grid, ws);
+            cg_wide_spmv_finalize_kernel<<<kWideParts, kFinalizeBlock, 0, stream>>>(parts, (int)grid, ws);
             WA_LAUNCH_CHECK();
         }
         return kOk;
