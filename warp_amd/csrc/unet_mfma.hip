@@ -236,8 +236,13 @@ template <int CI, int IN> struct HaloStage {
 // workgroups.  The k loop is fully unrolled: every LDS address is a
 // compile-time offset from one per-lane base, so the compiler can issue the
 // operand reads of later blocks while the MFMAs of earlier ones run.
+// Minimum waves per SIMD the compiler must fit in registers: the 2 -> 16
+// input layer at 6 (96 -> 69 registers; 30.8 -> 28.6 us at 128^3), the others
+// as the compiler chooses (a bound of 3 on the 16 -> 32 layer measured the same).
+template <int CI, int CO> constexpr int conv_min_waves() { return (CI == 2 && CO == 16) ? 6 : 1; }
+
 template <int CI, int CO, int IN, int CBG>
-__global__ __launch_bounds__(kBlock) void conv3_mfma_kernel(
+__global__ __launch_bounds__(kBlock, (conv_min_waves<CI, CO>())) void conv3_mfma_kernel(
     int m, const unsigned short* __restrict__ x, const unsigned short* __restrict__ coarse,
     const bf16x8* __restrict__ wsw, const float* __restrict__ bias, int flags, unsigned short* __restrict__ y,
     float* __restrict__ yf, unsigned short* __restrict__ pooled
@@ -518,8 +523,9 @@ template <int IN> struct Co1Plane {
     }
 };
 
+// (at least 3 waves per SIMD: 172 -> 166 registers, 37.0 -> 32.0 us at 128^3)
 template <int IN>
-__global__ __launch_bounds__(kBlock) void conv3_co1_stream_kernel(int m, const unsigned short* __restrict__ x,
+__global__ __launch_bounds__(kBlock, 3) void conv3_co1_stream_kernel(int m, const unsigned short* __restrict__ x,
                                                                  const unsigned short* __restrict__ coarse,
                                                                  const unsigned* __restrict__ wpk,
                                                                  const float* __restrict__ bias, int flags,
