@@ -611,7 +611,7 @@ def main() -> int:
 
     if args.gpus != 1:
         print(
-            f"bench.py measures one GPU only (got --gpus {args.gpus}); the sharded design is described in DESIGN.md",
+            f"bench.py measures one GPU only (got --gpus {args.gpus}): the sparse path runs on a single MI355X",
             file=sys.stderr,
         )
         return 2

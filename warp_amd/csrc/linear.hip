@@ -556,7 +556,7 @@ template <typename S> struct CgLauncher {
     // reduced to kWideParts by a small kernel (256^3 7-point operator, 65,536 tiles:
     // FP32 inner CG iteration 423 -> 391 us); otherwise at most kMaxSpmvGrid
     // persistent workgroups whose partials the update sums in its prologue
-    // (C2, 17,164 tiles: 66 us per iteration, 71 with the wide grid).
+    // (64^3 3x3 elasticity, 17,164 tiles: 66 us per iteration, 71 with the wide grid).
     static constexpr unsigned kWideSpmvTiles = 16u * kMaxSpmvGrid;
     template <int BR, int G> bool wide_spmv() const { return !fused && rows_grid<BR, G>() > kWideSpmvTiles; }
     template <int BR, int G> unsigned spmv_grid() const

@@ -289,8 +289,8 @@ __global__ __launch_bounds__(kBlock) void tile_scatter_kernel(
 // LDS in bucket order, then copied out so that consecutive threads write
 // consecutive keys of one bucket's run.  1024 threads per workgroup, four
 // triplets each per sub-tile: the sub-tile's chain of LDS phases is bound by
-// latency, which more waves per sub-tile hide (C3: 194 us with 256 threads,
-// 104 with 1024).
+// latency, which more waves per sub-tile hide (16.8 M shuffled element-loop
+// triplets of a 64^3 hex mesh: 194 us with 256 threads, 104 with 1024).
 constexpr int kStagedThreads = 1024;
 __global__ __launch_bounds__(kStagedThreads) void tile_scatter_staged_kernel(
     int T, const int* __restrict__ trow, const int* __restrict__ cols, Partition part,
@@ -1215,7 +1215,7 @@ __global__ __launch_bounds__(kWave * kRowsPerBlock) void finalize_small_rows_ker
         run_len = max(run_len, __shfl_xor(run_len, o));
     // kFinSlots (block, scalar) pairs per lane at a time, the first kFinDepth
     // blocks of each of their runs loaded together before any add (one
-    // memory round trip for the rows of C3); longer runs continue four deep.
+    // memory round trip for rows of up to 28 blocks); longer runs continue four deep.
     // Cross-lane reads (ds_bpermute) only in wave-uniform control flow.
     constexpr int kFinSlots = 4;
     constexpr int kFinDepth = 4;
