@@ -8,6 +8,7 @@ missing, every GPU entry point raises.
 from __future__ import annotations
 
 import ctypes
+import functools
 import os
 import threading
 
@@ -280,6 +281,15 @@ def ptr(t) -> int | None:
 
 
 _raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
+def stream_getter(device: torch.device):
+    """A no-argument function returning the raw handle of ``device``'s current
+    stream (what stream_of returns), for hot paths that query it per call."""
+    index = device.index if device.index is not None else torch.cuda.current_device()
+    if _raw_stream is not None:
+        return functools.partial(_raw_stream, index)
+    return lambda: torch.cuda.current_stream(index).cuda_stream
 
 
 def stream_of(device: torch.device) -> int:

@@ -1791,7 +1791,7 @@ class _MvPlan:
     memory-side cache."""
 
     __slots__ = ("version", "values", "nnz", "dtype", "nx", "ny", "device", "dev_index", "args", "refs", "turn",
-                 "fn")
+                 "fn", "stream")
 
     def __init__(self, A: "BsrMatrix", nnz_hint: int, torch_dtype: torch.dtype, nx: int, ny: int):
         br, bc = A.block_shape
@@ -1818,6 +1818,7 @@ class _MvPlan:
         self.refs = (ctypes.byref(a), ctypes.byref(b))
         self.turn = 0
         self.fn = _native.lib().warp_amd_bsr_mv_packed
+        self.stream = _native.stream_getter(self.device)
 
 
 def bsr_mv(
@@ -1841,13 +1842,14 @@ def bsr_mv(
     if type(A) is BsrMatrix and y is not None and not transpose and work_buffer is None and tile_size == 0:
         # repeated products with one matrix: its arguments are packed once
         # (_MvPlan), and x / y get the same checks in a cheaper form
-        plan = A.__dict__.get("_mv_plan")
+        d = A.__dict__
+        plan = d.get("_mv_plan")
         if (
             plan is not None
-            and plan.version == A.__dict__.get("_topology_version", 0)
-            and plan.values is A.values
-            and plan.nnz == A.nnz
-            and not A.__dict__.get("_nnz_pending", False)
+            and plan.version == d.get("_topology_version", 0)
+            and plan.values is d.get("values")
+            and plan.nnz == d.get("nnz")
+            and not d.get("_nnz_pending", False)
             and x.dtype is plan.dtype
             and y.dtype is plan.dtype
             and x.numel() == plan.nx
@@ -1863,7 +1865,7 @@ def bsr_mv(
             if xp != yp:
                 k = plan.turn
                 plan.turn = k ^ 1
-                st = plan.fn(plan.refs[k], xp, yp, float(alpha), float(beta), _native.stream_of(plan.device))
+                st = plan.fn(plan.refs[k], xp, yp, float(alpha), float(beta), plan.stream())
                 if st:
                     _native.check(st, "warp_amd_bsr_mv_packed")
                 return y
