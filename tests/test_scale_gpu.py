@@ -215,3 +215,35 @@ def test_cg_elasticity32_vs_oracle(device, oracle_mod, ptype, prec_kind):
     assert it == it_o
     assert err <= atol
     assert dx <= 1e-5
+
+
+def test_cg_wide_spmv_grid_vs_oracle(device, oracle_mod):
+    """Jacobi-CG on the 208^3 7-point variable-coefficient operator (8,998,912
+    rows: more SpMV tiles than the persistent grid takes, so each product runs
+    one workgroup per tile and its partials are reduced in two fixed-order
+    steps): 40 iterations match the oracle's 40 in the kernels' SpMV order to
+    FP32 rounding of the reductions, and the recursive residual agrees."""
+    n = 208
+    N = n**3
+    off, col, val, b, _ = problems.fv7_problem(n, 2.3, 16.0 * n / 256, 5, 6, device=device)
+    A = sparse.bsr_zeros(N, N, wa.float32, device=device)
+    A.offsets, A.columns, A.values, A.nnz = off, col, val, col.numel()
+    nnz = int(col.numel())
+    g = _native.lib().warp_amd_bsr_mv_lanes(N, 1, 1, nnz)
+    assert N > 32768 * (256 // g) * 4, "the system must take the one-workgroup-per-tile SpMV"
+    bt = b.to(torch.float32).contiguous()
+    M = linear.preconditioner(A, "diag")
+    x = torch.zeros_like(bt)
+    iters = 40
+    it, err, _ = linear.cg(A, bt, x, tol=1e-30, atol=0.0, maxiter=iters, M=M)
+    offh, colh, valh = off.cpu().numpy(), col.cpu().numpy(), val.cpu().numpy()
+    prec = M._prec_data.cpu().numpy()
+    xo, it_o, err_o, _, _ = oracle_mod.cg(offh, colh, valh, bt.cpu().numpy(), np.zeros(N, np.float32), tol=1e-30,
+                                          atol=0.0, maxiter=iters, prec_kind=oracle_mod.PREC_DIAG, prec=prec, group=g)
+    xh = x.cpu().numpy()
+    dx = float(np.linalg.norm(xh - xo) / np.linalg.norm(xo))
+    _say(f"[scale] CG 208^3 7-point (wide SpMV grid, lanes {g}): {it} iterations (oracle {it_o}), |r| {err:.4e} "
+         f"(oracle {err_o:.4e}); solution vs oracle {dx:.2e}")
+    assert it == it_o == iters
+    assert abs(err - err_o) <= 1e-3 * err_o
+    assert dx <= 1e-4
