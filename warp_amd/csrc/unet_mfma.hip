@@ -478,7 +478,10 @@ void launch_tiles(int ntiles, hipStream_t s, Args... args)
 // barriers per plane.)
 constexpr int kCo1T = 16;             // columns per side of a workgroup
 constexpr int kCo1H = kCo1T + 2;      // halo plane side
-constexpr int kCo1Z = 8;              // output slices per workgroup
+// Output slices per workgroup: at 128^3, 12 gives 8 x 8 x 11 = 704 tiles for
+// the 768 slots of three workgroups per CU (30.7 us; 8 slices: 1024 tiles in
+// two uneven waves, 32.3 us; 16 slices: 512 tiles, 32.4 us).
+constexpr int kCo1Z = 12;
 constexpr int kCo1VB = 48;            // LDS bytes per voxel: 16 BF16 channels + 16 bytes of padding
 constexpr int kCo1KB = 5;             // k blocks of 32: (dy, dx, c) = 144, padded to 160
 
