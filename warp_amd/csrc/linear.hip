@@ -433,15 +433,29 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(
 
 // p = z + beta p (unfused path).  The prologue sums the update's partials
 // into r.r and r.z of the new iterate, forms beta and `done`; workgroup 0
-// records them.
+// records them.  z and p are 256-byte aligned workspace slices, moved in
+// 16-byte vectors (the first ones loaded before the prologue, the next ones
+// while the current are stored); one fma per entry, as a scalar loop.
 template <typename S>
 __global__ __launch_bounds__(kBlock) void cg_direction_kernel(int64_t n, const S* __restrict__ z, S* __restrict__ p,
                                                               int upd_parts, CgWs ws)
 {
+    constexpr int kV = 16 / (int)sizeof(S);
+    typedef S V __attribute__((ext_vector_type(kV)));
     __shared__ double lds[kBlock / kWave];
     CgState* state = ws.state;
     if (state->done_prev)
         return;
+    const V* __restrict__ zv = reinterpret_cast<const V*>(z);
+    V* __restrict__ pv = reinterpret_cast<V*>(p);
+    const int64_t nv = n / kV;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    V zc = {}, pc = {};
+    if (i < nv) {
+        zc = zv[i];
+        pc = pv[i];
+    }
     double t[2];
     sum_parts<2, kMaxVecGrid>(ws.upd_partials, (size_t)upd_parts, upd_parts, lds, t);
     const bool live = t[0] > state->atol_sq;
@@ -453,8 +467,24 @@ __global__ __launch_bounds__(kBlock) void cg_direction_kernel(int64_t n, const S
         state->done = !live;
     }
     const S beta = (S)bt;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-        p[i] = fma(beta, p[i], z[i]);
+    for (; i < nv; i += stride) {
+        V zn = {}, pn = {};
+        if (i + stride < nv) {
+            zn = zv[i + stride];
+            pn = pv[i + stride];
+        }
+        V o;
+#pragma unroll
+        for (int k = 0; k < kV; ++k)
+            o[k] = fma(beta, pc[k], zc[k]);
+        pv[i] = o;
+        zc = zn;
+        pc = pn;
+    }
+    // the last n % kV entries
+    const int64_t j = nv * kV + threadIdx.x;
+    if (blockIdx.x == 0 && j < n)
+        p[j] = fma(beta, p[j], z[j]);
 }
 
 // Fused path at a residual check: r.r, r.z and `done` of the current iterate
