@@ -251,3 +251,22 @@ def test_unet_backward_bf16_vs_oracle(device):
     print(f"[unet bf16] cosine of the whole gradient {cos:.5f}")
     assert all(v < 0.15 for v in rels.values()), rels
     assert cos > 0.995, cos
+
+
+@pytest.mark.parametrize("nvox,offset", [(12288, 0), (1001, 0), (4096, 1)])
+def test_pack_input_bf16_vs_torch_rounding(device, nvox, offset):
+    """Network input (r * inv_scale, logk) as BF16 pairs: 16-byte path (aligned,
+    nvox % 4 == 0) and the per-voxel path (odd count, misaligned views), both
+    against torch's round-to-nearest-even BF16 conversion."""
+    g = torch.Generator().manual_seed(nvox + offset)
+    r_h = torch.randn(nvox + offset, generator=g, dtype=torch.float64) * 3.0
+    k_h = torch.randn(nvox + offset, generator=g)
+    r = r_h.to(device)[offset:]
+    logk = k_h.to(device)[offset:]
+    inv = torch.tensor([0.37], dtype=torch.float64, device=device)
+    out = torch.empty(2 * nvox, dtype=torch.int16, device=device)
+    _native.call("warp_amd_unet_pack_input_bf16", nvox, _native.ptr(r), _native.ptr(logk), _native.ptr(inv),
+                 _native.ptr(out), _native.stream_of(device))
+    torch.cuda.synchronize()
+    want = torch.stack([(r_h[offset:] * 0.37).float().to(torch.bfloat16), k_h[offset:].to(torch.bfloat16)], 1)
+    assert torch.equal(out.cpu(), want.view(torch.int16).reshape(-1))
