@@ -13,7 +13,8 @@
  *     host thread).
  *   - Entry points taking (workspace, workspace_bytes) never allocate; the
  *     matching *_workspace_size() query gives the byte count (host-only
- *     call, no GPU needed).
+ *     call, no GPU needed).  A workspace is 16-byte aligned (device
+ *     allocations are); the CG solvers reject one that is not.
  *   - Reference paths below are relative to the NVIDIA Warp 1.17.0.dev4
  *     checkout.
  */

@@ -73,3 +73,20 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_native, "LIB_PATH", str(tmp_path / "nope.so"))
     with pytest.raises(_native.NativeError, match="no CPU fallback"):
         _native.lib()
+
+
+def test_cg_rejects_misaligned_workspace_without_gpu():
+    """The CG solvers move workspace slices in 16-byte vectors: a workspace
+    that is not 16-byte aligned is an argument error, raised before any
+    device work (the pointers are never dereferenced)."""
+    from warp_amd import _native
+
+    lib = _native.lib()
+    n = 64
+    need = lib.warp_amd_cg_workspace_size(n, 4)
+    fake = 1 << 20  # host-side check only: never dereferenced
+    args = (n, 1, fake, fake, fake, 3 * n, 0, None, fake, fake, 1e-6, 0.0, 10, 1, 0, 0, _native.CG_CALLBACK(), None)
+    st = lib.warp_amd_cg_solve_f32(*args, fake + 8, need, None, None, None, None)
+    assert st == -2
+    assert b"16-byte aligned" in lib.warp_amd_last_error()
+    lib.warp_amd_clear_error()

@@ -867,6 +867,8 @@ int cg_solve_impl(
     CgWs ws = carve_cg(workspace, workspace_bytes, pb.nrow, n, sizeof(S), &needed);
     WA_REQUIRE(workspace && needed <= workspace_bytes, "cg_solve: workspace too small (%zu < %zu bytes)",
                workspace_bytes, needed);
+    // the vector passes move workspace slices in 16-byte vectors
+    WA_REQUIRE((uintptr_t)workspace % 16 == 0, "cg_solve: workspace must be 16-byte aligned");
     if (maxiter <= 0)
         maxiter = (int)n;
 
